@@ -1,0 +1,1084 @@
+/*
+ * ice3_oracle.c -- CPU restatement (plain C, FP64, scalar) of the PHYEX CY50T1
+ * ice_adjust and rain_ice path, pinned AROME ICE3 configuration (KRR=6).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke entry
+ * point and the cpu_baseline leg of bench.py may load it; the product (the HIP
+ * library under dwarf-p-ice3_amd/) never links, loads or calls it.
+ *
+ * Arrays are (nlev, ncol) row-major, column index contiguous: element (k, i)
+ * is a[k*ncol + i], byte-identical to the reference's Fortran (NIJT, NKT)
+ * arrays; level 0 is the lowest level (D%NKL = 1).
+ *
+ * Constants and lookup tables are not recomputed here: they are read from the
+ * golden dump of the compiled reference (tests/golden/constants.txt, written
+ * after INI_CST + INI_RAIN_ICE), so the oracle's parameters are the
+ * reference's own values.
+ *
+ * Every function cites the reference routine it restates (paths relative to
+ * the reference repository, PHYEX-IAL_CY50T1/).  Expression order follows the
+ * Fortran so that the restatement agrees with the compiled reference to
+ * rounding (gcc, no FMA contraction, glibc libm).
+ *
+ * rain_ice is evaluated with per-point time splitting: each microphysics point
+ * runs ICE4_STEPPING alone, which is the reference's NPROMICRO=1 behaviour.
+ * All outputs except EVAP3D are independent of the chunking in the pinned
+ * configuration (XMRSTEP=0, XTSTEP_TS=0); EVAP3D is the evaporation rate of
+ * the point's own last tendency call.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ice3_oracle.h"
+
+/* ---------------------------------------------------------------- constants */
+
+typedef struct {
+  /* CST_t (aux/modd_cst.F90) */
+  double XTT, XLVTT, XLSTT, XLMTT, XCPD, XCPV, XCL, XCI, XRD, XRV, XEPSILO;
+  double XALPW, XBETAW, XGAMW, XALPI, XBETAI, XGAMI, XESTT, XP00, XRHOLW, XPI, XG, XMNH_TINY;
+  /* RAIN_ICE_DESCR_t */
+  double XRTMIN[7]; /* 1-based */
+  double XCEXVT, XLBR, XLBEXR, XLBS, XLBEXS, XLBG, XLBEXG, XLBI, XLBEXI, XLBDAS_MAX, XDI, XBS,
+      XCXS, XCXG, XDG, XCONC_LAND;
+  /* RAIN_ICE_PARAM_t */
+  double XCRIAUTC, XCRIAUTI, XACRIAUTI, XBCRIAUTI, XNU10, XALPHA1, XBETA1, XNU20, XALPHA2,
+      XBETA2, XMNU0, XALPHA3, XBETA3, XHON, XSCFAC, X0DEPI, X2DEPI, X0DEPS, X1DEPS, XEX0DEPS,
+      XEX1DEPS, X0DEPG, X1DEPG, XEX0DEPG, XEX1DEPG, XTIMAUTI, XTEXAUTI, XCOLEXIS, XFIAGGS,
+      XEXIAGGS, XTIMAUTC, XFCACCR, XEXCACCR, X0EVAR, X1EVAR, XEX0EVAR, XEX1EVAR, XEXCRIMSS,
+      XCRIMSS, XEXCRIMSG, XCRIMSG, XSRIMCG, XEXSRIMCG, XSRIMCG2, XSRIMCG3, XEXSRIMCG2,
+      XRIMINTP1, XRIMINTP2, XFRACCSS, XLBRACCS1, XLBRACCS2, XLBRACCS3, XFSACCRG, XLBSACCR1,
+      XLBSACCR2, XLBSACCR3, XACCINTP1S, XACCINTP2S, XACCINTP1R, XACCINTP2R, XFSCVMG, XEXRCFRI,
+      XRCFRI, XEXICFRR, XICFRR, XFCDRYG, XCOLIG, XCOLEXIG, XFIDRYG, XCOLSG, XCOLEXSG, XFSDRYG,
+      XLBSDRYG1, XLBSDRYG2, XLBSDRYG3, XFRDRYG, XLBRDRYG1, XLBRDRYG2, XLBRDRYG3, XDRYINTP1R,
+      XDRYINTP2R, XDRYINTP1S, XDRYINTP2S, XDRYINTP1G, XDRYINTP2G, XFSEDR, XEXSEDR, XFSEDS,
+      XEXSEDS, XFSEDG, XEXSEDG, XFSEDI, XEXCSEDI;
+  int NGAMINC, NACCLBDAS, NACCLBDAR, NDRYLBDAR, NDRYLBDAS, NDRYLBDAG;
+  double *GAMINC_RIM1, *GAMINC_RIM2, *GAMINC_RIM4;     /* NGAMINC */
+  double *KER_RACCSS, *KER_RACCS;                       /* (NACCLBDAS, NACCLBDAR) */
+  double *KER_SACCRG;                                   /* (NACCLBDAR, NACCLBDAS) */
+  double *KER_SDRYG;                                    /* (NDRYLBDAG, NDRYLBDAS) */
+  double *KER_RDRYG;                                    /* (NDRYLBDAG, NDRYLBDAR) */
+} params_t;
+
+static params_t P;
+static int P_loaded = 0;
+
+/* NEB_t / PARAM_ICE_t pinned values (oracle/ref_driver.F90 REF_INIT). */
+#define XTMINMIX 253.16
+#define XTMAXMIX 273.16
+
+/* MODD_FIELDS_ADDRESS (micro/modd_fields_address.F90) */
+enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6 };
+enum {
+  IRCHONI = 1, IRVDEPS, IRIAGGS, IRIAUTS, IRVDEPG, IRCAUTR, IRCACCR, IRREVAV, IRCBERI, IRHMLTR,
+  IRSMLTG, IRCMLTSR, IRRACCSS, IRRACCSG, IRSACCRG, IRCRIMSS, IRCRIMSG, IRSRIMCG, IRICFRRG,
+  IRRCFRIG, IRICFRR, IRCWETG, IRIWETG, IRRWETG, IRSWETG, IRCDRYG, IRIDRYG, IRRDRYG, IRSDRYG,
+  IRWETGH, IRGMLTR, IRCWETH, IRIWETH, IRSWETH, IRGWETH, IRRWETH, IRCDRYH, IRIDRYH, IRSDRYH,
+  IRRDRYH, IRGDRYH, IRDRYHG, IRILARS, IRVDEPI, IRVHENI_MR, IRRHONG_MR, IRIMLTC_MR, IRSRIMCG_MR,
+  IRWETGH_MR, IBUNUM = 49
+};
+/* local PRS_TEND / PRG_TEND slots (mode_ice4_fast_rs.F90:87, mode_ice4_fast_rg.F90:98) */
+enum { RS_RCRIMS = 1, RS_RCRIMSS, RS_RSRIMCG, RS_RRACCS, RS_RRACCSS, RS_RSACCRG, RS_FREEZ1, RS_FREEZ2 };
+enum { RG_RCDRYG = 1, RG_RIDRYG, RG_RIWETG, RG_RSDRYG, RG_RSWETG, RG_RRDRYG, RG_FREEZ1, RG_FREEZ2 };
+
+/* ZSRC_1D(-22:11), micro/condensation.F90:208-216 */
+static const double ZSRC_1D[34] = {
+    0.,            0.,           2.0094444E-04, 0.316670E-03, 4.9965648E-04, 0.785956E-03,
+    1.2341294E-03, 0.193327E-02, 3.0190963E-03, 0.470144E-02, 7.2950651E-03, 0.112759E-01,
+    1.7350994E-02, 0.265640E-01, 4.0427860E-02, 0.610997E-01, 9.1578111E-02, 0.135888E+00,
+    0.1991484,     0.230756E+00, 0.2850565,     0.375050E+00, 0.5000000,     0.691489E+00,
+    0.8413813,     0.933222E+00, 0.9772662,     0.993797E+00, 0.9986521,     0.999768E+00,
+    0.9999684,     0.999997E+00, 1.0000000,     1.000000};
+
+/* ------------------------------------------------------ constants file I/O */
+
+typedef struct { const char *name; double *dst; } scal_t;
+
+static int read_constants(const char *path) {
+  FILE *fp = fopen(path, "r");
+  if (!fp) return -1;
+  scal_t s[] = {
+      {"XTT", &P.XTT}, {"XLVTT", &P.XLVTT}, {"XLSTT", &P.XLSTT}, {"XLMTT", &P.XLMTT},
+      {"XCPD", &P.XCPD}, {"XCPV", &P.XCPV}, {"XCL", &P.XCL}, {"XCI", &P.XCI}, {"XRD", &P.XRD},
+      {"XRV", &P.XRV}, {"XEPSILO", &P.XEPSILO}, {"XALPW", &P.XALPW}, {"XBETAW", &P.XBETAW},
+      {"XGAMW", &P.XGAMW}, {"XALPI", &P.XALPI}, {"XBETAI", &P.XBETAI}, {"XGAMI", &P.XGAMI},
+      {"XESTT", &P.XESTT}, {"XP00", &P.XP00}, {"XRHOLW", &P.XRHOLW}, {"XPI", &P.XPI},
+      {"XG", &P.XG}, {"XMNH_TINY", &P.XMNH_TINY}, {"XCEXVT", &P.XCEXVT}, {"XLBR", &P.XLBR},
+      {"XLBEXR", &P.XLBEXR}, {"XLBS", &P.XLBS}, {"XLBEXS", &P.XLBEXS}, {"XLBG", &P.XLBG},
+      {"XLBEXG", &P.XLBEXG}, {"XLBI", &P.XLBI}, {"XLBEXI", &P.XLBEXI},
+      {"XLBDAS_MAX", &P.XLBDAS_MAX}, {"XDI", &P.XDI}, {"XBS", &P.XBS}, {"XCXS", &P.XCXS},
+      {"XCXG", &P.XCXG}, {"XDG", &P.XDG}, {"XCONC_LAND", &P.XCONC_LAND},
+      {"XCRIAUTC", &P.XCRIAUTC}, {"XCRIAUTI", &P.XCRIAUTI}, {"XACRIAUTI", &P.XACRIAUTI},
+      {"XBCRIAUTI", &P.XBCRIAUTI}, {"XNU10", &P.XNU10}, {"XALPHA1", &P.XALPHA1},
+      {"XBETA1", &P.XBETA1}, {"XNU20", &P.XNU20}, {"XALPHA2", &P.XALPHA2}, {"XBETA2", &P.XBETA2},
+      {"XMNU0", &P.XMNU0}, {"XALPHA3", &P.XALPHA3}, {"XBETA3", &P.XBETA3}, {"XHON", &P.XHON},
+      {"XSCFAC", &P.XSCFAC}, {"X0DEPI", &P.X0DEPI}, {"X2DEPI", &P.X2DEPI}, {"X0DEPS", &P.X0DEPS},
+      {"X1DEPS", &P.X1DEPS}, {"XEX0DEPS", &P.XEX0DEPS}, {"XEX1DEPS", &P.XEX1DEPS},
+      {"X0DEPG", &P.X0DEPG}, {"X1DEPG", &P.X1DEPG}, {"XEX0DEPG", &P.XEX0DEPG},
+      {"XEX1DEPG", &P.XEX1DEPG}, {"XTIMAUTI", &P.XTIMAUTI}, {"XTEXAUTI", &P.XTEXAUTI},
+      {"XCOLEXIS", &P.XCOLEXIS}, {"XFIAGGS", &P.XFIAGGS}, {"XEXIAGGS", &P.XEXIAGGS},
+      {"XTIMAUTC", &P.XTIMAUTC}, {"XFCACCR", &P.XFCACCR}, {"XEXCACCR", &P.XEXCACCR},
+      {"X0EVAR", &P.X0EVAR}, {"X1EVAR", &P.X1EVAR}, {"XEX0EVAR", &P.XEX0EVAR},
+      {"XEX1EVAR", &P.XEX1EVAR}, {"XEXCRIMSS", &P.XEXCRIMSS}, {"XCRIMSS", &P.XCRIMSS},
+      {"XEXCRIMSG", &P.XEXCRIMSG}, {"XCRIMSG", &P.XCRIMSG}, {"XSRIMCG", &P.XSRIMCG},
+      {"XEXSRIMCG", &P.XEXSRIMCG}, {"XSRIMCG2", &P.XSRIMCG2}, {"XSRIMCG3", &P.XSRIMCG3},
+      {"XEXSRIMCG2", &P.XEXSRIMCG2}, {"XRIMINTP1", &P.XRIMINTP1}, {"XRIMINTP2", &P.XRIMINTP2},
+      {"XFRACCSS", &P.XFRACCSS}, {"XLBRACCS1", &P.XLBRACCS1}, {"XLBRACCS2", &P.XLBRACCS2},
+      {"XLBRACCS3", &P.XLBRACCS3}, {"XFSACCRG", &P.XFSACCRG}, {"XLBSACCR1", &P.XLBSACCR1},
+      {"XLBSACCR2", &P.XLBSACCR2}, {"XLBSACCR3", &P.XLBSACCR3}, {"XACCINTP1S", &P.XACCINTP1S},
+      {"XACCINTP2S", &P.XACCINTP2S}, {"XACCINTP1R", &P.XACCINTP1R},
+      {"XACCINTP2R", &P.XACCINTP2R}, {"XFSCVMG", &P.XFSCVMG}, {"XEXRCFRI", &P.XEXRCFRI},
+      {"XRCFRI", &P.XRCFRI}, {"XEXICFRR", &P.XEXICFRR}, {"XICFRR", &P.XICFRR},
+      {"XFCDRYG", &P.XFCDRYG}, {"XCOLIG", &P.XCOLIG}, {"XCOLEXIG", &P.XCOLEXIG},
+      {"XFIDRYG", &P.XFIDRYG}, {"XCOLSG", &P.XCOLSG}, {"XCOLEXSG", &P.XCOLEXSG},
+      {"XFSDRYG", &P.XFSDRYG}, {"XLBSDRYG1", &P.XLBSDRYG1}, {"XLBSDRYG2", &P.XLBSDRYG2},
+      {"XLBSDRYG3", &P.XLBSDRYG3}, {"XFRDRYG", &P.XFRDRYG}, {"XLBRDRYG1", &P.XLBRDRYG1},
+      {"XLBRDRYG2", &P.XLBRDRYG2}, {"XLBRDRYG3", &P.XLBRDRYG3}, {"XDRYINTP1R", &P.XDRYINTP1R},
+      {"XDRYINTP2R", &P.XDRYINTP2R}, {"XDRYINTP1S", &P.XDRYINTP1S},
+      {"XDRYINTP2S", &P.XDRYINTP2S}, {"XDRYINTP1G", &P.XDRYINTP1G},
+      {"XDRYINTP2G", &P.XDRYINTP2G}, {"XFSEDR", &P.XFSEDR}, {"XEXSEDR", &P.XEXSEDR},
+      {"XFSEDS", &P.XFSEDS}, {"XEXSEDS", &P.XEXSEDS}, {"XFSEDG", &P.XFSEDG},
+      {"XEXSEDG", &P.XEXSEDG}, {"XFSEDI", &P.XFSEDI}, {"XEXCSEDI", &P.XEXCSEDI}, {NULL, NULL}};
+  char name[64];
+  int n, nfound = 0;
+  while (fscanf(fp, "%63s %d", name, &n) == 2) {
+    double *buf = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int j = 0; j < n; ++j)
+      if (fscanf(fp, "%lf", &buf[j]) != 1) { free(buf); fclose(fp); return -2; }
+    int used = 0;
+    for (int j = 0; s[j].name; ++j)
+      if (!strcmp(s[j].name, name)) { *s[j].dst = buf[0]; used = 1; nfound++; }
+    if (!strcmp(name, "XRTMIN")) { for (int j = 0; j < 6; ++j) P.XRTMIN[j + 1] = buf[j]; used = 1; }
+    else if (!strcmp(name, "NGAMINC")) { P.NGAMINC = (int)buf[0]; }
+    else if (!strcmp(name, "NACCLBDAS")) { P.NACCLBDAS = (int)buf[0]; }
+    else if (!strcmp(name, "NACCLBDAR")) { P.NACCLBDAR = (int)buf[0]; }
+    else if (!strcmp(name, "NDRYLBDAR")) { P.NDRYLBDAR = (int)buf[0]; }
+    else if (!strcmp(name, "NDRYLBDAS")) { P.NDRYLBDAS = (int)buf[0]; }
+    else if (!strcmp(name, "NDRYLBDAG")) { P.NDRYLBDAG = (int)buf[0]; }
+    if (!strcmp(name, "XGAMINC_RIM1")) { P.GAMINC_RIM1 = buf; used = 2; }
+    else if (!strcmp(name, "XGAMINC_RIM2")) { P.GAMINC_RIM2 = buf; used = 2; }
+    else if (!strcmp(name, "XGAMINC_RIM4")) { P.GAMINC_RIM4 = buf; used = 2; }
+    else if (!strcmp(name, "XKER_RACCSS")) { P.KER_RACCSS = buf; used = 2; }
+    else if (!strcmp(name, "XKER_RACCS")) { P.KER_RACCS = buf; used = 2; }
+    else if (!strcmp(name, "XKER_SACCRG")) { P.KER_SACCRG = buf; used = 2; }
+    else if (!strcmp(name, "XKER_SDRYG")) { P.KER_SDRYG = buf; used = 2; }
+    else if (!strcmp(name, "XKER_RDRYG")) { P.KER_RDRYG = buf; used = 2; }
+    if (used != 2) free(buf);
+  }
+  fclose(fp);
+  if (!P.GAMINC_RIM1 || !P.KER_RACCSS || !P.KER_SDRYG || !P.KER_RDRYG || nfound < 130) return -3;
+  return 0;
+}
+
+int oracle_load_constants(const char *path) {
+  int rc = read_constants(path);
+  P_loaded = (rc == 0);
+  return rc;
+}
+
+/* --------------------------------------------------------------- helpers */
+
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double sq(double a) { return a * a; }
+static inline double pow4(double a) { double a2 = a * a; return a2 * a2; }
+
+/* INTERP_MICRO_1D, packed branch (micro/interp_micro.func.h:34-88), one point */
+static double interp1d(double x, int knum, double p1, double p2, const double *lt) {
+  double idx = dmax(1.00001, dmin((double)knum - 0.00001, p1 * log(x) + p2));
+  int i = (int)idx;
+  double f = idx - (double)i;
+  return lt[i] * f - lt[i - 1] * (f - 1.0); /* PLT(i+1), PLT(i), 1-based */
+}
+
+/* INTERP_MICRO_2D, packed branch (micro/interp_micro.func.h:159-223), one point.
+ * lt is (knum1, knum2) column-major; lt3 (optional) is addressed transposed. */
+static void interp2d(double x1, double x2, int knum1, int knum2, double p11, double p12,
+                     double p21, double p22, const double *lt, double *o1,
+                     const double *lt2, double *o2, const double *lt3, double *o3) {
+  double a = dmax(1.00001, dmin((double)knum1 - 0.00001, p11 * log(x1) + p12));
+  int i1 = (int)a;
+  double f1 = a - (double)i1;
+  double b = dmax(1.00001, dmin((double)knum2 - 0.00001, p21 * log(x2) + p22));
+  int i2 = (int)b;
+  double f2 = b - (double)i2;
+#define T(L, i, j) (L)[((i)-1) + ((j)-1) * knum1]
+  *o1 = (T(lt, i1 + 1, i2 + 1) * f2 - T(lt, i1 + 1, i2) * (f2 - 1.0)) * f1 -
+        (T(lt, i1, i2 + 1) * f2 - T(lt, i1, i2) * (f2 - 1.0)) * (f1 - 1.0);
+  if (lt2)
+    *o2 = (T(lt2, i1 + 1, i2 + 1) * f2 - T(lt2, i1 + 1, i2) * (f2 - 1.0)) * f1 -
+          (T(lt2, i1, i2 + 1) * f2 - T(lt2, i1, i2) * (f2 - 1.0)) * (f1 - 1.0);
+#undef T
+#define T3(L, i, j) (L)[((i)-1) + ((j)-1) * knum2]
+  if (lt3)
+    *o3 = (T3(lt3, i2 + 1, i1 + 1) * f1 - T3(lt3, i2 + 1, i1) * (f1 - 1.0)) * f2 -
+          (T3(lt3, i2, i1 + 1) * f1 - T3(lt3, i2, i1) * (f1 - 1.0)) * (f2 - 1.0);
+#undef T3
+}
+
+static inline double esat_w(double t) { return exp(P.XALPW - P.XBETAW / t - P.XGAMW * log(t)); }
+static inline double esat_i(double t) { return exp(P.XALPI - P.XBETAI / t - P.XGAMI * log(t)); }
+
+/* ============================================================ ICE_ADJUST */
+
+/* ICE_ADJUST (micro/ice_adjust.F90:7-485) + ITERATION (:488-565) + CONDENSATION
+ * (micro/condensation.F90:7-658) for one point; branch: LSUBG_COND=T, OSIGMAS=T,
+ * LSTATNW=F, LCONDBORN=F, OCND2=F, HCONDENS='CB02', HLAMBDA3='CB', HFRAC_ICE='T',
+ * CSUBG_MF_PDF='TRIANGLE', ITERMAX=1, PWEIGHT_MF_CLOUD=0, KRR=6. */
+static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs, double th,
+                             double exn, double exnref, double rhodref, double rv, double rc,
+                             double ri, double rr, double rs, double rg, double cf_mf,
+                             double rc_mf, double ri_mf, double *rvs, double *rcs, double *ris,
+                             double *ths, double *o) {
+  /* 2.3 (ice_adjust.F90:264-270) */
+  double zt = th * exn;
+  double lv = P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT);
+  double ls = P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT);
+  /* ITERATION, KRR=6 (:507-510) */
+  double cph = P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg);
+  /* CONDENSATION */
+  double rt = rv + rc + ri * 1.0;                                   /* :275 */
+  double pv = dmin(esat_w(zt), .99 * pabs);                         /* :407 */
+  double piv = dmin(esat_i(zt), .99 * pabs);                        /* :408 */
+  double frac = 0.;
+  if (rc + ri > 1.E-20) frac = ri / (rc + ri);                      /* :413-415 */
+  frac = dmax(0., dmin(1., ((XTMAXMIX - zt) / (XTMAXMIX - XTMINMIX)))); /* compute_frac_ice.func.h:31 */
+  double qsl = P.XRD / P.XRV * pv / (pabs - pv);                   /* :419 */
+  double qsi = P.XRD / P.XRV * piv / (pabs - piv);                 /* :420 */
+  qsl = (1. - frac) * qsl + frac * qsi;                            /* :423 */
+  double lvs = (1. - frac) * lv + frac * ls;                       /* :424 */
+  double ah = lvs * qsl / (P.XRV * sq(zt)) * (P.XRV * qsl / P.XRD + 1.); /* :428 */
+  double a = 1. / (1. + lvs / cph * ah);                           /* :429 */
+  double sbar = a * (rt - qsl + ah * lvs * (rc + ri * 1.0) / cph); /* :431 */
+  double sigma;
+  if (sigqsat != 0.)
+    sigma = sqrt(sq(2 * sigs) + sq(sigqsat * qsl * a));            /* :446 */
+  else
+    sigma = 2 * sigs;                                              /* :452 */
+  sigma = dmax(1.E-10, sigma);                                     /* :475 */
+  double q1 = sbar / sigma;                                        /* :478 */
+  double cond;                                                     /* CB02 :543-571 */
+  if (q1 > 0. && q1 <= 2)
+    cond = dmin(exp(-1.) + .66 * q1 + .086 * sq(q1), 2.);
+  else if (q1 > 2.)
+    cond = q1;
+  else
+    cond = exp(1.2 * q1 - 1.);
+  cond = cond * sigma;
+  double cldfr;
+  if (cond < 1.E-12)
+    cldfr = 0.;
+  else
+    cldfr = dmax(0., dmin(1., 0.5 + 0.36 * atan(1.55 * q1)));
+  if (cldfr == 0.) cond = 0.;
+  int inq1 = (int)floor(dmin(100., dmax(-100., 2 * q1)));
+  if (inq1 < -22) inq1 = -22;
+  if (inq1 > 10) inq1 = 10;
+  double inc = 2. * q1 - (double)inq1;
+  double sigrc = dmin(1., (1. - inc) * ZSRC_1D[inq1 + 22] + inc * ZSRC_1D[inq1 + 23]);
+  double hlc_hcf = 0., hlc_hrc = 0., hli_hcf = 0., hli_hri = 0.;
+  double rc_out = (1. - frac) * cond;                              /* :582 */
+  double ri_out = frac * cond;                                     /* :583 */
+  zt = zt + ((rc_out - rc) * lv + (ri_out - ri) * ls) / cph;       /* :584-586 */
+  double rv_out = rt - rc_out - ri_out * 1.0;                      /* :587 */
+  sigrc = sigrc * dmin(3., dmax(1., 1. - q1));                     /* lambda3 CB :650 */
+
+  /* 5.0-5.1 sources (ice_adjust.F90:313-345) */
+  double w1 = (rc_out - rc) / dt;
+  double w2 = (ri_out - ri) / dt;
+  if (w1 < 0.0) w1 = dmax(w1, -*rcs); else w1 = dmin(w1, *rvs);
+  *rvs = *rvs - w1;
+  *rcs = *rcs + w1;
+  *ths = *ths + w1 * lv / (cph * exnref);
+  if (w2 < 0.0) w2 = dmax(w2, -*ris); else w2 = dmin(w2, *rvs);
+  *rvs = *rvs - w2;
+  *ris = *ris + w2;
+  *ths = *ths + w2 * ls / (cph * exnref);
+  /* 5.2 subgrid branch (:370-466) */
+  w1 = rc_mf / dt;
+  w2 = ri_mf / dt;
+  if (w1 + w2 > *rvs) {
+    w1 = w1 * *rvs / (w1 + w2);
+    w2 = *rvs - w1;
+  }
+  cldfr = dmin(1., cldfr + cf_mf);
+  *rcs = *rcs + w1;
+  *ris = *ris + w2;
+  *rvs = *rvs - (w1 + w2);
+  *ths = *ths + (w1 * lv + w2 * ls) / cph / exnref;
+  { /* droplets, TRIANGLE pdf (:389-412) */
+    double criaut = P.XCRIAUTC / rhodref, hcf, hr;
+    double x = w1 * dt, c = criaut * cf_mf;
+    if (x > cf_mf * criaut) {
+      hcf = 1. - .5 * sq(criaut * cf_mf / dmax(1.E-20, x));
+      hr = x - (c * c * c) / (3 * sq(dmax(1.E-20, x)));
+    } else if (2. * w1 * dt <= cf_mf * criaut) {
+      hcf = 0.;
+      hr = 0.;
+    } else {
+      hcf = sq(2. * w1 * dt - criaut * cf_mf) / (2. * sq(dmax(1.E-20, x)));
+      hr = (4. * (x * x * x) - 3. * w1 * dt * sq(c) + (c * c * c)) / (3 * sq(dmax(1.E-20, x)));
+    }
+    hcf = hcf * cf_mf;
+    hlc_hcf = dmin(1., hlc_hcf + hcf);
+    hlc_hrc = hlc_hrc + hr;
+  }
+  { /* ice, TRIANGLE pdf (:413-446); zt is the adjusted temperature */
+    double criaut = dmin(P.XCRIAUTI, pow(10., P.XACRIAUTI * (zt - P.XTT) + P.XBCRIAUTI)), hcf, hr;
+    double x = w2 * dt, c = criaut * cf_mf;
+    if (x > cf_mf * criaut) {
+      hcf = 1. - .5 * sq(criaut * cf_mf / x);
+      hr = x - (c * c * c) / (3 * sq(x));
+    } else if (2. * w2 * dt <= cf_mf * criaut) {
+      hcf = 0.;
+      hr = 0.;
+    } else {
+      hcf = sq(2. * w2 * dt - criaut * cf_mf) / (2. * sq(x));
+      hr = (4. * (x * x * x) - 3. * w2 * dt * sq(c) + (c * c * c)) / (3 * sq(x));
+    }
+    hcf = hcf * cf_mf;
+    hli_hcf = dmin(1., hli_hcf + hcf);
+    hli_hri = hli_hri + hr;
+  }
+  { /* adjusted state with the mass-flux condensate (:449-466) */
+    double v1 = rc_mf, v2 = ri_mf;
+    if (v1 + v2 > rv_out) {
+      v1 = v1 * rv_out / (v1 + v2);
+      v2 = rv_out - v1;
+    }
+    rc_out = rc_out + v1;
+    ri_out = ri_out + v2;
+    rv_out = rv_out - (v1 + v2);
+    zt = zt + (v1 * lv + v2 * ls) / cph;
+  }
+  o[OA_T] = zt;
+  o[OA_RV] = rv_out;
+  o[OA_RC] = rc_out;
+  o[OA_RI] = ri_out;
+  o[OA_CLDFR] = cldfr;
+  o[OA_HLC_HRC] = hlc_hrc;
+  o[OA_HLC_HCF] = hlc_hcf;
+  o[OA_HLI_HRI] = hli_hri;
+  o[OA_HLI_HCF] = hli_hcf;
+  o[OA_CPH] = cph;
+  o[OA_LV] = lv;
+  o[OA_LS] = ls;
+  o[OA_SRCS] = sigrc;
+  o[OA_TH] = zt / exn;
+}
+
+int oracle_ice_adjust(int ncol, int nlev, double dt, const double *sigqsat,
+                      const double *const in[ORACLE_ADJ_NIN], double *const out[ORACLE_ADJ_NOUT]) {
+  if (!P_loaded) return -1;
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) {
+      size_t j = (size_t)k * ncol + i;
+      double rvs = in[IA_RVS][j], rcs = in[IA_RCS][j], ris = in[IA_RIS][j], ths = in[IA_THS][j];
+      double o[ORACLE_ADJ_NOUT_POINT];
+      ice_adjust_point(dt, sigqsat[i], in[IA_PABS][j], in[IA_SIGS][j], in[IA_TH][j], in[IA_EXN][j],
+                       in[IA_EXNREF][j], in[IA_RHODREF][j], in[IA_RV][j], in[IA_RC][j],
+                       in[IA_RI][j], in[IA_RR][j], in[IA_RS][j], in[IA_RG][j], in[IA_CF_MF][j],
+                       in[IA_RC_MF][j], in[IA_RI_MF][j], &rvs, &rcs, &ris, &ths, o);
+      for (int m = 0; m < ORACLE_ADJ_NOUT_POINT; ++m) out[m][j] = o[m];
+      out[OA_RVS][j] = rvs;
+      out[OA_RCS][j] = rcs;
+      out[OA_RIS][j] = ris;
+      out[OA_THS][j] = ths;
+    }
+  return 0;
+}
+
+/* ============================================================== RAIN_ICE */
+
+/* ICE4_NUCLEATION (micro/ice4_nucleation.func.h:6-144), .NOT.LOCND2, LFEEDBACKT */
+static void ice4_nucleation(int ldcompute, double tht, double pabst, double rhodref, double exn,
+                            double lsfact, double t, double rvt, double *cit, double *rvheni_mr) {
+  int gnegt = ldcompute ? (t < P.XTT && rvt > P.XRTMIN[1]) : 0;
+  double usw = 0., zzw = 0., ssi = 0., zw;
+  if (gnegt) {
+    zzw = log(t);
+    usw = exp(P.XALPW - P.XBETAW / t - P.XGAMW * zzw);
+    zzw = exp(P.XALPI - P.XBETAI / t - P.XGAMI * zzw);
+  }
+  if (gnegt) {
+    zzw = dmin(pabst / 2., zzw);
+    ssi = rvt * (pabst - zzw) / (P.XEPSILO * zzw) - 1.0;
+    usw = dmin(pabst / 2., usw);
+    usw = (usw / zzw) * ((pabst - zzw) / (pabst - usw)) - 1.0;
+    ssi = dmin(ssi, usw);
+  }
+  zzw = 0.;
+  if (gnegt) {
+    if (t < P.XTT - 5.0 && ssi > 0.0)
+      zzw = P.XNU20 * exp(P.XALPHA2 * ssi - P.XBETA2);
+    else if (t <= P.XTT - 2.0 && t >= P.XTT - 5.0 && ssi > 0.0)
+      zzw = dmax(P.XNU20 * exp(-P.XBETA2),
+                 P.XNU10 * exp(-P.XBETA1 * (t - P.XTT)) * pow(ssi / usw, P.XALPHA1));
+  }
+  if (gnegt) {
+    zzw = zzw - *cit;
+    zzw = dmin(zzw, 50.E3);
+  }
+  *rvheni_mr = 0.;
+  if (gnegt) {
+    *rvheni_mr = dmax(zzw, 0.0) * P.XMNU0 / rhodref;
+    *rvheni_mr = dmin(rvt, *rvheni_mr);
+  }
+  zw = 0.;
+  if (gnegt)
+    zw = dmin(*rvheni_mr, dmax(0., (P.XTT / exn - tht) / lsfact)) / dmax(*rvheni_mr, 1.E-20);
+  *rvheni_mr = *rvheni_mr * zw;
+  zzw = zzw * zw;
+  if (gnegt) *cit = dmax(zzw + *cit, *cit);
+}
+
+/* Per-point persistent state of ICE4_STEPPING (micro/mode_ice4_stepping.F90:128-158) */
+typedef struct {
+  double bu[IBUNUM + 1];   /* ZBU_INST */
+  double rs_tend[9];       /* ZRS_TEND (1-based) */
+  double rg_tend[9];       /* ZRG_TEND (1-based) */
+  double ssi;              /* ZSSI */
+  double hlc_hcf, hlc_lcf, hlc_hrc, hlc_lrc, hli_hcf, hli_lcf, hli_hri, hli_lri;
+} step_state_t;
+
+/* ICE4_TENDENCIES (micro/mode_ice4_tendencies.F90:9-726) for one point, with the
+ * process routines it calls inlined in call order. */
+static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, double lvfact,
+                            double lsfact, double pres, double cf, double sigma_rc, double *cit,
+                            double pt, const double *pvart, step_state_t *S, double *pa,
+                            double *pb, double *rainfr) {
+  double *bu = S->bu, *rs_tend = S->rs_tend, *rg_tend = S->rg_tend;
+  double zt = pt, zvart[7];
+  double ka = 0., dv = 0., ai = 0., cj = 0.;
+  for (int jv = 0; jv < 7; ++jv) { zvart[jv] = pvart[jv]; pa[jv] = 0.; pb[jv] = 0.; }
+  if (soft) {
+    bu[IRVHENI_MR] = 0.; bu[IRRHONG_MR] = 0.; bu[IRIMLTC_MR] = 0.; bu[IRSRIMCG_MR] = 0.;
+  } else {
+    /* 2. nucleation (:180-193) */
+    ice4_nucleation(ldc, zvart[ITH], pres, rhodref, exn, lsfact, zt, zvart[IRV], cit,
+                    &bu[IRVHENI_MR]);
+    zvart[ITH] = zvart[ITH] + bu[IRVHENI_MR] * lsfact;
+    zt = zvart[ITH] * exn;
+    zvart[IRV] = zvart[IRV] - bu[IRVHENI_MR];
+    zvart[IRI] = zvart[IRI] + bu[IRVHENI_MR];
+    /* 3.3 RRHONG (mode_ice4_rrhong.F90:65-75) */
+    if (zt < P.XTT - 35.0 && zvart[IRR] > P.XRTMIN[3] && ldc) {
+      bu[IRRHONG_MR] = zvart[IRR];
+      bu[IRRHONG_MR] = dmin(bu[IRRHONG_MR],
+                            dmax(0., ((P.XTT - 35.) / exn - zvart[ITH]) / (lsfact - lvfact)));
+    } else {
+      bu[IRRHONG_MR] = 0.;
+    }
+    zvart[ITH] = zvart[ITH] + bu[IRRHONG_MR] * (lsfact - lvfact);
+    zt = zvart[ITH] * exn;
+    zvart[IRR] = zvart[IRR] - bu[IRRHONG_MR];
+    zvart[IRG] = zvart[IRG] + bu[IRRHONG_MR];
+    /* 7.1 RIMLTC (mode_ice4_rimltc.F90:64-74) */
+    if (zvart[IRI] > 0. && zt > P.XTT && ldc) {
+      bu[IRIMLTC_MR] = zvart[IRI];
+      bu[IRIMLTC_MR] = dmin(bu[IRIMLTC_MR],
+                            dmax(0., (zvart[ITH] - P.XTT / exn) / (lsfact - lvfact)));
+    } else {
+      bu[IRIMLTC_MR] = 0.;
+    }
+    zvart[ITH] = zvart[ITH] - bu[IRIMLTC_MR] * (lsfact - lvfact);
+    zt = zvart[ITH] * exn;
+    zvart[IRC] = zvart[IRC] + bu[IRIMLTC_MR];
+    zvart[IRI] = zvart[IRI] - bu[IRIMLTC_MR];
+    bu[IRSRIMCG_MR] = 0.;
+    /* instantaneous changes (:278-296) */
+    pb[ITH] = pb[ITH] + bu[IRVHENI_MR] * lsfact;
+    pb[ITH] = pb[ITH] + bu[IRRHONG_MR] * (lsfact - lvfact);
+    pb[ITH] = pb[ITH] - bu[IRIMLTC_MR] * (lsfact - lvfact);
+    pb[IRV] = pb[IRV] - bu[IRVHENI_MR];
+    pb[IRC] = pb[IRC] + bu[IRIMLTC_MR];
+    pb[IRR] = pb[IRR] - bu[IRRHONG_MR];
+    pb[IRI] = pb[IRI] + bu[IRVHENI_MR];
+    pb[IRI] = pb[IRI] - bu[IRIMLTC_MR];
+    pb[IRS] = pb[IRS] - bu[IRSRIMCG_MR];
+    pb[IRG] = pb[IRG] + bu[IRRHONG_MR];
+    pb[IRG] = pb[IRG] + bu[IRSRIMCG_MR];
+    /* derived fields (:303-346), not LOCND2 */
+    double zzw = exp(P.XALPI - P.XBETAI / zt - P.XGAMI * log(zt));
+    S->ssi = zvart[IRV] * (pres - zzw) / (P.XEPSILO * zzw) - 1.0;
+    ka = 2.38E-2 + 0.0071E-2 * (zt - P.XTT);
+    dv = 0.211E-4 * pow(zt / P.XTT, 1.94) * (P.XP00 / pres);
+    ai = sq(P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / (ka * P.XRV * sq(zt)) +
+         (P.XRV * zt) / (dv * zzw);
+    cj = P.XSCFAC * pow(rhodref, 0.3) / sqrt(1.718E-5 + 0.0049E-5 * (zt - P.XTT));
+  }
+  const double colf = 1.0, acrf = 1.0; /* ZCOLF, ZACRF without LOCND2 (:323-324) */
+
+  /* ICE4_COMPUTE_PDF (mode_ice4_compute_pdf.F90), rc: 'PDF'+'SIGM', ri: 'NONE' */
+  {
+    double rcrautc = ldc ? P.XCRIAUTC / rhodref : 0.;
+    double rc = zvart[IRC], ri = zvart[IRI];
+    if (rc > rcrautc + sigma_rc && ldc) {
+      S->hlc_hcf = 1.; S->hlc_lcf = 0.; S->hlc_hrc = rc; S->hlc_lrc = 0.;
+    } else if (rc > (rcrautc - sigma_rc) && rc <= (rcrautc + sigma_rc) && ldc) {
+      S->hlc_hcf = (rc + sigma_rc - rcrautc) / (2. * sigma_rc);
+      S->hlc_lcf = dmax(0., cf - S->hlc_hcf);
+      S->hlc_hrc = (rc + sigma_rc - rcrautc) * (rc + sigma_rc + rcrautc) / (4. * sigma_rc);
+      S->hlc_lrc = dmax(0., rc - S->hlc_hrc);
+    } else if (rc > P.XRTMIN[2] && cf > 0. && ldc) {
+      S->hlc_hcf = 0.; S->hlc_lcf = cf; S->hlc_hrc = 0.; S->hlc_lrc = rc;
+    } else {
+      S->hlc_hcf = 0.; S->hlc_lcf = 0.; S->hlc_hrc = 0.; S->hlc_lrc = 0.;
+    }
+    double criauti = ldc ? dmin(P.XCRIAUTI, pow(10., P.XACRIAUTI * (zt - P.XTT) + P.XBCRIAUTI)) : 0.;
+    if (ri > criauti && ldc) {
+      S->hli_hcf = 1.; S->hli_lcf = 0.; S->hli_hri = ri; S->hli_lri = 0.;
+    } else if (ri > P.XRTMIN[4] && ldc) {
+      S->hli_hcf = 0.; S->hli_lcf = 1.; S->hli_hri = 0.; S->hli_lri = ri;
+    } else {
+      S->hli_hcf = 0.; S->hli_lcf = 0.; S->hli_hri = 0.; S->hli_lri = 0.;
+    }
+  }
+  *rainfr = 1.; /* no PRFR option (:379-383) */
+
+  /* slope parameters (:389-433), LSNOW_T=F, KRR=6 */
+  double lbdar = zvart[IRR] > 0. ? P.XLBR * pow(rhodref * dmax(zvart[IRR], P.XRTMIN[3]), P.XLBEXR) : 0.;
+  double lbdas = zvart[IRS] > 0.
+                     ? dmin(P.XLBDAS_MAX, P.XLBS * pow(rhodref * dmax(zvart[IRS], P.XRTMIN[5]), P.XLBEXS))
+                     : 0.;
+  double lbdag = zvart[IRG] > 0. ? P.XLBG * pow(rhodref * dmax(zvart[IRG], P.XRTMIN[6]), P.XLBEXG) : 0.;
+  const double rv = zvart[IRV], rc = zvart[IRC], rr = zvart[IRR], ri = zvart[IRI],
+               rs = zvart[IRS], rg = zvart[IRG];
+  const double ssi = S->ssi;
+
+  /* ICE4_SLOW (mode_ice4_slow.F90:106-217), LNEWCOEFF=F, ZREDGR=ZREDSN=1 */
+  if (zt < P.XTT - 35.0 && rc > P.XRTMIN[2] && ldc) {
+    if (!soft)
+      bu[IRCHONI] = dmin(1000., P.XHON * rhodref * rc * exp(P.XALPHA3 * (zt - P.XTT) - P.XBETA3));
+  } else bu[IRCHONI] = 0.;
+  if (rv > P.XRTMIN[1] && rs > P.XRTMIN[5] && ldc) {
+    if (!soft) {
+      bu[IRVDEPS] = (ssi / (rhodref * ai)) *
+                    (P.X0DEPS * pow(lbdas, P.XEX0DEPS) + P.X1DEPS * cj * pow(lbdas, P.XEX1DEPS));
+      bu[IRVDEPS] = bu[IRVDEPS] * 1.;
+    }
+  } else bu[IRVDEPS] = 0.;
+  if (ri > P.XRTMIN[4] && rs > P.XRTMIN[5] && ldc) {
+    if (!soft)
+      bu[IRIAGGS] = P.XFIAGGS * exp(P.XCOLEXIS * (zt - P.XTT)) * ri * pow(lbdas, P.XEXIAGGS) *
+                    pow(rhodref, -P.XCEXVT);
+  } else bu[IRIAGGS] = 0.;
+  if (S->hli_hri > P.XRTMIN[4] && ldc) {
+    if (!soft) {
+      double criauti = dmin(P.XCRIAUTI, pow(10., P.XACRIAUTI * (zt - P.XTT) + P.XBCRIAUTI));
+      bu[IRIAUTS] = P.XTIMAUTI * exp(P.XTEXAUTI * (zt - P.XTT)) *
+                    dmax(S->hli_hri - criauti * S->hli_hcf, 0.);
+    }
+  } else bu[IRIAUTS] = 0.;
+  if (rv > P.XRTMIN[1] && rg > P.XRTMIN[6] && ldc) {
+    if (!soft) {
+      bu[IRVDEPG] = (ssi / (rhodref * ai)) *
+                    (P.X0DEPG * pow(lbdag, P.XEX0DEPG) + P.X1DEPG * cj * pow(lbdag, P.XEX1DEPG));
+      bu[IRVDEPG] = bu[IRVDEPG] * 1.;
+    }
+  } else bu[IRVDEPG] = 0.;
+
+  /* ICE4_WARM (mode_ice4_warm.F90:100-197), HSUBG_RC_RR_ACCR='NONE', HSUBG_RR_EVAP='NONE' */
+  if (S->hlc_hrc > P.XRTMIN[2] && S->hlc_hcf > 0. && ldc) {
+    if (!soft)
+      bu[IRCAUTR] = P.XTIMAUTC * dmax(S->hlc_hrc - S->hlc_hcf * P.XCRIAUTC / rhodref, 0.0);
+  } else bu[IRCAUTR] = 0.;
+  if (rc > P.XRTMIN[2] && rr > P.XRTMIN[3] && ldc) {
+    if (!soft)
+      bu[IRCACCR] = P.XFCACCR * rc * acrf * pow(lbdar, P.XEXCACCR) * pow(rhodref, -P.XCEXVT);
+  } else bu[IRCACCR] = 0.;
+  if (rr > P.XRTMIN[3] && rc <= P.XRTMIN[2] && ldc) {
+    if (!soft) {
+      double ev = exp(P.XALPW - P.XBETAW / zt - P.XGAMW * log(zt));
+      double usw = 1. - rv * (pres - ev) / (P.XEPSILO * ev);
+      double x = sq(P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / (ka * P.XRV * sq(zt)) +
+                 (P.XRV * zt) / (dv * ev);
+      bu[IRREVAV] = (dmax(0., usw) / (rhodref * x)) *
+                    (P.X0EVAR * pow(lbdar, P.XEX0EVAR) + P.X1EVAR * cj * pow(lbdar, P.XEX1EVAR));
+    }
+  } else bu[IRREVAV] = 0.;
+
+  /* ICE4_FAST_RS (mode_ice4_fast_rs.F90:109-441), LNEWCOEFF=F, LEVLIMIT=T, M90 */
+  double freez;
+  if (rs > P.XRTMIN[5] && ldc) {
+    if (!soft) {
+      double f1 = rv * pres / (P.XEPSILO + rv);
+      f1 = dmin(f1, exp(P.XALPI - P.XBETAI / zt - P.XGAMI * log(zt)));
+      f1 = ka * (P.XTT - zt) +
+           (dv * (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) * (P.XESTT - f1) / (P.XRV * zt));
+      f1 = f1 * (P.X0DEPS * pow(lbdas, P.XEX0DEPS) + P.X1DEPS * cj * pow(lbdas, P.XEX1DEPS)) /
+           (rhodref * (P.XLMTT - P.XCL * (P.XTT - zt)));
+      rs_tend[RS_FREEZ1] = f1;
+      rs_tend[RS_FREEZ2] = (rhodref * (P.XLMTT + (P.XCI - P.XCL) * (P.XTT - zt))) /
+                           (rhodref * (P.XLMTT - P.XCL * (P.XTT - zt)));
+    }
+    freez = dmax(0., dmax(0., rs_tend[RS_FREEZ1] + rs_tend[RS_FREEZ2] * bu[IRIAGGS]) - bu[IRIAGGS]);
+  } else {
+    rs_tend[RS_FREEZ1] = 0.; rs_tend[RS_FREEZ2] = 0.; freez = 0.;
+  }
+  int grim = rc > P.XRTMIN[2] && rs > P.XRTMIN[5] && ldc;
+  if (!grim) { rs_tend[RS_RCRIMS] = 0.; rs_tend[RS_RCRIMSS] = 0.; rs_tend[RS_RSRIMCG] = 0.; }
+  if (!soft && grim) {
+    double zzw1 = interp1d(lbdas, P.NGAMINC, P.XRIMINTP1, P.XRIMINTP2, P.GAMINC_RIM1);
+    double zzw2 = interp1d(lbdas, P.NGAMINC, P.XRIMINTP1, P.XRIMINTP2, P.GAMINC_RIM2);
+    double zzw3 = interp1d(lbdas, P.NGAMINC, P.XRIMINTP1, P.XRIMINTP2, P.GAMINC_RIM4);
+    rs_tend[RS_RCRIMSS] = P.XCRIMSS * zzw1 * rc * colf * pow(lbdas, P.XEXCRIMSS) * pow(rhodref, -P.XCEXVT);
+    rs_tend[RS_RCRIMS] = P.XCRIMSG * rc * colf * pow(lbdas, P.XEXCRIMSG) * pow(rhodref, -P.XCEXVT);
+    double zzw = rs_tend[RS_RCRIMS] - rs_tend[RS_RCRIMSS];
+    double x = P.XSRIMCG * pow(lbdas, P.XEXSRIMCG) * (1.0 - zzw2);
+    rs_tend[RS_RSRIMCG] = zzw * x /
+                          dmax(1.E-20, P.XSRIMCG3 * P.XSRIMCG2 * pow(lbdas, P.XEXSRIMCG2) * (1. - zzw3) -
+                                           P.XSRIMCG3 * x);
+  }
+  if (grim && zt < P.XTT) {
+    bu[IRCRIMSS] = dmin(freez, rs_tend[RS_RCRIMSS]);
+    freez = dmax(0., freez - bu[IRCRIMSS]);
+    double zzw0d = dmin(1., freez / dmax(1.E-20, rs_tend[RS_RCRIMS] - bu[IRCRIMSS]));
+    bu[IRCRIMSG] = zzw0d * dmax(0., rs_tend[RS_RCRIMS] - bu[IRCRIMSS]);
+    freez = dmax(0., freez - bu[IRCRIMSG]);
+    bu[IRSRIMCG] = zzw0d * rs_tend[RS_RSRIMCG];
+    bu[IRSRIMCG] = bu[IRSRIMCG] * dmax(0., -copysign(1., -bu[IRCRIMSG]));
+    bu[IRCRIMSG] = dmax(0., bu[IRCRIMSG]);
+  } else {
+    bu[IRCRIMSS] = 0.; bu[IRCRIMSG] = 0.; bu[IRSRIMCG] = 0.;
+  }
+  int gacc = rr > P.XRTMIN[3] && rs > P.XRTMIN[5] && ldc;
+  if (!gacc) { rs_tend[RS_RRACCS] = 0.; rs_tend[RS_RRACCSS] = 0.; rs_tend[RS_RSACCRG] = 0.; }
+  if (!soft) {
+    rs_tend[RS_RRACCS] = 0.; rs_tend[RS_RRACCSS] = 0.; rs_tend[RS_RSACCRG] = 0.;
+    if (gacc) {
+      double zzw1, zzw2, zzw3;
+      interp2d(lbdas, lbdar, P.NACCLBDAS, P.NACCLBDAR, P.XACCINTP1S, P.XACCINTP2S, P.XACCINTP1R,
+               P.XACCINTP2R, P.KER_RACCSS, &zzw1, P.KER_RACCS, &zzw2, P.KER_SACCRG, &zzw3);
+      double zzw = P.XFRACCSS * (pow(lbdas, P.XCXS)) * (pow(rhodref, -P.XCEXVT - 1.)) *
+                   (P.XLBRACCS1 / (sq(lbdas)) + P.XLBRACCS2 / (lbdas * lbdar) +
+                    P.XLBRACCS3 / (sq(lbdar))) / pow4(lbdar);
+      rs_tend[RS_RRACCSS] = zzw1 * zzw;
+      rs_tend[RS_RRACCS] = zzw2 * zzw;
+      rs_tend[RS_RSACCRG] = P.XFSACCRG * zzw3 * (pow(lbdas, P.XCXS - P.XBS)) *
+                            (pow(rhodref, -P.XCEXVT - 1.)) *
+                            (P.XLBSACCR1 / (sq(lbdar)) + P.XLBSACCR2 / (lbdar * lbdas) +
+                             P.XLBSACCR3 / (sq(lbdas))) / lbdar;
+    }
+  }
+  if (gacc && zt < P.XTT) {
+    bu[IRRACCSS] = dmin(freez, rs_tend[RS_RRACCSS]);
+    freez = dmax(0., freez - bu[IRRACCSS]);
+    double zzw = dmin(1., freez / dmax(1.E-20, rs_tend[RS_RRACCS] - bu[IRRACCSS]));
+    bu[IRRACCSG] = zzw * dmax(0., rs_tend[RS_RRACCS] - bu[IRRACCSS]);
+    freez = dmax(0., freez - bu[IRRACCSG]);
+    bu[IRSACCRG] = zzw * rs_tend[RS_RSACCRG];
+    bu[IRSACCRG] = bu[IRSACCRG] * dmax(0., -copysign(1., -bu[IRRACCSG]));
+    bu[IRRACCSG] = dmax(0., bu[IRRACCSG]);
+  } else {
+    bu[IRRACCSS] = 0.; bu[IRRACCSG] = 0.; bu[IRSACCRG] = 0.;
+  }
+  if (rs > P.XRTMIN[5] && zt > P.XTT && ldc) {
+    if (!soft) {
+      double m = rv * pres / (P.XEPSILO + rv);
+      m = dmin(m, exp(P.XALPW - P.XBETAW / zt - P.XGAMW * log(zt)));
+      m = ka * (P.XTT - zt) +
+          (dv * (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) * (P.XESTT - m) / (P.XRV * zt));
+      bu[IRSMLTG] = P.XFSCVMG *
+                    dmax(0., (-m * (P.X0DEPS * pow(lbdas, P.XEX0DEPS) +
+                                    P.X1DEPS * cj * pow(lbdas, P.XEX1DEPS)) -
+                              (rs_tend[RS_RCRIMS] + rs_tend[RS_RRACCS]) *
+                                  (rhodref * P.XCL * (P.XTT - zt))) /
+                                 (rhodref * P.XLMTT));
+      bu[IRCMLTSR] = rs_tend[RS_RCRIMS];
+    }
+  } else {
+    bu[IRSMLTG] = 0.; bu[IRCMLTSR] = 0.;
+  }
+
+  /* ICE4_FAST_RG (mode_ice4_fast_rg.F90:121-410), KRR=6, LCRFLIMIT=T, LNULLWETG=T, LWETGPOST=T */
+  if (ri > P.XRTMIN[4] && rr > P.XRTMIN[3] && ldc) {
+    if (!soft) {
+      bu[IRICFRRG] = P.XICFRR * ri * pow(lbdar, P.XEXICFRR) * pow(rhodref, -P.XCEXVT);
+      bu[IRRCFRIG] = P.XRCFRI * *cit * pow(lbdar, P.XEXRCFRI) * pow(rhodref, -P.XCEXVT - 1.);
+      double zzw0d = dmax(0., dmin(1., (bu[IRICFRRG] * P.XCI + bu[IRRCFRIG] * P.XCL) * (P.XTT - zt) /
+                                           dmax(1.E-20, P.XLVTT * bu[IRRCFRIG])));
+      bu[IRRCFRIG] = zzw0d * bu[IRRCFRIG];
+      bu[IRICFRR] = (1. - zzw0d) * bu[IRICFRRG];
+      bu[IRICFRRG] = zzw0d * bu[IRICFRRG];
+    }
+  } else {
+    bu[IRICFRRG] = 0.; bu[IRRCFRIG] = 0.; bu[IRICFRR] = 0.;
+  }
+  if (rg > P.XRTMIN[6] && rc > P.XRTMIN[2] && ldc) {
+    if (!soft) {
+      rg_tend[RG_RCDRYG] = pow(lbdag, P.XCXG - P.XDG - 2.) * pow(rhodref, -P.XCEXVT);
+      rg_tend[RG_RCDRYG] = P.XFCDRYG * rc * rg_tend[RG_RCDRYG];
+    }
+  } else rg_tend[RG_RCDRYG] = 0.;
+  if (rg > P.XRTMIN[6] && ri > P.XRTMIN[4] && ldc) {
+    if (!soft) {
+      rg_tend[RG_RIDRYG] = pow(lbdag, P.XCXG - P.XDG - 2.) * pow(rhodref, -P.XCEXVT);
+      rg_tend[RG_RIDRYG] = P.XFIDRYG * exp(P.XCOLEXIG * (zt - P.XTT)) * ri * rg_tend[RG_RIDRYG];
+      rg_tend[RG_RIWETG] = rg_tend[RG_RIDRYG] / (P.XCOLIG * exp(P.XCOLEXIG * (zt - P.XTT)));
+    }
+  } else {
+    rg_tend[RG_RIDRYG] = 0.; rg_tend[RG_RIWETG] = 0.;
+  }
+  int gdry = rs > P.XRTMIN[5] && rg > P.XRTMIN[6] && ldc;
+  if (!gdry) { rg_tend[RG_RSDRYG] = 0.; rg_tend[RG_RSWETG] = 0.; }
+  if (!soft && gdry) {
+    double zzw;
+    interp2d(lbdag, lbdas, P.NDRYLBDAG, P.NDRYLBDAS, P.XDRYINTP1G, P.XDRYINTP2G, P.XDRYINTP1S,
+             P.XDRYINTP2S, P.KER_SDRYG, &zzw, NULL, NULL, NULL, NULL);
+    rg_tend[RG_RSWETG] = P.XFSDRYG * zzw / P.XCOLSG * (pow(lbdas, P.XCXS - P.XBS)) *
+                         (pow(lbdag, P.XCXG)) * (pow(rhodref, -P.XCEXVT - 1.)) *
+                         (P.XLBSDRYG1 / (sq(lbdag)) + P.XLBSDRYG2 / (lbdag * lbdas) +
+                          P.XLBSDRYG3 / (sq(lbdas)));
+    rg_tend[RG_RSDRYG] = rg_tend[RG_RSWETG] * P.XCOLSG * exp(P.XCOLEXSG * (zt - P.XTT));
+  }
+  gdry = rr > P.XRTMIN[3] && rg > P.XRTMIN[6] && ldc;
+  if (!gdry) rg_tend[RG_RRDRYG] = 0.;
+  if (!soft && gdry) {
+    double zzw;
+    interp2d(lbdag, lbdar, P.NDRYLBDAG, P.NDRYLBDAR, P.XDRYINTP1G, P.XDRYINTP2G, P.XDRYINTP1R,
+             P.XDRYINTP2R, P.KER_RDRYG, &zzw, NULL, NULL, NULL, NULL);
+    rg_tend[RG_RRDRYG] = P.XFRDRYG * zzw * (1.0 / pow4(lbdar)) * (pow(lbdag, P.XCXG)) *
+                         (pow(rhodref, -P.XCEXVT - 1.)) *
+                         (P.XLBRDRYG1 / (sq(lbdag)) + P.XLBRDRYG2 / (lbdag * lbdar) +
+                          P.XLBRDRYG3 / (sq(lbdar)));
+  }
+  double rdryg_init = rg_tend[RG_RCDRYG] + rg_tend[RG_RIDRYG] + rg_tend[RG_RSDRYG] + rg_tend[RG_RRDRYG];
+  double rwetg_init;
+  int ldwetg, lldryg;
+  if (rg > P.XRTMIN[6] && ldc) {
+    if (!soft) {
+      double f1 = rv * pres / (P.XEPSILO + rv);
+      f1 = dmin(f1, exp(P.XALPI - P.XBETAI / zt - P.XGAMI * log(zt)));
+      f1 = ka * (P.XTT - zt) +
+           (dv * (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) * (P.XESTT - f1) / (P.XRV * zt));
+      f1 = f1 * (P.X0DEPG * pow(lbdag, P.XEX0DEPG) + P.X1DEPG * cj * pow(lbdag, P.XEX1DEPG)) /
+           (rhodref * (P.XLMTT - P.XCL * (P.XTT - zt)));
+      rg_tend[RG_FREEZ1] = f1;
+      rg_tend[RG_FREEZ2] = (rhodref * (P.XLMTT + (P.XCI - P.XCL) * (P.XTT - zt))) /
+                           (rhodref * (P.XLMTT - P.XCL * (P.XTT - zt)));
+    }
+    rwetg_init = dmax(rg_tend[RG_RIWETG] + rg_tend[RG_RSWETG],
+                      dmax(0., rg_tend[RG_FREEZ1] +
+                                   rg_tend[RG_FREEZ2] * (rg_tend[RG_RIWETG] + rg_tend[RG_RSWETG])));
+    ldwetg = dmax(0., rwetg_init - rg_tend[RG_RIWETG] - rg_tend[RG_RSWETG]) <=
+             dmax(0., rdryg_init - rg_tend[RG_RIDRYG] - rg_tend[RG_RSDRYG]);
+    ldwetg = ldwetg && rdryg_init > 0.;
+    lldryg = zt < P.XTT && rdryg_init > 1.E-20 &&
+             dmax(0., rwetg_init - rg_tend[RG_RIWETG] - rg_tend[RG_RSWETG]) >
+                 dmax(0., rdryg_init - rg_tend[RG_RIDRYG] - rg_tend[RG_RSDRYG]);
+  } else {
+    rg_tend[RG_FREEZ1] = 0.; rg_tend[RG_FREEZ2] = 0.;
+    rwetg_init = 0.; ldwetg = 0; lldryg = 0;
+  }
+  bu[IRWETGH] = 0.; bu[IRWETGH_MR] = 0.;
+  if (ldwetg) {
+    bu[IRRWETG] = -(rg_tend[RG_RIWETG] + rg_tend[RG_RSWETG] + rg_tend[RG_RCDRYG] - rwetg_init);
+    bu[IRCWETG] = rg_tend[RG_RCDRYG];
+    bu[IRIWETG] = rg_tend[RG_RIWETG];
+    bu[IRSWETG] = rg_tend[RG_RSWETG];
+  } else {
+    bu[IRRWETG] = 0.; bu[IRCWETG] = 0.; bu[IRIWETG] = 0.; bu[IRSWETG] = 0.;
+  }
+  if (lldryg) {
+    bu[IRCDRYG] = rg_tend[RG_RCDRYG];
+    bu[IRRDRYG] = rg_tend[RG_RRDRYG];
+    bu[IRIDRYG] = rg_tend[RG_RIDRYG];
+    bu[IRSDRYG] = rg_tend[RG_RSDRYG];
+  } else {
+    bu[IRCDRYG] = 0.; bu[IRRDRYG] = 0.; bu[IRIDRYG] = 0.; bu[IRSDRYG] = 0.;
+  }
+  if (rg > P.XRTMIN[6] && zt > P.XTT && ldc) {
+    if (!soft) {
+      double m = rv * pres / (P.XEPSILO + rv);
+      m = dmin(m, exp(P.XALPW - P.XBETAW / zt - P.XGAMW * log(zt)));
+      m = ka * (P.XTT - zt) +
+          dv * (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) * (P.XESTT - m) / (P.XRV * zt);
+      bu[IRGMLTR] = dmax(0., (-m * (P.X0DEPG * pow(lbdag, P.XEX0DEPG) +
+                                     P.X1DEPG * cj * pow(lbdag, P.XEX1DEPG)) -
+                              (rg_tend[RG_RCDRYG] + rg_tend[RG_RRDRYG]) *
+                                  (rhodref * P.XCL * (P.XTT - zt))) /
+                                 (rhodref * P.XLMTT));
+    }
+  } else bu[IRGMLTR] = 0.;
+
+  /* ICE4_FAST_RI (mode_ice4_fast_ri.F90:70-81) */
+  if (ssi > 0. && rc > P.XRTMIN[2] && ri > P.XRTMIN[4] && *cit > 1.E-20 && ldc) {
+    if (!soft) {
+      double l = dmin(1.E8, P.XLBI * pow(rhodref * ri / *cit, P.XLBEXI));
+      bu[IRCBERI] = (ssi / (rhodref * ai)) * *cit *
+                    (P.X0DEPI / l + P.X2DEPI * cj * cj / pow(l, P.XDI + 2.0));
+    }
+  } else bu[IRCBERI] = 0.;
+
+  /* aggregation into rates PA and instantaneous changes PB (:606-717), KRR=6, no LOCND2 */
+  double ls_lv = lsfact - lvfact;
+  pa[ITH] = pa[ITH] + bu[IRVDEPG] * lsfact;
+  pa[ITH] = pa[ITH] + bu[IRCHONI] * ls_lv;
+  pa[ITH] = pa[ITH] + bu[IRVDEPS] * lsfact;
+  pa[ITH] = pa[ITH] - bu[IRREVAV] * lvfact;
+  pa[ITH] = pa[ITH] + bu[IRCRIMSS] * ls_lv;
+  pa[ITH] = pa[ITH] + bu[IRCRIMSG] * ls_lv;
+  pa[ITH] = pa[ITH] + bu[IRRACCSS] * ls_lv;
+  pa[ITH] = pa[ITH] + bu[IRRACCSG] * ls_lv;
+  pa[ITH] = pa[ITH] + (bu[IRRCFRIG] - bu[IRICFRR]) * ls_lv;
+  pa[ITH] = pa[ITH] + (bu[IRCWETG] + bu[IRRWETG]) * ls_lv;
+  pa[ITH] = pa[ITH] + (bu[IRCDRYG] + bu[IRRDRYG]) * ls_lv;
+  pa[ITH] = pa[ITH] - bu[IRGMLTR] * ls_lv;
+  pa[ITH] = pa[ITH] + bu[IRCBERI] * ls_lv;
+
+  pa[IRV] = pa[IRV] - bu[IRVDEPG];
+  pa[IRV] = pa[IRV] - bu[IRVDEPS];
+  pa[IRV] = pa[IRV] + bu[IRREVAV];
+
+  pa[IRC] = pa[IRC] - bu[IRCHONI];
+  pa[IRC] = pa[IRC] - bu[IRCAUTR];
+  pa[IRC] = pa[IRC] - bu[IRCACCR];
+  pa[IRC] = pa[IRC] - bu[IRCRIMSS];
+  pa[IRC] = pa[IRC] - bu[IRCRIMSG];
+  pa[IRC] = pa[IRC] - bu[IRCMLTSR];
+  pa[IRC] = pa[IRC] - bu[IRCWETG];
+  pa[IRC] = pa[IRC] - bu[IRCDRYG];
+  pa[IRC] = pa[IRC] - bu[IRCBERI];
+
+  pa[IRR] = pa[IRR] + bu[IRCAUTR];
+  pa[IRR] = pa[IRR] + bu[IRCACCR];
+  pa[IRR] = pa[IRR] - bu[IRREVAV];
+  pa[IRR] = pa[IRR] - bu[IRRACCSS];
+  pa[IRR] = pa[IRR] - bu[IRRACCSG];
+  pa[IRR] = pa[IRR] + bu[IRCMLTSR];
+  pa[IRR] = pa[IRR] - bu[IRRCFRIG] + bu[IRICFRR];
+  pa[IRR] = pa[IRR] - bu[IRRWETG];
+  pa[IRR] = pa[IRR] - bu[IRRDRYG];
+  pa[IRR] = pa[IRR] + bu[IRGMLTR];
+
+  pa[IRI] = pa[IRI] + bu[IRCHONI];
+  pa[IRI] = pa[IRI] - bu[IRIAGGS];
+  pa[IRI] = pa[IRI] - bu[IRIAUTS];
+  pa[IRI] = pa[IRI] - bu[IRICFRRG] - bu[IRICFRR];
+  pa[IRI] = pa[IRI] - bu[IRIWETG];
+  pa[IRI] = pa[IRI] - bu[IRIDRYG];
+  pa[IRI] = pa[IRI] + bu[IRCBERI];
+
+  pa[IRS] = pa[IRS] + bu[IRVDEPS];
+  pa[IRS] = pa[IRS] + bu[IRIAGGS];
+  pa[IRS] = pa[IRS] + bu[IRIAUTS];
+  pa[IRS] = pa[IRS] + bu[IRCRIMSS];
+  pa[IRS] = pa[IRS] - bu[IRSRIMCG];
+  pa[IRS] = pa[IRS] + bu[IRRACCSS];
+  pa[IRS] = pa[IRS] - bu[IRSACCRG];
+  pa[IRS] = pa[IRS] - bu[IRSMLTG];
+  pa[IRS] = pa[IRS] - bu[IRSWETG];
+  pa[IRS] = pa[IRS] - bu[IRSDRYG];
+
+  pa[IRG] = pa[IRG] + bu[IRVDEPG] * 1.0;
+  pa[IRG] = pa[IRG] + bu[IRCRIMSG] + bu[IRSRIMCG];
+  pa[IRG] = pa[IRG] + bu[IRRACCSG] + bu[IRSACCRG];
+  pa[IRG] = pa[IRG] + bu[IRSMLTG];
+  pa[IRG] = pa[IRG] + bu[IRICFRRG] + bu[IRRCFRIG];
+  pa[IRG] = pa[IRG] + bu[IRCWETG] + bu[IRIWETG] + bu[IRSWETG] + bu[IRRWETG];
+  pa[IRG] = pa[IRG] - bu[IRWETGH];
+  pb[IRG] = pb[IRG] - bu[IRWETGH_MR];
+  pa[IRG] = pa[IRG] + bu[IRCDRYG] + bu[IRIDRYG] + bu[IRSDRYG] + bu[IRRDRYG];
+  pa[IRG] = pa[IRG] - bu[IRGMLTR];
+}
+
+#define ORACLE_MAX_CALLS 100000
+
+/* ICE4_STEPPING (micro/mode_ice4_stepping.F90:180-537) for one microphysics point,
+ * XTSTEP_TS=0, XMRSTEP=0, LFEEDBACKT=T, LDEXT_TEND=T, LDSIGMA_RC=T. */
+static int ice4_stepping_point(double dt, double exn, double rhodref, double pres, double cf,
+                               double sigs, double *cit, double *vart, double *extpk,
+                               double *rainfr, double *rrevav, int *ncalls) {
+  step_state_t S;
+  memset(&S, 0, sizeof(S));
+  double zinv = 1. / dt;
+  for (int jv = 0; jv <= 6; ++jv) extpk[jv] = extpk[jv] - vart[jv] * zinv; /* :201-205 */
+  double sigma_rc = sigs * 2.;                                              /* :211-213 */
+  double time = 0., za[7], zb[7];
+  int calls = 0;
+  while (time < dt) {                                                      /* :253 */
+    int llcompute = 1, soft = 0;
+    while (llcompute) {                                                     /* :275 */
+      if (++calls > ORACLE_MAX_CALLS) { *ncalls = calls; return -2; }
+      double sum2 = vart[IRI];
+      sum2 = sum2 + vart[IRS];
+      sum2 = sum2 + vart[IRG];
+      double zdevide = (P.XCPD + P.XCPV * vart[IRV] + P.XCL * (vart[IRC] + vart[IRR]) + P.XCI * sum2) * exn;
+      double zzt = vart[ITH] * exn;
+      double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
+      double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
+      ice4_tendencies(soft, llcompute, exn, rhodref, lvfact, lsfact, pres, cf, sigma_rc, cit, zzt,
+                      vart, &S, za, zb, rainfr);
+      for (int jv = 0; jv <= 6; ++jv) za[jv] = za[jv] + extpk[jv];          /* :320-324 */
+      double maxtime = llcompute ? (dt - time) : 0.;                        /* :335-341 */
+      double zx = P.XTT / exn;                                              /* :344-360 */
+      if ((vart[ITH] - zx) * (vart[ITH] + zb[ITH] - zx) < 0.) maxtime = 0.;
+      if (fabs(za[ITH]) > 1.E-20) {
+        double tth = (zx - zb[ITH] - vart[ITH]) / za[ITH];
+        if (tth > 0.) maxtime = dmin(maxtime, tth);
+      }
+      for (int jv = 1; jv <= 6; ++jv)                                       /* :367-375 */
+        if (za[jv] < -1.E-20 && vart[jv] > P.XRTMIN[jv]) {
+          maxtime = dmin(maxtime, -(zb[jv] + vart[jv]) / za[jv]);
+          maxtime = dmax(maxtime, P.XMNH_TINY);
+        }
+      if (time + maxtime >= dt) llcompute = 0;                              /* :380-384 */
+      for (int jv = 0; jv <= 6; ++jv) vart[jv] = vart[jv] + za[jv] * maxtime + zb[jv]; /* :461-467 */
+      if (vart[IRI] <= 0.) *cit = 0.;                                       /* :471-474 */
+      time = time + maxtime;
+      soft = 1;                                                             /* :516 */
+    }
+  }
+  for (int jv = 0; jv <= 6; ++jv) vart[jv] = vart[jv] - extpk[jv] * dt;   /* :524-530 */
+  *rrevav = S.bu[IRREVAV];                                                  /* :535-537 */
+  *ncalls = calls;
+  return 0;
+}
+
+/* ICE4_SEDIMENTATION_STAT (micro/mode_ice4_sedimentation_stat.F90:9-455) for one
+ * column, LSEDIC=F, LNEWCOEFF=F; r?s are the source arrays (updated), r?t the
+ * mixing ratios PR?S*PTSTEP (mode_ice4_sedimentation.F90:115-124). */
+static void sedimentation_column(int ncol, int nlev, int i, double dt, const double *dzz,
+                                 const double *rhodref, double *const io[ORACLE_RAIN_NIO],
+                                 double *const col[ORACLE_RAIN_NCOL]) {
+  double zinv = 1. / dt;
+  double above[7] = {0, 0, 0, 0, 0, 0, 0}, cur[7];
+  const int slot[7] = {-1, -1, RO_RCS, RO_RRS, RO_RIS, RO_RSS, RO_RGS};
+  for (int k = nlev - 1; k >= 0; --k) {
+    size_t j = (size_t)k * ncol + i;
+    double rho = rhodref[j], dz = dzz[j];
+    double tsorhodz = dt / (rho * dz);
+    cur[2] = 0.; /* cloud: LSEDIC = F */
+    for (int jrr = 3; jrr <= 6; ++jrr) {
+      double rx = io[slot[jrr]][j] * dt; /* ZR?T = PR?S*PTSTEP */
+      double qp = above[jrr] * tsorhodz, w1 = 0., w2 = 0.;
+      if (jrr == 4) { /* PRISTINE_ICE (:303-333) */
+        if ((rx > P.XRTMIN[4]) || (qp > P.XRTMIN[4])) {
+          if (rx > dmax(P.XRTMIN[4], 1.0E-7))
+            w1 = P.XFSEDI * pow(rho, -P.XCEXVT) *
+                 pow(dmax(0.05E6, -0.15319E6 - 0.021454E6 * log(rho * rx)), P.XEXCSEDI);
+          if (qp > dmax(P.XRTMIN[4], 1.0E-7))
+            w2 = P.XFSEDI * pow(rho, -P.XCEXVT) *
+                 pow(dmax(0.05E6, -0.15319E6 - 0.021454E6 * log(rho * qp)), P.XEXCSEDI);
+        }
+      } else { /* OTHER_SPECIES (:403-427) */
+        double fsed = jrr == 3 ? P.XFSEDR : (jrr == 5 ? P.XFSEDS : P.XFSEDG);
+        double exsed = jrr == 3 ? P.XEXSEDR : (jrr == 5 ? P.XEXSEDS : P.XEXSEDG);
+        if ((rx > P.XRTMIN[jrr]) || (qp > P.XRTMIN[jrr])) {
+          if (rx > P.XRTMIN[jrr]) w1 = fsed * pow(rx, exsed - 1) * pow(rho, exsed - P.XCEXVT - 1);
+          if (qp > P.XRTMIN[jrr]) w2 = fsed * pow(qp, exsed - 1) * pow(rho, exsed - P.XCEXVT - 1);
+        }
+      }
+      double f1 = dmin(rho * dz * rx * zinv, w1 * rho * rx); /* FWSED1 :446 */
+      if (w2 != 0.)
+        cur[jrr] = f1 + dmax(0., 1. - dz / (dt * w2)) * above[jrr]; /* FWSED2 :452 */
+      else
+        cur[jrr] = f1;
+    }
+    for (int jrr = 2; jrr <= 6; ++jrr) /* :202-211 */
+      io[slot[jrr]][j] = io[slot[jrr]][j] + tsorhodz * (above[jrr] - cur[jrr]) * zinv;
+    if (k == 0) { /* :213-224 */
+      col[RC_INPRR][i] = cur[3] / P.XRHOLW;
+      col[RC_INPRI][i] = cur[4] / P.XRHOLW;
+      col[RC_INPRS][i] = cur[5] / P.XRHOLW;
+      col[RC_INPRG][i] = cur[6] / P.XRHOLW;
+    }
+    for (int jrr = 2; jrr <= 6; ++jrr) above[jrr] = cur[jrr];
+  }
+}
+
+/* RAIN_ICE (micro/rain_ice.F90:7-803) for KRR=6 with the commented-out
+ * sedimentation (:336-348), total tendencies (:649-684) and negativity
+ * correction (:712-714) reinstated. */
+int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
+                    double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
+                    int *iters) {
+  if (!P_loaded) return -1;
+  int status = 0;
+  double zinv = 1. / dt;
+  /* 2. sedimentation before microphysics, per column */
+  for (int i = 0; i < ncol; ++i) {
+    col[RC_INPRC][i] = 0.;
+    sedimentation_column(ncol, nlev, i, dt, in[RI_DZZ], in[RI_RHODREF], io, col);
+    col[RC_INPRS][i] = col[RC_INPRS][i] + col[RC_INPRI][i]; /* mode_ice4_sedimentation.F90:134 */
+  }
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) {
+      size_t j = (size_t)k * ncol + i;
+      double exn = in[RI_EXN][j], rhodref = in[RI_RHODREF][j], pabs = in[RI_PABS][j];
+      double tht = in[RI_THT][j], rvt = in[RI_RVT][j], rct = in[RI_RCT][j], rrt = in[RI_RRT][j];
+      double rit = in[RI_RIT][j], rst = in[RI_RST][j], rgt = in[RI_RGT][j];
+      /* 1. factors and mask (rain_ice.F90:225-273) */
+      double zrice = rit + rst + rgt;
+      double zdevide = P.XCPD + P.XCPV * rvt + P.XCL * (rct + rrt) + P.XCI * zrice;
+      double zt = tht * exn;
+      double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide;
+      double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide;
+      int llmicro = rct > P.XRTMIN[2] || rrt > P.XRTMIN[3] || rit > P.XRTMIN[4] ||
+                    rst > P.XRTMIN[5] || rgt > P.XRTMIN[6];
+      /* 3. saved state (:357-381) */
+      double wr[7] = {tht, rvt, rct, rrt, rit, rst, rgt};
+      double evap3d = 0., rainfr;
+      double cit = io[RO_CI][j];
+      /* 4.1 nucleation outside the mask (:395-431) */
+      double rvheni = 0.;
+      if (!llmicro) cit = 0.;
+      ice4_nucleation(!llmicro, tht, pabs, rhodref, exn, llmicro ? 0. : lsfact / exn, zt, rvt,
+                      &cit, &rvheni);
+      rvheni = dmin(io[RO_RVS][j], rvheni / dt);
+      rainfr = 1.; /* 4.2 (:495) */
+      /* 5. tendencies (ICE4_PACK -> ICE4_STEPPING), external tendencies = sources */
+      int ncalls = 0;
+      if (llmicro) {
+        double extpk[7] = {io[RO_THS][j], io[RO_RVS][j], io[RO_RCS][j], io[RO_RRS][j],
+                           io[RO_RIS][j], io[RO_RSS][j], io[RO_RGS][j]};
+        double rrevav = 0.;
+        if (ice4_stepping_point(dt, exn, rhodref, pabs, in[RI_CLDFR][j], in[RI_SIGS][j], &cit, wr,
+                                extpk, &rainfr, &rrevav, &ncalls) != 0)
+          status = -2;
+        evap3d = rrevav;
+      }
+      if (iters) iters[j] = ncalls;
+      /* 8.1 total tendencies (:649-684) */
+      double exnref = in[RI_EXNREF][j];
+      lsfact = lsfact / exnref;
+      lvfact = lvfact / exnref;
+      double wrv = (wr[IRV] - rvt) * zinv, wrc = (wr[IRC] - rct) * zinv;
+      double wrr = (wr[IRR] - rrt) * zinv, wri = (wr[IRI] - rit) * zinv;
+      double wrs = (wr[IRS] - rst) * zinv, wrg = (wr[IRG] - rgt) * zinv;
+      double wth = (wrc + wrr) * lvfact + (wri + wrs + wrg + 0.) * lsfact;
+      double ths = io[RO_THS][j] + wth + rvheni * lsfact;
+      double rvs = io[RO_RVS][j] + wrv - rvheni;
+      double rcs = io[RO_RCS][j] + wrc;
+      double rrs = io[RO_RRS][j] + wrr;
+      double ris = io[RO_RIS][j] + wri + rvheni;
+      double rss = io[RO_RSS][j] + wrs;
+      double rgs = io[RO_RGS][j] + wrg;
+      /* 8.2 ICE4_CORRECT_NEGATIVITIES (mode_ice4_correct_negativities.F90:51-171) */
+      double zw;
+      zw = rcs - dmax(rcs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rcs = rcs - zw;
+      zw = rrs - dmax(rrs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rrs = rrs - zw;
+      zw = ris - dmax(ris, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; ris = ris - zw;
+      zw = rss - dmax(rss, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rss = rss - zw;
+      zw = rgs - dmax(rgs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rgs = rgs - zw;
+      zw = dmin(1., dmax(P.XRTMIN[1] - rvs, 0.) / dmax(rcs + ris, 1.E-20));
+      ths = ths - zw * (rcs * lvfact + ris * lsfact);
+      rvs = rvs + zw * (rcs + ris);
+      rcs = (1. - zw) * rcs;
+      ris = (1. - zw) * ris;
+      zw = dmin(dmax(rrs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+      rvs = rvs + zw; rrs = rrs - zw; ths = ths - zw * lvfact;
+      zw = dmin(dmax(rss, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+      rvs = rvs + zw; rss = rss - zw; ths = ths - zw * lsfact;
+      zw = dmin(dmax(rgs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+      rvs = rvs + zw; rgs = rgs - zw; ths = ths - zw * lsfact;
+      io[RO_THS][j] = ths; io[RO_RVS][j] = rvs; io[RO_RCS][j] = rcs; io[RO_RRS][j] = rrs;
+      io[RO_RIS][j] = ris; io[RO_RSS][j] = rss; io[RO_RGS][j] = rgs; io[RO_CI][j] = cit;
+      io[RO_EVAP3D][j] = evap3d; io[RO_RAINFR][j] = rainfr;
+    }
+  return status;
+}

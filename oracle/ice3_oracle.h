@@ -1,0 +1,58 @@
+/*
+ * ice3_oracle.h -- C restatement of the PHYEX CY50T1 ice_adjust / rain_ice path.
+ * Test infrastructure only (see ice3_oracle.c).  All arrays are float64,
+ * shape (nlev, ncol) with the column index contiguous; per-column arrays have
+ * length ncol.
+ */
+#ifndef ICE3_ORACLE_H
+#define ICE3_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ice_adjust inputs (JAX IceAdjustJAX argument order minus sigqsat/timestep) */
+enum {
+  IA_PABS, IA_SIGS, IA_TH, IA_EXN, IA_EXNREF, IA_RHODREF, IA_RV, IA_RC, IA_RI, IA_RR, IA_RS,
+  IA_RG, IA_CF_MF, IA_RC_MF, IA_RI_MF, IA_RVS, IA_RCS, IA_RIS, IA_THS, ORACLE_ADJ_NIN
+};
+/* ice_adjust outputs: the 16-tuple of IceAdjustJAX (t .. ths) + srcs + th_out */
+enum {
+  OA_T, OA_RV, OA_RC, OA_RI, OA_CLDFR, OA_HLC_HRC, OA_HLC_HCF, OA_HLI_HRI, OA_HLI_HCF, OA_CPH,
+  OA_LV, OA_LS, OA_SRCS, OA_TH, ORACLE_ADJ_NOUT_POINT,
+  OA_RVS = ORACLE_ADJ_NOUT_POINT, OA_RCS, OA_RIS, OA_THS, ORACLE_ADJ_NOUT
+};
+
+/* rain_ice inputs (per point) */
+enum {
+  RI_EXN, RI_DZZ, RI_RHODJ, RI_RHODREF, RI_EXNREF, RI_PABS, RI_CLDFR, RI_HLC_HRC, RI_HLC_HCF,
+  RI_HLI_HRI, RI_HLI_HCF, RI_THT, RI_RVT, RI_RCT, RI_RRT, RI_RIT, RI_RST, RI_RGT, RI_SIGS,
+  ORACLE_RAIN_NIN
+};
+/* rain_ice in/out per point: sources, ci; out per point: evap3d, rainfr */
+enum {
+  RO_THS, RO_RVS, RO_RCS, RO_RRS, RO_RIS, RO_RSS, RO_RGS, RO_CI, RO_EVAP3D, RO_RAINFR,
+  ORACLE_RAIN_NIO
+};
+/* per-column outputs */
+enum { RC_INPRC, RC_INPRR, RC_INPRI, RC_INPRS, RC_INPRG, ORACLE_RAIN_NCOL };
+
+/* Reads the constants dump of the compiled reference (tests/golden/constants.txt). */
+int oracle_load_constants(const char *path);
+
+/* ICE_ADJUST; returns 0, or -1 when constants are not loaded. */
+int oracle_ice_adjust(int ncol, int nlev, double dt, const double *sigqsat,
+                      const double *const in[ORACLE_ADJ_NIN], double *const out[ORACLE_ADJ_NOUT]);
+
+/* RAIN_ICE with sedimentation, total tendencies and negativity correction;
+ * io[RO_THS..RO_CI] are read and updated in place, io[RO_EVAP3D/RAINFR] written.
+ * iters (optional, ncol*nlev ints) receives each point's tendency-call count.
+ * Returns 0, -1 without constants, -2 if a point exceeded the sub-step cap. */
+int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
+                    double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
+                    int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

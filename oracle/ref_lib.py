@@ -1,0 +1,123 @@
+"""ctypes binding to the compiled FP64 PHYEX reference (oracle/_ref/libphyex_ref.so).
+
+Test infrastructure: used to write golden vectors (tests/golden/make_golden.py),
+to pin the C restatement in oracle/, and as the CPU baseline of bench.py when
+the library has been built.  The library is produced in this container by
+``make -f oracle/ref_build.mk`` from the reference sources plus
+oracle/ref_driver.F90.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_ref", "libphyex_ref.so")
+_lib = None
+_init_state = None
+
+
+def available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise FileNotFoundError(f"{LIB_PATH} not built (make -f oracle/ref_build.mk)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.ref_init.argtypes = [ctypes.c_double, ctypes.c_int]
+        _lib.ref_set_nproma.argtypes = [ctypes.c_int]
+        _lib.ref_dump_constants.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    return _lib
+
+
+def init(dt: float = 50.0, nproma: int = 1):
+    """INI_CST + INI_RAIN_ICE for the pinned AROME ICE3 configuration."""
+    global _init_state
+    lib = _load()
+    if _init_state is None or _init_state[0] != dt:
+        lib.ref_init(dt, nproma)
+        _init_state = (dt, nproma)
+    elif _init_state[1] != nproma:
+        lib.ref_set_nproma(nproma)
+        _init_state = (dt, nproma)
+    return lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def dump_constants(path: str, dt: float = 50.0):
+    lib = init(dt)
+    b = path.encode()
+    lib.ref_dump_constants(b, len(b))
+
+
+def ice_adjust(f: dict, dt: float = 50.0) -> dict:
+    """ICE_ADJUST on a synthetic field dict; returns the 16 JAX-order outputs + srcs."""
+    lib = init(dt)
+    nlev, ncol = f["th"].shape
+    rvs, rcs, ris, ths = (f[k].copy() for k in ("rvs", "rcs", "ris", "ths"))
+    names = ("cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf", "rv_out", "rc_out", "ri_out",
+             "th_out", "srcs", "cph", "lv", "ls")
+    out = {k: np.zeros((nlev, ncol)) for k in names}
+    lib.ref_ice_adjust(
+        ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt), _p(f["sigqsat"]),
+        *[_p(f[k]) for k in ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc", "ri",
+                             "rr", "rs", "rg", "cf_mf", "rc_mf", "ri_mf")],
+        _p(rvs), _p(rcs), _p(ris), _p(ths),
+        *[_p(out[k]) for k in names])
+    out["t"] = out["th_out"] * f["exn"]
+    out.update(rvs=rvs, rcs=rcs, ris=ris, ths=ths)
+    return out
+
+
+def rain_ice(s: dict, dt: float = 50.0, nproma: int = 1) -> dict:
+    """Reinstated RAIN_ICE; ``s`` holds exn, dzz, rhodj, rhodref, exn_ref, pabs, ci, cldfr,
+    hlc_hrc, hlc_hcf, hli_hri, hli_hcf, tht, rvt, rct, rrt, rit, rst, rgt, sigs and the
+    sources ths..rgs.  Returns updated sources, ci, diagnostics."""
+    lib = init(dt, nproma)
+    nlev, ncol = s["tht"].shape
+    io = {k: s[k].copy() for k in ("ci", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
+                                   "ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs")}
+    col = {k: np.zeros(ncol) for k in ("inprc", "inprr", "inpri", "inprs", "inprg")}
+    o2 = {k: np.zeros((nlev, ncol)) for k in ("evap3d", "rainfr")}
+    lib.ref_rain_ice(
+        ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt),
+        *[_p(s[k]) for k in ("exn", "dzz", "rhodj", "rhodref", "exn_ref", "pabs")],
+        _p(io["ci"]), _p(s["cldfr"]),
+        *[_p(io[k]) for k in ("hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf")],
+        *[_p(s[k]) for k in ("tht", "rvt", "rct", "rrt", "rit", "rst", "rgt", "sigs")],
+        *[_p(io[k]) for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs")],
+        *[_p(col[k]) for k in ("inprc", "inprr", "inpri", "inprs", "inprg")],
+        _p(o2["evap3d"]), _p(o2["rainfr"]))
+    io.update(col)
+    io.update(o2)
+    return io
+
+
+def pipeline_state(f: dict, adj: dict, dt: float = 50.0) -> dict:
+    """rain_ice inputs for the fused ice_adjust + rain_ice workload.
+
+    The t-state handed to rain_ice is the post-adjustment S-variables times dt
+    (the synthetic sources carry no external forcing, rxs = rx/dt).  The POUT_*
+    adjusted state is not used: its mass-flux limiter (micro/ice_adjust.F90:449-466)
+    divides 0/0 when CB02 condenses more than the total water and rc_mf = ri_mf = 0,
+    whereas the S-variables are limited (:313-345) and stay finite.
+    """
+    return {
+        "exn": f["exn"], "dzz": f["dzz"], "rhodj": f["rhodj"], "rhodref": f["rho_dry_ref"],
+        "exn_ref": f["exn_ref"], "pabs": f["pabs"], "ci": f["ci"], "cldfr": adj["cldfr"],
+        "hlc_hrc": adj["hlc_hrc"], "hlc_hcf": adj["hlc_hcf"], "hli_hri": adj["hli_hri"],
+        "hli_hcf": adj["hli_hcf"], "tht": adj["ths"] * dt, "rvt": adj["rvs"] * dt,
+        "rct": adj["rcs"] * dt, "rrt": f["rrs"] * dt, "rit": adj["ris"] * dt, "rst": f["rss"] * dt,
+        "rgt": f["rgs"] * dt, "sigs": f["sigs"],
+        "ths": adj["ths"], "rvs": adj["rvs"], "rcs": adj["rcs"], "rrs": f["rrs"], "ris": adj["ris"],
+        "rss": f["rss"], "rgs": f["rgs"],
+    }

@@ -1,0 +1,82 @@
+"""Write the golden vectors of the ice_adjust / rain_ice path from the compiled reference.
+
+Runs in the build container only (it needs oracle/_ref/libphyex_ref.so, built by
+``make -f oracle/ref_build.mk`` from the PHYEX CY50T1 sources in FP64).  Output:
+
+* ``constants.txt``  every constant/table the path reads, dumped after
+  INI_CST + INI_RAIN_ICE (text, ``NAME n v1 .. vn``, Fortran column order);
+* ``ice_adjust_fp64.npz``  inputs and outputs of ICE_ADJUST on synthetic columns
+  (generator variant "adjust"), plus a small case with non-zero mass-flux inputs;
+* ``rain_ice_fp64.npz``  inputs and outputs of the reinstated RAIN_ICE on the
+  fused-workload generator (variant "pipeline"), evaluated per point
+  (NPROMICRO=1).
+
+Usage: python3 tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "dwarf-p-ice3_amd"))
+
+from ice3_amd import synthetic  # noqa: E402
+from oracle import ref_lib  # noqa: E402
+
+DT = 50.0
+ADJ_IN = synthetic.FIELDS_ADJUST
+ADJ_OUT = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
+           "cph", "lv", "ls", "rvs", "rcs", "ris", "ths", "srcs", "th_out")
+RAIN_IN = ("exn", "dzz", "rhodj", "rhodref", "exn_ref", "pabs", "ci", "cldfr", "hlc_hrc",
+           "hlc_hcf", "hli_hri", "hli_hcf", "tht", "rvt", "rct", "rrt", "rit", "rst", "rgt", "sigs",
+           "ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs")
+RAIN_OUT = ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "ci", "evap3d", "rainfr",
+            "inprc", "inprr", "inpri", "inprs", "inprg")
+
+
+def _adjust_case(f, tag, out):
+    a = ref_lib.ice_adjust(f, DT)
+    for k in ADJ_IN:
+        out[f"{tag}_in_{k}"] = f[k]
+    out[f"{tag}_in_sigqsat"] = f["sigqsat"]
+    for k in ADJ_OUT:
+        out[f"{tag}_out_{k}"] = a[k]
+    return a
+
+
+def main():
+    ref_lib.init(DT, 1)
+    ref_lib.dump_constants(os.path.join(HERE, "constants.txt"), DT)
+
+    out = {}
+    f = synthetic.generate(32, 90, seed=20260001, variant="adjust", dt=DT)
+    _adjust_case(f, "base", out)
+    # mass-flux branch (TRIANGLE pdf, ice_adjust.F90:364-466): non-zero cf/rc/ri_mf
+    g = synthetic.generate(8, 90, seed=20260011, variant="adjust", dt=DT)
+    rng = np.random.Generator(np.random.Philox(20260012))
+    g["cf_mf"] = rng.uniform(0.0, 0.3, g["th"].shape) * (rng.uniform(0, 1, g["th"].shape) < 0.7)
+    g["rc_mf"] = rng.lognormal(np.log(2e-5), 1.0, g["th"].shape) * (g["cf_mf"] > 0)
+    g["ri_mf"] = rng.lognormal(np.log(1e-5), 1.0, g["th"].shape) * (g["cf_mf"] > 0)
+    _adjust_case(g, "mf", out)
+    np.savez_compressed(os.path.join(HERE, "ice_adjust_fp64.npz"), dt=DT, **out)
+
+    out = {}
+    f = synthetic.generate(32, 90, seed=20260003, variant="pipeline", dt=DT)
+    a = ref_lib.ice_adjust(f, DT)
+    s = ref_lib.pipeline_state(f, a, DT)
+    r = ref_lib.rain_ice(s, DT, nproma=1)
+    for k in RAIN_IN:
+        out[f"in_{k}"] = s[k]
+    for k in RAIN_OUT:
+        out[f"out_{k}"] = r[k]
+    np.savez_compressed(os.path.join(HERE, "rain_ice_fp64.npz"), dt=DT, **out)
+    print("golden vectors written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
