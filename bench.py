@@ -1,0 +1,184 @@
+"""Benchmark: grid columns/s of the fused ice_adjust + rain_ice step (90 levels, FP64).
+
+Workload (the headline configuration): 512 x 512 = 262,144 synthetic columns x 90
+levels, dt = 50 s, generator variant "pipeline" (precipitating bands in 20 % of the
+columns), seed 20260003.  A step is one pass of ICE_ADJUST -> statistical
+sedimentation -> ICE4 microphysics -> total tendencies -> negativity correction over
+all columns (one launch of the fused column kernel).  Inputs are resident in HBM
+before the timed region; the K steps are bracketed by synchronisations and timed
+with HIP events on the launch stream.
+
+Also reported: the roofline of the fused kernel (algorithmic bytes 320 B/point:
+25 float64 inputs + 15 outputs, plus 48 B/column), the mixed-precision variant
+with its per-field error against FP64, and a CPU baseline (the compiled FP64
+PHYEX reference when its library is present, else the C restatement of the
+oracle) timed on a bounded sample on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncol N] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "dwarf-p-ice3_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+NLEV = 90
+DT = 50.0
+SEED = 20260003
+BYTES_PER_POINT = 320.0   # 25 inputs + 15 outputs, float64
+BYTES_PER_COLUMN = 48.0   # sigqsat + 5 surface precipitation fields
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+
+
+def cpu_baseline(budget_s: float = 20.0, workers: int | None = None) -> dict:
+    """Time the CPU path on a bounded sample of the same workload, before the GPU is touched.
+
+    Kind "reference": the FP64 flang build of the PHYEX sources (oracle/_ref), run by
+    independent worker processes (its Fortran module state is not thread-safe), each on
+    its own slab of columns from the same generator, with NPROMICRO=128 cache blocking.
+    Kind "port": the C restatement of the oracle.  Each worker generates its slab, then
+    times only the physics; the rate is the total column count over the slowest worker.
+    """
+    import multiprocessing as mp
+
+    from oracle import ref_lib
+
+    kind = "reference" if ref_lib.available() else "port"
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    workers = workers or max(1, min(16, ncpu))
+    per_col = _cpu_slab((kind, 256, SEED))[1] / 256          # calibration on one core
+    ncol_w = int(max(64, min(4096, budget_s / per_col / 1.5)))
+    ctx = mp.get_context("fork")
+    with ctx.Pool(workers) as pool:
+        res = pool.map_async(_cpu_slab, [(kind, ncol_w, SEED + 1 + w) for w in range(workers)]).get(
+            40 * budget_s + 300)
+    wall = max(t for _, t in res)
+    ncol = ncol_w * workers
+    return {"value": ncol / wall, "unit": "columns/s", "cores": workers, "kind": kind,
+            "sample": f"{workers} processes x {ncol_w} columns x {NLEV} levels of the same generator "
+                      f"(seeds {SEED + 1}..{SEED + workers}), fused ice_adjust+rain_ice FP64, "
+                      f"slowest worker {wall:.1f} s"}
+
+
+def _cpu_slab(arg):
+    kind, ncol, seed = arg
+    from ice3_amd import synthetic
+    from oracle import oracle_lib, ref_lib
+    f = synthetic.generate(ncol, NLEV, seed=seed, variant="pipeline", dt=DT)
+    if kind == "reference":
+        ref_lib.init(DT, 128)
+        t0 = time.perf_counter()
+        a = ref_lib.ice_adjust(f, DT)
+        ref_lib.rain_ice(ref_lib.pipeline_state(f, a, DT), DT, nproma=128)
+    else:
+        t0 = time.perf_counter()
+        a = oracle_lib.ice_adjust(f, DT)
+        oracle_lib.rain_ice(ref_lib.pipeline_state(f, a, DT), DT)
+    return ncol, time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ncol", type=int, default=512 * 512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(columns are independent: a multi-GPU run is ncol/N columns per GPU, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    import numpy as np
+    import torch
+
+    from ice3_amd import _lib, synthetic
+    from ice3_amd._fields import ptr
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    ncol = args.ncol
+    f = synthetic.generate(ncol, NLEV, seed=SEED, variant="pipeline", dt=DT)
+    f["rhodref"] = f.pop("rho_dry_ref")
+    g = {k: torch.from_numpy(v).to(dev) for k, v in f.items()}
+    del f
+    lib = _lib.load()
+    ctx = _lib.context(0)
+
+    def outputs():
+        o = {k: torch.empty((NLEV, ncol), dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[:15]}
+        o.update({k: torch.empty(ncol, dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[15:]})
+        return o
+
+    cin = _lib.PipeIn(*[ptr(g[k]) for k in _lib.PIPE_IN_FIELDS])
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    def run(prec, out, steps, warmup):
+        cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
+        for _ in range(warmup):
+            _lib.check(lib.ice3_adjust_rain_ice(ctx, prec, ncol, NLEV, ncol, DT, ctypes.byref(cin),
+                                                ctypes.byref(cout), None, sp), "warmup")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            lib.ice3_adjust_rain_ice(ctx, prec, ncol, NLEV, ncol, DT, ctypes.byref(cin), ctypes.byref(cout),
+                                     None, sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps  # ms per step
+
+    out64 = outputs()
+    ms = run(_lib.PREC_FP64, out64, args.steps, args.warmup)
+    npts = ncol * NLEV
+    alg_bytes = BYTES_PER_POINT * npts + BYTES_PER_COLUMN * ncol
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    res = {
+        "metric": "grid columns/s (90 lev) ice_adjust+rain_ice, FP64",
+        "value": ncol / (ms * 1e-3),
+        "unit": "columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "fused ice_adjust+rain_ice, statistical sedimentation, FP64",
+                   "columns": ncol, "levels": NLEV, "dt_s": DT, "generator": "pipeline", "seed": SEED},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if not args.no_mixed:
+        out32 = outputs()
+        ms32 = run(_lib.PREC_MIXED, out32, args.steps, args.warmup)
+        err = {}
+        for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs"):
+            d = (out32[k] - out64[k]).abs()
+            scale = out64[k].abs().max().item()
+            err[k] = {"max_abs": d.max().item(), "rms_abs": d.pow(2).mean().sqrt().item(),
+                      "max_rel_to_field_max": d.max().item() / scale if scale > 0 else 0.0}
+        res["variants"] = {"mixed": {"columns_per_s": ncol / (ms32 * 1e-3), "ms_per_step": ms32,
+                                     "speedup_vs_fp64": ms / ms32, "error_vs_fp64": err}}
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

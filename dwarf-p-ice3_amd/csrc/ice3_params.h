@@ -1,0 +1,46 @@
+// ice3_params.h -- constants and lookup tables of the AROME ICE3 configuration
+// (KRR=6), as INI_CST (aux/mode_ini_cst.F90:77-193) and INI_RAIN_ICE
+// (micro/mode_ini_rain_ice.F90:10-1571) define them.  Plain data shared by the
+// host builder (ice3_params.cpp) and the device kernels.
+#pragma once
+
+struct ice3_params {
+  // CST_t
+  double XTT, XLVTT, XLSTT, XLMTT, XCPD, XCPV, XCL, XCI, XRD, XRV, XEPSILO;
+  double XALPW, XBETAW, XGAMW, XALPI, XBETAI, XGAMI, XESTT, XP00, XRHOLW, XPI, XG, XMNH_TINY;
+  // NEB_t (pinned: 'T' ice fraction between these temperatures)
+  double XTMINMIX, XTMAXMIX;
+  // RAIN_ICE_DESCR_t
+  double XRTMIN[7];  // 1-based, [1..6]
+  double XCEXVT, XLBR, XLBEXR, XLBS, XLBEXS, XLBG, XLBEXG, XLBI, XLBEXI, XLBDAS_MAX, XDI, XBS,
+      XCXS, XCXG, XDG, XCONC_LAND;
+  // RAIN_ICE_PARAM_t
+  double XCRIAUTC, XCRIAUTI, XACRIAUTI, XBCRIAUTI, XT0CRIAUTI, XNU10, XALPHA1, XBETA1, XNU20,
+      XALPHA2, XBETA2, XMNU0, XALPHA3, XBETA3, XHON, XSCFAC, X0DEPI, X2DEPI, X0DEPS, X1DEPS,
+      XEX0DEPS, XEX1DEPS, X0DEPG, X1DEPG, XEX0DEPG, XEX1DEPG, XTIMAUTI, XTEXAUTI, XCOLEXIS,
+      XFIAGGS, XEXIAGGS, XTIMAUTC, XFCACCR, XEXCACCR, X0EVAR, X1EVAR, XEX0EVAR, XEX1EVAR,
+      XEXCRIMSS, XCRIMSS, XEXCRIMSG, XCRIMSG, XSRIMCG, XEXSRIMCG, XSRIMCG2, XSRIMCG3, XEXSRIMCG2,
+      XRIMINTP1, XRIMINTP2, XFRACCSS, XLBRACCS1, XLBRACCS2, XLBRACCS3, XFSACCRG, XLBSACCR1,
+      XLBSACCR2, XLBSACCR3, XACCINTP1S, XACCINTP2S, XACCINTP1R, XACCINTP2R, XFSCVMG, XEXRCFRI,
+      XRCFRI, XEXICFRR, XICFRR, XFCDRYG, XCOLIG, XCOLEXIG, XFIDRYG, XCOLSG, XCOLEXSG, XFSDRYG,
+      XLBSDRYG1, XLBSDRYG2, XLBSDRYG3, XFRDRYG, XLBRDRYG1, XLBRDRYG2, XLBRDRYG3, XDRYINTP1R,
+      XDRYINTP2R, XDRYINTP1S, XDRYINTP2S, XDRYINTP1G, XDRYINTP2G, XFSEDR, XEXSEDR, XFSEDS,
+      XEXSEDS, XFSEDG, XEXSEDG, XFSEDI, XEXCSEDI;
+  int NGAMINC, NACCLBDAS, NACCLBDAR, NDRYLBDAR, NDRYLBDAS, NDRYLBDAG;
+};
+
+// Table block: one contiguous float64 array, offsets in doubles.
+enum ice3_table_offsets {
+  ICE3_T_RIM1 = 0,                      // GAMINC_RIM1(80)
+  ICE3_T_RIM2 = 80,                     // GAMINC_RIM2(80)
+  ICE3_T_RIM4 = 160,                    // GAMINC_RIM4(80)
+  ICE3_T_RACCSS = 240,                  // (40,40) lbda_s x lbda_r
+  ICE3_T_RACCS = ICE3_T_RACCSS + 1600,  // (40,40)
+  ICE3_T_SACCRG = ICE3_T_RACCS + 1600,  // (40,40) lbda_r x lbda_s
+  ICE3_T_SDRYG = ICE3_T_SACCRG + 1600,  // (40,80) lbda_g x lbda_s
+  ICE3_T_RDRYG = ICE3_T_SDRYG + 3200,   // (40,40) lbda_g x lbda_r
+  ICE3_T_TOTAL = ICE3_T_RDRYG + 1600    // 9840 doubles
+};
+
+// Builds every constant and the table block (tables must hold ICE3_T_TOTAL doubles).
+void ice3_build_params(ice3_params *p, double *tables);

@@ -1,0 +1,777 @@
+// ice3_physics.hpp -- device-side ICE3 point and column physics for CDNA4.
+//
+// Each function cites the PHYEX CY50T1 routine it implements (paths relative to
+// PHYEX-IAL_CY50T1/).  Branch set: AROME ICE3, KRR=6, sigma_s subgrid
+// condensation (LSUBG_COND, OSIGMAS, CB02, lambda3='CB', ice fraction 'T'),
+// TRIANGLE mass-flux pdf, statistical sedimentation without cloud droplets,
+// rc autoconversion pdf 'PDF'+'SIGM', ri 'NONE', accretion/evaporation 'NONE',
+// LFEEDBACKT, LEVLIMIT, LCRFLIMIT, LNULLWETG, LWETGPOST, snow riming 'M90',
+// XMRSTEP = XTSTEP_TS = 0 (per-point independent time splitting).
+//
+// Precision: template parameter R is the arithmetic of the process rates and
+// table lookups (double: plain FP64; float: the mixed path).  The prognostic
+// state, the time-splitting update, the saturation adjustment and the
+// sedimentation fluxes are always double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ice3_params.h"
+
+namespace ice3 {
+
+// Device copy of the constants (one per device), filled by ice3_ctx_create.
+// This header is compiled in exactly one translation unit (ice3_kernels.hip).
+__constant__ ice3_params c_P;
+
+enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6 };
+
+template <class R> __device__ inline R rexp(R x);
+template <class R> __device__ inline R rlog(R x);
+template <class R> __device__ inline R rpow(R x, R y);
+template <> __device__ inline double rexp<double>(double x) { return exp(x); }
+template <> __device__ inline double rlog<double>(double x) { return log(x); }
+template <> __device__ inline double rpow<double>(double x, double y) { return pow(x, y); }
+template <> __device__ inline float rexp<float>(float x) { return expf(x); }
+template <> __device__ inline float rlog<float>(float x) { return logf(x); }
+template <> __device__ inline float rpow<float>(float x, float y) { return powf(x, y); }
+
+__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+__device__ inline float dmin(float a, float b) { return a < b ? a : b; }
+__device__ inline float dmax(float a, float b) { return a > b ? a : b; }
+template <class T> __device__ inline T sq(T a) { return a * a; }
+
+// ---------------------------------------------------------------- ice_adjust
+
+struct AdjustOut {
+  double t, rv, rc, ri, cldfr, hlc_hrc, hlc_hcf, hli_hri, hli_hcf, cph, lv, ls, srcs;
+};
+
+// ZSRC_1D(-22:11), micro/condensation.F90:208-216
+__device__ inline double zsrc_1d(int i) {
+  switch (i) {
+    case -22: return 0.;            case -21: return 0.;
+    case -20: return 2.0094444E-04; case -19: return 0.316670E-03;
+    case -18: return 4.9965648E-04; case -17: return 0.785956E-03;
+    case -16: return 1.2341294E-03; case -15: return 0.193327E-02;
+    case -14: return 3.0190963E-03; case -13: return 0.470144E-02;
+    case -12: return 7.2950651E-03; case -11: return 0.112759E-01;
+    case -10: return 1.7350994E-02; case -9: return 0.265640E-01;
+    case -8: return 4.0427860E-02;  case -7: return 0.610997E-01;
+    case -6: return 9.1578111E-02;  case -5: return 0.135888E+00;
+    case -4: return 0.1991484;      case -3: return 0.230756E+00;
+    case -2: return 0.2850565;      case -1: return 0.375050E+00;
+    case 0: return 0.5000000;       case 1: return 0.691489E+00;
+    case 2: return 0.8413813;       case 3: return 0.933222E+00;
+    case 4: return 0.9772662;       case 5: return 0.993797E+00;
+    case 6: return 0.9986521;       case 7: return 0.999768E+00;
+    case 8: return 0.9999684;       case 9: return 0.999997E+00;
+    case 10: return 1.0000000;      default: return 1.000000;
+  }
+}
+
+// ICE_ADJUST (micro/ice_adjust.F90:257-472) + ITERATION (:488-565) + CONDENSATION
+// (micro/condensation.F90:270-650) for one point; rvs..ths updated in place.
+__device__ inline void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs,
+                                        double th, double exn, double exnref, double rhodref,
+                                        double rv, double rc, double ri, double rr, double rs,
+                                        double rg, double cf_mf, double rc_mf, double ri_mf,
+                                        double &rvs, double &rcs, double &ris, double &ths,
+                                        AdjustOut &o) {
+  const ice3_params &P = c_P;
+  double zt = th * exn;
+  const double lv = P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT);
+  const double ls = P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT);
+  const double cph = P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg);
+  const double rt = rv + rc + ri * 1.0;
+  const double logt = log(zt);
+  const double pv = dmin(exp(P.XALPW - P.XBETAW / zt - P.XGAMW * logt), .99 * pabs);
+  const double piv = dmin(exp(P.XALPI - P.XBETAI / zt - P.XGAMI * logt), .99 * pabs);
+  const double frac = dmax(0., dmin(1., ((P.XTMAXMIX - zt) / (P.XTMAXMIX - P.XTMINMIX))));
+  double qsl = P.XRD / P.XRV * pv / (pabs - pv);
+  const double qsi = P.XRD / P.XRV * piv / (pabs - piv);
+  qsl = (1. - frac) * qsl + frac * qsi;
+  const double lvs = (1. - frac) * lv + frac * ls;
+  const double ah = lvs * qsl / (P.XRV * sq(zt)) * (P.XRV * qsl / P.XRD + 1.);
+  const double a = 1. / (1. + lvs / cph * ah);
+  const double sbar = a * (rt - qsl + ah * lvs * (rc + ri * 1.0) / cph);
+  double sigma = sigqsat != 0. ? sqrt(sq(2 * sigs) + sq(sigqsat * qsl * a)) : 2 * sigs;
+  sigma = dmax(1.E-10, sigma);
+  const double q1 = sbar / sigma;
+  double cond;
+  if (q1 > 0. && q1 <= 2)
+    cond = dmin(exp(-1.) + .66 * q1 + .086 * sq(q1), 2.);
+  else if (q1 > 2.)
+    cond = q1;
+  else
+    cond = exp(1.2 * q1 - 1.);
+  cond = cond * sigma;
+  double cldfr = cond < 1.E-12 ? 0. : dmax(0., dmin(1., 0.5 + 0.36 * atan(1.55 * q1)));
+  if (cldfr == 0.) cond = 0.;
+  int inq1 = (int)floor(dmin(100., dmax(-100., 2 * q1)));
+  inq1 = inq1 < -22 ? -22 : (inq1 > 10 ? 10 : inq1);
+  const double inc = 2. * q1 - (double)inq1;
+  double sigrc = dmin(1., (1. - inc) * zsrc_1d(inq1) + inc * zsrc_1d(inq1 + 1));
+  double rc_out = (1. - frac) * cond;
+  double ri_out = frac * cond;
+  zt = zt + ((rc_out - rc) * lv + (ri_out - ri) * ls) / cph;
+  double rv_out = rt - rc_out - ri_out * 1.0;
+  sigrc = sigrc * dmin(3., dmax(1., 1. - q1));
+  // 5.0-5.1 sources
+  double w1 = (rc_out - rc) / dt;
+  double w2 = (ri_out - ri) / dt;
+  w1 = w1 < 0.0 ? dmax(w1, -rcs) : dmin(w1, rvs);
+  rvs = rvs - w1;
+  rcs = rcs + w1;
+  ths = ths + w1 * lv / (cph * exnref);
+  w2 = w2 < 0.0 ? dmax(w2, -ris) : dmin(w2, rvs);
+  rvs = rvs - w2;
+  ris = ris + w2;
+  ths = ths + w2 * ls / (cph * exnref);
+  // 5.2 subgrid: mass-flux contribution, TRIANGLE pdf
+  w1 = rc_mf / dt;
+  w2 = ri_mf / dt;
+  if (w1 + w2 > rvs) {
+    w1 = w1 * rvs / (w1 + w2);
+    w2 = rvs - w1;
+  }
+  cldfr = dmin(1., cldfr + cf_mf);
+  rcs = rcs + w1;
+  ris = ris + w2;
+  rvs = rvs - (w1 + w2);
+  ths = ths + (w1 * lv + w2 * ls) / cph / exnref;
+  double hlc_hcf = 0., hlc_hrc = 0., hli_hcf = 0., hli_hri = 0.;
+  {
+    const double criaut = P.XCRIAUTC / rhodref, x = w1 * dt, c = criaut * cf_mf;
+    double hcf, hr;
+    if (x > cf_mf * criaut) {
+      hcf = 1. - .5 * sq(criaut * cf_mf / dmax(1.E-20, x));
+      hr = x - (c * c * c) / (3 * sq(dmax(1.E-20, x)));
+    } else if (2. * w1 * dt <= cf_mf * criaut) {
+      hcf = 0.;
+      hr = 0.;
+    } else {
+      hcf = sq(2. * w1 * dt - criaut * cf_mf) / (2. * sq(dmax(1.E-20, x)));
+      hr = (4. * (x * x * x) - 3. * w1 * dt * sq(c) + (c * c * c)) / (3 * sq(dmax(1.E-20, x)));
+    }
+    hlc_hcf = dmin(1., hlc_hcf + hcf * cf_mf);
+    hlc_hrc = hlc_hrc + hr;
+  }
+  // the 10**(aT+b) threshold only matters with mass-flux ice or cloud fraction
+  if (w2 != 0. || cf_mf != 0.) {
+    const double criaut = dmin(P.XCRIAUTI, pow(10., P.XACRIAUTI * (zt - P.XTT) + P.XBCRIAUTI));
+    const double x = w2 * dt, c = criaut * cf_mf;
+    double hcf, hr;
+    if (x > cf_mf * criaut) {
+      hcf = 1. - .5 * sq(criaut * cf_mf / x);
+      hr = x - (c * c * c) / (3 * sq(x));
+    } else if (2. * w2 * dt <= cf_mf * criaut) {
+      hcf = 0.;
+      hr = 0.;
+    } else {
+      hcf = sq(2. * w2 * dt - criaut * cf_mf) / (2. * sq(x));
+      hr = (4. * (x * x * x) - 3. * w2 * dt * sq(c) + (c * c * c)) / (3 * sq(x));
+    }
+    hli_hcf = dmin(1., hli_hcf + hcf * cf_mf);
+    hli_hri = hli_hri + hr;
+  }
+  {  // adjusted state with the mass-flux condensate (POUT_*, :449-466)
+    double v1 = rc_mf, v2 = ri_mf;
+    if (v1 + v2 > rv_out) {
+      v1 = v1 * rv_out / (v1 + v2);
+      v2 = rv_out - v1;
+    }
+    rc_out = rc_out + v1;
+    ri_out = ri_out + v2;
+    rv_out = rv_out - (v1 + v2);
+    zt = zt + (v1 * lv + v2 * ls) / cph;
+  }
+  o.t = zt; o.rv = rv_out; o.rc = rc_out; o.ri = ri_out; o.cldfr = cldfr;
+  o.hlc_hrc = hlc_hrc; o.hlc_hcf = hlc_hcf; o.hli_hri = hli_hri; o.hli_hcf = hli_hcf;
+  o.cph = cph; o.lv = lv; o.ls = ls; o.srcs = sigrc;
+}
+
+// ---------------------------------------------------------------- rain_ice
+
+// ICE4_NUCLEATION (micro/ice4_nucleation.func.h:68-143), .NOT.LOCND2, LFEEDBACKT
+template <class R>
+__device__ inline void nucleation(bool ldcompute, double tht, double pabst, double rhodref,
+                                  double exn, double lsfact, double t, double rvt, double &cit,
+                                  double &rvheni_mr) {
+  const ice3_params &P = c_P;
+  const bool gnegt = ldcompute && t < P.XTT && rvt > P.XRTMIN[1];
+  rvheni_mr = 0.;
+  if (!gnegt) return;
+  const R rt = (R)t;
+  const R lt = rlog<R>(rt);
+  R usw = rexp<R>((R)P.XALPW - (R)P.XBETAW / rt - (R)P.XGAMW * lt);
+  R zzw = rexp<R>((R)P.XALPI - (R)P.XBETAI / rt - (R)P.XGAMI * lt);
+  const R pab = (R)pabst;
+  zzw = dmin(pab / (R)2., zzw);
+  R ssi = (R)rvt * (pab - zzw) / ((R)P.XEPSILO * zzw) - (R)1.0;
+  usw = dmin(pab / (R)2., usw);
+  usw = (usw / zzw) * ((pab - zzw) / (pab - usw)) - (R)1.0;
+  ssi = dmin(ssi, usw);
+  R nn = 0.;
+  if (t < P.XTT - 5.0 && ssi > (R)0.0)
+    nn = (R)P.XNU20 * rexp<R>((R)P.XALPHA2 * ssi - (R)P.XBETA2);
+  else if (t <= P.XTT - 2.0 && t >= P.XTT - 5.0 && ssi > (R)0.0)
+    nn = dmax((R)P.XNU20 * rexp<R>(-(R)P.XBETA2),
+              (R)P.XNU10 * rexp<R>(-(R)P.XBETA1 * (rt - (R)P.XTT)) * rpow<R>(ssi / usw, (R)P.XALPHA1));
+  double zz = (double)nn - cit;
+  zz = dmin(zz, 50.E3);
+  double rv_mr = dmax(zz, 0.0) * P.XMNU0 / rhodref;
+  rv_mr = dmin(rvt, rv_mr);
+  const double zw = dmin(rv_mr, dmax(0., (P.XTT / exn - tht) / lsfact)) / dmax(rv_mr, 1.E-20);
+  rvheni_mr = rv_mr * zw;
+  zz = zz * zw;
+  cit = dmax(zz + cit, cit);
+}
+
+// Per-point state persisting across the tendency calls of one time step
+// (ZBU_INST, ZRS_TEND, ZRG_TEND and ZSSI of micro/mode_ice4_stepping.F90:132-158):
+// only the rates that soft calls reuse are kept.
+template <class R> struct StepState {
+  R rchoni, rvdeps, riaggs, riauts, rvdepg, rcautr, rcaccr, rrevav, rcberi;
+  R ricfrrg, rrcfrig, ricfrr, rsmltg, rcmltsr, rgmltr;
+  R rs_rcrims, rs_rcrimss, rs_rsrimcg, rs_rraccs, rs_rraccss, rs_rsaccrg, rs_freez1, rs_freez2;
+  R rg_rcdryg, rg_ridryg, rg_riwetg, rg_rsdryg, rg_rswetg, rg_rrdryg, rg_freez1, rg_freez2;
+  R ssi;
+};
+
+// INTERP_MICRO_1D (micro/interp_micro.func.h:34-88) for one point
+template <class R>
+__device__ inline R interp1d(R x, int knum, R p1, R p2, const double *__restrict__ lt) {
+  R idx = dmax((R)1.00001, dmin((R)knum - (R)0.00001, p1 * rlog<R>(x) + p2));
+  int i = (int)idx;
+  R f = idx - (R)i;
+  return (R)lt[i] * f - (R)lt[i - 1] * (f - (R)1.0);
+}
+
+// INTERP_MICRO_2D (micro/interp_micro.func.h:159-223) for one point: indices + weights
+template <class R> struct Idx2 { int i1, i2; R f1, f2; };
+template <class R>
+__device__ inline Idx2<R> interp2d_idx(R x1, R x2, int n1, int n2, R p11, R p12, R p21, R p22) {
+  Idx2<R> r;
+  R a = dmax((R)1.00001, dmin((R)n1 - (R)0.00001, p11 * rlog<R>(x1) + p12));
+  r.i1 = (int)a;
+  r.f1 = a - (R)r.i1;
+  R b = dmax((R)1.00001, dmin((R)n2 - (R)0.00001, p21 * rlog<R>(x2) + p22));
+  r.i2 = (int)b;
+  r.f2 = b - (R)r.i2;
+  return r;
+}
+// table lt is (n1, n2) column-major, addressed lt(i1, i2) (1-based)
+template <class R>
+__device__ inline R interp2d(const Idx2<R> &x, int n1, const double *__restrict__ lt) {
+#define T(i, j) ((R)lt[((i)-1) + ((j)-1) * n1])
+  return (T(x.i1 + 1, x.i2 + 1) * x.f2 - T(x.i1 + 1, x.i2) * (x.f2 - (R)1.0)) * x.f1 -
+         (T(x.i1, x.i2 + 1) * x.f2 - T(x.i1, x.i2) * (x.f2 - (R)1.0)) * (x.f1 - (R)1.0);
+#undef T
+}
+// transposed addressing lt(i2, i1) of an (n2, n1) table (:211-216)
+template <class R>
+__device__ inline R interp2d_t(const Idx2<R> &x, int n2, const double *__restrict__ lt) {
+#define T(i, j) ((R)lt[((i)-1) + ((j)-1) * n2])
+  return (T(x.i2 + 1, x.i1 + 1) * x.f1 - T(x.i2 + 1, x.i1) * (x.f1 - (R)1.0)) * x.f2 -
+         (T(x.i2, x.i1 + 1) * x.f1 - T(x.i2, x.i1) * (x.f1 - (R)1.0)) * (x.f2 - (R)1.0);
+#undef T
+}
+
+// ICE4_TENDENCIES (micro/mode_ice4_tendencies.F90:155-717) for one point that is
+// being integrated (LDCOMPUTE true), with ICE4_COMPUTE_PDF, ICE4_SLOW, ICE4_WARM,
+// ICE4_FAST_RS, ICE4_FAST_RG and ICE4_FAST_RI inlined in call order.
+// Outputs: pa (rates) and pb (instantaneous changes), both double.
+// Soft calls (reuse of the hard call's rates) skip everything that only feeds
+// rate evaluations guarded by .NOT.LDSOFT, such as the slope parameters.
+template <class R>
+__device__ inline void tendencies(bool soft, double exn, double rhodref, double lvfact,
+                                  double lsfact, double pres, double cf, double sigma_rc,
+                                  double &cit, double pt, const double *pvart,
+                                  StepState<R> &S, const double *__restrict__ tab, double *pa,
+                                  double *pb) {
+  const ice3_params &P = c_P;
+  double zt = pt;
+  double zv[7];
+#pragma unroll
+  for (int jv = 0; jv < 7; ++jv) { zv[jv] = pvart[jv]; pa[jv] = 0.; pb[jv] = 0.; }
+  R ka = 0, dv = 0, ai = 0, cj = 0;
+  if (!soft) {
+    double rvheni, rrhong, rimltc;
+    nucleation<R>(true, zv[ITH], pres, rhodref, exn, lsfact, zt, zv[IRV], cit, rvheni);
+    zv[ITH] = zv[ITH] + rvheni * lsfact;
+    zt = zv[ITH] * exn;
+    zv[IRV] = zv[IRV] - rvheni;
+    zv[IRI] = zv[IRI] + rvheni;
+    // RRHONG (micro/mode_ice4_rrhong.F90:65-75)
+    if (zt < P.XTT - 35.0 && zv[IRR] > P.XRTMIN[3])
+      rrhong = dmin(zv[IRR], dmax(0., ((P.XTT - 35.) / exn - zv[ITH]) / (lsfact - lvfact)));
+    else
+      rrhong = 0.;
+    zv[ITH] = zv[ITH] + rrhong * (lsfact - lvfact);
+    zt = zv[ITH] * exn;
+    zv[IRR] = zv[IRR] - rrhong;
+    zv[IRG] = zv[IRG] + rrhong;
+    // RIMLTC (micro/mode_ice4_rimltc.F90:64-74)
+    if (zv[IRI] > 0. && zt > P.XTT)
+      rimltc = dmin(zv[IRI], dmax(0., (zv[ITH] - P.XTT / exn) / (lsfact - lvfact)));
+    else
+      rimltc = 0.;
+    zv[ITH] = zv[ITH] - rimltc * (lsfact - lvfact);
+    zt = zv[ITH] * exn;
+    zv[IRC] = zv[IRC] + rimltc;
+    zv[IRI] = zv[IRI] - rimltc;
+    pb[ITH] = pb[ITH] + rvheni * lsfact;
+    pb[ITH] = pb[ITH] + rrhong * (lsfact - lvfact);
+    pb[ITH] = pb[ITH] - rimltc * (lsfact - lvfact);
+    pb[IRV] = pb[IRV] - rvheni;
+    pb[IRC] = pb[IRC] + rimltc;
+    pb[IRR] = pb[IRR] - rrhong;
+    pb[IRI] = pb[IRI] + rvheni;
+    pb[IRI] = pb[IRI] - rimltc;
+    pb[IRG] = pb[IRG] + rrhong;
+    // derived fields (:312-320)
+    const R t = (R)zt;
+    const R zzw = rexp<R>((R)P.XALPI - (R)P.XBETAI / t - (R)P.XGAMI * rlog<R>(t));
+    S.ssi = (R)zv[IRV] * ((R)pres - zzw) / ((R)P.XEPSILO * zzw) - (R)1.0;
+    ka = (R)2.38E-2 + (R)0.0071E-2 * (t - (R)P.XTT);
+    dv = (R)0.211E-4 * rpow<R>(t / (R)P.XTT, (R)1.94) * ((R)P.XP00 / (R)pres);
+    ai = sq((R)P.XLSTT + (R)(P.XCPV - P.XCI) * (t - (R)P.XTT)) / (ka * (R)P.XRV * sq(t)) +
+         ((R)P.XRV * t) / (dv * zzw);
+    cj = (R)P.XSCFAC * rpow<R>((R)rhodref, (R)0.3) / sqrt((R)1.718E-5 + (R)0.0049E-5 * (t - (R)P.XTT));
+  }
+  const R t = (R)zt, rho = (R)rhodref;
+  const R rv = (R)zv[IRV], rc = (R)zv[IRC], rr = (R)zv[IRR], ri = (R)zv[IRI], rs = (R)zv[IRS],
+          rg = (R)zv[IRG];
+  const bool cold = zt < P.XTT, warm = zt > P.XTT;
+
+  // ICE4_COMPUTE_PDF (micro/mode_ice4_compute_pdf.F90:88-390): rc 'PDF'+'SIGM', ri 'NONE'
+  R hlc_hcf, hlc_hrc, hli_hcf, hli_hri;
+  {
+    const double rcrautc = P.XCRIAUTC / rhodref, zrc = zv[IRC];
+    if (zrc > rcrautc + sigma_rc) {
+      hlc_hcf = 1.; hlc_hrc = (R)zrc;
+    } else if (zrc > (rcrautc - sigma_rc) && zrc <= (rcrautc + sigma_rc)) {
+      hlc_hcf = (R)((zrc + sigma_rc - rcrautc) / (2. * sigma_rc));
+      hlc_hrc = (R)((zrc + sigma_rc - rcrautc) * (zrc + sigma_rc + rcrautc) / (4. * sigma_rc));
+    } else {
+      hlc_hcf = 0.; hlc_hrc = 0.;
+    }
+    const R criauti = dmin((R)P.XCRIAUTI, rpow<R>((R)10., (R)P.XACRIAUTI * (t - (R)P.XTT) + (R)P.XBCRIAUTI));
+    if (ri > criauti) {
+      hli_hcf = 1.; hli_hri = ri;
+    } else {
+      hli_hcf = 0.; hli_hri = 0.;
+    }
+  }
+
+  // slope parameters (:389-433), hard calls only (soft calls reuse rates)
+  R lbdar = 0, lbdas = 0, lbdag = 0;
+  if (!soft) {
+    if (rr > (R)0.) lbdar = (R)P.XLBR * rpow<R>(rho * dmax(rr, (R)P.XRTMIN[3]), (R)P.XLBEXR);
+    if (rs > (R)0.) lbdas = dmin((R)P.XLBDAS_MAX, (R)P.XLBS * rpow<R>(rho * dmax(rs, (R)P.XRTMIN[5]), (R)P.XLBEXS));
+    if (rg > (R)0.) lbdag = (R)P.XLBG * rpow<R>(rho * dmax(rg, (R)P.XRTMIN[6]), (R)P.XLBEXG);
+  }
+  const R rhomcexvt = soft ? (R)0 : rpow<R>(rho, -(R)P.XCEXVT);
+  const R ssi = S.ssi;
+  const bool rv_ok = zv[IRV] > P.XRTMIN[1], rc_ok = zv[IRC] > P.XRTMIN[2],
+             rr_ok = zv[IRR] > P.XRTMIN[3], ri_ok = zv[IRI] > P.XRTMIN[4],
+             rs_ok = zv[IRS] > P.XRTMIN[5], rg_ok = zv[IRG] > P.XRTMIN[6];
+
+  // ICE4_SLOW (micro/mode_ice4_slow.F90:106-217)
+  if (zt < P.XTT - 35.0 && rc_ok) {
+    if (!soft) S.rchoni = dmin((R)1000., (R)P.XHON * rho * rc * rexp<R>((R)P.XALPHA3 * (t - (R)P.XTT) - (R)P.XBETA3));
+  } else S.rchoni = 0;
+  if (rv_ok && rs_ok) {
+    if (!soft) S.rvdeps = (ssi / (rho * ai)) * ((R)P.X0DEPS * rpow<R>(lbdas, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * rpow<R>(lbdas, (R)P.XEX1DEPS));
+  } else S.rvdeps = 0;
+  if (ri_ok && rs_ok) {
+    if (!soft) S.riaggs = (R)P.XFIAGGS * rexp<R>((R)P.XCOLEXIS * (t - (R)P.XTT)) * ri * rpow<R>(lbdas, (R)P.XEXIAGGS) * rhomcexvt;
+  } else S.riaggs = 0;
+  if (hli_hri > (R)P.XRTMIN[4]) {
+    if (!soft) {
+      const R criauti = dmin((R)P.XCRIAUTI, rpow<R>((R)10., (R)P.XACRIAUTI * (t - (R)P.XTT) + (R)P.XBCRIAUTI));
+      S.riauts = (R)P.XTIMAUTI * rexp<R>((R)P.XTEXAUTI * (t - (R)P.XTT)) * dmax(hli_hri - criauti * hli_hcf, (R)0.);
+    }
+  } else S.riauts = 0;
+  if (rv_ok && rg_ok) {
+    if (!soft) S.rvdepg = (ssi / (rho * ai)) * ((R)P.X0DEPG * rpow<R>(lbdag, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * rpow<R>(lbdag, (R)P.XEX1DEPG));
+  } else S.rvdepg = 0;
+
+  // ICE4_WARM (micro/mode_ice4_warm.F90:100-197)
+  if (hlc_hrc > (R)P.XRTMIN[2] && hlc_hcf > (R)0.) {
+    if (!soft) S.rcautr = (R)P.XTIMAUTC * dmax(hlc_hrc - hlc_hcf * (R)P.XCRIAUTC / rho, (R)0.0);
+  } else S.rcautr = 0;
+  if (rc_ok && rr_ok) {
+    if (!soft) S.rcaccr = (R)P.XFCACCR * rc * (R)1.0 * rpow<R>(lbdar, (R)P.XEXCACCR) * rhomcexvt;
+  } else S.rcaccr = 0;
+  if (rr_ok && !rc_ok) {
+    if (!soft) {
+      const R ev = rexp<R>((R)P.XALPW - (R)P.XBETAW / t - (R)P.XGAMW * rlog<R>(t));
+      const R usw = (R)1. - rv * ((R)pres - ev) / ((R)P.XEPSILO * ev);
+      const R x = sq((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) / (ka * (R)P.XRV * sq(t)) + ((R)P.XRV * t) / (dv * ev);
+      S.rrevav = (dmax((R)0., usw) / (rho * x)) * ((R)P.X0EVAR * rpow<R>(lbdar, (R)P.XEX0EVAR) + (R)P.X1EVAR * cj * rpow<R>(lbdar, (R)P.XEX1EVAR));
+    }
+  } else S.rrevav = 0;
+
+  // ICE4_FAST_RS (micro/mode_ice4_fast_rs.F90:109-441)
+  R freez;
+  if (rs_ok) {
+    if (!soft) {
+      R f1 = rv * (R)pres / ((R)P.XEPSILO + rv);
+      f1 = dmin(f1, rexp<R>((R)P.XALPI - (R)P.XBETAI / t - (R)P.XGAMI * rlog<R>(t)));
+      f1 = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - f1) / ((R)P.XRV * t));
+      f1 = f1 * ((R)P.X0DEPS * rpow<R>(lbdas, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * rpow<R>(lbdas, (R)P.XEX1DEPS)) /
+           (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
+      S.rs_freez1 = f1;
+      S.rs_freez2 = (rho * ((R)P.XLMTT + (R)(P.XCI - P.XCL) * ((R)P.XTT - t))) / (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
+    }
+    freez = dmax((R)0., dmax((R)0., S.rs_freez1 + S.rs_freez2 * S.riaggs) - S.riaggs);
+  } else {
+    S.rs_freez1 = 0; S.rs_freez2 = 0; freez = 0;
+  }
+  const bool grim = rc_ok && rs_ok;
+  if (!grim) { S.rs_rcrims = 0; S.rs_rcrimss = 0; S.rs_rsrimcg = 0; }
+  if (!soft && grim) {
+    const R zzw1 = interp1d<R>(lbdas, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM1);
+    const R zzw2 = interp1d<R>(lbdas, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM2);
+    const R zzw3 = interp1d<R>(lbdas, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM4);
+    const R lexc = rpow<R>(lbdas, (R)P.XEXCRIMSS);
+    S.rs_rcrimss = (R)P.XCRIMSS * zzw1 * rc * (R)1.0 * lexc * rhomcexvt;
+    S.rs_rcrims = (R)P.XCRIMSG * rc * (R)1.0 * lexc * rhomcexvt;
+    const R zzw = S.rs_rcrims - S.rs_rcrimss;
+    const R x = (R)P.XSRIMCG * rpow<R>(lbdas, (R)P.XEXSRIMCG) * ((R)1.0 - zzw2);
+    S.rs_rsrimcg = zzw * x / dmax((R)1.E-20, (R)P.XSRIMCG3 * (R)P.XSRIMCG2 * rpow<R>(lbdas, (R)P.XEXSRIMCG2) * ((R)1. - zzw3) - (R)P.XSRIMCG3 * x);
+  }
+  R rcrimss = 0, rcrimsg = 0, rsrimcg = 0;
+  if (grim && cold) {
+    rcrimss = dmin(freez, S.rs_rcrimss);
+    freez = dmax((R)0., freez - rcrimss);
+    const R zzw0d = dmin((R)1., freez / dmax((R)1.E-20, S.rs_rcrims - rcrimss));
+    rcrimsg = zzw0d * dmax((R)0., S.rs_rcrims - rcrimss);
+    freez = dmax((R)0., freez - rcrimsg);
+    rsrimcg = zzw0d * S.rs_rsrimcg;
+    rsrimcg = rsrimcg * dmax((R)0., -copysign((R)1., -rcrimsg));
+    rcrimsg = dmax((R)0., rcrimsg);
+  }
+  const bool gacc = rr_ok && rs_ok;
+  if (!gacc) { S.rs_rraccs = 0; S.rs_rraccss = 0; S.rs_rsaccrg = 0; }
+  if (!soft) {
+    S.rs_rraccs = 0; S.rs_rraccss = 0; S.rs_rsaccrg = 0;
+    if (gacc) {
+      const Idx2<R> ix = interp2d_idx<R>(lbdas, lbdar, P.NACCLBDAS, P.NACCLBDAR, (R)P.XACCINTP1S,
+                                         (R)P.XACCINTP2S, (R)P.XACCINTP1R, (R)P.XACCINTP2R);
+      const R zzw1 = interp2d<R>(ix, P.NACCLBDAS, tab + ICE3_T_RACCSS);
+      const R zzw2 = interp2d<R>(ix, P.NACCLBDAS, tab + ICE3_T_RACCS);
+      const R zzw3 = interp2d_t<R>(ix, P.NACCLBDAR, tab + ICE3_T_SACCRG);
+      const R rhom = rpow<R>(rho, -(R)P.XCEXVT - (R)1.);
+      const R lr2 = lbdar * lbdar;
+      const R zzw = (R)P.XFRACCSS * (rpow<R>(lbdas, (R)P.XCXS)) * rhom *
+                    ((R)P.XLBRACCS1 / (sq(lbdas)) + (R)P.XLBRACCS2 / (lbdas * lbdar) + (R)P.XLBRACCS3 / (lr2)) / (lr2 * lr2);
+      S.rs_rraccss = zzw1 * zzw;
+      S.rs_rraccs = zzw2 * zzw;
+      S.rs_rsaccrg = (R)P.XFSACCRG * zzw3 * (rpow<R>(lbdas, (R)(P.XCXS - P.XBS))) * rhom *
+                     ((R)P.XLBSACCR1 / (lr2) + (R)P.XLBSACCR2 / (lbdar * lbdas) + (R)P.XLBSACCR3 / (sq(lbdas))) / lbdar;
+    }
+  }
+  R rraccss = 0, rraccsg = 0, rsaccrg = 0;
+  if (gacc && cold) {
+    rraccss = dmin(freez, S.rs_rraccss);
+    freez = dmax((R)0., freez - rraccss);
+    const R zzw = dmin((R)1., freez / dmax((R)1.E-20, S.rs_rraccs - rraccss));
+    rraccsg = zzw * dmax((R)0., S.rs_rraccs - rraccss);
+    freez = dmax((R)0., freez - rraccsg);
+    rsaccrg = zzw * S.rs_rsaccrg;
+    rsaccrg = rsaccrg * dmax((R)0., -copysign((R)1., -rraccsg));
+    rraccsg = dmax((R)0., rraccsg);
+  }
+  if (rs_ok && warm) {
+    if (!soft) {
+      R m = rv * (R)pres / ((R)P.XEPSILO + rv);
+      m = dmin(m, rexp<R>((R)P.XALPW - (R)P.XBETAW / t - (R)P.XGAMW * rlog<R>(t)));
+      m = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - m) / ((R)P.XRV * t));
+      S.rsmltg = (R)P.XFSCVMG * dmax((R)0., (-m * ((R)P.X0DEPS * rpow<R>(lbdas, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * rpow<R>(lbdas, (R)P.XEX1DEPS)) -
+                                              (S.rs_rcrims + S.rs_rraccs) * (rho * (R)P.XCL * ((R)P.XTT - t))) / (rho * (R)P.XLMTT));
+      S.rcmltsr = S.rs_rcrims;
+    }
+  } else {
+    S.rsmltg = 0; S.rcmltsr = 0;
+  }
+
+  // ICE4_FAST_RG (micro/mode_ice4_fast_rg.F90:121-410)
+  if (ri_ok && rr_ok) {
+    if (!soft) {
+      R icf = (R)P.XICFRR * ri * rpow<R>(lbdar, (R)P.XEXICFRR) * rhomcexvt;
+      R rcf = (R)P.XRCFRI * (R)cit * rpow<R>(lbdar, (R)P.XEXRCFRI) * rpow<R>(rho, -(R)P.XCEXVT - (R)1.);
+      const R zzw0d = dmax((R)0., dmin((R)1., (icf * (R)P.XCI + rcf * (R)P.XCL) * ((R)P.XTT - t) / dmax((R)1.E-20, (R)P.XLVTT * rcf)));
+      S.rrcfrig = zzw0d * rcf;
+      S.ricfrr = ((R)1. - zzw0d) * icf;
+      S.ricfrrg = zzw0d * icf;
+    }
+  } else {
+    S.ricfrrg = 0; S.rrcfrig = 0; S.ricfrr = 0;
+  }
+  const R lgx = (!soft && rg_ok) ? rpow<R>(lbdag, (R)(P.XCXG - P.XDG - 2.)) * rhomcexvt : (R)0;
+  if (rg_ok && rc_ok) {
+    if (!soft) S.rg_rcdryg = (R)P.XFCDRYG * rc * lgx;
+  } else S.rg_rcdryg = 0;
+  if (rg_ok && ri_ok) {
+    if (!soft) {
+      const R e = rexp<R>((R)P.XCOLEXIG * (t - (R)P.XTT));
+      S.rg_ridryg = (R)P.XFIDRYG * e * ri * lgx;
+      S.rg_riwetg = S.rg_ridryg / ((R)P.XCOLIG * e);
+    }
+  } else {
+    S.rg_ridryg = 0; S.rg_riwetg = 0;
+  }
+  const R rhom1 = (!soft && rg_ok) ? rpow<R>(rho, -(R)P.XCEXVT - (R)1.) : (R)0;
+  const bool gdrys = rs_ok && rg_ok;
+  if (!gdrys) { S.rg_rsdryg = 0; S.rg_rswetg = 0; }
+  if (!soft && gdrys) {
+    const Idx2<R> ix = interp2d_idx<R>(lbdag, lbdas, P.NDRYLBDAG, P.NDRYLBDAS, (R)P.XDRYINTP1G,
+                                       (R)P.XDRYINTP2G, (R)P.XDRYINTP1S, (R)P.XDRYINTP2S);
+    const R zzw = interp2d<R>(ix, P.NDRYLBDAG, tab + ICE3_T_SDRYG);
+    S.rg_rswetg = (R)P.XFSDRYG * zzw / (R)P.XCOLSG * (rpow<R>(lbdas, (R)(P.XCXS - P.XBS))) * (rpow<R>(lbdag, (R)P.XCXG)) * rhom1 *
+                  ((R)P.XLBSDRYG1 / (sq(lbdag)) + (R)P.XLBSDRYG2 / (lbdag * lbdas) + (R)P.XLBSDRYG3 / (sq(lbdas)));
+    S.rg_rsdryg = S.rg_rswetg * (R)P.XCOLSG * rexp<R>((R)P.XCOLEXSG * (t - (R)P.XTT));
+  }
+  const bool gdryr = rr_ok && rg_ok;
+  if (!gdryr) S.rg_rrdryg = 0;
+  if (!soft && gdryr) {
+    const Idx2<R> ix = interp2d_idx<R>(lbdag, lbdar, P.NDRYLBDAG, P.NDRYLBDAR, (R)P.XDRYINTP1G,
+                                       (R)P.XDRYINTP2G, (R)P.XDRYINTP1R, (R)P.XDRYINTP2R);
+    const R zzw = interp2d<R>(ix, P.NDRYLBDAG, tab + ICE3_T_RDRYG);
+    const R lr2 = lbdar * lbdar;
+    S.rg_rrdryg = (R)P.XFRDRYG * zzw * ((R)1.0 / (lr2 * lr2)) * (rpow<R>(lbdag, (R)P.XCXG)) * rhom1 *
+                  ((R)P.XLBRDRYG1 / (sq(lbdag)) + (R)P.XLBRDRYG2 / (lbdag * lbdar) + (R)P.XLBRDRYG3 / (lr2));
+  }
+  const R rdryg_init = S.rg_rcdryg + S.rg_ridryg + S.rg_rsdryg + S.rg_rrdryg;
+  R rwetg_init = 0;
+  bool ldwetg = false, lldryg = false;
+  if (rg_ok) {
+    if (!soft) {
+      R f1 = rv * (R)pres / ((R)P.XEPSILO + rv);
+      f1 = dmin(f1, rexp<R>((R)P.XALPI - (R)P.XBETAI / t - (R)P.XGAMI * rlog<R>(t)));
+      f1 = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - f1) / ((R)P.XRV * t));
+      f1 = f1 * ((R)P.X0DEPG * rpow<R>(lbdag, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * rpow<R>(lbdag, (R)P.XEX1DEPG)) /
+           (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
+      S.rg_freez1 = f1;
+      S.rg_freez2 = (rho * ((R)P.XLMTT + (R)(P.XCI - P.XCL) * ((R)P.XTT - t))) / (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
+    }
+    rwetg_init = dmax(S.rg_riwetg + S.rg_rswetg, dmax((R)0., S.rg_freez1 + S.rg_freez2 * (S.rg_riwetg + S.rg_rswetg)));
+    const R wet = dmax((R)0., rwetg_init - S.rg_riwetg - S.rg_rswetg);
+    const R dry = dmax((R)0., rdryg_init - S.rg_ridryg - S.rg_rsdryg);
+    ldwetg = wet <= dry && rdryg_init > (R)0.;
+    lldryg = cold && rdryg_init > (R)1.E-20 && wet > dry;
+  } else {
+    S.rg_freez1 = 0; S.rg_freez2 = 0;
+  }
+  R rrwetg = 0, rcwetg = 0, riwetg = 0, rswetg = 0, rcdryg = 0, rrdryg = 0, ridryg = 0, rsdryg = 0;
+  if (ldwetg) {
+    rrwetg = -(S.rg_riwetg + S.rg_rswetg + S.rg_rcdryg - rwetg_init);
+    rcwetg = S.rg_rcdryg; riwetg = S.rg_riwetg; rswetg = S.rg_rswetg;
+  }
+  if (lldryg) {
+    rcdryg = S.rg_rcdryg; rrdryg = S.rg_rrdryg; ridryg = S.rg_ridryg; rsdryg = S.rg_rsdryg;
+  }
+  if (rg_ok && warm) {
+    if (!soft) {
+      R m = rv * (R)pres / ((R)P.XEPSILO + rv);
+      m = dmin(m, rexp<R>((R)P.XALPW - (R)P.XBETAW / t - (R)P.XGAMW * rlog<R>(t)));
+      m = ka * ((R)P.XTT - t) + dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - m) / ((R)P.XRV * t);
+      S.rgmltr = dmax((R)0., (-m * ((R)P.X0DEPG * rpow<R>(lbdag, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * rpow<R>(lbdag, (R)P.XEX1DEPG)) -
+                              (S.rg_rcdryg + S.rg_rrdryg) * (rho * (R)P.XCL * ((R)P.XTT - t))) / (rho * (R)P.XLMTT));
+    }
+  } else S.rgmltr = 0;
+
+  // ICE4_FAST_RI (micro/mode_ice4_fast_ri.F90:70-81)
+  if (ssi > (R)0. && rc_ok && ri_ok && cit > 1.E-20) {
+    if (!soft) {
+      const R l = dmin((R)1.E8, (R)P.XLBI * rpow<R>(rho * ri / (R)cit, (R)P.XLBEXI));
+      S.rcberi = (ssi / (rho * ai)) * (R)cit * ((R)P.X0DEPI / l + (R)P.X2DEPI * cj * cj / rpow<R>(l, (R)(P.XDI + 2.0)));
+    }
+  } else S.rcberi = 0;
+
+  // aggregation into PA / PB (micro/mode_ice4_tendencies.F90:606-717), KRR=6
+  const double lsl = lsfact - lvfact;
+  double a;
+  a = 0.;
+  a = a + (double)S.rvdepg * lsfact;
+  a = a + (double)S.rchoni * lsl;
+  a = a + (double)S.rvdeps * lsfact;
+  a = a - (double)S.rrevav * lvfact;
+  a = a + (double)rcrimss * lsl;
+  a = a + (double)rcrimsg * lsl;
+  a = a + (double)rraccss * lsl;
+  a = a + (double)rraccsg * lsl;
+  a = a + ((double)S.rrcfrig - (double)S.ricfrr) * lsl;
+  a = a + ((double)rcwetg + (double)rrwetg) * lsl;
+  a = a + ((double)rcdryg + (double)rrdryg) * lsl;
+  a = a - (double)S.rgmltr * lsl;
+  a = a + (double)S.rcberi * lsl;
+  pa[ITH] = a;
+  a = 0.;
+  a = a - (double)S.rvdepg;
+  a = a - (double)S.rvdeps;
+  a = a + (double)S.rrevav;
+  pa[IRV] = a;
+  a = 0.;
+  a = a - (double)S.rchoni;
+  a = a - (double)S.rcautr;
+  a = a - (double)S.rcaccr;
+  a = a - (double)rcrimss;
+  a = a - (double)rcrimsg;
+  a = a - (double)S.rcmltsr;
+  a = a - (double)rcwetg;
+  a = a - (double)rcdryg;
+  a = a - (double)S.rcberi;
+  pa[IRC] = a;
+  a = 0.;
+  a = a + (double)S.rcautr;
+  a = a + (double)S.rcaccr;
+  a = a - (double)S.rrevav;
+  a = a - (double)rraccss;
+  a = a - (double)rraccsg;
+  a = a + (double)S.rcmltsr;
+  a = a - (double)S.rrcfrig + (double)S.ricfrr;
+  a = a - (double)rrwetg;
+  a = a - (double)rrdryg;
+  a = a + (double)S.rgmltr;
+  pa[IRR] = a;
+  a = 0.;
+  a = a + (double)S.rchoni;
+  a = a - (double)S.riaggs;
+  a = a - (double)S.riauts;
+  a = a - (double)S.ricfrrg - (double)S.ricfrr;
+  a = a - (double)riwetg;
+  a = a - (double)ridryg;
+  a = a + (double)S.rcberi;
+  pa[IRI] = a;
+  a = 0.;
+  a = a + (double)S.rvdeps;
+  a = a + (double)S.riaggs;
+  a = a + (double)S.riauts;
+  a = a + (double)rcrimss;
+  a = a - (double)rsrimcg;
+  a = a + (double)rraccss;
+  a = a - (double)rsaccrg;
+  a = a - (double)S.rsmltg;
+  a = a - (double)rswetg;
+  a = a - (double)rsdryg;
+  pa[IRS] = a;
+  a = 0.;
+  a = a + (double)S.rvdepg * 1.0;
+  a = a + (double)rcrimsg + (double)rsrimcg;
+  a = a + (double)rraccsg + (double)rsaccrg;
+  a = a + (double)S.rsmltg;
+  a = a + (double)S.ricfrrg + (double)S.rrcfrig;
+  a = a + (double)rcwetg + (double)riwetg + (double)rswetg + (double)rrwetg;
+  a = a + (double)rcdryg + (double)ridryg + (double)rsdryg + (double)rrdryg;
+  a = a - (double)S.rgmltr;
+  pa[IRG] = a;
+}
+
+constexpr int MAX_CALLS = 4096;  // sub-step cap per point (the reference has none)
+
+// ICE4_STEPPING (micro/mode_ice4_stepping.F90:199-537) for one microphysics point.
+// Returns the number of tendency calls, or -calls if the cap was hit.
+template <class R>
+__device__ inline int stepping_point(double dt, double exn, double rhodref, double pres,
+                                     double cf, double sigs, double &cit, double *vart,
+                                     double *extpk, double &rrevav,
+                                     const double *__restrict__ tab) {
+  const ice3_params &P = c_P;
+  StepState<R> S;
+  const double zinv = 1. / dt;
+#pragma unroll
+  for (int jv = 0; jv < 7; ++jv) extpk[jv] = extpk[jv] - vart[jv] * zinv;
+  const double sigma_rc = sigs * 2.;
+  double time = 0., za[7], zb[7];
+  int calls = 0;
+  while (time < dt) {
+    bool llcompute = true, soft = false;
+    while (llcompute) {
+      if (++calls > MAX_CALLS) return -calls;
+      const double sum2 = vart[IRI] + vart[IRS] + vart[IRG];
+      const double zdevide =
+          (P.XCPD + P.XCPV * vart[IRV] + P.XCL * (vart[IRC] + vart[IRR]) + P.XCI * sum2) * exn;
+      const double zzt = vart[ITH] * exn;
+      const double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
+      const double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
+      tendencies<R>(soft, exn, rhodref, lvfact, lsfact, pres, cf, sigma_rc, cit, zzt, vart, S,
+                    tab, za, zb);
+#pragma unroll
+      for (int jv = 0; jv < 7; ++jv) za[jv] = za[jv] + extpk[jv];
+      double maxtime = dt - time;
+      const double zx = P.XTT / exn;
+      if ((vart[ITH] - zx) * (vart[ITH] + zb[ITH] - zx) < 0.) maxtime = 0.;
+      if (fabs(za[ITH]) > 1.E-20) {
+        const double tth = (zx - zb[ITH] - vart[ITH]) / za[ITH];
+        if (tth > 0.) maxtime = dmin(maxtime, tth);
+      }
+#pragma unroll
+      for (int jv = 1; jv < 7; ++jv)
+        if (za[jv] < -1.E-20 && vart[jv] > P.XRTMIN[jv]) {
+          maxtime = dmin(maxtime, -(zb[jv] + vart[jv]) / za[jv]);
+          maxtime = dmax(maxtime, P.XMNH_TINY);
+        }
+      if (time + maxtime >= dt) llcompute = false;
+#pragma unroll
+      for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] + za[jv] * maxtime + zb[jv];
+      if (vart[IRI] <= 0.) cit = 0.;
+      time = time + maxtime;
+      soft = true;
+    }
+  }
+#pragma unroll
+  for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] - extpk[jv] * dt;
+  rrevav = (double)S.rrevav;
+  return calls;
+}
+
+// Fall speed of one species at one level, OTHER_SPECIES / PRISTINE_ICE
+// (micro/mode_ice4_sedimentation_stat.F90:295-431); jrr = 3 (r), 4 (i), 5 (s), 6 (g).
+template <class R>
+__device__ inline R fall_speed(int jrr, double rx, double rho) {
+  const ice3_params &P = c_P;
+  if (jrr == 4) {
+    if (!(rx > dmax(P.XRTMIN[4], 1.0E-7))) return (R)0;
+    return (R)P.XFSEDI * rpow<R>((R)rho, -(R)P.XCEXVT) *
+           rpow<R>(dmax((R)0.05E6, (R)-0.15319E6 - (R)0.021454E6 * rlog<R>((R)(rho * rx))), (R)P.XEXCSEDI);
+  }
+  if (!(rx > P.XRTMIN[jrr])) return (R)0;
+  const double fsed = jrr == 3 ? P.XFSEDR : (jrr == 5 ? P.XFSEDS : P.XFSEDG);
+  const double exsed = jrr == 3 ? P.XEXSEDR : (jrr == 5 ? P.XEXSEDS : P.XEXSEDG);
+  return (R)fsed * rpow<R>((R)rx, (R)(exsed - 1)) * rpow<R>((R)rho, (R)(exsed - P.XCEXVT - 1));
+}
+
+// One level of ICE4_SEDIMENTATION_STAT (:143-229) for one column: flux[] holds the
+// fluxes leaving the level above on entry and this level's on exit (double).
+template <class R>
+__device__ inline void sedimentation_level(double dt, double rho, double dz, double &rrs,
+                                           double &ris, double &rss, double &rgs,
+                                           double flux[7]) {
+  const ice3_params &P = c_P;
+  const double zinv = 1. / dt;
+  const double tsorhodz = dt / (rho * dz);
+  double *src[7] = {nullptr, nullptr, nullptr, &rrs, &ris, &rss, &rgs};
+#pragma unroll
+  for (int jrr = 3; jrr <= 6; ++jrr) {
+    const double rx = *src[jrr] * dt;
+    const double qp = flux[jrr] * tsorhodz;
+    const double thr = P.XRTMIN[jrr];
+    double cur;
+    if (rx > thr || qp > thr) {
+      const double w1 = (double)fall_speed<R>(jrr, rx, rho);
+      const double w2 = (double)fall_speed<R>(jrr, qp, rho);
+      const double f1 = dmin(rho * dz * rx * zinv, w1 * rho * rx);
+      cur = w2 != 0. ? f1 + dmax(0., 1. - dz / (dt * w2)) * flux[jrr] : f1;
+    } else {
+      cur = dmin(rho * dz * rx * zinv, 0. * rho * rx);
+    }
+    *src[jrr] = *src[jrr] + tsorhodz * (flux[jrr] - cur) * zinv;
+    flux[jrr] = cur;
+  }
+}
+
+}  // namespace ice3

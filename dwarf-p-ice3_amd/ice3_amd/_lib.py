@@ -1,0 +1,119 @@
+"""ctypes binding of libice3_hip.so (C ABI in include/ice3.h).
+
+The library is built in-tree by ``ice3_amd.build.build()`` (hipcc, gfx950) and
+loaded from ``ice3_amd/_lib/libice3_hip.so``.  There is no fallback: when the
+library is missing every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libice3_hip.so")
+
+ICE3_OK, ICE3_EINVAL, ICE3_EDEVICE, ICE3_ESUBSTEP = 0, -1, -2, -3
+PREC_FP64, PREC_MIXED = 0, 1
+
+_P = ctypes.c_void_p
+
+
+def _struct(name, fields):
+    return type(name, (ctypes.Structure,), {"_fields_": [(f, _P) for f in fields]})
+
+
+ADJ_IN_FIELDS = ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc", "ri", "rr",
+                 "rs", "rg", "cf_mf", "rc_mf", "ri_mf", "rvs", "rcs", "ris", "ths", "sigqsat")
+ADJ_OUT_FIELDS = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri",
+                  "hli_hcf", "cph", "lv", "ls", "rvs", "rcs", "ris", "ths", "srcs")
+RAIN_IN_FIELDS = ("exn", "dzz", "rhodj", "rhodref", "exn_ref", "pabs", "cldfr", "hlc_hrc",
+                  "hlc_hcf", "hli_hri", "hli_hcf", "sigs", "tht", "rvt", "rct", "rrt", "rit",
+                  "rst", "rgt")
+RAIN_IO_FIELDS = ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "ci")
+RAIN_OUT_FIELDS = ("evap3d", "rainfr", "inprc", "inprr", "inpri", "inprs", "inprg")
+PIPE_IN_FIELDS = ("pabs", "sigs", "th", "exn", "exn_ref", "rhodref", "rhodj", "dzz", "rv", "rc",
+                  "ri", "rr", "rs", "rg", "ci", "cf_mf", "rc_mf", "ri_mf", "ths", "rvs", "rcs",
+                  "rrs", "ris", "rss", "rgs", "sigqsat")
+PIPE_OUT_FIELDS = ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "cldfr", "hlc_hrc",
+                   "hlc_hcf", "hli_hri", "hli_hcf", "ci", "evap3d", "rainfr", "inprc", "inprr",
+                   "inpri", "inprs", "inprg")
+
+AdjustIn = _struct("ice3_ice_adjust_in", ADJ_IN_FIELDS)
+AdjustOut = _struct("ice3_ice_adjust_out", ADJ_OUT_FIELDS)
+RainIn = _struct("ice3_rain_ice_in", RAIN_IN_FIELDS)
+RainIO = _struct("ice3_rain_ice_inout", RAIN_IO_FIELDS)
+RainOut = _struct("ice3_rain_ice_out", RAIN_OUT_FIELDS)
+PipeIn = _struct("ice3_pipeline_in", PIPE_IN_FIELDS)
+PipeOut = _struct("ice3_pipeline_out", PIPE_OUT_FIELDS)
+Stats = _struct("ice3_stats", ("calls", "counters"))
+
+EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_constants",
+           "ice3_ice_adjust", "ice3_rain_ice", "ice3_adjust_rain_ice")
+
+_lib = None
+
+
+class Ice3Error(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise Ice3Error(f"{LIB_PATH} is missing: build it with ice3_amd.build.build()")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.ice3_ctx_create.argtypes = [ctypes.POINTER(_P)]
+        lib.ice3_ctx_destroy.argtypes = [_P]
+        lib.ice3_ctx_destroy.restype = None
+        lib.ice3_strerror.argtypes = [ctypes.c_int]
+        lib.ice3_strerror.restype = ctypes.c_char_p
+        lib.ice3_dump_constants.argtypes = [_P, ctypes.c_char_p, ctypes.c_long]
+        lib.ice3_dump_constants.restype = ctypes.c_long
+        common = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        lib.ice3_ice_adjust.argtypes = common + [ctypes.POINTER(AdjustIn), ctypes.POINTER(AdjustOut), _P]
+        lib.ice3_rain_ice.argtypes = common + [ctypes.POINTER(RainIn), ctypes.POINTER(RainIO),
+                                               ctypes.POINTER(RainOut), ctypes.POINTER(Stats), _P]
+        lib.ice3_adjust_rain_ice.argtypes = common + [ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut),
+                                                      ctypes.POINTER(Stats), _P]
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != ICE3_OK:
+        msg = load().ice3_strerror(rc).decode()
+        raise Ice3Error(f"{what} failed ({rc}): {msg}")
+
+
+def host_constants() -> dict:
+    """The library's derived constants and tables, built on the host (no GPU needed)."""
+    lib = load()
+    n = lib.ice3_dump_constants(None, None, 0)
+    if n < 0:
+        raise Ice3Error(f"ice3_dump_constants failed: {n}")
+    buf = ctypes.create_string_buffer(n + 1)
+    lib.ice3_dump_constants(None, buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        p = line.split()
+        out[p[0]] = [float(x) for x in p[2:]]
+    return out
+
+
+_ctx = {}
+
+
+def context(device: int | None = None):
+    """Per-device context (constants uploaded once per device, kept until exit)."""
+    import torch
+
+    dev = torch.cuda.current_device() if device is None else device
+    if dev not in _ctx:
+        lib = load()
+        with torch.cuda.device(dev):
+            h = _P()
+            check(lib.ice3_ctx_create(ctypes.byref(h)), "ice3_ctx_create")
+        _ctx[dev] = h
+    return _ctx[dev]

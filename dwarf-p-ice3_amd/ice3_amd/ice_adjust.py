@@ -1,0 +1,83 @@
+"""IceAdjust: drop-in for the reference's IceAdjustJAX (src/ice3/jax/ice_adjust.py:40-230).
+
+Same call signature, argument names and 16-tuple result
+``(t, rv_out, rc_out, ri_out, cldfr, hlc_hrc, hlc_hcf, hli_hri, hli_hcf, cph, lv, ls,
+rvs, rcs, ris, ths)``; the arithmetic is PHYEX ICE_ADJUST (micro/ice_adjust.F90) with
+the sigma_s subgrid scheme, evaluated by the HIP kernel ``k_ice_adjust``.
+
+Semantics follow the Fortran, where the JAX stencil diverges from it:
+* ``t, rv_out, rc_out, ri_out`` are ICE_ADJUST's POUT_* adjusted state including the
+  mass-flux condensate (micro/ice_adjust.F90:449-472).  Like the Fortran, they are NaN
+  where CB02 condenses more than the total water while rc_mf = ri_mf = 0 (the
+  limiter divides 0/0 there); the sources rvs..ths are limited and stay finite.
+* ``sigqsat`` is per column, as PSIGQSAT(NIJT); a (..., nz) array is accepted and its
+  lowest level is used.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._fields import as_columns, from_columns, ptr, stream_ptr
+
+_PREC = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}
+_OUT_ORDER = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri",
+              "hli_hcf", "cph", "lv", "ls", "rvs", "rcs", "ris", "ths")
+
+
+class IceAdjust:
+    """ICE_ADJUST on MI355X.
+
+    Parameters
+    ----------
+    phyex : ignored placeholder for API compatibility (the configuration is the pinned
+        AROME ICE3 one); anything but None/"AROME" raises.
+    jit : accepted for API compatibility (kernels are compiled ahead of time).
+    layout : "columns" for (nlev, ncol) arrays with columns contiguous (the Fortran
+        (NIJT, NKT) memory, no copy), or "jax" for the JAX components' (nx, ny, nz).
+    device : CUDA device (default: current).
+    """
+
+    def __init__(self, phyex=None, jit: bool = True, layout: str = "columns", device=None):
+        if phyex not in (None, "AROME"):
+            raise ValueError("only the pinned AROME ICE3 configuration is implemented")
+        if layout not in ("columns", "jax"):
+            raise ValueError("layout must be 'columns' or 'jax'")
+        self.layout = layout
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        self._lib = _lib.load()
+        self._ctx = _lib.context(self.device.index)
+
+    def __call__(self, sigqsat, pabs, sigs, th, exn, exn_ref, rho_dry_ref, rv, rc, ri, rr, rs, rg,
+                 cf_mf, rc_mf, ri_mf, rvs, rcs, ris, ths, timestep: float,
+                 precision: str = "fp64", return_srcs: bool = False):
+        args = dict(pabs=pabs, sigs=sigs, th=th, exn=exn, exn_ref=exn_ref, rho_dry_ref=rho_dry_ref,
+                    rv=rv, rc=rc, ri=ri, rr=rr, rs=rs, rg=rg, cf_mf=cf_mf, rc_mf=rc_mf, ri_mf=ri_mf,
+                    rvs=rvs, rcs=rcs, ris=ris, ths=ths)
+        shape = tuple(torch.as_tensor(th).shape)
+        f = {k: as_columns(v, self.layout, device=self.device, name=k) for k, v in args.items()}
+        nlev, ncol = f["th"].shape
+        for k, v in f.items():
+            if tuple(v.shape) != (nlev, ncol):
+                raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
+        sq = torch.as_tensor(sigqsat, dtype=torch.float64, device=self.device)
+        if sq.dim() > 1:  # a full field: PSIGQSAT is per column, take the lowest level
+            sq = as_columns(sq, self.layout, device=self.device, name="sigqsat")[0]
+        sq = sq.reshape(-1).contiguous()
+        if sq.numel() != ncol:
+            raise ValueError(f"sigqsat: {sq.numel()} values for {ncol} columns")
+        out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) for k in _OUT_ORDER}
+        srcs = torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) if return_srcs else None
+        cin = _lib.AdjustIn(*[ptr(f[k]) for k in _lib.ADJ_IN_FIELDS[:-1]], ptr(sq))
+        cout = _lib.AdjustOut(*[ptr(out[k]) for k in _lib.ADJ_OUT_FIELDS[:-1]], ptr(srcs))
+        with torch.cuda.device(self.device):
+            rc_ = self._lib.ice3_ice_adjust(self._ctx, _PREC[precision], ncol, nlev, ncol, float(timestep),
+                                            ctypes.byref(cin), ctypes.byref(cout), stream_ptr(self.device))
+        _lib.check(rc_, "ice3_ice_adjust")
+        res = [out[k] for k in _OUT_ORDER]
+        res = [from_columns(r, self.layout, shape) for r in res]
+        if srcs is not None:
+            srcs = from_columns(srcs, self.layout, shape)
+        return (*res, srcs) if return_srcs else tuple(res)

@@ -1,0 +1,144 @@
+"""RainIce / AdjustRainIce: drop-ins for the reference's RainIceJAX (src/ice3/jax/rain_ice.py:20-420).
+
+``RainIce(constants)(state, dt)`` returns ``(state, diagnostics)`` like RainIceJAX: the
+state dict's sources ``ths, rvs, rcs, rrs, ris, rss, rgs`` and ``ci_t`` are replaced by
+their updated values, ``evap3d``/``rainfr`` are added, and the diagnostics hold the
+surface precipitation ``inprc, inprr, inpri, inprs, inprg``.  Arithmetic is PHYEX
+RAIN_ICE (micro/rain_ice.F90) with statistical sedimentation before the
+microphysics, the per-point ICE4 time splitting, the total tendencies and the
+negativity correction, evaluated by the HIP column kernel ``k_column``.
+
+State keys read (RainIceJAX names): exn, rhodref, pres, dzz, th_t, rv_t, rc_t, rr_t, ri_t,
+rs_t, rg_t, ci_t, ths, rvs, rcs, rrs, ris, rss, rgs; optional: exnref (default exn),
+rhodj (default rhodref), cf (cloud fraction, default 0), sigs (default 0), hlc_hrc,
+hlc_hcf, hli_hri, hli_hcf (default 0).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._fields import as_columns, from_columns, ptr, stream_ptr
+
+_PREC = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}
+
+
+class _Base:
+    def __init__(self, constants=None, layout: str = "columns", device=None):
+        if constants not in (None, "AROME") and not isinstance(constants, dict):
+            raise ValueError("only the pinned AROME ICE3 configuration is implemented")
+        if layout not in ("columns", "jax"):
+            raise ValueError("layout must be 'columns' or 'jax'")
+        self.layout = layout
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        self._lib = _lib.load()
+        self._ctx = _lib.context(self.device.index)
+        self.last_counters = None
+
+    def _col(self, x, name):
+        return as_columns(x, self.layout, device=self.device, name=name)
+
+    def _stats(self, n, with_calls):
+        counters = torch.zeros(3, dtype=torch.int64, device=self.device)
+        calls = torch.zeros(n, dtype=torch.int32, device=self.device) if with_calls else None
+        return counters, calls, _lib.Stats(ptr(calls), ptr(counters))
+
+    def _finish_counters(self, counters):
+        c = counters.cpu().tolist()
+        self.last_counters = {"tendency_calls": c[0], "capped_points": c[1], "micro_points": c[2]}
+        if c[1]:
+            raise _lib.Ice3Error(f"{c[1]} points hit the ICE4 sub-step cap ({_lib.load().ice3_strerror(-3).decode()})")
+
+
+class RainIce(_Base):
+    """RAIN_ICE on MI355X (see module docstring)."""
+
+    def __call__(self, state: dict, dt: float, lsedim_after: bool = False, ldeposc: bool = False,
+                 ldsoft: bool = False, lfeedbackt: bool = True, max_iterations: int | None = None,
+                 precision: str = "fp64", return_calls: bool = False):
+        if lsedim_after or ldeposc or ldsoft or not lfeedbackt:
+            raise ValueError("only LSEDIM_AFTER=F, LDEPOSC=F, LFEEDBACKT=T are implemented")
+        shape = tuple(torch.as_tensor(state["th_t"]).shape)
+        g = lambda k: self._col(state[k], k)  # noqa: E731
+        th = g("th_t")
+        nlev, ncol = th.shape
+        z = torch.zeros((nlev, ncol), dtype=torch.float64, device=self.device)
+        opt = lambda k, d: self._col(state[k], k) if k in state else d  # noqa: E731
+        exn = g("exn")
+        rhodref = g("rhodref")
+        inp = dict(exn=exn, dzz=g("dzz"), rhodj=opt("rhodj", rhodref), rhodref=rhodref,
+                   exn_ref=opt("exnref", exn), pabs=g("pres"), cldfr=opt("cf", z),
+                   hlc_hrc=opt("hlc_hrc", z), hlc_hcf=opt("hlc_hcf", z), hli_hri=opt("hli_hri", z),
+                   hli_hcf=opt("hli_hcf", z), sigs=opt("sigs", z), tht=th, rvt=g("rv_t"), rct=g("rc_t"),
+                   rrt=g("rr_t"), rit=g("ri_t"), rst=g("rs_t"), rgt=g("rg_t"))
+        # sources and ci are updated in place on private copies
+        io = {k: g(k).clone() for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs")}
+        io["ci"] = g("ci_t").clone()
+        out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) for k in ("evap3d", "rainfr")}
+        out.update({k: torch.empty(ncol, dtype=torch.float64, device=self.device)
+                    for k in ("inprc", "inprr", "inpri", "inprs", "inprg")})
+        for k, v in list(inp.items()) + list(io.items()):
+            if tuple(v.shape) != (nlev, ncol):
+                raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
+        counters, calls, st = self._stats(nlev * ncol, return_calls)
+        cin = _lib.RainIn(*[ptr(inp[k]) for k in _lib.RAIN_IN_FIELDS])
+        cio = _lib.RainIO(*[ptr(io[k]) for k in _lib.RAIN_IO_FIELDS])
+        cout = _lib.RainOut(*[ptr(out[k]) for k in _lib.RAIN_OUT_FIELDS])
+        with torch.cuda.device(self.device):
+            rc = self._lib.ice3_rain_ice(self._ctx, _PREC[precision], ncol, nlev, ncol, float(dt),
+                                         ctypes.byref(cin), ctypes.byref(cio), ctypes.byref(cout),
+                                         ctypes.byref(st), stream_ptr(self.device))
+        _lib.check(rc, "ice3_rain_ice")
+        self._finish_counters(counters)
+        new = dict(state)
+        for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs"):
+            new[k] = from_columns(io[k], self.layout, shape)
+        new["ci_t"] = from_columns(io["ci"], self.layout, shape)
+        new["evap3d"] = from_columns(out["evap3d"], self.layout, shape)
+        new["rainfr"] = from_columns(out["rainfr"], self.layout, shape)
+        diags = {k: out[k] for k in ("inprc", "inprr", "inpri", "inprs", "inprg")}
+        if return_calls:
+            diags["calls"] = from_columns(calls.view(nlev, ncol), self.layout, shape)
+        return new, diags
+
+
+class AdjustRainIce(_Base):
+    """ICE_ADJUST followed by RAIN_ICE in one device pass (the fused time step).
+
+    Inputs are the IceAdjust arguments plus ``rhodj, dzz, ci, rrs, rss, rgs``; the
+    rain_ice t-state is the adjusted sources times dt.  Returns a dict with the final
+    sources, ``cldfr, hlc_*, hli_*, ci, evap3d, rainfr`` and the surface precipitation.
+    """
+
+    def __call__(self, f: dict, dt: float, precision: str = "fp64", return_calls: bool = False):
+        shape = tuple(torch.as_tensor(f["th"]).shape)
+        g = {k: self._col(f[k], k) for k in _lib.PIPE_IN_FIELDS if k != "sigqsat"}
+        nlev, ncol = g["th"].shape
+        sq = torch.as_tensor(f["sigqsat"], dtype=torch.float64, device=self.device)
+        if sq.dim() > 1:
+            sq = self._col(sq, "sigqsat")[0]
+        sq = sq.reshape(-1).contiguous()
+        if sq.numel() != ncol:
+            raise ValueError(f"sigqsat: {sq.numel()} values for {ncol} columns")
+        for k, v in g.items():
+            if tuple(v.shape) != (nlev, ncol):
+                raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
+        out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device)
+               for k in _lib.PIPE_OUT_FIELDS[:15]}
+        out.update({k: torch.empty(ncol, dtype=torch.float64, device=self.device)
+                    for k in _lib.PIPE_OUT_FIELDS[15:]})
+        counters, calls, st = self._stats(nlev * ncol, return_calls)
+        cin = _lib.PipeIn(*[ptr(g[k]) for k in _lib.PIPE_IN_FIELDS[:-1]], ptr(sq))
+        cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
+        with torch.cuda.device(self.device):
+            rc = self._lib.ice3_adjust_rain_ice(self._ctx, _PREC[precision], ncol, nlev, ncol, float(dt),
+                                                ctypes.byref(cin), ctypes.byref(cout), ctypes.byref(st),
+                                                stream_ptr(self.device))
+        _lib.check(rc, "ice3_adjust_rain_ice")
+        self._finish_counters(counters)
+        res = {k: (from_columns(v, self.layout, shape) if v.dim() == 2 else v) for k, v in out.items()}
+        if return_calls:
+            res["calls"] = from_columns(calls.view(nlev, ncol), self.layout, shape)
+        return res

@@ -1,0 +1,128 @@
+/*
+ * ice3.h -- C ABI of libice3_hip.so, the MI355X implementation of the ICE3
+ * microphysics column kernels ice_adjust and rain_ice (PHYEX CY50T1 numerics,
+ * AROME ICE3 configuration, KRR=6).
+ *
+ * Plain C: pointers, sizes and return codes only.  Every field pointer is a
+ * device pointer to float64 data laid out (nlev, ld): element (level k,
+ * column i) is at p[k*ld + i], columns contiguous, level 0 the lowest model
+ * level.  This is byte-identical to the reference's Fortran (NIJT, NKT) arrays
+ * (PHYEX-IAL_CY50T1/bridge/phyex_bridge.F90:96-118, NKL=1 at :128).
+ * Per-column arrays have ncol entries.  Calls are asynchronous on the given
+ * HIP stream (NULL = default stream); buffers stay owned by the caller.
+ *
+ * Entry points and the reference interfaces they replace:
+ *   ice3_ice_adjust      c_ice_adjust   (bridge/phyex_bridge.F90:27-34)  and the JAX
+ *                        IceAdjustJAX.__call__ (src/ice3/jax/ice_adjust.py:139-230)
+ *   ice3_rain_ice        c_rain_ice     (bridge/phyex_bridge.F90:242-254) and
+ *                        RainIceJAX.__call__ (src/ice3/jax/rain_ice.py:181-420)
+ *   ice3_adjust_rain_ice the two calls in sequence, fused on the device
+ *   ice3_ctx_create      c_ini_rain_ice (bridge/phyex_bridge.F90:215-239): INI_CST +
+ *                        INI_RAIN_ICE constants and lookup tables, uploaded once
+ */
+#ifndef ICE3_H
+#define ICE3_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define ICE3_OK 0
+#define ICE3_EINVAL (-1)     /* bad size, stride, precision or NULL pointer */
+#define ICE3_EDEVICE (-2)    /* HIP runtime error (launch, allocation, copy) */
+#define ICE3_ESUBSTEP (-3)   /* reported through ice3_stats: a point hit the sub-step cap */
+
+/* arithmetic of the microphysics kernels */
+#define ICE3_PREC_FP64 0     /* FP64 everywhere: parity with the FP64 reference */
+#define ICE3_PREC_MIXED 1    /* FP32 process rates and lookups; FP64 saturation adjustment,
+                                state update, sedimentation fluxes and budgets */
+
+typedef struct ice3_ctx ice3_ctx;
+
+/* INI_CST + INI_RAIN_ICE for the pinned configuration; uploads constants and the
+ * collection-kernel / incomplete-gamma tables to the current HIP device. */
+int ice3_ctx_create(ice3_ctx **ctx);
+void ice3_ctx_destroy(ice3_ctx *ctx);
+const char *ice3_strerror(int code);
+
+/* Host copy of the derived constants, written as text lines "NAME n v1 .. vn"
+ * (the format of the reference constants dump) into buf; returns the number
+ * of bytes needed (excluding the terminating NUL).  ctx may be NULL: the
+ * constants are then built on the host without touching a device. */
+long ice3_dump_constants(const ice3_ctx *ctx, char *buf, long buflen);
+
+/* ---- ice_adjust: ICE_ADJUST (micro/ice_adjust.F90:7) with CONDENSATION ---- */
+typedef struct {
+  const double *pabs, *sigs, *th, *exn, *exn_ref, *rho_dry_ref;
+  const double *rv, *rc, *ri, *rr, *rs, *rg;
+  const double *cf_mf, *rc_mf, *ri_mf;
+  const double *rvs, *rcs, *ris, *ths;  /* sources at entry */
+  const double *sigqsat;                /* per column */
+} ice3_ice_adjust_in;
+
+typedef struct {   /* the IceAdjustJAX 16-tuple, same order */
+  double *t, *rv_out, *rc_out, *ri_out, *cldfr;
+  double *hlc_hrc, *hlc_hcf, *hli_hri, *hli_hcf;
+  double *cph, *lv, *ls;
+  double *rvs, *rcs, *ris, *ths;        /* updated sources; may alias the inputs */
+  double *srcs;                         /* optional (NULL): s'rc'/2sigma_s^2 * lambda3 */
+} ice3_ice_adjust_out;
+
+int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double dt,
+                    const ice3_ice_adjust_in *in, const ice3_ice_adjust_out *out,
+                    void *stream);
+
+/* ---- rain_ice: RAIN_ICE (micro/rain_ice.F90:7) with statistical sedimentation,
+ *      total tendencies and negativity correction ---- */
+typedef struct {
+  const double *exn, *dzz, *rhodj, *rhodref, *exn_ref, *pabs;
+  const double *cldfr, *hlc_hrc, *hlc_hcf, *hli_hri, *hli_hcf, *sigs;
+  const double *tht, *rvt, *rct, *rrt, *rit, *rst, *rgt;   /* state at t */
+} ice3_rain_ice_in;
+
+typedef struct {   /* read and updated in place */
+  double *ths, *rvs, *rcs, *rrs, *ris, *rss, *rgs, *ci;
+} ice3_rain_ice_inout;
+
+typedef struct {
+  double *evap3d, *rainfr;                           /* per point */
+  double *inprc, *inprr, *inpri, *inprs, *inprg;     /* per column, surface precipitation */
+} ice3_rain_ice_out;
+
+typedef struct {   /* optional diagnostics, device pointers, each may be NULL */
+  int *calls;               /* per point: number of ICE4_TENDENCIES calls */
+  unsigned long long *counters; /* [0] total tendency calls, [1] points that hit the cap,
+                                   [2] microphysics points; accumulated (caller zeroes) */
+} ice3_stats;
+
+int ice3_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double dt,
+                  const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
+                  const ice3_rain_ice_out *out, const ice3_stats *stats, void *stream);
+
+/* ---- fused ice_adjust + rain_ice (the "adjust then microphysics" time step) ----
+ * ice_adjust runs on the state at t; its updated sources, times dt, are the
+ * t-state of rain_ice (the synthetic and AROME-like inputs carry the state at t
+ * in the sources, rxs = rx/dt); sedimentation and microphysics follow. */
+typedef struct {
+  const double *pabs, *sigs, *th, *exn, *exn_ref, *rhodref, *rhodj, *dzz;
+  const double *rv, *rc, *ri, *rr, *rs, *rg, *ci;
+  const double *cf_mf, *rc_mf, *ri_mf;
+  const double *ths, *rvs, *rcs, *rrs, *ris, *rss, *rgs;
+  const double *sigqsat;                 /* per column */
+} ice3_pipeline_in;
+
+typedef struct {
+  double *ths, *rvs, *rcs, *rrs, *ris, *rss, *rgs;   /* final sources */
+  double *cldfr, *hlc_hrc, *hlc_hcf, *hli_hri, *hli_hcf, *ci, *evap3d, *rainfr;
+  double *inprc, *inprr, *inpri, *inprs, *inprg;     /* per column */
+} ice3_pipeline_out;
+
+int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double dt,
+                         const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                         const ice3_stats *stats, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICE3_H */

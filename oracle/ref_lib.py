@@ -59,8 +59,28 @@ def dump_constants(path: str, dt: float = 50.0):
     lib.ref_dump_constants(b, len(b))
 
 
+# The compiled reference keeps its work arrays on the stack (Fortran automatic
+# arrays sized NIJT x NKT); calls are made on column chunks of this width.
+CHUNK = 32
+
+
+def _chunked(fn, d: dict, *a, **kw) -> dict:
+    ncol = next(v for v in d.values() if getattr(v, "ndim", 0) == 2).shape[1]
+    if ncol <= CHUNK:
+        return fn(d, *a, **kw)
+    parts = []
+    for c0 in range(0, ncol, CHUNK):
+        sub = {k: np.ascontiguousarray(v[..., c0:c0 + CHUNK]) for k, v in d.items()}
+        parts.append(fn(sub, *a, **kw))
+    return {k: np.concatenate([p[k] for p in parts], axis=-1) for k in parts[0]}
+
+
 def ice_adjust(f: dict, dt: float = 50.0) -> dict:
     """ICE_ADJUST on a synthetic field dict; returns the 16 JAX-order outputs + srcs."""
+    return _chunked(_ice_adjust, f, dt)
+
+
+def _ice_adjust(f: dict, dt: float) -> dict:
     lib = init(dt)
     nlev, ncol = f["th"].shape
     rvs, rcs, ris, ths = (f[k].copy() for k in ("rvs", "rcs", "ris", "ths"))
@@ -82,6 +102,10 @@ def rain_ice(s: dict, dt: float = 50.0, nproma: int = 1) -> dict:
     """Reinstated RAIN_ICE; ``s`` holds exn, dzz, rhodj, rhodref, exn_ref, pabs, ci, cldfr,
     hlc_hrc, hlc_hcf, hli_hri, hli_hcf, tht, rvt, rct, rrt, rit, rst, rgt, sigs and the
     sources ths..rgs.  Returns updated sources, ci, diagnostics."""
+    return _chunked(_rain_ice, s, dt, nproma)
+
+
+def _rain_ice(s: dict, dt: float, nproma: int) -> dict:
     lib = init(dt, nproma)
     nlev, ncol = s["tht"].shape
     io = {k: s[k].copy() for k in ("ci", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",

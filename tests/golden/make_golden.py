@@ -9,7 +9,10 @@ Runs in the build container only (it needs oracle/_ref/libphyex_ref.so, built by
   (generator variant "adjust"), plus a small case with non-zero mass-flux inputs;
 * ``rain_ice_fp64.npz``  inputs and outputs of the reinstated RAIN_ICE on the
   fused-workload generator (variant "pipeline"), evaluated per point
-  (NPROMICRO=1).
+  (NPROMICRO=1); keys ``in_*``/``out_*``.  A second case (``cov_in_*``/``cov_out_*``)
+  puts one 20-level precipitating band in every column, with band bases spread
+  from the ground to level 70, so that melting, riming, wet/dry graupel growth and
+  homogeneous freezing are all exercised.
 
 Usage: python3 tests/golden/make_golden.py
 """
@@ -74,6 +77,22 @@ def main():
         out[f"in_{k}"] = s[k]
     for k in RAIN_OUT:
         out[f"out_{k}"] = r[k]
+    # coverage case: one precipitating band per column, bases from the ground upward
+    f = synthetic.generate(16, 90, seed=20260004, variant="adjust", dt=DT)
+    rng = np.random.Generator(np.random.Philox(20260005))
+    lev = np.arange(90)[:, None]
+    start = np.linspace(0, 70, 16).astype(int)[None, :]
+    band = (lev >= start) & (lev < start + 20)
+    for k in ("rr", "rs", "rg"):
+        f[k] = np.ascontiguousarray(np.where(band, rng.uniform(1.0e-4, 5.0e-3, band.shape), f[k]))
+        f[k + "s"] = f[k] / DT
+    a = ref_lib.ice_adjust(f, DT)
+    s = ref_lib.pipeline_state(f, a, DT)
+    r = ref_lib.rain_ice(s, DT, nproma=1)
+    for k in RAIN_IN:
+        out[f"cov_in_{k}"] = s[k]
+    for k in RAIN_OUT:
+        out[f"cov_out_{k}"] = r[k]
     np.savez_compressed(os.path.join(HERE, "rain_ice_fp64.npz"), dt=DT, **out)
     print("golden vectors written to", HERE)
 

@@ -1,0 +1,83 @@
+"""The HIP library on the host: it loads, exports every C-ABI symbol of include/ice3.h,
+rejects bad arguments, and its constant/table builder reproduces the compiled
+reference's INI_CST + INI_RAIN_ICE (no GPU needed for any of this)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ice3_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return _lib.load()
+
+
+def test_header_declares_what_the_library_exports(lib):
+    hdr = open(os.path.join(ROOT, "include", "ice3.h")).read()
+    declared = set(re.findall(r"\b(ice3_\w+)\s*\(", hdr))
+    assert declared == set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts_match_header():
+    hdr = open(os.path.join(ROOT, "include", "ice3.h")).read()
+    for cname, py in (("ice3_ice_adjust_in", _lib.AdjustIn), ("ice3_ice_adjust_out", _lib.AdjustOut),
+                      ("ice3_rain_ice_in", _lib.RainIn), ("ice3_rain_ice_inout", _lib.RainIO),
+                      ("ice3_rain_ice_out", _lib.RainOut), ("ice3_pipeline_in", _lib.PipeIn),
+                      ("ice3_pipeline_out", _lib.PipeOut), ("ice3_stats", _lib.Stats)):
+        body = re.search(r"typedef struct \{([^}]*)\} %s;" % cname, hdr).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        names = re.findall(r"\*\s*(\w+)", body)
+        assert tuple(names) == tuple(f for f, _ in py._fields_), cname
+
+
+def test_bad_arguments_are_rejected_without_a_device(lib):
+    p = ctypes.c_void_p
+    assert lib.ice3_ice_adjust(None, 0, 4, 90, 4, 50.0, None, None, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_rain_ice(None, 0, 4, 90, 4, 50.0, None, None, None, None, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_adjust_rain_ice(None, 0, 4, 90, 4, 50.0, None, None, None, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_ctx_create(None) == _lib.ICE3_EINVAL
+    assert b"invalid" in lib.ice3_strerror(_lib.ICE3_EINVAL)
+    assert b"sub-step" in lib.ice3_strerror(_lib.ICE3_ESUBSTEP)
+
+
+def _golden_constants(golden_dir):
+    out = {}
+    for line in open(os.path.join(golden_dir, "constants.txt")):
+        p = line.split()
+        out[p[0]] = np.array([float(x) for x in p[2:]])
+    return out
+
+
+def test_constants_match_reference_ini_rain_ice(golden_dir):
+    ref = _golden_constants(golden_dir)
+    mine = {k: np.array(v) for k, v in _lib.host_constants().items()}
+    missing = set(mine) - set(ref)
+    assert missing <= {"XT0CRIAUTI"} or not missing
+    worst = {}
+    for k, v in mine.items():
+        if k not in ref:
+            continue
+        r = ref[k]
+        assert v.shape == r.shape, k
+        err = np.max(np.abs(v - r) / np.maximum(np.abs(r), 1e-300))
+        worst[k] = err
+    bad = {k: e for k, e in worst.items() if e > 4e-16 and not k.startswith("XGAMINC")}
+    assert not bad, bad
+    # GAMMA_INC tables: same algorithm, same abscissae (powers by binary powering)
+    for k in ("XGAMINC_RIM1", "XGAMINC_RIM2", "XGAMINC_RIM4"):
+        assert worst[k] <= 1e-14, (k, worst[k])
+
+
+def test_collection_kernel_tables_are_bit_exact(golden_dir):
+    ref = _golden_constants(golden_dir)
+    mine = _lib.host_constants()
+    for k in ("XKER_RACCSS", "XKER_RACCS", "XKER_SACCRG", "XKER_SDRYG", "XKER_RDRYG"):
+        assert np.array_equal(np.array(mine[k]), ref[k]), k
