@@ -25,21 +25,21 @@ __constant__ ice3_params c_P;
 
 enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6 };
 
-template <class R> __device__ inline R rexp(R x);
-template <class R> __device__ inline R rlog(R x);
-template <class R> __device__ inline R rpow(R x, R y);
-template <> __device__ inline double rexp<double>(double x) { return exp(x); }
-template <> __device__ inline double rlog<double>(double x) { return log(x); }
-template <> __device__ inline double rpow<double>(double x, double y) { return pow(x, y); }
-template <> __device__ inline float rexp<float>(float x) { return expf(x); }
-template <> __device__ inline float rlog<float>(float x) { return logf(x); }
-template <> __device__ inline float rpow<float>(float x, float y) { return powf(x, y); }
+template <class R> __device__ __forceinline__ R rexp(R x);
+template <class R> __device__ __forceinline__ R rlog(R x);
+template <class R> __device__ __forceinline__ R rpow(R x, R y);
+template <> __device__ __forceinline__ double rexp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ double rlog<double>(double x) { return log(x); }
+template <> __device__ __forceinline__ double rpow<double>(double x, double y) { return pow(x, y); }
+template <> __device__ __forceinline__ float rexp<float>(float x) { return expf(x); }
+template <> __device__ __forceinline__ float rlog<float>(float x) { return logf(x); }
+template <> __device__ __forceinline__ float rpow<float>(float x, float y) { return powf(x, y); }
 
-__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
-__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
-__device__ inline float dmin(float a, float b) { return a < b ? a : b; }
-__device__ inline float dmax(float a, float b) { return a > b ? a : b; }
-template <class T> __device__ inline T sq(T a) { return a * a; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ float dmin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float dmax(float a, float b) { return a > b ? a : b; }
+template <class T> __device__ __forceinline__ T sq(T a) { return a * a; }
 
 // ---------------------------------------------------------------- ice_adjust
 
@@ -48,7 +48,7 @@ struct AdjustOut {
 };
 
 // ZSRC_1D(-22:11), micro/condensation.F90:208-216
-__device__ inline double zsrc_1d(int i) {
+__device__ __forceinline__ double zsrc_1d(int i) {
   switch (i) {
     case -22: return 0.;            case -21: return 0.;
     case -20: return 2.0094444E-04; case -19: return 0.316670E-03;
@@ -72,7 +72,7 @@ __device__ inline double zsrc_1d(int i) {
 
 // ICE_ADJUST (micro/ice_adjust.F90:257-472) + ITERATION (:488-565) + CONDENSATION
 // (micro/condensation.F90:270-650) for one point; rvs..ths updated in place.
-__device__ inline void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs,
+__device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs,
                                         double th, double exn, double exnref, double rhodref,
                                         double rv, double rc, double ri, double rr, double rs,
                                         double rg, double cf_mf, double rc_mf, double ri_mf,
@@ -195,7 +195,7 @@ __device__ inline void ice_adjust_point(double dt, double sigqsat, double pabs, 
 
 // ICE4_NUCLEATION (micro/ice4_nucleation.func.h:68-143), .NOT.LOCND2, LFEEDBACKT
 template <class R>
-__device__ inline void nucleation(bool ldcompute, double tht, double pabst, double rhodref,
+__device__ __forceinline__ void nucleation(bool ldcompute, double tht, double pabst, double rhodref,
                                   double exn, double lsfact, double t, double rvt, double &cit,
                                   double &rvheni_mr) {
   const ice3_params &P = c_P;
@@ -241,7 +241,7 @@ template <class R> struct StepState {
 
 // INTERP_MICRO_1D (micro/interp_micro.func.h:34-88) for one point
 template <class R>
-__device__ inline R interp1d(R x, int knum, R p1, R p2, const double *__restrict__ lt) {
+__device__ __forceinline__ R interp1d(R x, int knum, R p1, R p2, const double *__restrict__ lt) {
   R idx = dmax((R)1.00001, dmin((R)knum - (R)0.00001, p1 * rlog<R>(x) + p2));
   int i = (int)idx;
   R f = idx - (R)i;
@@ -251,7 +251,7 @@ __device__ inline R interp1d(R x, int knum, R p1, R p2, const double *__restrict
 // INTERP_MICRO_2D (micro/interp_micro.func.h:159-223) for one point: indices + weights
 template <class R> struct Idx2 { int i1, i2; R f1, f2; };
 template <class R>
-__device__ inline Idx2<R> interp2d_idx(R x1, R x2, int n1, int n2, R p11, R p12, R p21, R p22) {
+__device__ __forceinline__ Idx2<R> interp2d_idx(R x1, R x2, int n1, int n2, R p11, R p12, R p21, R p22) {
   Idx2<R> r;
   R a = dmax((R)1.00001, dmin((R)n1 - (R)0.00001, p11 * rlog<R>(x1) + p12));
   r.i1 = (int)a;
@@ -263,7 +263,7 @@ __device__ inline Idx2<R> interp2d_idx(R x1, R x2, int n1, int n2, R p11, R p12,
 }
 // table lt is (n1, n2) column-major, addressed lt(i1, i2) (1-based)
 template <class R>
-__device__ inline R interp2d(const Idx2<R> &x, int n1, const double *__restrict__ lt) {
+__device__ __forceinline__ R interp2d(const Idx2<R> &x, int n1, const double *__restrict__ lt) {
 #define T(i, j) ((R)lt[((i)-1) + ((j)-1) * n1])
   return (T(x.i1 + 1, x.i2 + 1) * x.f2 - T(x.i1 + 1, x.i2) * (x.f2 - (R)1.0)) * x.f1 -
          (T(x.i1, x.i2 + 1) * x.f2 - T(x.i1, x.i2) * (x.f2 - (R)1.0)) * (x.f1 - (R)1.0);
@@ -271,7 +271,7 @@ __device__ inline R interp2d(const Idx2<R> &x, int n1, const double *__restrict_
 }
 // transposed addressing lt(i2, i1) of an (n2, n1) table (:211-216)
 template <class R>
-__device__ inline R interp2d_t(const Idx2<R> &x, int n2, const double *__restrict__ lt) {
+__device__ __forceinline__ R interp2d_t(const Idx2<R> &x, int n2, const double *__restrict__ lt) {
 #define T(i, j) ((R)lt[((i)-1) + ((j)-1) * n2])
   return (T(x.i2 + 1, x.i1 + 1) * x.f1 - T(x.i2 + 1, x.i1) * (x.f1 - (R)1.0)) * x.f2 -
          (T(x.i2, x.i1 + 1) * x.f1 - T(x.i2, x.i1) * (x.f1 - (R)1.0)) * (x.f2 - (R)1.0);
@@ -285,7 +285,7 @@ __device__ inline R interp2d_t(const Idx2<R> &x, int n2, const double *__restric
 // Soft calls (reuse of the hard call's rates) skip everything that only feeds
 // rate evaluations guarded by .NOT.LDSOFT, such as the slope parameters.
 template <class R>
-__device__ inline void tendencies(bool soft, double exn, double rhodref, double lvfact,
+__device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref, double lvfact,
                                   double lsfact, double pres, double cf, double sigma_rc,
                                   double &cit, double pt, const double *pvart,
                                   StepState<R> &S, const double *__restrict__ tab, double *pa,
@@ -676,7 +676,7 @@ constexpr int MAX_CALLS = 4096;  // sub-step cap per point (the reference has no
 // ICE4_STEPPING (micro/mode_ice4_stepping.F90:199-537) for one microphysics point.
 // Returns the number of tendency calls, or -calls if the cap was hit.
 template <class R>
-__device__ inline int stepping_point(double dt, double exn, double rhodref, double pres,
+__device__ __forceinline__ int stepping_point(double dt, double exn, double rhodref, double pres,
                                      double cf, double sigs, double &cit, double *vart,
                                      double *extpk, double &rrevav,
                                      const double *__restrict__ tab) {
@@ -732,7 +732,7 @@ __device__ inline int stepping_point(double dt, double exn, double rhodref, doub
 // Fall speed of one species at one level, OTHER_SPECIES / PRISTINE_ICE
 // (micro/mode_ice4_sedimentation_stat.F90:295-431); jrr = 3 (r), 4 (i), 5 (s), 6 (g).
 template <class R>
-__device__ inline R fall_speed(int jrr, double rx, double rho) {
+__device__ __forceinline__ R fall_speed(int jrr, double rx, double rho) {
   const ice3_params &P = c_P;
   if (jrr == 4) {
     if (!(rx > dmax(P.XRTMIN[4], 1.0E-7))) return (R)0;
@@ -745,33 +745,31 @@ __device__ inline R fall_speed(int jrr, double rx, double rho) {
   return (R)fsed * rpow<R>((R)rx, (R)(exsed - 1)) * rpow<R>((R)rho, (R)(exsed - P.XCEXVT - 1));
 }
 
-// One level of ICE4_SEDIMENTATION_STAT (:143-229) for one column: flux[] holds the
-// fluxes leaving the level above on entry and this level's on exit (double).
+// One species at one level of ICE4_SEDIMENTATION_STAT (:143-229) for one column:
+// flux holds the flux leaving the level above on entry and this level's on exit
+// (double); rs is the species' source, updated by the flux divergence.
+// jrr = 3 (r), 4 (i), 5 (s), 6 (g); the species are independent, so the
+// reference's species-outer loop order and this level-outer order agree.
 template <class R>
-__device__ inline void sedimentation_level(double dt, double rho, double dz, double &rrs,
-                                           double &ris, double &rss, double &rgs,
-                                           double flux[7]) {
+__device__ __forceinline__ void sediment_species(int jrr, double dt, double rho, double dz,
+                                                 double &rs, double &flux) {
   const ice3_params &P = c_P;
   const double zinv = 1. / dt;
   const double tsorhodz = dt / (rho * dz);
-  double *src[7] = {nullptr, nullptr, nullptr, &rrs, &ris, &rss, &rgs};
-#pragma unroll
-  for (int jrr = 3; jrr <= 6; ++jrr) {
-    const double rx = *src[jrr] * dt;
-    const double qp = flux[jrr] * tsorhodz;
-    const double thr = P.XRTMIN[jrr];
-    double cur;
-    if (rx > thr || qp > thr) {
-      const double w1 = (double)fall_speed<R>(jrr, rx, rho);
-      const double w2 = (double)fall_speed<R>(jrr, qp, rho);
-      const double f1 = dmin(rho * dz * rx * zinv, w1 * rho * rx);
-      cur = w2 != 0. ? f1 + dmax(0., 1. - dz / (dt * w2)) * flux[jrr] : f1;
-    } else {
-      cur = dmin(rho * dz * rx * zinv, 0. * rho * rx);
-    }
-    *src[jrr] = *src[jrr] + tsorhodz * (flux[jrr] - cur) * zinv;
-    flux[jrr] = cur;
+  const double rx = rs * dt;
+  const double qp = flux * tsorhodz;
+  const double thr = P.XRTMIN[jrr];
+  double cur;
+  if (rx > thr || qp > thr) {
+    const double w1 = (double)fall_speed<R>(jrr, rx, rho);
+    const double w2 = (double)fall_speed<R>(jrr, qp, rho);
+    const double f1 = dmin(rho * dz * rx * zinv, w1 * rho * rx);
+    cur = w2 != 0. ? f1 + dmax(0., 1. - dz / (dt * w2)) * flux : f1;
+  } else {
+    cur = dmin(rho * dz * rx * zinv, 0. * rho * rx);
   }
+  rs = rs + tsorhodz * (flux - cur) * zinv;
+  flux = cur;
 }
 
 }  // namespace ice3

@@ -1,0 +1,255 @@
+// ice3_stages.hpp -- the stages of the ice_adjust / rain_ice path as per-thread
+// device functions:
+//   adjust_at     one point of ICE_ADJUST;
+//   sedim_column  one column, levels walked top-down: the statistical
+//                 sedimentation flux recurrence (fluxes stay in registers) and
+//                 the surface precipitation;
+//   micro_at      one point: RAIN_ICE sections 1, 4, 5 and 8 (nucleation
+//                 outside the microphysics mask, the ICE4 time splitting, the
+//                 total tendencies and the negativity correction).
+// The kernels in ice3_kernels.hip launch one thread per column / per point on
+// these functions.  Everything here is force-inlined and every array index is a
+// compile-time constant after unrolling, so the state lives in registers.
+#pragma once
+#include "../../include/ice3.h"
+#include "ice3_physics.hpp"
+
+namespace ice3 {
+
+struct AdjustArgs {
+  int ncol, nlev, ld;
+  double dt;
+  ice3_ice_adjust_in in;
+  ice3_ice_adjust_out out;  // t, rv_out, rc_out, ri_out, cph, lv, ls, srcs may be NULL
+  double *ris_copy;         // optional second copy of the adjusted ris
+};
+
+// ICE_ADJUST at point j of column i (micro/ice_adjust.F90:257-472).
+__device__ __forceinline__ void adjust_at(const AdjustArgs &a, int i, size_t j) {
+  const ice3_ice_adjust_in &in = a.in;
+  double rvs = in.rvs[j], rcs = in.rcs[j], ris = in.ris[j], ths = in.ths[j];
+  AdjustOut o;
+  ice_adjust_point(a.dt, in.sigqsat[i], in.pabs[j], in.sigs[j], in.th[j], in.exn[j],
+                   in.exn_ref[j], in.rho_dry_ref[j], in.rv[j], in.rc[j], in.ri[j], in.rr[j],
+                   in.rs[j], in.rg[j], in.cf_mf[j], in.rc_mf[j], in.ri_mf[j], rvs, rcs, ris, ths,
+                   o);
+  const ice3_ice_adjust_out &out = a.out;
+  if (out.t) out.t[j] = o.t;
+  if (out.rv_out) out.rv_out[j] = o.rv;
+  if (out.rc_out) out.rc_out[j] = o.rc;
+  if (out.ri_out) out.ri_out[j] = o.ri;
+  out.cldfr[j] = o.cldfr;
+  out.hlc_hrc[j] = o.hlc_hrc;
+  out.hlc_hcf[j] = o.hlc_hcf;
+  out.hli_hri[j] = o.hli_hri;
+  out.hli_hcf[j] = o.hli_hcf;
+  if (out.cph) out.cph[j] = o.cph;
+  if (out.lv) out.lv[j] = o.lv;
+  if (out.ls) out.ls[j] = o.ls;
+  out.rvs[j] = rvs;
+  out.rcs[j] = rcs;
+  out.ris[j] = ris;
+  out.ths[j] = ths;
+  if (out.srcs) out.srcs[j] = o.srcs;
+  if (a.ris_copy) a.ris_copy[j] = ris;
+}
+
+// Sedimentation stage arguments.  src_* are the sources entering RAIN_ICE,
+// dst_* receive them after sedimentation (dst may alias src).
+struct SedimArgs {
+  int ncol, nlev, ld;
+  double dt;
+  const double *rhodref, *dzz;
+  const double *src_rr, *src_ri, *src_rs, *src_rg;
+  double *dst_rr, *dst_ri, *dst_rs, *dst_rg;
+  double *inprc, *inprr, *inpri, *inprs, *inprg;
+};
+
+// ICE4_SEDIMENTATION_STAT (micro/mode_ice4_sedimentation_stat.F90:143-229) for column i,
+// followed by the surface precipitation of RAIN_ICE (PINPRS += PINPRI, PINPRC = 0 with
+// LSEDIC=F).
+template <class R>
+__device__ __forceinline__ void sedim_column(const SedimArgs &a, int i) {
+  const double dt = a.dt;
+  double fr = 0., fi = 0., fs = 0., fg = 0.;
+  for (int k = a.nlev - 1; k >= 0; --k) {
+    const size_t j = (size_t)k * a.ld + i;
+    const double rho = a.rhodref[j], dz = a.dzz[j];
+    double rrs = a.src_rr[j], ris = a.src_ri[j], rss = a.src_rs[j], rgs = a.src_rg[j];
+    sediment_species<R>(IRR, dt, rho, dz, rrs, fr);
+    sediment_species<R>(IRI, dt, rho, dz, ris, fi);
+    sediment_species<R>(IRS, dt, rho, dz, rss, fs);
+    sediment_species<R>(IRG, dt, rho, dz, rgs, fg);
+    a.dst_rr[j] = rrs; a.dst_ri[j] = ris; a.dst_rs[j] = rss; a.dst_rg[j] = rgs;
+  }
+  a.inprc[i] = 0.;
+  a.inprr[i] = fr / c_P.XRHOLW;
+  a.inpri[i] = fi / c_P.XRHOLW;
+  a.inprs[i] = fs / c_P.XRHOLW + fi / c_P.XRHOLW;
+  a.inprg[i] = fg / c_P.XRHOLW;
+}
+
+// Microphysics stage arguments.  The t-state of species v is t[v][j] * tscale
+// (tscale = 1 when the caller hands the t-state over, dt when it is the adjusted
+// sources); src[v] are the sources after sedimentation, dst[v] the results (may
+// alias src).  Species order: th, rv, rc, rr, ri, rs, rg.
+struct MicroArgs {
+  int ncol, nlev, ld;
+  double dt, tscale;
+  const double *tab;
+  const double *exn, *rhodref, *exn_ref, *pabs, *cldfr, *sigs;
+  const double *t[7];
+  const double *src[7];
+  const double *src_ci;
+  double *dst[7];
+  double *ci, *evap3d, *rainfr;
+  int *calls;
+  unsigned long long *counters;  // [0] tendency calls, [1] capped points, [2] micro points
+};
+
+// RAIN_ICE sections 1, 4, 5, 8 (micro/rain_ice.F90:224-273, 391-431, 529-543 and the
+// reinstated 649-714) for point j.  Returns the number of tendency calls, negative if the
+// sub-step cap was hit.
+template <class R>
+__device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j) {
+  const ice3_params &P = c_P;
+  const double dt = a.dt, zinv = 1. / dt, ts = a.tscale;
+  const double exn = a.exn[j], rhodref = a.rhodref[j], exnref = a.exn_ref[j];
+  const double pabs = a.pabs[j], cldfr = a.cldfr[j], sigs = a.sigs[j];
+  const double tht = a.t[ITH][j] * ts, rvt = a.t[IRV][j] * ts, rct = a.t[IRC][j] * ts;
+  const double rrt = a.t[IRR][j] * ts, rit = a.t[IRI][j] * ts, rst = a.t[IRS][j] * ts;
+  const double rgt = a.t[IRG][j] * ts;
+  double s[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) s[v] = a.src[v][j];
+  double cit = a.src_ci[j];
+  const double zrice = rit + rst + rgt;
+  const double zdevide = P.XCPD + P.XCPV * rvt + P.XCL * (rct + rrt) + P.XCI * zrice;
+  const double zt = tht * exn;
+  double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide;
+  double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide;
+  const bool llmicro = rct > P.XRTMIN[2] || rrt > P.XRTMIN[3] || rit > P.XRTMIN[4] ||
+                       rst > P.XRTMIN[5] || rgt > P.XRTMIN[6];
+  double wr[7] = {tht, rvt, rct, rrt, rit, rst, rgt};
+  double rvheni = 0., evap3d = 0.;
+  int calls = 0;
+  if (!llmicro) {
+    cit = 0.;
+    nucleation<R>(true, tht, pabs, rhodref, exn, lsfact / exn, zt, rvt, cit, rvheni);
+  }
+  rvheni = dmin(s[IRV], rvheni / dt);
+  if (llmicro) {
+    double extpk[7] = {s[0], s[1], s[2], s[3], s[4], s[5], s[6]};
+    calls = stepping_point<R>(dt, exn, rhodref, pabs, cldfr, sigs, cit, wr, extpk, evap3d, a.tab);
+  }
+  // 8.1 total tendencies
+  lsfact = lsfact / exnref;
+  lvfact = lvfact / exnref;
+  const double wrv = (wr[IRV] - rvt) * zinv, wrc = (wr[IRC] - rct) * zinv;
+  const double wrr = (wr[IRR] - rrt) * zinv, wri = (wr[IRI] - rit) * zinv;
+  const double wrs = (wr[IRS] - rst) * zinv, wrg = (wr[IRG] - rgt) * zinv;
+  const double wth = (wrc + wrr) * lvfact + (wri + wrs + wrg + 0.) * lsfact;
+  double ths = s[ITH] + wth + rvheni * lsfact;
+  double rvs = s[IRV] + wrv - rvheni;
+  double rcs = s[IRC] + wrc;
+  double rrs = s[IRR] + wrr;
+  double ris = s[IRI] + wri + rvheni;
+  double rss = s[IRS] + wrs;
+  double rgs = s[IRG] + wrg;
+  // 8.2 ICE4_CORRECT_NEGATIVITIES (micro/mode_ice4_correct_negativities.F90:51-171)
+  double zw;
+  zw = rcs - dmax(rcs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rcs = rcs - zw;
+  zw = rrs - dmax(rrs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rrs = rrs - zw;
+  zw = ris - dmax(ris, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; ris = ris - zw;
+  zw = rss - dmax(rss, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rss = rss - zw;
+  zw = rgs - dmax(rgs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rgs = rgs - zw;
+  zw = dmin(1., dmax(P.XRTMIN[1] - rvs, 0.) / dmax(rcs + ris, 1.E-20));
+  ths = ths - zw * (rcs * lvfact + ris * lsfact);
+  rvs = rvs + zw * (rcs + ris);
+  rcs = (1. - zw) * rcs;
+  ris = (1. - zw) * ris;
+  zw = dmin(dmax(rrs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+  rvs = rvs + zw; rrs = rrs - zw; ths = ths - zw * lvfact;
+  zw = dmin(dmax(rss, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+  rvs = rvs + zw; rss = rss - zw; ths = ths - zw * lsfact;
+  zw = dmin(dmax(rgs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+  rvs = rvs + zw; rgs = rgs - zw; ths = ths - zw * lsfact;
+  a.dst[ITH][j] = ths; a.dst[IRV][j] = rvs; a.dst[IRC][j] = rcs; a.dst[IRR][j] = rrs;
+  a.dst[IRI][j] = ris; a.dst[IRS][j] = rss; a.dst[IRG][j] = rgs;
+  a.ci[j] = cit;
+  a.evap3d[j] = evap3d;
+  a.rainfr[j] = 1.;
+  if (a.calls) a.calls[j] = calls;
+  return calls;
+}
+
+// ---------------------------------------------------------------- argument set-up
+// Host-side wiring of the C-ABI structs into the stage arguments, shared by the
+// kernel launchers and by CPU builds of the stages.
+
+// rain_ice: sedimentation updates the precipitating sources in place, then the
+// microphysics reads the handed-over t-state and updates all sources in place.
+inline void rain_ice_args(int ncol, int nlev, int ld, double dt, const double *tab,
+                          const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
+                          const ice3_rain_ice_out *out, const ice3_stats *stats,
+                          SedimArgs &sa, MicroArgs &m) {
+  sa = SedimArgs{};
+  sa.ncol = ncol; sa.nlev = nlev; sa.ld = ld; sa.dt = dt;
+  sa.rhodref = in->rhodref; sa.dzz = in->dzz;
+  sa.src_rr = sa.dst_rr = io->rrs; sa.src_ri = sa.dst_ri = io->ris;
+  sa.src_rs = sa.dst_rs = io->rss; sa.src_rg = sa.dst_rg = io->rgs;
+  sa.inprc = out->inprc; sa.inprr = out->inprr; sa.inpri = out->inpri; sa.inprs = out->inprs;
+  sa.inprg = out->inprg;
+  m = MicroArgs{};
+  m.ncol = ncol; m.nlev = nlev; m.ld = ld; m.dt = dt; m.tscale = 1.; m.tab = tab;
+  m.exn = in->exn; m.rhodref = in->rhodref; m.exn_ref = in->exn_ref; m.pabs = in->pabs;
+  m.cldfr = in->cldfr; m.sigs = in->sigs;
+  const double *t[7] = {in->tht, in->rvt, in->rct, in->rrt, in->rit, in->rst, in->rgt};
+  double *v[7] = {io->ths, io->rvs, io->rcs, io->rrs, io->ris, io->rss, io->rgs};
+  for (int q = 0; q < 7; ++q) { m.t[q] = t[q]; m.src[q] = v[q]; m.dst[q] = v[q]; }
+  m.src_ci = m.ci = io->ci;
+  m.evap3d = out->evap3d; m.rainfr = out->rainfr;
+  if (stats) { m.calls = stats->calls; m.counters = stats->counters; }
+}
+
+// The fused time step.
+// 1. ICE_ADJUST writes the adjusted th, rv, rc, ri sources into the outputs and
+//    stages a copy of the adjusted ris in out->evap3d (the microphysics reads it
+//    there as the ri t-state, then overwrites it with the evaporation).
+// 2. Sedimentation of rr, rs, rg (input sources) and ri (adjusted) into the outputs.
+// 3. Microphysics with the t-state = adjusted sources x dt.
+inline void pipeline_args(int ncol, int nlev, int ld, double dt, const double *tab,
+                          const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                          const ice3_stats *stats, AdjustArgs &aa, SedimArgs &sa, MicroArgs &m) {
+  aa = AdjustArgs{};
+  aa.ncol = ncol; aa.nlev = nlev; aa.ld = ld; aa.dt = dt;
+  aa.in.pabs = in->pabs; aa.in.sigs = in->sigs; aa.in.th = in->th; aa.in.exn = in->exn;
+  aa.in.exn_ref = in->exn_ref; aa.in.rho_dry_ref = in->rhodref; aa.in.rv = in->rv;
+  aa.in.rc = in->rc; aa.in.ri = in->ri; aa.in.rr = in->rr; aa.in.rs = in->rs; aa.in.rg = in->rg;
+  aa.in.cf_mf = in->cf_mf; aa.in.rc_mf = in->rc_mf; aa.in.ri_mf = in->ri_mf;
+  aa.in.rvs = in->rvs; aa.in.rcs = in->rcs; aa.in.ris = in->ris; aa.in.ths = in->ths;
+  aa.in.sigqsat = in->sigqsat;
+  aa.out.cldfr = out->cldfr; aa.out.hlc_hrc = out->hlc_hrc; aa.out.hlc_hcf = out->hlc_hcf;
+  aa.out.hli_hri = out->hli_hri; aa.out.hli_hcf = out->hli_hcf;
+  aa.out.rvs = out->rvs; aa.out.rcs = out->rcs; aa.out.ris = out->ris; aa.out.ths = out->ths;
+  aa.ris_copy = out->evap3d;
+  sa = SedimArgs{};
+  sa.ncol = ncol; sa.nlev = nlev; sa.ld = ld; sa.dt = dt;
+  sa.rhodref = in->rhodref; sa.dzz = in->dzz;
+  sa.src_rr = in->rrs; sa.src_ri = out->ris; sa.src_rs = in->rss; sa.src_rg = in->rgs;
+  sa.dst_rr = out->rrs; sa.dst_ri = out->ris; sa.dst_rs = out->rss; sa.dst_rg = out->rgs;
+  sa.inprc = out->inprc; sa.inprr = out->inprr; sa.inpri = out->inpri; sa.inprs = out->inprs;
+  sa.inprg = out->inprg;
+  m = MicroArgs{};
+  m.ncol = ncol; m.nlev = nlev; m.ld = ld; m.dt = dt; m.tscale = dt; m.tab = tab;
+  m.exn = in->exn; m.rhodref = in->rhodref; m.exn_ref = in->exn_ref; m.pabs = in->pabs;
+  m.cldfr = out->cldfr; m.sigs = in->sigs;
+  const double *t[7] = {out->ths, out->rvs, out->rcs, in->rrs, out->evap3d, in->rss, in->rgs};
+  double *v[7] = {out->ths, out->rvs, out->rcs, out->rrs, out->ris, out->rss, out->rgs};
+  for (int q = 0; q < 7; ++q) { m.t[q] = t[q]; m.src[q] = v[q]; m.dst[q] = v[q]; }
+  m.src_ci = in->ci; m.ci = out->ci;
+  m.evap3d = out->evap3d; m.rainfr = out->rainfr;
+  if (stats) { m.calls = stats->calls; m.counters = stats->counters; }
+}
+
+}  // namespace ice3

@@ -1,0 +1,137 @@
+// CPU build of the device stages of libice3_hip (dwarf-p-ice3_amd/csrc/ice3_stages.hpp),
+// for tests only: the same adjust_at / sedim_column / micro_at code the kernels run,
+// executed point by point in the kernels' launch order (all ICE_ADJUST points, then
+// every column's sedimentation, then every microphysics point), so that the kernel
+// source can be checked against the oracle and under sanitizers without a GPU.
+// Entry points mirror include/ice3.h with host arrays and no context or stream.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "ice3_stages.hpp"
+
+using namespace ice3;
+
+static double *g_tab = nullptr;
+
+extern "C" int hst_init(void) {
+  if (!g_tab) {
+    g_tab = (double *)malloc(sizeof(double) * ICE3_T_TOTAL);
+    if (!g_tab) return -1;
+    ice3_build_params(&c_P, g_tab);
+  }
+  return 0;
+}
+
+template <class R>
+static void run_rain_ice(const SedimArgs &sa, const MicroArgs &m) {
+  for (int i = 0; i < sa.ncol; ++i) sedim_column<R>(sa, i);
+  unsigned long long n = 0, c = 0, p = 0;
+  for (int k = 0; k < m.nlev; ++k)
+    for (int i = 0; i < m.ncol; ++i) {
+      const int calls = micro_at<R>(m, (size_t)k * m.ld + i);
+      n += calls < 0 ? -calls : calls;
+      c += calls < 0;
+      p += calls != 0;
+    }
+  if (m.counters) { m.counters[0] += n; m.counters[1] += c; m.counters[2] += p; }
+}
+
+extern "C" int hst_ice_adjust(int ncol, int nlev, int ld, double dt, const ice3_ice_adjust_in *in,
+                              const ice3_ice_adjust_out *out) {
+  if (hst_init()) return -1;
+  AdjustArgs a = AdjustArgs{};
+  a.ncol = ncol; a.nlev = nlev; a.ld = ld; a.dt = dt; a.in = *in; a.out = *out;
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) adjust_at(a, i, (size_t)k * ld + i);
+  return 0;
+}
+
+extern "C" int hst_rain_ice(int prec, int ncol, int nlev, int ld, double dt,
+                            const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
+                            const ice3_rain_ice_out *out, const ice3_stats *stats) {
+  if (hst_init()) return -1;
+  SedimArgs sa;
+  MicroArgs m;
+  rain_ice_args(ncol, nlev, ld, dt, g_tab, in, io, out, stats, sa, m);
+  if (prec == ICE3_PREC_FP64) run_rain_ice<double>(sa, m);
+  else run_rain_ice<float>(sa, m);
+  return 0;
+}
+
+extern "C" int hst_adjust_rain_ice(int prec, int ncol, int nlev, int ld, double dt,
+                                   const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                                   const ice3_stats *stats) {
+  if (hst_init()) return -1;
+  AdjustArgs aa;
+  SedimArgs sa;
+  MicroArgs m;
+  pipeline_args(ncol, nlev, ld, dt, g_tab, in, out, stats, aa, sa, m);
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) adjust_at(aa, i, (size_t)k * ld + i);
+  if (prec == ICE3_PREC_FP64) run_rain_ice<double>(sa, m);
+  else run_rain_ice<float>(sa, m);
+  return 0;
+}
+
+#ifdef HST_MAIN
+// Sanitizer driver: host_stages_san <dir> <ncol> <nlev> <dt> <prec> reads
+// <dir>/<field>.bin (float64, (nlev, ncol), sigqsat (ncol)) for every
+// ice3_pipeline_in field, runs the fused step with exact-size heap buffers and
+// writes <dir>/out_<field>.bin.
+static double *load(const char *dir, const char *name, size_t n) {
+  char path[1024];
+  snprintf(path, sizeof path, "%s/%s.bin", dir, name);
+  FILE *f = fopen(path, "rb");
+  if (!f) { fprintf(stderr, "missing %s\n", path); exit(2); }
+  double *v = (double *)malloc(n * sizeof(double));
+  if (fread(v, sizeof(double), n, f) != n) { fprintf(stderr, "short %s\n", path); exit(2); }
+  fclose(f);
+  return v;
+}
+
+static void save(const char *dir, const char *name, const void *v, size_t bytes) {
+  char path[1024];
+  snprintf(path, sizeof path, "%s/out_%s.bin", dir, name);
+  FILE *f = fopen(path, "wb");
+  if (!f) exit(2);
+  fwrite(v, 1, bytes, f);
+  fclose(f);
+}
+
+int main(int argc, char **argv) {
+  if (argc != 6) return 2;
+  const char *dir = argv[1];
+  const int ncol = atoi(argv[2]), nlev = atoi(argv[3]), prec = atoi(argv[5]);
+  const double dt = atof(argv[4]);
+  const size_t n = (size_t)ncol * nlev;
+  static const char *IN[] = {"pabs", "sigs", "th", "exn", "exn_ref", "rhodref", "rhodj", "dzz",
+                             "rv", "rc", "ri", "rr", "rs", "rg", "ci", "cf_mf", "rc_mf", "ri_mf",
+                             "ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "sigqsat"};
+  static const char *OUT[] = {"ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "cldfr",
+                              "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf", "ci", "evap3d",
+                              "rainfr", "inprc", "inprr", "inpri", "inprs", "inprg"};
+  const double *inp[26];
+  for (int q = 0; q < 26; ++q) inp[q] = load(dir, IN[q], q == 25 ? (size_t)ncol : n);
+  double *outp[20];
+  for (int q = 0; q < 20; ++q) outp[q] = (double *)malloc((q >= 15 ? (size_t)ncol : n) * 8);
+  ice3_pipeline_in pin;
+  ice3_pipeline_out pout;
+  static_assert(sizeof pin == 26 * sizeof(void *), "ice3_pipeline_in layout");
+  static_assert(sizeof pout == 20 * sizeof(void *), "ice3_pipeline_out layout");
+  memcpy(&pin, inp, sizeof pin);
+  memcpy(&pout, outp, sizeof pout);
+  int *calls = (int *)malloc(n * sizeof(int));
+  unsigned long long counters[3] = {0, 0, 0};
+  ice3_stats st = {calls, counters};
+  if (hst_adjust_rain_ice(prec, ncol, nlev, ncol, dt, &pin, &pout, &st)) return 3;
+  for (int q = 0; q < 20; ++q) save(dir, OUT[q], outp[q], (q >= 15 ? (size_t)ncol : n) * 8);
+  save(dir, "calls", calls, n * sizeof(int));
+  printf("calls %llu capped %llu micro_points %llu\n", counters[0], counters[1], counters[2]);
+  for (int q = 0; q < 26; ++q) free((void *)inp[q]);
+  for (int q = 0; q < 20; ++q) free(outp[q]);
+  free(calls);
+  free(g_tab);
+  return 0;
+}
+#endif

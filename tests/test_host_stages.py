@@ -1,0 +1,112 @@
+"""The kernels' own stage source (dwarf-p-ice3_amd/csrc/ice3_stages.hpp), compiled for
+the CPU by g++ (tests/host_stages), against the oracle, the reference's golden
+vectors and under AddressSanitizer / UndefinedBehaviorSanitizer.
+
+This checks the arithmetic and indexing of the device code without a GPU; the GPU
+parity tests then check the hipcc build of the same source."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import floor_for, rel_err
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "host_stages"))
+import host_lib  # noqa: E402
+
+TOL = 1e-11
+RAIN_OUT = ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "ci", "evap3d", "rainfr",
+            "inprc", "inprr", "inpri", "inprs", "inprg")
+ADJ_OUT = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
+           "cph", "lv", "ls", "rvs", "rcs", "ris", "ths")
+
+
+def _worst(out, ref, names, flips=None, tol=TOL):
+    worst = {}
+    for k in names:
+        r = rel_err(out[k], ref[k], floor_for(k, tol))
+        if flips is not None and r.shape == flips.shape:
+            r = np.where(flips, 0.0, r)
+        elif flips is not None and r.ndim == 1:
+            r = np.where(flips.any(axis=0), 0.0, r)
+        worst[k] = float(np.nanmax(r))
+    return worst
+
+
+def _pipeline_input(ncol, seed=20260003):
+    from ice3_amd import synthetic
+    f = synthetic.generate(ncol, 90, seed=seed, variant="pipeline")
+    f["rhodref"] = f["rho_dry_ref"]
+    return f
+
+
+def test_host_pipeline_matches_oracle():
+    from oracle import oracle_lib, ref_lib
+    f = _pipeline_input(128)
+    out = host_lib.adjust_rain_ice(f, 50.0, 0)
+    a = oracle_lib.ice_adjust(f)
+    ref = oracle_lib.rain_ice(ref_lib.pipeline_state(f, a), with_iters=True)
+    flips = out["calls"] != ref["iters"]
+    assert flips.mean() < 2e-3, int(flips.sum())
+    w = _worst(out, ref, RAIN_OUT, flips)
+    w.update(_worst(out, a, ("cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf")))
+    print("host stages vs oracle, fused step: flips", int(flips.sum()), "worst", max(w.values()))
+    assert max(w.values()) <= TOL, w
+    # counters: total calls and microphysics points agree with the per-point record
+    calls = out["calls"]
+    assert int(out["counters"][0]) == int(np.abs(calls).sum())
+    assert int(out["counters"][2]) == int((calls != 0).sum())
+    assert int(out["counters"][1]) == 0
+
+
+@pytest.mark.parametrize("tag", ["base", "cov"])
+def test_host_rain_ice_golden(golden_dir, tag):
+    from oracle import oracle_lib
+    g = np.load(os.path.join(golden_dir, "rain_ice_fp64.npz"))
+    p = "" if tag == "base" else tag + "_"
+    s = {k[len(p) + 3:]: g[k] for k in g.files if k.startswith(p + "in_")}
+    ref = {k: g[p + "out_" + k] for k in RAIN_OUT}
+    out = host_lib.rain_ice(s, float(g["dt"]), 0)
+    iters = oracle_lib.rain_ice(s, float(g["dt"]), with_iters=True)["iters"]
+    flips = out["calls"] != iters
+    w = _worst(out, ref, RAIN_OUT, flips)
+    print(f"host stages rain_ice golden {tag}: flips {int(flips.sum())}, worst {max(w.values()):.2e}")
+    assert flips.mean() < 2e-3
+    assert max(w.values()) <= TOL, w
+
+
+def test_host_mixed_precision_water_budget():
+    """Mixed path (float rates and lookups): every column's total water changes only by
+    its surface precipitation, and the sources stay close to FP64."""
+    f = _pipeline_input(256, seed=20260009)
+    dt = 50.0
+    o64 = host_lib.adjust_rain_ice(f, dt, 0)
+    o32 = host_lib.adjust_rain_ice(f, dt, 1)
+    rho, dz = f["rhodref"], f["dzz"]
+    w0 = sum(f[k] for k in ("rvs", "rcs", "rrs", "ris", "rss", "rgs"))
+    for o in (o64, o32):
+        w1 = sum(o[k] for k in ("rvs", "rcs", "rrs", "ris", "rss", "rgs"))
+        col0 = (rho * dz * w0).sum(0) * dt
+        col1 = (rho * dz * w1).sum(0) * dt
+        prec = -1000.0 * (o["inprr"] + o["inprs"] + o["inprg"]) * dt
+        assert np.max(np.abs(col1 - col0 - prec) / col0) < 1e-12
+    for k in ("rrs", "rss", "rgs"):
+        scale = np.abs(o64[k]).max()
+        assert np.abs(o32[k] - o64[k]).max() <= 1e-2 * scale, k
+
+
+def test_host_stages_sanitized(tmp_path):
+    """ASan + UBSan (float-to-int casts included) over the fused step, FP64 and mixed."""
+    from ice3_amd import _lib as L
+    exe = host_lib.build_sanitized()
+    f = _pipeline_input(64, seed=20260010)
+    for k in L.PIPE_IN_FIELDS:
+        np.ascontiguousarray(f[k], dtype=np.float64).tofile(str(tmp_path / f"{k}.bin"))
+    for prec in ("0", "1"):
+        r = subprocess.run([exe, str(tmp_path), "64", "90", "50", prec], capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-3000:]
+        assert "calls" in r.stdout
