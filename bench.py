@@ -4,15 +4,19 @@ Workload (the headline configuration): 512 x 512 = 262,144 synthetic columns x 9
 levels, dt = 50 s, generator variant "pipeline" (precipitating bands in 20 % of the
 columns), seed 20260003.  A step is one pass of ICE_ADJUST -> statistical
 sedimentation -> ICE4 microphysics -> total tendencies -> negativity correction over
-all columns (one launch of the fused column kernel).  Inputs are resident in HBM
-before the timed region; the K steps are bracketed by synchronisations and timed
-with HIP events on the launch stream.
+all columns: one ice3_adjust_rain_ice call, i.e. the kernels k_ice_adjust, k_sedim
+and k_micro on one stream.  Inputs are resident in HBM before the timed region;
+the K steps are bracketed by synchronisations and timed with HIP events on the
+launch stream.
 
-Also reported: the roofline of the fused kernel (algorithmic bytes 320 B/point:
-25 float64 inputs + 15 outputs, plus 48 B/column), the mixed-precision variant
-with its per-field error against FP64, and a CPU baseline (the compiled FP64
-PHYEX reference when its library is present, else the C restatement of the
-oracle) timed on a bounded sample on the host cores.
+Also reported: the duration of each stage (HIP events recorded by the library
+around each kernel, in a second pass of K steps), the roofline of the dominant
+kernel from its algorithmic bytes per point (k_micro: 18 float64 reads + 10
+writes = 224 B/point), the whole step's algorithmic rate (25 float64 inputs + 15
+outputs = 320 B/point plus 48 B/column), the mixed-precision variant with its
+per-field error against FP64, and a CPU baseline (the compiled FP64 PHYEX
+reference when its library is present, else the C restatement of the oracle)
+timed on a bounded sample on the host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncol N] [--no-cpu]
 """
@@ -33,8 +37,13 @@ for _p in (ROOT, os.path.join(ROOT, "dwarf-p-ice3_amd")):
 NLEV = 90
 DT = 50.0
 SEED = 20260003
-BYTES_PER_POINT = 320.0   # 25 inputs + 15 outputs, float64
+BYTES_PER_POINT = 320.0   # whole step: 25 inputs + 15 outputs, float64
 BYTES_PER_COLUMN = 48.0   # sigqsat + 5 surface precipitation fields
+# algorithmic bytes of each stage (per point, per column) as launched by the fused step
+STAGE_BYTES = {"ice_adjust": (232.0, 8.0),      # 19 reads + 10 writes; sigqsat
+               "sedimentation": (80.0, 40.0),   # 6 reads + 4 writes; 5 precipitation fields
+               "microphysics": (224.0, 0.0)}    # 18 reads + 10 writes
+STAGE_KERNELS = {"ice_adjust": "k_ice_adjust", "sedimentation": "k_sedim", "microphysics": "k_micro"}
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 
 
@@ -93,6 +102,9 @@ def main():
     ap.add_argument("--ncol", type=int, default=512 * 512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device twice (a plain copy of known size, "
+                         "to scale HBM byte counters read by a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -123,30 +135,56 @@ def main():
         o.update({k: torch.empty(ncol, dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[15:]})
         return o
 
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration << 17, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     cin = _lib.PipeIn(*[ptr(g[k]) for k in _lib.PIPE_IN_FIELDS])
     stream = torch.cuda.current_stream(dev)
     sp = ctypes.c_void_p(stream.cuda_stream)
 
+    def call(prec, cout):
+        return lib.ice3_adjust_rain_ice(ctx, prec, ncol, NLEV, ncol, DT, ctypes.byref(cin),
+                                        ctypes.byref(cout), None, sp)
+
     def run(prec, out, steps, warmup):
         cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
         for _ in range(warmup):
-            _lib.check(lib.ice3_adjust_rain_ice(ctx, prec, ncol, NLEV, ncol, DT, ctypes.byref(cin),
-                                                ctypes.byref(cout), None, sp), "warmup")
+            _lib.check(call(prec, cout), "warmup")
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(steps):
-            lib.ice3_adjust_rain_ice(ctx, prec, ncol, NLEV, ncol, DT, ctypes.byref(cin), ctypes.byref(cout),
-                                     None, sp)
+            call(prec, cout)
         e1.record(stream)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps  # ms per step
 
+    def stage_times(prec, out, steps):
+        """Average duration of each stage's kernel, from the library's HIP events."""
+        cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
+        _lib.check(lib.ice3_stage_timing(ctx, 1), "ice3_stage_timing")
+        acc = [0.0, 0.0, 0.0]
+        ms3 = (ctypes.c_float * 3)()
+        for _ in range(steps):
+            _lib.check(call(prec, cout), "timed step")
+            _lib.check(lib.ice3_stage_times(ctx, ms3), "ice3_stage_times")
+            acc = [x + y for x, y in zip(acc, ms3)]
+        _lib.check(lib.ice3_stage_timing(ctx, 0), "ice3_stage_timing")
+        return dict(zip(("ice_adjust", "sedimentation", "microphysics"), [x / steps for x in acc]))
+
     out64 = outputs()
     ms = run(_lib.PREC_FP64, out64, args.steps, args.warmup)
+    stages = stage_times(_lib.PREC_FP64, out64, args.steps)
     npts = ncol * NLEV
     alg_bytes = BYTES_PER_POINT * npts + BYTES_PER_COLUMN * ncol
-    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    dom = max(stages, key=stages.get)
+    bp, bc = STAGE_BYTES[dom]
+    dom_achieved = (bp * npts + bc * ncol) / (stages[dom] * 1e-3) / 1e9
     res = {
         "metric": "grid columns/s (90 lev) ice_adjust+rain_ice, FP64",
         "value": ncol / (ms * 1e-3),
@@ -160,15 +198,31 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "fused ice_adjust+rain_ice, statistical sedimentation, FP64",
+        "config": {"workload": "fused ice_adjust+rain_ice, statistical sedimentation, FP64, "
+                               "512x512 columns x 90 levels, dt=50 s",
                    "columns": ncol, "levels": NLEV, "dt_s": DT, "generator": "pipeline", "seed": SEED},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": STAGE_KERNELS[dom], "achieved": dom_achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom_achieved / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "bytes_per_point": bp, "bytes_per_column": bc, "ms_per_launch": stages[dom]},
+        "stages_ms": stages,
+        "step_algorithmic_GBs": alg_bytes / (ms * 1e-3) / 1e9,
         "cpu_baseline": cpu,
     }
+    def budget_err(out):
+        """Max over columns of |column water change - surface precipitation| / column water."""
+        w = ("rvs", "rcs", "rrs", "ris", "rss", "rgs")
+        mass = g["rhodref"] * g["dzz"]
+        col0 = (mass * sum(g[k] for k in w)).sum(0) * DT
+        col1 = (mass * sum(out[k] for k in w)).sum(0) * DT
+        prec = -1000.0 * (out["inprr"] + out["inprs"] + out["inprg"]) * DT
+        return ((col1 - col0 - prec).abs() / col0).max().item()
+
+    res["water_budget_max_rel_err"] = budget_err(out64)
     if not args.no_mixed:
         out32 = outputs()
         ms32 = run(_lib.PREC_MIXED, out32, args.steps, args.warmup)
+        stages32 = stage_times(_lib.PREC_MIXED, out32, args.steps)
         err = {}
         for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs"):
             d = (out32[k] - out64[k]).abs()
@@ -176,7 +230,9 @@ def main():
             err[k] = {"max_abs": d.max().item(), "rms_abs": d.pow(2).mean().sqrt().item(),
                       "max_rel_to_field_max": d.max().item() / scale if scale > 0 else 0.0}
         res["variants"] = {"mixed": {"columns_per_s": ncol / (ms32 * 1e-3), "ms_per_step": ms32,
-                                     "speedup_vs_fp64": ms / ms32, "error_vs_fp64": err}}
+                                     "speedup_vs_fp64": ms / ms32, "stages_ms": stages32,
+                                     "water_budget_max_rel_err": budget_err(out32),
+                                     "error_vs_fp64": err}}
     print(json.dumps(res))
 
 

@@ -78,6 +78,8 @@ struct ice3_ctx {
   ice3_params params;
   double *d_tab;
   double h_tab[ICE3_T_TOTAL];
+  int timing;         // record ev[] around the stages of each call
+  hipEvent_t ev[4];   // before adjust, after adjust, after sedimentation, after microphysics
 };
 
 using namespace ice3;
@@ -108,8 +110,36 @@ extern "C" int ice3_ctx_create(ice3_ctx **out) {
 extern "C" void ice3_ctx_destroy(ice3_ctx *c) {
   if (!c) return;
   if (c->d_tab) (void)hipFree(c->d_tab);
+  if (c->timing)
+    for (int q = 0; q < 4; ++q) (void)hipEventDestroy(c->ev[q]);
   free(c);
 }
+
+extern "C" int ice3_stage_timing(ice3_ctx *c, int enable) {
+  if (!c) return ICE3_EINVAL;
+  if (enable && !c->timing) {
+    for (int q = 0; q < 4; ++q) HIPCHK(hipEventCreate(&c->ev[q]));
+    c->timing = 1;
+  } else if (!enable && c->timing) {
+    for (int q = 0; q < 4; ++q) (void)hipEventDestroy(c->ev[q]);
+    c->timing = 0;
+  }
+  return ICE3_OK;
+}
+
+extern "C" int ice3_stage_times(ice3_ctx *c, float ms[3]) {
+  if (!c || !ms || !c->timing) return ICE3_EINVAL;
+  HIPCHK(hipEventSynchronize(c->ev[3]));
+  for (int q = 0; q < 3; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
+  return ICE3_OK;
+}
+
+// Record stage event q of a timed call (no-op when timing is off).
+static int mark(ice3_ctx *c, int q, hipStream_t s) {
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[q], s));
+  return ICE3_OK;
+}
+#define MARK(q) do { int rc_ = mark(ctx, (q), s); if (rc_) return rc_; } while (0)
 
 extern "C" const char *ice3_strerror(int code) {
   switch (code) {
@@ -204,25 +234,27 @@ extern "C" int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int 
   AdjustArgs a = AdjustArgs{};  // ris_copy stays NULL
   a.ncol = ncol; a.nlev = nlev; a.ld = ld; a.dt = dt; a.in = *in; a.out = *out;
   dim3 grid((ncol + 255) / 256, nlev);
-  hipLaunchKernelGGL(k_ice_adjust, grid, dim3(256), 0, (hipStream_t)stream, a);
+  hipStream_t s = (hipStream_t)stream;
+  MARK(0);
+  hipLaunchKernelGGL(k_ice_adjust, grid, dim3(256), 0, s, a);
   HIPCHK(hipGetLastError());
+  MARK(1); MARK(2); MARK(3);
   return ICE3_OK;
 }
 
 // Launch the sedimentation then the microphysics stage on one stream.
-static int launch_rain_ice(int prec, const SedimArgs &sa, const MicroArgs &ma, void *stream) {
+static int launch_rain_ice(ice3_ctx *ctx, int prec, const SedimArgs &sa, const MicroArgs &ma,
+                           void *stream) {
   hipStream_t s = (hipStream_t)stream;
   const dim3 gs((sa.ncol + 255) / 256), gm((ma.ncol + 255) / 256, ma.nlev);
-  if (prec == ICE3_PREC_FP64) {
-    hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_micro<double>, gm, dim3(256), 0, s, ma);
-  } else {
-    hipLaunchKernelGGL(k_sedim<float>, gs, dim3(256), 0, s, sa);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_micro<float>, gm, dim3(256), 0, s, ma);
-  }
+  if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
+  else hipLaunchKernelGGL(k_sedim<float>, gs, dim3(256), 0, s, sa);
   HIPCHK(hipGetLastError());
+  MARK(2);
+  if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_micro<double>, gm, dim3(256), 0, s, ma);
+  else hipLaunchKernelGGL(k_micro<float>, gm, dim3(256), 0, s, ma);
+  HIPCHK(hipGetLastError());
+  MARK(3);
   return ICE3_OK;
 }
 
@@ -243,7 +275,9 @@ extern "C" int ice3_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld
   SedimArgs sa;
   MicroArgs m;
   rain_ice_args(ncol, nlev, ld, dt, ctx->d_tab, in, io, out, stats, sa, m);
-  return launch_rain_ice(prec, sa, m, stream);
+  hipStream_t s = (hipStream_t)stream;
+  MARK(0); MARK(1);
+  return launch_rain_ice(ctx, prec, sa, m, stream);
 }
 
 extern "C" int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld,
@@ -265,8 +299,10 @@ extern "C" int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev,
   SedimArgs sa;
   MicroArgs m;
   pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, stats, aa, sa, m);
-  hipLaunchKernelGGL(k_ice_adjust, dim3((ncol + 255) / 256, nlev), dim3(256), 0,
-                     (hipStream_t)stream, aa);
+  hipStream_t s = (hipStream_t)stream;
+  MARK(0);
+  hipLaunchKernelGGL(k_ice_adjust, dim3((ncol + 255) / 256, nlev), dim3(256), 0, s, aa);
   HIPCHK(hipGetLastError());
-  return launch_rain_ice(prec, sa, m, stream);
+  MARK(1);
+  return launch_rain_ice(ctx, prec, sa, m, stream);
 }

@@ -48,7 +48,8 @@ PipeOut = _struct("ice3_pipeline_out", PIPE_OUT_FIELDS)
 Stats = _struct("ice3_stats", ("calls", "counters"))
 
 EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_constants",
-           "ice3_ice_adjust", "ice3_rain_ice", "ice3_adjust_rain_ice")
+           "ice3_ice_adjust", "ice3_rain_ice", "ice3_adjust_rain_ice", "ice3_stage_timing",
+           "ice3_stage_times")
 
 _lib = None
 
@@ -77,6 +78,8 @@ def load():
                                                ctypes.POINTER(RainOut), ctypes.POINTER(Stats), _P]
         lib.ice3_adjust_rain_ice.argtypes = common + [ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut),
                                                       ctypes.POINTER(Stats), _P]
+        lib.ice3_stage_timing.argtypes = [_P, ctypes.c_int]
+        lib.ice3_stage_times.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
         _lib = lib
     return _lib
 

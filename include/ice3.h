@@ -122,6 +122,15 @@ int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, do
                          const ice3_pipeline_in *in, const ice3_pipeline_out *out,
                          const ice3_stats *stats, void *stream);
 
+/* ---- per-stage timing (no counterpart in the reference: a measurement hook) ----
+ * ice3_stage_timing(ctx, 1) makes the three entry points above record HIP events
+ * around each of their kernels on the call's stream; ice3_stage_times waits for
+ * the last such call and returns its stage durations in milliseconds:
+ * ms[0] ICE_ADJUST, ms[1] sedimentation, ms[2] microphysics (0 for a stage the
+ * call did not run).  Off by default; a context's events serve one call at a time. */
+int ice3_stage_timing(ice3_ctx *ctx, int enable);
+int ice3_stage_times(ice3_ctx *ctx, float ms[3]);
+
 #ifdef __cplusplus
 }
 #endif
