@@ -45,6 +45,20 @@ STAGE_BYTES = {"ice_adjust": (232.0, 8.0),      # 19 reads + 10 writes; sigqsat
                "microphysics": (224.0, 0.0)}    # 18 reads + 10 writes
 STAGE_KERNELS = {"ice_adjust": "k_ice_adjust", "sedimentation": "k_sedim", "microphysics": "k_micro"}
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this workload
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(kernel: str, ncol: int):
+    """Profiled HBM bytes per dispatch of `kernel` for the headline workload, else None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload", {}).get("columns") != ncol or t["workload"].get("levels") != NLEV:
+        return None
+    return t.get("bytes_per_dispatch", {}).get(kernel)
 
 
 def cpu_baseline(budget_s: float = 20.0, workers: int | None = None) -> dict:
@@ -203,7 +217,7 @@ def main():
                    "columns": ncol, "levels": NLEV, "dt_s": DT, "generator": "pipeline", "seed": SEED},
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNELS[dom], "achieved": dom_achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom_achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": measured_traffic(STAGE_KERNELS[dom], ncol),
                      "bytes_per_point": bp, "bytes_per_column": bc, "ms_per_launch": stages[dom]},
         "stages_ms": stages,
         "step_algorithmic_GBs": alg_bytes / (ms * 1e-3) / 1e9,
