@@ -116,6 +116,7 @@ def main():
     ap.add_argument("--ncol", type=int, default=512 * 512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-adjust", action="store_true", help="skip the ice_adjust-alone line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device twice (a plain copy of known size, "
                          "to scale HBM byte counters read by a profiler)")
@@ -247,7 +248,44 @@ def main():
                                      "speedup_vs_fp64": ms / ms32, "stages_ms": stages32,
                                      "water_budget_max_rel_err": budget_err(out32),
                                      "error_vs_fp64": err}}
+    if not args.no_adjust:
+        res.setdefault("variants", {})["ice_adjust_256x256"] = ice_adjust_alone(lib, ctx, dev, args)
     print(json.dumps(res))
+
+
+def ice_adjust_alone(lib, ctx, dev, args, ncol: int = 256 * 256):
+    """ice3_ice_adjust alone, FP64, 256x256 columns x 90 levels of the "adjust" generator:
+    19 inputs + 16 outputs (the IceAdjustJAX tuple) = 280 B/point + 8 B/column."""
+    import torch
+
+    from ice3_amd import _lib, synthetic
+    from ice3_amd._fields import ptr
+    f = synthetic.generate(ncol, NLEV, seed=20260002, variant="adjust", dt=DT)
+    g = {k: torch.from_numpy(v).to(dev) for k, v in f.items()}
+    del f
+    out = {k: torch.empty((NLEV, ncol), dtype=torch.float64, device=dev) for k in _lib.ADJ_OUT_FIELDS[:-1]}
+    cin = _lib.AdjustIn(*[ptr(g[k]) for k in _lib.ADJ_IN_FIELDS])
+    cout = _lib.AdjustOut(*[ptr(out[k]) for k in _lib.ADJ_OUT_FIELDS[:-1]], ctypes.c_void_p(None))
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    def call():
+        return lib.ice3_ice_adjust(ctx, _lib.PREC_FP64, ncol, NLEV, ncol, DT, ctypes.byref(cin),
+                                   ctypes.byref(cout), sp)
+    for _ in range(args.warmup):
+        _lib.check(call(), "ice_adjust warmup")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = (280.0 * ncol * NLEV + 8.0 * ncol) / (ms * 1e-3) / 1e9
+    return {"columns_per_s": ncol / (ms * 1e-3), "ms_per_step": ms, "columns": ncol,
+            "kernel": "k_ice_adjust", "bytes_per_point": 280.0, "achieved_GBs": gbs,
+            "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
 
 
 if __name__ == "__main__":

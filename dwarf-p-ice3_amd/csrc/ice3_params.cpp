@@ -289,4 +289,13 @@ void ice3_build_params(ice3_params *P, double *tables) {
   std::memcpy(tables + ICE3_T_SACCRG, XKER_SACCRG_DATA, sizeof(XKER_SACCRG_DATA));
   std::memcpy(tables + ICE3_T_SDRYG, XKER_SDRYG_DATA, sizeof(XKER_SDRYG_DATA));
   std::memcpy(tables + ICE3_T_RDRYG, XKER_RDRYG_DATA, sizeof(XKER_RDRYG_DATA));
+  // logarithms used by the device code's power evaluations
+  P->LN_XLBR = std::log(P->XLBR);
+  P->LN_XLBS = std::log(P->XLBS);
+  P->LN_XLBG = std::log(P->XLBG);
+  P->LN_XLBI = std::log(P->XLBI);
+  P->LN_XLBDAS_MAX = std::log(P->XLBDAS_MAX);
+  P->LN_XTT = std::log(P->XTT);
+  P->LN_1E8 = std::log(1.E8);
+  P->LN_10 = std::log(10.);
 }

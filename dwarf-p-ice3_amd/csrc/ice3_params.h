@@ -27,6 +27,8 @@ struct ice3_params {
       XDRYINTP2R, XDRYINTP1S, XDRYINTP2S, XDRYINTP1G, XDRYINTP2G, XFSEDR, XEXSEDR, XFSEDS,
       XEXSEDS, XFSEDG, XEXSEDG, XFSEDI, XEXCSEDI;
   int NGAMINC, NACCLBDAS, NACCLBDAR, NDRYLBDAR, NDRYLBDAS, NDRYLBDAG;
+  // natural logarithms of constants, for powers evaluated as exp(c*ln x) on the device
+  double LN_XLBR, LN_XLBS, LN_XLBG, LN_XLBI, LN_XLBDAS_MAX, LN_XTT, LN_1E8, LN_10;
 };
 
 // Table block: one contiguous float64 array, offsets in doubles.

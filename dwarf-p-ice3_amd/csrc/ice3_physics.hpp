@@ -240,9 +240,10 @@ template <class R> struct StepState {
 };
 
 // INTERP_MICRO_1D (micro/interp_micro.func.h:34-88) for one point
+// (the argument is ln x: the callers keep the logarithm of each slope parameter)
 template <class R>
-__device__ __forceinline__ R interp1d(R x, int knum, R p1, R p2, const double *__restrict__ lt) {
-  R idx = dmax((R)1.00001, dmin((R)knum - (R)0.00001, p1 * rlog<R>(x) + p2));
+__device__ __forceinline__ R interp1d(R lnx, int knum, R p1, R p2, const double *__restrict__ lt) {
+  R idx = dmax((R)1.00001, dmin((R)knum - (R)0.00001, p1 * lnx + p2));
   int i = (int)idx;
   R f = idx - (R)i;
   return (R)lt[i] * f - (R)lt[i - 1] * (f - (R)1.0);
@@ -251,12 +252,13 @@ __device__ __forceinline__ R interp1d(R x, int knum, R p1, R p2, const double *_
 // INTERP_MICRO_2D (micro/interp_micro.func.h:159-223) for one point: indices + weights
 template <class R> struct Idx2 { int i1, i2; R f1, f2; };
 template <class R>
-__device__ __forceinline__ Idx2<R> interp2d_idx(R x1, R x2, int n1, int n2, R p11, R p12, R p21, R p22) {
+__device__ __forceinline__ Idx2<R> interp2d_idx(R lnx1, R lnx2, int n1, int n2, R p11, R p12, R p21,
+                                                R p22) {
   Idx2<R> r;
-  R a = dmax((R)1.00001, dmin((R)n1 - (R)0.00001, p11 * rlog<R>(x1) + p12));
+  R a = dmax((R)1.00001, dmin((R)n1 - (R)0.00001, p11 * lnx1 + p12));
   r.i1 = (int)a;
   r.f1 = a - (R)r.i1;
-  R b = dmax((R)1.00001, dmin((R)n2 - (R)0.00001, p21 * rlog<R>(x2) + p22));
+  R b = dmax((R)1.00001, dmin((R)n2 - (R)0.00001, p21 * lnx2 + p22));
   r.i2 = (int)b;
   r.f2 = b - (R)r.i2;
   return r;
@@ -278,6 +280,30 @@ __device__ __forceinline__ R interp2d_t(const Idx2<R> &x, int n2, const double *
 #undef T
 }
 
+// Powers of the reference density, constant over the time splitting of one point.
+// x^c is evaluated as exp(c*ln x) throughout the hard tendency call: one logarithm
+// per base (rho, T, each slope parameter) instead of one per power.
+template <class R> struct PointConst {
+  R lnrho;       // ln rhodref
+  R rhomcexvt;   // rho^-XCEXVT
+  R rhom;        // rho^(-XCEXVT-1)
+  R rho03;       // rho^0.3
+};
+
+template <class R>
+__device__ __forceinline__ PointConst<R> point_const(double rhodref) {
+  const ice3_params &P = c_P;
+  PointConst<R> c;
+  c.lnrho = rlog<R>((R)rhodref);
+  c.rhomcexvt = rexp<R>(-(R)P.XCEXVT * c.lnrho);
+  c.rhom = rexp<R>((-(R)P.XCEXVT - (R)1.) * c.lnrho);
+  c.rho03 = rexp<R>((R)0.3 * c.lnrho);
+  return c;
+}
+
+// x^c for x > 0 from ln x
+template <class R> __device__ __forceinline__ R powl_(R lnx, R c) { return rexp<R>(c * lnx); }
+
 // ICE4_TENDENCIES (micro/mode_ice4_tendencies.F90:155-717) for one point that is
 // being integrated (LDCOMPUTE true), with ICE4_COMPUTE_PDF, ICE4_SLOW, ICE4_WARM,
 // ICE4_FAST_RS, ICE4_FAST_RG and ICE4_FAST_RI inlined in call order.
@@ -288,14 +314,14 @@ template <class R>
 __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref, double lvfact,
                                   double lsfact, double pres, double cf, double sigma_rc,
                                   double &cit, double pt, const double *pvart,
-                                  StepState<R> &S, const double *__restrict__ tab, double *pa,
-                                  double *pb) {
+                                  StepState<R> &S, const PointConst<R> &pc,
+                                  const double *__restrict__ tab, double *pa, double *pb) {
   const ice3_params &P = c_P;
   double zt = pt;
   double zv[7];
 #pragma unroll
   for (int jv = 0; jv < 7; ++jv) { zv[jv] = pvart[jv]; pa[jv] = 0.; pb[jv] = 0.; }
-  R ka = 0, dv = 0, ai = 0, cj = 0;
+  R ka = 0, dv = 0, ai = 0, cj = 0, lnt = 0, esi = 0, esw = 0;
   if (!soft) {
     double rvheni, rrhong, rimltc;
     nucleation<R>(true, zv[ITH], pres, rhodref, exn, lsfact, zt, zv[IRV], cit, rvheni);
@@ -332,13 +358,16 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     pb[IRG] = pb[IRG] + rrhong;
     // derived fields (:312-320)
     const R t = (R)zt;
-    const R zzw = rexp<R>((R)P.XALPI - (R)P.XBETAI / t - (R)P.XGAMI * rlog<R>(t));
+    lnt = rlog<R>(t);
+    esi = rexp<R>((R)P.XALPI - (R)P.XBETAI / t - (R)P.XGAMI * lnt);
+    esw = rexp<R>((R)P.XALPW - (R)P.XBETAW / t - (R)P.XGAMW * lnt);
+    const R zzw = esi;
     S.ssi = (R)zv[IRV] * ((R)pres - zzw) / ((R)P.XEPSILO * zzw) - (R)1.0;
     ka = (R)2.38E-2 + (R)0.0071E-2 * (t - (R)P.XTT);
-    dv = (R)0.211E-4 * rpow<R>(t / (R)P.XTT, (R)1.94) * ((R)P.XP00 / (R)pres);
+    dv = (R)0.211E-4 * powl_<R>(lnt - (R)P.LN_XTT, (R)1.94) * ((R)P.XP00 / (R)pres);
     ai = sq((R)P.XLSTT + (R)(P.XCPV - P.XCI) * (t - (R)P.XTT)) / (ka * (R)P.XRV * sq(t)) +
          ((R)P.XRV * t) / (dv * zzw);
-    cj = (R)P.XSCFAC * rpow<R>((R)rhodref, (R)0.3) / sqrt((R)1.718E-5 + (R)0.0049E-5 * (t - (R)P.XTT));
+    cj = (R)P.XSCFAC * pc.rho03 / sqrt((R)1.718E-5 + (R)0.0049E-5 * (t - (R)P.XTT));
   }
   const R t = (R)zt, rho = (R)rhodref;
   const R rv = (R)zv[IRV], rc = (R)zv[IRC], rr = (R)zv[IRR], ri = (R)zv[IRI], rs = (R)zv[IRS],
@@ -346,7 +375,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   const bool cold = zt < P.XTT, warm = zt > P.XTT;
 
   // ICE4_COMPUTE_PDF (micro/mode_ice4_compute_pdf.F90:88-390): rc 'PDF'+'SIGM', ri 'NONE'
-  R hlc_hcf, hlc_hrc, hli_hcf, hli_hri;
+  R hlc_hcf, hlc_hrc, hli_hcf, hli_hri, criauti;
   {
     const double rcrautc = P.XCRIAUTC / rhodref, zrc = zv[IRC];
     if (zrc > rcrautc + sigma_rc) {
@@ -357,7 +386,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     } else {
       hlc_hcf = 0.; hlc_hrc = 0.;
     }
-    const R criauti = dmin((R)P.XCRIAUTI, rpow<R>((R)10., (R)P.XACRIAUTI * (t - (R)P.XTT) + (R)P.XBCRIAUTI));
+    criauti = dmin((R)P.XCRIAUTI, rexp<R>((R)P.LN_10 * ((R)P.XACRIAUTI * (t - (R)P.XTT) + (R)P.XBCRIAUTI)));
     if (ri > criauti) {
       hli_hcf = 1.; hli_hri = ri;
     } else {
@@ -366,13 +395,28 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   }
 
   // slope parameters (:389-433), hard calls only (soft calls reuse rates)
-  R lbdar = 0, lbdas = 0, lbdag = 0;
+  // ln(lbda) is kept beside each lbda: every power and table index below uses it.
+  R lbdar = 0, lbdas = 0, lbdag = 0, llr = 0, lls = 0, llg = 0;
   if (!soft) {
-    if (rr > (R)0.) lbdar = (R)P.XLBR * rpow<R>(rho * dmax(rr, (R)P.XRTMIN[3]), (R)P.XLBEXR);
-    if (rs > (R)0.) lbdas = dmin((R)P.XLBDAS_MAX, (R)P.XLBS * rpow<R>(rho * dmax(rs, (R)P.XRTMIN[5]), (R)P.XLBEXS));
-    if (rg > (R)0.) lbdag = (R)P.XLBG * rpow<R>(rho * dmax(rg, (R)P.XRTMIN[6]), (R)P.XLBEXG);
+    if (rr > (R)0.) {
+      llr = (R)P.LN_XLBR + (R)P.XLBEXR * rlog<R>(rho * dmax(rr, (R)P.XRTMIN[3]));
+      lbdar = rexp<R>(llr);
+    }
+    if (rs > (R)0.) {
+      lls = (R)P.LN_XLBS + (R)P.XLBEXS * rlog<R>(rho * dmax(rs, (R)P.XRTMIN[5]));
+      if (lls < (R)P.LN_XLBDAS_MAX) {
+        lbdas = rexp<R>(lls);
+      } else {
+        lbdas = (R)P.XLBDAS_MAX;
+        lls = (R)P.LN_XLBDAS_MAX;
+      }
+    }
+    if (rg > (R)0.) {
+      llg = (R)P.LN_XLBG + (R)P.XLBEXG * rlog<R>(rho * dmax(rg, (R)P.XRTMIN[6]));
+      lbdag = rexp<R>(llg);
+    }
   }
-  const R rhomcexvt = soft ? (R)0 : rpow<R>(rho, -(R)P.XCEXVT);
+  const R rhomcexvt = soft ? (R)0 : pc.rhomcexvt;
   const R ssi = S.ssi;
   const bool rv_ok = zv[IRV] > P.XRTMIN[1], rc_ok = zv[IRC] > P.XRTMIN[2],
              rr_ok = zv[IRR] > P.XRTMIN[3], ri_ok = zv[IRI] > P.XRTMIN[4],
@@ -383,19 +427,18 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     if (!soft) S.rchoni = dmin((R)1000., (R)P.XHON * rho * rc * rexp<R>((R)P.XALPHA3 * (t - (R)P.XTT) - (R)P.XBETA3));
   } else S.rchoni = 0;
   if (rv_ok && rs_ok) {
-    if (!soft) S.rvdeps = (ssi / (rho * ai)) * ((R)P.X0DEPS * rpow<R>(lbdas, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * rpow<R>(lbdas, (R)P.XEX1DEPS));
+    if (!soft) S.rvdeps = (ssi / (rho * ai)) * ((R)P.X0DEPS * powl_<R>(lls, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * powl_<R>(lls, (R)P.XEX1DEPS));
   } else S.rvdeps = 0;
   if (ri_ok && rs_ok) {
-    if (!soft) S.riaggs = (R)P.XFIAGGS * rexp<R>((R)P.XCOLEXIS * (t - (R)P.XTT)) * ri * rpow<R>(lbdas, (R)P.XEXIAGGS) * rhomcexvt;
+    if (!soft) S.riaggs = (R)P.XFIAGGS * rexp<R>((R)P.XCOLEXIS * (t - (R)P.XTT)) * ri * powl_<R>(lls, (R)P.XEXIAGGS) * rhomcexvt;
   } else S.riaggs = 0;
   if (hli_hri > (R)P.XRTMIN[4]) {
     if (!soft) {
-      const R criauti = dmin((R)P.XCRIAUTI, rpow<R>((R)10., (R)P.XACRIAUTI * (t - (R)P.XTT) + (R)P.XBCRIAUTI));
       S.riauts = (R)P.XTIMAUTI * rexp<R>((R)P.XTEXAUTI * (t - (R)P.XTT)) * dmax(hli_hri - criauti * hli_hcf, (R)0.);
     }
   } else S.riauts = 0;
   if (rv_ok && rg_ok) {
-    if (!soft) S.rvdepg = (ssi / (rho * ai)) * ((R)P.X0DEPG * rpow<R>(lbdag, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * rpow<R>(lbdag, (R)P.XEX1DEPG));
+    if (!soft) S.rvdepg = (ssi / (rho * ai)) * ((R)P.X0DEPG * powl_<R>(llg, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * powl_<R>(llg, (R)P.XEX1DEPG));
   } else S.rvdepg = 0;
 
   // ICE4_WARM (micro/mode_ice4_warm.F90:100-197)
@@ -403,14 +446,14 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     if (!soft) S.rcautr = (R)P.XTIMAUTC * dmax(hlc_hrc - hlc_hcf * (R)P.XCRIAUTC / rho, (R)0.0);
   } else S.rcautr = 0;
   if (rc_ok && rr_ok) {
-    if (!soft) S.rcaccr = (R)P.XFCACCR * rc * (R)1.0 * rpow<R>(lbdar, (R)P.XEXCACCR) * rhomcexvt;
+    if (!soft) S.rcaccr = (R)P.XFCACCR * rc * (R)1.0 * powl_<R>(llr, (R)P.XEXCACCR) * rhomcexvt;
   } else S.rcaccr = 0;
   if (rr_ok && !rc_ok) {
     if (!soft) {
-      const R ev = rexp<R>((R)P.XALPW - (R)P.XBETAW / t - (R)P.XGAMW * rlog<R>(t));
+      const R ev = esw;
       const R usw = (R)1. - rv * ((R)pres - ev) / ((R)P.XEPSILO * ev);
       const R x = sq((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) / (ka * (R)P.XRV * sq(t)) + ((R)P.XRV * t) / (dv * ev);
-      S.rrevav = (dmax((R)0., usw) / (rho * x)) * ((R)P.X0EVAR * rpow<R>(lbdar, (R)P.XEX0EVAR) + (R)P.X1EVAR * cj * rpow<R>(lbdar, (R)P.XEX1EVAR));
+      S.rrevav = (dmax((R)0., usw) / (rho * x)) * ((R)P.X0EVAR * powl_<R>(llr, (R)P.XEX0EVAR) + (R)P.X1EVAR * cj * powl_<R>(llr, (R)P.XEX1EVAR));
     }
   } else S.rrevav = 0;
 
@@ -419,9 +462,9 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   if (rs_ok) {
     if (!soft) {
       R f1 = rv * (R)pres / ((R)P.XEPSILO + rv);
-      f1 = dmin(f1, rexp<R>((R)P.XALPI - (R)P.XBETAI / t - (R)P.XGAMI * rlog<R>(t)));
+      f1 = dmin(f1, esi);
       f1 = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - f1) / ((R)P.XRV * t));
-      f1 = f1 * ((R)P.X0DEPS * rpow<R>(lbdas, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * rpow<R>(lbdas, (R)P.XEX1DEPS)) /
+      f1 = f1 * ((R)P.X0DEPS * powl_<R>(lls, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * powl_<R>(lls, (R)P.XEX1DEPS)) /
            (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
       S.rs_freez1 = f1;
       S.rs_freez2 = (rho * ((R)P.XLMTT + (R)(P.XCI - P.XCL) * ((R)P.XTT - t))) / (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
@@ -433,15 +476,15 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   const bool grim = rc_ok && rs_ok;
   if (!grim) { S.rs_rcrims = 0; S.rs_rcrimss = 0; S.rs_rsrimcg = 0; }
   if (!soft && grim) {
-    const R zzw1 = interp1d<R>(lbdas, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM1);
-    const R zzw2 = interp1d<R>(lbdas, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM2);
-    const R zzw3 = interp1d<R>(lbdas, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM4);
-    const R lexc = rpow<R>(lbdas, (R)P.XEXCRIMSS);
+    const R zzw1 = interp1d<R>(lls, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM1);
+    const R zzw2 = interp1d<R>(lls, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM2);
+    const R zzw3 = interp1d<R>(lls, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM4);
+    const R lexc = powl_<R>(lls, (R)P.XEXCRIMSS);
     S.rs_rcrimss = (R)P.XCRIMSS * zzw1 * rc * (R)1.0 * lexc * rhomcexvt;
     S.rs_rcrims = (R)P.XCRIMSG * rc * (R)1.0 * lexc * rhomcexvt;
     const R zzw = S.rs_rcrims - S.rs_rcrimss;
-    const R x = (R)P.XSRIMCG * rpow<R>(lbdas, (R)P.XEXSRIMCG) * ((R)1.0 - zzw2);
-    S.rs_rsrimcg = zzw * x / dmax((R)1.E-20, (R)P.XSRIMCG3 * (R)P.XSRIMCG2 * rpow<R>(lbdas, (R)P.XEXSRIMCG2) * ((R)1. - zzw3) - (R)P.XSRIMCG3 * x);
+    const R x = (R)P.XSRIMCG * powl_<R>(lls, (R)P.XEXSRIMCG) * ((R)1.0 - zzw2);
+    S.rs_rsrimcg = zzw * x / dmax((R)1.E-20, (R)P.XSRIMCG3 * (R)P.XSRIMCG2 * powl_<R>(lls, (R)P.XEXSRIMCG2) * ((R)1. - zzw3) - (R)P.XSRIMCG3 * x);
   }
   R rcrimss = 0, rcrimsg = 0, rsrimcg = 0;
   if (grim && cold) {
@@ -459,18 +502,18 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   if (!soft) {
     S.rs_rraccs = 0; S.rs_rraccss = 0; S.rs_rsaccrg = 0;
     if (gacc) {
-      const Idx2<R> ix = interp2d_idx<R>(lbdas, lbdar, P.NACCLBDAS, P.NACCLBDAR, (R)P.XACCINTP1S,
+      const Idx2<R> ix = interp2d_idx<R>(lls, llr, P.NACCLBDAS, P.NACCLBDAR, (R)P.XACCINTP1S,
                                          (R)P.XACCINTP2S, (R)P.XACCINTP1R, (R)P.XACCINTP2R);
       const R zzw1 = interp2d<R>(ix, P.NACCLBDAS, tab + ICE3_T_RACCSS);
       const R zzw2 = interp2d<R>(ix, P.NACCLBDAS, tab + ICE3_T_RACCS);
       const R zzw3 = interp2d_t<R>(ix, P.NACCLBDAR, tab + ICE3_T_SACCRG);
-      const R rhom = rpow<R>(rho, -(R)P.XCEXVT - (R)1.);
+      const R rhom = pc.rhom;
       const R lr2 = lbdar * lbdar;
-      const R zzw = (R)P.XFRACCSS * (rpow<R>(lbdas, (R)P.XCXS)) * rhom *
+      const R zzw = (R)P.XFRACCSS * (powl_<R>(lls, (R)P.XCXS)) * rhom *
                     ((R)P.XLBRACCS1 / (sq(lbdas)) + (R)P.XLBRACCS2 / (lbdas * lbdar) + (R)P.XLBRACCS3 / (lr2)) / (lr2 * lr2);
       S.rs_rraccss = zzw1 * zzw;
       S.rs_rraccs = zzw2 * zzw;
-      S.rs_rsaccrg = (R)P.XFSACCRG * zzw3 * (rpow<R>(lbdas, (R)(P.XCXS - P.XBS))) * rhom *
+      S.rs_rsaccrg = (R)P.XFSACCRG * zzw3 * (powl_<R>(lls, (R)(P.XCXS - P.XBS))) * rhom *
                      ((R)P.XLBSACCR1 / (lr2) + (R)P.XLBSACCR2 / (lbdar * lbdas) + (R)P.XLBSACCR3 / (sq(lbdas))) / lbdar;
     }
   }
@@ -488,9 +531,9 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   if (rs_ok && warm) {
     if (!soft) {
       R m = rv * (R)pres / ((R)P.XEPSILO + rv);
-      m = dmin(m, rexp<R>((R)P.XALPW - (R)P.XBETAW / t - (R)P.XGAMW * rlog<R>(t)));
+      m = dmin(m, esw);
       m = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - m) / ((R)P.XRV * t));
-      S.rsmltg = (R)P.XFSCVMG * dmax((R)0., (-m * ((R)P.X0DEPS * rpow<R>(lbdas, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * rpow<R>(lbdas, (R)P.XEX1DEPS)) -
+      S.rsmltg = (R)P.XFSCVMG * dmax((R)0., (-m * ((R)P.X0DEPS * powl_<R>(lls, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * powl_<R>(lls, (R)P.XEX1DEPS)) -
                                               (S.rs_rcrims + S.rs_rraccs) * (rho * (R)P.XCL * ((R)P.XTT - t))) / (rho * (R)P.XLMTT));
       S.rcmltsr = S.rs_rcrims;
     }
@@ -501,8 +544,8 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   // ICE4_FAST_RG (micro/mode_ice4_fast_rg.F90:121-410)
   if (ri_ok && rr_ok) {
     if (!soft) {
-      R icf = (R)P.XICFRR * ri * rpow<R>(lbdar, (R)P.XEXICFRR) * rhomcexvt;
-      R rcf = (R)P.XRCFRI * (R)cit * rpow<R>(lbdar, (R)P.XEXRCFRI) * rpow<R>(rho, -(R)P.XCEXVT - (R)1.);
+      R icf = (R)P.XICFRR * ri * powl_<R>(llr, (R)P.XEXICFRR) * rhomcexvt;
+      R rcf = (R)P.XRCFRI * (R)cit * powl_<R>(llr, (R)P.XEXRCFRI) * pc.rhom;
       const R zzw0d = dmax((R)0., dmin((R)1., (icf * (R)P.XCI + rcf * (R)P.XCL) * ((R)P.XTT - t) / dmax((R)1.E-20, (R)P.XLVTT * rcf)));
       S.rrcfrig = zzw0d * rcf;
       S.ricfrr = ((R)1. - zzw0d) * icf;
@@ -511,7 +554,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   } else {
     S.ricfrrg = 0; S.rrcfrig = 0; S.ricfrr = 0;
   }
-  const R lgx = (!soft && rg_ok) ? rpow<R>(lbdag, (R)(P.XCXG - P.XDG - 2.)) * rhomcexvt : (R)0;
+  const R lgx = (!soft && rg_ok) ? powl_<R>(llg, (R)(P.XCXG - P.XDG - 2.)) * rhomcexvt : (R)0;
   if (rg_ok && rc_ok) {
     if (!soft) S.rg_rcdryg = (R)P.XFCDRYG * rc * lgx;
   } else S.rg_rcdryg = 0;
@@ -524,25 +567,25 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   } else {
     S.rg_ridryg = 0; S.rg_riwetg = 0;
   }
-  const R rhom1 = (!soft && rg_ok) ? rpow<R>(rho, -(R)P.XCEXVT - (R)1.) : (R)0;
+  const R rhom1 = (!soft && rg_ok) ? pc.rhom : (R)0;
   const bool gdrys = rs_ok && rg_ok;
   if (!gdrys) { S.rg_rsdryg = 0; S.rg_rswetg = 0; }
   if (!soft && gdrys) {
-    const Idx2<R> ix = interp2d_idx<R>(lbdag, lbdas, P.NDRYLBDAG, P.NDRYLBDAS, (R)P.XDRYINTP1G,
+    const Idx2<R> ix = interp2d_idx<R>(llg, lls, P.NDRYLBDAG, P.NDRYLBDAS, (R)P.XDRYINTP1G,
                                        (R)P.XDRYINTP2G, (R)P.XDRYINTP1S, (R)P.XDRYINTP2S);
     const R zzw = interp2d<R>(ix, P.NDRYLBDAG, tab + ICE3_T_SDRYG);
-    S.rg_rswetg = (R)P.XFSDRYG * zzw / (R)P.XCOLSG * (rpow<R>(lbdas, (R)(P.XCXS - P.XBS))) * (rpow<R>(lbdag, (R)P.XCXG)) * rhom1 *
+    S.rg_rswetg = (R)P.XFSDRYG * zzw / (R)P.XCOLSG * (powl_<R>(lls, (R)(P.XCXS - P.XBS))) * (powl_<R>(llg, (R)P.XCXG)) * rhom1 *
                   ((R)P.XLBSDRYG1 / (sq(lbdag)) + (R)P.XLBSDRYG2 / (lbdag * lbdas) + (R)P.XLBSDRYG3 / (sq(lbdas)));
     S.rg_rsdryg = S.rg_rswetg * (R)P.XCOLSG * rexp<R>((R)P.XCOLEXSG * (t - (R)P.XTT));
   }
   const bool gdryr = rr_ok && rg_ok;
   if (!gdryr) S.rg_rrdryg = 0;
   if (!soft && gdryr) {
-    const Idx2<R> ix = interp2d_idx<R>(lbdag, lbdar, P.NDRYLBDAG, P.NDRYLBDAR, (R)P.XDRYINTP1G,
+    const Idx2<R> ix = interp2d_idx<R>(llg, llr, P.NDRYLBDAG, P.NDRYLBDAR, (R)P.XDRYINTP1G,
                                        (R)P.XDRYINTP2G, (R)P.XDRYINTP1R, (R)P.XDRYINTP2R);
     const R zzw = interp2d<R>(ix, P.NDRYLBDAG, tab + ICE3_T_RDRYG);
     const R lr2 = lbdar * lbdar;
-    S.rg_rrdryg = (R)P.XFRDRYG * zzw * ((R)1.0 / (lr2 * lr2)) * (rpow<R>(lbdag, (R)P.XCXG)) * rhom1 *
+    S.rg_rrdryg = (R)P.XFRDRYG * zzw * ((R)1.0 / (lr2 * lr2)) * (powl_<R>(llg, (R)P.XCXG)) * rhom1 *
                   ((R)P.XLBRDRYG1 / (sq(lbdag)) + (R)P.XLBRDRYG2 / (lbdag * lbdar) + (R)P.XLBRDRYG3 / (lr2));
   }
   const R rdryg_init = S.rg_rcdryg + S.rg_ridryg + S.rg_rsdryg + S.rg_rrdryg;
@@ -551,9 +594,9 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   if (rg_ok) {
     if (!soft) {
       R f1 = rv * (R)pres / ((R)P.XEPSILO + rv);
-      f1 = dmin(f1, rexp<R>((R)P.XALPI - (R)P.XBETAI / t - (R)P.XGAMI * rlog<R>(t)));
+      f1 = dmin(f1, esi);
       f1 = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - f1) / ((R)P.XRV * t));
-      f1 = f1 * ((R)P.X0DEPG * rpow<R>(lbdag, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * rpow<R>(lbdag, (R)P.XEX1DEPG)) /
+      f1 = f1 * ((R)P.X0DEPG * powl_<R>(llg, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * powl_<R>(llg, (R)P.XEX1DEPG)) /
            (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
       S.rg_freez1 = f1;
       S.rg_freez2 = (rho * ((R)P.XLMTT + (R)(P.XCI - P.XCL) * ((R)P.XTT - t))) / (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
@@ -577,9 +620,9 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   if (rg_ok && warm) {
     if (!soft) {
       R m = rv * (R)pres / ((R)P.XEPSILO + rv);
-      m = dmin(m, rexp<R>((R)P.XALPW - (R)P.XBETAW / t - (R)P.XGAMW * rlog<R>(t)));
+      m = dmin(m, esw);
       m = ka * ((R)P.XTT - t) + dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - m) / ((R)P.XRV * t);
-      S.rgmltr = dmax((R)0., (-m * ((R)P.X0DEPG * rpow<R>(lbdag, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * rpow<R>(lbdag, (R)P.XEX1DEPG)) -
+      S.rgmltr = dmax((R)0., (-m * ((R)P.X0DEPG * powl_<R>(llg, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * powl_<R>(llg, (R)P.XEX1DEPG)) -
                               (S.rg_rcdryg + S.rg_rrdryg) * (rho * (R)P.XCL * ((R)P.XTT - t))) / (rho * (R)P.XLMTT));
     }
   } else S.rgmltr = 0;
@@ -587,8 +630,15 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   // ICE4_FAST_RI (micro/mode_ice4_fast_ri.F90:70-81)
   if (ssi > (R)0. && rc_ok && ri_ok && cit > 1.E-20) {
     if (!soft) {
-      const R l = dmin((R)1.E8, (R)P.XLBI * rpow<R>(rho * ri / (R)cit, (R)P.XLBEXI));
-      S.rcberi = (ssi / (rho * ai)) * (R)cit * ((R)P.X0DEPI / l + (R)P.X2DEPI * cj * cj / rpow<R>(l, (R)(P.XDI + 2.0)));
+      R lli = (R)P.LN_XLBI + (R)P.XLBEXI * rlog<R>(rho * ri / (R)cit);
+      R l;
+      if (lli < (R)P.LN_1E8) {
+        l = rexp<R>(lli);
+      } else {
+        l = (R)1.E8;
+        lli = (R)P.LN_1E8;
+      }
+      S.rcberi = (ssi / (rho * ai)) * (R)cit * ((R)P.X0DEPI / l + (R)P.X2DEPI * cj * cj / powl_<R>(lli, (R)(P.XDI + 2.0)));
     }
   } else S.rcberi = 0;
 
@@ -682,6 +732,7 @@ __device__ __forceinline__ int stepping_point(double dt, double exn, double rhod
                                      const double *__restrict__ tab) {
   const ice3_params &P = c_P;
   StepState<R> S;
+  const PointConst<R> pc = point_const<R>(rhodref);
   const double zinv = 1. / dt;
 #pragma unroll
   for (int jv = 0; jv < 7; ++jv) extpk[jv] = extpk[jv] - vart[jv] * zinv;
@@ -698,7 +749,7 @@ __device__ __forceinline__ int stepping_point(double dt, double exn, double rhod
       const double zzt = vart[ITH] * exn;
       const double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
       const double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
-      tendencies<R>(soft, exn, rhodref, lvfact, lsfact, pres, cf, sigma_rc, cit, zzt, vart, S,
+      tendencies<R>(soft, exn, rhodref, lvfact, lsfact, pres, cf, sigma_rc, cit, zzt, vart, S, pc,
                     tab, za, zb);
 #pragma unroll
       for (int jv = 0; jv < 7; ++jv) za[jv] = za[jv] + extpk[jv];
@@ -731,18 +782,21 @@ __device__ __forceinline__ int stepping_point(double dt, double exn, double rhod
 
 // Fall speed of one species at one level, OTHER_SPECIES / PRISTINE_ICE
 // (micro/mode_ice4_sedimentation_stat.F90:295-431); jrr = 3 (r), 4 (i), 5 (s), 6 (g).
+// lnrho = ln(rhodref) of the level; the powers are evaluated as exp(c*ln x).
 template <class R>
-__device__ __forceinline__ R fall_speed(int jrr, double rx, double rho) {
+__device__ __forceinline__ R fall_speed(int jrr, double rx, R lnrho) {
   const ice3_params &P = c_P;
   if (jrr == 4) {
     if (!(rx > dmax(P.XRTMIN[4], 1.0E-7))) return (R)0;
-    return (R)P.XFSEDI * rpow<R>((R)rho, -(R)P.XCEXVT) *
-           rpow<R>(dmax((R)0.05E6, (R)-0.15319E6 - (R)0.021454E6 * rlog<R>((R)(rho * rx))), (R)P.XEXCSEDI);
+    const R lnrx = rlog<R>((R)rx);
+    return (R)P.XFSEDI * rexp<R>(-(R)P.XCEXVT * lnrho) *
+           rexp<R>((R)P.XEXCSEDI *
+                   rlog<R>(dmax((R)0.05E6, (R)-0.15319E6 - (R)0.021454E6 * (lnrho + lnrx))));
   }
   if (!(rx > P.XRTMIN[jrr])) return (R)0;
   const double fsed = jrr == 3 ? P.XFSEDR : (jrr == 5 ? P.XFSEDS : P.XFSEDG);
   const double exsed = jrr == 3 ? P.XEXSEDR : (jrr == 5 ? P.XEXSEDS : P.XEXSEDG);
-  return (R)fsed * rpow<R>((R)rx, (R)(exsed - 1)) * rpow<R>((R)rho, (R)(exsed - P.XCEXVT - 1));
+  return (R)fsed * rexp<R>((R)(exsed - 1) * rlog<R>((R)rx) + (R)(exsed - P.XCEXVT - 1) * lnrho);
 }
 
 // One species at one level of ICE4_SEDIMENTATION_STAT (:143-229) for one column:
@@ -751,7 +805,7 @@ __device__ __forceinline__ R fall_speed(int jrr, double rx, double rho) {
 // jrr = 3 (r), 4 (i), 5 (s), 6 (g); the species are independent, so the
 // reference's species-outer loop order and this level-outer order agree.
 template <class R>
-__device__ __forceinline__ void sediment_species(int jrr, double dt, double rho, double dz,
+__device__ __forceinline__ void sediment_species(int jrr, double dt, double rho, R lnrho, double dz,
                                                  double &rs, double &flux) {
   const ice3_params &P = c_P;
   const double zinv = 1. / dt;
@@ -761,8 +815,8 @@ __device__ __forceinline__ void sediment_species(int jrr, double dt, double rho,
   const double thr = P.XRTMIN[jrr];
   double cur;
   if (rx > thr || qp > thr) {
-    const double w1 = (double)fall_speed<R>(jrr, rx, rho);
-    const double w2 = (double)fall_speed<R>(jrr, qp, rho);
+    const double w1 = (double)fall_speed<R>(jrr, rx, lnrho);
+    const double w2 = (double)fall_speed<R>(jrr, qp, lnrho);
     const double f1 = dmin(rho * dz * rx * zinv, w1 * rho * rx);
     cur = w2 != 0. ? f1 + dmax(0., 1. - dz / (dt * w2)) * flux : f1;
   } else {

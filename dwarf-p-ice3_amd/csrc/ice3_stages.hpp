@@ -75,11 +75,12 @@ __device__ __forceinline__ void sedim_column(const SedimArgs &a, int i) {
   for (int k = a.nlev - 1; k >= 0; --k) {
     const size_t j = (size_t)k * a.ld + i;
     const double rho = a.rhodref[j], dz = a.dzz[j];
+    const R lnrho = rlog<R>((R)rho);
     double rrs = a.src_rr[j], ris = a.src_ri[j], rss = a.src_rs[j], rgs = a.src_rg[j];
-    sediment_species<R>(IRR, dt, rho, dz, rrs, fr);
-    sediment_species<R>(IRI, dt, rho, dz, ris, fi);
-    sediment_species<R>(IRS, dt, rho, dz, rss, fs);
-    sediment_species<R>(IRG, dt, rho, dz, rgs, fg);
+    sediment_species<R>(IRR, dt, rho, lnrho, dz, rrs, fr);
+    sediment_species<R>(IRI, dt, rho, lnrho, dz, ris, fi);
+    sediment_species<R>(IRS, dt, rho, lnrho, dz, rss, fs);
+    sediment_species<R>(IRG, dt, rho, lnrho, dz, rgs, fg);
     a.dst_rr[j] = rrs; a.dst_ri[j] = ris; a.dst_rs[j] = rss; a.dst_rg[j] = rgs;
   }
   a.inprc[i] = 0.;

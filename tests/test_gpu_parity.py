@@ -5,13 +5,14 @@ FP64 bar: relative error <= 1e-11 with an absolute floor of 1e-18 kg/kg on mixin
 ratios (1e-18/dt kg/kg/s on their sources).  Points whose ICE4 time-splitting
 trajectory differs from the reference's (a threshold decided the other way by a
 last-bit difference: different number of tendency calls) are counted as branch
-flips, reported, and excluded from the 1e-11 statistic; they must stay rare."""
+flips and reported.  A point that misses the bar must be such a flip, and fewer
+than 0.2 % of the points may miss it (helpers.check_with_flips)."""
 import os
 
 import numpy as np
 import pytest
 
-from helpers import floor_for, rel_err
+from helpers import check_with_flips, floor_for, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -113,9 +114,9 @@ def test_rain_ice_golden(dev, golden_dir, tag):
     out = _rain_gpu(s, float(g["dt"]))
     iters = oracle_lib.rain_ice(s, float(g["dt"]), with_iters=True)["iters"]
     flips = out["calls"] != iters
-    print(f"rain_ice golden {tag}: {int(flips.sum())} branch flips of {flips.size} points")
-    assert flips.mean() < 2e-3
-    _check(out, ref, RAIN_OUT, flips)
+    w, nf, nb = check_with_flips(out, ref, RAIN_OUT, flips, TOL)
+    print(f"rain_ice golden {tag}: {nf} branch flips, {nb} divergent points of {flips.size}, "
+          f"worst {max(w.values()):.2e}")
 
 
 def test_pipeline_vs_oracle(dev):
@@ -129,9 +130,8 @@ def test_pipeline_vs_oracle(dev):
     s = ref_lib.pipeline_state(f, a)
     ref = oracle_lib.rain_ice(s, with_iters=True)
     flips = out["calls"] != ref["iters"]
-    print(f"pipeline: {int(flips.sum())} branch flips of {flips.size} points")
-    assert flips.mean() < 2e-3
-    _check(out, ref, RAIN_OUT, flips)
+    w, nf, nb = check_with_flips(out, ref, RAIN_OUT, flips, TOL)
+    print(f"pipeline: {nf} branch flips, {nb} divergent points of {flips.size}, worst {max(w.values()):.2e}")
     _check(out, a, ("cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf"))
 
 
