@@ -11,7 +11,11 @@
 //                  statistical sedimentation flux recurrence, whose fluxes
 //                  never leave registers, and the surface precipitation;
 //   k_micro<R>     one thread per (level, column) point: the ICE4 time-split
-//                  microphysics, total tendencies and negativity correction.
+//                  microphysics, total tendencies and negativity correction;
+//   k_surrogate    the learned surrogate of k_micro: a persistent grid, MLP weights
+//                  resident in LDS, 128-point tiles on the bf16 matrix cores; points it
+//                  rejects are queued for k_micro_list, the plain FP64 microphysics over
+//                  a point list.
 // rain_ice is k_sedim then k_micro on one stream; the fused time step is
 // k_ice_adjust, k_sedim, k_micro.  R is the arithmetic of the process rates
 // and lookups (double, or float for the mixed path).
@@ -26,6 +30,7 @@
 #include "ice3_params.h"
 #include "ice3_physics.hpp"
 #include "ice3_stages.hpp"
+#include "ice3_mlp.hpp"
 
 namespace ice3 {
 
@@ -67,6 +72,182 @@ __global__ void __launch_bounds__(256) k_micro(MicroArgs a) {
       atomicAdd(&a.counters[2], (unsigned long long)p);
     }
   }
+}
+
+// Plain microphysics over a list of points (the surrogate's fallback queue):
+// queue[0] = count, queue[1 + q] = point offset.  Grid-stride loop: every wave exits.
+template <class R>
+__global__ void __launch_bounds__(64) k_micro_list(MicroArgs a, const unsigned int *queue) {
+  const unsigned int n = queue[0];
+  for (unsigned int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    micro_at<R>(a, (size_t)queue[1 + q]);
+}
+
+__global__ void k_zero_u32(unsigned int *p) { *p = 0u; }
+
+struct SurrogateArgs {
+  MicroArgs m;
+  unsigned int npts;                 // nlev * ncol; point q = k * ncol + i
+  const bf16x8 *wpk;
+  const float *params;
+  const ice3_surrogate_norm *norm;
+  float budget_tol;
+  unsigned int *queue;               // [0] count, [1 + q] point offsets
+  unsigned long long *counters;      // 5 entries, see include/ice3.h
+};
+
+// Reasons a point is not taken from the network.
+enum { SG_OK = 0, SG_RANGE = 1, SG_NEG = 2, SG_BUDGET = 3, SG_NOMICRO = 4, SG_NONE = 5 };
+
+__global__ void __launch_bounds__(256) k_surrogate(SurrogateArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  bf16x8 *sW = (bf16x8 *)smem;
+  float *sB = (float *)(smem + MLP_NW * 2);
+  __bf16 *sH = (__bf16 *)(smem + MLP_NW * 2 + MLP_NB * 4);
+  __bf16 *sX = sH + MLP_TILE * MLP_HS;
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64, row0 = w * MLP_ROWS;
+  const MicroArgs &m = a.m;
+  const ice3_surrogate_norm &nm = *a.norm;
+  block_copy16(sW, a.wpk, MLP_NW * 2);
+  for (int q = threadIdx.x; q < MLP_NB; q += blockDim.x) sB[q] = a.params[MLP_NW + q];
+  __syncthreads();
+  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+  const unsigned int ntiles = (a.npts + MLP_TILE - 1) / MLP_TILE;
+  for (unsigned int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // features of this wave's rows: lane r < 32 handles row row0 + r
+    int flag = SG_NONE;
+    size_t j = 0;
+    if (lane < MLP_ROWS) {
+      const unsigned int q = tile * MLP_TILE + row0 + lane;
+      __bf16 *xr = sX + (row0 + lane) * MLP_XS;
+      float f[MLP_NFEAT];
+      if (q < a.npts) {
+        j = (size_t)(q / m.ncol) * m.ld + q % m.ncol;
+        if (!surrogate_features(m, j, f)) {
+          flag = SG_NOMICRO;
+        } else {
+          flag = SG_OK;
+#pragma unroll
+          for (int c = 0; c < MLP_NFEAT; ++c)
+            if (!(f[c] >= nm.feat_lo[c] && f[c] <= nm.feat_hi[c])) flag = SG_RANGE;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < MLP_IN; ++c)
+        xr[c] = (__bf16)(flag == SG_OK && c < MLP_NFEAT ? (f[c] - nm.feat_mean[c]) * nm.feat_scale[c] : 0.f);
+    }
+    // the network on the wave's 32 rows (each wave reads and writes only its own rows)
+    {
+      f32x4 acc[MLP_MT][MLP_H / 16];
+      mlp_layer<MLP_IN, MLP_H, MLP_XS>(sX, row0, sW + mlp_woff(0) / 8, acc, lane);
+#pragma unroll
+      for (int mt = 0; mt < MLP_MT; ++mt)
+#pragma unroll
+        for (int nn = 0; nn < MLP_H / 16; ++nn)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = row0 + mt * 16 + 4 * (lane / 16) + i, col = nn * 16 + lane % 16;
+            sH[row * MLP_HS + col] = (__bf16)silu(acc[mt][nn][i] + sB[col]);
+          }
+    }
+#pragma unroll 1
+    for (int l = 1; l <= 3; ++l) {
+      f32x4 acc[MLP_MT][MLP_H / 16];
+      mlp_layer<MLP_H, MLP_H, MLP_HS>(sH, row0, sW + mlp_woff(l) / 8, acc, lane);
+#pragma unroll
+      for (int mt = 0; mt < MLP_MT; ++mt)
+#pragma unroll
+        for (int nn = 0; nn < MLP_H / 16; ++nn)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = row0 + mt * 16 + 4 * (lane / 16) + i, col = nn * 16 + lane % 16;
+            sH[row * MLP_HS + col] = (__bf16)silu(acc[mt][nn][i] + sB[l * MLP_H + col]);
+          }
+    }
+    // output layer; the outputs (fp32, 9-float rows) overwrite this wave's rows of sH
+    float *sY = (float *)(sH + row0 * MLP_HS);
+    {
+      f32x4 acc[MLP_MT][MLP_OUT / 16];
+      mlp_layer<MLP_H, MLP_OUT, MLP_HS>(sH, row0, sW + mlp_woff(4) / 8, acc, lane);
+#pragma unroll
+      for (int mt = 0; mt < MLP_MT; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = mt * 16 + 4 * (lane / 16) + i, col = lane % 16;
+          if (col < MLP_NTARG) sY[r * 9 + col] = acc[mt][0][i] + sB[4 * MLP_H + col];
+        }
+    }
+    // decode, physical checks, results or fallback queue
+    double out[7], cinew = 0.;
+    if (flag == SG_OK) {
+      double rsum = 0., rmag = 0.;
+#pragma unroll
+      for (int v = 0; v < 7; ++v) {
+        const float u = nm.targ_mean[v] + nm.targ_std[v] * sY[lane * 9 + v];
+        const double y = (double)(nm.targ_unit[v] * sinhf(u));
+        out[v] = m.src[v][j] + y;
+        if (v >= 1) { rsum += y; rmag += fabs(y); }
+      }
+      {
+        const float u = nm.targ_mean[7] + nm.targ_std[7] * sY[lane * 9 + 7];
+        cinew = fmax(0., m.src_ci[j] + (double)(nm.targ_unit[7] * sinhf(u)));
+      }
+      if (fabs(rsum) > (double)a.budget_tol * rmag + 1e-18) {
+        flag = SG_BUDGET;
+      } else {
+        out[IRV] -= rsum;  // move the water residual into vapour
+#pragma unroll
+        for (int v = 1; v < 7; ++v)
+          if (!(out[v] >= 0.)) flag = SG_NEG;
+      }
+    }
+    if (flag == SG_OK) {
+#pragma unroll
+      for (int v = 0; v < 7; ++v) m.dst[v][j] = out[v];
+      m.ci[j] = cinew;
+      m.evap3d[j] = 0.;
+      m.rainfr[j] = 1.;
+      if (m.calls) m.calls[j] = 0;
+    }
+    if (flag < SG_NONE) ++cnt[flag];
+    // queue the rejected points (one atomic per wave)
+    const bool rej = lane < MLP_ROWS && flag != SG_OK && flag != SG_NONE;
+    const unsigned long long mask = __ballot(rej);
+    if (mask) {
+      unsigned int base = 0;
+      if (lane == 0) base = atomicAdd(&a.queue[0], (unsigned int)__popcll(mask));
+      base = __shfl(base, 0, 64);
+      if (rej) a.queue[1 + base + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned int)j;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    unsigned long long v = cnt[c];
+#pragma unroll
+    for (int mm = 32; mm >= 1; mm >>= 1) v += __shfl_xor(v, mm, 64);
+    if (lane == 0 && v) atomicAdd(&a.counters[c], v);
+  }
+}
+
+constexpr size_t SG_LDS = (size_t)MLP_NW * 2 + MLP_NB * 4 + MLP_TILE * MLP_HS * 2 +
+                          MLP_TILE * MLP_XS * 2;
+
+// Surrogate features of the points of MicroArgs (training data and tests).
+__global__ void __launch_bounds__(256) k_features(MicroArgs m, unsigned int npts,
+                                                  const ice3_surrogate_norm *norm, float *raw,
+                                                  __bf16 *x, unsigned char *micro) {
+  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npts) return;
+  const size_t j = (size_t)(q / m.ncol) * m.ld + q % m.ncol;
+  float f[MLP_NFEAT];
+  const bool mi = surrogate_features(m, j, f);
+  if (micro) micro[q] = mi ? 1 : 0;
+  if (raw)
+    for (int c = 0; c < MLP_NFEAT; ++c) raw[(size_t)q * MLP_NFEAT + c] = f[c];
+  if (x)
+    for (int c = 0; c < MLP_IN; ++c)
+      x[(size_t)q * MLP_IN + c] =
+          (__bf16)(c < MLP_NFEAT ? (f[c] - norm->feat_mean[c]) * norm->feat_scale[c] : 0.f);
 }
 
 }  // namespace ice3
@@ -280,29 +461,137 @@ extern "C" int ice3_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld
   return launch_rain_ice(ctx, prec, sa, m, stream);
 }
 
-extern "C" int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld,
-                                    double dt, const ice3_pipeline_in *in,
-                                    const ice3_pipeline_out *out, const ice3_stats *stats,
-                                    void *stream) {
+static bool bad_pipeline_ptrs(const ice3_pipeline_in *in, const ice3_pipeline_out *out) {
+  const void *const pin[] = {in->pabs, in->sigs, in->th, in->exn, in->exn_ref, in->rhodref,
+                             in->dzz, in->rv, in->rc, in->ri, in->rr, in->rs, in->rg, in->ci,
+                             in->cf_mf, in->rc_mf, in->ri_mf, in->ths, in->rvs, in->rcs,
+                             in->rrs, in->ris, in->rss, in->rgs, in->sigqsat};
+  const void *const pout[] = {out->ths, out->rvs, out->rcs, out->rrs, out->ris, out->rss,
+                              out->rgs, out->cldfr, out->hlc_hrc, out->hlc_hcf, out->hli_hri,
+                              out->hli_hcf, out->ci, out->evap3d, out->rainfr, out->inprc,
+                              out->inprr, out->inpri, out->inprs, out->inprg};
+  for (const void *p : pin)
+    if (!p) return true;
+  for (const void *p : pout)
+    if (!p) return true;
+  return false;
+}
+
+extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld,
+                                           double dt, const ice3_pipeline_in *in,
+                                           const ice3_pipeline_out *out,
+                                           const ice3_stats *stats, int stages, void *stream) {
   if (!ctx || !in || !out || bad_shape(ncol, nlev, ld, dt)) return ICE3_EINVAL;
   if (prec != ICE3_PREC_FP64 && prec != ICE3_PREC_MIXED) return ICE3_EINVAL;
-  if (nlev > 65535) return ICE3_EINVAL;
-  NN(in->pabs); NN(in->sigs); NN(in->th); NN(in->exn); NN(in->exn_ref); NN(in->rhodref);
-  NN(in->dzz); NN(in->rv); NN(in->rc); NN(in->ri); NN(in->rr); NN(in->rs);
-  NN(in->rg); NN(in->ci); NN(in->cf_mf); NN(in->rc_mf); NN(in->ri_mf); NN(in->ths);
-  NN(in->rvs); NN(in->rcs); NN(in->rrs); NN(in->ris); NN(in->rss); NN(in->rgs); NN(in->sigqsat);
-  NN(out->ths); NN(out->rvs); NN(out->rcs); NN(out->rrs); NN(out->ris); NN(out->rss);
-  NN(out->rgs); NN(out->cldfr); NN(out->hlc_hrc); NN(out->hlc_hcf); NN(out->hli_hri);
-  NN(out->hli_hcf); NN(out->ci); NN(out->evap3d); NN(out->rainfr); NN(out->inprc);
-  NN(out->inprr); NN(out->inpri); NN(out->inprs); NN(out->inprg);
+  if (nlev > 65535 || stages <= 0 || stages > 7) return ICE3_EINVAL;
+  if (bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
   AdjustArgs aa;
   SedimArgs sa;
   MicroArgs m;
   pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, stats, aa, sa, m);
   hipStream_t s = (hipStream_t)stream;
+  const dim3 gs((ncol + 255) / 256), gp((ncol + 255) / 256, nlev);
   MARK(0);
-  hipLaunchKernelGGL(k_ice_adjust, dim3((ncol + 255) / 256, nlev), dim3(256), 0, s, aa);
+  if (stages & ICE3_STAGE_ADJUST) {
+    hipLaunchKernelGGL(k_ice_adjust, gp, dim3(256), 0, s, aa);
+    HIPCHK(hipGetLastError());
+  }
+  MARK(1);
+  if (stages & ICE3_STAGE_SEDIMENTATION) {
+    if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
+    else hipLaunchKernelGGL(k_sedim<float>, gs, dim3(256), 0, s, sa);
+    HIPCHK(hipGetLastError());
+  }
+  MARK(2);
+  if (stages & ICE3_STAGE_MICROPHYSICS) {
+    if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_micro<double>, gp, dim3(256), 0, s, m);
+    else hipLaunchKernelGGL(k_micro<float>, gp, dim3(256), 0, s, m);
+    HIPCHK(hipGetLastError());
+  }
+  MARK(3);
+  return ICE3_OK;
+}
+
+extern "C" int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld,
+                                    double dt, const ice3_pipeline_in *in,
+                                    const ice3_pipeline_out *out, const ice3_stats *stats,
+                                    void *stream) {
+  return ice3_adjust_rain_ice_stages(ctx, prec, ncol, nlev, ld, dt, in, out, stats,
+                                     ICE3_STAGE_ADJUST | ICE3_STAGE_SEDIMENTATION |
+                                         ICE3_STAGE_MICROPHYSICS,
+                                     stream);
+}
+
+static int n_cus(int device) {
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      n <= 0)
+    n = 256;
+  return n;
+}
+
+extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev, int ld,
+                                              double dt, const ice3_pipeline_in *in,
+                                              const ice3_pipeline_out *out,
+                                              const ice3_surrogate *sg, unsigned int *work,
+                                              unsigned long long *sg_counters,
+                                              const ice3_stats *stats, void *stream) {
+  if (!ctx || !in || !out || !sg || !work || !sg_counters || bad_shape(ncol, nlev, ld, dt))
+    return ICE3_EINVAL;
+  if (nlev > 65535 || (double)nlev * ncol >= 4.0e9 || (double)nlev * ld >= 4.0e9)
+    return ICE3_EINVAL;
+  if (!sg->weights_packed || !sg->params || !sg->norm || !(sg->budget_tol >= 0.f))
+    return ICE3_EINVAL;
+  if (bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
+  AdjustArgs aa;
+  SedimArgs sa;
+  MicroArgs m;
+  pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, stats, aa, sa, m);
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 gs((ncol + 255) / 256), gp((ncol + 255) / 256, nlev);
+  MARK(0);
+  hipLaunchKernelGGL(k_ice_adjust, gp, dim3(256), 0, s, aa);
   HIPCHK(hipGetLastError());
   MARK(1);
-  return launch_rain_ice(ctx, prec, sa, m, stream);
+  hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
+  HIPCHK(hipGetLastError());
+  MARK(2);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, work);
+  HIPCHK(hipGetLastError());
+  SurrogateArgs g;
+  g.m = m;
+  g.npts = (unsigned int)nlev * (unsigned int)ncol;
+  g.wpk = (const bf16x8 *)sg->weights_packed;
+  g.params = sg->params;
+  g.norm = sg->norm;
+  g.budget_tol = sg->budget_tol;
+  g.queue = work;
+  g.counters = sg_counters;
+  const int ncu = n_cus(ctx->device);
+  const unsigned int ntiles = (g.npts + MLP_TILE - 1) / MLP_TILE;
+  const unsigned int nblk = ntiles < (unsigned int)ncu ? ntiles : (unsigned int)ncu;
+  hipLaunchKernelGGL(k_surrogate, dim3(nblk), dim3(256), SG_LDS, s, g);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_micro_list<double>, dim3(4 * ncu), dim3(64), 0, s, m,
+                     (const unsigned int *)work);
+  HIPCHK(hipGetLastError());
+  MARK(3);
+  return ICE3_OK;
+}
+
+extern "C" int ice3_surrogate_features(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                       const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                                       const ice3_surrogate_norm *norm, float *raw, void *x,
+                                       unsigned char *micro, void *stream) {
+  if (!ctx || !in || !out || bad_shape(ncol, nlev, ld, dt) || (x && !norm)) return ICE3_EINVAL;
+  if ((double)nlev * ncol >= 4.0e9 || bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
+  AdjustArgs aa;
+  SedimArgs sa;
+  MicroArgs m;
+  pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, nullptr, aa, sa, m);
+  const unsigned int npts = (unsigned int)nlev * (unsigned int)ncol;
+  hipLaunchKernelGGL(k_features, dim3((npts + 255) / 256), dim3(256), 0, (hipStream_t)stream, m,
+                     npts, norm, raw, (__bf16 *)x, micro);
+  HIPCHK(hipGetLastError());
+  return ICE3_OK;
 }

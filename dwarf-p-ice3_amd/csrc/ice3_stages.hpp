@@ -184,6 +184,30 @@ __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j) {
   return calls;
 }
 
+// Input features of the surrogate at point j (the microphysics inputs of micro_at):
+// T, ln p, rhodref, ln(r + 1e-9) of the six water species of the t-state, ln(1 + ci),
+// sigma_s, exner, dt, cloud fraction.  Returns false where RAIN_ICE would not run the
+// ICE4 microphysics (no hydrometeor above XRTMIN), as micro_at decides it.
+__device__ __forceinline__ bool surrogate_features(const MicroArgs &a, size_t j, float *f) {
+  const ice3_params &P = c_P;
+  const double ts = a.tscale, exn = a.exn[j];
+  double r[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) r[v] = a.t[v][j] * ts;
+  f[0] = (float)(r[ITH] * exn);
+  f[1] = logf((float)a.pabs[j]);
+  f[2] = (float)a.rhodref[j];
+#pragma unroll
+  for (int v = 1; v < 7; ++v) f[2 + v] = logf((float)fmax(r[v], 0.) + 1e-9f);
+  f[9] = logf(1.f + (float)fmax(a.src_ci[j], 0.));
+  f[10] = (float)a.sigs[j];
+  f[11] = (float)exn;
+  f[12] = (float)a.dt;
+  f[13] = (float)a.cldfr[j];
+  return r[IRC] > P.XRTMIN[2] || r[IRR] > P.XRTMIN[3] || r[IRI] > P.XRTMIN[4] ||
+         r[IRS] > P.XRTMIN[5] || r[IRG] > P.XRTMIN[6];
+}
+
 // ---------------------------------------------------------------- argument set-up
 // Host-side wiring of the C-ABI structs into the stage arguments, shared by the
 // kernel launchers and by CPU builds of the stages.

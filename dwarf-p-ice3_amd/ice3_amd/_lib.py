@@ -47,9 +47,28 @@ PipeIn = _struct("ice3_pipeline_in", PIPE_IN_FIELDS)
 PipeOut = _struct("ice3_pipeline_out", PIPE_OUT_FIELDS)
 Stats = _struct("ice3_stats", ("calls", "counters"))
 
+STAGE_ADJUST, STAGE_SEDIMENTATION, STAGE_MICROPHYSICS = 1, 2, 4
+NFEAT, NTARG = 14, 8
+
+
+class SurrogateNorm(ctypes.Structure):
+    """ice3_surrogate_norm (include/ice3.h)."""
+    _fields_ = [("feat_mean", ctypes.c_float * NFEAT), ("feat_scale", ctypes.c_float * NFEAT),
+                ("feat_lo", ctypes.c_float * NFEAT), ("feat_hi", ctypes.c_float * NFEAT),
+                ("targ_unit", ctypes.c_float * NTARG), ("targ_mean", ctypes.c_float * NTARG),
+                ("targ_std", ctypes.c_float * NTARG)]
+
+
+class Surrogate(ctypes.Structure):
+    """ice3_surrogate (include/ice3.h)."""
+    _fields_ = [("weights_packed", _P), ("params", _P), ("norm", _P), ("budget_tol", ctypes.c_float)]
+
+
 EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_constants",
            "ice3_ice_adjust", "ice3_rain_ice", "ice3_adjust_rain_ice", "ice3_stage_timing",
-           "ice3_stage_times")
+           "ice3_stage_times", "ice3_adjust_rain_ice_stages", "ice3_adjust_rain_ice_surrogate",
+           "ice3_surrogate_features", "ice3_mlp_sizes", "ice3_mlp_forward", "ice3_mlp_backward",
+           "ice3_mlp_adam")
 
 _lib = None
 
@@ -78,6 +97,18 @@ def load():
                                                ctypes.POINTER(RainOut), ctypes.POINTER(Stats), _P]
         lib.ice3_adjust_rain_ice.argtypes = common + [ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut),
                                                       ctypes.POINTER(Stats), _P]
+        lib.ice3_adjust_rain_ice_stages.argtypes = common + [
+            ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut), ctypes.POINTER(Stats), ctypes.c_int, _P]
+        lib.ice3_adjust_rain_ice_surrogate.argtypes = [
+            _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(PipeIn),
+            ctypes.POINTER(PipeOut), ctypes.POINTER(Surrogate), _P, _P, ctypes.POINTER(Stats), _P]
+        lib.ice3_surrogate_features.argtypes = [
+            _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(PipeIn),
+            ctypes.POINTER(PipeOut), _P, _P, _P, _P, _P]
+        lib.ice3_mlp_sizes.argtypes = [ctypes.POINTER(ctypes.c_int)] * 5
+        lib.ice3_mlp_forward.argtypes = [ctypes.c_int] + [_P] * 10
+        lib.ice3_mlp_backward.argtypes = [ctypes.c_int] + [_P] * 8
+        lib.ice3_mlp_adam.argtypes = [ctypes.c_int] + [ctypes.c_float] * 4 + [_P] * 7
         lib.ice3_stage_timing.argtypes = [_P, ctypes.c_int]
         lib.ice3_stage_times.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
         _lib = lib

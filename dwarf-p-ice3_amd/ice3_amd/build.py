@@ -11,7 +11,7 @@ CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(HERE, "_lib", "libice3_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ("ice3_kernels.hip", "ice3_params.cpp")
+SOURCES = ("ice3_kernels.hip", "ice3_mlp.hip", "ice3_params.cpp")
 # -ffp-contract=off keeps FP64 a*b+c as two roundings, the sequence of the CPU reference.
 # Packed FP32 instructions are disabled for the device code (hardware notes).
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",

@@ -122,6 +122,77 @@ int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, do
                          const ice3_pipeline_in *in, const ice3_pipeline_out *out,
                          const ice3_stats *stats, void *stream);
 
+/* Stages of the fused step, for ice3_adjust_rain_ice_stages: run any subset, in this order,
+ * on the same in/out buffers (the outputs of one call are the state the next call reads). */
+#define ICE3_STAGE_ADJUST 1          /* ICE_ADJUST into out (adjusted sources, cloud state) */
+#define ICE3_STAGE_SEDIMENTATION 2   /* ICE4_SEDIMENTATION_STAT into out (+ surface precip) */
+#define ICE3_STAGE_MICROPHYSICS 4    /* ICE4 time splitting, total tendencies, negativities */
+int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double dt,
+                                const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                                const ice3_stats *stats, int stages, void *stream);
+
+/* ---- learned surrogate of the ICE4 microphysics (no counterpart in the reference) ----
+ * A per-point MLP (14 features -> 4 x 128 SiLU -> 8 outputs, bf16 weights on the matrix
+ * cores, fp32 accumulation) replaces the microphysics stage of the fused step.  Its
+ * features are T, ln p, rhodref, ln(r + 1e-9) of rv, rc, rr, ri, rs, rg (t-state),
+ * ln(1 + ci), sigma_s, exner, dt and the cloud fraction; its outputs are the changes the
+ * microphysics makes to the sources of th, rv, rc, rr, ri, rs, rg and to ci, each as
+ * unit * sinh(mean + std * y).  A point goes back to the plain FP64 microphysics when a
+ * feature lies outside the training range, when a resulting source of rv..rg would be
+ * negative, when the water residual sum(d rv..d rg) exceeds budget_tol * sum|d r|, or when
+ * it has no hydrometeor (the plain kernel's nucleation-only branch).  Accepted points
+ * have their water residual moved into rv, so total water is conserved. */
+typedef struct {
+  float feat_mean[14], feat_scale[14];   /* x = (f - mean) * scale */
+  float feat_lo[14], feat_hi[14];        /* training range of f */
+  float targ_unit[8], targ_mean[8], targ_std[8];
+} ice3_surrogate_norm;
+
+typedef struct {
+  const void *weights_packed;            /* bf16 packed fragments (ice3_mlp_adam output) */
+  const float *params;                   /* fp32 parameters (biases) */
+  const ice3_surrogate_norm *norm;       /* device pointer */
+  float budget_tol;
+} ice3_surrogate;
+
+/* Fused step with the surrogate microphysics.  work: nlev*ncol + 1 unsigned ints of
+ * device scratch (fallback queue).  sg_counters (device, 5 entries, accumulated):
+ * [0] points taken from the network, [1] out of range, [2] negative source,
+ * [3] water residual, [4] no hydrometeor.  stats as for ice3_adjust_rain_ice (fallback
+ * points only; network points report 0 calls). */
+int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                   const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                                   const ice3_surrogate *sg, unsigned int *work,
+                                   unsigned long long *sg_counters, const ice3_stats *stats,
+                                   void *stream);
+
+/* Surrogate features of the state between the sedimentation and microphysics stages
+ * (in/out as after ice3_adjust_rain_ice_stages(ADJUST | SEDIMENTATION)).  raw: [npts][14]
+ * fp32 (may be NULL); x: [npts][32] bf16 normalised with norm (both may be NULL);
+ * micro: [npts] 1 where the point runs the microphysics (may be NULL).  npts = nlev*ncol,
+ * point q = k*ncol + i. */
+int ice3_surrogate_features(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                            const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                            const ice3_surrogate_norm *norm, float *raw, void *x,
+                            unsigned char *micro, void *stream);
+
+/* Training kernels of the MLP (device pointers, fp32 parameters in ice3_mlp_sizes order:
+ * row-major weights of the 5 layers, then the biases).  x: [npts][32] bf16 features;
+ * target: [npts][8] fp32 normalised outputs.  ice3_mlp_forward: y [npts][8] (may be
+ * NULL); for training also zsave [4][npts][128] fp32, hsave [4][npts][128] bf16,
+ * dy [npts][32] fp32 and loss_part [ceil(npts/128)] (sum of squared errors per tile).
+ * ice3_mlp_backward: gpart [ceil(npts/128)][n_params] scratch, grad [n_params].
+ * ice3_mlp_adam: one Adam step (grad NULL: only rewrite the bf16 copies wrm, wpk). */
+int ice3_mlp_sizes(int *n_params, int *n_weights, int *tile, int *in_width, int *out_width);
+int ice3_mlp_forward(int npts, const void *wpk, const float *params, const void *x,
+                     const float *target, float *y, float *zsave, void *hsave, float *dy,
+                     float *loss_part, void *stream);
+int ice3_mlp_backward(int npts, const void *wrm, const void *x, const float *zsave,
+                      const void *hsave, const float *dy, float *gpart, float *grad,
+                      void *stream);
+int ice3_mlp_adam(int step, float lr, float beta1, float beta2, float eps, float *params,
+                  float *m, float *v, const float *grad, void *wrm, void *wpk, void *stream);
+
 /* ---- per-stage timing (no counterpart in the reference: a measurement hook) ----
  * ice3_stage_timing(ctx, 1) makes the three entry points above record HIP events
  * around each of their kernels on the call's stream; ice3_stage_times waits for

@@ -1,0 +1,158 @@
+"""The surrogate's matrix-core kernels against the CPU restatement (oracle/mlp_oracle.py),
+its training on FP64 pairs, and the fused step with the surrogate and its fallback."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_device(0)
+    return torch.device("cuda", 0)
+
+
+def _random_batch(n, seed):
+    rng = np.random.Generator(np.random.Philox(seed))
+    x = np.zeros((n, 32), dtype=np.float32)
+    x[:, :14] = rng.normal(0, 1, (n, 14))
+    t = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    return x, t
+
+
+def test_mlp_forward_matches_cpu(dev):
+    """Forward pass of the bf16 matrix-core kernel (operand layouts, packing, SiLU, biases)
+    against the CPU restatement with the same bf16 rounding; a ragged last tile."""
+    from ice3_amd import surrogate as S
+    from oracle import mlp_oracle as mo
+    p = S.init_params(11)
+    p[mo.NW:mo.NW + 4 * 128] = np.random.Generator(np.random.Philox(5)).normal(0, 0.2, 4 * 128)
+    mlp = S.MLP(device=0, params=p)
+    x, _ = _random_batch(300, 1)
+    xb = torch.as_tensor(x).to(dev).to(torch.bfloat16)
+    y = mlp.forward(xb).cpu().numpy()
+    yr, _, _ = mo.forward(p, xb.float().cpu().numpy(), emulate=True)
+    yr = yr[:, :8]
+    err = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+    print("mlp forward rel L2 error vs CPU", err, "max abs", np.abs(y - yr).max())
+    assert err < 5e-3
+    assert np.abs(y - yr).max() < 2e-2 * np.abs(yr).max()
+
+
+def test_mlp_gradient_matches_cpu(dev):
+    """Backward kernels (dW = H^T dZ, dH = dZ W^T on the matrix cores, SiLU', bias sums,
+    per-tile partials and their reduction) against the CPU restatement."""
+    from ice3_amd import surrogate as S
+    from oracle import mlp_oracle as mo
+    p = S.init_params(12)
+    mlp = S.MLP(device=0, params=p)
+    x, t = _random_batch(500, 2)
+    xb = torch.as_tensor(x).to(dev).to(torch.bfloat16)
+    tt = torch.as_tensor(t).to(dev)
+    sse = mlp.gradient(xb, tt).item()
+    g = mlp.grad.cpu().numpy()
+    loss, gr = mo.loss_and_grad(p, xb.float().cpu().numpy(), t, emulate=True)
+    assert abs(sse - loss) <= 1e-3 * loss
+    o = 0
+    for l, (k, n) in enumerate(mo.DIMS):
+        a, b = g[o:o + k * n], gr[o:o + k * n]
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"layer {l}: dW rel L2 {rel:.2e}, cosine {cos:.6f}")
+        assert rel < 2e-2 and cos > 0.9995
+        o += k * n
+    rel_b = np.linalg.norm(g[mo.NW:] - gr[mo.NW:]) / np.linalg.norm(gr[mo.NW:])
+    print(f"biases: rel L2 {rel_b:.2e}")
+    assert rel_b < 2e-2
+    # padded rows/columns receive exactly zero gradient
+    assert np.all(g[14 * 128:32 * 128] == 0)
+
+
+def test_mlp_training_reduces_loss(dev):
+    """Adam on the training kernels fits a smooth map of the inputs."""
+    from ice3_amd import surrogate as S
+    rng = np.random.Generator(np.random.Philox(3))
+    n = 1 << 16
+    x = np.zeros((n, 32), dtype=np.float32)
+    x[:, :14] = rng.normal(0, 1, (n, 14))
+    a = rng.normal(0, 0.5, (14, 8))
+    t = np.tanh(x[:, :14] @ a) + 0.3 * np.sin(2 * x[:, :8])
+    xb = torch.as_tensor(x).to(dev).to(torch.bfloat16)
+    tt = torch.as_tensor(t, dtype=torch.float32).to(dev)
+    mlp = S.MLP(device=0, seed=4)
+    before = S.evaluate(mlp, xb, tt)
+    hist = S.train(mlp, xb, tt, epochs=8, batch=4096, lr=3e-3, lr_min=1e-4)
+    after = S.evaluate(mlp, xb, tt)
+    print("mse before", before, "after", after, [round(h["train_mse"], 5) for h in hist])
+    assert after < 0.05 * before
+
+
+@pytest.fixture(scope="module")
+def small_surrogate(dev):
+    """A surrogate trained in the test on FP64 pairs of 8 generator seeds x 4096 columns."""
+    from ice3_amd import surrogate as S
+    raw, targ = S.make_pairs(2_000_000, ncol=4096, seed0=20260200)
+    norm = S.fit_norm(raw, targ)
+    x, t = S.encode_features(raw, norm), S.encode_targets(targ, norm)
+    mlp = S.MLP(device=0, seed=1)
+    hist = S.train(mlp, x, t, epochs=6, batch=16384, lr=3e-3)
+    print("surrogate training", [round(h["train_mse"], 4) for h in hist])
+    return S.TrainedSurrogate(mlp.params.cpu().numpy(), norm, budget_tol=0.1, device=0), norm
+
+
+def test_surrogate_step_fallback_and_budget(dev, small_surrogate):
+    """Fused step with the surrogate: every point is either taken from the network or
+    recomputed by the plain FP64 kernel (then identical to the plain step); sources of
+    rv..rg stay non-negative; every column's water changes only by its surface
+    precipitation; the trained network beats the untrained one and the "no microphysics"
+    baseline against the FP64 step."""
+    from ice3_amd import AdjustRainIce, synthetic
+    from ice3_amd import surrogate as S
+    sg, norm = small_surrogate
+    f = synthetic.generate(2048, 90, seed=20260300, variant="pipeline")
+    f["rhodref"] = f["rho_dry_ref"]
+    ref = AdjustRainIce()(f, 50.0, return_calls=True)
+    run = S.AdjustRainIceSurrogate(sg)
+    out = run(f, 50.0, return_calls=True)
+    c = run.last_counters
+    npts = 2048 * 90
+    print("surrogate counters", c)
+    assert c["network"] + c["out_of_range"] + c["negative"] + c["water_residual"] + c["no_hydrometeor"] == npts
+    assert c["network"] > 0.3 * npts
+    micro = ref["calls"] != 0                 # points that run the ICE4 microphysics
+    net = micro & (out["calls"] == 0)          # ... and were taken from the network
+    fall = ~net
+    assert int(net.sum()) == c["network"]
+    for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "ci"):
+        assert torch.equal(out[k][fall], ref[k][fall]), k
+    for k in ("rvs", "rcs", "rrs", "ris", "rss", "rgs"):
+        assert bool((out[k] >= 0).all()), k
+    rho, dz = torch.as_tensor(f["rho_dry_ref"], device=dev), torch.as_tensor(f["dzz"], device=dev)
+    w0 = sum(torch.as_tensor(f[k], device=dev) for k in ("rvs", "rcs", "rrs", "ris", "rss", "rgs"))
+    w1 = sum(out[k] for k in ("rvs", "rcs", "rrs", "ris", "rss", "rgs"))
+    col0, col1 = (rho * dz * w0).sum(0) * 50.0, (rho * dz * w1).sum(0) * 50.0
+    pr = -1000.0 * (out["inprr"] + out["inprs"] + out["inprg"]) * 50.0
+    berr = ((col1 - col0 - pr).abs() / col0).max().item()
+    print("surrogate column water budget max rel err", berr)
+    assert berr < 1e-12
+
+    # what training gains: relative RMS error against the FP64 step over the points both
+    # networks answered, trained versus untrained (same normalisation, no budget check)
+    untrained = S.TrainedSurrogate(S.init_params(1), norm, budget_tol=1e30, device=0)
+    run_u = S.AdjustRainIceSurrogate(untrained)
+    out_u = run_u(f, 50.0, return_calls=True)
+    both = net & (out_u["calls"] == 0)
+    assert int(both.sum()) > 1000
+    e_t, e_u = {}, {}
+    for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs"):
+        scale = ref[k][both].abs().mean()
+        e_t[k] = ((out[k] - ref[k])[both].pow(2).mean().sqrt() / scale).item()
+        e_u[k] = ((out_u[k] - ref[k])[both].pow(2).mean().sqrt() / scale).item()
+    print("relative RMS error vs FP64 (points both answered): trained", e_t, "untrained", e_u,
+          "untrained counters", run_u.last_counters)
+    for k in ("rcs", "rrs", "ris", "rss", "rgs"):
+        assert e_t[k] < 0.5 * e_u[k], k

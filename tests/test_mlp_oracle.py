@@ -1,0 +1,87 @@
+"""The CPU restatement of the surrogate MLP (oracle/mlp_oracle.py), pinned by finite
+differences of its own float64 forward pass, and the layout of the surrogate's C structs."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from oracle import mlp_oracle as mo
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _net(seed=1):
+    rng = np.random.Generator(np.random.Philox(seed))
+    p = np.zeros(mo.NP, dtype=np.float64)
+    o = 0
+    for k, n in mo.DIMS:
+        p[o:o + k * n] = rng.normal(0, 1 / np.sqrt(k), k * n)
+        o += k * n
+    p[mo.NW:] = rng.normal(0, 0.1, mo.NB)
+    x = np.zeros((6, mo.IN))
+    x[:, :14] = rng.normal(0, 1, (6, 14))
+    t = rng.normal(0, 1, (6, mo.NTARG))
+    return p, x, t
+
+
+def test_backward_matches_finite_differences():
+    p, x, t = _net()
+    loss, g = mo.loss_and_grad(p, x, t, emulate=False)
+    rng = np.random.Generator(np.random.Philox(7))
+    # parameters from every layer's weights and biases that influence the loss
+    idx = []
+    o = 0
+    for l, (k, n) in enumerate(mo.DIMS):
+        rows = 14 if l == 0 else k
+        cols = mo.NTARG if l == 4 else n
+        for _ in range(4):
+            idx.append(o + rng.integers(rows) * n + rng.integers(cols))
+        o += k * n
+    ob = mo.NW
+    for l, (k, n) in enumerate(mo.DIMS):
+        cols = mo.NTARG if l == 4 else n
+        idx.append(ob + rng.integers(cols))
+        ob += n
+    B = x.shape[0]
+    for i in idx:
+        h = 1e-6 * max(1.0, abs(p[i]))
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        lp = mo.loss_and_grad(pp, x, t, emulate=False)[0] / (B * mo.NTARG)
+        lm = mo.loss_and_grad(pm, x, t, emulate=False)[0] / (B * mo.NTARG)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1e-8, abs(fd)) + 1e-10, (i, fd, g[i])
+
+
+def test_padding_gets_no_gradient():
+    p, x, t = _net(3)
+    _, g = mo.loss_and_grad(p, x, t, emulate=True)
+    w0 = g[:mo.IN * mo.H].reshape(mo.IN, mo.H)
+    assert np.all(w0[14:] == 0)
+    o4 = mo.NW - mo.H * mo.OUT
+    w4 = g[o4:mo.NW].reshape(mo.H, mo.OUT)
+    assert np.all(w4[:, mo.NTARG:] == 0)
+    assert np.all(g[mo.NW + 4 * mo.H + mo.NTARG:] == 0)
+
+
+def test_bf16_rounding_is_nearest_even():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1e-40, 3.0e38], dtype=np.float32)
+    r = mo.bf16(v)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6 and r[3] == -2.5
+    assert r[5] == np.float32(3.0e38).view(np.uint32) >> 16 << 16 or np.isfinite(r[5])
+
+
+def test_surrogate_structs_match_header():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "dwarf-p-ice3_amd"))
+    from ice3_amd import _lib
+    hdr = open(os.path.join(ROOT, "include", "ice3.h")).read()
+    body = re.search(r"typedef struct \{([^}]*)\} ice3_surrogate_norm;", hdr).group(1)
+    names = re.findall(r"(\w+)\[(\d+)\]", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
+    assert [(n, int(c)) for n, c in names] == [(f, t._length_) for f, t in _lib.SurrogateNorm._fields_]
+    assert ctypes.sizeof(_lib.SurrogateNorm) == 4 * sum(int(c) for _, c in names)
+    body = re.search(r"typedef struct \{([^}]*)\} ice3_surrogate;", hdr).group(1)
+    names = re.findall(r"(\w+);", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
+    assert names == [f for f, _ in _lib.Surrogate._fields_]
