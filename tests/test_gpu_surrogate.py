@@ -85,10 +85,11 @@ def test_mlp_training_reduces_loss(dev):
     tt = torch.as_tensor(t, dtype=torch.float32).to(dev)
     mlp = S.MLP(device=0, seed=4)
     before = S.evaluate(mlp, xb, tt)
-    hist = S.train(mlp, xb, tt, epochs=8, batch=4096, lr=3e-3, lr_min=1e-4)
+    hist = S.train(mlp, xb, tt, epochs=16, batch=4096, lr=3e-3, lr_min=1e-4)
     after = S.evaluate(mlp, xb, tt)
     print("mse before", before, "after", after, [round(h["train_mse"], 5) for h in hist])
-    assert after < 0.05 * before
+    assert after < 0.1 * before
+    assert all(b["train_mse"] <= a["train_mse"] * 1.05 for a, b in zip(hist, hist[1:]))
 
 
 @pytest.fixture(scope="module")
