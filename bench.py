@@ -117,6 +117,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjust", action="store_true", help="skip the ice_adjust-alone line")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device twice (a plain copy of known size, "
                          "to scale HBM byte counters read by a profiler)")
@@ -184,7 +185,7 @@ def main():
         cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
         _lib.check(lib.ice3_stage_timing(ctx, 1), "ice3_stage_timing")
         acc = [0.0, 0.0, 0.0]
-        ms3 = (ctypes.c_float * 3)()
+        ms3 = (ctypes.c_float * 4)()
         for _ in range(steps):
             _lib.check(call(prec, cout), "timed step")
             _lib.check(lib.ice3_stage_times(ctx, ms3), "ice3_stage_times")
@@ -248,9 +249,112 @@ def main():
                                      "speedup_vs_fp64": ms / ms32, "stages_ms": stages32,
                                      "water_budget_max_rel_err": budget_err(out32),
                                      "error_vs_fp64": err}}
+    if not args.no_surrogate:
+        del out64
+        if not args.no_mixed:
+            del out32
+        res.setdefault("variants", {})["surrogate_1024x1024"] = surrogate_variant(lib, ctx, dev, g, args)
+    del g
     if not args.no_adjust:
         res.setdefault("variants", {})["ice_adjust_256x256"] = ice_adjust_alone(lib, ctx, dev, args)
     print(json.dumps(res))
+
+
+MLP_FLOP_PER_POINT = 2.0 * (14 * 128 + 3 * 128 * 128 + 128 * 8)   # 103,936
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
+    """The learned surrogate at 1024x1024 columns x 90 levels (the 512x512 slab tiled 4x along
+    the columns): FP64 step vs surrogate step (network + FP64 fallback), per-field error of the
+    surrogate against FP64, fallback statistics, water budget, matrix-core roofline of the
+    network kernel (103,936 FLOP per point)."""
+    import torch
+
+    from ice3_amd import _lib
+    from ice3_amd import surrogate as S
+    from ice3_amd._fields import ptr
+    if not os.path.exists(S.DEFAULT_WEIGHTS):
+        return {"skipped": "no trained weights (" + os.path.relpath(S.DEFAULT_WEIGHTS, ROOT) + ")"}
+    sg = S.TrainedSurrogate.load(S.DEFAULT_WEIGHTS, device=dev.index)
+    G = {k: (v.repeat(1, reps) if v.dim() == 2 else v.repeat(reps)).contiguous() for k, v in g.items()}
+    nlev, ncol = G["th"].shape
+    npts = nlev * ncol
+
+    def outputs():
+        o = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[:15]}
+        o.update({k: torch.empty(ncol, dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[15:]})
+        return o
+    cin = _lib.PipeIn(*[ptr(G[k]) for k in _lib.PIPE_IN_FIELDS])
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    ref, out = outputs(), outputs()
+    cref = _lib.PipeOut(*[ptr(ref[k]) for k in _lib.PIPE_OUT_FIELDS])
+    cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
+    work = torch.empty(npts + 1, dtype=torch.int32, device=dev)
+    sgc = torch.zeros(5, dtype=torch.int64, device=dev)
+    h = sg.handle()
+
+    def plain():
+        return lib.ice3_adjust_rain_ice(ctx, _lib.PREC_FP64, ncol, nlev, ncol, DT, ctypes.byref(cin),
+                                        ctypes.byref(cref), None, sp)
+
+    def surrogate():
+        return lib.ice3_adjust_rain_ice_surrogate(ctx, ncol, nlev, ncol, DT, ctypes.byref(cin), ctypes.byref(cout),
+                                                  ctypes.byref(h), ptr(work), ptr(sgc), None, sp)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            _lib.check(fn(), "warmup")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    ms_plain = timed(plain)
+    ms_sg = timed(surrogate)
+    _lib.check(lib.ice3_stage_timing(ctx, 1), "ice3_stage_timing")
+    acc = [0.0] * 4
+    ms4 = (ctypes.c_float * 4)()
+    for _ in range(args.steps):
+        _lib.check(surrogate(), "timed surrogate step")
+        _lib.check(lib.ice3_stage_times(ctx, ms4), "ice3_stage_times")
+        acc = [x + y for x, y in zip(acc, ms4)]
+    _lib.check(lib.ice3_stage_timing(ctx, 0), "ice3_stage_timing")
+    stages = dict(zip(("ice_adjust", "sedimentation", "network", "fallback"), [x / args.steps for x in acc]))
+    sgc.zero_()
+    _lib.check(surrogate(), "surrogate step")
+    torch.cuda.synchronize()
+    c = sgc.cpu().tolist()
+    err = {}
+    for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "ci"):
+        d = (out[k] - ref[k]).abs()
+        scale = ref[k].abs().max().item()
+        err[k] = {"max_abs": d.max().item(), "rms_abs": d.pow(2).mean().sqrt().item(),
+                  "max_rel_to_field_max": d.max().item() / scale if scale > 0 else 0.0,
+                  "rms_rel_to_field_rms": (d.pow(2).mean().sqrt() / ref[k].pow(2).mean().sqrt()).item()}
+    w = ("rvs", "rcs", "rrs", "ris", "rss", "rgs")
+    mass = G["rhodref"] * G["dzz"]
+    col0 = (mass * sum(G[k] for k in w)).sum(0) * DT
+    col1 = (mass * sum(out[k] for k in w)).sum(0) * DT
+    pr = -1000.0 * (out["inprr"] + out["inprs"] + out["inprg"]) * DT
+    budget = ((col1 - col0 - pr).abs() / col0).max().item()
+    tflops = MLP_FLOP_PER_POINT * npts / (stages["network"] * 1e-3) / 1e12
+    return {"columns": ncol, "levels": nlev, "weights": os.path.relpath(S.DEFAULT_WEIGHTS, ROOT),
+            "train_pairs": int(sg.meta.get("pairs", 0)), "budget_tol": sg.budget_tol,
+            "fp64_ms_per_step": ms_plain, "fp64_columns_per_s": ncol / (ms_plain * 1e-3),
+            "surrogate_ms_per_step": ms_sg, "surrogate_columns_per_s": ncol / (ms_sg * 1e-3),
+            "speedup_vs_fp64": ms_plain / ms_sg, "stages_ms": stages,
+            "points": npts, "network_points": c[0], "fallback": {"out_of_range": c[1], "negative_source": c[2],
+                                                                 "water_residual": c[3], "no_hydrometeor": c[4]},
+            "network_fraction": c[0] / npts, "water_budget_max_rel_err": budget, "error_vs_fp64": err,
+            "roofline": {"bound": "mfma", "kernel": "k_surrogate", "achieved": tflops,
+                         "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
+                         "flop_per_point": MLP_FLOP_PER_POINT, "ms_per_launch": stages["network"]}}
 
 
 def ice_adjust_alone(lib, ctx, dev, args, ncol: int = 256 * 256):

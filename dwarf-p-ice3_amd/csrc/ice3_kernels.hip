@@ -91,7 +91,7 @@ struct SurrogateArgs {
   const bf16x8 *wpk;
   const float *params;
   const ice3_surrogate_norm *norm;
-  float budget_tol;
+  float budget_tol, neg_tol;
   unsigned int *queue;               // [0] count, [1 + q] point offsets
   unsigned long long *counters;      // 5 entries, see include/ice3.h
 };
@@ -180,25 +180,32 @@ __global__ void __launch_bounds__(256) k_surrogate(SurrogateArgs a) {
     // decode, physical checks, results or fallback queue
     double out[7], cinew = 0.;
     if (flag == SG_OK) {
-      double rsum = 0., rmag = 0.;
+      double rsum = 0., rmag = 0., y[7];
 #pragma unroll
       for (int v = 0; v < 7; ++v) {
         const float u = nm.targ_mean[v] + nm.targ_std[v] * sY[lane * 9 + v];
-        const double y = (double)(nm.targ_unit[v] * sinhf(u));
-        out[v] = m.src[v][j] + y;
-        if (v >= 1) { rsum += y; rmag += fabs(y); }
+        y[v] = (double)(nm.targ_unit[v] * sinhf(u));
+        out[v] = m.src[v][j] + y[v];
+        if (v >= 1) { rsum += y[v]; rmag += fabs(y[v]); }
       }
       {
         const float u = nm.targ_mean[7] + nm.targ_std[7] * sY[lane * 9 + 7];
         cinew = fmax(0., m.src_ci[j] + (double)(nm.targ_unit[7] * sinhf(u)));
       }
-      if (fabs(rsum) > (double)a.budget_tol * rmag + 1e-18) {
+      if (!(fabs(rsum) <= (double)a.budget_tol * rmag + 1e-18)) {
         flag = SG_BUDGET;
       } else {
         out[IRV] -= rsum;  // move the water residual into vapour
 #pragma unroll
-        for (int v = 1; v < 7; ++v)
-          if (!(out[v] >= 0.)) flag = SG_NEG;
+        for (int v = IRC; v <= IRG; ++v)
+          if (!(out[v] >= -(double)a.neg_tol * (fabs(m.src[v][j]) + fabs(y[v])))) flag = SG_NEG;
+        if (flag == SG_OK) {
+          double lvfact, lsfact;
+          latent_factors(m, j, lvfact, lsfact);
+          correct_negativities(out[ITH], out[IRV], out[IRC], out[IRR], out[IRI], out[IRS],
+                               out[IRG], lvfact, lsfact);
+          if (!(out[IRV] >= 0.)) flag = SG_NEG;
+        }
       }
     }
     if (flag == SG_OK) {
@@ -260,7 +267,8 @@ struct ice3_ctx {
   double *d_tab;
   double h_tab[ICE3_T_TOTAL];
   int timing;         // record ev[] around the stages of each call
-  hipEvent_t ev[4];   // before adjust, after adjust, after sedimentation, after microphysics
+  hipEvent_t ev[5];   // before adjust, after adjust, after sedimentation, after microphysics
+                      // (the network, for the surrogate step), after the fallback
 };
 
 using namespace ice3;
@@ -292,26 +300,26 @@ extern "C" void ice3_ctx_destroy(ice3_ctx *c) {
   if (!c) return;
   if (c->d_tab) (void)hipFree(c->d_tab);
   if (c->timing)
-    for (int q = 0; q < 4; ++q) (void)hipEventDestroy(c->ev[q]);
+    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(c->ev[q]);
   free(c);
 }
 
 extern "C" int ice3_stage_timing(ice3_ctx *c, int enable) {
   if (!c) return ICE3_EINVAL;
   if (enable && !c->timing) {
-    for (int q = 0; q < 4; ++q) HIPCHK(hipEventCreate(&c->ev[q]));
+    for (int q = 0; q < 5; ++q) HIPCHK(hipEventCreate(&c->ev[q]));
     c->timing = 1;
   } else if (!enable && c->timing) {
-    for (int q = 0; q < 4; ++q) (void)hipEventDestroy(c->ev[q]);
+    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(c->ev[q]);
     c->timing = 0;
   }
   return ICE3_OK;
 }
 
-extern "C" int ice3_stage_times(ice3_ctx *c, float ms[3]) {
+extern "C" int ice3_stage_times(ice3_ctx *c, float ms[4]) {
   if (!c || !ms || !c->timing) return ICE3_EINVAL;
-  HIPCHK(hipEventSynchronize(c->ev[3]));
-  for (int q = 0; q < 3; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
+  HIPCHK(hipEventSynchronize(c->ev[4]));
+  for (int q = 0; q < 4; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
   return ICE3_OK;
 }
 
@@ -419,7 +427,7 @@ extern "C" int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int 
   MARK(0);
   hipLaunchKernelGGL(k_ice_adjust, grid, dim3(256), 0, s, a);
   HIPCHK(hipGetLastError());
-  MARK(1); MARK(2); MARK(3);
+  MARK(1); MARK(2); MARK(3); MARK(4);
   return ICE3_OK;
 }
 
@@ -435,7 +443,7 @@ static int launch_rain_ice(ice3_ctx *ctx, int prec, const SedimArgs &sa, const M
   if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_micro<double>, gm, dim3(256), 0, s, ma);
   else hipLaunchKernelGGL(k_micro<float>, gm, dim3(256), 0, s, ma);
   HIPCHK(hipGetLastError());
-  MARK(3);
+  MARK(3); MARK(4);
   return ICE3_OK;
 }
 
@@ -508,7 +516,7 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
     else hipLaunchKernelGGL(k_micro<float>, gp, dim3(256), 0, s, m);
     HIPCHK(hipGetLastError());
   }
-  MARK(3);
+  MARK(3); MARK(4);
   return ICE3_OK;
 }
 
@@ -540,7 +548,8 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
     return ICE3_EINVAL;
   if (nlev > 65535 || (double)nlev * ncol >= 4.0e9 || (double)nlev * ld >= 4.0e9)
     return ICE3_EINVAL;
-  if (!sg->weights_packed || !sg->params || !sg->norm || !(sg->budget_tol >= 0.f))
+  if (!sg->weights_packed || !sg->params || !sg->norm || !(sg->budget_tol >= 0.f) ||
+      !(sg->neg_tol >= 0.f))
     return ICE3_EINVAL;
   if (bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
   AdjustArgs aa;
@@ -565,6 +574,7 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   g.params = sg->params;
   g.norm = sg->norm;
   g.budget_tol = sg->budget_tol;
+  g.neg_tol = sg->neg_tol;
   g.queue = work;
   g.counters = sg_counters;
   const int ncu = n_cus(ctx->device);
@@ -572,10 +582,11 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   const unsigned int nblk = ntiles < (unsigned int)ncu ? ntiles : (unsigned int)ncu;
   hipLaunchKernelGGL(k_surrogate, dim3(nblk), dim3(256), SG_LDS, s, g);
   HIPCHK(hipGetLastError());
+  MARK(3);
   hipLaunchKernelGGL(k_micro_list<double>, dim3(4 * ncu), dim3(64), 0, s, m,
                      (const unsigned int *)work);
   HIPCHK(hipGetLastError());
-  MARK(3);
+  MARK(4);
   return ICE3_OK;
 }
 

@@ -1,6 +1,6 @@
 // ice3_mlp.hpp -- the per-point MLP of the rain_ice surrogate on CDNA4 matrix cores.
 //
-// Network: 14 features (padded to MLP_IN = 32) -> 4 hidden layers of 128 units with
+// Network: 18 features (padded to MLP_IN = 32) -> 4 hidden layers of 128 units with
 // SiLU -> 8 outputs (padded to MLP_OUT = 32).  Weights are bf16 on the matrix cores
 // with fp32 accumulation (v_mfma_f32_16x16x32_bf16); master weights, biases and the
 // optimiser state are fp32.
@@ -21,7 +21,7 @@ namespace ice3 {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int MLP_NFEAT = 14;   // features used
+constexpr int MLP_NFEAT = 18;   // features used
 constexpr int MLP_NTARG = 8;    // outputs used
 constexpr int MLP_IN = 32;      // padded input width
 constexpr int MLP_H = 128;      // hidden width

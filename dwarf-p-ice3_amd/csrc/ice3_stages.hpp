@@ -108,6 +108,32 @@ struct MicroArgs {
   unsigned long long *counters;  // [0] tendency calls, [1] capped points, [2] micro points
 };
 
+// ICE4_CORRECT_NEGATIVITIES (micro/mode_ice4_correct_negativities.F90:51-171) on one
+// point's sources: negative condensate goes back to vapour with its latent heat, then
+// vapour below XRTMIN(1) is restored from cloud + ice, rain, snow and graupel.
+__device__ __forceinline__ void correct_negativities(double &ths, double &rvs, double &rcs,
+                                                     double &rrs, double &ris, double &rss,
+                                                     double &rgs, double lvfact, double lsfact) {
+  const ice3_params &P = c_P;
+  double zw;
+  zw = rcs - dmax(rcs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rcs = rcs - zw;
+  zw = rrs - dmax(rrs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rrs = rrs - zw;
+  zw = ris - dmax(ris, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; ris = ris - zw;
+  zw = rss - dmax(rss, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rss = rss - zw;
+  zw = rgs - dmax(rgs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rgs = rgs - zw;
+  zw = dmin(1., dmax(P.XRTMIN[1] - rvs, 0.) / dmax(rcs + ris, 1.E-20));
+  ths = ths - zw * (rcs * lvfact + ris * lsfact);
+  rvs = rvs + zw * (rcs + ris);
+  rcs = (1. - zw) * rcs;
+  ris = (1. - zw) * ris;
+  zw = dmin(dmax(rrs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+  rvs = rvs + zw; rrs = rrs - zw; ths = ths - zw * lvfact;
+  zw = dmin(dmax(rss, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+  rvs = rvs + zw; rss = rss - zw; ths = ths - zw * lsfact;
+  zw = dmin(dmax(rgs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+  rvs = rvs + zw; rgs = rgs - zw; ths = ths - zw * lsfact;
+}
+
 // RAIN_ICE sections 1, 4, 5, 8 (micro/rain_ice.F90:224-273, 391-431, 529-543 and the
 // reinstated 649-714) for point j.  Returns the number of tendency calls, negative if the
 // sub-step cap was hit.
@@ -157,24 +183,8 @@ __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j) {
   double ris = s[IRI] + wri + rvheni;
   double rss = s[IRS] + wrs;
   double rgs = s[IRG] + wrg;
-  // 8.2 ICE4_CORRECT_NEGATIVITIES (micro/mode_ice4_correct_negativities.F90:51-171)
-  double zw;
-  zw = rcs - dmax(rcs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rcs = rcs - zw;
-  zw = rrs - dmax(rrs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rrs = rrs - zw;
-  zw = ris - dmax(ris, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; ris = ris - zw;
-  zw = rss - dmax(rss, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rss = rss - zw;
-  zw = rgs - dmax(rgs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rgs = rgs - zw;
-  zw = dmin(1., dmax(P.XRTMIN[1] - rvs, 0.) / dmax(rcs + ris, 1.E-20));
-  ths = ths - zw * (rcs * lvfact + ris * lsfact);
-  rvs = rvs + zw * (rcs + ris);
-  rcs = (1. - zw) * rcs;
-  ris = (1. - zw) * ris;
-  zw = dmin(dmax(rrs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
-  rvs = rvs + zw; rrs = rrs - zw; ths = ths - zw * lvfact;
-  zw = dmin(dmax(rss, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
-  rvs = rvs + zw; rss = rss - zw; ths = ths - zw * lsfact;
-  zw = dmin(dmax(rgs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
-  rvs = rvs + zw; rgs = rgs - zw; ths = ths - zw * lsfact;
+  // 8.2 ICE4_CORRECT_NEGATIVITIES
+  correct_negativities(ths, rvs, rcs, rrs, ris, rss, rgs, lvfact, lsfact);
   a.dst[ITH][j] = ths; a.dst[IRV][j] = rvs; a.dst[IRC][j] = rcs; a.dst[IRR][j] = rrs;
   a.dst[IRI][j] = ris; a.dst[IRS][j] = rss; a.dst[IRG][j] = rgs;
   a.ci[j] = cit;
@@ -186,8 +196,10 @@ __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j) {
 
 // Input features of the surrogate at point j (the microphysics inputs of micro_at):
 // T, ln p, rhodref, ln(r + 1e-9) of the six water species of the t-state, ln(1 + ci),
-// sigma_s, exner, dt, cloud fraction.  Returns false where RAIN_ICE would not run the
-// ICE4 microphysics (no hydrometeor above XRTMIN), as micro_at decides it.
+// sigma_s, exner, dt, cloud fraction, and asinh(1e6 * dt * e) of the external tendency
+// e = source - t-state/dt of rr, ri, rs, rg (the sedimentation, which the time
+// splitting integrates beside the processes).  Returns false where RAIN_ICE would not
+// run the ICE4 microphysics (no hydrometeor above XRTMIN), as micro_at decides it.
 __device__ __forceinline__ bool surrogate_features(const MicroArgs &a, size_t j, float *f) {
   const ice3_params &P = c_P;
   const double ts = a.tscale, exn = a.exn[j];
@@ -204,8 +216,26 @@ __device__ __forceinline__ bool surrogate_features(const MicroArgs &a, size_t j,
   f[11] = (float)exn;
   f[12] = (float)a.dt;
   f[13] = (float)a.cldfr[j];
+#pragma unroll
+  for (int v = IRR; v <= IRG; ++v)
+    f[14 + v - IRR] = asinhf((float)(1e6 * (a.src[v][j] * a.dt - r[v])));
   return r[IRC] > P.XRTMIN[2] || r[IRR] > P.XRTMIN[3] || r[IRI] > P.XRTMIN[4] ||
          r[IRS] > P.XRTMIN[5] || r[IRG] > P.XRTMIN[6];
+}
+
+// Latent heat factors L/(cph * exnref) of the total tendencies at point j (t-state), as
+// micro_at computes them (micro/rain_ice.F90:334-347 divided by PEXNREF).
+__device__ __forceinline__ void latent_factors(const MicroArgs &a, size_t j, double &lvfact,
+                                               double &lsfact) {
+  const ice3_params &P = c_P;
+  const double ts = a.tscale;
+  const double tht = a.t[ITH][j] * ts, rvt = a.t[IRV][j] * ts, rct = a.t[IRC][j] * ts;
+  const double rrt = a.t[IRR][j] * ts, rit = a.t[IRI][j] * ts, rst = a.t[IRS][j] * ts;
+  const double rgt = a.t[IRG][j] * ts;
+  const double zdevide = P.XCPD + P.XCPV * rvt + P.XCL * (rct + rrt) + P.XCI * (rit + rst + rgt);
+  const double zt = tht * a.exn[j];
+  lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide / a.exn_ref[j];
+  lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide / a.exn_ref[j];
 }
 
 // ---------------------------------------------------------------- argument set-up

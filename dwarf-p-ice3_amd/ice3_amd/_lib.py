@@ -48,7 +48,7 @@ PipeOut = _struct("ice3_pipeline_out", PIPE_OUT_FIELDS)
 Stats = _struct("ice3_stats", ("calls", "counters"))
 
 STAGE_ADJUST, STAGE_SEDIMENTATION, STAGE_MICROPHYSICS = 1, 2, 4
-NFEAT, NTARG = 14, 8
+NFEAT, NTARG = 18, 8
 
 
 class SurrogateNorm(ctypes.Structure):
@@ -61,7 +61,8 @@ class SurrogateNorm(ctypes.Structure):
 
 class Surrogate(ctypes.Structure):
     """ice3_surrogate (include/ice3.h)."""
-    _fields_ = [("weights_packed", _P), ("params", _P), ("norm", _P), ("budget_tol", ctypes.c_float)]
+    _fields_ = [("weights_packed", _P), ("params", _P), ("norm", _P), ("budget_tol", ctypes.c_float),
+                ("neg_tol", ctypes.c_float)]
 
 
 EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_constants",

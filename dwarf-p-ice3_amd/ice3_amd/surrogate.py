@@ -1,6 +1,6 @@
 """Learned surrogate of the ICE4 microphysics: training on the GPU and the fused step with it.
 
-The surrogate is a per-point MLP (14 features -> 4 x 128 SiLU -> 8 outputs) whose
+The surrogate is a per-point MLP (18 features -> 4 x 128 SiLU -> 8 outputs) whose
 weights run in bf16 on the matrix cores with fp32 accumulation.  It is trained here,
 on the GPU, by the library's own kernels (``ice3_mlp_forward`` / ``ice3_mlp_backward``
 / ``ice3_mlp_adam``), on state -> tendency pairs that the FP64 kernels produce
@@ -9,8 +9,9 @@ on the GPU, by the library's own kernels (``ice3_mlp_forward`` / ``ice3_mlp_back
 
 ``AdjustRainIceSurrogate`` runs the fused step with the network in place of the
 microphysics (``ice3_adjust_rain_ice_surrogate``); every point the network cannot
-vouch for (features outside the training range, a negative source, a water residual
-above ``budget_tol``, or no hydrometeor) is recomputed by the plain FP64 kernel.
+vouch for (features outside the training range, a water residual above ``budget_tol``,
+a source negative beyond ``neg_tol``, or no hydrometeor) is recomputed by the plain FP64
+kernel; accepted points pass the same negativity correction as the plain kernel's.
 
 The surrogate has no counterpart in the reference; it is a build addition beside the
 drop-in components.
@@ -29,7 +30,7 @@ from ._fields import ptr
 
 NFEAT, NTARG = _lib.NFEAT, _lib.NTARG
 FEATURES = ("T", "ln_p", "rhodref", "ln_rv", "ln_rc", "ln_rr", "ln_ri", "ln_rs", "ln_rg", "ln_1p_ci",
-            "sigs", "exn", "dt", "cldfr")
+            "sigs", "exn", "dt", "cldfr", "sed_rr", "sed_ri", "sed_rs", "sed_rg")
 TARGETS = ("d_ths", "d_rvs", "d_rcs", "d_rrs", "d_ris", "d_rss", "d_rgs", "d_ci")
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate.npz")
 
@@ -282,11 +283,13 @@ def evaluate(mlp: MLP, x: torch.Tensor, t: torch.Tensor, batch: int = 1 << 20) -
 class TrainedSurrogate:
     """Trained parameters + normalisation, resident on one device."""
 
-    def __init__(self, params: np.ndarray, norm: dict, budget_tol: float = 0.1, device=None, meta=None):
+    def __init__(self, params: np.ndarray, norm: dict, budget_tol: float = 0.1, neg_tol: float = 0.05,
+                 device=None, meta=None):
         self.mlp = MLP(device=device, params=params)
         self.dev = self.mlp.dev
         self.norm = {k: np.asarray(v, dtype=np.float32) for k, v in norm.items()}
         self.budget_tol = float(budget_tol)
+        self.neg_tol = float(neg_tol)
         self.meta = dict(meta or {})
         cn = _lib.SurrogateNorm()
         for k in ("feat_mean", "feat_scale", "feat_lo", "feat_hi", "targ_unit", "targ_mean", "targ_std"):
@@ -295,11 +298,12 @@ class TrainedSurrogate:
         self.norm_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.dev)
 
     def handle(self) -> "_lib.Surrogate":
-        return _lib.Surrogate(ptr(self.mlp.wpk), ptr(self.mlp.params), ptr(self.norm_dev), self.budget_tol)
+        return _lib.Surrogate(ptr(self.mlp.wpk), ptr(self.mlp.params), ptr(self.norm_dev), self.budget_tol,
+                              self.neg_tol)
 
     def save(self, path: str):
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-        np.savez(path, params=self.mlp.params.cpu().numpy(), budget_tol=self.budget_tol,
+        np.savez(path, params=self.mlp.params.cpu().numpy(), budget_tol=self.budget_tol, neg_tol=self.neg_tol,
                  **{f"norm_{k}": v for k, v in self.norm.items()},
                  **{f"meta_{k}": np.asarray(v) for k, v in self.meta.items()})
 
@@ -308,7 +312,7 @@ class TrainedSurrogate:
         z = np.load(path, allow_pickle=False)
         norm = {k[5:]: z[k] for k in z.files if k.startswith("norm_")}
         meta = {k[5:]: z[k] for k in z.files if k.startswith("meta_")}
-        return cls(z["params"], norm, float(z["budget_tol"]), device=device, meta=meta)
+        return cls(z["params"], norm, float(z["budget_tol"]), float(z["neg_tol"]), device=device, meta=meta)
 
 
 class AdjustRainIceSurrogate:

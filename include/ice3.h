@@ -132,19 +132,22 @@ int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, int nlev, int
                                 const ice3_stats *stats, int stages, void *stream);
 
 /* ---- learned surrogate of the ICE4 microphysics (no counterpart in the reference) ----
- * A per-point MLP (14 features -> 4 x 128 SiLU -> 8 outputs, bf16 weights on the matrix
+ * A per-point MLP (18 features -> 4 x 128 SiLU -> 8 outputs, bf16 weights on the matrix
  * cores, fp32 accumulation) replaces the microphysics stage of the fused step.  Its
  * features are T, ln p, rhodref, ln(r + 1e-9) of rv, rc, rr, ri, rs, rg (t-state),
- * ln(1 + ci), sigma_s, exner, dt and the cloud fraction; its outputs are the changes the
+ * ln(1 + ci), sigma_s, exner, dt, the cloud fraction and asinh(1e6 dt e) of the
+ * sedimentation tendency e of rr, ri, rs, rg; its outputs are the changes the
  * microphysics makes to the sources of th, rv, rc, rr, ri, rs, rg and to ci, each as
  * unit * sinh(mean + std * y).  A point goes back to the plain FP64 microphysics when a
- * feature lies outside the training range, when a resulting source of rv..rg would be
- * negative, when the water residual sum(d rv..d rg) exceeds budget_tol * sum|d r|, or when
- * it has no hydrometeor (the plain kernel's nucleation-only branch).  Accepted points
- * have their water residual moved into rv, so total water is conserved. */
+ * feature lies outside the training range, when the water residual sum(d rv..d rg)
+ * exceeds budget_tol * sum|d r|, when a resulting source of rc..rg is negative by more
+ * than neg_tol * (|source| + |change|), or when it has no hydrometeor (the plain kernel's
+ * nucleation-only branch).  Accepted points have their water residual moved into rv and
+ * then pass ICE4_CORRECT_NEGATIVITIES, as the plain kernel's results do, so total water
+ * is conserved and every source is non-negative. */
 typedef struct {
-  float feat_mean[14], feat_scale[14];   /* x = (f - mean) * scale */
-  float feat_lo[14], feat_hi[14];        /* training range of f */
+  float feat_mean[18], feat_scale[18];   /* x = (f - mean) * scale */
+  float feat_lo[18], feat_hi[18];        /* training range of f */
   float targ_unit[8], targ_mean[8], targ_std[8];
 } ice3_surrogate_norm;
 
@@ -153,6 +156,7 @@ typedef struct {
   const float *params;                   /* fp32 parameters (biases) */
   const ice3_surrogate_norm *norm;       /* device pointer */
   float budget_tol;
+  float neg_tol;
 } ice3_surrogate;
 
 /* Fused step with the surrogate microphysics.  work: nlev*ncol + 1 unsigned ints of
@@ -167,7 +171,7 @@ int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev, int ld, do
                                    void *stream);
 
 /* Surrogate features of the state between the sedimentation and microphysics stages
- * (in/out as after ice3_adjust_rain_ice_stages(ADJUST | SEDIMENTATION)).  raw: [npts][14]
+ * (in/out as after ice3_adjust_rain_ice_stages(ADJUST | SEDIMENTATION)).  raw: [npts][18]
  * fp32 (may be NULL); x: [npts][32] bf16 normalised with norm (both may be NULL);
  * micro: [npts] 1 where the point runs the microphysics (may be NULL).  npts = nlev*ncol,
  * point q = k*ncol + i. */
@@ -197,10 +201,11 @@ int ice3_mlp_adam(int step, float lr, float beta1, float beta2, float eps, float
  * ice3_stage_timing(ctx, 1) makes the three entry points above record HIP events
  * around each of their kernels on the call's stream; ice3_stage_times waits for
  * the last such call and returns its stage durations in milliseconds:
- * ms[0] ICE_ADJUST, ms[1] sedimentation, ms[2] microphysics (0 for a stage the
+ * ms[0] ICE_ADJUST, ms[1] sedimentation, ms[2] microphysics (the network, for the
+ * surrogate step), ms[3] the surrogate step's fallback microphysics (0 for a stage the
  * call did not run).  Off by default; a context's events serve one call at a time. */
 int ice3_stage_timing(ice3_ctx *ctx, int enable);
-int ice3_stage_times(ice3_ctx *ctx, float ms[3]);
+int ice3_stage_times(ice3_ctx *ctx, float ms[4]);
 
 #ifdef __cplusplus
 }

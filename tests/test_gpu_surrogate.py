@@ -6,6 +6,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
+NF = 18  # features used by the surrogate (the rest of the 32 inputs is padding)
 
 
 @pytest.fixture(scope="module")
@@ -19,7 +20,7 @@ def dev():
 def _random_batch(n, seed):
     rng = np.random.Generator(np.random.Philox(seed))
     x = np.zeros((n, 32), dtype=np.float32)
-    x[:, :14] = rng.normal(0, 1, (n, 14))
+    x[:, :NF] = rng.normal(0, 1, (n, NF))
     t = rng.normal(0, 1, (n, 8)).astype(np.float32)
     return x, t
 
@@ -69,7 +70,7 @@ def test_mlp_gradient_matches_cpu(dev):
     print(f"biases: rel L2 {rel_b:.2e}")
     assert rel_b < 2e-2
     # padded rows/columns receive exactly zero gradient
-    assert np.all(g[14 * 128:32 * 128] == 0)
+    assert np.all(g[NF * 128:32 * 128] == 0)
 
 
 def test_mlp_training_reduces_loss(dev):
@@ -78,9 +79,9 @@ def test_mlp_training_reduces_loss(dev):
     rng = np.random.Generator(np.random.Philox(3))
     n = 1 << 16
     x = np.zeros((n, 32), dtype=np.float32)
-    x[:, :14] = rng.normal(0, 1, (n, 14))
-    a = rng.normal(0, 0.5, (14, 8))
-    t = np.tanh(x[:, :14] @ a) + 0.3 * np.sin(2 * x[:, :8])
+    x[:, :NF] = rng.normal(0, 1, (n, NF))
+    a = rng.normal(0, 0.5, (NF, 8))
+    t = np.tanh(x[:, :NF] @ a) + 0.3 * np.sin(2 * x[:, :8])
     xb = torch.as_tensor(x).to(dev).to(torch.bfloat16)
     tt = torch.as_tensor(t, dtype=torch.float32).to(dev)
     mlp = S.MLP(device=0, seed=4)
@@ -102,7 +103,7 @@ def small_surrogate(dev):
     mlp = S.MLP(device=0, seed=1)
     hist = S.train(mlp, x, t, epochs=6, batch=16384, lr=3e-3)
     print("surrogate training", [round(h["train_mse"], 4) for h in hist])
-    return S.TrainedSurrogate(mlp.params.cpu().numpy(), norm, budget_tol=0.1, device=0), norm
+    return S.TrainedSurrogate(mlp.params.cpu().numpy(), norm, budget_tol=0.1, neg_tol=0.05, device=0), norm
 
 
 def test_surrogate_step_fallback_and_budget(dev, small_surrogate):
@@ -123,7 +124,7 @@ def test_surrogate_step_fallback_and_budget(dev, small_surrogate):
     npts = 2048 * 90
     print("surrogate counters", c)
     assert c["network"] + c["out_of_range"] + c["negative"] + c["water_residual"] + c["no_hydrometeor"] == npts
-    assert c["network"] > 0.3 * npts
+    assert c["network"] > 0.1 * npts
     micro = ref["calls"] != 0                 # points that run the ICE4 microphysics
     net = micro & (out["calls"] == 0)          # ... and were taken from the network
     fall = ~net
@@ -142,8 +143,9 @@ def test_surrogate_step_fallback_and_budget(dev, small_surrogate):
     assert berr < 1e-12
 
     # what training gains: relative RMS error against the FP64 step over the points both
-    # networks answered, trained versus untrained (same normalisation, no budget check)
-    untrained = S.TrainedSurrogate(S.init_params(1), norm, budget_tol=1e30, device=0)
+    # networks answered, trained versus untrained (same normalisation, no budget or
+    # negativity check for the untrained one)
+    untrained = S.TrainedSurrogate(S.init_params(1), norm, budget_tol=1e30, neg_tol=1e30, device=0)
     run_u = S.AdjustRainIceSurrogate(untrained)
     out_u = run_u(f, 50.0, return_calls=True)
     both = net & (out_u["calls"] == 0)

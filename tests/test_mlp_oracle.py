@@ -9,6 +9,7 @@ import numpy as np
 from oracle import mlp_oracle as mo
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NF = 18  # features used by the surrogate (the rest of the 32 inputs is padding)
 
 
 def _net(seed=1):
@@ -20,7 +21,7 @@ def _net(seed=1):
         o += k * n
     p[mo.NW:] = rng.normal(0, 0.1, mo.NB)
     x = np.zeros((6, mo.IN))
-    x[:, :14] = rng.normal(0, 1, (6, 14))
+    x[:, :NF] = rng.normal(0, 1, (6, NF))
     t = rng.normal(0, 1, (6, mo.NTARG))
     return p, x, t
 
@@ -33,7 +34,7 @@ def test_backward_matches_finite_differences():
     idx = []
     o = 0
     for l, (k, n) in enumerate(mo.DIMS):
-        rows = 14 if l == 0 else k
+        rows = NF if l == 0 else k
         cols = mo.NTARG if l == 4 else n
         for _ in range(4):
             idx.append(o + rng.integers(rows) * n + rng.integers(cols))
@@ -59,7 +60,7 @@ def test_padding_gets_no_gradient():
     p, x, t = _net(3)
     _, g = mo.loss_and_grad(p, x, t, emulate=True)
     w0 = g[:mo.IN * mo.H].reshape(mo.IN, mo.H)
-    assert np.all(w0[14:] == 0)
+    assert np.all(w0[NF:] == 0)
     o4 = mo.NW - mo.H * mo.OUT
     w4 = g[o4:mo.NW].reshape(mo.H, mo.OUT)
     assert np.all(w4[:, mo.NTARG:] == 0)
