@@ -291,7 +291,7 @@ def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
     ref, out = outputs(), outputs()
     cref = _lib.PipeOut(*[ptr(ref[k]) for k in _lib.PIPE_OUT_FIELDS])
     cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
-    work = torch.empty(npts + 1, dtype=torch.int32, device=dev)
+    work = torch.empty(lib.ice3_surrogate_work_bytes(ncol, nlev), dtype=torch.uint8, device=dev)
     sgc = torch.zeros(5, dtype=torch.int64, device=dev)
     h = sg.handle()
 
@@ -345,7 +345,7 @@ def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
     budget = ((col1 - col0 - pr).abs() / col0).max().item()
     tflops = MLP_FLOP_PER_POINT * npts / (stages["network"] * 1e-3) / 1e12
     return {"columns": ncol, "levels": nlev, "weights": os.path.relpath(S.DEFAULT_WEIGHTS, ROOT),
-            "train_pairs": int(sg.meta.get("pairs", 0)), "budget_tol": sg.budget_tol,
+            "train_pairs": int(sg.meta.get("pairs", 0)), "budget_tol": sg.budget_tol, "neg_tol": sg.neg_tol,
             "fp64_ms_per_step": ms_plain, "fp64_columns_per_s": ncol / (ms_plain * 1e-3),
             "surrogate_ms_per_step": ms_sg, "surrogate_columns_per_s": ncol / (ms_sg * 1e-3),
             "speedup_vs_fp64": ms_plain / ms_sg, "stages_ms": stages,

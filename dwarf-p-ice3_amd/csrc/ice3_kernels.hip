@@ -12,10 +12,11 @@
 //                  never leave registers, and the surface precipitation;
 //   k_micro<R>     one thread per (level, column) point: the ICE4 time-split
 //                  microphysics, total tendencies and negativity correction;
-//   k_surrogate    the learned surrogate of k_micro: a persistent grid, MLP weights
-//                  resident in LDS, 128-point tiles on the bf16 matrix cores; points it
-//                  rejects are queued for k_micro_list, the plain FP64 microphysics over
-//                  a point list.
+//   k_sg_features, k_mlp_infer, k_sg_decode
+//                  the learned surrogate of k_micro: features and range check per point;
+//                  the MLP on the bf16 matrix cores (persistent grid, weights resident in
+//                  LDS); decode, water and sign checks per point.  Points it rejects are
+//                  queued for k_micro_list, the plain FP64 microphysics over a point list.
 // rain_ice is k_sedim then k_micro on one stream; the fused time step is
 // k_ice_adjust, k_sedim, k_micro.  R is the arithmetic of the process rates
 // and lookups (double, or float for the mixed path).
@@ -88,71 +89,97 @@ __global__ void k_zero_u32(unsigned int *p) { *p = 0u; }
 struct SurrogateArgs {
   MicroArgs m;
   unsigned int npts;                 // nlev * ncol; point q = k * ncol + i
-  const bf16x8 *wpk;
-  const float *params;
   const ice3_surrogate_norm *norm;
   float budget_tol, neg_tol;
+  __bf16 *x;                         // [npts][MLP_IN] network inputs
+  float *y;                          // [npts][MLP_NTARG] network outputs
+  unsigned char *flag;               // [npts] SG_* reason, SG_OK = ask the network
   unsigned int *queue;               // [0] count, [1 + q] point offsets
   unsigned long long *counters;      // 5 entries, see include/ice3.h
 };
 
 // Reasons a point is not taken from the network.
-enum { SG_OK = 0, SG_RANGE = 1, SG_NEG = 2, SG_BUDGET = 3, SG_NOMICRO = 4, SG_NONE = 5 };
+enum { SG_OK = 0, SG_RANGE = 1, SG_NEG = 2, SG_BUDGET = 3, SG_NOMICRO = 4 };
 
-__global__ void __launch_bounds__(256) k_surrogate(SurrogateArgs a) {
+__device__ __forceinline__ size_t point_offset(const MicroArgs &m, unsigned int q) {
+  return (size_t)(q / m.ncol) * m.ld + q % m.ncol;
+}
+
+// Stage 1 of the surrogate: features, range check, network inputs (one thread per point).
+__global__ void __launch_bounds__(256) k_sg_features(SurrogateArgs a) {
+  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.npts) return;
+  const ice3_surrogate_norm &nm = *a.norm;
+  float f[MLP_NFEAT];
+  int flag = SG_NOMICRO;
+  if (surrogate_features(a.m, point_offset(a.m, q), f)) {
+    flag = SG_OK;
+#pragma unroll
+    for (int c = 0; c < MLP_NFEAT; ++c)
+      if (!(f[c] >= nm.feat_lo[c] && f[c] <= nm.feat_hi[c])) flag = SG_RANGE;
+  }
+  a.flag[q] = (unsigned char)flag;
+  bf16x8 v[MLP_IN / 8];
+#pragma unroll
+  for (int c = 0; c < MLP_IN; ++c)
+    v[c / 8][c % 8] = (__bf16)(flag == SG_OK && c < MLP_NFEAT ? (f[c] - nm.feat_mean[c]) * nm.feat_scale[c] : 0.f);
+  bf16x8 *xr = (bf16x8 *)(a.x + (size_t)q * MLP_IN);
+#pragma unroll
+  for (int c = 0; c < MLP_IN / 8; ++c) xr[c] = v[c];
+}
+
+// Stage 2: the network over all points.  Persistent grid, all weights resident in LDS,
+// 128-point tiles (32 rows per wave); layer-0 operands come straight from HBM as matrix-
+// core fragments and the next tile's are fetched while the current one is computed.
+__global__ void __launch_bounds__(256) k_mlp_infer(const bf16x8 *__restrict__ wpk,
+                                                   const float *__restrict__ params,
+                                                   const __bf16 *__restrict__ x,
+                                                   float *__restrict__ y, unsigned int npts) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   bf16x8 *sW = (bf16x8 *)smem;
   float *sB = (float *)(smem + MLP_NW * 2);
   __bf16 *sH = (__bf16 *)(smem + MLP_NW * 2 + MLP_NB * 4);
-  __bf16 *sX = sH + MLP_TILE * MLP_HS;
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64, row0 = w * MLP_ROWS;
-  const MicroArgs &m = a.m;
-  const ice3_surrogate_norm &nm = *a.norm;
-  block_copy16(sW, a.wpk, MLP_NW * 2);
-  for (int q = threadIdx.x; q < MLP_NB; q += blockDim.x) sB[q] = a.params[MLP_NW + q];
+  block_copy16(sW, wpk, MLP_NW * 2);
+  for (int q = threadIdx.x; q < MLP_NB; q += blockDim.x) sB[q] = params[MLP_NW + q];
   __syncthreads();
-  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
-  const unsigned int ntiles = (a.npts + MLP_TILE - 1) / MLP_TILE;
-  for (unsigned int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    // features of this wave's rows: lane r < 32 handles row row0 + r
-    int flag = SG_NONE;
-    size_t j = 0;
-    if (lane < MLP_ROWS) {
-      const unsigned int q = tile * MLP_TILE + row0 + lane;
-      __bf16 *xr = sX + (row0 + lane) * MLP_XS;
-      float f[MLP_NFEAT];
-      if (q < a.npts) {
-        j = (size_t)(q / m.ncol) * m.ld + q % m.ncol;
-        if (!surrogate_features(m, j, f)) {
-          flag = SG_NOMICRO;
-        } else {
-          flag = SG_OK;
+  const unsigned int ntiles = (npts + MLP_TILE - 1) / MLP_TILE;
+  // layer-0 A fragments of a tile: rows row0 + mt*16 + lane%16, inputs 8*(lane/16)..+7
+  auto load_a = [&](unsigned int tile, bf16x8 (&a0)[MLP_MT]) {
 #pragma unroll
-          for (int c = 0; c < MLP_NFEAT; ++c)
-            if (!(f[c] >= nm.feat_lo[c] && f[c] <= nm.feat_hi[c])) flag = SG_RANGE;
+    for (int mt = 0; mt < MLP_MT; ++mt) {
+      const unsigned int p = tile * MLP_TILE + row0 + mt * 16 + lane % 16;
+      if (p < npts) a0[mt] = *(const bf16x8 *)(x + (size_t)p * MLP_IN + 8 * (lane / 16));
+      else a0[mt] = bf16x8{};
+    }
+  };
+  bf16x8 a0[MLP_MT];
+  unsigned int tile = blockIdx.x;
+  if (tile < ntiles) load_a(tile, a0);
+  for (; tile < ntiles; tile += gridDim.x) {
+    f32x4 acc[MLP_MT][MLP_H / 16];
+#pragma unroll
+    for (int mt = 0; mt < MLP_MT; ++mt)
+#pragma unroll
+      for (int nn = 0; nn < MLP_H / 16; ++nn) acc[mt][nn] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int nn = 0; nn < MLP_H / 16; ++nn) {
+      const bf16x8 b = sW[mlp_woff(0) / 8 + nn * 64 + lane];
+#pragma unroll
+      for (int mt = 0; mt < MLP_MT; ++mt) acc[mt][nn] = mfma_bf16(a0[mt], b, acc[mt][nn]);
+    }
+    if (tile + gridDim.x < ntiles) load_a(tile + gridDim.x, a0);  // prefetch the next tile
+#pragma unroll
+    for (int mt = 0; mt < MLP_MT; ++mt)
+#pragma unroll
+      for (int nn = 0; nn < MLP_H / 16; ++nn)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = row0 + mt * 16 + 4 * (lane / 16) + i, col = nn * 16 + lane % 16;
+          sH[row * MLP_HS + col] = (__bf16)silu(acc[mt][nn][i] + sB[col]);
         }
-      }
-#pragma unroll
-      for (int c = 0; c < MLP_IN; ++c)
-        xr[c] = (__bf16)(flag == SG_OK && c < MLP_NFEAT ? (f[c] - nm.feat_mean[c]) * nm.feat_scale[c] : 0.f);
-    }
-    // the network on the wave's 32 rows (each wave reads and writes only its own rows)
-    {
-      f32x4 acc[MLP_MT][MLP_H / 16];
-      mlp_layer<MLP_IN, MLP_H, MLP_XS>(sX, row0, sW + mlp_woff(0) / 8, acc, lane);
-#pragma unroll
-      for (int mt = 0; mt < MLP_MT; ++mt)
-#pragma unroll
-        for (int nn = 0; nn < MLP_H / 16; ++nn)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int row = row0 + mt * 16 + 4 * (lane / 16) + i, col = nn * 16 + lane % 16;
-            sH[row * MLP_HS + col] = (__bf16)silu(acc[mt][nn][i] + sB[col]);
-          }
-    }
 #pragma unroll 1
     for (int l = 1; l <= 3; ++l) {
-      f32x4 acc[MLP_MT][MLP_H / 16];
       mlp_layer<MLP_H, MLP_H, MLP_HS>(sH, row0, sW + mlp_woff(l) / 8, acc, lane);
 #pragma unroll
       for (int mt = 0; mt < MLP_MT; ++mt)
@@ -164,48 +191,57 @@ __global__ void __launch_bounds__(256) k_surrogate(SurrogateArgs a) {
             sH[row * MLP_HS + col] = (__bf16)silu(acc[mt][nn][i] + sB[l * MLP_H + col]);
           }
     }
-    // output layer; the outputs (fp32, 9-float rows) overwrite this wave's rows of sH
-    float *sY = (float *)(sH + row0 * MLP_HS);
-    {
-      f32x4 acc[MLP_MT][MLP_OUT / 16];
-      mlp_layer<MLP_H, MLP_OUT, MLP_HS>(sH, row0, sW + mlp_woff(4) / 8, acc, lane);
+    f32x4 out[MLP_MT][MLP_OUT / 16];
+    mlp_layer<MLP_H, MLP_OUT, MLP_HS>(sH, row0, sW + mlp_woff(4) / 8, out, lane);
 #pragma unroll
-      for (int mt = 0; mt < MLP_MT; ++mt)
+    for (int mt = 0; mt < MLP_MT; ++mt)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int r = mt * 16 + 4 * (lane / 16) + i, col = lane % 16;
-          if (col < MLP_NTARG) sY[r * 9 + col] = acc[mt][0][i] + sB[4 * MLP_H + col];
-        }
+      for (int i = 0; i < 4; ++i) {
+        const unsigned int p = tile * MLP_TILE + row0 + mt * 16 + 4 * (lane / 16) + i;
+        const int col = lane % 16;
+        if (col < MLP_NTARG && p < npts) y[(size_t)p * MLP_NTARG + col] = out[mt][0][i] + sB[4 * MLP_H + col];
+      }
+  }
+}
+
+constexpr size_t INFER_LDS = (size_t)MLP_NW * 2 + MLP_NB * 4 + MLP_TILE * MLP_HS * 2;
+
+// Stage 3: decode, physical checks, results or fallback queue (one thread per point).
+__global__ void __launch_bounds__(256) k_sg_decode(SurrogateArgs a) {
+  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x % 64;
+  const MicroArgs &m = a.m;
+  const ice3_surrogate_norm &nm = *a.norm;
+  int flag = -1;
+  size_t j = 0;
+  if (q < a.npts) {
+    flag = a.flag[q];
+    j = point_offset(m, q);
+  }
+  if (flag == SG_OK) {
+    double out[7], y[7], rsum = 0., rmag = 0.;
+    const float *yq = a.y + (size_t)q * MLP_NTARG;
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      y[v] = (double)(nm.targ_unit[v] * sinhf(nm.targ_mean[v] + nm.targ_std[v] * yq[v]));
+      out[v] = m.src[v][j] + y[v];
+      if (v >= 1) { rsum += y[v]; rmag += fabs(y[v]); }
     }
-    // decode, physical checks, results or fallback queue
-    double out[7], cinew = 0.;
-    if (flag == SG_OK) {
-      double rsum = 0., rmag = 0., y[7];
+    const double cinew =
+        fmax(0., m.src_ci[j] + (double)(nm.targ_unit[7] * sinhf(nm.targ_mean[7] + nm.targ_std[7] * yq[7])));
+    if (!(fabs(rsum) <= (double)a.budget_tol * rmag + 1e-18)) {
+      flag = SG_BUDGET;
+    } else {
+      out[IRV] -= rsum;  // move the water residual into vapour
 #pragma unroll
-      for (int v = 0; v < 7; ++v) {
-        const float u = nm.targ_mean[v] + nm.targ_std[v] * sY[lane * 9 + v];
-        y[v] = (double)(nm.targ_unit[v] * sinhf(u));
-        out[v] = m.src[v][j] + y[v];
-        if (v >= 1) { rsum += y[v]; rmag += fabs(y[v]); }
-      }
-      {
-        const float u = nm.targ_mean[7] + nm.targ_std[7] * sY[lane * 9 + 7];
-        cinew = fmax(0., m.src_ci[j] + (double)(nm.targ_unit[7] * sinhf(u)));
-      }
-      if (!(fabs(rsum) <= (double)a.budget_tol * rmag + 1e-18)) {
-        flag = SG_BUDGET;
-      } else {
-        out[IRV] -= rsum;  // move the water residual into vapour
-#pragma unroll
-        for (int v = IRC; v <= IRG; ++v)
-          if (!(out[v] >= -(double)a.neg_tol * (fabs(m.src[v][j]) + fabs(y[v])))) flag = SG_NEG;
-        if (flag == SG_OK) {
-          double lvfact, lsfact;
-          latent_factors(m, j, lvfact, lsfact);
-          correct_negativities(out[ITH], out[IRV], out[IRC], out[IRR], out[IRI], out[IRS],
-                               out[IRG], lvfact, lsfact);
-          if (!(out[IRV] >= 0.)) flag = SG_NEG;
-        }
+      for (int v = IRC; v <= IRG; ++v)
+        if (!(out[v] >= -(double)a.neg_tol * (fabs(m.src[v][j]) + fabs(y[v])))) flag = SG_NEG;
+      if (flag == SG_OK) {
+        double lvfact, lsfact;
+        latent_factors(m, j, lvfact, lsfact);
+        correct_negativities(out[ITH], out[IRV], out[IRC], out[IRR], out[IRI], out[IRS], out[IRG],
+                             lvfact, lsfact);
+        if (!(out[IRV] >= 0.)) flag = SG_NEG;
       }
     }
     if (flag == SG_OK) {
@@ -216,28 +252,37 @@ __global__ void __launch_bounds__(256) k_surrogate(SurrogateArgs a) {
       m.rainfr[j] = 1.;
       if (m.calls) m.calls[j] = 0;
     }
-    if (flag < SG_NONE) ++cnt[flag];
-    // queue the rejected points (one atomic per wave)
-    const bool rej = lane < MLP_ROWS && flag != SG_OK && flag != SG_NONE;
-    const unsigned long long mask = __ballot(rej);
-    if (mask) {
-      unsigned int base = 0;
-      if (lane == 0) base = atomicAdd(&a.queue[0], (unsigned int)__popcll(mask));
-      base = __shfl(base, 0, 64);
-      if (rej) a.queue[1 + base + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned int)j;
-    }
   }
+  // counters and fallback queue (one atomic per wave and reason)
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
-    unsigned long long v = cnt[c];
-#pragma unroll
-    for (int mm = 32; mm >= 1; mm >>= 1) v += __shfl_xor(v, mm, 64);
-    if (lane == 0 && v) atomicAdd(&a.counters[c], v);
+    const unsigned long long mask = __ballot(flag == c);
+    if (lane == 0 && mask) atomicAdd(&a.counters[c], (unsigned long long)__popcll(mask));
+  }
+  const bool rej = flag > SG_OK;
+  const unsigned long long mask = __ballot(rej);
+  if (mask) {
+    unsigned int base = 0;
+    if (lane == 0) base = atomicAdd(&a.queue[0], (unsigned int)__popcll(mask));
+    base = __shfl(base, 0, 64);
+    if (rej) a.queue[1 + base + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned int)j;
   }
 }
 
-constexpr size_t SG_LDS = (size_t)MLP_NW * 2 + MLP_NB * 4 + MLP_TILE * MLP_HS * 2 +
-                          MLP_TILE * MLP_XS * 2;
+// Scratch layout of the surrogate step (bytes, 256-aligned blocks).
+struct SgWork {
+  size_t x, y, flag, queue, total;
+};
+static SgWork sg_work_layout(size_t npts) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  SgWork w;
+  w.x = 0;
+  w.y = w.x + al(npts * MLP_IN * 2);
+  w.flag = w.y + al(npts * MLP_NTARG * 4);
+  w.queue = w.flag + al(npts);
+  w.total = w.queue + al((npts + 1) * 4);
+  return w;
+}
 
 // Surrogate features of the points of MicroArgs (training data and tests).
 __global__ void __launch_bounds__(256) k_features(MicroArgs m, unsigned int npts,
@@ -538,16 +583,22 @@ static int n_cus(int device) {
   return n;
 }
 
+extern "C" size_t ice3_surrogate_work_bytes(int ncol, int nlev) {
+  if (ncol <= 0 || nlev <= 0) return 0;
+  return sg_work_layout((size_t)ncol * nlev).total;
+}
+
 extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev, int ld,
                                               double dt, const ice3_pipeline_in *in,
                                               const ice3_pipeline_out *out,
-                                              const ice3_surrogate *sg, unsigned int *work,
+                                              const ice3_surrogate *sg, void *work,
                                               unsigned long long *sg_counters,
                                               const ice3_stats *stats, void *stream) {
   if (!ctx || !in || !out || !sg || !work || !sg_counters || bad_shape(ncol, nlev, ld, dt))
     return ICE3_EINVAL;
   if (nlev > 65535 || (double)nlev * ncol >= 4.0e9 || (double)nlev * ld >= 4.0e9)
     return ICE3_EINVAL;
+  if (((uintptr_t)work & 255) != 0) return ICE3_EINVAL;
   if (!sg->weights_packed || !sg->params || !sg->norm || !(sg->budget_tol >= 0.f) ||
       !(sg->neg_tol >= 0.f))
     return ICE3_EINVAL;
@@ -558,6 +609,23 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, stats, aa, sa, m);
   hipStream_t s = (hipStream_t)stream;
   const dim3 gs((ncol + 255) / 256), gp((ncol + 255) / 256, nlev);
+  const unsigned int npts = (unsigned int)nlev * (unsigned int)ncol;
+  const SgWork wl = sg_work_layout(npts);
+  unsigned char *wb = (unsigned char *)work;
+  SurrogateArgs g;
+  g.m = m;
+  g.npts = npts;
+  g.norm = sg->norm;
+  g.budget_tol = sg->budget_tol;
+  g.neg_tol = sg->neg_tol;
+  g.x = (__bf16 *)(wb + wl.x);
+  g.y = (float *)(wb + wl.y);
+  g.flag = wb + wl.flag;
+  g.queue = (unsigned int *)(wb + wl.queue);
+  g.counters = sg_counters;
+  const int ncu = n_cus(ctx->device);
+  const unsigned int ntiles = (npts + MLP_TILE - 1) / MLP_TILE;
+  const unsigned int nblk = ntiles < (unsigned int)ncu ? ntiles : (unsigned int)ncu;
   MARK(0);
   hipLaunchKernelGGL(k_ice_adjust, gp, dim3(256), 0, s, aa);
   HIPCHK(hipGetLastError());
@@ -565,26 +633,18 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
   HIPCHK(hipGetLastError());
   MARK(2);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, work);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, g.queue);
   HIPCHK(hipGetLastError());
-  SurrogateArgs g;
-  g.m = m;
-  g.npts = (unsigned int)nlev * (unsigned int)ncol;
-  g.wpk = (const bf16x8 *)sg->weights_packed;
-  g.params = sg->params;
-  g.norm = sg->norm;
-  g.budget_tol = sg->budget_tol;
-  g.neg_tol = sg->neg_tol;
-  g.queue = work;
-  g.counters = sg_counters;
-  const int ncu = n_cus(ctx->device);
-  const unsigned int ntiles = (g.npts + MLP_TILE - 1) / MLP_TILE;
-  const unsigned int nblk = ntiles < (unsigned int)ncu ? ntiles : (unsigned int)ncu;
-  hipLaunchKernelGGL(k_surrogate, dim3(nblk), dim3(256), SG_LDS, s, g);
+  hipLaunchKernelGGL(k_sg_features, dim3((npts + 255) / 256), dim3(256), 0, s, g);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_mlp_infer, dim3(nblk), dim3(256), INFER_LDS, s,
+                     (const bf16x8 *)sg->weights_packed, sg->params, (const __bf16 *)g.x, g.y, npts);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_sg_decode, dim3((npts + 255) / 256), dim3(256), 0, s, g);
   HIPCHK(hipGetLastError());
   MARK(3);
   hipLaunchKernelGGL(k_micro_list<double>, dim3(4 * ncu), dim3(64), 0, s, m,
-                     (const unsigned int *)work);
+                     (const unsigned int *)g.queue);
   HIPCHK(hipGetLastError());
   MARK(4);
   return ICE3_OK;

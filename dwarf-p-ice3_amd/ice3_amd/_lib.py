@@ -68,7 +68,7 @@ class Surrogate(ctypes.Structure):
 EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_constants",
            "ice3_ice_adjust", "ice3_rain_ice", "ice3_adjust_rain_ice", "ice3_stage_timing",
            "ice3_stage_times", "ice3_adjust_rain_ice_stages", "ice3_adjust_rain_ice_surrogate",
-           "ice3_surrogate_features", "ice3_mlp_sizes", "ice3_mlp_forward", "ice3_mlp_backward",
+           "ice3_surrogate_features", "ice3_surrogate_work_bytes", "ice3_mlp_sizes", "ice3_mlp_forward", "ice3_mlp_backward",
            "ice3_mlp_adam")
 
 _lib = None
@@ -103,6 +103,8 @@ def load():
         lib.ice3_adjust_rain_ice_surrogate.argtypes = [
             _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(PipeIn),
             ctypes.POINTER(PipeOut), ctypes.POINTER(Surrogate), _P, _P, ctypes.POINTER(Stats), _P]
+        lib.ice3_surrogate_work_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.ice3_surrogate_work_bytes.restype = ctypes.c_size_t
         lib.ice3_surrogate_features.argtypes = [
             _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(PipeIn),
             ctypes.POINTER(PipeOut), _P, _P, _P, _P, _P]

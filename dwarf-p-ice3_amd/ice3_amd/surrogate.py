@@ -283,7 +283,7 @@ def evaluate(mlp: MLP, x: torch.Tensor, t: torch.Tensor, batch: int = 1 << 20) -
 class TrainedSurrogate:
     """Trained parameters + normalisation, resident on one device."""
 
-    def __init__(self, params: np.ndarray, norm: dict, budget_tol: float = 0.1, neg_tol: float = 0.05,
+    def __init__(self, params: np.ndarray, norm: dict, budget_tol: float = 0.1, neg_tol: float = 0.25,
                  device=None, meta=None):
         self.mlp = MLP(device=device, params=params)
         self.dev = self.mlp.dev
@@ -334,7 +334,7 @@ class AdjustRainIceSurrogate:
         sq = torch.as_tensor(f["sigqsat"], dtype=torch.float64, device=self.dev).reshape(-1)[:ncol].contiguous()
         out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.dev) for k in _lib.PIPE_OUT_FIELDS[:15]}
         out.update({k: torch.empty(ncol, dtype=torch.float64, device=self.dev) for k in _lib.PIPE_OUT_FIELDS[15:]})
-        work = torch.empty(nlev * ncol + 1, dtype=torch.int32, device=self.dev)
+        work = torch.empty(self.lib.ice3_surrogate_work_bytes(ncol, nlev), dtype=torch.uint8, device=self.dev)
         sgc = torch.zeros(5, dtype=torch.int64, device=self.dev)
         counters = torch.zeros(3, dtype=torch.int64, device=self.dev)
         calls = torch.zeros(nlev * ncol, dtype=torch.int32, device=self.dev) if return_calls else None
@@ -367,7 +367,8 @@ def base_col(x, dev):
 
 
 def train_surrogate(n_pairs: int = 50_000_000, epochs: int = 6, batch: int = 32768, lr: float = 2e-3,
-                    ncol: int = 262144, seed: int = 0, budget_tol: float = 0.1, device=None, log=print):
+                    ncol: int = 262144, seed: int = 0, budget_tol: float = 0.1, neg_tol: float = 0.25,
+                    device=None, log=print):
     """Generate n_pairs FP64 pairs, fit the normalisation, train; returns (TrainedSurrogate, history)."""
     import time
     t0 = time.time()
@@ -384,7 +385,7 @@ def train_surrogate(n_pairs: int = 50_000_000, epochs: int = 6, batch: int = 327
     hist = train(mlp, x[nval:], t[nval:], epochs=epochs, batch=batch, lr=lr, seed=seed, log=log, val=(xv, tv))
     torch.cuda.synchronize()
     log(f"training: {epochs} epochs over {x.shape[0] - nval} pairs in {time.time() - t1:.1f} s")
-    sg = TrainedSurrogate(mlp.params.cpu().numpy(), norm, budget_tol=budget_tol, device=device,
+    sg = TrainedSurrogate(mlp.params.cpu().numpy(), norm, budget_tol=budget_tol, neg_tol=neg_tol, device=device,
                           meta={"pairs": x.shape[0], "epochs": epochs, "batch": batch, "lr": lr,
                                 "val_mse": hist[-1]["val_mse"], "train_mse": hist[-1]["train_mse"]})
     return sg, hist

@@ -23,6 +23,8 @@
 #ifndef ICE3_H
 #define ICE3_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -159,14 +161,16 @@ typedef struct {
   float neg_tol;
 } ice3_surrogate;
 
-/* Fused step with the surrogate microphysics.  work: nlev*ncol + 1 unsigned ints of
- * device scratch (fallback queue).  sg_counters (device, 5 entries, accumulated):
+/* Fused step with the surrogate microphysics.  work: device scratch of
+ * ice3_surrogate_work_bytes(ncol, nlev) bytes, 256-byte aligned (network inputs and
+ * outputs, reasons, fallback queue).  sg_counters (device, 5 entries, accumulated):
  * [0] points taken from the network, [1] out of range, [2] negative source,
  * [3] water residual, [4] no hydrometeor.  stats as for ice3_adjust_rain_ice (fallback
  * points only; network points report 0 calls). */
+size_t ice3_surrogate_work_bytes(int ncol, int nlev);
 int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
                                    const ice3_pipeline_in *in, const ice3_pipeline_out *out,
-                                   const ice3_surrogate *sg, unsigned int *work,
+                                   const ice3_surrogate *sg, void *work,
                                    unsigned long long *sg_counters, const ice3_stats *stats,
                                    void *stream);
 
