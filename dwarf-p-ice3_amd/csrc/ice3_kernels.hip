@@ -16,7 +16,7 @@
 //                  the learned surrogate of k_micro: features and range check per point;
 //                  the MLP on the bf16 matrix cores (persistent grid, weights resident in
 //                  LDS); decode, water and sign checks per point.  Points it rejects are
-//                  queued for k_micro_list, the plain FP64 microphysics over a point list.
+//                  queued for k_micro_segments, the plain FP64 microphysics over the queue.
 // rain_ice is k_sedim then k_micro on one stream; the fused time step is
 // k_ice_adjust, k_sedim, k_micro.  R is the arithmetic of the process rates
 // and lookups (double, or float for the mixed path).
@@ -74,17 +74,6 @@ __global__ void __launch_bounds__(256) k_micro(MicroArgs a) {
     }
   }
 }
-
-// Plain microphysics over a list of points (the surrogate's fallback queue):
-// queue[0] = count, queue[1 + q] = point offset.  Grid-stride loop: every wave exits.
-template <class R>
-__global__ void __launch_bounds__(64) k_micro_list(MicroArgs a, const unsigned int *queue) {
-  const unsigned int n = queue[0];
-  for (unsigned int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
-    micro_at<R>(a, (size_t)queue[1 + q]);
-}
-
-__global__ void k_zero_u32(unsigned int *p) { *p = 0u; }
 
 struct SurrogateArgs {
   MicroArgs m;
@@ -206,72 +195,95 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const bf16x8 *__restrict__ wp
 
 constexpr size_t INFER_LDS = (size_t)MLP_NW * 2 + MLP_NB * 4 + MLP_TILE * MLP_HS * 2;
 
-// Stage 3: decode, physical checks, results or fallback queue (one thread per point).
-__global__ void __launch_bounds__(256) k_sg_decode(SurrogateArgs a) {
-  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x % 64;
+// Stage 3: decode, physical checks, results or fallback queue.  Persistent grid: block b
+// owns the contiguous point segment [b*seg, (b+1)*seg) and appends its rejected points to
+// queue[b*seg ...], counting them in qcount[b]; no global atomics but the 5 counters.
+constexpr int SG_MAX_BLOCKS = 65536;
+
+__device__ __forceinline__ int sg_decode_point(const SurrogateArgs &a, unsigned int q, size_t j) {
   const MicroArgs &m = a.m;
   const ice3_surrogate_norm &nm = *a.norm;
-  int flag = -1;
-  size_t j = 0;
-  if (q < a.npts) {
-    flag = a.flag[q];
-    j = point_offset(m, q);
+  int flag = a.flag[q];
+  if (flag != SG_OK) return flag;
+  double out[7], y[7], rsum = 0., rmag = 0.;
+  const float *yq = a.y + (size_t)q * MLP_NTARG;
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    y[v] = (double)(nm.targ_unit[v] * sinhf(nm.targ_mean[v] + nm.targ_std[v] * yq[v]));
+    out[v] = m.src[v][j] + y[v];
+    if (v >= 1) { rsum += y[v]; rmag += fabs(y[v]); }
   }
-  if (flag == SG_OK) {
-    double out[7], y[7], rsum = 0., rmag = 0.;
-    const float *yq = a.y + (size_t)q * MLP_NTARG;
+  if (!(fabs(rsum) <= (double)a.budget_tol * rmag + 1e-18)) return SG_BUDGET;
+  out[IRV] -= rsum;  // move the water residual into vapour
 #pragma unroll
-    for (int v = 0; v < 7; ++v) {
-      y[v] = (double)(nm.targ_unit[v] * sinhf(nm.targ_mean[v] + nm.targ_std[v] * yq[v]));
-      out[v] = m.src[v][j] + y[v];
-      if (v >= 1) { rsum += y[v]; rmag += fabs(y[v]); }
-    }
-    const double cinew =
-        fmax(0., m.src_ci[j] + (double)(nm.targ_unit[7] * sinhf(nm.targ_mean[7] + nm.targ_std[7] * yq[7])));
-    if (!(fabs(rsum) <= (double)a.budget_tol * rmag + 1e-18)) {
-      flag = SG_BUDGET;
-    } else {
-      out[IRV] -= rsum;  // move the water residual into vapour
+  for (int v = IRC; v <= IRG; ++v)
+    if (!(out[v] >= -(double)a.neg_tol * (fabs(m.src[v][j]) + fabs(y[v])))) return SG_NEG;
+  double lvfact, lsfact;
+  latent_factors(m, j, lvfact, lsfact);
+  correct_negativities(out[ITH], out[IRV], out[IRC], out[IRR], out[IRI], out[IRS], out[IRG],
+                       lvfact, lsfact);
+  if (!(out[IRV] >= 0.)) return SG_NEG;
+  const double cinew =
+      fmax(0., m.src_ci[j] + (double)(nm.targ_unit[7] * sinhf(nm.targ_mean[7] + nm.targ_std[7] * yq[7])));
 #pragma unroll
-      for (int v = IRC; v <= IRG; ++v)
-        if (!(out[v] >= -(double)a.neg_tol * (fabs(m.src[v][j]) + fabs(y[v])))) flag = SG_NEG;
-      if (flag == SG_OK) {
-        double lvfact, lsfact;
-        latent_factors(m, j, lvfact, lsfact);
-        correct_negativities(out[ITH], out[IRV], out[IRC], out[IRR], out[IRI], out[IRS], out[IRG],
-                             lvfact, lsfact);
-        if (!(out[IRV] >= 0.)) flag = SG_NEG;
-      }
+  for (int v = 0; v < 7; ++v) m.dst[v][j] = out[v];
+  m.ci[j] = cinew;
+  m.evap3d[j] = 0.;
+  m.rainfr[j] = 1.;
+  if (m.calls) m.calls[j] = 0;
+  return SG_OK;
+}
+
+__global__ void __launch_bounds__(256) k_sg_decode(SurrogateArgs a, unsigned int seg,
+                                                   unsigned int *qcount) {
+  __shared__ unsigned int sWave[MLP_WAVES];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const unsigned int q0 = blockIdx.x * seg;
+  const unsigned int q1 = q0 + seg < a.npts ? q0 + seg : a.npts;
+  unsigned int nq = 0;                 // rejected points of this block so far (uniform)
+  unsigned int cnt[5] = {0, 0, 0, 0, 0};
+  for (unsigned int base = q0; base < q1; base += blockDim.x) {
+    const unsigned int q = base + threadIdx.x;
+    int flag = -1;
+    size_t j = 0;
+    if (q < q1) {
+      j = point_offset(a.m, q);
+      flag = sg_decode_point(a, q, j);
+      ++cnt[flag];
     }
-    if (flag == SG_OK) {
-#pragma unroll
-      for (int v = 0; v < 7; ++v) m.dst[v][j] = out[v];
-      m.ci[j] = cinew;
-      m.evap3d[j] = 0.;
-      m.rainfr[j] = 1.;
-      if (m.calls) m.calls[j] = 0;
-    }
+    const bool rej = flag > SG_OK;
+    const unsigned long long mask = __ballot(rej);
+    if (lane == 0) sWave[w] = (unsigned int)__popcll(mask);
+    __syncthreads();
+    unsigned int off = nq;
+    for (int v = 0; v < w; ++v) off += sWave[v];
+    if (rej) a.queue[1 + q0 + off + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned int)j;
+    for (int v = 0; v < MLP_WAVES; ++v) nq += sWave[v];
+    __syncthreads();
   }
-  // counters and fallback queue (one atomic per wave and reason)
+  if (threadIdx.x == 0) qcount[blockIdx.x] = nq;
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
-    const unsigned long long mask = __ballot(flag == c);
-    if (lane == 0 && mask) atomicAdd(&a.counters[c], (unsigned long long)__popcll(mask));
+    unsigned long long v = cnt[c];
+#pragma unroll
+    for (int mm = 32; mm >= 1; mm >>= 1) v += __shfl_xor(v, mm, 64);
+    if (lane == 0 && v) atomicAdd(&a.counters[c], v);
   }
-  const bool rej = flag > SG_OK;
-  const unsigned long long mask = __ballot(rej);
-  if (mask) {
-    unsigned int base = 0;
-    if (lane == 0) base = atomicAdd(&a.queue[0], (unsigned int)__popcll(mask));
-    base = __shfl(base, 0, 64);
-    if (rej) a.queue[1 + base + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned int)j;
-  }
+}
+
+// Plain microphysics over the surrogate's rejected points: block b runs the qcount[b]
+// points that decode block b queued.
+template <class R>
+__global__ void __launch_bounds__(64) k_micro_segments(MicroArgs a, const unsigned int *queue,
+                                                       const unsigned int *qcount, unsigned int seg) {
+  const unsigned int n = qcount[blockIdx.x];
+  const unsigned int *list = queue + 1 + (size_t)blockIdx.x * seg;
+  for (unsigned int e = threadIdx.x; e < n; e += blockDim.x) micro_at<R>(a, (size_t)list[e]);
 }
 
 // Scratch layout of the surrogate step (bytes, 256-aligned blocks).
 struct SgWork {
-  size_t x, y, flag, queue, total;
+  size_t x, y, flag, queue, qcount, total;
 };
 static SgWork sg_work_layout(size_t npts) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -280,7 +292,8 @@ static SgWork sg_work_layout(size_t npts) {
   w.y = w.x + al(npts * MLP_IN * 2);
   w.flag = w.y + al(npts * MLP_NTARG * 4);
   w.queue = w.flag + al(npts);
-  w.total = w.queue + al((npts + 1) * 4);
+  w.qcount = w.queue + al((npts + 1) * 4);
+  w.total = w.qcount + al((size_t)SG_MAX_BLOCKS * 4);
   return w;
 }
 
@@ -633,18 +646,22 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
   HIPCHK(hipGetLastError());
   MARK(2);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, g.queue);
-  HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_sg_features, dim3((npts + 255) / 256), dim3(256), 0, s, g);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_mlp_infer, dim3(nblk), dim3(256), INFER_LDS, s,
                      (const bf16x8 *)sg->weights_packed, sg->params, (const __bf16 *)g.x, g.y, npts);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_sg_decode, dim3((npts + 255) / 256), dim3(256), 0, s, g);
+  // decode: a persistent grid of at most 4 blocks per CU, one point segment per block
+  unsigned int *qcount = (unsigned int *)(wb + wl.qcount);
+  unsigned int ndec = (unsigned int)(4 * ncu);
+  if (ndec > SG_MAX_BLOCKS) ndec = SG_MAX_BLOCKS;
+  const unsigned int seg = (npts + ndec - 1) / ndec;
+  ndec = (npts + seg - 1) / seg;
+  hipLaunchKernelGGL(k_sg_decode, dim3(ndec), dim3(256), 0, s, g, seg, qcount);
   HIPCHK(hipGetLastError());
   MARK(3);
-  hipLaunchKernelGGL(k_micro_list<double>, dim3(4 * ncu), dim3(64), 0, s, m,
-                     (const unsigned int *)g.queue);
+  hipLaunchKernelGGL(k_micro_segments<double>, dim3(ndec), dim3(64), 0, s, m,
+                     (const unsigned int *)g.queue, (const unsigned int *)qcount, seg);
   HIPCHK(hipGetLastError());
   MARK(4);
   return ICE3_OK;

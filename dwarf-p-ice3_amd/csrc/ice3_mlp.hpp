@@ -57,9 +57,14 @@ __device__ __forceinline__ f32x4 mfma_bf16(const bf16x8 &a, const bf16x8 &b, con
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+// SiLU and its derivative from the hardware exp2 and reciprocal (about 1 ulp in fp32;
+// the activations are rounded to bf16 afterwards).
+__device__ __forceinline__ float sigmoid_fast(float z) {
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.f / (1.f + __expf(-z));
+  const float s = sigmoid_fast(z);
   return s * (1.f + z * (1.f - s));
 }
 
