@@ -196,9 +196,14 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const bf16x8 *__restrict__ wp
 constexpr size_t INFER_LDS = (size_t)MLP_NW * 2 + MLP_NB * 4 + MLP_TILE * MLP_HS * 2;
 
 // Stage 3: decode, physical checks, results or fallback queue.  Persistent grid: block b
-// owns the contiguous point segment [b*seg, (b+1)*seg) and appends its rejected points to
-// queue[b*seg ...], counting them in qcount[b]; no global atomics but the 5 counters.
-constexpr int SG_MAX_BLOCKS = 65536;
+// decodes the 256-point chunks b, b + gridDim.x, ... (spread over all levels, so every
+// block meets the same mix of cheap and costly fallback points) and appends its rejected
+// points to its own queue segment queue[1 + b*seg ...], counting them in qcount[b]; no
+// global atomics but the 5 counters.
+// Decode blocks: a fixed count (8 per CU on a 256-CU MI355X), so that the scratch size
+// depends on the problem size only.  Block b's queue segment holds seg =
+// ceil(chunks / SG_DEC_BLOCKS) * 256 entries; all segments fit in npts + 256 * (SG_DEC_BLOCKS + 1).
+constexpr unsigned int SG_DEC_BLOCKS = 2048;
 
 __device__ __forceinline__ int sg_decode_point(const SurrogateArgs &a, unsigned int q, size_t j) {
   const MicroArgs &m = a.m;
@@ -213,11 +218,18 @@ __device__ __forceinline__ int sg_decode_point(const SurrogateArgs &a, unsigned 
     out[v] = m.src[v][j] + y[v];
     if (v >= 1) { rsum += y[v]; rmag += fabs(y[v]); }
   }
-  if (!(fabs(rsum) <= (double)a.budget_tol * rmag + 1e-18)) return SG_BUDGET;
+  // water residual against the predicted changes, with a floor of budget_tol times the
+  // smallest typical change (median |change| of the training targets)
+  double umin = nm.targ_unit[IRV];
+#pragma unroll
+  for (int v = IRC; v <= IRG; ++v) umin = fmin(umin, (double)nm.targ_unit[v]);
+  if (!(fabs(rsum) <= (double)a.budget_tol * (rmag + umin))) return SG_BUDGET;
   out[IRV] -= rsum;  // move the water residual into vapour
+  // a source may go negative by neg_tol of (its value + its change + the typical change)
 #pragma unroll
   for (int v = IRC; v <= IRG; ++v)
-    if (!(out[v] >= -(double)a.neg_tol * (fabs(m.src[v][j]) + fabs(y[v])))) return SG_NEG;
+    if (!(out[v] >= -(double)a.neg_tol * (fabs(m.src[v][j]) + fabs(y[v]) + (double)nm.targ_unit[v])))
+      return SG_NEG;
   double lvfact, lsfact;
   latent_factors(m, j, lvfact, lsfact);
   correct_negativities(out[ITH], out[IRV], out[IRC], out[IRR], out[IRI], out[IRS], out[IRG],
@@ -238,15 +250,14 @@ __global__ void __launch_bounds__(256) k_sg_decode(SurrogateArgs a, unsigned int
                                                    unsigned int *qcount) {
   __shared__ unsigned int sWave[MLP_WAVES];
   const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
-  const unsigned int q0 = blockIdx.x * seg;
-  const unsigned int q1 = q0 + seg < a.npts ? q0 + seg : a.npts;
+  const unsigned int q0 = blockIdx.x * seg;   // this block's queue segment
   unsigned int nq = 0;                 // rejected points of this block so far (uniform)
   unsigned int cnt[5] = {0, 0, 0, 0, 0};
-  for (unsigned int base = q0; base < q1; base += blockDim.x) {
+  for (unsigned int base = blockIdx.x * blockDim.x; base < a.npts; base += gridDim.x * blockDim.x) {
     const unsigned int q = base + threadIdx.x;
     int flag = -1;
     size_t j = 0;
-    if (q < q1) {
+    if (q < a.npts) {
       j = point_offset(a.m, q);
       flag = sg_decode_point(a, q, j);
       ++cnt[flag];
@@ -292,8 +303,8 @@ static SgWork sg_work_layout(size_t npts) {
   w.y = w.x + al(npts * MLP_IN * 2);
   w.flag = w.y + al(npts * MLP_NTARG * 4);
   w.queue = w.flag + al(npts);
-  w.qcount = w.queue + al((npts + 1) * 4);
-  w.total = w.qcount + al((size_t)SG_MAX_BLOCKS * 4);
+  w.qcount = w.queue + al((npts + 1 + 256 * ((size_t)SG_DEC_BLOCKS + 1)) * 4);
+  w.total = w.qcount + al((size_t)SG_DEC_BLOCKS * 4);
   return w;
 }
 
@@ -651,12 +662,12 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   hipLaunchKernelGGL(k_mlp_infer, dim3(nblk), dim3(256), INFER_LDS, s,
                      (const bf16x8 *)sg->weights_packed, sg->params, (const __bf16 *)g.x, g.y, npts);
   HIPCHK(hipGetLastError());
-  // decode: a persistent grid of at most 4 blocks per CU, one point segment per block
+  // decode: a persistent grid of SG_DEC_BLOCKS blocks; block b decodes chunks
+  // b, b + ndec, ... of 256 points and owns a queue segment of seg entries
   unsigned int *qcount = (unsigned int *)(wb + wl.qcount);
-  unsigned int ndec = (unsigned int)(4 * ncu);
-  if (ndec > SG_MAX_BLOCKS) ndec = SG_MAX_BLOCKS;
-  const unsigned int seg = (npts + ndec - 1) / ndec;
-  ndec = (npts + seg - 1) / seg;
+  const unsigned int nchunks = (npts + 255) / 256;
+  const unsigned int ndec = nchunks < SG_DEC_BLOCKS ? nchunks : SG_DEC_BLOCKS;
+  const unsigned int seg = ((nchunks + ndec - 1) / ndec) * 256;
   hipLaunchKernelGGL(k_sg_decode, dim3(ndec), dim3(256), 0, s, g, seg, qcount);
   HIPCHK(hipGetLastError());
   MARK(3);
