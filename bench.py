@@ -124,7 +124,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(columns are independent: a multi-GPU run is ncol/N columns per GPU, see DESIGN.md)",
+              "(columns are independent: a multi-GPU run would give each GPU ncol/N columns)",
               file=sys.stderr)
         sys.exit(2)
 

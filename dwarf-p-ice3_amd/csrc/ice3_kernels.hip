@@ -352,8 +352,12 @@ extern "C" int ice3_ctx_create(ice3_ctx **out) {
   ice3_ctx *c = (ice3_ctx *)calloc(1, sizeof(ice3_ctx));
   if (!c) return ICE3_EINVAL;
   ice3_build_params(&c->params, c->h_tab);
+  // the kernels carry the constants as literals (P); they must be the host builder's
+  if (memcmp(&c->params, &P, sizeof(ice3_params)) != 0) {
+    free(c);
+    return ICE3_ECONFIG;
+  }
   if (hipGetDevice(&c->device) != hipSuccess ||
-      hipMemcpyToSymbol(HIP_SYMBOL(c_P), &c->params, sizeof(ice3_params)) != hipSuccess ||
       hipMalloc((void **)&c->d_tab, sizeof(double) * ICE3_T_TOTAL) != hipSuccess ||
       hipMemcpy(c->d_tab, c->h_tab, sizeof(double) * ICE3_T_TOTAL, hipMemcpyHostToDevice) !=
           hipSuccess) {
@@ -405,6 +409,7 @@ extern "C" const char *ice3_strerror(int code) {
     case ICE3_EINVAL: return "invalid argument (size, stride, precision or NULL pointer)";
     case ICE3_EDEVICE: return "HIP runtime error";
     case ICE3_ESUBSTEP: return "a point reached the sub-step cap of the ICE4 time splitting";
+    case ICE3_ECONFIG: return "constants compiled into the kernels differ from the host builder (rebuild)";
     default: return "unknown error";
   }
 }

@@ -16,12 +16,13 @@
 #include <hip/hip_runtime.h>
 
 #include "ice3_params.h"
+#include "ice3_params_gen.inc"
 
 namespace ice3 {
 
-// Device copy of the constants (one per device), filled by ice3_ctx_create.
-// This header is compiled in exactly one translation unit (ice3_kernels.hip).
-__constant__ ice3_params c_P;
+// The constants of the pinned configuration are the compile-time object P
+// (ice3_params_gen.inc, generated from ice3_build_params): every P.X below is a
+// literal in the generated code.
 
 enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6 };
 
@@ -78,7 +79,6 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
                                         double rg, double cf_mf, double rc_mf, double ri_mf,
                                         double &rvs, double &rcs, double &ris, double &ths,
                                         AdjustOut &o) {
-  const ice3_params &P = c_P;
   double zt = th * exn;
   const double lv = P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT);
   const double ls = P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT);
@@ -198,7 +198,6 @@ template <class R>
 __device__ __forceinline__ void nucleation(bool ldcompute, double tht, double pabst, double rhodref,
                                   double exn, double lsfact, double t, double rvt, double &cit,
                                   double &rvheni_mr) {
-  const ice3_params &P = c_P;
   const bool gnegt = ldcompute && t < P.XTT && rvt > P.XRTMIN[1];
   rvheni_mr = 0.;
   if (!gnegt) return;
@@ -292,7 +291,6 @@ template <class R> struct PointConst {
 
 template <class R>
 __device__ __forceinline__ PointConst<R> point_const(double rhodref) {
-  const ice3_params &P = c_P;
   PointConst<R> c;
   c.lnrho = rlog<R>((R)rhodref);
   c.rhomcexvt = rexp<R>(-(R)P.XCEXVT * c.lnrho);
@@ -316,7 +314,6 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
                                   double &cit, double pt, const double *pvart,
                                   StepState<R> &S, const PointConst<R> &pc,
                                   const double *__restrict__ tab, double *pa, double *pb) {
-  const ice3_params &P = c_P;
   double zt = pt;
   double zv[7];
 #pragma unroll
@@ -730,7 +727,6 @@ __device__ __forceinline__ int stepping_point(double dt, double exn, double rhod
                                      double cf, double sigs, double &cit, double *vart,
                                      double *extpk, double &rrevav,
                                      const double *__restrict__ tab) {
-  const ice3_params &P = c_P;
   StepState<R> S;
   const PointConst<R> pc = point_const<R>(rhodref);
   const double zinv = 1. / dt;
@@ -785,7 +781,6 @@ __device__ __forceinline__ int stepping_point(double dt, double exn, double rhod
 // lnrho = ln(rhodref) of the level; the powers are evaluated as exp(c*ln x).
 template <class R>
 __device__ __forceinline__ R fall_speed(int jrr, double rx, R lnrho) {
-  const ice3_params &P = c_P;
   if (jrr == 4) {
     if (!(rx > dmax(P.XRTMIN[4], 1.0E-7))) return (R)0;
     const R lnrx = rlog<R>((R)rx);
@@ -807,7 +802,6 @@ __device__ __forceinline__ R fall_speed(int jrr, double rx, R lnrho) {
 template <class R>
 __device__ __forceinline__ void sediment_species(int jrr, double dt, double rho, R lnrho, double dz,
                                                  double &rs, double &flux) {
-  const ice3_params &P = c_P;
   const double zinv = 1. / dt;
   const double tsorhodz = dt / (rho * dz);
   const double rx = rs * dt;

@@ -84,10 +84,10 @@ __device__ __forceinline__ void sedim_column(const SedimArgs &a, int i) {
     a.dst_rr[j] = rrs; a.dst_ri[j] = ris; a.dst_rs[j] = rss; a.dst_rg[j] = rgs;
   }
   a.inprc[i] = 0.;
-  a.inprr[i] = fr / c_P.XRHOLW;
-  a.inpri[i] = fi / c_P.XRHOLW;
-  a.inprs[i] = fs / c_P.XRHOLW + fi / c_P.XRHOLW;
-  a.inprg[i] = fg / c_P.XRHOLW;
+  a.inprr[i] = fr / P.XRHOLW;
+  a.inpri[i] = fi / P.XRHOLW;
+  a.inprs[i] = fs / P.XRHOLW + fi / P.XRHOLW;
+  a.inprg[i] = fg / P.XRHOLW;
 }
 
 // Microphysics stage arguments.  The t-state of species v is t[v][j] * tscale
@@ -114,7 +114,6 @@ struct MicroArgs {
 __device__ __forceinline__ void correct_negativities(double &ths, double &rvs, double &rcs,
                                                      double &rrs, double &ris, double &rss,
                                                      double &rgs, double lvfact, double lsfact) {
-  const ice3_params &P = c_P;
   double zw;
   zw = rcs - dmax(rcs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rcs = rcs - zw;
   zw = rrs - dmax(rrs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rrs = rrs - zw;
@@ -139,7 +138,6 @@ __device__ __forceinline__ void correct_negativities(double &ths, double &rvs, d
 // sub-step cap was hit.
 template <class R>
 __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j) {
-  const ice3_params &P = c_P;
   const double dt = a.dt, zinv = 1. / dt, ts = a.tscale;
   const double exn = a.exn[j], rhodref = a.rhodref[j], exnref = a.exn_ref[j];
   const double pabs = a.pabs[j], cldfr = a.cldfr[j], sigs = a.sigs[j];
@@ -201,7 +199,6 @@ __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j) {
 // splitting integrates beside the processes).  Returns false where RAIN_ICE would not
 // run the ICE4 microphysics (no hydrometeor above XRTMIN), as micro_at decides it.
 __device__ __forceinline__ bool surrogate_features(const MicroArgs &a, size_t j, float *f) {
-  const ice3_params &P = c_P;
   const double ts = a.tscale, exn = a.exn[j];
   double r[7];
 #pragma unroll
@@ -227,7 +224,6 @@ __device__ __forceinline__ bool surrogate_features(const MicroArgs &a, size_t j,
 // micro_at computes them (micro/rain_ice.F90:334-347 divided by PEXNREF).
 __device__ __forceinline__ void latent_factors(const MicroArgs &a, size_t j, double &lvfact,
                                                double &lsfact) {
-  const ice3_params &P = c_P;
   const double ts = a.tscale;
   const double tht = a.t[ITH][j] * ts, rvt = a.t[IRV][j] * ts, rct = a.t[IRC][j] * ts;
   const double rrt = a.t[IRR][j] * ts, rit = a.t[IRI][j] * ts, rst = a.t[IRS][j] * ts;

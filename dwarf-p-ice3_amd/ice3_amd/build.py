@@ -13,14 +13,21 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ("ice3_kernels.hip", "ice3_mlp.hip", "ice3_params.cpp")
 # -ffp-contract=off keeps FP64 a*b+c as two roundings, the sequence of the CPU reference.
-# Packed FP32 instructions are disabled for the device code (hardware notes).
+# Packed FP32 instructions (v_pk_*_f32) are disabled for the device code: some of their
+# forms have given wrong results on this hardware without any error.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + os.path.join(ROOT, "include")]
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source into ice3_amd/_lib/libice3_hip.so; returns its path."""
+    """Regenerate the constants header from the host builder, then compile every HIP
+    source into ice3_amd/_lib/libice3_hip.so; returns its path."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen_params_inc", os.path.join(CSRC, "gen_params_inc.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    gen.generate(os.path.join(CSRC, "ice3_params_gen.inc"))
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     tmp = OUT + ".tmp"
     cmd = [HIPCC, *FLAGS, *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]

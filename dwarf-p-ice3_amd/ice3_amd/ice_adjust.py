@@ -32,8 +32,9 @@ class IceAdjust:
 
     Parameters
     ----------
-    phyex : ignored placeholder for API compatibility (the configuration is the pinned
-        AROME ICE3 one); anything but None/"AROME" raises.
+    phyex : placeholder for API compatibility: the configuration is the pinned AROME
+        ICE3 one (KRR=6, sigma_s subgrid condensation, CB02); anything but None/"AROME"
+        raises.
     jit : accepted for API compatibility (kernels are compiled ahead of time).
     layout : "columns" for (nlev, ncol) arrays with columns contiguous (the Fortran
         (NIJT, NKT) memory, no copy), or "jax" for the JAX components' (nx, ny, nz).

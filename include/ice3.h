@@ -34,6 +34,7 @@ extern "C" {
 #define ICE3_EINVAL (-1)     /* bad size, stride, precision or NULL pointer */
 #define ICE3_EDEVICE (-2)    /* HIP runtime error (launch, allocation, copy) */
 #define ICE3_ESUBSTEP (-3)   /* reported through ice3_stats: a point hit the sub-step cap */
+#define ICE3_ECONFIG (-4)    /* kernels built with constants other than the host builder's */
 
 /* arithmetic of the microphysics kernels */
 #define ICE3_PREC_FP64 0     /* FP64 everywhere: parity with the FP64 reference */

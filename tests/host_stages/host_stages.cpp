@@ -18,7 +18,9 @@ extern "C" int hst_init(void) {
   if (!g_tab) {
     g_tab = (double *)malloc(sizeof(double) * ICE3_T_TOTAL);
     if (!g_tab) return -1;
-    ice3_build_params(&c_P, g_tab);
+    ice3_params p;
+    ice3_build_params(&p, g_tab);
+    if (memcmp(&p, &P, sizeof p) != 0) return -2;  // stale generated constants
   }
   return 0;
 }
