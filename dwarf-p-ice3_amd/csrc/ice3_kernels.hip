@@ -57,40 +57,12 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
-// One block = 256 consecutive columns of one level.  The block first sorts its points by
-// regime class (micro_class) in LDS, so that each wavefront runs points that switch on the
-// same process blocks, then thread t runs the t-th point of that order.
 template <class R>
 __global__ void __launch_bounds__(256) k_micro(MicroArgs a) {
-  __shared__ unsigned int sCount[64];
-  __shared__ unsigned short sPerm[256];
-  const int t = threadIdx.x;
-  const int i0 = blockIdx.x * blockDim.x;
-  const size_t row = (size_t)blockIdx.y * a.ld;
-  if (t < 64) sCount[t] = 0u;
-  __syncthreads();
-  int key = -1;
-  unsigned int rank = 0;
-  if (i0 + t < a.ncol) {
-    key = micro_class(a, row + i0 + t);
-    rank = atomicAdd(&sCount[key], 1u);
-  }
-  __syncthreads();
-  if (t < 64) {  // exclusive prefix sum of the 64 class counts (one wavefront)
-    const unsigned int c = sCount[t];
-    unsigned int v = c;
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-      const unsigned int u = __shfl_up(v, m, 64);
-      if (t >= m) v += u;
-    }
-    sCount[t] = v - c;
-  }
-  __syncthreads();
-  if (key >= 0) sPerm[sCount[key] + rank] = (unsigned short)t;
-  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
   int calls = 0;
-  if (i0 + t < a.ncol) calls = micro_at<R>(a, row + i0 + sPerm[t]);
+  if (i < a.ncol) calls = micro_at<R>(a, (size_t)k * a.ld + i);
   if (a.counters) {  // uniform branch: every lane of the wave takes part in the sums
     const int n = wave_sum(calls < 0 ? -calls : calls);
     const int c = wave_sum(calls < 0 ? 1 : 0);

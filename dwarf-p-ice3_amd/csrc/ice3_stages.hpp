@@ -192,19 +192,6 @@ __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j) {
   return calls;
 }
 
-// Regime class of point j for grouping similar points into the same wavefronts: which
-// hydrometeors of the t-state exceed XRTMIN (the conditions that switch the costly process
-// blocks of ICE4_TENDENCIES on and off), most expensive first; 32 + ... for points that run
-// the ICE4 microphysics at all.  A scheduling hint only: results do not depend on it.
-__device__ __forceinline__ int micro_class(const MicroArgs &a, size_t j) {
-  const double ts = a.tscale;
-  const bool rc = a.t[IRC][j] * ts > P.XRTMIN[2], rr = a.t[IRR][j] * ts > P.XRTMIN[3];
-  const bool ri = a.t[IRI][j] * ts > P.XRTMIN[4], rs = a.t[IRS][j] * ts > P.XRTMIN[5];
-  const bool rg = a.t[IRG][j] * ts > P.XRTMIN[6];
-  const bool micro = rc || rr || ri || rs || rg;
-  return (micro ? 32 : 0) | (rs ? 16 : 0) | (rg ? 8 : 0) | (rr ? 4 : 0) | (rc ? 2 : 0) | (ri ? 1 : 0);
-}
-
 // Input features of the surrogate at point j (the microphysics inputs of micro_at):
 // T, ln p, rhodref, ln(r + 1e-9) of the six water species of the t-state, ln(1 + ci),
 // sigma_s, exner, dt, cloud fraction, and asinh(1e6 * dt * e) of the external tendency
