@@ -414,6 +414,19 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     }
   }
   const R rhomcexvt = soft ? (R)0 : pc.rhomcexvt;
+  // powers of the snow and graupel slopes used by several processes of the hard call
+  // (vapour deposition, freezing budget, melting; accretion and dry growth)
+  R vent_s = 0, vent_g = 0, lbs_cxs_bs = 0, lbg_cxg = 0;
+  if (!soft) {
+    if (rs > (R)0.) {
+      vent_s = (R)P.X0DEPS * powl_<R>(lls, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * powl_<R>(lls, (R)P.XEX1DEPS);
+      lbs_cxs_bs = powl_<R>(lls, (R)(P.XCXS - P.XBS));
+    }
+    if (rg > (R)0.) {
+      vent_g = (R)P.X0DEPG * powl_<R>(llg, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * powl_<R>(llg, (R)P.XEX1DEPG);
+      lbg_cxg = powl_<R>(llg, (R)P.XCXG);
+    }
+  }
   const R ssi = S.ssi;
   const bool rv_ok = zv[IRV] > P.XRTMIN[1], rc_ok = zv[IRC] > P.XRTMIN[2],
              rr_ok = zv[IRR] > P.XRTMIN[3], ri_ok = zv[IRI] > P.XRTMIN[4],
@@ -424,7 +437,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     if (!soft) S.rchoni = dmin((R)1000., (R)P.XHON * rho * rc * rexp<R>((R)P.XALPHA3 * (t - (R)P.XTT) - (R)P.XBETA3));
   } else S.rchoni = 0;
   if (rv_ok && rs_ok) {
-    if (!soft) S.rvdeps = (ssi / (rho * ai)) * ((R)P.X0DEPS * powl_<R>(lls, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * powl_<R>(lls, (R)P.XEX1DEPS));
+    if (!soft) S.rvdeps = (ssi / (rho * ai)) * vent_s;
   } else S.rvdeps = 0;
   if (ri_ok && rs_ok) {
     if (!soft) S.riaggs = (R)P.XFIAGGS * rexp<R>((R)P.XCOLEXIS * (t - (R)P.XTT)) * ri * powl_<R>(lls, (R)P.XEXIAGGS) * rhomcexvt;
@@ -435,7 +448,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     }
   } else S.riauts = 0;
   if (rv_ok && rg_ok) {
-    if (!soft) S.rvdepg = (ssi / (rho * ai)) * ((R)P.X0DEPG * powl_<R>(llg, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * powl_<R>(llg, (R)P.XEX1DEPG));
+    if (!soft) S.rvdepg = (ssi / (rho * ai)) * vent_g;
   } else S.rvdepg = 0;
 
   // ICE4_WARM (micro/mode_ice4_warm.F90:100-197)
@@ -461,7 +474,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       R f1 = rv * (R)pres / ((R)P.XEPSILO + rv);
       f1 = dmin(f1, esi);
       f1 = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - f1) / ((R)P.XRV * t));
-      f1 = f1 * ((R)P.X0DEPS * powl_<R>(lls, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * powl_<R>(lls, (R)P.XEX1DEPS)) /
+      f1 = f1 * vent_s /
            (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
       S.rs_freez1 = f1;
       S.rs_freez2 = (rho * ((R)P.XLMTT + (R)(P.XCI - P.XCL) * ((R)P.XTT - t))) / (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
@@ -510,7 +523,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
                     ((R)P.XLBRACCS1 / (sq(lbdas)) + (R)P.XLBRACCS2 / (lbdas * lbdar) + (R)P.XLBRACCS3 / (lr2)) / (lr2 * lr2);
       S.rs_rraccss = zzw1 * zzw;
       S.rs_rraccs = zzw2 * zzw;
-      S.rs_rsaccrg = (R)P.XFSACCRG * zzw3 * (powl_<R>(lls, (R)(P.XCXS - P.XBS))) * rhom *
+      S.rs_rsaccrg = (R)P.XFSACCRG * zzw3 * lbs_cxs_bs * rhom *
                      ((R)P.XLBSACCR1 / (lr2) + (R)P.XLBSACCR2 / (lbdar * lbdas) + (R)P.XLBSACCR3 / (sq(lbdas))) / lbdar;
     }
   }
@@ -530,7 +543,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       R m = rv * (R)pres / ((R)P.XEPSILO + rv);
       m = dmin(m, esw);
       m = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - m) / ((R)P.XRV * t));
-      S.rsmltg = (R)P.XFSCVMG * dmax((R)0., (-m * ((R)P.X0DEPS * powl_<R>(lls, (R)P.XEX0DEPS) + (R)P.X1DEPS * cj * powl_<R>(lls, (R)P.XEX1DEPS)) -
+      S.rsmltg = (R)P.XFSCVMG * dmax((R)0., (-m * vent_s -
                                               (S.rs_rcrims + S.rs_rraccs) * (rho * (R)P.XCL * ((R)P.XTT - t))) / (rho * (R)P.XLMTT));
       S.rcmltsr = S.rs_rcrims;
     }
@@ -571,7 +584,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     const Idx2<R> ix = interp2d_idx<R>(llg, lls, P.NDRYLBDAG, P.NDRYLBDAS, (R)P.XDRYINTP1G,
                                        (R)P.XDRYINTP2G, (R)P.XDRYINTP1S, (R)P.XDRYINTP2S);
     const R zzw = interp2d<R>(ix, P.NDRYLBDAG, tab + ICE3_T_SDRYG);
-    S.rg_rswetg = (R)P.XFSDRYG * zzw / (R)P.XCOLSG * (powl_<R>(lls, (R)(P.XCXS - P.XBS))) * (powl_<R>(llg, (R)P.XCXG)) * rhom1 *
+    S.rg_rswetg = (R)P.XFSDRYG * zzw / (R)P.XCOLSG * lbs_cxs_bs * lbg_cxg * rhom1 *
                   ((R)P.XLBSDRYG1 / (sq(lbdag)) + (R)P.XLBSDRYG2 / (lbdag * lbdas) + (R)P.XLBSDRYG3 / (sq(lbdas)));
     S.rg_rsdryg = S.rg_rswetg * (R)P.XCOLSG * rexp<R>((R)P.XCOLEXSG * (t - (R)P.XTT));
   }
@@ -582,7 +595,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
                                        (R)P.XDRYINTP2G, (R)P.XDRYINTP1R, (R)P.XDRYINTP2R);
     const R zzw = interp2d<R>(ix, P.NDRYLBDAG, tab + ICE3_T_RDRYG);
     const R lr2 = lbdar * lbdar;
-    S.rg_rrdryg = (R)P.XFRDRYG * zzw * ((R)1.0 / (lr2 * lr2)) * (powl_<R>(llg, (R)P.XCXG)) * rhom1 *
+    S.rg_rrdryg = (R)P.XFRDRYG * zzw * ((R)1.0 / (lr2 * lr2)) * lbg_cxg * rhom1 *
                   ((R)P.XLBRDRYG1 / (sq(lbdag)) + (R)P.XLBRDRYG2 / (lbdag * lbdar) + (R)P.XLBRDRYG3 / (lr2));
   }
   const R rdryg_init = S.rg_rcdryg + S.rg_ridryg + S.rg_rsdryg + S.rg_rrdryg;
@@ -593,7 +606,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       R f1 = rv * (R)pres / ((R)P.XEPSILO + rv);
       f1 = dmin(f1, esi);
       f1 = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - f1) / ((R)P.XRV * t));
-      f1 = f1 * ((R)P.X0DEPG * powl_<R>(llg, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * powl_<R>(llg, (R)P.XEX1DEPG)) /
+      f1 = f1 * vent_g /
            (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
       S.rg_freez1 = f1;
       S.rg_freez2 = (rho * ((R)P.XLMTT + (R)(P.XCI - P.XCL) * ((R)P.XTT - t))) / (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
@@ -619,7 +632,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       R m = rv * (R)pres / ((R)P.XEPSILO + rv);
       m = dmin(m, esw);
       m = ka * ((R)P.XTT - t) + dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - m) / ((R)P.XRV * t);
-      S.rgmltr = dmax((R)0., (-m * ((R)P.X0DEPG * powl_<R>(llg, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * powl_<R>(llg, (R)P.XEX1DEPG)) -
+      S.rgmltr = dmax((R)0., (-m * vent_g -
                               (S.rg_rcdryg + S.rg_rrdryg) * (rho * (R)P.XCL * ((R)P.XTT - t))) / (rho * (R)P.XLMTT));
     }
   } else S.rgmltr = 0;
