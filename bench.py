@@ -185,7 +185,7 @@ def main():
         cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
         _lib.check(lib.ice3_stage_timing(ctx, 1), "ice3_stage_timing")
         acc = [0.0, 0.0, 0.0]
-        ms3 = (ctypes.c_float * 4)()
+        ms3 = (ctypes.c_float * 6)()
         for _ in range(steps):
             _lib.check(call(prec, cout), "timed step")
             _lib.check(lib.ice3_stage_times(ctx, ms3), "ice3_stage_times")
@@ -318,14 +318,15 @@ def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
     ms_plain = timed(plain)
     ms_sg = timed(surrogate)
     _lib.check(lib.ice3_stage_timing(ctx, 1), "ice3_stage_timing")
-    acc = [0.0] * 4
-    ms4 = (ctypes.c_float * 4)()
+    acc = [0.0] * 6
+    ms4 = (ctypes.c_float * 6)()
     for _ in range(args.steps):
         _lib.check(surrogate(), "timed surrogate step")
         _lib.check(lib.ice3_stage_times(ctx, ms4), "ice3_stage_times")
         acc = [x + y for x, y in zip(acc, ms4)]
     _lib.check(lib.ice3_stage_timing(ctx, 0), "ice3_stage_timing")
-    stages = dict(zip(("ice_adjust", "sedimentation", "network", "fallback"), [x / args.steps for x in acc]))
+    stages = dict(zip(("ice_adjust", "sedimentation", "features", "network", "decode", "fallback"),
+                      [x / args.steps for x in acc]))
     sgc.zero_()
     _lib.check(surrogate(), "surrogate step")
     torch.cuda.synchronize()
@@ -352,7 +353,7 @@ def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
             "points": npts, "network_points": c[0], "fallback": {"out_of_range": c[1], "negative_source": c[2],
                                                                  "water_residual": c[3], "no_hydrometeor": c[4]},
             "network_fraction": c[0] / npts, "water_budget_max_rel_err": budget, "error_vs_fp64": err,
-            "roofline": {"bound": "mfma", "kernel": "k_surrogate", "achieved": tflops,
+            "roofline": {"bound": "mfma", "kernel": "k_mlp_infer", "achieved": tflops,
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
                          "flop_per_point": MLP_FLOP_PER_POINT, "ms_per_launch": stages["network"]}}
 

@@ -336,8 +336,9 @@ struct ice3_ctx {
   double *d_tab;
   double h_tab[ICE3_T_TOTAL];
   int timing;         // record ev[] around the stages of each call
-  hipEvent_t ev[5];   // before adjust, after adjust, after sedimentation, after microphysics
-                      // (the network, for the surrogate step), after the fallback
+  hipEvent_t ev[7];   // before adjust, after adjust, after sedimentation, then after the
+                      // microphysics (plain step: events 3..6 together) or, for the surrogate
+                      // step, after the features, the network, the decode and the fallback
 };
 
 using namespace ice3;
@@ -373,26 +374,26 @@ extern "C" void ice3_ctx_destroy(ice3_ctx *c) {
   if (!c) return;
   if (c->d_tab) (void)hipFree(c->d_tab);
   if (c->timing)
-    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(c->ev[q]);
+    for (int q = 0; q < 7; ++q) (void)hipEventDestroy(c->ev[q]);
   free(c);
 }
 
 extern "C" int ice3_stage_timing(ice3_ctx *c, int enable) {
   if (!c) return ICE3_EINVAL;
   if (enable && !c->timing) {
-    for (int q = 0; q < 5; ++q) HIPCHK(hipEventCreate(&c->ev[q]));
+    for (int q = 0; q < 7; ++q) HIPCHK(hipEventCreate(&c->ev[q]));
     c->timing = 1;
   } else if (!enable && c->timing) {
-    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(c->ev[q]);
+    for (int q = 0; q < 7; ++q) (void)hipEventDestroy(c->ev[q]);
     c->timing = 0;
   }
   return ICE3_OK;
 }
 
-extern "C" int ice3_stage_times(ice3_ctx *c, float ms[4]) {
+extern "C" int ice3_stage_times(ice3_ctx *c, float ms[6]) {
   if (!c || !ms || !c->timing) return ICE3_EINVAL;
-  HIPCHK(hipEventSynchronize(c->ev[4]));
-  for (int q = 0; q < 4; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
+  HIPCHK(hipEventSynchronize(c->ev[6]));
+  for (int q = 0; q < 6; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
   return ICE3_OK;
 }
 
@@ -501,7 +502,7 @@ extern "C" int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int 
   MARK(0);
   hipLaunchKernelGGL(k_ice_adjust, grid, dim3(256), 0, s, a);
   HIPCHK(hipGetLastError());
-  MARK(1); MARK(2); MARK(3); MARK(4);
+  MARK(1); MARK(2); MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;
 }
 
@@ -517,7 +518,7 @@ static int launch_rain_ice(ice3_ctx *ctx, int prec, const SedimArgs &sa, const M
   if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_micro<double>, gm, dim3(256), 0, s, ma);
   else hipLaunchKernelGGL(k_micro<float>, gm, dim3(256), 0, s, ma);
   HIPCHK(hipGetLastError());
-  MARK(3); MARK(4);
+  MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;
 }
 
@@ -590,7 +591,7 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
     else hipLaunchKernelGGL(k_micro<float>, gp, dim3(256), 0, s, m);
     HIPCHK(hipGetLastError());
   }
-  MARK(3); MARK(4);
+  MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;
 }
 
@@ -664,9 +665,11 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   MARK(2);
   hipLaunchKernelGGL(k_sg_features, dim3((npts + 255) / 256), dim3(256), 0, s, g);
   HIPCHK(hipGetLastError());
+  MARK(3);
   hipLaunchKernelGGL(k_mlp_infer, dim3(nblk), dim3(256), INFER_LDS, s,
                      (const bf16x8 *)sg->weights_packed, sg->params, (const __bf16 *)g.x, g.y, npts);
   HIPCHK(hipGetLastError());
+  MARK(4);
   // decode: a persistent grid of SG_DEC_BLOCKS blocks; block b decodes chunks
   // b, b + ndec, ... of 256 points and owns a queue segment of seg entries
   unsigned int *qcount = (unsigned int *)(wb + wl.qcount);
@@ -675,11 +678,11 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   const unsigned int seg = ((nchunks + ndec - 1) / ndec) * 256;
   hipLaunchKernelGGL(k_sg_decode, dim3(ndec), dim3(256), 0, s, g, seg, qcount);
   HIPCHK(hipGetLastError());
-  MARK(3);
+  MARK(5);
   hipLaunchKernelGGL(k_micro_segments<double>, dim3(ndec), dim3(64), 0, s, m,
                      (const unsigned int *)g.queue, (const unsigned int *)qcount, seg);
   HIPCHK(hipGetLastError());
-  MARK(4);
+  MARK(6);
   return ICE3_OK;
 }
 
