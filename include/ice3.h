@@ -143,9 +143,10 @@ int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, int nlev, int
  * microphysics makes to the sources of th, rv, rc, rr, ri, rs, rg and to ci, each as
  * unit * sinh(mean + std * y).  A point goes back to the plain FP64 microphysics when a
  * feature lies outside the training range, when the water residual sum(d rv..d rg)
- * exceeds budget_tol * sum|d r|, when a resulting source of rc..rg is negative by more
- * than neg_tol * (|source| + |change|), or when it has no hydrometeor (the plain kernel's
- * nucleation-only branch).  Accepted points have their water residual moved into rv and
+ * exceeds budget_tol * (sum|d r| + u_min) (u_min: the smallest targ_unit of rv..rg),
+ * when a resulting source of rc..rg is negative by more than
+ * neg_tol * (|source| + |change| + targ_unit), or when it has no hydrometeor (the plain
+ * kernel's nucleation-only branch).  Accepted points have their water residual moved into rv and
  * then pass ICE4_CORRECT_NEGATIVITIES, as the plain kernel's results do, so total water
  * is conserved and every source is non-negative. */
 typedef struct {
