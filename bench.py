@@ -234,7 +234,28 @@ def main():
         prec = -1000.0 * (out["inprr"] + out["inprs"] + out["inprg"]) * DT
         return ((col1 - col0 - prec).abs() / col0).max().item()
 
+    def substeps(prec, out):
+        """ICE4 time-splitting statistics of one extra (untimed) step: tendency calls per
+        point, their histogram, and the mean over 64-point groups (one wavefront of the
+        point kernel) of the largest count, which sets how many passes a wavefront makes."""
+        cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
+        calls = torch.zeros(npts, dtype=torch.int32, device=dev)
+        cnt = torch.zeros(3, dtype=torch.int64, device=dev)
+        st = _lib.Stats(ptr(calls), ptr(cnt))
+        _lib.check(lib.ice3_adjust_rain_ice(ctx, prec, ncol, NLEV, ncol, DT, ctypes.byref(cin),
+                                            ctypes.byref(cout), ctypes.byref(st), sp), "substep step")
+        torch.cuda.synchronize()
+        c = calls.abs().to(torch.int64)
+        hist = torch.bincount(c).tolist()
+        n_calls, capped, n_micro = cnt.tolist()
+        wmax = c.view(NLEV, ncol // 64, 64).amax(-1).double() if ncol % 64 == 0 else None
+        return {"tendency_calls": n_calls, "points": npts, "points_with_microphysics": n_micro,
+                "calls_per_microphysics_point": n_calls / max(n_micro, 1),
+                "capped_points": capped, "histogram_calls": hist,
+                "wavefront_max_calls_mean": wmax.mean().item() if wmax is not None else None}
+
     res["water_budget_max_rel_err"] = budget_err(out64)
+    res["ice4_substeps"] = substeps(_lib.PREC_FP64, out64)
     if not args.no_mixed:
         out32 = outputs()
         ms32 = run(_lib.PREC_MIXED, out32, args.steps, args.warmup)

@@ -10,8 +10,10 @@
 //   k_sedim<R>     one thread per column, walking the levels top-down: the
 //                  statistical sedimentation flux recurrence, whose fluxes
 //                  never leave registers, and the surface precipitation;
-//   k_micro<R>     one thread per (level, column) point: the ICE4 time-split
-//                  microphysics, total tendencies and negativity correction;
+//   k_micro<R>     persistent, one workgroup per compute unit, lookup tables in LDS:
+//                  the ICE4 time-split microphysics, total tendencies and negativity
+//                  correction per point, with the points still being time-stepped
+//                  compacted into full wavefronts (ballot, LDS parking);
 //   k_sg_features, k_mlp_infer, k_sg_decode
 //                  the learned surrogate of k_micro: features and range check per point;
 //                  the MLP on the bf16 matrix cores (persistent grid, weights resident in
@@ -51,26 +53,182 @@ __global__ void __launch_bounds__(256) k_sedim(SedimArgs a) {
 }
 
 // Sum of v over the 64 lanes of the wavefront (every lane gets the total).
-__device__ __forceinline__ int wave_sum(int v) {
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
   return v;
 }
 
+// ---- k_micro: the ICE4 microphysics, persistent, with the time splitting compacted
+// across each wavefront.
+//
+// Every point makes one hard tendency call, and 42 % of them (on the 512x512x90 synthetic
+// workload) then make soft calls, most of them one, a few up to five.  Run one point per
+// lane to the end and a wavefront makes as many soft passes as its slowest lane needs,
+// nearly all with few lanes active.  Here a wavefront takes 64-point chunks in turn; after
+// the chunk's hard call, the lanes whose time step goes on are joined by points parked in
+// LDS by earlier chunks and a soft pass runs only over a full wavefront (64 points);
+// unfinished points are parked again (ballot + prefix count give each its slot).  The
+// per-point arithmetic is that of micro_at, only the order in which points are advanced
+// differs, so the results are those of one thread per point.
+// The block's four wavefronts share the lookup tables, staged once into LDS.
+constexpr int MICRO_WAVES = 4;   // wavefronts per workgroup
+constexpr int MICRO_PARK = 63;   // parked points per wavefront (fewer than 64 by construction)
+
+// Points parked by one wavefront: state between tendency calls, one row per field.
+template <class R> struct MicroPark {
+  double d[9][MICRO_PARK];   // vart[7], cit, time
+  R s[32][MICRO_PARK];       // StepState
+  unsigned int q[MICRO_PARK];  // point index k * ncol + i
+  int calls[MICRO_PARK];
+};
+static_assert(sizeof(StepState<double>) == 32 * sizeof(double), "StepState layout");
+static_assert(sizeof(StepState<float>) == 32 * sizeof(float), "StepState layout");
+
+template <class R> constexpr size_t micro_tab_bytes() {
+  return ((size_t)ICE3_T_TOTAL * sizeof(R) + 15) / 16 * 16;
+}
+template <class R> constexpr size_t micro_lds_bytes() {
+  return micro_tab_bytes<R>() + MICRO_WAVES * sizeof(MicroPark<R>);
+}
+static_assert(micro_lds_bytes<double>() <= 160 * 1024, "k_micro<double> LDS");
+static_assert(micro_lds_bytes<float>() <= 160 * 1024, "k_micro<float> LDS");
+
 template <class R>
-__global__ void __launch_bounds__(256) k_micro(MicroArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
+__device__ __forceinline__ void park_store(MicroPark<R> &pk, int e, unsigned int q,
+                                           const double *vart, double cit, double time,
+                                           const StepState<R> &S, int calls) {
+#pragma unroll
+  for (int f = 0; f < 7; ++f) pk.d[f][e] = vart[f];
+  pk.d[7][e] = cit;
+  pk.d[8][e] = time;
+  const R *sv = reinterpret_cast<const R *>(&S);
+#pragma unroll
+  for (int f = 0; f < 32; ++f) pk.s[f][e] = sv[f];
+  pk.q[e] = q;
+  pk.calls[e] = calls;
+}
+
+template <class R>
+__device__ __forceinline__ unsigned int park_load(const MicroPark<R> &pk, int e, double *vart,
+                                                  double &cit, double &time, StepState<R> &S,
+                                                  int &calls) {
+#pragma unroll
+  for (int f = 0; f < 7; ++f) vart[f] = pk.d[f][e];
+  cit = pk.d[7][e];
+  time = pk.d[8][e];
+  R *sv = reinterpret_cast<R *>(&S);
+#pragma unroll
+  for (int f = 0; f < 32; ++f) sv[f] = pk.s[f][e];
+  calls = pk.calls[e];
+  return pk.q[e];
+}
+
+template <class R>
+__global__ void __launch_bounds__(64 * MICRO_WAVES) k_micro(MicroArgs a) {
+  extern __shared__ __align__(16) unsigned char micro_lds[];
+  R *tab = reinterpret_cast<R *>(micro_lds);
+  for (int e = threadIdx.x; e < ICE3_T_TOTAL; e += blockDim.x) tab[e] = (R)a.tab[e];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  MicroPark<R> &pk = reinterpret_cast<MicroPark<R> *>(micro_lds + micro_tab_bytes<R>())[wid];
+  const unsigned long long below = (1ull << lane) - 1ull;  // lanes under this one
+  const int cpl = (a.ncol + 63) >> 6;                       // chunks per level
+  const int nchunk = cpl * a.nlev;
+  const double dt = a.dt;
+  const PointConst<R> pc_soft{};  // soft calls read no point constants
+  int parked = 0;                 // wave-uniform
+  bool active = false;            // this lane holds a point whose time step goes on
+  unsigned int q = 0;             // that point: index k * ncol + i, offset j
+  size_t j = 0;
+  StepPoint p;
+  double vart[7], cit = 0., time = 0.;
+  StepState<R> S;
   int calls = 0;
-  if (i < a.ncol) calls = micro_at<R>(a, (size_t)k * a.ld + i);
-  if (a.counters) {  // uniform branch: every lane of the wave takes part in the sums
-    const int n = wave_sum(calls < 0 ? -calls : calls);
-    const int c = wave_sum(calls < 0 ? 1 : 0);
-    const int p = wave_sum(calls != 0 ? 1 : 0);
-    if ((threadIdx.x & 63) == 0) {
-      atomicAdd(&a.counters[0], (unsigned long long)n);
-      atomicAdd(&a.counters[1], (unsigned long long)c);
-      atomicAdd(&a.counters[2], (unsigned long long)p);
+  unsigned long long ncalls = 0, ncap = 0, nmicro = 0;
+  for (int c = blockIdx.x * MICRO_WAVES + wid;; c += gridDim.x * MICRO_WAVES) {
+    const bool more = c < nchunk;  // wave-uniform
+    if (more) {
+      const int k = c / cpl, i = (c - k * cpl) * 64 + lane;
+      if (i < a.ncol) {
+        q = (unsigned int)k * (unsigned int)a.ncol + (unsigned int)i;
+        j = (size_t)k * a.ld + i;
+        double t[7], s[7];
+        micro_load(a, j, t, s);
+        cit = a.src_ci[j];
+        double rvheni = 0., evap3d = 0.;
+        int st = 0;
+        calls = 0;
+        if (micro_mask(t)) {
+          ++nmicro;
+          step_point(dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j], s, t, p);
+#pragma unroll
+          for (int f = 0; f < 7; ++f) vart[f] = t[f];
+          time = 0.;
+          const PointConst<R> pc = point_const<R>(p.rhodref);
+          st = step_call<R, false>(dt, p, pc, vart, cit, time, S, calls, tab);
+          if (st == 0) evap3d = (double)S.rrevav;
+        } else {
+          micro_nucleation<R>(a, j, t, cit, rvheni);
+#pragma unroll
+          for (int f = 0; f < 7; ++f) vart[f] = t[f];
+        }
+        if (st > 0) {
+          active = true;
+        } else {
+          ncalls += calls;
+          ncap += st < 0;
+          micro_finish(a, j, vart, cit, evap3d, rvheni, st < 0 ? -calls : calls);
+        }
+      }
+    }
+    // soft passes, each over a full wavefront while enough points are at hand (all of
+    // them once the chunks are done)
+    for (;;) {
+      const unsigned long long am = __ballot(active);
+      const int nact = __popcll(am), need = 64 - nact;
+      if (more ? parked < need : nact + parked == 0) break;
+      if (!active) {
+        const int r = __popcll(~am & below);  // rank among the idle lanes
+        if (r < parked) {
+          q = park_load(pk, parked - 1 - r, vart, cit, time, S, calls);
+          const unsigned int k = q / (unsigned int)a.ncol;
+          j = (size_t)k * a.ld + (q - k * (unsigned int)a.ncol);
+          double t[7], s[7];
+          micro_load(a, j, t, s);
+          step_point(dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j], s, t, p);
+          active = true;
+        }
+      }
+      parked -= need < parked ? need : parked;
+      if (active) {
+        const int st = step_call<R, true>(dt, p, pc_soft, vart, cit, time, S, calls, tab);
+        if (st <= 0) {
+          ncalls += calls;
+          ncap += st < 0;
+          micro_finish(a, j, vart, cit, st == 0 ? (double)S.rrevav : 0., 0.,
+                       st < 0 ? -calls : calls);
+          active = false;
+        }
+      }
+    }
+    if (!more) break;
+    // park the points whose time step goes on (parked + their number < 64 here)
+    const unsigned long long am = __ballot(active);
+    if (active) {
+      park_store(pk, parked + __popcll(am & below), q, vart, cit, time, S, calls);
+      active = false;
+    }
+    parked += __popcll(am);
+    // the stores above are read by other lanes of this wavefront in later passes
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  if (a.counters) {  // uniform branch: every lane of the wavefront takes part in the sums
+    const unsigned long long n = wave_sum(ncalls), cc = wave_sum(ncap), m = wave_sum(nmicro);
+    if (lane == 0) {
+      atomicAdd(&a.counters[0], n);
+      atomicAdd(&a.counters[1], cc);
+      atomicAdd(&a.counters[2], m);
     }
   }
 }
@@ -289,7 +447,7 @@ __global__ void __launch_bounds__(64) k_micro_segments(MicroArgs a, const unsign
                                                        const unsigned int *qcount, unsigned int seg) {
   const unsigned int n = qcount[blockIdx.x];
   const unsigned int *list = queue + 1 + (size_t)blockIdx.x * seg;
-  for (unsigned int e = threadIdx.x; e < n; e += blockDim.x) micro_at<R>(a, (size_t)list[e]);
+  for (unsigned int e = threadIdx.x; e < n; e += blockDim.x) micro_at<R>(a, (size_t)list[e], a.tab);
 }
 
 // Scratch layout of the surrogate step (bytes, 256-aligned blocks).
@@ -332,6 +490,7 @@ __global__ void __launch_bounds__(256) k_features(MicroArgs m, unsigned int npts
 
 struct ice3_ctx {
   int device;
+  int ncu;            // compute units of the device: k_micro runs one workgroup on each
   ice3_params params;
   double *d_tab;
   double h_tab[ICE3_T_TOTAL];
@@ -359,6 +518,8 @@ extern "C" int ice3_ctx_create(ice3_ctx **out) {
     return ICE3_ECONFIG;
   }
   if (hipGetDevice(&c->device) != hipSuccess ||
+      hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+          hipSuccess || c->ncu <= 0 ||
       hipMalloc((void **)&c->d_tab, sizeof(double) * ICE3_T_TOTAL) != hipSuccess ||
       hipMemcpy(c->d_tab, c->h_tab, sizeof(double) * ICE3_T_TOTAL, hipMemcpyHostToDevice) !=
           hipSuccess) {
@@ -477,7 +638,7 @@ extern "C" long ice3_dump_constants(const ice3_ctx *ctx, char *buf, long buflen)
 
 static bool bad_shape(int ncol, int nlev, int ld, double dt) {
   return ncol <= 0 || nlev <= 0 || ld < ncol || !(dt > 0.) ||
-         (double)ld * (double)nlev > 9.0e15;
+         (double)ld * (double)nlev > 9.0e15 || (double)ncol * (double)nlev > 4294967295.0;
 }
 
 #define NN(p) if (!(p)) return ICE3_EINVAL
@@ -506,18 +667,28 @@ extern "C" int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int 
   return ICE3_OK;
 }
 
+// Launch k_micro: one workgroup of four wavefronts per compute unit (its LDS holds the
+// tables and the parked points, so one workgroup fits a CU).
+static hipError_t launch_micro(const ice3_ctx *ctx, int prec, const MicroArgs &m, hipStream_t s) {
+  if (prec == ICE3_PREC_FP64)
+    hipLaunchKernelGGL(k_micro<double>, dim3(ctx->ncu), dim3(64 * MICRO_WAVES),
+                       micro_lds_bytes<double>(), s, m);
+  else
+    hipLaunchKernelGGL(k_micro<float>, dim3(ctx->ncu), dim3(64 * MICRO_WAVES),
+                       micro_lds_bytes<float>(), s, m);
+  return hipGetLastError();
+}
+
 // Launch the sedimentation then the microphysics stage on one stream.
 static int launch_rain_ice(ice3_ctx *ctx, int prec, const SedimArgs &sa, const MicroArgs &ma,
                            void *stream) {
   hipStream_t s = (hipStream_t)stream;
-  const dim3 gs((sa.ncol + 255) / 256), gm((ma.ncol + 255) / 256, ma.nlev);
+  const dim3 gs((sa.ncol + 255) / 256);
   if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
   else hipLaunchKernelGGL(k_sedim<float>, gs, dim3(256), 0, s, sa);
   HIPCHK(hipGetLastError());
   MARK(2);
-  if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_micro<double>, gm, dim3(256), 0, s, ma);
-  else hipLaunchKernelGGL(k_micro<float>, gm, dim3(256), 0, s, ma);
-  HIPCHK(hipGetLastError());
+  HIPCHK(launch_micro(ctx, prec, ma, s));
   MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;
 }
@@ -587,9 +758,7 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
   }
   MARK(2);
   if (stages & ICE3_STAGE_MICROPHYSICS) {
-    if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_micro<double>, gp, dim3(256), 0, s, m);
-    else hipLaunchKernelGGL(k_micro<float>, gp, dim3(256), 0, s, m);
-    HIPCHK(hipGetLastError());
+    HIPCHK(launch_micro(ctx, prec, m, s));
   }
   MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;

@@ -240,8 +240,8 @@ template <class R> struct StepState {
 
 // INTERP_MICRO_1D (micro/interp_micro.func.h:34-88) for one point
 // (the argument is ln x: the callers keep the logarithm of each slope parameter)
-template <class R>
-__device__ __forceinline__ R interp1d(R lnx, int knum, R p1, R p2, const double *__restrict__ lt) {
+template <class R, class TB>
+__device__ __forceinline__ R interp1d(R lnx, int knum, R p1, R p2, const TB *__restrict__ lt) {
   R idx = dmax((R)1.00001, dmin((R)knum - (R)0.00001, p1 * lnx + p2));
   int i = (int)idx;
   R f = idx - (R)i;
@@ -263,16 +263,16 @@ __device__ __forceinline__ Idx2<R> interp2d_idx(R lnx1, R lnx2, int n1, int n2, 
   return r;
 }
 // table lt is (n1, n2) column-major, addressed lt(i1, i2) (1-based)
-template <class R>
-__device__ __forceinline__ R interp2d(const Idx2<R> &x, int n1, const double *__restrict__ lt) {
+template <class R, class TB>
+__device__ __forceinline__ R interp2d(const Idx2<R> &x, int n1, const TB *__restrict__ lt) {
 #define T(i, j) ((R)lt[((i)-1) + ((j)-1) * n1])
   return (T(x.i1 + 1, x.i2 + 1) * x.f2 - T(x.i1 + 1, x.i2) * (x.f2 - (R)1.0)) * x.f1 -
          (T(x.i1, x.i2 + 1) * x.f2 - T(x.i1, x.i2) * (x.f2 - (R)1.0)) * (x.f1 - (R)1.0);
 #undef T
 }
 // transposed addressing lt(i2, i1) of an (n2, n1) table (:211-216)
-template <class R>
-__device__ __forceinline__ R interp2d_t(const Idx2<R> &x, int n2, const double *__restrict__ lt) {
+template <class R, class TB>
+__device__ __forceinline__ R interp2d_t(const Idx2<R> &x, int n2, const TB *__restrict__ lt) {
 #define T(i, j) ((R)lt[((i)-1) + ((j)-1) * n2])
   return (T(x.i2 + 1, x.i1 + 1) * x.f1 - T(x.i2 + 1, x.i1) * (x.f1 - (R)1.0)) * x.f2 -
          (T(x.i2, x.i1 + 1) * x.f1 - T(x.i2, x.i1) * (x.f1 - (R)1.0)) * (x.f2 - (R)1.0);
@@ -308,12 +308,12 @@ template <class R> __device__ __forceinline__ R powl_(R lnx, R c) { return rexp<
 // Outputs: pa (rates) and pb (instantaneous changes), both double.
 // Soft calls (reuse of the hard call's rates) skip everything that only feeds
 // rate evaluations guarded by .NOT.LDSOFT, such as the slope parameters.
-template <class R>
+template <class R, class TB>
 __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref, double lvfact,
                                   double lsfact, double pres, double cf, double sigma_rc,
                                   double &cit, double pt, const double *pvart,
                                   StepState<R> &S, const PointConst<R> &pc,
-                                  const double *__restrict__ tab, double *pa, double *pb) {
+                                  const TB *__restrict__ tab, double *pa, double *pb) {
   double zt = pt;
   double zv[7];
 #pragma unroll
@@ -733,60 +733,93 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
 
 constexpr int MAX_CALLS = 4096;  // sub-step cap per point (the reference has none)
 
-// ICE4_STEPPING (micro/mode_ice4_stepping.F90:199-537) for one microphysics point.
-// Returns the number of tendency calls, or -calls if the cap was hit.
-template <class R>
-__device__ __forceinline__ int stepping_point(double dt, double exn, double rhodref, double pres,
-                                     double cf, double sigs, double &cit, double *vart,
-                                     double *extpk, double &rrevav,
-                                     const double *__restrict__ tab) {
-  StepState<R> S;
-  const PointConst<R> pc = point_const<R>(rhodref);
+// Fixed inputs of ICE4_STEPPING for one point.
+struct StepPoint {
+  double exn, rhodref, pres, cf, sigma_rc;
+  double extpk[7];  // external tendencies: sources minus t-state / dt
+};
+
+// Set-up of ICE4_STEPPING (micro/mode_ice4_stepping.F90:199-240) for one point: the
+// external tendencies from the sources s and the t-state vart.
+__device__ __forceinline__ void step_point(double dt, double exn, double rhodref, double pres,
+                                           double cf, double sigs, const double *s,
+                                           const double *vart, StepPoint &p) {
   const double zinv = 1. / dt;
+  p.exn = exn; p.rhodref = rhodref; p.pres = pres; p.cf = cf; p.sigma_rc = sigs * 2.;
 #pragma unroll
-  for (int jv = 0; jv < 7; ++jv) extpk[jv] = extpk[jv] - vart[jv] * zinv;
-  const double sigma_rc = sigs * 2.;
-  double time = 0., za[7], zb[7];
-  int calls = 0;
-  while (time < dt) {
-    bool llcompute = true, soft = false;
-    while (llcompute) {
-      if (++calls > MAX_CALLS) return -calls;
-      const double sum2 = vart[IRI] + vart[IRS] + vart[IRG];
-      const double zdevide =
-          (P.XCPD + P.XCPV * vart[IRV] + P.XCL * (vart[IRC] + vart[IRR]) + P.XCI * sum2) * exn;
-      const double zzt = vart[ITH] * exn;
-      const double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
-      const double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
-      tendencies<R>(soft, exn, rhodref, lvfact, lsfact, pres, cf, sigma_rc, cit, zzt, vart, S, pc,
-                    tab, za, zb);
+  for (int jv = 0; jv < 7; ++jv) p.extpk[jv] = s[jv] - vart[jv] * zinv;
+}
+
+// One pass of the inner loop of ICE4_STEPPING (:256-516): an ICE4_TENDENCIES call (hard
+// when SOFT is false, else reusing the rates kept in S), the time-step limitation and the
+// state update.  The point's state between calls: vart (th, rv, rc, rr, ri, rs, rg), cit,
+// time (part of the step integrated so far), S (rates of the hard call) and calls.
+// Returns 1 while the point's time step goes on, 0 when it is complete (vart then holds
+// the end state, :520-537), -1 when the sub-step cap was hit (vart as after the last
+// completed call).  pc is read by hard calls only.
+template <class R, bool SOFT, class TB>
+__device__ __forceinline__ int step_call(double dt, const StepPoint &p, const PointConst<R> &pc,
+                                         double *vart, double &cit, double &time_,
+                                         StepState<R> &S, int &calls,
+                                         const TB *__restrict__ tab) {
+  if (++calls > MAX_CALLS) return -1;
+  double za[7], zb[7];
+  const double exn = p.exn;
+  const double sum2 = vart[IRI] + vart[IRS] + vart[IRG];
+  const double zdevide =
+      (P.XCPD + P.XCPV * vart[IRV] + P.XCL * (vart[IRC] + vart[IRR]) + P.XCI * sum2) * exn;
+  const double zzt = vart[ITH] * exn;
+  const double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
+  const double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
+  tendencies<R>(SOFT, exn, p.rhodref, lvfact, lsfact, p.pres, p.cf, p.sigma_rc, cit, zzt, vart,
+                S, pc, tab, za, zb);
 #pragma unroll
-      for (int jv = 0; jv < 7; ++jv) za[jv] = za[jv] + extpk[jv];
-      double maxtime = dt - time;
-      const double zx = P.XTT / exn;
-      if ((vart[ITH] - zx) * (vart[ITH] + zb[ITH] - zx) < 0.) maxtime = 0.;
-      if (fabs(za[ITH]) > 1.E-20) {
-        const double tth = (zx - zb[ITH] - vart[ITH]) / za[ITH];
-        if (tth > 0.) maxtime = dmin(maxtime, tth);
-      }
-#pragma unroll
-      for (int jv = 1; jv < 7; ++jv)
-        if (za[jv] < -1.E-20 && vart[jv] > P.XRTMIN[jv]) {
-          maxtime = dmin(maxtime, -(zb[jv] + vart[jv]) / za[jv]);
-          maxtime = dmax(maxtime, P.XMNH_TINY);
-        }
-      if (time + maxtime >= dt) llcompute = false;
-#pragma unroll
-      for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] + za[jv] * maxtime + zb[jv];
-      if (vart[IRI] <= 0.) cit = 0.;
-      time = time + maxtime;
-      soft = true;
-    }
+  for (int jv = 0; jv < 7; ++jv) za[jv] = za[jv] + p.extpk[jv];
+  const double time = time_;
+  double maxtime = dt - time;
+  const double zx = P.XTT / exn;
+  if ((vart[ITH] - zx) * (vart[ITH] + zb[ITH] - zx) < 0.) maxtime = 0.;
+  if (fabs(za[ITH]) > 1.E-20) {
+    const double tth = (zx - zb[ITH] - vart[ITH]) / za[ITH];
+    if (tth > 0.) maxtime = dmin(maxtime, tth);
   }
 #pragma unroll
-  for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] - extpk[jv] * dt;
-  rrevav = (double)S.rrevav;
-  return calls;
+  for (int jv = 1; jv < 7; ++jv)
+    if (za[jv] < -1.E-20 && vart[jv] > P.XRTMIN[jv]) {
+      maxtime = dmin(maxtime, -(zb[jv] + vart[jv]) / za[jv]);
+      maxtime = dmax(maxtime, P.XMNH_TINY);
+    }
+  const bool done = time + maxtime >= dt;
+#pragma unroll
+  for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] + za[jv] * maxtime + zb[jv];
+  if (vart[IRI] <= 0.) cit = 0.;
+  time_ = time + maxtime;
+  if (!done) return 1;
+#pragma unroll
+  for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] - p.extpk[jv] * dt;
+  return 0;
+}
+
+// ICE4_STEPPING (micro/mode_ice4_stepping.F90:199-537) for one microphysics point, all
+// calls in a row: vart (t-state in, end state out), cit and rrevav are updated, s holds
+// the sources.  Each point's time step is one outer iteration: a hard call, then soft
+// calls until the step is complete.  Returns the number of tendency calls, or -calls if
+// the cap was hit.
+template <class R, class TB>
+__device__ __forceinline__ int stepping_point(double dt, double exn, double rhodref, double pres,
+                                              double cf, double sigs, double &cit, double *vart,
+                                              const double *s, double &rrevav,
+                                              const TB *__restrict__ tab) {
+  StepPoint p;
+  step_point(dt, exn, rhodref, pres, cf, sigs, s, vart, p);
+  StepState<R> S;
+  double time = 0.;
+  int calls = 0;
+  const PointConst<R> pc = point_const<R>(rhodref);
+  int st = step_call<R, false>(dt, p, pc, vart, cit, time, S, calls, tab);
+  while (st > 0) st = step_call<R, true>(dt, p, pc, vart, cit, time, S, calls, tab);
+  if (st == 0) rrevav = (double)S.rrevav;
+  return st < 0 ? -calls : calls;
 }
 
 // Fall speed of one species at one level, OTHER_SPECIES / PRISTINE_ICE

@@ -133,46 +133,63 @@ __device__ __forceinline__ void correct_negativities(double &ths, double &rvs, d
   rvs = rvs + zw; rgs = rgs - zw; ths = ths - zw * lsfact;
 }
 
-// RAIN_ICE sections 1, 4, 5, 8 (micro/rain_ice.F90:224-273, 391-431, 529-543 and the
-// reinstated 649-714) for point j.  Returns the number of tendency calls, negative if the
-// sub-step cap was hit.
-template <class R>
-__device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j) {
-  const double dt = a.dt, zinv = 1. / dt, ts = a.tscale;
-  const double exn = a.exn[j], rhodref = a.rhodref[j], exnref = a.exn_ref[j];
-  const double pabs = a.pabs[j], cldfr = a.cldfr[j], sigs = a.sigs[j];
-  const double tht = a.t[ITH][j] * ts, rvt = a.t[IRV][j] * ts, rct = a.t[IRC][j] * ts;
-  const double rrt = a.t[IRR][j] * ts, rit = a.t[IRI][j] * ts, rst = a.t[IRS][j] * ts;
-  const double rgt = a.t[IRG][j] * ts;
-  double s[7];
+// t-state (times tscale) and sources of the seven variables at point j.
+__device__ __forceinline__ void micro_load(const MicroArgs &a, size_t j, double *t, double *s) {
+  const double ts = a.tscale;
+#pragma unroll
+  for (int v = 0; v < 7; ++v) t[v] = a.t[v][j] * ts;
 #pragma unroll
   for (int v = 0; v < 7; ++v) s[v] = a.src[v][j];
-  double cit = a.src_ci[j];
-  const double zrice = rit + rst + rgt;
-  const double zdevide = P.XCPD + P.XCPV * rvt + P.XCL * (rct + rrt) + P.XCI * zrice;
-  const double zt = tht * exn;
-  double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide;
-  double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide;
-  const bool llmicro = rct > P.XRTMIN[2] || rrt > P.XRTMIN[3] || rit > P.XRTMIN[4] ||
-                       rst > P.XRTMIN[5] || rgt > P.XRTMIN[6];
-  double wr[7] = {tht, rvt, rct, rrt, rit, rst, rgt};
-  double rvheni = 0., evap3d = 0.;
-  int calls = 0;
-  if (!llmicro) {
-    cit = 0.;
-    nucleation<R>(true, tht, pabs, rhodref, exn, lsfact / exn, zt, rvt, cit, rvheni);
-  }
+}
+
+// RAIN_ICE's latent heat factors L / cph (micro/rain_ice.F90:334-347) from the t-state t.
+__device__ __forceinline__ void rain_ice_factors(const double *t, double exn, double &lsfact,
+                                                 double &lvfact) {
+  const double zrice = t[IRI] + t[IRS] + t[IRG];
+  const double zdevide = P.XCPD + P.XCPV * t[IRV] + P.XCL * (t[IRC] + t[IRR]) + P.XCI * zrice;
+  const double zt = t[ITH] * exn;
+  lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide;
+  lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide;
+}
+
+// RAIN_ICE's microphysics mask (micro/rain_ice.F90:391-410): a hydrometeor above XRTMIN.
+__device__ __forceinline__ bool micro_mask(const double *t) {
+  return t[IRC] > P.XRTMIN[2] || t[IRR] > P.XRTMIN[3] || t[IRI] > P.XRTMIN[4] ||
+         t[IRS] > P.XRTMIN[5] || t[IRG] > P.XRTMIN[6];
+}
+
+// Nucleation outside the microphysics mask (micro/rain_ice.F90:224-273) at point j with
+// t-state t: cit is reset and updated, rvheni receives the nucleated vapour.
+template <class R>
+__device__ __forceinline__ void micro_nucleation(const MicroArgs &a, size_t j, const double *t,
+                                                 double &cit, double &rvheni) {
+  const double exn = a.exn[j];
+  double lsfact, lvfact;
+  rain_ice_factors(t, exn, lsfact, lvfact);
+  cit = 0.;
+  nucleation<R>(true, t[ITH], a.pabs[j], a.rhodref[j], exn, lsfact / exn, t[ITH] * exn, t[IRV],
+                cit, rvheni);
+}
+
+// RAIN_ICE section 8 at point j (micro/rain_ice.F90:529-543 and the reinstated 649-714):
+// total tendencies from the end state wr of the time splitting, ICE4_CORRECT_NEGATIVITIES,
+// and the stores.  The t-state and sources are read again from memory (they are not
+// written before this point's stores).  rvheni: vapour nucleated outside the mask.
+__device__ __forceinline__ void micro_finish(const MicroArgs &a, size_t j, const double *wr,
+                                             double cit, double evap3d, double rvheni, int calls) {
+  const double dt = a.dt, zinv = 1. / dt;
+  double t[7], s[7];
+  micro_load(a, j, t, s);
+  double lsfact, lvfact;
+  rain_ice_factors(t, a.exn[j], lsfact, lvfact);
   rvheni = dmin(s[IRV], rvheni / dt);
-  if (llmicro) {
-    double extpk[7] = {s[0], s[1], s[2], s[3], s[4], s[5], s[6]};
-    calls = stepping_point<R>(dt, exn, rhodref, pabs, cldfr, sigs, cit, wr, extpk, evap3d, a.tab);
-  }
   // 8.1 total tendencies
+  const double exnref = a.exn_ref[j];
   lsfact = lsfact / exnref;
   lvfact = lvfact / exnref;
-  const double wrv = (wr[IRV] - rvt) * zinv, wrc = (wr[IRC] - rct) * zinv;
-  const double wrr = (wr[IRR] - rrt) * zinv, wri = (wr[IRI] - rit) * zinv;
-  const double wrs = (wr[IRS] - rst) * zinv, wrg = (wr[IRG] - rgt) * zinv;
+  const double wrv = (wr[IRV] - t[IRV]) * zinv, wrc = (wr[IRC] - t[IRC]) * zinv;
+  const double wrr = (wr[IRR] - t[IRR]) * zinv, wri = (wr[IRI] - t[IRI]) * zinv;
+  const double wrs = (wr[IRS] - t[IRS]) * zinv, wrg = (wr[IRG] - t[IRG]) * zinv;
   const double wth = (wrc + wrr) * lvfact + (wri + wrs + wrg + 0.) * lsfact;
   double ths = s[ITH] + wth + rvheni * lsfact;
   double rvs = s[IRV] + wrv - rvheni;
@@ -189,6 +206,23 @@ __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j) {
   a.evap3d[j] = evap3d;
   a.rainfr[j] = 1.;
   if (a.calls) a.calls[j] = calls;
+}
+
+// RAIN_ICE sections 1, 4, 5, 8 (micro/rain_ice.F90:224-273, 391-431, 529-543 and the
+// reinstated 649-714) for point j, all in one thread.  Returns the number of tendency
+// calls, negative if the sub-step cap was hit.
+template <class R, class TB>
+__device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j, const TB *__restrict__ tab) {
+  double wr[7], s[7];
+  micro_load(a, j, wr, s);
+  double cit = a.src_ci[j], rvheni = 0., evap3d = 0.;
+  int calls = 0;
+  if (micro_mask(wr))
+    calls = stepping_point<R>(a.dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j], cit,
+                              wr, s, evap3d, tab);
+  else
+    micro_nucleation<R>(a, j, wr, cit, rvheni);
+  micro_finish(a, j, wr, cit, evap3d, rvheni, calls);
   return calls;
 }
 
@@ -224,14 +258,11 @@ __device__ __forceinline__ bool surrogate_features(const MicroArgs &a, size_t j,
 // micro_at computes them (micro/rain_ice.F90:334-347 divided by PEXNREF).
 __device__ __forceinline__ void latent_factors(const MicroArgs &a, size_t j, double &lvfact,
                                                double &lsfact) {
-  const double ts = a.tscale;
-  const double tht = a.t[ITH][j] * ts, rvt = a.t[IRV][j] * ts, rct = a.t[IRC][j] * ts;
-  const double rrt = a.t[IRR][j] * ts, rit = a.t[IRI][j] * ts, rst = a.t[IRS][j] * ts;
-  const double rgt = a.t[IRG][j] * ts;
-  const double zdevide = P.XCPD + P.XCPV * rvt + P.XCL * (rct + rrt) + P.XCI * (rit + rst + rgt);
-  const double zt = tht * a.exn[j];
-  lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide / a.exn_ref[j];
-  lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide / a.exn_ref[j];
+  double t[7], s[7];
+  micro_load(a, j, t, s);
+  rain_ice_factors(t, a.exn[j], lsfact, lvfact);
+  lsfact = lsfact / a.exn_ref[j];
+  lvfact = lvfact / a.exn_ref[j];
 }
 
 // ---------------------------------------------------------------- argument set-up

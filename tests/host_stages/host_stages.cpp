@@ -31,7 +31,7 @@ static void run_rain_ice(const SedimArgs &sa, const MicroArgs &m) {
   unsigned long long n = 0, c = 0, p = 0;
   for (int k = 0; k < m.nlev; ++k)
     for (int i = 0; i < m.ncol; ++i) {
-      const int calls = micro_at<R>(m, (size_t)k * m.ld + i);
+      const int calls = micro_at<R>(m, (size_t)k * m.ld + i, m.tab);
       n += calls < 0 ? -calls : calls;
       c += calls < 0;
       p += calls != 0;
