@@ -67,20 +67,24 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // lane to the end and a wavefront makes as many soft passes as its slowest lane needs,
 // nearly all with few lanes active.  Here a wavefront takes 64-point chunks in turn; after
 // the chunk's hard call, the lanes whose time step goes on are joined by points parked in
-// LDS by earlier chunks and a soft pass runs only over a full wavefront (64 points);
-// unfinished points are parked again (ballot + prefix count give each its slot).  The
-// per-point arithmetic is that of micro_at, only the order in which points are advanced
-// differs, so the results are those of one thread per point.
-// The block's four wavefronts share the lookup tables, staged once into LDS.
-constexpr int MICRO_WAVES = 4;   // wavefronts per workgroup
-constexpr int MICRO_PARK = 63;   // parked points per wavefront (fewer than 64 by construction)
+// LDS by earlier chunks, and a soft pass runs over a full wavefront (64 points) whenever
+// that many are at hand, or over more than MicroPark::CAP points when the park would
+// otherwise overflow; unfinished points are parked again (ballot + prefix count give
+// each its slot).  The per-point arithmetic is that of micro_at, only the order in which
+// points are advanced differs, so the results are those of one thread per point.
+// Eight wavefronts per workgroup, two per SIMD: the register budget of 256 per wavefront
+// costs a few spills (FP64), but the second wavefront hides the memory latency the first
+// one waits on.  The workgroup's wavefronts share the lookup tables, staged once into LDS.
+constexpr int MICRO_WAVES = 8;  // wavefronts per workgroup
 
 // Points parked by one wavefront: state between tendency calls, one row per field.
+// Capacity: what fits the LDS beside the tables (fewer than 64).
 template <class R> struct MicroPark {
-  double d[9][MICRO_PARK];   // vart[7], cit, time
-  R s[32][MICRO_PARK];       // StepState
-  unsigned int q[MICRO_PARK];  // point index k * ncol + i
-  int calls[MICRO_PARK];
+  static constexpr int CAP = sizeof(R) == 8 ? 31 : 63;
+  double d[9][CAP];        // vart[7], cit, time
+  R s[32][CAP];            // StepState
+  unsigned int q[CAP];     // point index k * ncol + i
+  int calls[CAP];
 };
 static_assert(sizeof(StepState<double>) == 32 * sizeof(double), "StepState layout");
 static_assert(sizeof(StepState<float>) == 32 * sizeof(float), "StepState layout");
@@ -182,12 +186,12 @@ __global__ void __launch_bounds__(64 * MICRO_WAVES) k_micro(MicroArgs a) {
         }
       }
     }
-    // soft passes, each over a full wavefront while enough points are at hand (all of
-    // them once the chunks are done)
+    // soft passes: over a full wavefront while 64 points are at hand, over all of them
+    // while they would not fit the park, and over all of them once the chunks are done
     for (;;) {
       const unsigned long long am = __ballot(active);
       const int nact = __popcll(am), need = 64 - nact;
-      if (more ? parked < need : nact + parked == 0) break;
+      if (more ? parked < need && nact + parked <= MicroPark<R>::CAP : nact + parked == 0) break;
       if (!active) {
         const int r = __popcll(~am & below);  // rank among the idle lanes
         if (r < parked) {
@@ -213,7 +217,7 @@ __global__ void __launch_bounds__(64 * MICRO_WAVES) k_micro(MicroArgs a) {
       }
     }
     if (!more) break;
-    // park the points whose time step goes on (parked + their number < 64 here)
+    // park the points whose time step goes on (parked + their number <= CAP here)
     const unsigned long long am = __ballot(active);
     if (active) {
       park_store(pk, parked + __popcll(am & below), q, vart, cit, time, S, calls);
@@ -443,7 +447,7 @@ __global__ void __launch_bounds__(256) k_sg_decode(SurrogateArgs a, unsigned int
 // Plain microphysics over the surrogate's rejected points: block b runs the qcount[b]
 // points that decode block b queued.
 template <class R>
-__global__ void __launch_bounds__(64) k_micro_segments(MicroArgs a, const unsigned int *queue,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_micro_segments(MicroArgs a, const unsigned int *queue,
                                                        const unsigned int *qcount, unsigned int seg) {
   const unsigned int n = qcount[blockIdx.x];
   const unsigned int *list = queue + 1 + (size_t)blockIdx.x * seg;
@@ -667,7 +671,7 @@ extern "C" int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int 
   return ICE3_OK;
 }
 
-// Launch k_micro: one workgroup of four wavefronts per compute unit (its LDS holds the
+// Launch k_micro: one workgroup of eight wavefronts per compute unit (its LDS holds the
 // tables and the parked points, so one workgroup fits a CU).
 static hipError_t launch_micro(const ice3_ctx *ctx, int prec, const MicroArgs &m, hipStream_t s) {
   if (prec == ICE3_PREC_FP64)
