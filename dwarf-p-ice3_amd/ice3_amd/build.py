@@ -15,8 +15,14 @@ SOURCES = ("ice3_kernels.hip", "ice3_mlp.hip", "ice3_params.cpp")
 # -ffp-contract=off keeps FP64 a*b+c as two roundings, the sequence of the CPU reference.
 # Packed FP32 instructions (v_pk_*_f32) are disabled for the device code: some of their
 # forms have given wrong results on this hardware without any error.
+# -disable-machine-licm: machine-level loop-invariant code motion hoists the FP64 constants
+# of every inlined exp/log out of k_micro's persistent loop into VGPRs, which then spill to
+# scratch under the 256-register budget of two wavefronts per SIMD; each polynomial step of
+# a log became a dependent scratch reload.  Rebuilding the constants at their uses is
+# cheaper (k_micro<double>: 256 VGPRs + 104 B/lane of spills -> 216 VGPRs, none; -10 %).
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-disable-machine-licm",
          "-I" + os.path.join(ROOT, "include")]
 
 
