@@ -16,8 +16,8 @@
 //                  compacted into full wavefronts (ballot, LDS parking);
 //   k_sg_features, k_mlp_infer, k_sg_decode
 //                  the learned surrogate of k_micro: features and range check per point;
-//                  the MLP on the bf16 matrix cores (persistent grid, weights resident in
-//                  LDS); decode, water and sign checks per point.  Points it rejects are
+//                  the MLP on the bf16 matrix cores (k_mlp_infer, in ice3_mlp.hip);
+//                  decode, water and sign checks per point.  Points it rejects are
 //                  queued for k_micro_segments, the plain FP64 microphysics over the queue.
 // rain_ice is k_sedim then k_micro on one stream; the fused time step is
 // k_ice_adjust, k_sedim, k_micro.  R is the arithmetic of the process rates
@@ -279,83 +279,9 @@ __global__ void __launch_bounds__(256) k_sg_features(SurrogateArgs a) {
   for (int c = 0; c < MLP_IN / 8; ++c) xr[c] = v[c];
 }
 
-// Stage 2: the network over all points.  Persistent grid, all weights resident in LDS,
-// 128-point tiles (32 rows per wave); layer-0 operands come straight from HBM as matrix-
-// core fragments and the next tile's are fetched while the current one is computed.
-__global__ void __launch_bounds__(256) k_mlp_infer(const bf16x8 *__restrict__ wpk,
-                                                   const float *__restrict__ params,
-                                                   const __bf16 *__restrict__ x,
-                                                   float *__restrict__ y, unsigned int npts) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  bf16x8 *sW = (bf16x8 *)smem;
-  float *sB = (float *)(smem + MLP_NW * 2);
-  __bf16 *sH = (__bf16 *)(smem + MLP_NW * 2 + MLP_NB * 4);
-  const int lane = threadIdx.x % 64, w = threadIdx.x / 64, row0 = w * MLP_ROWS;
-  block_copy16(sW, wpk, MLP_NW * 2);
-  for (int q = threadIdx.x; q < MLP_NB; q += blockDim.x) sB[q] = params[MLP_NW + q];
-  __syncthreads();
-  const unsigned int ntiles = (npts + MLP_TILE - 1) / MLP_TILE;
-  // layer-0 A fragments of a tile: rows row0 + mt*16 + lane%16, inputs 8*(lane/16)..+7
-  auto load_a = [&](unsigned int tile, bf16x8 (&a0)[MLP_MT]) {
-#pragma unroll
-    for (int mt = 0; mt < MLP_MT; ++mt) {
-      const unsigned int p = tile * MLP_TILE + row0 + mt * 16 + lane % 16;
-      if (p < npts) a0[mt] = *(const bf16x8 *)(x + (size_t)p * MLP_IN + 8 * (lane / 16));
-      else a0[mt] = bf16x8{};
-    }
-  };
-  bf16x8 a0[MLP_MT];
-  unsigned int tile = blockIdx.x;
-  if (tile < ntiles) load_a(tile, a0);
-  for (; tile < ntiles; tile += gridDim.x) {
-    f32x4 acc[MLP_MT][MLP_H / 16];
-#pragma unroll
-    for (int mt = 0; mt < MLP_MT; ++mt)
-#pragma unroll
-      for (int nn = 0; nn < MLP_H / 16; ++nn) acc[mt][nn] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int nn = 0; nn < MLP_H / 16; ++nn) {
-      const bf16x8 b = sW[mlp_woff(0) / 8 + nn * 64 + lane];
-#pragma unroll
-      for (int mt = 0; mt < MLP_MT; ++mt) acc[mt][nn] = mfma_bf16(a0[mt], b, acc[mt][nn]);
-    }
-    if (tile + gridDim.x < ntiles) load_a(tile + gridDim.x, a0);  // prefetch the next tile
-#pragma unroll
-    for (int mt = 0; mt < MLP_MT; ++mt)
-#pragma unroll
-      for (int nn = 0; nn < MLP_H / 16; ++nn)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int row = row0 + mt * 16 + 4 * (lane / 16) + i, col = nn * 16 + lane % 16;
-          sH[row * MLP_HS + col] = (__bf16)silu(acc[mt][nn][i] + sB[col]);
-        }
-#pragma unroll 1
-    for (int l = 1; l <= 3; ++l) {
-      mlp_layer<MLP_H, MLP_H, MLP_HS>(sH, row0, sW + mlp_woff(l) / 8, acc, lane);
-#pragma unroll
-      for (int mt = 0; mt < MLP_MT; ++mt)
-#pragma unroll
-        for (int nn = 0; nn < MLP_H / 16; ++nn)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int row = row0 + mt * 16 + 4 * (lane / 16) + i, col = nn * 16 + lane % 16;
-            sH[row * MLP_HS + col] = (__bf16)silu(acc[mt][nn][i] + sB[l * MLP_H + col]);
-          }
-    }
-    f32x4 out[MLP_MT][MLP_OUT / 16];
-    mlp_layer<MLP_H, MLP_OUT, MLP_HS>(sH, row0, sW + mlp_woff(4) / 8, out, lane);
-#pragma unroll
-    for (int mt = 0; mt < MLP_MT; ++mt)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const unsigned int p = tile * MLP_TILE + row0 + mt * 16 + 4 * (lane / 16) + i;
-        const int col = lane % 16;
-        if (col < MLP_NTARG && p < npts) y[(size_t)p * MLP_NTARG + col] = out[mt][0][i] + sB[4 * MLP_H + col];
-      }
-  }
-}
+// Stage 2: the network over all points, k_mlp_infer (ice3_mlp.hip, launched by
+// mlp_infer_launch).
 
-constexpr size_t INFER_LDS = (size_t)MLP_NW * 2 + MLP_NB * 4 + MLP_TILE * MLP_HS * 2;
 
 // Stage 3: decode, physical checks, results or fallback queue.  Persistent grid: block b
 // decodes the 256-point chunks b, b + gridDim.x, ... (spread over all levels, so every
@@ -839,9 +765,7 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   hipLaunchKernelGGL(k_sg_features, dim3((npts + 255) / 256), dim3(256), 0, s, g);
   HIPCHK(hipGetLastError());
   MARK(3);
-  hipLaunchKernelGGL(k_mlp_infer, dim3(nblk), dim3(256), INFER_LDS, s,
-                     (const bf16x8 *)sg->weights_packed, sg->params, (const __bf16 *)g.x, g.y, npts);
-  HIPCHK(hipGetLastError());
+  HIPCHK(mlp_infer_launch(sg->weights_packed, sg->params, g.x, g.y, npts, nblk, s));
   MARK(4);
   // decode: a persistent grid of SG_DEC_BLOCKS blocks; block b decodes chunks
   // b, b + ndec, ... of 256 points and owns a queue segment of seg entries

@@ -1,5 +1,5 @@
-// ice3_mlp.hip -- training kernels of the rain_ice surrogate MLP (bf16 matrix cores,
-// fp32 accumulation) and their C ABI (declared in include/ice3.h).
+// ice3_mlp.hip -- kernels of the rain_ice surrogate MLP (bf16 matrix cores, fp32
+// accumulation): training, with its C ABI (declared in include/ice3.h), and inference.
 //
 //   k_mlp_forward   one 128-point tile per workgroup: 5 layers on v_mfma_f32_16x16x32_bf16
 //                   with every weight resident in LDS; saves pre-activations (fp32) and
@@ -10,6 +10,8 @@
 //   k_mlp_reduce    sum of the partial gradients.
 //   k_mlp_adam      Adam on the fp32 master parameters; rewrites the bf16 row-major and
 //                   packed-fragment copies the matrix cores read.
+//   k_mlp_infer     inference over all points of the surrogate step: persistent grid,
+//                   weights resident in LDS (launched by mlp_infer_launch).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -295,6 +297,92 @@ __global__ void __launch_bounds__(256) k_mlp_adam(int step, float lr, float b1, 
     wrm[e] = b;
     wpk[mlp_pack_index(l, k, n)] = b;
   }
+}
+
+// Inference over all points (the surrogate step's network stage).  Persistent grid, all
+// weights resident in LDS,
+// 128-point tiles (32 rows per wave); layer-0 operands come straight from HBM as matrix-
+// core fragments and the next tile's are fetched while the current one is computed.
+__global__ void __launch_bounds__(256) k_mlp_infer(const bf16x8 *__restrict__ wpk,
+                                                   const float *__restrict__ params,
+                                                   const __bf16 *__restrict__ x,
+                                                   float *__restrict__ y, unsigned int npts) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  bf16x8 *sW = (bf16x8 *)smem;
+  float *sB = (float *)(smem + MLP_NW * 2);
+  __bf16 *sH = (__bf16 *)(smem + MLP_NW * 2 + MLP_NB * 4);
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64, row0 = w * MLP_ROWS;
+  block_copy16(sW, wpk, MLP_NW * 2);
+  for (int q = threadIdx.x; q < MLP_NB; q += blockDim.x) sB[q] = params[MLP_NW + q];
+  __syncthreads();
+  const unsigned int ntiles = (npts + MLP_TILE - 1) / MLP_TILE;
+  // layer-0 A fragments of a tile: rows row0 + mt*16 + lane%16, inputs 8*(lane/16)..+7
+  auto load_a = [&](unsigned int tile, bf16x8 (&a0)[MLP_MT]) {
+#pragma unroll
+    for (int mt = 0; mt < MLP_MT; ++mt) {
+      const unsigned int p = tile * MLP_TILE + row0 + mt * 16 + lane % 16;
+      if (p < npts) a0[mt] = *(const bf16x8 *)(x + (size_t)p * MLP_IN + 8 * (lane / 16));
+      else a0[mt] = bf16x8{};
+    }
+  };
+  bf16x8 a0[MLP_MT];
+  unsigned int tile = blockIdx.x;
+  if (tile < ntiles) load_a(tile, a0);
+  for (; tile < ntiles; tile += gridDim.x) {
+    f32x4 acc[MLP_MT][MLP_H / 16];
+#pragma unroll
+    for (int mt = 0; mt < MLP_MT; ++mt)
+#pragma unroll
+      for (int nn = 0; nn < MLP_H / 16; ++nn) acc[mt][nn] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int nn = 0; nn < MLP_H / 16; ++nn) {
+      const bf16x8 b = sW[mlp_woff(0) / 8 + nn * 64 + lane];
+#pragma unroll
+      for (int mt = 0; mt < MLP_MT; ++mt) acc[mt][nn] = mfma_bf16(a0[mt], b, acc[mt][nn]);
+    }
+    if (tile + gridDim.x < ntiles) load_a(tile + gridDim.x, a0);  // prefetch the next tile
+#pragma unroll
+    for (int mt = 0; mt < MLP_MT; ++mt)
+#pragma unroll
+      for (int nn = 0; nn < MLP_H / 16; ++nn)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = row0 + mt * 16 + 4 * (lane / 16) + i, col = nn * 16 + lane % 16;
+          sH[row * MLP_HS + col] = (__bf16)silu(acc[mt][nn][i] + sB[col]);
+        }
+#pragma unroll 1
+    for (int l = 1; l <= 3; ++l) {
+      mlp_layer<MLP_H, MLP_H, MLP_HS>(sH, row0, sW + mlp_woff(l) / 8, acc, lane);
+#pragma unroll
+      for (int mt = 0; mt < MLP_MT; ++mt)
+#pragma unroll
+        for (int nn = 0; nn < MLP_H / 16; ++nn)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = row0 + mt * 16 + 4 * (lane / 16) + i, col = nn * 16 + lane % 16;
+            sH[row * MLP_HS + col] = (__bf16)silu(acc[mt][nn][i] + sB[l * MLP_H + col]);
+          }
+    }
+    f32x4 out[MLP_MT][MLP_OUT / 16];
+    mlp_layer<MLP_H, MLP_OUT, MLP_HS>(sH, row0, sW + mlp_woff(4) / 8, out, lane);
+#pragma unroll
+    for (int mt = 0; mt < MLP_MT; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const unsigned int p = tile * MLP_TILE + row0 + mt * 16 + 4 * (lane / 16) + i;
+        const int col = lane % 16;
+        if (col < MLP_NTARG && p < npts) y[(size_t)p * MLP_NTARG + col] = out[mt][0][i] + sB[4 * MLP_H + col];
+      }
+  }
+}
+
+constexpr size_t INFER_LDS = (size_t)MLP_NW * 2 + MLP_NB * 4 + MLP_TILE * MLP_HS * 2;
+
+hipError_t mlp_infer_launch(const void *wpk, const float *params, const void *x, float *y,
+                            unsigned int npts, unsigned int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_mlp_infer, dim3(nblk), dim3(256), INFER_LDS, s, (const bf16x8 *)wpk,
+                     params, (const __bf16 *)x, y, npts);
+  return hipGetLastError();
 }
 
 }  // namespace ice3

@@ -99,4 +99,10 @@ __device__ __forceinline__ void block_copy16(void *dst, const void *src, int byt
   for (int q = threadIdx.x; q < bytes / 16; q += blockDim.x) d[q] = s[q];
 }
 
+// Launch the inference kernel k_mlp_infer (ice3_mlp.hip): y[npts][MLP_NTARG] from the bf16
+// inputs x[npts][MLP_IN], the packed bf16 weights wpk and the fp32 parameters (biases), on
+// nblk persistent workgroups of 256 threads.
+hipError_t mlp_infer_launch(const void *wpk, const float *params, const void *x, float *y,
+                            unsigned int npts, unsigned int nblk, hipStream_t s);
+
 }  // namespace ice3

@@ -15,15 +15,17 @@ SOURCES = ("ice3_kernels.hip", "ice3_mlp.hip", "ice3_params.cpp")
 # -ffp-contract=off keeps FP64 a*b+c as two roundings, the sequence of the CPU reference.
 # Packed FP32 instructions (v_pk_*_f32) are disabled for the device code: some of their
 # forms have given wrong results on this hardware without any error.
-# -disable-machine-licm: machine-level loop-invariant code motion hoists the FP64 constants
-# of every inlined exp/log out of k_micro's persistent loop into VGPRs, which then spill to
-# scratch under the 256-register budget of two wavefronts per SIMD; each polynomial step of
-# a log became a dependent scratch reload.  Rebuilding the constants at their uses is
-# cheaper (k_micro<double>: 256 VGPRs + 104 B/lane of spills -> 216 VGPRs, none; -10 %).
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-mllvm", "-disable-machine-licm",
          "-I" + os.path.join(ROOT, "include")]
+# Per-source flags.  -disable-machine-licm for the physics kernels: machine-level
+# loop-invariant code motion hoists the FP64 constants of every inlined exp/log out of
+# k_micro's persistent loop into VGPRs, which then spill to scratch under the 256-register
+# budget of two wavefronts per SIMD (each polynomial step of a log became a dependent
+# scratch reload); rebuilding the constants at their uses is cheaper (k_micro<double>:
+# 256 VGPRs + 104 B/lane of spills -> 216 VGPRs, none; -10 %).  The matrix-core kernels
+# (ice3_mlp.hip) keep it: their hoisted LDS addresses pay.
+SOURCE_FLAGS = {"ice3_kernels.hip": ["-mllvm", "-disable-machine-licm"]}
 
 
 def build(verbose: bool = False) -> str:
@@ -36,12 +38,22 @@ def build(verbose: bool = False) -> str:
     gen.generate(os.path.join(CSRC, "ice3_params_gen.inc"))
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr}")
-    if verbose and r.stderr:
-        print(r.stderr)
+    objs = []
+
+    def run(cmd):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed ({r.returncode}):\n{' '.join(cmd)}\n{r.stderr}")
+        if verbose and r.stderr:
+            print(r.stderr)
+
+    for src in SOURCES:
+        obj = os.path.join(os.path.dirname(OUT), os.path.splitext(src)[0] + ".o")
+        run([HIPCC, *FLAGS, *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    for obj in objs:
+        os.remove(obj)
     os.replace(tmp, OUT)
     return OUT
 
