@@ -26,12 +26,84 @@ namespace ice3 {
 
 enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6 };
 
+// ---------------------------------------------------------------- FP64 exp and log
+// The physics evaluates about ten FP64 logarithms and as many exponentials per point and
+// sedimentation level, and more in a hard tendency call.  The library versions keep
+// extended-precision intermediates (about 100 VALU instructions for a log, 40 for an
+// exp); these two stay within about 1 ulp in about 40 and 25, and the parity bar of the
+// physics (1e-11 relative) is five orders of magnitude wider.  Every step is a correctly
+// rounded IEEE operation (frexp, ldexp, rint, fma, division), so host and device builds
+// give the same bits.
+
+// log x: x = 2^e m with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m - 1)/(m + 1),
+// |s| <= 0.1716, from the series 2 atanh(s) = 2s + sum_{k>=1} 2 s^(2k+1) / (2k+1) up to
+// s^21 (the remainder is below 1e-18 relative); log x = e ln2 + log m, ln2 split into a
+// high part with 42 significant bits (e * LN2_HI is exact) and the rest.
+__host__ __device__ __forceinline__ double log_f64(double x) {
+  constexpr double LN2_HI = 0x1.62e42fefa3800p-1, LN2_LO = 0x1.ef35793c76730p-45;
+  constexpr double SQRT_HALF = 0x1.6a09e667f3bcdp-1;
+  int e;
+  double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+  if (m < SQRT_HALF) {
+    m = m + m;
+    e = e - 1;
+  }
+  const double f = m - 1.0;  // exact
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double p = 0x1.8618618618618p-4;       // 2/21
+  p = fma(p, z, 0x1.af286bca1af28p-4);   // 2/19
+  p = fma(p, z, 0x1.e1e1e1e1e1e1ep-4);   // 2/17
+  p = fma(p, z, 0x1.1111111111111p-3);   // 2/15
+  p = fma(p, z, 0x1.3b13b13b13b14p-3);   // 2/13
+  p = fma(p, z, 0x1.745d1745d1746p-3);   // 2/11
+  p = fma(p, z, 0x1.c71c71c71c71cp-3);   // 2/9
+  p = fma(p, z, 0x1.2492492492492p-2);   // 2/7
+  p = fma(p, z, 0x1.999999999999ap-2);   // 2/5
+  p = fma(p, z, 0x1.5555555555555p-1);   // 2/3
+  const double r = fma(s * z, p, s + s);  // log m
+  const double de = (double)e;
+  const double y = fma(de, LN2_HI, fma(de, LN2_LO, r));
+  if (x > 0. && x < INFINITY) return y;
+  return x == 0. ? -INFINITY : (x == INFINITY ? x : NAN);  // log 0, log inf, x < 0 or NaN
+}
+
+// exp x: x = n ln2 + r, n = rint(x / ln2), |r| <= 0.347; e^r from its Taylor series up to
+// r^13 (the remainder is below 5e-18 relative); e^x = 2^n e^r.  x is first clamped to
+// [-760, 720], where e^x is 0 or infinite at either end.
+__host__ __device__ __forceinline__ double exp_f64(double x) {
+  constexpr double LN2_HI = 0x1.62e42fefa3800p-1, LN2_LO = 0x1.ef35793c76730p-45;
+  constexpr double LOG2E = 0x1.71547652b82fep+0;
+  const double xc = fmin(fmax(x, -760.), 720.);
+  const double n = rint(xc * LOG2E);
+  double r = fma(-n, LN2_HI, xc);  // exact: n LN2_HI is exact and close to xc
+  r = fma(-n, LN2_LO, r);
+  double p = 0x1.6124613a86d09p-33;      // 1/13!
+  p = fma(p, r, 0x1.1eed8eff8d898p-29);  // 1/12!
+  p = fma(p, r, 0x1.ae64567f544e4p-26);  // 1/11!
+  p = fma(p, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
+  p = fma(p, r, 0x1.71de3a556c734p-19);  // 1/9!
+  p = fma(p, r, 0x1.a01a01a01a01ap-16);  // 1/8!
+  p = fma(p, r, 0x1.a01a01a01a01ap-13);  // 1/7!
+  p = fma(p, r, 0x1.6c16c16c16c17p-10);  // 1/6!
+  p = fma(p, r, 0x1.1111111111111p-7);   // 1/5!
+  p = fma(p, r, 0x1.5555555555555p-5);   // 1/4!
+  p = fma(p, r, 0x1.5555555555555p-3);   // 1/3!
+  p = fma(p, r, 0.5);                    // 1/2!
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  const double y = ldexp(p, (int)n);
+  return x == x ? y : x;  // NaN stays NaN
+}
+
 template <class R> __device__ __forceinline__ R rexp(R x);
 template <class R> __device__ __forceinline__ R rlog(R x);
 template <class R> __device__ __forceinline__ R rpow(R x, R y);
-template <> __device__ __forceinline__ double rexp<double>(double x) { return exp(x); }
-template <> __device__ __forceinline__ double rlog<double>(double x) { return log(x); }
-template <> __device__ __forceinline__ double rpow<double>(double x, double y) { return pow(x, y); }
+template <> __device__ __forceinline__ double rexp<double>(double x) { return exp_f64(x); }
+template <> __device__ __forceinline__ double rlog<double>(double x) { return log_f64(x); }
+template <> __device__ __forceinline__ double rpow<double>(double x, double y) {
+  return exp_f64(y * log_f64(x));  // x >= 0 at every call site
+}
 template <> __device__ __forceinline__ float rexp<float>(float x) { return expf(x); }
 template <> __device__ __forceinline__ float rlog<float>(float x) { return logf(x); }
 template <> __device__ __forceinline__ float rpow<float>(float x, float y) { return powf(x, y); }
@@ -84,9 +156,9 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   const double ls = P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT);
   const double cph = P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg);
   const double rt = rv + rc + ri * 1.0;
-  const double logt = log(zt);
-  const double pv = dmin(exp(P.XALPW - P.XBETAW / zt - P.XGAMW * logt), .99 * pabs);
-  const double piv = dmin(exp(P.XALPI - P.XBETAI / zt - P.XGAMI * logt), .99 * pabs);
+  const double logt = log_f64(zt);
+  const double pv = dmin(exp_f64(P.XALPW - P.XBETAW / zt - P.XGAMW * logt), .99 * pabs);
+  const double piv = dmin(exp_f64(P.XALPI - P.XBETAI / zt - P.XGAMI * logt), .99 * pabs);
   const double frac = dmax(0., dmin(1., ((P.XTMAXMIX - zt) / (P.XTMAXMIX - P.XTMINMIX))));
   double qsl = P.XRD / P.XRV * pv / (pabs - pv);
   const double qsi = P.XRD / P.XRV * piv / (pabs - piv);
@@ -104,7 +176,7 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   else if (q1 > 2.)
     cond = q1;
   else
-    cond = exp(1.2 * q1 - 1.);
+    cond = exp_f64(1.2 * q1 - 1.);
   cond = cond * sigma;
   double cldfr = cond < 1.E-12 ? 0. : dmax(0., dmin(1., 0.5 + 0.36 * atan(1.55 * q1)));
   if (cldfr == 0.) cond = 0.;
@@ -159,7 +231,7 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   }
   // the 10**(aT+b) threshold only matters with mass-flux ice or cloud fraction
   if (w2 != 0. || cf_mf != 0.) {
-    const double criaut = dmin(P.XCRIAUTI, pow(10., P.XACRIAUTI * (zt - P.XTT) + P.XBCRIAUTI));
+    const double criaut = dmin(P.XCRIAUTI, exp_f64(P.LN_10 * (P.XACRIAUTI * (zt - P.XTT) + P.XBCRIAUTI)));
     const double x = w2 * dt, c = criaut * cf_mf;
     double hcf, hr;
     if (x > cf_mf * criaut) {

@@ -55,6 +55,9 @@ def load():
         lib.hst_ice_adjust.argtypes = common + [ctypes.POINTER(L.AdjustIn), ctypes.POINTER(L.AdjustOut)]
         lib.hst_rain_ice.argtypes = [ctypes.c_int] + common + [
             ctypes.POINTER(L.RainIn), ctypes.POINTER(L.RainIO), ctypes.POINTER(L.RainOut), ctypes.POINTER(L.Stats)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.hst_exp_f64.argtypes = [dp, dp, ctypes.c_long]
+        lib.hst_log_f64.argtypes = [dp, dp, ctypes.c_long]
         lib.hst_adjust_rain_ice.argtypes = [ctypes.c_int] + common + [
             ctypes.POINTER(L.PipeIn), ctypes.POINTER(L.PipeOut), ctypes.POINTER(L.Stats)]
         _lib = lib
@@ -117,3 +120,14 @@ def rain_ice(s: dict, dt: float = 50.0, prec: int = 0):
     assert rc == 0
     res = {**io, **out, "calls": calls, "counters": counters}
     return res
+
+
+def elementwise(fn: str, x):
+    """y = fn(x) for the kernels' FP64 exp ("exp") or log ("log") on a float64 array."""
+    import numpy as np
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    dp = ctypes.POINTER(ctypes.c_double)
+    getattr(lib, f"hst_{fn}_f64")(x.ctypes.data_as(dp), y.ctypes.data_as(dp), x.size)
+    return y

@@ -137,3 +137,11 @@ int main(int argc, char **argv) {
   return 0;
 }
 #endif
+
+// The FP64 exp and log of the physics kernels (ice3_physics.hpp), elementwise.
+extern "C" void hst_exp_f64(const double *x, double *y, long n) {
+  for (long q = 0; q < n; ++q) y[q] = exp_f64(x[q]);
+}
+extern "C" void hst_log_f64(const double *x, double *y, long n) {
+  for (long q = 0; q < n; ++q) y[q] = log_f64(x[q]);
+}

@@ -13,6 +13,12 @@
 #define __forceinline__ inline
 #define __launch_bounds__(x)
 using std::copysign;
+using std::fma;
+using std::fmax;
+using std::fmin;
+using std::frexp;
+using std::ldexp;
+using std::rint;
 using std::exp;
 using std::fabs;
 using std::log;
