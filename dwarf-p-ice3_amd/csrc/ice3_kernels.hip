@@ -7,6 +7,8 @@
 //
 // Kernels:
 //   k_ice_adjust   one thread per (level, column) point, pointwise ICE_ADJUST;
+//   k_adjust_sedim one thread per column: ICE_ADJUST and the sedimentation in one
+//                  top-down walk (the FP64 fused step);
 //   k_sedim<R>     one thread per column, walking the levels top-down: the
 //                  statistical sedimentation flux recurrence, whose fluxes
 //                  never leave registers, and the surface precipitation;
@@ -43,6 +45,14 @@ __global__ void __launch_bounds__(256) k_ice_adjust(AdjustArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.ncol) return;
   adjust_at(a, i, (size_t)blockIdx.y * a.ld + i);
+}
+
+// The FP64 fused step's ICE_ADJUST and sedimentation in one column walk: the adjustment's
+// memory traffic overlaps the sedimentation's FP64 arithmetic.
+__global__ void __launch_bounds__(64) k_adjust_sedim(AdjustArgs aa, SedimArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncol) return;
+  adjust_sedim_column<double>(aa, a, i);
 }
 
 template <class R>
@@ -676,7 +686,14 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
   hipStream_t s = (hipStream_t)stream;
   const dim3 gs((ncol + 255) / 256), gp((ncol + 255) / 256, nlev);
   MARK(0);
-  if (stages & ICE3_STAGE_ADJUST) {
+  if (prec == ICE3_PREC_FP64 && (stages & ICE3_STAGE_ADJUST) &&
+      (stages & ICE3_STAGE_SEDIMENTATION)) {
+    // one column walk for both stages (in the mixed path the float sedimentation is cheap
+    // and the two kernels are faster apart)
+    hipLaunchKernelGGL(k_adjust_sedim, dim3((ncol + 63) / 64), dim3(64), 0, s, aa, sa);
+    HIPCHK(hipGetLastError());
+    stages &= ~ICE3_STAGE_SEDIMENTATION;
+  } else if (stages & ICE3_STAGE_ADJUST) {
     hipLaunchKernelGGL(k_ice_adjust, gp, dim3(256), 0, s, aa);
     HIPCHK(hipGetLastError());
   }

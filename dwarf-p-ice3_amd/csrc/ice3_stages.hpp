@@ -4,6 +4,7 @@
 //   sedim_column  one column, levels walked top-down: the statistical
 //                 sedimentation flux recurrence (fluxes stay in registers) and
 //                 the surface precipitation;
+//   adjust_sedim_column  ICE_ADJUST and the sedimentation of one column in one walk;
 //   micro_at      one point: RAIN_ICE sections 1, 4, 5 and 8 (nucleation
 //                 outside the microphysics mask, the ICE4 time splitting, the
 //                 total tendencies and the negativity correction).
@@ -24,8 +25,8 @@ struct AdjustArgs {
   double *ris_copy;         // optional second copy of the adjusted ris
 };
 
-// ICE_ADJUST at point j of column i (micro/ice_adjust.F90:257-472).
-__device__ __forceinline__ void adjust_at(const AdjustArgs &a, int i, size_t j) {
+// ICE_ADJUST at point j of column i (micro/ice_adjust.F90:257-472); returns the adjusted ris.
+__device__ __forceinline__ double adjust_at(const AdjustArgs &a, int i, size_t j) {
   const ice3_ice_adjust_in &in = a.in;
   double rvs = in.rvs[j], rcs = in.rcs[j], ris = in.ris[j], ths = in.ths[j];
   AdjustOut o;
@@ -52,6 +53,7 @@ __device__ __forceinline__ void adjust_at(const AdjustArgs &a, int i, size_t j) 
   out.ths[j] = ths;
   if (out.srcs) out.srcs[j] = o.srcs;
   if (a.ris_copy) a.ris_copy[j] = ris;
+  return ris;
 }
 
 // Sedimentation stage arguments.  src_* are the sources entering RAIN_ICE,
@@ -77,6 +79,34 @@ __device__ __forceinline__ void sedim_column(const SedimArgs &a, int i) {
     const double rho = a.rhodref[j], dz = a.dzz[j];
     const R lnrho = rlog<R>((R)rho);
     double rrs = a.src_rr[j], ris = a.src_ri[j], rss = a.src_rs[j], rgs = a.src_rg[j];
+    sediment_species<R>(IRR, dt, rho, lnrho, dz, rrs, fr);
+    sediment_species<R>(IRI, dt, rho, lnrho, dz, ris, fi);
+    sediment_species<R>(IRS, dt, rho, lnrho, dz, rss, fs);
+    sediment_species<R>(IRG, dt, rho, lnrho, dz, rgs, fg);
+    a.dst_rr[j] = rrs; a.dst_ri[j] = ris; a.dst_rs[j] = rss; a.dst_rg[j] = rgs;
+  }
+  a.inprc[i] = 0.;
+  a.inprr[i] = fr / P.XRHOLW;
+  a.inpri[i] = fi / P.XRHOLW;
+  a.inprs[i] = fs / P.XRHOLW + fi / P.XRHOLW;
+  a.inprg[i] = fg / P.XRHOLW;
+}
+
+// ICE_ADJUST and the sedimentation of column i in one top-down walk (the FP64 fused step):
+// at each level the point is adjusted, then its precipitating species sediment, ri from the
+// adjusted source held in a register.  The sedimentation of a level needs only that level
+// and the flux from above, so the results are those of adjust_at over all points followed
+// by sedim_column.
+template <class R>
+__device__ __forceinline__ void adjust_sedim_column(const AdjustArgs &aa, const SedimArgs &a, int i) {
+  const double dt = a.dt;
+  double fr = 0., fi = 0., fs = 0., fg = 0.;
+  for (int k = a.nlev - 1; k >= 0; --k) {
+    const size_t j = (size_t)k * a.ld + i;
+    double ris = adjust_at(aa, i, j);
+    const double rho = a.rhodref[j], dz = a.dzz[j];
+    const R lnrho = rlog<R>((R)rho);
+    double rrs = a.src_rr[j], rss = a.src_rs[j], rgs = a.src_rg[j];
     sediment_species<R>(IRR, dt, rho, lnrho, dz, rrs, fr);
     sediment_species<R>(IRI, dt, rho, lnrho, dz, ris, fi);
     sediment_species<R>(IRS, dt, rho, lnrho, dz, rss, fs);

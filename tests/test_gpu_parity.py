@@ -155,3 +155,55 @@ def test_pipeline_full_size_water_budget(dev):
     assert err < 1e-12
     for k in ("rcs", "rrs", "ris", "rss", "rgs"):
         assert torch.isfinite(res[k]).all() and (res[k] >= 0).all(), k
+
+
+def _fused_raw(lib, ctx, dev, f, ncol, nlev, ld, stages=(7,), calls=False):
+    """The fused step through the C ABI on fields padded to a row stride ld >= ncol;
+    stages: the ICE3_STAGE_* masks of successive ice3_adjust_rain_ice_stages calls."""
+    import ctypes
+    from ice3_amd import _lib
+    from ice3_amd._fields import ptr
+    g = {}
+    for k in _lib.PIPE_IN_FIELDS:
+        v = torch.as_tensor(f[k], dtype=torch.float64, device=dev)
+        if v.dim() == 2:
+            pad = torch.full((nlev, ld), -1.0, dtype=torch.float64, device=dev)
+            pad[:, :ncol] = v
+            v = pad
+        g[k] = v.contiguous()
+    o = {k: torch.full((nlev, ld), -7.0, dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[:15]}
+    o.update({k: torch.full((ncol,), -7.0, dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[15:]})
+    c = torch.zeros(nlev * ld, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(3, dtype=torch.int64, device=dev)
+    st = _lib.Stats(ptr(c), ptr(cnt))
+    cin = _lib.PipeIn(*[ptr(g[k]) for k in _lib.PIPE_IN_FIELDS])
+    cout = _lib.PipeOut(*[ptr(o[k]) for k in _lib.PIPE_OUT_FIELDS])
+    sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    for m in stages:
+        rc = lib.ice3_adjust_rain_ice_stages(ctx, 0, ncol, nlev, ld, 50.0, ctypes.byref(cin),
+                                             ctypes.byref(cout), ctypes.byref(st), m, sp)
+        assert rc == 0, rc
+    torch.cuda.synchronize()
+    res = {k: (v[:, :ncol] if v.dim() == 2 else v).cpu().numpy() for k, v in o.items()}
+    res["calls"] = c.view(nlev, ld)[:, :ncol].cpu().numpy()
+    res["counters"] = cnt.cpu().numpy()
+    return res
+
+
+def test_fused_layout_and_stages_bitwise(dev):
+    """The microphysics kernel runs 64-column chunks with the points still being
+    time-stepped parked and resumed across chunks: a row stride ld > ncol, a partial last
+    chunk of each level, and the step split into separate stage calls must all give the
+    dense single-call results bit for bit (the per-point arithmetic does not depend on the
+    order in which points are advanced)."""
+    from ice3_amd import _lib, synthetic
+    lib, ctx = _lib.load(), _lib.context(0)
+    ncol, nlev = 1000, 12                      # 15 full chunks + 40 columns per level
+    f = synthetic.generate(ncol, nlev, seed=20260011, variant="pipeline")
+    f["rhodref"] = f["rho_dry_ref"]
+    dense = _fused_raw(lib, ctx, dev, f, ncol, nlev, ncol)
+    assert dense["counters"][1] == 0 and (dense["calls"] > 1).any()
+    for ld, stages in ((1003, (7,)), (1000, (1 | 2, 4)), (1064, (1, 2, 4))):
+        other = _fused_raw(lib, ctx, dev, f, ncol, nlev, ld, stages)
+        for k in _lib.PIPE_OUT_FIELDS + ("calls", "counters"):
+            assert np.array_equal(np.nan_to_num(other[k], nan=1.0), np.nan_to_num(dense[k], nan=1.0)), (ld, stages, k)
