@@ -754,7 +754,6 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   MicroArgs m;
   pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, stats, aa, sa, m);
   hipStream_t s = (hipStream_t)stream;
-  const dim3 gs((ncol + 255) / 256), gp((ncol + 255) / 256, nlev);
   const unsigned int npts = (unsigned int)nlev * (unsigned int)ncol;
   const SgWork wl = sg_work_layout(npts);
   unsigned char *wb = (unsigned char *)work;
@@ -773,11 +772,9 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   const unsigned int ntiles = (npts + MLP_TILE - 1) / MLP_TILE;
   const unsigned int nblk = ntiles < (unsigned int)ncu ? ntiles : (unsigned int)ncu;
   MARK(0);
-  hipLaunchKernelGGL(k_ice_adjust, gp, dim3(256), 0, s, aa);
+  hipLaunchKernelGGL(k_adjust_sedim, dim3((ncol + 63) / 64), dim3(64), 0, s, aa, sa);
   HIPCHK(hipGetLastError());
   MARK(1);
-  hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
-  HIPCHK(hipGetLastError());
   MARK(2);
   hipLaunchKernelGGL(k_sg_features, dim3((npts + 255) / 256), dim3(256), 0, s, g);
   HIPCHK(hipGetLastError());

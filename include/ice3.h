@@ -207,8 +207,9 @@ int ice3_mlp_adam(int step, float lr, float beta1, float beta2, float eps, float
  * ice3_stage_timing(ctx, 1) makes the three entry points above record HIP events
  * around each of their kernels on the call's stream; ice3_stage_times waits for
  * the last such call and returns its stage durations in milliseconds:
- * ms[0] ICE_ADJUST, ms[1] sedimentation, ms[2] microphysics (in the FP64 fused step the
- * first two stages run as one kernel: ms[0] covers both and ms[1] is about 0); for the surrogate step
+ * ms[0] ICE_ADJUST, ms[1] sedimentation, ms[2] microphysics (in the FP64 fused step and
+ * the surrogate step the first two stages run as one kernel: ms[0] covers both and ms[1]
+ * is about 0); for the surrogate step
  * ms[2] is the feature kernel and ms[3] the network, ms[4] the decode and ms[5] the
  * fallback microphysics (0 for a stage the call did not run).  Off by default; a
  * context's events serve one call at a time. */
