@@ -18,14 +18,20 @@ SOURCES = ("ice3_kernels.hip", "ice3_mlp.hip", "ice3_params.cpp")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + os.path.join(ROOT, "include")]
-# Per-source flags.  -disable-machine-licm for the physics kernels: machine-level
-# loop-invariant code motion hoists the FP64 constants of every inlined exp/log out of
-# k_micro's persistent loop into VGPRs, which then spill to scratch under the 256-register
-# budget of two wavefronts per SIMD (each polynomial step of a log became a dependent
-# scratch reload); rebuilding the constants at their uses is cheaper (k_micro<double>:
-# 256 VGPRs + 104 B/lane of spills -> 216 VGPRs, none; -10 %).  The matrix-core kernels
-# (ice3_mlp.hip) keep it: their hoisted LDS addresses pay.
-SOURCE_FLAGS = {"ice3_kernels.hip": ["-mllvm", "-disable-machine-licm"]}
+# Per-source flags.
+# -disable-machine-licm for the physics kernels: machine-level loop-invariant code motion
+# hoists the FP64 constants of every inlined exp/log out of k_micro's persistent loop into
+# VGPRs, which then spill to scratch under the 256-register budget of two wavefronts per
+# SIMD (each polynomial step of a log became a dependent scratch reload); rebuilding the
+# constants at their uses is cheaper (k_micro<double>: 256 VGPRs + 104 B/lane of spills ->
+# 216 VGPRs, none; -10 %).  The matrix-core kernels (ice3_mlp.hip) keep it: their hoisted
+# LDS addresses pay.
+# The iterative ILP scheduler, for both: the kernels run at a fixed two (k_micro), four
+# (k_adjust_sedim) or one (k_mlp_infer) wavefronts per SIMD and hide latency mostly within
+# a wavefront (k_micro -7 %, k_mlp_infer -5 %; results bit for bit the same).
+ILP = ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]
+SOURCE_FLAGS = {"ice3_kernels.hip": ["-mllvm", "-disable-machine-licm", *ILP],
+                "ice3_mlp.hip": ILP}
 
 
 def build(verbose: bool = False) -> str:
