@@ -27,14 +27,25 @@ _OUT_ORDER = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", 
               "hli_hcf", "cph", "lv", "ls", "rvs", "rcs", "ris", "ths")
 
 
+def check_configuration(cfg, what: str = "phyex"):
+    """Accept the configurations that mean the pinned AROME ICE3 one: None, "AROME", or a
+    configuration object with ``program == "AROME"`` (the reference's ``Phyex("AROME")``,
+    src/ice3/phyex_common/phyex.py:20-47).  Its fields are not read: the kernels implement
+    the pinned configuration only.  Anything else raises ValueError."""
+    prog = getattr(cfg, "program", cfg)
+    if prog is None or (isinstance(prog, str) and prog.upper() == "AROME"):
+        return
+    raise ValueError(f"{what}: only the pinned AROME ICE3 configuration is implemented, got {prog!r}")
+
+
 class IceAdjust:
     """ICE_ADJUST on MI355X.
 
     Parameters
     ----------
-    phyex : placeholder for API compatibility: the configuration is the pinned AROME
-        ICE3 one (KRR=6, sigma_s subgrid condensation, CB02); anything but None/"AROME"
-        raises.
+    phyex : the configuration, kept for API compatibility: None, "AROME" or an AROME
+        ``Phyex`` object; the kernels implement the pinned AROME ICE3 configuration (KRR=6,
+        sigma_s subgrid condensation, CB02) and anything else raises.
     jit : accepted for API compatibility (kernels are compiled ahead of time).
     layout : "columns" for (nlev, ncol) arrays with columns contiguous (the Fortran
         (NIJT, NKT) memory, no copy), or "jax" for the JAX components' (nx, ny, nz).
@@ -42,8 +53,7 @@ class IceAdjust:
     """
 
     def __init__(self, phyex=None, jit: bool = True, layout: str = "columns", device=None):
-        if phyex not in (None, "AROME"):
-            raise ValueError("only the pinned AROME ICE3 configuration is implemented")
+        check_configuration(phyex)
         if layout not in ("columns", "jax"):
             raise ValueError("layout must be 'columns' or 'jax'")
         self.layout = layout

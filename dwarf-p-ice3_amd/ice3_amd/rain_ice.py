@@ -21,14 +21,15 @@ import torch
 
 from . import _lib
 from ._fields import as_columns, from_columns, ptr, stream_ptr
+from .ice_adjust import check_configuration
 
 _PREC = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}
 
 
 class _Base:
     def __init__(self, constants=None, layout: str = "columns", device=None):
-        if constants not in (None, "AROME") and not isinstance(constants, dict):
-            raise ValueError("only the pinned AROME ICE3 configuration is implemented")
+        if not isinstance(constants, dict):  # a constants dict is accepted and not read
+            check_configuration(constants, "constants")
         if layout not in ("columns", "jax"):
             raise ValueError("layout must be 'columns' or 'jax'")
         self.layout = layout

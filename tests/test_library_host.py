@@ -81,3 +81,25 @@ def test_collection_kernel_tables_are_bit_exact(golden_dir):
     mine = _lib.host_constants()
     for k in ("XKER_RACCSS", "XKER_RACCS", "XKER_SACCRG", "XKER_SDRYG", "XKER_RDRYG"):
         assert np.array_equal(np.array(mine[k]), ref[k]), k
+
+
+def test_configuration_objects():
+    """The drop-ins accept what the reference's callers pass for the AROME configuration
+    (None, "AROME", a Phyex("AROME")-like object with .program, a constants dict for
+    RainIce) and reject any other configuration before touching a device."""
+    from ice3_amd.ice_adjust import IceAdjust, check_configuration
+    from ice3_amd.rain_ice import RainIce
+
+    class Cfg:
+        def __init__(self, program):
+            self.program = program
+
+    for ok in (None, "AROME", "arome", Cfg("AROME")):
+        check_configuration(ok)
+    for bad in ("MESO-NH", Cfg("MESO-NH"), 3):
+        with pytest.raises(ValueError):
+            check_configuration(bad)
+    with pytest.raises(ValueError):
+        IceAdjust(phyex=Cfg("MESO-NH"))
+    with pytest.raises(ValueError):
+        RainIce(constants="MESO-NH")
