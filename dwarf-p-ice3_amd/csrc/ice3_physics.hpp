@@ -120,28 +120,16 @@ struct AdjustOut {
   double t, rv, rc, ri, cldfr, hlc_hrc, hlc_hcf, hli_hri, hli_hcf, cph, lv, ls, srcs;
 };
 
-// ZSRC_1D(-22:11), micro/condensation.F90:208-216
-__device__ __forceinline__ double zsrc_1d(int i) {
-  switch (i) {
-    case -22: return 0.;            case -21: return 0.;
-    case -20: return 2.0094444E-04; case -19: return 0.316670E-03;
-    case -18: return 4.9965648E-04; case -17: return 0.785956E-03;
-    case -16: return 1.2341294E-03; case -15: return 0.193327E-02;
-    case -14: return 3.0190963E-03; case -13: return 0.470144E-02;
-    case -12: return 7.2950651E-03; case -11: return 0.112759E-01;
-    case -10: return 1.7350994E-02; case -9: return 0.265640E-01;
-    case -8: return 4.0427860E-02;  case -7: return 0.610997E-01;
-    case -6: return 9.1578111E-02;  case -5: return 0.135888E+00;
-    case -4: return 0.1991484;      case -3: return 0.230756E+00;
-    case -2: return 0.2850565;      case -1: return 0.375050E+00;
-    case 0: return 0.5000000;       case 1: return 0.691489E+00;
-    case 2: return 0.8413813;       case 3: return 0.933222E+00;
-    case 4: return 0.9772662;       case 5: return 0.993797E+00;
-    case 6: return 0.9986521;       case 7: return 0.999768E+00;
-    case 8: return 0.9999684;       case 9: return 0.999997E+00;
-    case 10: return 1.0000000;      default: return 1.000000;
-  }
-}
+// ZSRC_1D(-22:11), micro/condensation.F90:208-216; entry n is ZSRC_1D(n - 22).  A table in
+// memory: a switch over the index compiles to a branch tree that a wavefront with
+// scattered indices walks almost whole, twice per point.
+__device__ constexpr double ZSRC_1D[34] = {
+    0.,            0.,            2.0094444E-04, 0.316670E-03,  4.9965648E-04, 0.785956E-03,
+    1.2341294E-03, 0.193327E-02,  3.0190963E-03, 0.470144E-02,  7.2950651E-03, 0.112759E-01,
+    1.7350994E-02, 0.265640E-01,  4.0427860E-02, 0.610997E-01,  9.1578111E-02, 0.135888E+00,
+    0.1991484,     0.230756E+00,  0.2850565,     0.375050E+00,  0.5000000,     0.691489E+00,
+    0.8413813,     0.933222E+00,  0.9772662,     0.993797E+00,  0.9986521,     0.999768E+00,
+    0.9999684,     0.999997E+00,  1.0000000,     1.000000};
 
 // ICE_ADJUST (micro/ice_adjust.F90:257-472) + ITERATION (:488-565) + CONDENSATION
 // (micro/condensation.F90:270-650) for one point; rvs..ths updated in place.
@@ -183,7 +171,7 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   int inq1 = (int)floor(dmin(100., dmax(-100., 2 * q1)));
   inq1 = inq1 < -22 ? -22 : (inq1 > 10 ? 10 : inq1);
   const double inc = 2. * q1 - (double)inq1;
-  double sigrc = dmin(1., (1. - inc) * zsrc_1d(inq1) + inc * zsrc_1d(inq1 + 1));
+  double sigrc = dmin(1., (1. - inc) * ZSRC_1D[inq1 + 22] + inc * ZSRC_1D[inq1 + 23]);
   double rc_out = (1. - frac) * cond;
   double ri_out = frac * cond;
   zt = zt + ((rc_out - rc) * lv + (ri_out - ri) * ls) / cph;
