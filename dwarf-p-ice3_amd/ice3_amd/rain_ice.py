@@ -6,7 +6,8 @@ their updated values, ``evap3d``/``rainfr`` are added, and the diagnostics hold 
 surface precipitation ``inprc, inprr, inpri, inprs, inprg``.  Arithmetic is PHYEX
 RAIN_ICE (micro/rain_ice.F90) with statistical sedimentation before the
 microphysics, the per-point ICE4 time splitting, the total tendencies and the
-negativity correction, evaluated by the HIP column kernel ``k_column``.
+negativity correction, evaluated by the HIP kernels ``k_sedim`` (one column per thread)
+and ``k_micro`` (the persistent per-point microphysics).
 
 State keys read (RainIceJAX names): exn, rhodref, pres, dzz, th_t, rv_t, rc_t, rr_t, ri_t,
 rs_t, rg_t, ci_t, ths, rvs, rcs, rrs, ris, rss, rgs; optional: exnref (default exn),
