@@ -32,8 +32,9 @@ enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6 };
 // extended-precision intermediates (about 100 VALU instructions for a log, 40 for an
 // exp); these two stay within about 1 ulp in about 40 and 25, and the parity bar of the
 // physics (1e-11 relative) is five orders of magnitude wider.  Every step is a correctly
-// rounded IEEE operation (frexp, ldexp, rint, fma, division), so host and device builds
-// give the same bits.
+// rounded IEEE operation (frexp, ldexp, rint, fma, division), except that the kernels are
+// built with -fapprox-func, under which the device divides by a reciprocal refined with
+// Newton steps (within 1 ulp; in practice the IEEE quotient).
 
 // log x: x = 2^e m with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m - 1)/(m + 1),
 // |s| <= 0.1716, from the series 2 atanh(s) = 2s + sum_{k>=1} 2 s^(2k+1) / (2k+1) up to

@@ -4,8 +4,8 @@ They are built from textbook series (log: frexp reduction and 2 atanh((m-1)/(m+1
 n ln2 reduction and the Taylor series of e^r), so they are pinned here by what the
 mathematics fixes: the C library's values (numpy: glibc, within 1 ulp), the exact
 special cases, and identities exp(log x) = x and log(2^k) = k ln 2.  The CPU build runs the
-same header as the kernels; every step is a correctly rounded IEEE operation, so the GPU
-computes the same bits.
+same header as the kernels; every step is a correctly rounded IEEE operation, except that the
+kernels' division (log's (m - 1)/(m + 1)) is a reciprocal refined by Newton steps, within 1 ulp.
 """
 import os
 import sys
