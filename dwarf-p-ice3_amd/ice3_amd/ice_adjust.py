@@ -27,15 +27,41 @@ _OUT_ORDER = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", 
               "hli_hcf", "cph", "lv", "ls", "rvs", "rcs", "ris", "ths")
 
 
+# The switches of a Phyex object that the kernels implement in one setting only, with the
+# value the pinned configuration needs (the reference's enumerations:
+# src/ice3/phyex_common/nebn.py:6-46 and ice_parameters.py:10-68).  A Phyex("AROME") object
+# carries these values (nebn.py:261-266, ice_parameters.py:408-481); a changed one raises.
+# LSEDIC and LCRIAUTI are not checked: the Python AROME defaults set them (ice_parameters.py:412,
+# 454) where PHYEX's Fortran AROME configuration, which the kernels follow, does not.
+_PINNED = {
+    "nebn": {"LSUBG_COND": 1, "LSIGMAS": 1, "LSTATNW": 0, "LHGT_QS": 0, "CONDENS": 0,
+             "LAMBDA3": 0, "FRAC_ICE_ADJUST": 0},
+    "param_icen": {"LCOND2": 0, "SUBG_MF_PDF": 1, "SEDIM": 1, "LRED": 0, "MRSTEP": 0.0,
+                   "TSTEP_TS": 0.0, "LWARM": 1, "LFEEDBACKT": 1, "LEVLIMIT": 1, "LNULLWETG": 1,
+                   "LWETGPOST": 1, "LCRFLIMIT": 1, "LSNOW_T": 0, "LSEDIM_AFTER": 0, "LDEPOSC": 0,
+                   "SNOW_RIMING": 0, "SUBG_RC_RR_ACCR": 0, "SUBG_RR_EVAP": 0, "SUBG_PR_PDF": 0,
+                   "SUBG_AUCV_RC": 1, "SUBG_AUCV_RI": 0, "PRISTINE_ICE": "PLAT"},
+}
+
+
 def check_configuration(cfg, what: str = "phyex"):
     """Accept the configurations that mean the pinned AROME ICE3 one: None, "AROME", or a
     configuration object with ``program == "AROME"`` (the reference's ``Phyex("AROME")``,
-    src/ice3/phyex_common/phyex.py:20-47).  Its fields are not read: the kernels implement
-    the pinned configuration only.  Anything else raises ValueError."""
+    src/ice3/phyex_common/phyex.py:20-47).  When the object carries ``nebn`` and
+    ``param_icen``, every switch the kernels implement in one setting only must hold that
+    setting.  Anything else raises ValueError."""
     prog = getattr(cfg, "program", cfg)
-    if prog is None or (isinstance(prog, str) and prog.upper() == "AROME"):
-        return
-    raise ValueError(f"{what}: only the pinned AROME ICE3 configuration is implemented, got {prog!r}")
+    if not (prog is None or (isinstance(prog, str) and prog.upper() == "AROME")):
+        raise ValueError(f"{what}: only the pinned AROME ICE3 configuration is implemented, got {prog!r}")
+    for group, fields in _PINNED.items():
+        obj = getattr(cfg, group, None)
+        if obj is None:
+            continue
+        for name, want in fields.items():
+            have = getattr(obj, name, want)
+            if have != want:
+                raise ValueError(f"{what}: {group}.{name} = {have!r}; the kernels implement "
+                                 f"{want!r} only (pinned AROME ICE3 configuration)")
 
 
 class IceAdjust:

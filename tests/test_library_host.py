@@ -103,3 +103,32 @@ def test_configuration_objects():
         IceAdjust(phyex=Cfg("MESO-NH"))
     with pytest.raises(ValueError):
         RainIce(constants="MESO-NH")
+
+    # a Phyex("AROME")-like object with its switch groups: the AROME values pass (including
+    # the Python defaults LSEDIC = LCRIAUTI = 1, which the kernels do not read); a switch
+    # changed away from the pinned configuration raises before any device work
+    class Group:
+        def __init__(self, **kw):
+            self.__dict__.update(kw)
+
+    def arome():
+        c = Cfg("AROME")
+        c.nebn = Group(LSUBG_COND=1, LSIGMAS=1, LSTATNW=0, LHGT_QS=0, CONDENS=0, LAMBDA3=0,
+                       FRAC_ICE_ADJUST=0, VSIGQSAT=0.02)
+        c.param_icen = Group(LCOND2=0, SUBG_MF_PDF=1, SEDIM=1, LRED=0, MRSTEP=0.0, TSTEP_TS=0.0,
+                             LWARM=1, LFEEDBACKT=1, LEVLIMIT=1, LNULLWETG=1, LWETGPOST=1,
+                             LCRFLIMIT=1, LSNOW_T=0, LSEDIM_AFTER=0, LDEPOSC=0, SNOW_RIMING=0,
+                             SUBG_RC_RR_ACCR=0, SUBG_RR_EVAP=0, SUBG_PR_PDF=0, SUBG_AUCV_RC=1,
+                             SUBG_AUCV_RI=0, PRISTINE_ICE="PLAT", LSEDIC=1, LCRIAUTI=1)
+        return c
+
+    check_configuration(arome())
+    for group, name, value in (("nebn", "LSUBG_COND", 0), ("nebn", "FRAC_ICE_ADJUST", 3),
+                               ("param_icen", "SEDIM", 0), ("param_icen", "LCOND2", 1),
+                               ("param_icen", "PRISTINE_ICE", "COLU")):
+        c = arome()
+        setattr(getattr(c, group), name, value)
+        with pytest.raises(ValueError, match=name):
+            check_configuration(c)
+        with pytest.raises(ValueError, match=name):
+            IceAdjust(phyex=c)
