@@ -581,13 +581,22 @@ static bool bad_shape(int ncol, int nlev, int ld, double dt) {
          (double)ld * (double)nlev > 9.0e15 || (double)ncol * (double)nlev > 4294967295.0;
 }
 
+// A domain with no column or no level: a valid call with nothing to do (its field pointers
+// may be NULL, as an empty array's are).
+static bool empty_domain(int ncol, int nlev, int ld, double dt) {
+  return (ncol == 0 || nlev == 0) && ncol >= 0 && nlev >= 0 && ld >= ncol && dt > 0.;
+}
+
+static bool known_prec(int prec) { return prec == ICE3_PREC_FP64 || prec == ICE3_PREC_MIXED; }
+
 #define NN(p) if (!(p)) return ICE3_EINVAL
 
 extern "C" int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double dt,
                                const ice3_ice_adjust_in *in, const ice3_ice_adjust_out *out,
                                void *stream) {
-  if (!ctx || !in || !out || bad_shape(ncol, nlev, ld, dt)) return ICE3_EINVAL;
-  if (prec != ICE3_PREC_FP64 && prec != ICE3_PREC_MIXED) return ICE3_EINVAL;
+  if (!ctx || !in || !out || !known_prec(prec)) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt)) return ICE3_EINVAL;
   NN(in->pabs); NN(in->sigs); NN(in->th); NN(in->exn); NN(in->exn_ref); NN(in->rho_dry_ref);
   NN(in->rv); NN(in->rc); NN(in->ri); NN(in->rr); NN(in->rs); NN(in->rg); NN(in->cf_mf);
   NN(in->rc_mf); NN(in->ri_mf); NN(in->rvs); NN(in->rcs); NN(in->ris); NN(in->ths);
@@ -637,9 +646,9 @@ extern "C" int ice3_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld
                              const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
                              const ice3_rain_ice_out *out, const ice3_stats *stats,
                              void *stream) {
-  if (!ctx || !in || !io || !out || bad_shape(ncol, nlev, ld, dt)) return ICE3_EINVAL;
-  if (prec != ICE3_PREC_FP64 && prec != ICE3_PREC_MIXED) return ICE3_EINVAL;
-  if (nlev > 65535) return ICE3_EINVAL;
+  if (!ctx || !in || !io || !out || !known_prec(prec)) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev > 65535) return ICE3_EINVAL;
   NN(in->exn); NN(in->dzz); NN(in->rhodref); NN(in->exn_ref); NN(in->pabs); NN(in->cldfr);
   NN(in->sigs); NN(in->tht); NN(in->rvt); NN(in->rct); NN(in->rrt); NN(in->rit); NN(in->rst);
   NN(in->rgt);
@@ -675,9 +684,9 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
                                            double dt, const ice3_pipeline_in *in,
                                            const ice3_pipeline_out *out,
                                            const ice3_stats *stats, int stages, void *stream) {
-  if (!ctx || !in || !out || bad_shape(ncol, nlev, ld, dt)) return ICE3_EINVAL;
-  if (prec != ICE3_PREC_FP64 && prec != ICE3_PREC_MIXED) return ICE3_EINVAL;
-  if (nlev > 65535 || stages <= 0 || stages > 7) return ICE3_EINVAL;
+  if (!ctx || !in || !out || !known_prec(prec) || stages <= 0 || stages > 7) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev > 65535) return ICE3_EINVAL;
   if (bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
   AdjustArgs aa;
   SedimArgs sa;
@@ -740,8 +749,9 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
                                               const ice3_surrogate *sg, void *work,
                                               unsigned long long *sg_counters,
                                               const ice3_stats *stats, void *stream) {
-  if (!ctx || !in || !out || !sg || !work || !sg_counters || bad_shape(ncol, nlev, ld, dt))
-    return ICE3_EINVAL;
+  if (!ctx || !in || !out || !sg || !sg_counters) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;  // the work area is then 0 bytes
+  if (!work || bad_shape(ncol, nlev, ld, dt)) return ICE3_EINVAL;
   if (nlev > 65535 || (double)nlev * ncol >= 4.0e9 || (double)nlev * ld >= 4.0e9)
     return ICE3_EINVAL;
   if (((uintptr_t)work & 255) != 0) return ICE3_EINVAL;
@@ -801,7 +811,9 @@ extern "C" int ice3_surrogate_features(ice3_ctx *ctx, int ncol, int nlev, int ld
                                        const ice3_pipeline_in *in, const ice3_pipeline_out *out,
                                        const ice3_surrogate_norm *norm, float *raw, void *x,
                                        unsigned char *micro, void *stream) {
-  if (!ctx || !in || !out || bad_shape(ncol, nlev, ld, dt) || (x && !norm)) return ICE3_EINVAL;
+  if (!ctx || !in || !out || (x && !norm)) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt)) return ICE3_EINVAL;
   if ((double)nlev * ncol >= 4.0e9 || bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
   AdjustArgs aa;
   SedimArgs sa;

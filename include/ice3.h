@@ -9,7 +9,9 @@
  * level.  This is byte-identical to the reference's Fortran (NIJT, NKT) arrays
  * (PHYEX-IAL_CY50T1/bridge/phyex_bridge.F90:96-118, NKL=1 at :128).
  * Per-column arrays have ncol entries.  Calls are asynchronous on the given
- * HIP stream (NULL = default stream); buffers stay owned by the caller.
+ * HIP stream (NULL = default stream); buffers stay owned by the caller.  A call
+ * on an empty domain (ncol or nlev 0, with valid context, precision and dt)
+ * does nothing and returns ICE3_OK; its field pointers may be NULL.
  *
  * Entry points and the reference interfaces they replace:
  *   ice3_ice_adjust      c_ice_adjust   (bridge/phyex_bridge.F90:27-34)  and the JAX

@@ -209,6 +209,52 @@ def test_fused_layout_and_stages_bitwise(dev):
             assert np.array_equal(np.nan_to_num(other[k], nan=1.0), np.nan_to_num(dense[k], nan=1.0)), (ld, stages, k)
 
 
+def test_empty_and_tiny_domains(dev):
+    """An empty domain (no column or no level) is a call that does nothing and succeeds,
+    through the C ABI (NULL field pointers) and the drop-ins (empty arrays in, empty arrays
+    out); negative sizes are still rejected.  One column of two levels (a 64-point chunk
+    with one live lane, a two-level sedimentation walk) matches the oracle."""
+    import ctypes
+    from ice3_amd import AdjustRainIce, IceAdjust, RainIce, _lib, synthetic
+    from oracle import oracle_lib, ref_lib
+    lib, ctx = _lib.load(), _lib.context(0)
+    pin, pout = _lib.PipeIn(), _lib.PipeOut()
+    ain, aout = _lib.AdjustIn(), _lib.AdjustOut()
+    rin, rio, rout = _lib.RainIn(), _lib.RainIO(), _lib.RainOut()
+    for ncol, nlev in ((0, 90), (16, 0), (0, 0)):
+        for prec in (0, 1):
+            assert lib.ice3_ice_adjust(ctx, prec, ncol, nlev, ncol, 50.0, ctypes.byref(ain),
+                                       ctypes.byref(aout), None) == _lib.ICE3_OK
+            assert lib.ice3_rain_ice(ctx, prec, ncol, nlev, ncol, 50.0, ctypes.byref(rin), ctypes.byref(rio),
+                                     ctypes.byref(rout), None, None) == _lib.ICE3_OK
+            assert lib.ice3_adjust_rain_ice(ctx, prec, ncol, nlev, ncol, 50.0, ctypes.byref(pin),
+                                            ctypes.byref(pout), None, None) == _lib.ICE3_OK
+    assert lib.ice3_adjust_rain_ice(ctx, 0, -1, 90, 0, 50.0, ctypes.byref(pin), ctypes.byref(pout),
+                                    None, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_adjust_rain_ice(ctx, 0, 0, 90, 0, 0.0, ctypes.byref(pin), ctypes.byref(pout),
+                                    None, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_adjust_rain_ice(ctx, 7, 0, 90, 0, 50.0, ctypes.byref(pin), ctypes.byref(pout),
+                                    None, None) == _lib.ICE3_EINVAL
+
+    f = synthetic.generate(4, 90, seed=20260013, variant="pipeline")
+    empty = {k: (v[:, :0] if v.ndim == 2 else v[:0]) for k, v in f.items()}
+    names = ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc", "ri", "rr", "rs", "rg",
+             "cf_mf", "rc_mf", "ri_mf", "rvs", "rcs", "ris", "ths")
+    res = IceAdjust()(empty["sigqsat"], *[empty[k] for k in names], timestep=50.0)
+    assert len(res) == 16 and all(tuple(r.shape) == (90, 0) for r in res)
+    state = {k: v[:, :0] for k, v in _rain_state(ref_lib.pipeline_state(f, oracle_lib.ice_adjust(f))).items()}
+    new, diags = RainIce()(state, 50.0)
+    assert tuple(new["rrs"].shape) == (90, 0) and tuple(diags["inprr"].shape) == (0,)
+    out = AdjustRainIce()({**empty, "rhodref": empty["rho_dry_ref"]}, 50.0)
+    assert tuple(out["ths"].shape) == (90, 0)
+
+    g = synthetic.generate(1, 2, seed=20260014, variant="pipeline")
+    res = AdjustRainIce()({**g, "rhodref": g["rho_dry_ref"]}, 50.0, return_calls=True)
+    out = {k: _np(v) for k, v in res.items()}
+    ref = oracle_lib.rain_ice(ref_lib.pipeline_state(g, oracle_lib.ice_adjust(g)), with_iters=True)
+    check_with_flips(out, ref, RAIN_OUT, out["calls"] != ref["iters"], TOL, max_bad=1.0, max_flips=1.0)
+
+
 def test_mixed_precision_error_and_budget(dev):
     """The mixed path (float process rates, slopes and lookups; double state update, time
     splitting, saturation adjustment and sedimentation fluxes) on the benchmark generator
