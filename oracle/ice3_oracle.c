@@ -391,6 +391,275 @@ int oracle_ice_adjust(int ncol, int nlev, double dt, const double *sigqsat,
   return 0;
 }
 
+/* ------------------------------------------------ ICE_ADJUST, OCND2 branch */
+
+/* TIWMX saturation tables (micro/modd_tiwmx.F90: XNDEGR = 100, NSTART = 13200, NSTOP = 37316;
+ * micro/mode_ini_tiwmx.F90:14-40; micro/mode_tiwmx_fun.F90:8-20): entry jk holds the
+ * function at jk/100 K, and ESATW(TIWMX, T) reads entry NINT(100 T) (micro/mode_tiwmx.F90:9-13).
+ * The entries are evaluated here at that temperature instead of being stored. */
+static double tiwmx_temp(double t) {
+  long jk = lround(100. * t);                       /* NINT: half away from zero */
+  if (jk < 13200) jk = 13200;                       /* the reference reads out of bounds there */
+  if (jk > 37316) jk = 37316;
+  return (double)jk / 100.;
+}
+static double esatw_tab(double t) { return esat_w(tiwmx_temp(t)); }
+static double esati_tab(double t) {
+  double tj = tiwmx_temp(t);
+  if (tj > 273.15) return esat_w(tj);               /* mode_ini_tiwmx.F90:29-31 */
+  return esat_i(tj);                                /* mode_tiwmx_fun.F90:12-13, TT < XTT */
+}
+
+/* ICECLOUD (micro/mode_icecloud.F90:52-205) for one point as CONDENSATION calls it
+ * (condensation.F90:379-382): PR = rv, PTSTEP = 1, PPBLH = -1, PWCLD = -1, XW2D = 10.
+ * Outputs the ice cloud fraction and the super-/sub-saturation with respect to ice. */
+static void icecloud(double p, double z, double dz, double t, double rv, double *sifrc,
+                     double *ssio, double *ssiu) {
+  const double sigx = 3.E-4, sigy = 3.E-4, sigz = 1.E-2, xdist = 2500., ydist = 2500.;
+  double r = dmax(0., rv * 1.);
+  double s = 0.;
+  double za = r * p / (P.XEPSILO + r);
+  double esw = esatw_tab(t), esi = esati_tab(t);
+  double rhw = za / esw;
+  double rhi = za / esi;
+  double i2w = esw / esi;
+  double su = dmin(i2w, rhi), so = su;
+  if (t > 273.1 || r <= 0. || esi >= p * 0.5) {
+    su = su - 1.;
+    so = su;
+  } else {
+    double rhin = dmax(0.05, dmin(1., rhw));
+    double drhdz = rhin * P.XG / (t * P.XRD) * (P.XEPSILO * P.XLVTT / (P.XCPD * t) - 1.);
+    double zz = dmin(1., dmax(0., z * 0.001));      /* PPBLH < 0 */
+    double rhdist = sqrt(xdist * sq(sigx) + ydist * sq(sigy) + (1. - zz) * sq(dz * drhdz) +
+                         zz * dz * sq(sigz));
+    if (zz > 0.1) rhdist = rhdist / (1. + rhdist);
+    double rhlim = dmax(0.5, dmin(0.99, 1. - 0.5 * rhdist));
+    double rhdif = (1. - rhw) / (1.0 - rhlim);       /* PWCLD < 0 */
+    rhdif = 1. - sqrt(dmax(0., rhdif));
+    double wcld = dmin(1., dmax(rhdif, 0.0));
+    s = wcld;
+    double rhlimice = 1. + i2w * (rhlim - 1.);
+    if (rhlim <= 0.999) {
+      double rhliminv = 1. / (1. - rhlimice);
+      rhdif = (rhi - rhlimice) * rhliminv;
+      if (wcld == 0.) {
+        s = dmin(1., 0.5 * dmax(0., rhdif));
+      } else {
+        double a = 1. - 1. / i2w;
+        s = dmin(1., a * 0.5 / (1. - rhlim));
+        s = dmin(1., wcld + s);
+      }
+    }
+    if (s > 0.01) {
+      su = s + rhlimice * (1. - s);
+      so = (rhi - (1. - s) * su) / s;
+    } else {
+      s = 0.;
+      double a2 = dmin(0., rhi - rhlimice);
+      su = dmax(0., s + rhlimice * (1. - s) + 2. * a2);
+    }
+    so = dmin(i2w, so);
+    su = dmax(0., su);
+    su = su - 1.;
+    so = so - 1.;
+  }
+  *sifrc = s;
+  *ssio = so;
+  *ssiu = su;
+}
+
+/* ICE_ADJUST with PARAMI%LOCND2 = .TRUE. for one point: CONDENSATION's OCND2 branch
+ * (condensation.F90:263-265 ZPRIFACT = 0 and ZRSW = -9999, :373-394 ICECLOUD and the tabulated
+ * saturation pressures, :412 no ice fraction, :588-637 liquid condensate, noise check and the
+ * separate ice cloud) inside the ICE_ADJUST sequence of ice_adjust_point.  dz is the layer
+ * thickness ZDZ (:376-378), bottom marks the first level (JK == IKTB), cld_wgt PICE_CLD_WGT;
+ * x[] receives PICLDFR, PWCLDFR, PSSIO, PSSIU, PIFR. */
+static void ice_adjust_point_ocnd2(double dt, double sigqsat, double cld_wgt, double z, double dz,
+                                   int bottom, double pabs, double sigs, double th, double exn,
+                                   double exnref, double rhodref, double rv, double rc, double ri,
+                                   double rr, double rs, double rg, double cf_mf, double rc_mf,
+                                   double ri_mf, double *rvs, double *rcs, double *ris,
+                                   double *ths, double *o, double *x) {
+  double zt = th * exn;
+  double lv = P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT);
+  double ls = P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT);
+  double cph = P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg);
+  double rt = rv + rc + ri * 0.;                                    /* ZPRIFACT = 0 */
+  double icldfr, ssio, ssiu;
+  icecloud(pabs, z, dz, zt, rv, &icldfr, &ssio, &ssiu);
+  double pv = dmin(esatw_tab(zt), .99 * pabs);
+  double piv = dmin(esati_tab(zt), .99 * pabs);
+  double frac = 0.;
+  double qsl = P.XRD / P.XRV * pv / (pabs - pv);
+  double qsi = P.XRD / P.XRV * piv / (pabs - piv);
+  qsl = (1. - frac) * qsl + frac * qsi;
+  double lvs = (1. - frac) * lv + frac * ls;
+  double ah = lvs * qsl / (P.XRV * sq(zt)) * (P.XRV * qsl / P.XRD + 1.);
+  double a = 1. / (1. + lvs / cph * ah);
+  double sbar = a * (rt - qsl + ah * lvs * (rc + ri * 0.) / cph);
+  double sigma;
+  if (sigqsat != 0.)
+    sigma = sqrt(sq(2 * sigs) + sq(sigqsat * qsl * a));
+  else
+    sigma = 2 * sigs;
+  sigma = dmax(1.E-10, sigma);
+  double q1 = sbar / sigma;
+  double cond;
+  if (q1 > 0. && q1 <= 2)
+    cond = dmin(exp(-1.) + .66 * q1 + .086 * sq(q1), 2.);
+  else if (q1 > 2.)
+    cond = q1;
+  else
+    cond = exp(1.2 * q1 - 1.);
+  cond = cond * sigma;
+  double cldfr;
+  if (cond < 1.E-12)
+    cldfr = 0.;
+  else
+    cldfr = dmax(0., dmin(1., 0.5 + 0.36 * atan(1.55 * q1)));
+  if (cldfr == 0.) cond = 0.;
+  int inq1 = (int)floor(dmin(100., dmax(-100., 2 * q1)));
+  if (inq1 < -22) inq1 = -22;
+  if (inq1 > 10) inq1 = 10;
+  double inc = 2. * q1 - (double)inq1;
+  double sigrc = dmin(1., (1. - inc) * ZSRC_1D[inq1 + 22] + inc * ZSRC_1D[inq1 + 23]);
+  /* OCND2 condensate (:589-636); ZRSW = -9999, so PRV_IN < ZRSW never holds and ZRSDIF = 0 */
+  double rc_out = (1. - frac) * cond;
+  double lwinc = rc_out - rc, rcold;
+  if (fabs(lwinc) > 1.0E-12 && esatw_tab(zt) < pabs * 0.5) {
+    rcold = rc_out;
+    rc_out = cond - 0.;
+  } else {
+    rcold = rc;
+  }
+  double wcldfr = cldfr;
+  double dum1 = dmin(1.0, 20. * rc_out * sqrt(dz) / qsl);
+  double dum3 = dmax(0., icldfr - wcldfr);
+  double dum4 = bottom ? ri : ri + rs * 0.5 + rg * 0.25;
+  dum4 = dmax(0., dmin(1., cld_wgt * dum4 * sqrt(dz) / qsi));
+  double dum2 = (0.8 * cldfr + 0.2) * dmin(1., dum1 + dum4 * cldfr);
+  cldfr = dmin(1., dum2 + (0.5 * dum3 + 0.5) * dum4);
+  double ri_out = ri;
+  zt = zt + ((rc_out - rcold) * lv + (ri_out - ri) * ls) / cph;
+  double rv_out = rt - rc_out - ri_out * 0.;
+  sigrc = sigrc * dmin(3., dmax(1., 1. - q1));
+  /* ICE_ADJUST after CONDENSATION: as ice_adjust_point */
+  double w1 = (rc_out - rc) / dt;
+  double w2 = (ri_out - ri) / dt;
+  if (w1 < 0.0) w1 = dmax(w1, -*rcs); else w1 = dmin(w1, *rvs);
+  *rvs = *rvs - w1;
+  *rcs = *rcs + w1;
+  *ths = *ths + w1 * lv / (cph * exnref);
+  if (w2 < 0.0) w2 = dmax(w2, -*ris); else w2 = dmin(w2, *rvs);
+  *rvs = *rvs - w2;
+  *ris = *ris + w2;
+  *ths = *ths + w2 * ls / (cph * exnref);
+  double hlc_hcf = 0., hlc_hrc = 0., hli_hcf = 0., hli_hri = 0.;
+  w1 = rc_mf / dt;
+  w2 = ri_mf / dt;
+  if (w1 + w2 > *rvs) {
+    w1 = w1 * *rvs / (w1 + w2);
+    w2 = *rvs - w1;
+  }
+  cldfr = dmin(1., cldfr + cf_mf);
+  *rcs = *rcs + w1;
+  *ris = *ris + w2;
+  *rvs = *rvs - (w1 + w2);
+  *ths = *ths + (w1 * lv + w2 * ls) / cph / exnref;
+  {
+    double criaut = P.XCRIAUTC / rhodref, hcf, hr;
+    double xx = w1 * dt, c = criaut * cf_mf;
+    if (xx > cf_mf * criaut) {
+      hcf = 1. - .5 * sq(criaut * cf_mf / dmax(1.E-20, xx));
+      hr = xx - (c * c * c) / (3 * sq(dmax(1.E-20, xx)));
+    } else if (2. * w1 * dt <= cf_mf * criaut) {
+      hcf = 0.;
+      hr = 0.;
+    } else {
+      hcf = sq(2. * w1 * dt - criaut * cf_mf) / (2. * sq(dmax(1.E-20, xx)));
+      hr = (4. * (xx * xx * xx) - 3. * w1 * dt * sq(c) + (c * c * c)) / (3 * sq(dmax(1.E-20, xx)));
+    }
+    hlc_hcf = dmin(1., hlc_hcf + hcf * cf_mf);
+    hlc_hrc = hlc_hrc + hr;
+  }
+  {
+    double criaut = dmin(P.XCRIAUTI, pow(10., P.XACRIAUTI * (zt - P.XTT) + P.XBCRIAUTI)), hcf, hr;
+    double xx = w2 * dt, c = criaut * cf_mf;
+    if (xx > cf_mf * criaut) {
+      hcf = 1. - .5 * sq(criaut * cf_mf / xx);
+      hr = xx - (c * c * c) / (3 * sq(xx));
+    } else if (2. * w2 * dt <= cf_mf * criaut) {
+      hcf = 0.;
+      hr = 0.;
+    } else {
+      hcf = sq(2. * w2 * dt - criaut * cf_mf) / (2. * sq(xx));
+      hr = (4. * (xx * xx * xx) - 3. * w2 * dt * sq(c) + (c * c * c)) / (3 * sq(xx));
+    }
+    hli_hcf = dmin(1., hli_hcf + hcf * cf_mf);
+    hli_hri = hli_hri + hr;
+  }
+  {
+    double v1 = rc_mf, v2 = ri_mf;
+    if (v1 + v2 > rv_out) {
+      v1 = v1 * rv_out / (v1 + v2);
+      v2 = rv_out - v1;
+    }
+    rc_out = rc_out + v1;
+    ri_out = ri_out + v2;
+    rv_out = rv_out - (v1 + v2);
+    zt = zt + (v1 * lv + v2 * ls) / cph;
+  }
+  o[OA_T] = zt;
+  o[OA_RV] = rv_out;
+  o[OA_RC] = rc_out;
+  o[OA_RI] = ri_out;
+  o[OA_CLDFR] = cldfr;
+  o[OA_HLC_HRC] = hlc_hrc;
+  o[OA_HLC_HCF] = hlc_hcf;
+  o[OA_HLI_HRI] = hli_hri;
+  o[OA_HLI_HCF] = hli_hcf;
+  o[OA_CPH] = cph;
+  o[OA_LV] = lv;
+  o[OA_LS] = ls;
+  o[OA_SRCS] = sigrc;
+  o[OA_TH] = zt / exn;
+  x[OX_ICLDFR] = icldfr;
+  x[OX_WCLDFR] = wcldfr;
+  x[OX_SSIO] = ssio;
+  x[OX_SSIU] = ssiu;
+  x[OX_IFR] = 10.;                                  /* PIFR(:,:) = 10. (condensation.F90:260) */
+}
+
+int oracle_ice_adjust_ocnd2(int ncol, int nlev, double dt, const double *sigqsat,
+                            const double *ice_cld_wgt, const double *zz,
+                            const double *const in[ORACLE_ADJ_NIN],
+                            double *const out[ORACLE_ADJ_NOUT], double *const xout[ORACLE_OCND2_NX]) {
+  if (!P_loaded) return -1;
+  if (nlev < 2) return -2;                          /* ZDZ needs two levels */
+  for (int k = 0; k < nlev; ++k) {
+    int kp = k + 1 < nlev - 1 ? k + 1 : nlev - 1;     /* JKPK (condensation.F90:366) */
+    for (int i = 0; i < ncol; ++i) {
+      size_t j = (size_t)k * ncol + i;
+      double dz = zz[(size_t)kp * ncol + i] - zz[(size_t)(kp - 1) * ncol + i];
+      double rvs = in[IA_RVS][j], rcs = in[IA_RCS][j], ris = in[IA_RIS][j], ths = in[IA_THS][j];
+      double o[ORACLE_ADJ_NOUT_POINT], x[ORACLE_OCND2_NX];
+      ice_adjust_point_ocnd2(dt, sigqsat[i], ice_cld_wgt[i], zz[j], dz, k == 0, in[IA_PABS][j],
+                             in[IA_SIGS][j], in[IA_TH][j], in[IA_EXN][j], in[IA_EXNREF][j],
+                             in[IA_RHODREF][j], in[IA_RV][j], in[IA_RC][j], in[IA_RI][j],
+                             in[IA_RR][j], in[IA_RS][j], in[IA_RG][j], in[IA_CF_MF][j],
+                             in[IA_RC_MF][j], in[IA_RI_MF][j], &rvs, &rcs, &ris, &ths, o, x);
+      for (int m = 0; m < ORACLE_ADJ_NOUT_POINT; ++m) out[m][j] = o[m];
+      out[OA_RVS][j] = rvs;
+      out[OA_RCS][j] = rcs;
+      out[OA_RIS][j] = ris;
+      out[OA_THS][j] = ths;
+      for (int m = 0; m < ORACLE_OCND2_NX; ++m) xout[m][j] = x[m];
+    }
+  }
+  return 0;
+}
+
 /* ============================================================== RAIN_ICE */
 
 /* ICE4_NUCLEATION (micro/ice4_nucleation.func.h:6-144), .NOT.LOCND2, LFEEDBACKT */

@@ -44,6 +44,16 @@ int oracle_load_constants(const char *path);
 int oracle_ice_adjust(int ncol, int nlev, double dt, const double *sigqsat,
                       const double *const in[ORACLE_ADJ_NIN], double *const out[ORACLE_ADJ_NOUT]);
 
+/* extra outputs of ICE_ADJUST with OCND2 */
+enum { OX_ICLDFR, OX_WCLDFR, OX_SSIO, OX_SSIU, OX_IFR, ORACLE_OCND2_NX };
+
+/* ICE_ADJUST with PARAMI%LOCND2 = .TRUE.; zz (nlev, ncol) level heights, ice_cld_wgt per
+ * column.  Returns 0, -1 without constants, -2 when nlev < 2. */
+int oracle_ice_adjust_ocnd2(int ncol, int nlev, double dt, const double *sigqsat,
+                            const double *ice_cld_wgt, const double *zz,
+                            const double *const in[ORACLE_ADJ_NIN],
+                            double *const out[ORACLE_ADJ_NOUT], double *const xout[ORACLE_OCND2_NX]);
+
 /* RAIN_ICE with sedimentation, total tendencies and negativity correction;
  * io[RO_THS..RO_CI] are read and updated in place, io[RO_EVAP3D/RAINFR] written.
  * iters (optional, ncol*nlev ints) receives each point's tendency-call count.

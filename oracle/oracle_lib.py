@@ -71,6 +71,27 @@ def ice_adjust(f: dict, dt: float = 50.0) -> dict:
     return outs
 
 
+OCND2_X = ("icldfr", "wcldfr", "ssio", "ssiu", "ifr")
+
+
+def ice_adjust_ocnd2(f: dict, dt: float = 50.0) -> dict:
+    """ICE_ADJUST with LOCND2; ``f`` adds ``zz`` (nlev, ncol) and ``ice_cld_wgt`` (ncol)."""
+    lib = _load()
+    nlev, ncol = f["th"].shape
+    ins = [_chk(f[k]) for k in ADJ_IN]
+    outs = {k: np.empty((nlev, ncol)) for k in ADJ_OUT}
+    xs = {k: np.empty((nlev, ncol)) for k in OCND2_X}
+    sq, w, zz = _chk(f["sigqsat"]), _chk(f["ice_cld_wgt"]), _chk(f["zz"])
+    rc = lib.oracle_ice_adjust_ocnd2(ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt),
+                                     ctypes.c_void_p(sq.ctypes.data), ctypes.c_void_p(w.ctypes.data),
+                                     ctypes.c_void_p(zz.ctypes.data), _arr(ins),
+                                     _arr([outs[k] for k in ADJ_OUT]), _arr([xs[k] for k in OCND2_X]))
+    if rc != 0:
+        raise RuntimeError(f"oracle_ice_adjust_ocnd2 failed: {rc}")
+    outs.update(xs)
+    return outs
+
+
 def rain_ice(s: dict, dt: float = 50.0, with_iters: bool = False) -> dict:
     """``s`` uses the keys of RAIN_IN plus the sources ths..rgs and ci."""
     lib = _load()

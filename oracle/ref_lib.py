@@ -98,6 +98,33 @@ def _ice_adjust(f: dict, dt: float) -> dict:
     return out
 
 
+OCND2_OUT = ("cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf", "rv_out", "rc_out", "ri_out",
+             "th_out", "srcs", "icldfr", "wcldfr", "ssio", "ssiu", "ifr")
+
+
+def ice_adjust_ocnd2(f: dict, dt: float = 50.0) -> dict:
+    """ICE_ADJUST with LOCND2 = .TRUE.; ``f`` adds ``zz`` (level heights, (nlev, ncol)) and
+    ``ice_cld_wgt`` (per column) to the ice_adjust fields.  Returns the updated sources and
+    the outputs named in OCND2_OUT."""
+    return _chunked(_ice_adjust_ocnd2, f, dt)
+
+
+def _ice_adjust_ocnd2(f: dict, dt: float) -> dict:
+    lib = init(dt)
+    nlev, ncol = f["th"].shape
+    rvs, rcs, ris, ths = (f[k].copy() for k in ("rvs", "rcs", "ris", "ths"))
+    out = {k: np.zeros((nlev, ncol)) for k in OCND2_OUT}
+    lib.ref_ice_adjust_ocnd2(
+        ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt), _p(f["sigqsat"]),
+        _p(f["ice_cld_wgt"]), _p(f["zz"]),
+        *[_p(f[k]) for k in ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc", "ri",
+                             "rr", "rs", "rg", "cf_mf", "rc_mf", "ri_mf")],
+        _p(rvs), _p(rcs), _p(ris), _p(ths),
+        *[_p(out[k]) for k in OCND2_OUT])
+    out.update(rvs=rvs, rcs=rcs, ris=ris, ths=ths)
+    return out
+
+
 def rain_ice(s: dict, dt: float = 50.0, nproma: int = 1) -> dict:
     """Reinstated RAIN_ICE; ``s`` holds exn, dzz, rhodj, rhodref, exn_ref, pabs, ci, cldfr,
     hlc_hrc, hlc_hcf, hli_hri, hli_hcf, tht, rvt, rct, rrt, rit, rst, rgt, sigs and the

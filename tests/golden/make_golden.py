@@ -14,7 +14,11 @@ Runs in the build container only (it needs oracle/_ref/libphyex_ref.so, built by
   from the ground to level 70, so that melting, riming, wet/dry graupel growth and
   homogeneous freezing are all exercised.
 
-Usage: python3 tests/golden/make_golden.py
+* ``ice_adjust_ocnd2_fp64.npz``  (``python3 tests/golden/make_golden.py ocnd2``) ICE_ADJUST
+  with PARAMI%LOCND2 = .TRUE. on the "adjust" generator plus level heights and a per-column
+  ice-cloud weight, and a small case with mass-flux inputs; keys ``<tag>_in_*``/``<tag>_out_*``.
+
+Usage: python3 tests/golden/make_golden.py [ocnd2]
 """
 from __future__ import annotations
 
@@ -50,6 +54,40 @@ def _adjust_case(f, tag, out):
     for k in ADJ_OUT:
         out[f"{tag}_out_{k}"] = a[k]
     return a
+
+
+# cph, lv and ls do not depend on OCND2 (the driver restates them for the plain call only)
+OCND2_OUT = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
+             "rvs", "rcs", "ris", "ths", "srcs", "th_out", "icldfr", "wcldfr", "ssio", "ssiu", "ifr")
+
+
+def ocnd2_inputs(ncol, seed, mass_flux=False):
+    """The "adjust" generator's fields plus what the OCND2 branch reads: level heights zz
+    (the generator's hydrostatic heights) and PICE_CLD_WGT, uniform in [0.5, 1.5]."""
+    f = synthetic.generate(ncol, 90, seed=seed, variant="adjust", dt=DT)
+    rng = np.random.Generator(np.random.Philox(seed + 1))
+    z1 = synthetic.level_profile(90)[2]
+    f["zz"] = np.ascontiguousarray(np.broadcast_to(z1[:, None], f["th"].shape))
+    f["ice_cld_wgt"] = rng.uniform(0.5, 1.5, ncol)
+    if mass_flux:
+        f["cf_mf"] = rng.uniform(0.0, 0.3, f["th"].shape) * (rng.uniform(0, 1, f["th"].shape) < 0.7)
+        f["rc_mf"] = rng.lognormal(np.log(2e-5), 1.0, f["th"].shape) * (f["cf_mf"] > 0)
+        f["ri_mf"] = rng.lognormal(np.log(1e-5), 1.0, f["th"].shape) * (f["cf_mf"] > 0)
+    return f
+
+
+def ocnd2():
+    ref_lib.init(DT, 1)
+    out = {}
+    for tag, ncol, seed, mf in (("base", 32, 20260031, False), ("mf", 8, 20260033, True)):
+        f = ocnd2_inputs(ncol, seed, mf)
+        a = ref_lib.ice_adjust_ocnd2(f, DT)
+        a["t"] = a["th_out"] * f["exn"]
+        for k in ADJ_IN + ("sigqsat", "zz", "ice_cld_wgt"):
+            out[f"{tag}_in_{k}"] = f[k]
+        for k in OCND2_OUT:
+            out[f"{tag}_out_{k}"] = a[k]
+    np.savez_compressed(os.path.join(HERE, "ice_adjust_ocnd2_fp64.npz"), dt=DT, **out)
 
 
 def main():
@@ -98,4 +136,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    ocnd2() if sys.argv[1:] == ["ocnd2"] else main()

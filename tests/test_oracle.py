@@ -164,3 +164,61 @@ def test_oracle_matches_compiled_reference_fresh_seeds(seed):
     r = oracle_lib.rain_ice(s)
     for k in RAIN_OUT:
         assert rel_err(r[k], r_ref[k], floor_for(k, 1e-12)).max() <= 1e-12, k
+
+
+# ------------------------------------------------------------------------- OCND2 branch
+
+OCND2_OUT = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
+             "rvs", "rcs", "ris", "ths", "srcs", "icldfr", "wcldfr", "ssio", "ssiu", "ifr")
+
+
+def _ocnd2_case(g, tag):
+    f = {k[len(tag) + 4:]: g[k] for k in g.files if k.startswith(tag + "_in_")}
+    return f, {k: g[f"{tag}_out_{k}"] for k in OCND2_OUT}
+
+
+@pytest.mark.parametrize("tag", ["base", "mf"])
+def test_ice_adjust_ocnd2_matches_reference_golden(golden_dir, tag):
+    """ICE_ADJUST with LOCND2 (ICECLOUD, the TIWMX tables, the liquid-only condensate)
+    against the compiled reference's vectors."""
+    g = np.load(os.path.join(golden_dir, "ice_adjust_ocnd2_fp64.npz"))
+    f, ref = _ocnd2_case(g, tag)
+    o = oracle_lib.ice_adjust_ocnd2(f, float(g["dt"]))
+    for k in OCND2_OUT:
+        assert np.all(np.isnan(o[k]) == np.isnan(ref[k])), k
+        r = rel_err(o[k], ref[k], floor_for(k, 1e-14))
+        assert np.nanmax(np.where(np.isnan(ref[k]), 0.0, r)) <= 1e-14, (k, r.max())
+
+
+def test_ice_adjust_ocnd2_invariants():
+    """What the OCND2 branch fixes whatever its tables: the condensation leaves the ice
+    alone (ri_out = ri and, without mass flux, the ice source unchanged), total water is
+    conserved point by point, PIFR is its initial 10, PWCLDFR is the statistical cloud
+    fraction, and the fractions stay in [0, 1]."""
+    f = synthetic.generate(48, 90, seed=20260035, variant="adjust")
+    f["zz"] = np.ascontiguousarray(np.broadcast_to(synthetic.level_profile(90)[2][:, None], f["th"].shape))
+    f["ice_cld_wgt"] = np.full(48, 1.0)
+    o = oracle_lib.ice_adjust_ocnd2(f)
+    assert np.array_equal(o["ris"], f["ris"])
+    # the adjusted state POUT_* is NaN where its mass-flux limiter divides 0/0
+    # (ice_adjust.F90:449-466 with rc_mf = ri_mf = 0 and a negative adjusted vapour)
+    ok = np.isfinite(o["ri_out"])
+    assert ok.mean() > 0.8
+    assert np.array_equal(o["ri_out"][ok], f["ri"][ok])
+    water = f["rv"] + f["rc"]
+    assert np.all(np.abs(o["rv_out"] + o["rc_out"] - water)[ok] <= 4e-16 * water[ok])
+    assert np.all(o["ifr"] == 10.0)
+    for k in ("cldfr", "icldfr", "wcldfr"):
+        assert np.all((o[k] >= 0.0) & (o[k] <= 1.0)), k
+    assert np.all(o["ssiu"] >= -1.0) and np.all(o["ssiu"] <= 0.0)
+
+
+@pytest.mark.skipif(not ref_lib.available(), reason="compiled reference not built here")
+def test_ocnd2_oracle_matches_compiled_reference_fresh_seed():
+    f = synthetic.generate(40, 90, seed=303, variant="adjust")
+    f["zz"] = np.ascontiguousarray(np.broadcast_to(synthetic.level_profile(90)[2][:, None], f["th"].shape))
+    f["ice_cld_wgt"] = np.linspace(0.2, 2.0, 40)
+    r = ref_lib.ice_adjust_ocnd2(f)
+    o = oracle_lib.ice_adjust_ocnd2(f)
+    for k in ("rvs", "rcs", "ris", "ths", "cldfr", "srcs", "icldfr", "wcldfr", "ssio", "ssiu", "ifr"):
+        assert rel_err(o[k], r[k], floor_for(k, 1e-14)).max() <= 1e-14, k
