@@ -285,6 +285,7 @@ def main():
     del g
     if not args.no_adjust:
         res.setdefault("variants", {})["ice_adjust_256x256"] = ice_adjust_alone(lib, ctx, dev, args)
+        res["variants"]["ice_adjust_ocnd2_256x256"] = ice_adjust_alone(lib, ctx, dev, args, ocnd2=True)
     print(json.dumps(res))
 
 
@@ -387,9 +388,12 @@ def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
                          "flop_per_point": MLP_FLOP_PER_POINT, "ms_per_launch": stages["network"]}}
 
 
-def ice_adjust_alone(lib, ctx, dev, args, ncol: int = 256 * 256):
+def ice_adjust_alone(lib, ctx, dev, args, ncol: int = 256 * 256, ocnd2: bool = False):
     """ice3_ice_adjust alone, FP64, 256x256 columns x 90 levels of the "adjust" generator:
-    19 inputs + 16 outputs (the IceAdjustJAX tuple) = 280 B/point + 8 B/column."""
+    19 inputs + 16 outputs (the IceAdjustJAX tuple) = 280 B/point + 8 B/column.  With ocnd2,
+    ice3_ice_adjust_ocnd2 (the LOCND2 branch): the level heights are one more input and
+    PICLDFR, PWCLDFR, PSSIO, PSSIU, PIFR five more outputs (328 B/point), and PICE_CLD_WGT
+    one more per-column input (16 B/column)."""
     import torch
 
     from ice3_amd import _lib, synthetic
@@ -402,10 +406,24 @@ def ice_adjust_alone(lib, ctx, dev, args, ncol: int = 256 * 256):
     cout = _lib.AdjustOut(*[ptr(out[k]) for k in _lib.ADJ_OUT_FIELDS[:-1]], ctypes.c_void_p(None))
     stream = torch.cuda.current_stream(dev)
     sp = ctypes.c_void_p(stream.cuda_stream)
+    if ocnd2:
+        z1 = synthetic.level_profile(NLEV)[2]
+        zz = torch.from_numpy(z1).to(dev)[:, None].expand(NLEV, ncol).contiguous()
+        wgt = torch.ones(ncol, dtype=torch.float64, device=dev)
+        x_out = {k: torch.empty((NLEV, ncol), dtype=torch.float64, device=dev) for k in _lib.OCND2_OUT_FIELDS}
+        cin2 = _lib.Ocnd2In(ptr(zz), ptr(wgt))
+        cout2 = _lib.Ocnd2Out(*[ptr(x_out[k]) for k in _lib.OCND2_OUT_FIELDS])
+        bpp, bpc, kernel = 328.0, 16.0, "k_ice_adjust_ocnd2"
 
-    def call():
-        return lib.ice3_ice_adjust(ctx, _lib.PREC_FP64, ncol, NLEV, ncol, DT, ctypes.byref(cin),
-                                   ctypes.byref(cout), sp)
+        def call():
+            return lib.ice3_ice_adjust_ocnd2(ctx, ncol, NLEV, ncol, DT, ctypes.byref(cin), ctypes.byref(cin2),
+                                             ctypes.byref(cout), ctypes.byref(cout2), sp)
+    else:
+        bpp, bpc, kernel = 280.0, 8.0, "k_ice_adjust"
+
+        def call():
+            return lib.ice3_ice_adjust(ctx, _lib.PREC_FP64, ncol, NLEV, ncol, DT, ctypes.byref(cin),
+                                       ctypes.byref(cout), sp)
     for _ in range(args.warmup):
         _lib.check(call(), "ice_adjust warmup")
     torch.cuda.synchronize()
@@ -416,9 +434,9 @@ def ice_adjust_alone(lib, ctx, dev, args, ncol: int = 256 * 256):
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
-    gbs = (280.0 * ncol * NLEV + 8.0 * ncol) / (ms * 1e-3) / 1e9
+    gbs = (bpp * ncol * NLEV + bpc * ncol) / (ms * 1e-3) / 1e9
     return {"columns_per_s": ncol / (ms * 1e-3), "ms_per_step": ms, "columns": ncol,
-            "kernel": "k_ice_adjust", "bytes_per_point": 280.0, "achieved_GBs": gbs,
+            "kernel": kernel, "bytes_per_point": bpp, "achieved_GBs": gbs,
             "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
 
 

@@ -47,6 +47,14 @@ __global__ void __launch_bounds__(256) k_ice_adjust(AdjustArgs a) {
   adjust_at(a, i, (size_t)blockIdx.y * a.ld + i);
 }
 
+// ICE_ADJUST with the OCND2 branch: one thread per point, blockIdx.y the level (the layer
+// thickness reads the heights of the level above).
+__global__ void __launch_bounds__(256) k_ice_adjust_ocnd2(AdjustArgs a, Ocnd2Args x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncol) return;
+  adjust_at<true>(a, i, (size_t)blockIdx.y * a.ld + i, (int)blockIdx.y, &x);
+}
+
 // The FP64 fused step's ICE_ADJUST and sedimentation in one column walk: the adjustment's
 // memory traffic overlaps the sedimentation's FP64 arithmetic.
 __global__ void __launch_bounds__(64) k_adjust_sedim(AdjustArgs aa, SedimArgs a) {
@@ -434,6 +442,7 @@ struct ice3_ctx {
   ice3_params params;
   double *d_tab;
   double h_tab[ICE3_T_TOTAL];
+  double *d_tiwmx;    // TIWMX tables of the OCND2 branch: ESTABW then ESTABI
   int timing;         // record ev[] around the stages of each call
   hipEvent_t ev[7];   // before adjust, after adjust, after sedimentation, then after the
                       // microphysics (plain step: events 3..6 together) or, for the surrogate
@@ -457,16 +466,28 @@ extern "C" int ice3_ctx_create(ice3_ctx **out) {
     free(c);
     return ICE3_ECONFIG;
   }
+  double *tiwmx = (double *)malloc(sizeof(double) * 2 * ICE3_TIWMX_N);
+  if (!tiwmx) {
+    free(c);
+    return ICE3_EDEVICE;
+  }
+  ice3_build_tiwmx(&c->params, tiwmx, tiwmx + ICE3_TIWMX_N);
   if (hipGetDevice(&c->device) != hipSuccess ||
       hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
           hipSuccess || c->ncu <= 0 ||
       hipMalloc((void **)&c->d_tab, sizeof(double) * ICE3_T_TOTAL) != hipSuccess ||
       hipMemcpy(c->d_tab, c->h_tab, sizeof(double) * ICE3_T_TOTAL, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMalloc((void **)&c->d_tiwmx, sizeof(double) * 2 * ICE3_TIWMX_N) != hipSuccess ||
+      hipMemcpy(c->d_tiwmx, tiwmx, sizeof(double) * 2 * ICE3_TIWMX_N, hipMemcpyHostToDevice) !=
           hipSuccess) {
     if (c->d_tab) (void)hipFree(c->d_tab);
+    if (c->d_tiwmx) (void)hipFree(c->d_tiwmx);
+    free(tiwmx);
     free(c);
     return ICE3_EDEVICE;
   }
+  free(tiwmx);
   *out = c;
   return ICE3_OK;
 }
@@ -474,6 +495,7 @@ extern "C" int ice3_ctx_create(ice3_ctx **out) {
 extern "C" void ice3_ctx_destroy(ice3_ctx *c) {
   if (!c) return;
   if (c->d_tab) (void)hipFree(c->d_tab);
+  if (c->d_tiwmx) (void)hipFree(c->d_tiwmx);
   if (c->timing)
     for (int q = 0; q < 7; ++q) (void)hipEventDestroy(c->ev[q]);
   free(c);
@@ -611,6 +633,33 @@ extern "C" int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int 
   hipStream_t s = (hipStream_t)stream;
   MARK(0);
   hipLaunchKernelGGL(k_ice_adjust, grid, dim3(256), 0, s, a);
+  HIPCHK(hipGetLastError());
+  MARK(1); MARK(2); MARK(3); MARK(4); MARK(5); MARK(6);
+  return ICE3_OK;
+}
+
+extern "C" int ice3_ice_adjust_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                     const ice3_ice_adjust_in *in, const ice3_ocnd2_in *in2,
+                                     const ice3_ice_adjust_out *out, const ice3_ocnd2_out *out2,
+                                     void *stream) {
+  if (!ctx || !in || !in2 || !out || !out2) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev < 2 || nlev > 65535) return ICE3_EINVAL;
+  NN(in->pabs); NN(in->sigs); NN(in->th); NN(in->exn); NN(in->exn_ref); NN(in->rho_dry_ref);
+  NN(in->rv); NN(in->rc); NN(in->ri); NN(in->rr); NN(in->rs); NN(in->rg); NN(in->cf_mf);
+  NN(in->rc_mf); NN(in->ri_mf); NN(in->rvs); NN(in->rcs); NN(in->ris); NN(in->ths);
+  NN(in->sigqsat); NN(in2->zz); NN(in2->ice_cld_wgt);
+  NN(out->t); NN(out->rv_out); NN(out->rc_out); NN(out->ri_out); NN(out->cldfr);
+  NN(out->hlc_hrc); NN(out->hlc_hcf); NN(out->hli_hri); NN(out->hli_hcf); NN(out->cph);
+  NN(out->lv); NN(out->ls); NN(out->rvs); NN(out->rcs); NN(out->ris); NN(out->ths);
+  NN(out2->icldfr); NN(out2->wcldfr); NN(out2->ssio); NN(out2->ssiu); NN(out2->ifr);
+  AdjustArgs a = AdjustArgs{};
+  a.ncol = ncol; a.nlev = nlev; a.ld = ld; a.dt = dt; a.in = *in; a.out = *out;
+  Ocnd2Args x{*in2, *out2, ctx->d_tiwmx};
+  dim3 grid((ncol + 255) / 256, nlev);
+  hipStream_t s = (hipStream_t)stream;
+  MARK(0);
+  hipLaunchKernelGGL(k_ice_adjust_ocnd2, grid, dim3(256), 0, s, a, x);
   HIPCHK(hipGetLastError());
   MARK(1); MARK(2); MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;

@@ -299,3 +299,17 @@ void ice3_build_params(ice3_params *P, double *tables) {
   P->LN_1E8 = std::log(1.E8);
   P->LN_10 = std::log(10.);
 }
+
+// INI_TIWMX (micro/mode_ini_tiwmx.F90:14-40) for the two tables the OCND2 branch reads, with
+// ESATW / ESATI of micro/mode_tiwmx_fun.F90:8-14 at T = jk / XNDEGR; above 273.15 K the ice
+// table holds the liquid-water pressure.
+void ice3_build_tiwmx(const ice3_params *P, double *esw, double *esi) {
+  for (int jk = ICE3_TIWMX_START; jk <= ICE3_TIWMX_STOP; ++jk) {
+    const double t = (double)jk / 100.;
+    const double w = std::exp(P->XALPW - P->XBETAW / t - P->XGAMW * std::log(t));
+    const double i = std::exp(P->XALPI - P->XBETAI / t - P->XGAMI * std::log(t));
+    const double sgn = t - P->XTT >= 0. ? 0.5 : -0.5;  // SIGN(0.5, TT - XTT)
+    esw[jk - ICE3_TIWMX_START] = w;
+    esi[jk - ICE3_TIWMX_START] = t > 273.15 ? w : (0.5 + sgn) * w - (sgn - 0.5) * i;
+  }
+}

@@ -46,3 +46,11 @@ enum ice3_table_offsets {
 
 // Builds every constant and the table block (tables must hold ICE3_T_TOTAL doubles).
 void ice3_build_params(ice3_params *p, double *tables);
+
+// TIWMX saturation-pressure tables of the OCND2 branch (micro/modd_tiwmx.F90: XNDEGR = 100,
+// NSTART = 13200, NSTOP = 37316): entry jk - ICE3_TIWMX_START of ESTABW / ESTABI holds the
+// pressure at jk / 100 K, computed as INI_TIWMX does (micro/mode_ini_tiwmx.F90:14-40).
+enum { ICE3_TIWMX_START = 13200, ICE3_TIWMX_STOP = 37316,
+       ICE3_TIWMX_N = ICE3_TIWMX_STOP - ICE3_TIWMX_START + 1 };
+// esw and esi must hold ICE3_TIWMX_N doubles each; p are the constants ice3_build_params made.
+void ice3_build_tiwmx(const ice3_params *p, double *esw, double *esi);

@@ -132,30 +132,122 @@ __device__ constexpr double ZSRC_1D[34] = {
     0.8413813,     0.933222E+00,  0.9772662,     0.993797E+00,  0.9986521,     0.999768E+00,
     0.9999684,     0.999997E+00,  1.0000000,     1.000000};
 
+// ---- OCND2 branch (PARAM_ICE LOCND2 = .TRUE.)
+// TIWMX saturation tables (micro/modd_tiwmx.F90: 0.01 K steps from 132 K to 373.16 K):
+// ESATW(TIWMX, T) reads the entry NINT(100 T) (micro/mode_tiwmx.F90:9-13).  tab holds ESTABW
+// then ESTABI, built on the host as INI_TIWMX does (ice3_build_tiwmx).  Indices outside the
+// table are clamped (the reference reads out of bounds there).
+__host__ __device__ __forceinline__ int tiwmx_index(double t) {
+  const double jk = round(100. * t);  // NINT: half away from zero
+  return (int)(jk < ICE3_TIWMX_START ? ICE3_TIWMX_START : (jk > ICE3_TIWMX_STOP ? ICE3_TIWMX_STOP : jk)) -
+         ICE3_TIWMX_START;
+}
+
+// Per-point inputs and outputs of the OCND2 branch: the TIWMX tables, level height z, layer
+// thickness dz (ZDZ, condensation.F90:376-378), first level of the column, PICE_CLD_WGT;
+// PICLDFR, PWCLDFR, PSSIO, PSSIU out (PIFR is the constant 10, condensation.F90:260).
+struct Ocnd2Point {
+  const double *tiwmx;
+  double z, dz, cld_wgt;
+  bool bottom;
+  double icldfr, wcldfr, ssio, ssiu;
+};
+
+// ICECLOUD (micro/mode_icecloud.F90:52-205) for one point as CONDENSATION calls it
+// (condensation.F90:379-382): PR = rv, PTSTEP = 1, PPBLH = -1, PWCLD = -1.  Returns the ice
+// cloud fraction and the super-/sub-saturation with respect to ice; esw, esi are the
+// tabulated saturation pressures at t.
+__device__ __forceinline__ void icecloud(double p, double z, double dz, double t, double rv,
+                                         double esw, double esi, double &sifrc, double &ssio,
+                                         double &ssiu) {
+  const double sigx = 3.E-4, sigy = 3.E-4, sigz = 1.E-2, xdist = 2500., ydist = 2500.;
+  const double r = dmax(0., rv * 1.);
+  double s = 0.;
+  const double za = r * p / (P.XEPSILO + r);
+  const double rhw = za / esw, rhi = za / esi, i2w = esw / esi;
+  double su = dmin(i2w, rhi), so = su;
+  if (t > 273.1 || r <= 0. || esi >= p * 0.5) {
+    su = su - 1.;
+    so = su;
+  } else {
+    const double rhin = dmax(0.05, dmin(1., rhw));
+    const double drhdz = rhin * P.XG / (t * P.XRD) * (P.XEPSILO * P.XLVTT / (P.XCPD * t) - 1.);
+    const double zz = dmin(1., dmax(0., z * 0.001));  // PPBLH < 0
+    double rhdist = sqrt(xdist * sq(sigx) + ydist * sq(sigy) + (1. - zz) * sq(dz * drhdz) +
+                         zz * dz * sq(sigz));
+    if (zz > 0.1) rhdist = rhdist / (1. + rhdist);
+    const double rhlim = dmax(0.5, dmin(0.99, 1. - 0.5 * rhdist));
+    double rhdif = (1. - rhw) / (1.0 - rhlim);        // PWCLD < 0
+    rhdif = 1. - sqrt(dmax(0., rhdif));
+    const double wcld = dmin(1., dmax(rhdif, 0.0));
+    s = wcld;
+    const double rhlimice = 1. + i2w * (rhlim - 1.);
+    if (rhlim <= 0.999) {
+      rhdif = (rhi - rhlimice) * (1. / (1. - rhlimice));
+      if (wcld == 0.) {
+        s = dmin(1., 0.5 * dmax(0., rhdif));
+      } else {
+        s = dmin(1., (1. - 1. / i2w) * 0.5 / (1. - rhlim));
+        s = dmin(1., wcld + s);
+      }
+    }
+    if (s > 0.01) {
+      su = s + rhlimice * (1. - s);
+      so = (rhi - (1. - s) * su) / s;
+    } else {
+      s = 0.;
+      su = dmax(0., s + rhlimice * (1. - s) + 2. * dmin(0., rhi - rhlimice));
+    }
+    so = dmin(i2w, so);
+    su = dmax(0., su) - 1.;
+    so = so - 1.;
+  }
+  sifrc = s;
+  ssio = so;
+  ssiu = su;
+}
+
 // ICE_ADJUST (micro/ice_adjust.F90:257-472) + ITERATION (:488-565) + CONDENSATION
-// (micro/condensation.F90:270-650) for one point; rvs..ths updated in place.
+// (micro/condensation.F90:270-650) for one point; rvs..ths updated in place.  OCND2 selects
+// the branch of PARAM_ICE LOCND2 = .TRUE. (x then carries its inputs and outputs):
+// ZPRIFACT = 0 (condensation.F90:265), ICECLOUD and the tabulated saturation pressures
+// (:373-394), no ice fraction (:412), and the liquid condensate with its noise check and the
+// separate ice cloud (:588-637).
+template <bool OCND2 = false>
 __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs,
                                         double th, double exn, double exnref, double rhodref,
                                         double rv, double rc, double ri, double rr, double rs,
                                         double rg, double cf_mf, double rc_mf, double ri_mf,
                                         double &rvs, double &rcs, double &ris, double &ths,
-                                        AdjustOut &o) {
+                                        AdjustOut &o, Ocnd2Point *x = nullptr) {
   double zt = th * exn;
   const double lv = P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT);
   const double ls = P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT);
   const double cph = P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg);
-  const double rt = rv + rc + ri * 1.0;
-  const double logt = log_f64(zt);
-  const double pv = dmin(exp_f64(P.XALPW - P.XBETAW / zt - P.XGAMW * logt), .99 * pabs);
-  const double piv = dmin(exp_f64(P.XALPI - P.XBETAI / zt - P.XGAMI * logt), .99 * pabs);
-  const double frac = dmax(0., dmin(1., ((P.XTMAXMIX - zt) / (P.XTMAXMIX - P.XTMINMIX))));
+  const double prifact = OCND2 ? 0. : 1.0;
+  const double rt = rv + rc + ri * prifact;
+  double pv, piv, frac, esw_t = 0.;
+  if constexpr (OCND2) {
+    const int it = tiwmx_index(zt);
+    esw_t = x->tiwmx[it];
+    const double esi_t = x->tiwmx[ICE3_TIWMX_N + it];
+    icecloud(pabs, x->z, x->dz, zt, rv, esw_t, esi_t, x->icldfr, x->ssio, x->ssiu);
+    pv = dmin(esw_t, .99 * pabs);
+    piv = dmin(esi_t, .99 * pabs);
+    frac = 0.;
+  } else {
+    const double logt = log_f64(zt);
+    pv = dmin(exp_f64(P.XALPW - P.XBETAW / zt - P.XGAMW * logt), .99 * pabs);
+    piv = dmin(exp_f64(P.XALPI - P.XBETAI / zt - P.XGAMI * logt), .99 * pabs);
+    frac = dmax(0., dmin(1., ((P.XTMAXMIX - zt) / (P.XTMAXMIX - P.XTMINMIX))));
+  }
   double qsl = P.XRD / P.XRV * pv / (pabs - pv);
   const double qsi = P.XRD / P.XRV * piv / (pabs - piv);
   qsl = (1. - frac) * qsl + frac * qsi;
   const double lvs = (1. - frac) * lv + frac * ls;
   const double ah = lvs * qsl / (P.XRV * sq(zt)) * (P.XRV * qsl / P.XRD + 1.);
   const double a = 1. / (1. + lvs / cph * ah);
-  const double sbar = a * (rt - qsl + ah * lvs * (rc + ri * 1.0) / cph);
+  const double sbar = a * (rt - qsl + ah * lvs * (rc + ri * prifact) / cph);
   double sigma = sigqsat != 0. ? sqrt(sq(2 * sigs) + sq(sigqsat * qsl * a)) : 2 * sigs;
   sigma = dmax(1.E-10, sigma);
   const double q1 = sbar / sigma;
@@ -175,8 +267,26 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   double sigrc = dmin(1., (1. - inc) * ZSRC_1D[inq1 + 22] + inc * ZSRC_1D[inq1 + 23]);
   double rc_out = (1. - frac) * cond;
   double ri_out = frac * cond;
-  zt = zt + ((rc_out - rc) * lv + (ri_out - ri) * ls) / cph;
-  double rv_out = rt - rc_out - ri_out * 1.0;
+  if constexpr (OCND2) {
+    // ZRSW = -9999 (condensation.F90:263): PRV_IN < ZRSW never holds, ZRSDIF = 0
+    double rcold = rc;
+    if (fabs(rc_out - rc) > 1.0E-12 && esw_t < pabs * 0.5) {
+      rcold = rc_out;
+      rc_out = cond - 0.;
+    }
+    x->wcldfr = cldfr;
+    const double dum1 = dmin(1.0, 20. * rc_out * sqrt(x->dz) / qsl);
+    const double dum3 = dmax(0., x->icldfr - x->wcldfr);
+    double dum4 = x->bottom ? ri : ri + rs * 0.5 + rg * 0.25;
+    dum4 = dmax(0., dmin(1., x->cld_wgt * dum4 * sqrt(x->dz) / qsi));
+    const double dum2 = (0.8 * cldfr + 0.2) * dmin(1., dum1 + dum4 * cldfr);
+    cldfr = dmin(1., dum2 + (0.5 * dum3 + 0.5) * dum4);
+    ri_out = ri;
+    zt = zt + ((rc_out - rcold) * lv + (ri_out - ri) * ls) / cph;
+  } else {
+    zt = zt + ((rc_out - rc) * lv + (ri_out - ri) * ls) / cph;
+  }
+  double rv_out = rt - rc_out - ri_out * prifact;
   sigrc = sigrc * dmin(3., dmax(1., 1. - q1));
   // 5.0-5.1 sources
   double w1 = (rc_out - rc) / dt;

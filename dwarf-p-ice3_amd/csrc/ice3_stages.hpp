@@ -25,15 +25,43 @@ struct AdjustArgs {
   double *ris_copy;         // optional second copy of the adjusted ris
 };
 
-// ICE_ADJUST at point j of column i (micro/ice_adjust.F90:257-472); returns the adjusted ris.
-__device__ __forceinline__ double adjust_at(const AdjustArgs &a, int i, size_t j) {
+// The OCND2 branch's extra inputs and outputs (include/ice3.h: ice3_ocnd2_in/_out).
+struct Ocnd2Args {
+  ice3_ocnd2_in in;
+  ice3_ocnd2_out out;
+  const double *tiwmx;  // ESTABW then ESTABI, ICE3_TIWMX_N entries each
+};
+
+// ICE_ADJUST at point j (level k) of column i (micro/ice_adjust.F90:257-472); returns the
+// adjusted ris.  With OCND2 the branch's extra fields come from x: the layer thickness is
+// ZDZ(k) = PZZ(kp) - PZZ(kp-1), kp = min(k+1, nlev-1) (condensation.F90:366,376-378).
+template <bool OCND2 = false>
+__device__ __forceinline__ double adjust_at(const AdjustArgs &a, int i, size_t j, int k = 0,
+                                            const Ocnd2Args *x = nullptr) {
   const ice3_ice_adjust_in &in = a.in;
   double rvs = in.rvs[j], rcs = in.rcs[j], ris = in.ris[j], ths = in.ths[j];
   AdjustOut o;
-  ice_adjust_point(a.dt, in.sigqsat[i], in.pabs[j], in.sigs[j], in.th[j], in.exn[j],
-                   in.exn_ref[j], in.rho_dry_ref[j], in.rv[j], in.rc[j], in.ri[j], in.rr[j],
-                   in.rs[j], in.rg[j], in.cf_mf[j], in.rc_mf[j], in.ri_mf[j], rvs, rcs, ris, ths,
-                   o);
+  Ocnd2Point xp;
+  if constexpr (OCND2) {
+    const int kp = k + 1 < a.nlev - 1 ? k + 1 : a.nlev - 1;
+    const double *zz = x->in.zz;
+    xp.tiwmx = x->tiwmx;
+    xp.z = zz[j];
+    xp.dz = zz[(size_t)kp * a.ld + i] - zz[(size_t)(kp - 1) * a.ld + i];
+    xp.cld_wgt = x->in.ice_cld_wgt[i];
+    xp.bottom = k == 0;
+  }
+  ice_adjust_point<OCND2>(a.dt, in.sigqsat[i], in.pabs[j], in.sigs[j], in.th[j], in.exn[j],
+                          in.exn_ref[j], in.rho_dry_ref[j], in.rv[j], in.rc[j], in.ri[j], in.rr[j],
+                          in.rs[j], in.rg[j], in.cf_mf[j], in.rc_mf[j], in.ri_mf[j], rvs, rcs, ris,
+                          ths, o, &xp);
+  if constexpr (OCND2) {
+    x->out.icldfr[j] = xp.icldfr;
+    x->out.wcldfr[j] = xp.wcldfr;
+    x->out.ssio[j] = xp.ssio;
+    x->out.ssiu[j] = xp.ssiu;
+    x->out.ifr[j] = 10.;  // PIFR(:,:) = 10. (condensation.F90:260)
+  }
   const ice3_ice_adjust_out &out = a.out;
   if (out.t) out.t[j] = o.t;
   if (out.rv_out) out.rv_out[j] = o.rv;

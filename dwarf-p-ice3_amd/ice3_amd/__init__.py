@@ -8,7 +8,7 @@ the PHYEX CY50T1 Fortran's, in FP64 (``precision="fp64"``) or with FP32 process
 rates (``precision="mixed"``); everything runs in hand-written HIP kernels from
 libice3_hip.so, called through its C ABI (include/ice3.h).
 """
-from .ice_adjust import IceAdjust
+from .ice_adjust import IceAdjust, IceAdjustOCND2
 from .rain_ice import AdjustRainIce, RainIce
 
-__all__ = ["IceAdjust", "RainIce", "AdjustRainIce"]
+__all__ = ["IceAdjust", "IceAdjustOCND2", "RainIce", "AdjustRainIce"]

@@ -26,6 +26,8 @@ ADJ_IN_FIELDS = ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "r
                  "rs", "rg", "cf_mf", "rc_mf", "ri_mf", "rvs", "rcs", "ris", "ths", "sigqsat")
 ADJ_OUT_FIELDS = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri",
                   "hli_hcf", "cph", "lv", "ls", "rvs", "rcs", "ris", "ths", "srcs")
+OCND2_IN_FIELDS = ("zz", "ice_cld_wgt")
+OCND2_OUT_FIELDS = ("icldfr", "wcldfr", "ssio", "ssiu", "ifr")
 RAIN_IN_FIELDS = ("exn", "dzz", "rhodj", "rhodref", "exn_ref", "pabs", "cldfr", "hlc_hrc",
                   "hlc_hcf", "hli_hri", "hli_hcf", "sigs", "tht", "rvt", "rct", "rrt", "rit",
                   "rst", "rgt")
@@ -40,6 +42,8 @@ PIPE_OUT_FIELDS = ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "cldfr", "hl
 
 AdjustIn = _struct("ice3_ice_adjust_in", ADJ_IN_FIELDS)
 AdjustOut = _struct("ice3_ice_adjust_out", ADJ_OUT_FIELDS)
+Ocnd2In = _struct("ice3_ocnd2_in", OCND2_IN_FIELDS)
+Ocnd2Out = _struct("ice3_ocnd2_out", OCND2_OUT_FIELDS)
 RainIn = _struct("ice3_rain_ice_in", RAIN_IN_FIELDS)
 RainIO = _struct("ice3_rain_ice_inout", RAIN_IO_FIELDS)
 RainOut = _struct("ice3_rain_ice_out", RAIN_OUT_FIELDS)
@@ -66,7 +70,8 @@ class Surrogate(ctypes.Structure):
 
 
 EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_constants",
-           "ice3_ice_adjust", "ice3_rain_ice", "ice3_adjust_rain_ice", "ice3_stage_timing",
+           "ice3_ice_adjust", "ice3_ice_adjust_ocnd2", "ice3_rain_ice", "ice3_adjust_rain_ice",
+           "ice3_stage_timing",
            "ice3_stage_times", "ice3_adjust_rain_ice_stages", "ice3_adjust_rain_ice_surrogate",
            "ice3_surrogate_features", "ice3_surrogate_work_bytes", "ice3_mlp_sizes", "ice3_mlp_forward", "ice3_mlp_backward",
            "ice3_mlp_adam")
@@ -94,6 +99,9 @@ def load():
         lib.ice3_dump_constants.restype = ctypes.c_long
         common = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         lib.ice3_ice_adjust.argtypes = common + [ctypes.POINTER(AdjustIn), ctypes.POINTER(AdjustOut), _P]
+        lib.ice3_ice_adjust_ocnd2.argtypes = [
+            _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(AdjustIn),
+            ctypes.POINTER(Ocnd2In), ctypes.POINTER(AdjustOut), ctypes.POINTER(Ocnd2Out), _P]
         lib.ice3_rain_ice.argtypes = common + [ctypes.POINTER(RainIn), ctypes.POINTER(RainIO),
                                                ctypes.POINTER(RainOut), ctypes.POINTER(Stats), _P]
         lib.ice3_adjust_rain_ice.argtypes = common + [ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut),

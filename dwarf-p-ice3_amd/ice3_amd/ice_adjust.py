@@ -44,12 +44,13 @@ _PINNED = {
 }
 
 
-def check_configuration(cfg, what: str = "phyex"):
+def check_configuration(cfg, what: str = "phyex", ocnd2: bool = False):
     """Accept the configurations that mean the pinned AROME ICE3 one: None, "AROME", or a
     configuration object with ``program == "AROME"`` (the reference's ``Phyex("AROME")``,
     src/ice3/phyex_common/phyex.py:20-47).  When the object carries ``nebn`` and
     ``param_icen``, every switch the kernels implement in one setting only must hold that
-    setting.  Anything else raises ValueError."""
+    setting (``LCOND2`` = 1 for the OCND2 entry point, 0 otherwise).  Anything else raises
+    ValueError."""
     prog = getattr(cfg, "program", cfg)
     if not (prog is None or (isinstance(prog, str) and prog.upper() == "AROME")):
         raise ValueError(f"{what}: only the pinned AROME ICE3 configuration is implemented, got {prog!r}")
@@ -58,6 +59,8 @@ def check_configuration(cfg, what: str = "phyex"):
         if obj is None:
             continue
         for name, want in fields.items():
+            if name == "LCOND2":
+                want = 1 if ocnd2 else 0
             have = getattr(obj, name, want)
             if have != want:
                 raise ValueError(f"{what}: {group}.{name} = {have!r}; the kernels implement "
@@ -78,14 +81,35 @@ class IceAdjust:
     device : CUDA device (default: current).
     """
 
+    _OCND2 = False
+
     def __init__(self, phyex=None, jit: bool = True, layout: str = "columns", device=None):
-        check_configuration(phyex)
+        check_configuration(phyex, ocnd2=self._OCND2)
         if layout not in ("columns", "jax"):
             raise ValueError("layout must be 'columns' or 'jax'")
         self.layout = layout
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         self._lib = _lib.load()
         self._ctx = _lib.context(self.device.index)
+
+    def _inputs(self, sigqsat, args):
+        """Columns-layout device tensors of the fields and the per-column sigqsat."""
+        f = {k: as_columns(v, self.layout, device=self.device, name=k) for k, v in args.items()}
+        nlev, ncol = f["th"].shape
+        for k, v in f.items():
+            if tuple(v.shape) != (nlev, ncol):
+                raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
+        sq = self._per_column(sigqsat, "sigqsat", ncol)
+        return f, sq, nlev, ncol
+
+    def _per_column(self, x, name, ncol):
+        t = torch.as_tensor(x, dtype=torch.float64, device=self.device)
+        if t.dim() > 1:  # a full field: the value is per column, take the lowest level
+            t = as_columns(t, self.layout, device=self.device, name=name)[0]
+        t = t.reshape(-1).contiguous()
+        if t.numel() != ncol:
+            raise ValueError(f"{name}: {t.numel()} values for {ncol} columns")
+        return t
 
     def __call__(self, sigqsat, pabs, sigs, th, exn, exn_ref, rho_dry_ref, rv, rc, ri, rr, rs, rg,
                  cf_mf, rc_mf, ri_mf, rvs, rcs, ris, ths, timestep: float,
@@ -94,17 +118,7 @@ class IceAdjust:
                     rv=rv, rc=rc, ri=ri, rr=rr, rs=rs, rg=rg, cf_mf=cf_mf, rc_mf=rc_mf, ri_mf=ri_mf,
                     rvs=rvs, rcs=rcs, ris=ris, ths=ths)
         shape = tuple(torch.as_tensor(th).shape)
-        f = {k: as_columns(v, self.layout, device=self.device, name=k) for k, v in args.items()}
-        nlev, ncol = f["th"].shape
-        for k, v in f.items():
-            if tuple(v.shape) != (nlev, ncol):
-                raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
-        sq = torch.as_tensor(sigqsat, dtype=torch.float64, device=self.device)
-        if sq.dim() > 1:  # a full field: PSIGQSAT is per column, take the lowest level
-            sq = as_columns(sq, self.layout, device=self.device, name="sigqsat")[0]
-        sq = sq.reshape(-1).contiguous()
-        if sq.numel() != ncol:
-            raise ValueError(f"sigqsat: {sq.numel()} values for {ncol} columns")
+        f, sq, nlev, ncol = self._inputs(sigqsat, args)
         out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) for k in _OUT_ORDER}
         srcs = torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) if return_srcs else None
         cin = _lib.AdjustIn(*[ptr(f[k]) for k in _lib.ADJ_IN_FIELDS[:-1]], ptr(sq))
@@ -115,6 +129,52 @@ class IceAdjust:
         _lib.check(rc_, "ice3_ice_adjust")
         res = [out[k] for k in _OUT_ORDER]
         res = [from_columns(r, self.layout, shape) for r in res]
+        if srcs is not None:
+            srcs = from_columns(srcs, self.layout, shape)
+        return (*res, srcs) if return_srcs else tuple(res)
+
+
+class IceAdjustOCND2(IceAdjust):
+    """ICE_ADJUST with PARAM_ICE LOCND2 = .TRUE. on MI355X (``ice3_ice_adjust_ocnd2``): the
+    liquid-only condensation and the separate ice cloud of ICECLOUD over the TIWMX tables
+    (micro/condensation.F90:263-265,373-394,582-637, micro/mode_icecloud.F90).
+
+    The reference's JAX component pins OCND2 = False (src/ice3/jax/ice_adjust.py:116); this
+    class follows the Fortran ICE_ADJUST called with LOCND2.  The call takes IceAdjust's
+    arguments plus ``zz`` (level heights in m, increasing upward, PZZ) and ``ice_cld_wgt``
+    (PICE_CLD_WGT, per column), in FP64, and returns IceAdjust's 16-tuple followed by
+    ``icldfr, wcldfr, ssio, ssiu, ifr`` (PICLDFR, PWCLDFR, PSSIO, PSSIU, PIFR).  A ``phyex``
+    object must have ``param_icen.LCOND2 = 1``.
+    """
+
+    _OCND2 = True
+
+    def __call__(self, sigqsat, pabs, sigs, th, exn, exn_ref, rho_dry_ref, rv, rc, ri, rr, rs, rg,
+                 cf_mf, rc_mf, ri_mf, rvs, rcs, ris, ths, timestep: float, zz=None,
+                 ice_cld_wgt=None, return_srcs: bool = False):
+        if zz is None or ice_cld_wgt is None:
+            raise ValueError("the OCND2 branch needs zz (level heights) and ice_cld_wgt")
+        args = dict(pabs=pabs, sigs=sigs, th=th, exn=exn, exn_ref=exn_ref, rho_dry_ref=rho_dry_ref,
+                    rv=rv, rc=rc, ri=ri, rr=rr, rs=rs, rg=rg, cf_mf=cf_mf, rc_mf=rc_mf, ri_mf=ri_mf,
+                    rvs=rvs, rcs=rcs, ris=ris, ths=ths, zz=zz)
+        shape = tuple(torch.as_tensor(th).shape)
+        f, sq, nlev, ncol = self._inputs(sigqsat, args)
+        wgt = self._per_column(ice_cld_wgt, "ice_cld_wgt", ncol)
+        if nlev == 1:
+            raise ValueError("the OCND2 branch needs at least two levels")
+        names = _OUT_ORDER + _lib.OCND2_OUT_FIELDS
+        out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) for k in names}
+        srcs = torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) if return_srcs else None
+        cin = _lib.AdjustIn(*[ptr(f[k]) for k in _lib.ADJ_IN_FIELDS[:-1]], ptr(sq))
+        cin2 = _lib.Ocnd2In(ptr(f["zz"]), ptr(wgt))
+        cout = _lib.AdjustOut(*[ptr(out[k]) for k in _lib.ADJ_OUT_FIELDS[:-1]], ptr(srcs))
+        cout2 = _lib.Ocnd2Out(*[ptr(out[k]) for k in _lib.OCND2_OUT_FIELDS])
+        with torch.cuda.device(self.device):
+            rc_ = self._lib.ice3_ice_adjust_ocnd2(self._ctx, ncol, nlev, ncol, float(timestep),
+                                                  ctypes.byref(cin), ctypes.byref(cin2), ctypes.byref(cout),
+                                                  ctypes.byref(cout2), stream_ptr(self.device))
+        _lib.check(rc_, "ice3_ice_adjust_ocnd2")
+        res = [from_columns(out[k], self.layout, shape) for k in names]
         if srcs is not None:
             srcs = from_columns(srcs, self.layout, shape)
         return (*res, srcs) if return_srcs else tuple(res)

@@ -78,6 +78,27 @@ int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double 
                     const ice3_ice_adjust_in *in, const ice3_ice_adjust_out *out,
                     void *stream);
 
+/* ---- ice_adjust with OCND2: ICE_ADJUST called with PARAM_ICE LOCND2 = .TRUE.
+ * (micro/ice_adjust.F90:533-542 passing PARAMI%LOCND2 to CONDENSATION): liquid-only
+ * statistical condensation, its noise check, and the separate ice cloud of ICECLOUD
+ * (micro/mode_icecloud.F90) over the TIWMX saturation tables (micro/mode_tiwmx.F90); the
+ * OCND2 branch of micro/condensation.F90:263-265,373-394,582-637.  The reference reaches it
+ * through the Fortran ICE_ADJUST only (its JAX component pins OCND2 = False,
+ * src/ice3/jax/ice_adjust.py:116).  FP64; nlev >= 2 (the layer thickness needs two levels). */
+typedef struct {
+  const double *zz;           /* PZZ: level heights (m), (nlev, ld), increasing upward */
+  const double *ice_cld_wgt;  /* PICE_CLD_WGT, per column */
+} ice3_ocnd2_in;
+
+typedef struct {   /* PICLDFR, PWCLDFR, PSSIO, PSSIU, PIFR */
+  double *icldfr, *wcldfr, *ssio, *ssiu, *ifr;
+} ice3_ocnd2_out;
+
+int ice3_ice_adjust_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                          const ice3_ice_adjust_in *in, const ice3_ocnd2_in *in2,
+                          const ice3_ice_adjust_out *out, const ice3_ocnd2_out *out2,
+                          void *stream);
+
 /* ---- rain_ice: RAIN_ICE (micro/rain_ice.F90:7) with statistical sedimentation,
  *      total tendencies and negativity correction ---- */
 typedef struct {

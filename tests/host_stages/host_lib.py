@@ -53,6 +53,9 @@ def load():
         lib = ctypes.CDLL(so)
         common = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         lib.hst_ice_adjust.argtypes = common + [ctypes.POINTER(L.AdjustIn), ctypes.POINTER(L.AdjustOut)]
+        lib.hst_ice_adjust_ocnd2.argtypes = common + [
+            ctypes.POINTER(L.AdjustIn), ctypes.POINTER(L.Ocnd2In), ctypes.POINTER(L.AdjustOut),
+            ctypes.POINTER(L.Ocnd2Out)]
         lib.hst_rain_ice.argtypes = [ctypes.c_int] + common + [
             ctypes.POINTER(L.RainIn), ctypes.POINTER(L.RainIO), ctypes.POINTER(L.RainOut), ctypes.POINTER(L.Stats)]
         dp = ctypes.POINTER(ctypes.c_double)
@@ -97,6 +100,25 @@ def adjust_rain_ice(f: dict, dt: float = 50.0, prec: int = 0):
     out["calls"] = calls
     out["counters"] = counters
     return out
+
+
+def ice_adjust_ocnd2(f: dict, dt: float = 50.0):
+    """ICE_ADJUST with OCND2 on host arrays: the ice_adjust fields plus zz and ice_cld_wgt."""
+    import numpy as np
+    from ice3_amd import _lib as L
+    lib = load()
+    nlev, ncol = f["th"].shape
+    inp = {k: np.ascontiguousarray(f[k], dtype=np.float64) for k in L.ADJ_IN_FIELDS}
+    x_in = {k: np.ascontiguousarray(f[k], dtype=np.float64) for k in L.OCND2_IN_FIELDS}
+    out = {k: np.empty((nlev, ncol)) for k in L.ADJ_OUT_FIELDS}
+    x_out = {k: np.empty((nlev, ncol)) for k in L.OCND2_OUT_FIELDS}
+    rc = lib.hst_ice_adjust_ocnd2(
+        ncol, nlev, ncol, dt, ctypes.byref(L.AdjustIn(*[_p(inp[k]) for k in L.ADJ_IN_FIELDS])),
+        ctypes.byref(L.Ocnd2In(*[_p(x_in[k]) for k in L.OCND2_IN_FIELDS])),
+        ctypes.byref(L.AdjustOut(*[_p(out[k]) for k in L.ADJ_OUT_FIELDS])),
+        ctypes.byref(L.Ocnd2Out(*[_p(x_out[k]) for k in L.OCND2_OUT_FIELDS])))
+    assert rc == 0
+    return {**out, **x_out}
 
 
 def rain_ice(s: dict, dt: float = 50.0, prec: int = 0):

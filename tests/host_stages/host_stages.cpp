@@ -13,14 +13,24 @@
 using namespace ice3;
 
 static double *g_tab = nullptr;
+static double *g_tiwmx = nullptr;  // TIWMX tables of the OCND2 branch
 
 extern "C" int hst_init(void) {
-  if (!g_tab) {
-    g_tab = (double *)malloc(sizeof(double) * ICE3_T_TOTAL);
-    if (!g_tab) return -1;
+  if (!g_tab || !g_tiwmx) {
+    double *tab = (double *)malloc(sizeof(double) * ICE3_T_TOTAL);
+    double *tiwmx = (double *)malloc(sizeof(double) * 2 * ICE3_TIWMX_N);
     ice3_params p;
-    ice3_build_params(&p, g_tab);
-    if (memcmp(&p, &P, sizeof p) != 0) return -2;  // stale generated constants
+    if (tab && tiwmx) {
+      ice3_build_params(&p, tab);
+      ice3_build_tiwmx(&p, tiwmx, tiwmx + ICE3_TIWMX_N);
+    }
+    if (!tab || !tiwmx || memcmp(&p, &P, sizeof p) != 0) {  // no memory, or stale constants
+      free(tab);
+      free(tiwmx);
+      return tab && tiwmx ? -2 : -1;
+    }
+    g_tab = tab;
+    g_tiwmx = tiwmx;
   }
   return 0;
 }
@@ -46,6 +56,18 @@ extern "C" int hst_ice_adjust(int ncol, int nlev, int ld, double dt, const ice3_
   a.ncol = ncol; a.nlev = nlev; a.ld = ld; a.dt = dt; a.in = *in; a.out = *out;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) adjust_at(a, i, (size_t)k * ld + i);
+  return 0;
+}
+
+extern "C" int hst_ice_adjust_ocnd2(int ncol, int nlev, int ld, double dt,
+                                    const ice3_ice_adjust_in *in, const ice3_ocnd2_in *in2,
+                                    const ice3_ice_adjust_out *out, const ice3_ocnd2_out *out2) {
+  if (hst_init()) return -1;
+  AdjustArgs a = AdjustArgs{};
+  a.ncol = ncol; a.nlev = nlev; a.ld = ld; a.dt = dt; a.in = *in; a.out = *out;
+  const Ocnd2Args x{*in2, *out2, g_tiwmx};
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) adjust_at<true>(a, i, (size_t)k * ld + i, k, &x);
   return 0;
 }
 

@@ -100,6 +100,46 @@ def test_ice_adjust_vs_oracle_config2_slab(dev):
     _check(out, ref, ADJ_OUT + ("srcs",))
 
 
+OCND2_X = ("icldfr", "wcldfr", "ssio", "ssiu", "ifr")
+OCND2_CHECKED = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
+                 "rvs", "rcs", "ris", "ths", "srcs") + OCND2_X
+
+
+def _adjust_ocnd2_gpu(f, dt=50.0):
+    from ice3_amd import IceAdjustOCND2
+    args = [f[k] for k in ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc", "ri",
+                           "rr", "rs", "rg", "cf_mf", "rc_mf", "ri_mf", "rvs", "rcs", "ris", "ths")]
+    res = IceAdjustOCND2()(f["sigqsat"], *args, timestep=dt, zz=f["zz"], ice_cld_wgt=f["ice_cld_wgt"],
+                           return_srcs=True)
+    return {k: _np(v) for k, v in zip(ADJ_OUT + OCND2_X + ("srcs",), res)}
+
+
+@pytest.mark.parametrize("tag", ["base", "mf"])
+def test_ice_adjust_ocnd2_golden(dev, golden_dir, tag):
+    """ICE_ADJUST with OCND2 (ICECLOUD over the TIWMX tables, liquid-only condensate) against
+    the compiled reference's vectors."""
+    g = np.load(os.path.join(golden_dir, "ice_adjust_ocnd2_fp64.npz"))
+    f = {k[len(tag) + 4:]: g[k] for k in g.files if k.startswith(tag + "_in_")}
+    ref = {k: g[f"{tag}_out_{k}"] for k in OCND2_CHECKED}
+    out = _adjust_ocnd2_gpu(f, float(g["dt"]))
+    for k in OCND2_CHECKED:
+        assert np.array_equal(np.isnan(out[k]), np.isnan(ref[k])), k
+    w = _check(out, ref, OCND2_CHECKED)
+    print("ice_adjust OCND2 golden max rel err", tag, max(w.values()))
+
+
+def test_ice_adjust_ocnd2_vs_oracle_slab(dev):
+    from ice3_amd import synthetic
+    from oracle import oracle_lib
+    f = synthetic.generate(2048, 90, seed=20260036)
+    f["zz"] = np.ascontiguousarray(np.broadcast_to(synthetic.level_profile(90)[2][:, None], f["th"].shape))
+    f["ice_cld_wgt"] = np.random.default_rng(7).uniform(0.5, 1.5, 2048)
+    out = _adjust_ocnd2_gpu(f)
+    ref = oracle_lib.ice_adjust_ocnd2(f)
+    w = _check(out, ref, OCND2_CHECKED)
+    print("ice_adjust OCND2 vs oracle, 2048 columns: max rel err", max(w.values()))
+
+
 def _rain_case(g, tag):
     p = "" if tag == "base" else tag + "_"
     s = {k[len(p) + 3:]: g[k] for k in g.files if k.startswith(p + "in_")}

@@ -43,6 +43,24 @@ def _pipeline_input(ncol, seed=20260003):
     return f
 
 
+OCND2_OUT = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
+             "rvs", "rcs", "ris", "ths", "srcs", "icldfr", "wcldfr", "ssio", "ssiu", "ifr")
+
+
+@pytest.mark.parametrize("tag", ["base", "mf"])
+def test_host_ice_adjust_ocnd2_golden(golden_dir, tag):
+    """The kernels' OCND2 ICE_ADJUST source on the CPU against the compiled reference's vectors
+    (its TIWMX saturation pressures come from the kernels' FP64 exp/log, not the library's)."""
+    g = np.load(os.path.join(golden_dir, "ice_adjust_ocnd2_fp64.npz"))
+    f = {k[len(tag) + 4:]: g[k] for k in g.files if k.startswith(tag + "_in_")}
+    ref = {k: g[f"{tag}_out_{k}"] for k in OCND2_OUT}
+    out = host_lib.ice_adjust_ocnd2(f, float(g["dt"]))
+    for k in OCND2_OUT:
+        assert np.array_equal(np.isnan(out[k]), np.isnan(ref[k])), k
+    w = _worst(out, ref, OCND2_OUT)
+    assert max(w.values()) <= TOL, w
+
+
 def test_host_pipeline_matches_oracle():
     from oracle import oracle_lib, ref_lib
     f = _pipeline_input(128)

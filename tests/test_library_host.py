@@ -31,7 +31,8 @@ def test_struct_layouts_match_header():
     for cname, py in (("ice3_ice_adjust_in", _lib.AdjustIn), ("ice3_ice_adjust_out", _lib.AdjustOut),
                       ("ice3_rain_ice_in", _lib.RainIn), ("ice3_rain_ice_inout", _lib.RainIO),
                       ("ice3_rain_ice_out", _lib.RainOut), ("ice3_pipeline_in", _lib.PipeIn),
-                      ("ice3_pipeline_out", _lib.PipeOut), ("ice3_stats", _lib.Stats)):
+                      ("ice3_pipeline_out", _lib.PipeOut), ("ice3_stats", _lib.Stats),
+                      ("ice3_ocnd2_in", _lib.Ocnd2In), ("ice3_ocnd2_out", _lib.Ocnd2Out)):
         body = re.search(r"typedef struct \{([^}]*)\} %s;" % cname, hdr).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         names = re.findall(r"\*\s*(\w+)", body)
@@ -43,6 +44,7 @@ def test_bad_arguments_are_rejected_without_a_device(lib):
     assert lib.ice3_ice_adjust(None, 0, 4, 90, 4, 50.0, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_rain_ice(None, 0, 4, 90, 4, 50.0, None, None, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_adjust_rain_ice(None, 0, 4, 90, 4, 50.0, None, None, None, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_ice_adjust_ocnd2(None, 4, 90, 4, 50.0, None, None, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_ctx_create(None) == _lib.ICE3_EINVAL
     assert b"invalid" in lib.ice3_strerror(_lib.ICE3_EINVAL)
     assert b"sub-step" in lib.ice3_strerror(_lib.ICE3_ESUBSTEP)
