@@ -36,10 +36,12 @@ enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6 };
 // built with -fapprox-func, under which the device divides by a reciprocal refined with
 // Newton steps (within 1 ulp; in practice the IEEE quotient).
 
-// log x: x = 2^e m with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m - 1)/(m + 1),
-// |s| <= 0.1716, from the series 2 atanh(s) = 2s + sum_{k>=1} 2 s^(2k+1) / (2k+1) up to
-// s^21 (the remainder is below 1e-18 relative); log x = e ln2 + log m, ln2 split into a
-// high part with 42 significant bits (e * LN2_HI is exact) and the rest.
+// log x: x = 2^e m with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s) = 2s + s z p(z),
+// s = (m - 1)/(m + 1), z = s^2 <= 0.0295; p is a degree-6 near-minimax fit of
+// (log m - 2s)/(s z) on that interval (interpolation at Chebyshev nodes in 60-digit
+// arithmetic, fit_exp_log.py; the atanh series 2/3 + 2/5 z + ... needs three more terms
+// for the same accuracy).  log x = e ln2 + log m, ln2 split into a high part with 42
+// significant bits (e * LN2_HI is exact) and the rest.
 __host__ __device__ __forceinline__ double log_f64(double x) {
   constexpr double LN2_HI = 0x1.62e42fefa3800p-1, LN2_LO = 0x1.ef35793c76730p-45;
   constexpr double SQRT_HALF = 0x1.6a09e667f3bcdp-1;
@@ -52,16 +54,13 @@ __host__ __device__ __forceinline__ double log_f64(double x) {
   const double f = m - 1.0;  // exact
   const double s = f / (2.0 + f);
   const double z = s * s;
-  double p = 0x1.8618618618618p-4;       // 2/21
-  p = fma(p, z, 0x1.af286bca1af28p-4);   // 2/19
-  p = fma(p, z, 0x1.e1e1e1e1e1e1ep-4);   // 2/17
-  p = fma(p, z, 0x1.1111111111111p-3);   // 2/15
-  p = fma(p, z, 0x1.3b13b13b13b14p-3);   // 2/13
-  p = fma(p, z, 0x1.745d1745d1746p-3);   // 2/11
-  p = fma(p, z, 0x1.c71c71c71c71cp-3);   // 2/9
-  p = fma(p, z, 0x1.2492492492492p-2);   // 2/7
-  p = fma(p, z, 0x1.999999999999ap-2);   // 2/5
-  p = fma(p, z, 0x1.5555555555555p-1);   // 2/3
+  double p = 0x1.2b5900de53b32p-3;
+  p = fma(p, z, 0x1.39fe51a7c18f9p-3);
+  p = fma(p, z, 0x1.7462b51cb66b1p-3);
+  p = fma(p, z, 0x1.c71c62e3f11e6p-3);
+  p = fma(p, z, 0x1.2492492df281ap-2);
+  p = fma(p, z, 0x1.99999999952d7p-2);
+  p = fma(p, z, 0x1.5555555555558p-1);
   const double r = fma(s * z, p, s + s);  // log m
   const double de = (double)e;
   const double y = fma(de, LN2_HI, fma(de, LN2_LO, r));
@@ -69,9 +68,11 @@ __host__ __device__ __forceinline__ double log_f64(double x) {
   return x == 0. ? -INFINITY : (x == INFINITY ? x : NAN);  // log 0, log inf, x < 0 or NaN
 }
 
-// exp x: x = n ln2 + r, n = rint(x / ln2), |r| <= 0.347; e^r from its Taylor series up to
-// r^13 (the remainder is below 5e-18 relative); e^x = 2^n e^r.  x is first clamped to
-// [-760, 720], where e^x is 0 or infinite at either end.
+// exp x: x = n ln2 + r, n = rint(x / ln2), |r| <= 0.347; e^r = 1 + r (1 + r q(r)) with q a
+// degree-9 polynomial, q(0) = 1/2 exactly, fitted near-minimax to (e^r - 1 - r)/r^2 on
+// |r| <= 0.3466 (fit_exp_log.py; the Taylor series needs two more terms for the same
+// accuracy); e^x = 2^n e^r.  x is first clamped to [-760, 720], where e^x is 0 or infinite
+// at either end.
 __host__ __device__ __forceinline__ double exp_f64(double x) {
   constexpr double LN2_HI = 0x1.62e42fefa3800p-1, LN2_LO = 0x1.ef35793c76730p-45;
   constexpr double LOG2E = 0x1.71547652b82fep+0;
@@ -79,18 +80,16 @@ __host__ __device__ __forceinline__ double exp_f64(double x) {
   const double n = rint(xc * LOG2E);
   double r = fma(-n, LN2_HI, xc);  // exact: n LN2_HI is exact and close to xc
   r = fma(-n, LN2_LO, r);
-  double p = 0x1.6124613a86d09p-33;      // 1/13!
-  p = fma(p, r, 0x1.1eed8eff8d898p-29);  // 1/12!
-  p = fma(p, r, 0x1.ae64567f544e4p-26);  // 1/11!
-  p = fma(p, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
-  p = fma(p, r, 0x1.71de3a556c734p-19);  // 1/9!
-  p = fma(p, r, 0x1.a01a01a01a01ap-16);  // 1/8!
-  p = fma(p, r, 0x1.a01a01a01a01ap-13);  // 1/7!
-  p = fma(p, r, 0x1.6c16c16c16c17p-10);  // 1/6!
-  p = fma(p, r, 0x1.1111111111111p-7);   // 1/5!
-  p = fma(p, r, 0x1.5555555555555p-5);   // 1/4!
-  p = fma(p, r, 0x1.5555555555555p-3);   // 1/3!
-  p = fma(p, r, 0.5);                    // 1/2!
+  double p = 0x1.af236857dc4b7p-26;
+  p = fma(p, r, 0x1.28803f36ecdd5p-22);
+  p = fma(p, r, 0x1.71de17e4e5d5dp-19);
+  p = fma(p, r, 0x1.a019c9a6c2942p-16);
+  p = fma(p, r, 0x1.a01a01a4b2f7fp-13);
+  p = fma(p, r, 0x1.6c16c1738faf6p-10);
+  p = fma(p, r, 0x1.1111111110e6ap-7);
+  p = fma(p, r, 0x1.5555555554cb7p-5);
+  p = fma(p, r, 0x1.5555555555555p-3);
+  p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
   const double y = ldexp(p, (int)n);

@@ -1,7 +1,7 @@
 """The kernels' FP64 exp and log (ice3_physics.hpp: exp_f64, log_f64) against the C library.
 
-They are built from textbook series (log: frexp reduction and 2 atanh((m-1)/(m+1)); exp:
-n ln2 reduction and the Taylor series of e^r), so they are pinned here by what the
+They are built from textbook reductions (log: frexp and 2 atanh((m-1)/(m+1)); exp: n ln2)
+with near-minimax polynomials (csrc/fit_exp_log.py), so they are pinned here by what the
 mathematics fixes: the C library's values (numpy: glibc, within 1 ulp), the exact
 special cases, and identities exp(log x) = x and log(2^k) = k ln 2.  The CPU build runs the
 same header as the kernels; every step is a correctly rounded IEEE operation, except that the
