@@ -93,7 +93,30 @@ def cpu_baseline(budget_s: float = 20.0, workers: int | None = None) -> dict:
             "sample": f"{workers} processes x {ncol_w} columns x {NLEV} levels of the same generator "
                       f"(seeds {SEED + 1}..{SEED + workers}), fused ice_adjust+rain_ice FP64, "
                       f"slowest worker {wall:.1f} s",
+            "ice_adjust_1024x90": _cpu_adjust_1024(kind),
             "jax_cpu": "not timed: jax is not installed on this image and there is no package index"}
+
+
+def _cpu_adjust_1024(kind: str, runs: int = 7) -> dict:
+    """The reference's own CPU case: ice_adjust alone, FP64, 1024 columns x 90 levels of the
+    "adjust" generator, one process, median of `runs` calls (the call includes the
+    binding's column chunking and copies)."""
+    from ice3_amd import synthetic
+    from oracle import oracle_lib, ref_lib
+    f = synthetic.generate(1024, NLEV, seed=20260001, variant="adjust", dt=DT)
+    if kind == "reference":
+        ref_lib.init(DT, 128)
+        fn = lambda: ref_lib.ice_adjust(f, DT)  # noqa: E731
+    else:
+        fn = lambda: oracle_lib.ice_adjust(f, DT)  # noqa: E731
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ms = sorted(ts)[runs // 2] * 1e3
+    return {"ms": ms, "columns_per_s": 1024 / (ms * 1e-3), "cores": 1, "kind": kind}
 
 
 def _cpu_slab(arg):
@@ -286,6 +309,7 @@ def main():
     if not args.no_adjust:
         res.setdefault("variants", {})["ice_adjust_256x256"] = ice_adjust_alone(lib, ctx, dev, args)
         res["variants"]["ice_adjust_ocnd2_256x256"] = ice_adjust_alone(lib, ctx, dev, args, ocnd2=True)
+        res["variants"]["ice_adjust_1024x90"] = ice_adjust_alone(lib, ctx, dev, args, ncol=1024)
     print(json.dumps(res))
 
 
