@@ -178,11 +178,13 @@ def test_pipeline_vs_oracle(dev):
 def test_pipeline_full_size_water_budget(dev):
     """512x512 columns x 90 levels (the fused benchmark workload): every column's total
     water changes only by its surface precipitation; sources stay finite and
-    non-negative for the condensed species."""
+    non-negative for the condensed species; and every 128th column (2,048 columns,
+    184,320 points) matches the oracle at the FP64 bar, branch flips counted."""
     from ice3_amd import AdjustRainIce, synthetic
+    from oracle import oracle_lib, ref_lib
     f = synthetic.generate(512 * 512, 90, seed=20260003, variant="pipeline")
     dt = 50.0
-    res = AdjustRainIce()({**f, "rhodref": f["rho_dry_ref"]}, dt)
+    res = AdjustRainIce()({**f, "rhodref": f["rho_dry_ref"]}, dt, return_calls=True)
     rho = torch.as_tensor(f["rho_dry_ref"], device=dev)
     dz = torch.as_tensor(f["dzz"], device=dev)
     w0 = sum(torch.as_tensor(f[k], device=dev) for k in ("rvs", "rcs", "rrs", "ris", "rss", "rgs"))
@@ -195,6 +197,18 @@ def test_pipeline_full_size_water_budget(dev):
     assert err < 1e-12
     for k in ("rcs", "rrs", "ris", "rss", "rgs"):
         assert torch.isfinite(res[k]).all() and (res[k] >= 0).all(), k
+    del rho, dz, w0, w1, col0, col1, out
+
+    cols = np.arange(0, 512 * 512, 128)
+    idx = torch.as_tensor(cols, device=dev)
+    got = {k: _np(v.index_select(v.dim() - 1, idx)) for k, v in res.items()}
+    g = {k: np.ascontiguousarray(v[:, cols] if v.ndim == 2 else v[cols]) for k, v in f.items()}
+    del f, res
+    ref = oracle_lib.rain_ice(ref_lib.pipeline_state(g, oracle_lib.ice_adjust(g)), with_iters=True)
+    flips = got["calls"] != ref["iters"]
+    w, nf, nb = check_with_flips(got, ref, RAIN_OUT, flips, TOL)
+    print(f"512x512 step, sampled columns: {nf} branch flips, {nb} divergent points of {flips.size}, "
+          f"worst {max(w.values()):.2e}")
 
 
 def _fused_raw(lib, ctx, dev, f, ncol, nlev, ld, stages=(7,), calls=False):
