@@ -77,6 +77,34 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+// ---- k_count_nonfinite: input validation (ice3_count_nonfinite).  blockIdx.y is the
+// field; each thread takes one column (grid-stride) and walks the field's rows, so that a
+// wavefront's loads are 64 consecutive doubles.  A value is non-finite when its exponent
+// bits are all ones (NaN or an infinity).  Only a wavefront that found some adds to the
+// field's counter.
+struct CountArgs {
+  const double *f[ICE3_MAX_CHECK_FIELDS];
+  int rows[ICE3_MAX_CHECK_FIELDS];
+  int ncol, ld;
+  unsigned long long *counts;
+};
+
+__global__ void __launch_bounds__(256) k_count_nonfinite(CountArgs a) {
+  const int q = blockIdx.y;
+  const double *__restrict__ f = a.f[q];
+  const int rows = a.rows[q];
+  unsigned long long n = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.ncol; i += gridDim.x * blockDim.x)
+#pragma unroll 8
+    for (int r = 0; r < rows; ++r) {
+      const unsigned long long b = __double_as_longlong(f[(size_t)r * a.ld + i]);
+      n += (b & 0x7ff0000000000000ull) == 0x7ff0000000000000ull;
+    }
+  if (__ballot(n != 0) == 0ull) return;  // uniform: every lane of the wavefront sees it
+  n = wave_sum(n);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&a.counts[q], n);
+}
+
 // ---- k_micro: the ICE4 microphysics, persistent, with the time splitting compacted
 // across each wavefront.
 //
@@ -612,6 +640,29 @@ static bool empty_domain(int ncol, int nlev, int ld, double dt) {
 static bool known_prec(int prec) { return prec == ICE3_PREC_FP64 || prec == ICE3_PREC_MIXED; }
 
 #define NN(p) if (!(p)) return ICE3_EINVAL
+
+extern "C" int ice3_count_nonfinite(int nfield, const double *const *fields, const int *rows,
+                                    int ncol, int ld, unsigned long long *counts, void *stream) {
+  if (nfield < 0 || nfield > ICE3_MAX_CHECK_FIELDS || ncol < 0 || ld < ncol) return ICE3_EINVAL;
+  if (nfield == 0 || ncol == 0) return ICE3_OK;
+  if (!fields || !rows || !counts) return ICE3_EINVAL;
+  CountArgs a = CountArgs{};
+  a.ncol = ncol; a.ld = ld; a.counts = counts;
+  int maxrows = 0;
+  for (int q = 0; q < nfield; ++q) {
+    if (rows[q] < 0 || (rows[q] > 0 && !fields[q])) return ICE3_EINVAL;
+    if ((double)rows[q] * (double)ld > 9.0e15) return ICE3_EINVAL;
+    a.f[q] = fields[q];
+    a.rows[q] = rows[q];
+    maxrows = rows[q] > maxrows ? rows[q] : maxrows;
+  }
+  if (maxrows == 0) return ICE3_OK;
+  const int blocks = (ncol + 255) / 256;
+  dim3 grid(blocks < 1024 ? blocks : 1024, nfield);
+  hipLaunchKernelGGL(k_count_nonfinite, grid, dim3(256), 0, (hipStream_t)stream, a);
+  HIPCHK(hipGetLastError());
+  return ICE3_OK;
+}
 
 extern "C" int ice3_ice_adjust(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double dt,
                                const ice3_ice_adjust_in *in, const ice3_ice_adjust_out *out,

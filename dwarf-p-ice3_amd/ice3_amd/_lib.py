@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libice3_hip.so")
 
 ICE3_OK, ICE3_EINVAL, ICE3_EDEVICE, ICE3_ESUBSTEP = 0, -1, -2, -3
+ICE3_MAX_CHECK_FIELDS = 32
 PREC_FP64, PREC_MIXED = 0, 1
 
 _P = ctypes.c_void_p
@@ -74,7 +75,7 @@ EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_co
            "ice3_stage_timing",
            "ice3_stage_times", "ice3_adjust_rain_ice_stages", "ice3_adjust_rain_ice_surrogate",
            "ice3_surrogate_features", "ice3_surrogate_work_bytes", "ice3_mlp_sizes", "ice3_mlp_forward", "ice3_mlp_backward",
-           "ice3_mlp_adam")
+           "ice3_mlp_adam", "ice3_count_nonfinite")
 
 _lib = None
 
@@ -120,10 +121,41 @@ def load():
         lib.ice3_mlp_forward.argtypes = [ctypes.c_int] + [_P] * 10
         lib.ice3_mlp_backward.argtypes = [ctypes.c_int] + [_P] * 8
         lib.ice3_mlp_adam.argtypes = [ctypes.c_int] + [ctypes.c_float] * 4 + [_P] * 7
+        lib.ice3_count_nonfinite.argtypes = [ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int),
+                                             ctypes.c_int, ctypes.c_int, _P, _P]
         lib.ice3_stage_timing.argtypes = [_P, ctypes.c_int]
         lib.ice3_stage_times.argtypes = [_P, ctypes.POINTER(ctypes.c_float)]
         _lib = lib
     return _lib
+
+
+def require_finite(fields: dict, device):
+    """Raise ValueError naming every field of ``fields`` (name -> float64 CUDA tensor, a
+    (nlev, ncol) field or an (ncol,) per-column one, contiguous) that holds a NaN or an
+    infinity, with the number found.  One ice3_count_nonfinite launch on the device's
+    current stream, then a synchronising read of the counts."""
+    import torch
+    from ._fields import ptr, stream_ptr
+    items = [(k, v) for k, v in fields.items() if v is not None and v.numel()]
+    if not items:
+        return
+    if len(items) > ICE3_MAX_CHECK_FIELDS:
+        raise ValueError(f"at most {ICE3_MAX_CHECK_FIELDS} fields per check")
+    ncol = items[0][1].shape[-1]
+    for k, v in items:
+        if v.dtype != torch.float64 or not v.is_contiguous() or v.dim() not in (1, 2) or v.shape[-1] != ncol:
+            raise ValueError(f"{k}: expected a contiguous float64 field of {ncol} columns")
+    counts = torch.zeros(len(items), dtype=torch.int64, device=device)
+    fptr = (_P * len(items))(*[ptr(v).value for _, v in items])
+    rows = (ctypes.c_int * len(items))(*[v.shape[0] if v.dim() == 2 else 1 for _, v in items])
+    lib = load()
+    with torch.cuda.device(device):
+        check(lib.ice3_count_nonfinite(len(items), fptr, rows, ncol, ncol, ptr(counts), stream_ptr(device)),
+              "ice3_count_nonfinite")
+    n = counts.tolist()
+    bad = [f"{k} ({c})" for (k, _), c in zip(items, n) if c]
+    if bad:
+        raise ValueError("non-finite input values (NaN or infinity): " + ", ".join(bad))
 
 
 def check(rc: int, what: str):

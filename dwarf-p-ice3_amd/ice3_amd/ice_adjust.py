@@ -79,6 +79,10 @@ class IceAdjust:
     layout : "columns" for (nlev, ncol) arrays with columns contiguous (the Fortran
         (NIJT, NKT) memory, no copy), or "jax" for the JAX components' (nx, ny, nz).
     device : CUDA device (default: current).
+
+    The call takes IceAdjustJAX's arguments; ``check_inputs=True`` first counts the NaN and
+    infinite input values on the device (``ice3_count_nonfinite``) and raises ValueError
+    naming the fields that hold any (the reference does not check; off by default).
     """
 
     _OCND2 = False
@@ -113,12 +117,14 @@ class IceAdjust:
 
     def __call__(self, sigqsat, pabs, sigs, th, exn, exn_ref, rho_dry_ref, rv, rc, ri, rr, rs, rg,
                  cf_mf, rc_mf, ri_mf, rvs, rcs, ris, ths, timestep: float,
-                 precision: str = "fp64", return_srcs: bool = False):
+                 precision: str = "fp64", return_srcs: bool = False, check_inputs: bool = False):
         args = dict(pabs=pabs, sigs=sigs, th=th, exn=exn, exn_ref=exn_ref, rho_dry_ref=rho_dry_ref,
                     rv=rv, rc=rc, ri=ri, rr=rr, rs=rs, rg=rg, cf_mf=cf_mf, rc_mf=rc_mf, ri_mf=ri_mf,
                     rvs=rvs, rcs=rcs, ris=ris, ths=ths)
         shape = tuple(torch.as_tensor(th).shape)
         f, sq, nlev, ncol = self._inputs(sigqsat, args)
+        if check_inputs:
+            _lib.require_finite({**f, "sigqsat": sq}, self.device)
         out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) for k in _OUT_ORDER}
         srcs = torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) if return_srcs else None
         cin = _lib.AdjustIn(*[ptr(f[k]) for k in _lib.ADJ_IN_FIELDS[:-1]], ptr(sq))
@@ -151,7 +157,7 @@ class IceAdjustOCND2(IceAdjust):
 
     def __call__(self, sigqsat, pabs, sigs, th, exn, exn_ref, rho_dry_ref, rv, rc, ri, rr, rs, rg,
                  cf_mf, rc_mf, ri_mf, rvs, rcs, ris, ths, timestep: float, zz=None,
-                 ice_cld_wgt=None, return_srcs: bool = False):
+                 ice_cld_wgt=None, return_srcs: bool = False, check_inputs: bool = False):
         if zz is None or ice_cld_wgt is None:
             raise ValueError("the OCND2 branch needs zz (level heights) and ice_cld_wgt")
         args = dict(pabs=pabs, sigs=sigs, th=th, exn=exn, exn_ref=exn_ref, rho_dry_ref=rho_dry_ref,
@@ -162,6 +168,8 @@ class IceAdjustOCND2(IceAdjust):
         wgt = self._per_column(ice_cld_wgt, "ice_cld_wgt", ncol)
         if nlev == 1:
             raise ValueError("the OCND2 branch needs at least two levels")
+        if check_inputs:
+            _lib.require_finite({**f, "sigqsat": sq, "ice_cld_wgt": wgt}, self.device)
         names = _OUT_ORDER + _lib.OCND2_OUT_FIELDS
         out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) for k in names}
         srcs = torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) if return_srcs else None

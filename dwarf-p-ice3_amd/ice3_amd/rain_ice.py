@@ -13,6 +13,11 @@ State keys read (RainIceJAX names): exn, rhodref, pres, dzz, th_t, rv_t, rc_t, r
 rs_t, rg_t, ci_t, ths, rvs, rcs, rrs, ris, rss, rgs; optional: exnref (default exn),
 rhodj (default rhodref), cf (cloud fraction, default 0), sigs (default 0), hlc_hrc,
 hlc_hcf, hli_hri, hli_hcf (default 0).
+
+``check_inputs=True`` (RainIce, AdjustRainIce and IceAdjust) first counts the NaN and
+infinite input values on the device (``ice3_count_nonfinite``) and raises ValueError
+naming the fields that hold any; the reference has no such check, so it is off by
+default.
 """
 from __future__ import annotations
 
@@ -59,7 +64,7 @@ class RainIce(_Base):
 
     def __call__(self, state: dict, dt: float, lsedim_after: bool = False, ldeposc: bool = False,
                  ldsoft: bool = False, lfeedbackt: bool = True, max_iterations: int | None = None,
-                 precision: str = "fp64", return_calls: bool = False):
+                 precision: str = "fp64", return_calls: bool = False, check_inputs: bool = False):
         if lsedim_after or ldeposc or ldsoft or not lfeedbackt:
             raise ValueError("only LSEDIM_AFTER=F, LDEPOSC=F, LFEEDBACKT=T are implemented")
         shape = tuple(torch.as_tensor(state["th_t"]).shape)
@@ -84,6 +89,8 @@ class RainIce(_Base):
         for k, v in list(inp.items()) + list(io.items()):
             if tuple(v.shape) != (nlev, ncol):
                 raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
+        if check_inputs:
+            _lib.require_finite({**inp, **io}, self.device)
         counters, calls, st = self._stats(nlev * ncol, return_calls)
         cin = _lib.RainIn(*[ptr(inp[k]) for k in _lib.RAIN_IN_FIELDS])
         cio = _lib.RainIO(*[ptr(io[k]) for k in _lib.RAIN_IO_FIELDS])
@@ -114,7 +121,8 @@ class AdjustRainIce(_Base):
     sources, ``cldfr, hlc_*, hli_*, ci, evap3d, rainfr`` and the surface precipitation.
     """
 
-    def __call__(self, f: dict, dt: float, precision: str = "fp64", return_calls: bool = False):
+    def __call__(self, f: dict, dt: float, precision: str = "fp64", return_calls: bool = False,
+                 check_inputs: bool = False):
         shape = tuple(torch.as_tensor(f["th"]).shape)
         g = {k: self._col(f[k], k) for k in _lib.PIPE_IN_FIELDS if k != "sigqsat"}
         nlev, ncol = g["th"].shape
@@ -127,6 +135,8 @@ class AdjustRainIce(_Base):
         for k, v in g.items():
             if tuple(v.shape) != (nlev, ncol):
                 raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
+        if check_inputs:
+            _lib.require_finite({**g, "sigqsat": sq}, self.device)
         out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device)
                for k in _lib.PIPE_OUT_FIELDS[:15]}
         out.update({k: torch.empty(ncol, dtype=torch.float64, device=self.device)

@@ -57,6 +57,18 @@ const char *ice3_strerror(int code);
  * constants are then built on the host without touching a device. */
 long ice3_dump_constants(const ice3_ctx *ctx, char *buf, long buflen);
 
+/* ---- input validation (no counterpart in the reference: its Fortran and JAX paths take
+ * NaN and infinite inputs without a check; a NaN stays in its own column here, but a
+ * MIN/MAX or a threshold can turn it into a finite, wrong value) ----
+ * Counts the non-finite values (NaN, +-Inf) of nfield float64 device arrays: field q
+ * has rows[q] rows (nlev for a per-point field, 1 for a per-column one, 0 to skip it)
+ * of ncol values at row stride ld, and counts[q] (device) is increased by the number
+ * found (the caller zeroes counts).  fields and rows are host arrays.  One HBM pass over
+ * the fields; the Python drop-ins call it with check_inputs=True. */
+#define ICE3_MAX_CHECK_FIELDS 32
+int ice3_count_nonfinite(int nfield, const double *const *fields, const int *rows, int ncol,
+                         int ld, unsigned long long *counts, void *stream);
+
 /* ---- ice_adjust: ICE_ADJUST (micro/ice_adjust.F90:7) with CONDENSATION ---- */
 typedef struct {
   const double *pabs, *sigs, *th, *exn, *exn_ref, *rho_dry_ref;

@@ -263,6 +263,60 @@ def test_fused_layout_and_stages_bitwise(dev):
             assert np.array_equal(np.nan_to_num(other[k], nan=1.0), np.nan_to_num(dense[k], nan=1.0)), (ld, stages, k)
 
 
+def test_nonfinite_inputs(dev):
+    """ice3_count_nonfinite counts NaN and infinities per field (and nothing in the row
+    padding beyond ncol); the drop-ins' check_inputs=True names the fields that hold any
+    and leaves clean results unchanged; without the check, a NaN or an infinity in one
+    column changes no other column, and every point still ends its time step."""
+    import ctypes
+    from ice3_amd import AdjustRainIce, IceAdjust, _lib, synthetic
+    from ice3_amd._fields import ptr
+    lib = _lib.load()
+    gen = torch.Generator().manual_seed(5)
+    ncol, ld, nlev = 300, 320, 90
+    fields = [torch.rand((nlev, ld), generator=gen, dtype=torch.float64) for _ in range(3)]
+    fields.append(torch.rand((ld,), generator=gen, dtype=torch.float64))
+    fields[0][5, 7] = float("nan"); fields[0][89, 299] = float("inf")
+    fields[2][0, 0] = float("-inf"); fields[3][17] = float("nan")
+    for f in fields[:3]:
+        f[:, ncol:] = float("nan")  # padding beyond ncol is not read
+    fields[3][ncol:] = float("nan")
+    want = [int((~torch.isfinite(f[..., :ncol])).sum()) for f in fields]
+    assert want == [2, 0, 1, 1]
+    dfields = [f.to(dev) for f in fields]
+    counts = torch.zeros(4, dtype=torch.int64, device=dev)
+    fp = (ctypes.c_void_p * 4)(*[ptr(f).value for f in dfields])
+    rows = (ctypes.c_int * 4)(nlev, nlev, nlev, 1)
+    for _ in range(2):  # accumulates
+        assert lib.ice3_count_nonfinite(4, fp, rows, ncol, ld, ptr(counts), None) == _lib.ICE3_OK
+    assert counts.tolist() == [2 * w for w in want]
+
+    f = synthetic.generate(64, 90, seed=20260015, variant="pipeline")
+    step = AdjustRainIce()
+    clean = step({**f, "rhodref": f["rho_dry_ref"]}, 50.0, return_calls=True)
+    checked = step({**f, "rhodref": f["rho_dry_ref"]}, 50.0, return_calls=True, check_inputs=True)
+    for k in clean:
+        assert torch.equal(clean[k], checked[k]), k
+    names = ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc", "ri", "rr", "rs", "rg",
+             "cf_mf", "rc_mf", "ri_mf", "rvs", "rcs", "ris", "ths")
+    for fld, val in (("th", np.nan), ("rrs", np.inf), ("sigs", np.nan), ("rhodref", np.inf)):
+        g = {k: np.array(v, copy=True) for k, v in f.items()}
+        g["rhodref"] = g["rho_dry_ref"]
+        g[fld][[0, 30, 60, 89], 3] = val
+        with pytest.raises(ValueError, match=fld):
+            step(g, 50.0, check_inputs=True)
+        if fld in names:
+            with pytest.raises(ValueError, match=fld):
+                IceAdjust()(g["sigqsat"], *[g[k] for k in names], timestep=50.0, check_inputs=True)
+        res = step(g, 50.0, return_calls=True)
+        assert step.last_counters["capped_points"] == 0
+        for k in clean:
+            a, b = _np(res[k]), _np(clean[k])
+            keep = np.arange(a.shape[-1]) != 3
+            assert np.array_equal(a[..., keep], b[..., keep], equal_nan=True), (fld, k)
+    print("non-finite inputs: counted per field, named by check_inputs, confined to their column")
+
+
 def test_empty_and_tiny_domains(dev):
     """An empty domain (no column or no level) is a call that does nothing and succeeds,
     through the C ABI (NULL field pointers) and the drop-ins (empty arrays in, empty arrays

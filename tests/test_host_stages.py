@@ -125,3 +125,39 @@ def test_host_stages_sanitized(tmp_path):
                            text=True, timeout=300)
         assert r.returncode == 0, r.stderr[-3000:]
         assert "calls" in r.stdout
+
+
+def test_host_stages_nonfinite_inputs_stay_in_their_column(tmp_path):
+    """NaN and infinities in the inputs of the fused step (kernels' stage source on the CPU,
+    FP64 and mixed): every other column's outputs are bitwise those of the clean run, no
+    point reaches the sub-step cap, and the sanitizer build (ASan, UBSan with float-to-int
+    casts) stays clean, so no table index or loop bound is taken from a non-finite value."""
+    from ice3_amd import _lib as L
+    f = _pipeline_input(16, seed=20260010)
+    bad = {k: np.array(v, dtype=np.float64, copy=True) for k, v in f.items()}
+    plan = (("th", 1, np.nan), ("pabs", 2, np.inf), ("rhodref", 4, np.nan), ("sigs", 5, np.nan),
+            ("rrs", 6, np.inf), ("rgs", 7, -np.inf), ("ci", 8, np.nan), ("dzz", 9, np.nan),
+            ("rc_mf", 10, np.inf), ("sigqsat", 11, np.nan), ("exn", 12, np.nan), ("rv", 13, np.nan))
+    for k, col, val in plan:
+        if bad[k].ndim == 2:
+            bad[k][[0, 30, 60, 89], col] = val
+        else:
+            bad[k][col] = val
+    hit = sorted({col for _, col, _ in plan})
+    keep = np.setdiff1d(np.arange(16), hit)
+    for prec in (0, 1):
+        o0 = host_lib.adjust_rain_ice(f, 50.0, prec)
+        o1 = host_lib.adjust_rain_ice(bad, 50.0, prec)
+        assert o1["counters"][1] == 0
+        for k, v in o0.items():
+            if k == "counters":
+                continue
+            assert np.array_equal(o1[k][..., keep], v[..., keep], equal_nan=True), (prec, k)
+    exe = host_lib.build_sanitized()
+    for k in L.PIPE_IN_FIELDS:
+        np.ascontiguousarray(bad[k], dtype=np.float64).tofile(str(tmp_path / f"{k}.bin"))
+    for prec in ("0", "1"):
+        r = subprocess.run([exe, str(tmp_path), "16", "90", "50", prec], capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-3000:]
+        assert "capped 0" in r.stdout

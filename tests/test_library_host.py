@@ -46,6 +46,22 @@ def test_bad_arguments_are_rejected_without_a_device(lib):
     assert lib.ice3_adjust_rain_ice(None, 0, 4, 90, 4, 50.0, None, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_ice_adjust_ocnd2(None, 4, 90, 4, 50.0, None, None, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_ctx_create(None) == _lib.ICE3_EINVAL
+    # input validation: argument checks come before any launch
+    fp = (p * 2)(None, None)
+    rows = (ctypes.c_int * 2)(90, 1)
+    cnt = p(None)
+    assert lib.ice3_count_nonfinite(0, None, None, 4, 4, None, None) == _lib.ICE3_OK
+    assert lib.ice3_count_nonfinite(2, fp, rows, 0, 0, cnt, None) == _lib.ICE3_OK
+    assert lib.ice3_count_nonfinite(33, fp, rows, 4, 4, cnt, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_count_nonfinite(-1, fp, rows, 4, 4, cnt, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_count_nonfinite(2, fp, rows, 4, 3, cnt, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_count_nonfinite(2, None, rows, 4, 4, cnt, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_count_nonfinite(2, fp, rows, 4, 4, cnt, None) == _lib.ICE3_EINVAL  # NULL counts
+    assert lib.ice3_count_nonfinite(2, fp, rows, 4, 4, p(8), None) == _lib.ICE3_EINVAL  # NULL fields
+    zero = (ctypes.c_int * 2)(0, 0)
+    assert lib.ice3_count_nonfinite(2, fp, zero, 4, 4, p(8), None) == _lib.ICE3_OK  # nothing to read
+    neg = (ctypes.c_int * 2)(-1, 0)
+    assert lib.ice3_count_nonfinite(2, fp, neg, 4, 4, p(8), None) == _lib.ICE3_EINVAL
     assert b"invalid" in lib.ice3_strerror(_lib.ICE3_EINVAL)
     assert b"sub-step" in lib.ice3_strerror(_lib.ICE3_ESUBSTEP)
 
