@@ -121,12 +121,24 @@ __global__ void __launch_bounds__(256) k_count_nonfinite(CountArgs a) {
 // Eight wavefronts per workgroup, two per SIMD: the register budget of 256 per wavefront
 // costs a few spills (FP64), but the second wavefront hides the memory latency the first
 // one waits on.  The workgroup's wavefronts share the lookup tables, staged once into LDS.
-constexpr int MICRO_WAVES = 8;  // wavefronts per workgroup
+// Wavefronts per workgroup.  The mixed path (183 VGPRs at two wavefronts per SIMD) measured
+// slower with three (168 VGPRs, spills, 49-slot parks: +22 %) or four (+120 %) per SIMD.
+template <class R> constexpr int micro_waves() { return 8; }
+template <class R> constexpr size_t micro_tab_bytes() {
+  return ((size_t)ICE3_T_TOTAL * sizeof(R) + 15) / 16 * 16;
+}
+// Park slots per wavefront: what the LDS holds beside the tables, at most 63 (31 in FP64,
+// 63 in the mixed path).
+template <class R> constexpr int micro_park_cap() {
+  const size_t slot = 9 * sizeof(double) + 32 * sizeof(R) + 2 * sizeof(int);
+  const size_t n = (160 * 1024 - micro_tab_bytes<R>()) / micro_waves<R>() / slot;
+  return n < 63 ? (int)n : 63;
+}
 
 // Points parked by one wavefront: state between tendency calls, one row per field.
 // Capacity: what fits the LDS beside the tables (fewer than 64).
 template <class R> struct MicroPark {
-  static constexpr int CAP = sizeof(R) == 8 ? 31 : 63;
+  static constexpr int CAP = micro_park_cap<R>();
   double d[9][CAP];        // vart[7], cit, time
   R s[32][CAP];            // StepState
   unsigned int q[CAP];     // point index k * ncol + i
@@ -135,12 +147,10 @@ template <class R> struct MicroPark {
 static_assert(sizeof(StepState<double>) == 32 * sizeof(double), "StepState layout");
 static_assert(sizeof(StepState<float>) == 32 * sizeof(float), "StepState layout");
 
-template <class R> constexpr size_t micro_tab_bytes() {
-  return ((size_t)ICE3_T_TOTAL * sizeof(R) + 15) / 16 * 16;
-}
 template <class R> constexpr size_t micro_lds_bytes() {
-  return micro_tab_bytes<R>() + MICRO_WAVES * sizeof(MicroPark<R>);
+  return micro_tab_bytes<R>() + micro_waves<R>() * sizeof(MicroPark<R>);
 }
+static_assert(MicroPark<double>::CAP == 31 && MicroPark<float>::CAP == 63, "park capacity");
 static_assert(micro_lds_bytes<double>() <= 160 * 1024, "k_micro<double> LDS");
 static_assert(micro_lds_bytes<float>() <= 160 * 1024, "k_micro<float> LDS");
 
@@ -175,7 +185,7 @@ __device__ __forceinline__ unsigned int park_load(const MicroPark<R> &pk, int e,
 }
 
 template <class R>
-__global__ void __launch_bounds__(64 * MICRO_WAVES) k_micro(MicroArgs a) {
+__global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
   extern __shared__ __align__(16) unsigned char micro_lds[];
   R *tab = reinterpret_cast<R *>(micro_lds);
   for (int e = threadIdx.x; e < ICE3_T_TOTAL; e += blockDim.x) tab[e] = (R)a.tab[e];
@@ -196,7 +206,7 @@ __global__ void __launch_bounds__(64 * MICRO_WAVES) k_micro(MicroArgs a) {
   StepState<R> S;
   int calls = 0;
   unsigned long long ncalls = 0, ncap = 0, nmicro = 0;
-  for (int c = blockIdx.x * MICRO_WAVES + wid;; c += gridDim.x * MICRO_WAVES) {
+  for (int c = blockIdx.x * micro_waves<R>() + wid;; c += gridDim.x * micro_waves<R>()) {
     const bool more = c < nchunk;  // wave-uniform
     if (more) {
       const int k = c / cpl, i = (c - k * cpl) * 64 + lane;
@@ -720,10 +730,10 @@ extern "C" int ice3_ice_adjust_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, 
 // tables and the parked points, so one workgroup fits a CU).
 static hipError_t launch_micro(const ice3_ctx *ctx, int prec, const MicroArgs &m, hipStream_t s) {
   if (prec == ICE3_PREC_FP64)
-    hipLaunchKernelGGL(k_micro<double>, dim3(ctx->ncu), dim3(64 * MICRO_WAVES),
+    hipLaunchKernelGGL(k_micro<double>, dim3(ctx->ncu), dim3(64 * micro_waves<double>()),
                        micro_lds_bytes<double>(), s, m);
   else
-    hipLaunchKernelGGL(k_micro<float>, dim3(ctx->ncu), dim3(64 * MICRO_WAVES),
+    hipLaunchKernelGGL(k_micro<float>, dim3(ctx->ncu), dim3(64 * micro_waves<float>()),
                        micro_lds_bytes<float>(), s, m);
   return hipGetLastError();
 }
