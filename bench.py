@@ -82,7 +82,7 @@ def cpu_baseline(budget_s: float = 20.0, workers: int | None = None) -> dict:
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     workers = workers or max(1, min(16, ncpu))
     per_col = _cpu_slab((kind, 256, SEED))[1] / 256          # calibration on one core
-    ncol_w = int(max(64, min(4096, budget_s / per_col / 1.5)))
+    ncol_w = int(max(64, min(16384, budget_s / per_col / 1.5)))
     ctx = mp.get_context("fork")
     with ctx.Pool(workers) as pool:
         res = pool.map_async(_cpu_slab, [(kind, ncol_w, SEED + 1 + w) for w in range(workers)]).get(
