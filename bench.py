@@ -254,6 +254,7 @@ def main():
         "stages_ms": stages,
         "step_algorithmic_GBs": alg_bytes / (ms * 1e-3) / 1e9,
         "cpu_baseline": cpu,
+        "input_check": input_check(lib, g, ncol, stream, sp, args.steps),
     }
     def budget_err(out):
         """Max over columns of |column water change - surface precipitation| / column water."""
@@ -315,6 +316,35 @@ def main():
 
 MLP_FLOP_PER_POINT = 2.0 * (14 * 128 + 3 * 128 * 128 + 128 * 8)   # 103,936
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def input_check(lib, g, ncol, stream, sp, steps):
+    """Cost of the opt-in input check (ice3_count_nonfinite over the fused step's 26 input
+    fields, one HBM pass), timed with HIP events on the stream it is launched on."""
+    import torch
+    from ice3_amd import _lib
+    names = list(_lib.PIPE_IN_FIELDS)
+    fields = (ctypes.c_void_p * len(names))(*[g[k].data_ptr() for k in names])
+    rows = (ctypes.c_int * len(names))(*[g[k].shape[0] if g[k].dim() == 2 else 1 for k in names])
+    counts = torch.zeros(len(names), dtype=torch.int64, device=g["th"].device)
+    cp = ctypes.c_void_p(counts.data_ptr())
+
+    def call():
+        _lib.check(lib.ice3_count_nonfinite(len(names), fields, rows, ncol, ncol, cp, sp),
+                   "ice3_count_nonfinite")
+    call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nbytes = sum(g[k].numel() for k in names) * 8
+    return {"kernel": "k_count_nonfinite", "fields": len(names), "ms": ms,
+            "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
+            "nonfinite_per_call": int(counts.sum().item()) // (steps + 1)}
 
 
 def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
