@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
           for (int f = 0; f < 7; ++f) vart[f] = t[f];
           time = 0.;
           const PointConst<R> pc = point_const<R>(p.rhodref);
-          st = step_call<R, false>(dt, p, pc, vart, cit, time, S, calls, tab);
+          st = step_call<R, false>(dt, p, pc, vart, cit, time, S, calls, DEFAULT_MAX_CALLS, tab);
           if (st == 0) evap3d = (double)S.rrevav;
         } else {
           micro_nucleation<R>(a, j, t, cit, rvheni);
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
       }
       parked -= need < parked ? need : parked;
       if (active) {
-        const int st = step_call<R, true>(dt, p, pc_soft, vart, cit, time, S, calls, tab);
+        const int st = step_call<R, true>(dt, p, pc_soft, vart, cit, time, S, calls, DEFAULT_MAX_CALLS, tab);
         if (st <= 0) {
           ncalls += calls;
           ncap += st < 0;
@@ -283,9 +283,39 @@ __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
     // the stores above are read by other lanes of this wavefront in later passes
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
-  if (a.counters) {  // uniform branch: every lane of the wavefront takes part in the sums
-    const unsigned long long n = wave_sum(ncalls), cc = wave_sum(ncap), m = wave_sum(nmicro);
+  // every lane of the wavefront takes part in the sums (uniform branches)
+  const unsigned long long cc = wave_sum(ncap);
+  if (cc != 0 && lane == 0 && a.capped) atomicAdd(a.capped, cc);
+  if (a.counters) {
+    const unsigned long long n = wave_sum(ncalls), m = wave_sum(nmicro);
     if (lane == 0) {
+      atomicAdd(&a.counters[0], n);
+      atomicAdd(&a.counters[1], cc);
+      atomicAdd(&a.counters[2], m);
+    }
+  }
+}
+
+// The microphysics one thread per point, with the context's sub-step cap a.max_calls read at
+// run time (k_micro has the default cap compiled in: the register allocator fails on
+// k_micro<double> with a run-time bound).  Launched when a context sets another cap; the
+// per-point arithmetic is k_micro's, so the results are the same for the same cap.
+template <class R>
+__global__ void __launch_bounds__(256) k_micro_points(MicroArgs a) {
+  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long ncalls = 0, ncap = 0, nmicro = 0;
+  if (q < (unsigned int)a.nlev * (unsigned int)a.ncol) {
+    const unsigned int k = q / (unsigned int)a.ncol;
+    const int c = micro_at<R>(a, (size_t)k * a.ld + (q - k * (unsigned int)a.ncol), a.tab);
+    ncalls = c < 0 ? -c : c;
+    ncap = c < 0;
+    nmicro = c != 0;
+  }
+  const unsigned long long cc = wave_sum(ncap);
+  if (cc != 0 && (threadIdx.x & 63) == 0 && a.capped) atomicAdd(a.capped, cc);
+  if (a.counters) {
+    const unsigned long long n = wave_sum(ncalls), m = wave_sum(nmicro);
+    if ((threadIdx.x & 63) == 0) {
       atomicAdd(&a.counters[0], n);
       atomicAdd(&a.counters[1], cc);
       atomicAdd(&a.counters[2], m);
@@ -433,7 +463,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
                                                        const unsigned int *qcount, unsigned int seg) {
   const unsigned int n = qcount[blockIdx.x];
   const unsigned int *list = queue + 1 + (size_t)blockIdx.x * seg;
-  for (unsigned int e = threadIdx.x; e < n; e += blockDim.x) micro_at<R>(a, (size_t)list[e], a.tab);
+  for (unsigned int e = threadIdx.x; e < n; e += blockDim.x)
+    if (micro_at<R>(a, (size_t)list[e], a.tab) < 0 && a.capped) atomicAdd(a.capped, 1ull);
 }
 
 // Scratch layout of the surrogate step (bytes, 256-aligned blocks).
@@ -481,6 +512,8 @@ struct ice3_ctx {
   double *d_tab;
   double h_tab[ICE3_T_TOTAL];
   double *d_tiwmx;    // TIWMX tables of the OCND2 branch: ESTABW then ESTABI
+  unsigned long long *d_capped;  // points that hit the sub-step cap since the last status query
+  int max_calls;      // sub-step cap per point
   int timing;         // record ev[] around the stages of each call
   hipEvent_t ev[7];   // before adjust, after adjust, after sedimentation, then after the
                       // microphysics (plain step: events 3..6 together) or, for the surrogate
@@ -518,14 +551,18 @@ extern "C" int ice3_ctx_create(ice3_ctx **out) {
           hipSuccess ||
       hipMalloc((void **)&c->d_tiwmx, sizeof(double) * 2 * ICE3_TIWMX_N) != hipSuccess ||
       hipMemcpy(c->d_tiwmx, tiwmx, sizeof(double) * 2 * ICE3_TIWMX_N, hipMemcpyHostToDevice) !=
-          hipSuccess) {
+          hipSuccess ||
+      hipMalloc((void **)&c->d_capped, sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(c->d_capped, 0, sizeof(unsigned long long)) != hipSuccess) {
     if (c->d_tab) (void)hipFree(c->d_tab);
     if (c->d_tiwmx) (void)hipFree(c->d_tiwmx);
+    if (c->d_capped) (void)hipFree(c->d_capped);
     free(tiwmx);
     free(c);
     return ICE3_EDEVICE;
   }
   free(tiwmx);
+  c->max_calls = DEFAULT_MAX_CALLS;
   *out = c;
   return ICE3_OK;
 }
@@ -534,6 +571,7 @@ extern "C" void ice3_ctx_destroy(ice3_ctx *c) {
   if (!c) return;
   if (c->d_tab) (void)hipFree(c->d_tab);
   if (c->d_tiwmx) (void)hipFree(c->d_tiwmx);
+  if (c->d_capped) (void)hipFree(c->d_capped);
   if (c->timing)
     for (int q = 0; q < 7; ++q) (void)hipEventDestroy(c->ev[q]);
   free(c);
@@ -556,6 +594,31 @@ extern "C" int ice3_stage_times(ice3_ctx *c, float ms[6]) {
   HIPCHK(hipEventSynchronize(c->ev[6]));
   for (int q = 0; q < 6; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
   return ICE3_OK;
+}
+
+extern "C" int ice3_ctx_set_substep_cap(ice3_ctx *c, int max_calls) {
+  if (!c || max_calls < 1) return ICE3_EINVAL;
+  c->max_calls = max_calls;
+  return ICE3_OK;
+}
+
+extern "C" int ice3_ctx_status(ice3_ctx *c, void *stream, unsigned long long *capped) {
+  if (!c) return ICE3_EINVAL;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != c->device) HIPCHK(hipSetDevice(c->device));
+  const hipStream_t s = (hipStream_t)stream;
+  unsigned long long h = 0;
+  int rc = ICE3_OK;
+  if (hipMemcpyAsync(&h, c->d_capped, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess ||
+      (h != 0 && hipMemsetAsync(c->d_capped, 0, sizeof h, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess)
+    rc = ICE3_EDEVICE;
+  if (dev != c->device) (void)hipSetDevice(dev);
+  if (rc != ICE3_OK) return rc;
+  if (capped) *capped = h;
+  return h != 0 ? ICE3_ESUBSTEP : ICE3_OK;
 }
 
 // Record stage event q of a timed call (no-op when timing is off).
@@ -729,6 +792,13 @@ extern "C" int ice3_ice_adjust_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, 
 // Launch k_micro: one workgroup of eight wavefronts per compute unit (its LDS holds the
 // tables and the parked points, so one workgroup fits a CU).
 static hipError_t launch_micro(const ice3_ctx *ctx, int prec, const MicroArgs &m, hipStream_t s) {
+  if (m.max_calls != DEFAULT_MAX_CALLS) {  // a context-set cap: the point kernel reads it
+    const unsigned int npts = (unsigned int)m.nlev * (unsigned int)m.ncol;
+    const dim3 g((npts + 255) / 256);
+    if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_micro_points<double>, g, dim3(256), 0, s, m);
+    else hipLaunchKernelGGL(k_micro_points<float>, g, dim3(256), 0, s, m);
+    return hipGetLastError();
+  }
   if (prec == ICE3_PREC_FP64)
     hipLaunchKernelGGL(k_micro<double>, dim3(ctx->ncu), dim3(64 * micro_waves<double>()),
                        micro_lds_bytes<double>(), s, m);
@@ -769,6 +839,7 @@ extern "C" int ice3_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld
   SedimArgs sa;
   MicroArgs m;
   rain_ice_args(ncol, nlev, ld, dt, ctx->d_tab, in, io, out, stats, sa, m);
+  m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
   hipStream_t s = (hipStream_t)stream;
   MARK(0); MARK(1);
   return launch_rain_ice(ctx, prec, sa, m, stream);
@@ -802,6 +873,7 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
   SedimArgs sa;
   MicroArgs m;
   pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, stats, aa, sa, m);
+  m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
   hipStream_t s = (hipStream_t)stream;
   const dim3 gs((ncol + 255) / 256), gp((ncol + 255) / 256, nlev);
   MARK(0);
@@ -873,6 +945,7 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   SedimArgs sa;
   MicroArgs m;
   pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, stats, aa, sa, m);
+  m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
   hipStream_t s = (hipStream_t)stream;
   const unsigned int npts = (unsigned int)nlev * (unsigned int)ncol;
   const SgWork wl = sg_work_layout(npts);

@@ -905,7 +905,8 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   pa[IRG] = a;
 }
 
-constexpr int MAX_CALLS = 4096;  // sub-step cap per point (the reference has none)
+// Default sub-step cap per point (the reference's loop has none); a context can set another.
+constexpr int DEFAULT_MAX_CALLS = 4096;
 
 // Fixed inputs of ICE4_STEPPING for one point.
 struct StepPoint {
@@ -929,14 +930,21 @@ __device__ __forceinline__ void step_point(double dt, double exn, double rhodref
 // state update.  The point's state between calls: vart (th, rv, rc, rr, ri, rs, rg), cit,
 // time (part of the step integrated so far), S (rates of the hard call) and calls.
 // Returns 1 while the point's time step goes on, 0 when it is complete (vart then holds
-// the end state, :520-537), -1 when the sub-step cap was hit (vart as after the last
-// completed call).  pc is read by hard calls only.
+// the end state, :520-537), -1 when the point would make more than max_calls tendency
+// calls: vart then holds the state after the last completed call with the external
+// tendencies integrated so far taken out again (as the complete path does at :520-537 for
+// the whole step), so that the total tendencies count them once, through the sources.
+// pc is read by hard calls only.
 template <class R, bool SOFT, class TB>
 __device__ __forceinline__ int step_call(double dt, const StepPoint &p, const PointConst<R> &pc,
                                          double *vart, double &cit, double &time_,
-                                         StepState<R> &S, int &calls,
+                                         StepState<R> &S, int &calls, int max_calls,
                                          const TB *__restrict__ tab) {
-  if (++calls > MAX_CALLS) return -1;
+  if (++calls > max_calls) {
+#pragma unroll
+    for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] - p.extpk[jv] * time_;
+    return -1;
+  }
   double za[7], zb[7];
   const double exn = p.exn;
   const double sum2 = vart[IRI] + vart[IRS] + vart[IRG];
@@ -982,7 +990,7 @@ __device__ __forceinline__ int step_call(double dt, const StepPoint &p, const Po
 template <class R, class TB>
 __device__ __forceinline__ int stepping_point(double dt, double exn, double rhodref, double pres,
                                               double cf, double sigs, double &cit, double *vart,
-                                              const double *s, double &rrevav,
+                                              const double *s, double &rrevav, int max_calls,
                                               const TB *__restrict__ tab) {
   StepPoint p;
   step_point(dt, exn, rhodref, pres, cf, sigs, s, vart, p);
@@ -990,8 +998,8 @@ __device__ __forceinline__ int stepping_point(double dt, double exn, double rhod
   double time = 0.;
   int calls = 0;
   const PointConst<R> pc = point_const<R>(rhodref);
-  int st = step_call<R, false>(dt, p, pc, vart, cit, time, S, calls, tab);
-  while (st > 0) st = step_call<R, true>(dt, p, pc, vart, cit, time, S, calls, tab);
+  int st = step_call<R, false>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab);
+  while (st > 0) st = step_call<R, true>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab);
   if (st == 0) rrevav = (double)S.rrevav;
   return st < 0 ? -calls : calls;
 }

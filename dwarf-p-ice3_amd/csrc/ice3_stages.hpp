@@ -164,6 +164,8 @@ struct MicroArgs {
   double *ci, *evap3d, *rainfr;
   int *calls;
   unsigned long long *counters;  // [0] tendency calls, [1] capped points, [2] micro points
+  int max_calls;                 // sub-step cap per point
+  unsigned long long *capped;    // context counter of capped points (always counted)
 };
 
 // ICE4_CORRECT_NEGATIVITIES (micro/mode_ice4_correct_negativities.F90:51-171) on one
@@ -210,13 +212,13 @@ __device__ __forceinline__ void rain_ice_factors(const double *t, double exn, do
   lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide;
 }
 
-// RAIN_ICE's microphysics mask (micro/rain_ice.F90:391-410): a hydrometeor above XRTMIN.
+// RAIN_ICE's microphysics mask (micro/rain_ice.F90:365-381): a hydrometeor above XRTMIN.
 __device__ __forceinline__ bool micro_mask(const double *t) {
   return t[IRC] > P.XRTMIN[2] || t[IRR] > P.XRTMIN[3] || t[IRI] > P.XRTMIN[4] ||
          t[IRS] > P.XRTMIN[5] || t[IRG] > P.XRTMIN[6];
 }
 
-// Nucleation outside the microphysics mask (micro/rain_ice.F90:224-273) at point j with
+// Nucleation outside the microphysics mask (micro/rain_ice.F90:505-540) at point j with
 // t-state t: cit is reset and updated, rvheni receives the nucleated vapour.
 template <class R>
 __device__ __forceinline__ void micro_nucleation(const MicroArgs &a, size_t j, const double *t,
@@ -229,7 +231,7 @@ __device__ __forceinline__ void micro_nucleation(const MicroArgs &a, size_t j, c
                 cit, rvheni);
 }
 
-// RAIN_ICE section 8 at point j (micro/rain_ice.F90:529-543 and the reinstated 649-714):
+// RAIN_ICE section 8 at point j (micro/rain_ice.F90:612-652 and the reinstated 754-823):
 // total tendencies from the end state wr of the time splitting, ICE4_CORRECT_NEGATIVITIES,
 // and the stores.  The t-state and sources are read again from memory (they are not
 // written before this point's stores).  rvheni: vapour nucleated outside the mask.
@@ -266,8 +268,8 @@ __device__ __forceinline__ void micro_finish(const MicroArgs &a, size_t j, const
   if (a.calls) a.calls[j] = calls;
 }
 
-// RAIN_ICE sections 1, 4, 5, 8 (micro/rain_ice.F90:224-273, 391-431, 529-543 and the
-// reinstated 649-714) for point j, all in one thread.  Returns the number of tendency
+// RAIN_ICE sections 1, 4, 5, 8 (micro/rain_ice.F90:334-384, 505-540, 612-652 and the
+// reinstated 754-823) for point j, all in one thread.  Returns the number of tendency
 // calls, negative if the sub-step cap was hit.
 template <class R, class TB>
 __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j, const TB *__restrict__ tab) {
@@ -277,7 +279,7 @@ __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j, const TB *
   int calls = 0;
   if (micro_mask(wr))
     calls = stepping_point<R>(a.dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j], cit,
-                              wr, s, evap3d, tab);
+                              wr, s, evap3d, a.max_calls, tab);
   else
     micro_nucleation<R>(a, j, wr, cit, rvheni);
   micro_finish(a, j, wr, cit, evap3d, rvheni, calls);
@@ -342,6 +344,7 @@ inline void rain_ice_args(int ncol, int nlev, int ld, double dt, const double *t
   sa.inprg = out->inprg;
   m = MicroArgs{};
   m.ncol = ncol; m.nlev = nlev; m.ld = ld; m.dt = dt; m.tscale = 1.; m.tab = tab;
+  m.max_calls = DEFAULT_MAX_CALLS;
   m.exn = in->exn; m.rhodref = in->rhodref; m.exn_ref = in->exn_ref; m.pabs = in->pabs;
   m.cldfr = in->cldfr; m.sigs = in->sigs;
   const double *t[7] = {in->tht, in->rvt, in->rct, in->rrt, in->rit, in->rst, in->rgt};
@@ -382,6 +385,7 @@ inline void pipeline_args(int ncol, int nlev, int ld, double dt, const double *t
   sa.inprg = out->inprg;
   m = MicroArgs{};
   m.ncol = ncol; m.nlev = nlev; m.ld = ld; m.dt = dt; m.tscale = dt; m.tab = tab;
+  m.max_calls = DEFAULT_MAX_CALLS;
   m.exn = in->exn; m.rhodref = in->rhodref; m.exn_ref = in->exn_ref; m.pabs = in->pabs;
   m.cldfr = out->cldfr; m.sigs = in->sigs;
   const double *t[7] = {out->ths, out->rvs, out->rcs, in->rrs, out->evap3d, in->rss, in->rgs};

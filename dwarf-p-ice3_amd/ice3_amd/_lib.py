@@ -71,6 +71,7 @@ class Surrogate(ctypes.Structure):
 
 
 EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_constants",
+           "ice3_ctx_set_substep_cap", "ice3_ctx_status",
            "ice3_ice_adjust", "ice3_ice_adjust_ocnd2", "ice3_rain_ice", "ice3_adjust_rain_ice",
            "ice3_stage_timing",
            "ice3_stage_times", "ice3_adjust_rain_ice_stages", "ice3_adjust_rain_ice_surrogate",
@@ -96,6 +97,8 @@ def load():
         lib.ice3_ctx_destroy.restype = None
         lib.ice3_strerror.argtypes = [ctypes.c_int]
         lib.ice3_strerror.restype = ctypes.c_char_p
+        lib.ice3_ctx_set_substep_cap.argtypes = [_P, ctypes.c_int]
+        lib.ice3_ctx_status.argtypes = [_P, _P, ctypes.POINTER(ctypes.c_ulonglong)]
         lib.ice3_dump_constants.argtypes = [_P, ctypes.c_char_p, ctypes.c_long]
         lib.ice3_dump_constants.restype = ctypes.c_long
         common = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]

@@ -35,7 +35,7 @@ extern "C" {
 #define ICE3_OK 0
 #define ICE3_EINVAL (-1)     /* bad size, stride, precision or NULL pointer */
 #define ICE3_EDEVICE (-2)    /* HIP runtime error (launch, allocation, copy) */
-#define ICE3_ESUBSTEP (-3)   /* reported through ice3_stats: a point hit the sub-step cap */
+#define ICE3_ESUBSTEP (-3)   /* a point hit the sub-step cap (ice3_ctx_status, ice3_stats) */
 #define ICE3_ECONFIG (-4)    /* kernels built with constants other than the host builder's */
 
 /* arithmetic of the microphysics kernels */
@@ -50,6 +50,21 @@ typedef struct ice3_ctx ice3_ctx;
 int ice3_ctx_create(ice3_ctx **ctx);
 void ice3_ctx_destroy(ice3_ctx *ctx);
 const char *ice3_strerror(int code);
+
+/* ---- sub-step cap of the ICE4 time splitting (no counterpart in the reference: its loop
+ * has no bound, micro/mode_ice4_stepping.F90:256-516) ----
+ * A microphysics point stops after max_calls ICE4_TENDENCIES calls (default 4096): its
+ * outputs are then the processes integrated up to the time reached, with the external
+ * tendencies counted once (through the sources).  Every such point is counted in a
+ * context-owned device counter, whatever the stats argument of the call.
+ * ice3_ctx_set_substep_cap sets max_calls (>= 1) for the calls made after it.
+ * ice3_ctx_status waits for the work queued on stream, returns in *capped (may be NULL)
+ * the number of capped points of all calls on ctx since the previous query, resets the
+ * counter, and returns ICE3_ESUBSTEP if that number is not 0, else ICE3_OK.  Calls on
+ * other streams of the same context that overlap a query may be counted in it or in the
+ * next one. */
+int ice3_ctx_set_substep_cap(ice3_ctx *ctx, int max_calls);
+int ice3_ctx_status(ice3_ctx *ctx, void *stream, unsigned long long *capped);
 
 /* Host copy of the derived constants, written as text lines "NAME n v1 .. vn"
  * (the format of the reference constants dump) into buf; returns the number

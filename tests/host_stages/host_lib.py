@@ -63,6 +63,7 @@ def load():
         lib.hst_log_f64.argtypes = [dp, dp, ctypes.c_long]
         lib.hst_adjust_rain_ice.argtypes = [ctypes.c_int] + common + [
             ctypes.POINTER(L.PipeIn), ctypes.POINTER(L.PipeOut), ctypes.POINTER(L.Stats)]
+        lib.hst_set_substep_cap.argtypes = [ctypes.c_int]
         _lib = lib
     return _lib
 

@@ -14,6 +14,13 @@ using namespace ice3;
 
 static double *g_tab = nullptr;
 static double *g_tiwmx = nullptr;  // TIWMX tables of the OCND2 branch
+static int g_max_calls = DEFAULT_MAX_CALLS;  // sub-step cap (ice3_ctx_set_substep_cap)
+
+extern "C" int hst_set_substep_cap(int max_calls) {
+  if (max_calls < 1) return -1;
+  g_max_calls = max_calls;
+  return 0;
+}
 
 extern "C" int hst_init(void) {
   if (!g_tab || !g_tiwmx) {
@@ -78,6 +85,7 @@ extern "C" int hst_rain_ice(int prec, int ncol, int nlev, int ld, double dt,
   SedimArgs sa;
   MicroArgs m;
   rain_ice_args(ncol, nlev, ld, dt, g_tab, in, io, out, stats, sa, m);
+  m.max_calls = g_max_calls;
   if (prec == ICE3_PREC_FP64) run_rain_ice<double>(sa, m);
   else run_rain_ice<float>(sa, m);
   return 0;
@@ -91,6 +99,7 @@ extern "C" int hst_adjust_rain_ice(int prec, int ncol, int nlev, int ld, double 
   SedimArgs sa;
   MicroArgs m;
   pipeline_args(ncol, nlev, ld, dt, g_tab, in, out, stats, aa, sa, m);
+  m.max_calls = g_max_calls;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) adjust_at(aa, i, (size_t)k * ld + i);
   if (prec == ICE3_PREC_FP64) run_rain_ice<double>(sa, m);

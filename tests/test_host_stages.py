@@ -79,6 +79,37 @@ def test_host_pipeline_matches_oracle():
     assert int(out["counters"][1]) == 0
 
 
+def test_host_substep_cap_counts_external_tendencies_once():
+    """A point stopped by the sub-step cap keeps the processes integrated so far and counts
+    its external tendencies (here the sedimentation of rr, ri, rs, rg) once, through the
+    sources: with a cap of one call, every column's water still changes only by its surface
+    precipitation, the capped points are those that needed more calls, and the other points
+    are unchanged."""
+    lib = host_lib.load()
+    f = _pipeline_input(128, seed=20260021)
+    dt = 50.0
+    free = host_lib.adjust_rain_ice(f, dt, 0)
+    assert lib.hst_set_substep_cap(1) == 0
+    try:
+        capped = host_lib.adjust_rain_ice(f, dt, 0)
+    finally:
+        assert lib.hst_set_substep_cap(4096) == 0
+    more = np.abs(free["calls"]) > 1
+    assert more.sum() > 100
+    assert int(capped["counters"][1]) == int(more.sum())
+    assert np.array_equal(capped["calls"] < 0, more)
+    for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs"):
+        assert np.array_equal(capped[k][~more], free[k][~more]), k
+    rho, dz = f["rhodref"], f["dzz"]
+    w = ("rvs", "rcs", "rrs", "ris", "rss", "rgs")
+    col0 = (rho * dz * sum(f[k] for k in w)).sum(0) * dt
+    col1 = (rho * dz * sum(capped[k] for k in w)).sum(0) * dt
+    prec = -1000.0 * (capped["inprr"] + capped["inprs"] + capped["inprg"]) * dt
+    err = np.max(np.abs(col1 - col0 - prec) / col0)
+    print("cap of one call: capped points", int(more.sum()), "column water budget max rel err", err)
+    assert err < 1e-12
+
+
 @pytest.mark.parametrize("tag", ["base", "cov"])
 def test_host_rain_ice_golden(golden_dir, tag):
     from oracle import oracle_lib
