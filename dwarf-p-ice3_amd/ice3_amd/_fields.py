@@ -25,8 +25,35 @@ def as_columns(x, layout: str, *, device, name: str) -> torch.Tensor:
 
 
 def from_columns(t: torch.Tensor, layout: str, shape) -> torch.Tensor:
-    """Inverse of as_columns for results."""
-    return t if layout == "columns" else t.t().reshape(shape)
+    """Inverse of as_columns for results: a (nlev, ncol) field back to ``shape``, or a
+    per-column (ncol,) result to ``shape[:-1]`` (the JAX layout without its vertical axis)."""
+    if layout == "columns":
+        return t
+    if t.dim() == 1:
+        return t.reshape(tuple(shape[:-1]))
+    return t.t().reshape(shape)
+
+
+LAYOUTS = ("auto", "columns", "jax")
+
+
+def check_layout(layout: str) -> str:
+    if layout not in LAYOUTS:
+        raise ValueError(f"layout must be one of {LAYOUTS}, got {layout!r}")
+    return layout
+
+
+def resolve_layout(layout: str, x) -> str:
+    """The layout of one call: "auto" reads it from the array rank of ``x`` (a field of the
+    call): three or more dimensions are the JAX components' (nx, ny, nz) /
+    (ngpblks, nproma, nflevg) arrays, vertical index last; two are the Fortran
+    (nlev, ncol) memory, columns contiguous."""
+    if layout != "auto":
+        return layout
+    nd = getattr(x, "ndim", None)
+    if nd is None:
+        nd = torch.as_tensor(x).dim()
+    return "jax" if nd >= 3 else "columns"
 
 
 def ptr(t) -> ctypes.c_void_p:

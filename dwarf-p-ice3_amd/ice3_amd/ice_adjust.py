@@ -20,7 +20,7 @@ import ctypes
 import torch
 
 from . import _lib
-from ._fields import as_columns, from_columns, ptr, stream_ptr
+from ._fields import as_columns, check_layout, from_columns, ptr, resolve_layout, stream_ptr
 
 _PREC = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}
 _OUT_ORDER = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri",
@@ -76,8 +76,10 @@ class IceAdjust:
         ``Phyex`` object; the kernels implement the pinned AROME ICE3 configuration (KRR=6,
         sigma_s subgrid condensation, CB02) and anything else raises.
     jit : accepted for API compatibility (kernels are compiled ahead of time).
-    layout : "columns" for (nlev, ncol) arrays with columns contiguous (the Fortran
-        (NIJT, NKT) memory, no copy), or "jax" for the JAX components' (nx, ny, nz).
+    layout : "auto" (default): each call reads the layout from its arrays' rank, so that
+        the JAX components' (nx, ny, nz) arrays (vertical index last) and the Fortran
+        (nlev, ncol) memory (columns contiguous, no copy) are both taken as they are;
+        "jax" or "columns" forces one.  Results come back in the layout of the inputs.
     device : CUDA device (default: current).
 
     The call takes IceAdjustJAX's arguments; ``check_inputs=True`` first counts the NaN and
@@ -87,29 +89,27 @@ class IceAdjust:
 
     _OCND2 = False
 
-    def __init__(self, phyex=None, jit: bool = True, layout: str = "columns", device=None):
+    def __init__(self, phyex=None, jit: bool = True, layout: str = "auto", device=None):
         check_configuration(phyex, ocnd2=self._OCND2)
-        if layout not in ("columns", "jax"):
-            raise ValueError("layout must be 'columns' or 'jax'")
-        self.layout = layout
+        self.layout = check_layout(layout)
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         self._lib = _lib.load()
         self._ctx = _lib.context(self.device.index)
 
-    def _inputs(self, sigqsat, args):
+    def _inputs(self, sigqsat, args, layout):
         """Columns-layout device tensors of the fields and the per-column sigqsat."""
-        f = {k: as_columns(v, self.layout, device=self.device, name=k) for k, v in args.items()}
+        f = {k: as_columns(v, layout, device=self.device, name=k) for k, v in args.items()}
         nlev, ncol = f["th"].shape
         for k, v in f.items():
             if tuple(v.shape) != (nlev, ncol):
                 raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
-        sq = self._per_column(sigqsat, "sigqsat", ncol)
+        sq = self._per_column(sigqsat, "sigqsat", ncol, layout)
         return f, sq, nlev, ncol
 
-    def _per_column(self, x, name, ncol):
+    def _per_column(self, x, name, ncol, layout):
         t = torch.as_tensor(x, dtype=torch.float64, device=self.device)
-        if t.dim() > 1:  # a full field: the value is per column, take the lowest level
-            t = as_columns(t, self.layout, device=self.device, name=name)[0]
+        if t.dim() > 1 and t.numel() != ncol:  # a full field: per column, take the lowest level
+            t = as_columns(t, layout, device=self.device, name=name)[0]
         t = t.reshape(-1).contiguous()
         if t.numel() != ncol:
             raise ValueError(f"{name}: {t.numel()} values for {ncol} columns")
@@ -121,8 +121,9 @@ class IceAdjust:
         args = dict(pabs=pabs, sigs=sigs, th=th, exn=exn, exn_ref=exn_ref, rho_dry_ref=rho_dry_ref,
                     rv=rv, rc=rc, ri=ri, rr=rr, rs=rs, rg=rg, cf_mf=cf_mf, rc_mf=rc_mf, ri_mf=ri_mf,
                     rvs=rvs, rcs=rcs, ris=ris, ths=ths)
-        shape = tuple(torch.as_tensor(th).shape)
-        f, sq, nlev, ncol = self._inputs(sigqsat, args)
+        shape = tuple(th.shape) if hasattr(th, "shape") else tuple(torch.as_tensor(th).shape)
+        layout = resolve_layout(self.layout, th)
+        f, sq, nlev, ncol = self._inputs(sigqsat, args, layout)
         if check_inputs:
             _lib.require_finite({**f, "sigqsat": sq}, self.device)
         out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) for k in _OUT_ORDER}
@@ -134,9 +135,9 @@ class IceAdjust:
                                             ctypes.byref(cin), ctypes.byref(cout), stream_ptr(self.device))
         _lib.check(rc_, "ice3_ice_adjust")
         res = [out[k] for k in _OUT_ORDER]
-        res = [from_columns(r, self.layout, shape) for r in res]
+        res = [from_columns(r, layout, shape) for r in res]
         if srcs is not None:
-            srcs = from_columns(srcs, self.layout, shape)
+            srcs = from_columns(srcs, layout, shape)
         return (*res, srcs) if return_srcs else tuple(res)
 
 
@@ -163,9 +164,10 @@ class IceAdjustOCND2(IceAdjust):
         args = dict(pabs=pabs, sigs=sigs, th=th, exn=exn, exn_ref=exn_ref, rho_dry_ref=rho_dry_ref,
                     rv=rv, rc=rc, ri=ri, rr=rr, rs=rs, rg=rg, cf_mf=cf_mf, rc_mf=rc_mf, ri_mf=ri_mf,
                     rvs=rvs, rcs=rcs, ris=ris, ths=ths, zz=zz)
-        shape = tuple(torch.as_tensor(th).shape)
-        f, sq, nlev, ncol = self._inputs(sigqsat, args)
-        wgt = self._per_column(ice_cld_wgt, "ice_cld_wgt", ncol)
+        shape = tuple(th.shape) if hasattr(th, "shape") else tuple(torch.as_tensor(th).shape)
+        layout = resolve_layout(self.layout, th)
+        f, sq, nlev, ncol = self._inputs(sigqsat, args, layout)
+        wgt = self._per_column(ice_cld_wgt, "ice_cld_wgt", ncol, layout)
         if nlev == 1:
             raise ValueError("the OCND2 branch needs at least two levels")
         if check_inputs:
@@ -182,7 +184,7 @@ class IceAdjustOCND2(IceAdjust):
                                                   ctypes.byref(cin), ctypes.byref(cin2), ctypes.byref(cout),
                                                   ctypes.byref(cout2), stream_ptr(self.device))
         _lib.check(rc_, "ice3_ice_adjust_ocnd2")
-        res = [from_columns(out[k], self.layout, shape) for k in names]
+        res = [from_columns(out[k], layout, shape) for k in names]
         if srcs is not None:
-            srcs = from_columns(srcs, self.layout, shape)
+            srcs = from_columns(srcs, layout, shape)
         return (*res, srcs) if return_srcs else tuple(res)

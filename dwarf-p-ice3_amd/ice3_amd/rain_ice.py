@@ -14,6 +14,21 @@ rs_t, rg_t, ci_t, ths, rvs, rcs, rrs, ris, rss, rgs; optional: exnref (default e
 rhodj (default rhodref), cf (cloud fraction, default 0), sigs (default 0), hlc_hrc,
 hlc_hcf, hli_hri, hli_hcf (default 0).
 
+Layouts: with ``layout="auto"`` (the default) each call takes RainIceJAX's
+``(ngpblks, nproma, nflevg)`` arrays (vertical index last; any rank of 3 or more) or the
+Fortran ``(nlev, ncol)`` memory (columns contiguous, no copy), read from the rank of
+``th_t``; results come back in the same layout, and the per-column diagnostics
+``inpr*`` with the shape of the inputs without their vertical axis, as RainIceJAX
+returns them.
+
+Where the returned state differs from RainIceJAX's: RainIceJAX overwrites the sources
+with the microphysics tendency alone and returns the time-split, negativity-corrected
+t-state in ``th_t..rg_t`` (src/ice3/jax/rain_ice.py:347-373).  Here the semantics are the
+Fortran RAIN_ICE's: the sources are the input sources plus the sedimentation and the
+microphysics tendencies, corrected for negativities (micro/rain_ice.F90:754-823), and
+``th_t..rg_t``, ``exn``, ``rhodref`` and the other inputs are returned as they were given.
+The state at t + dt is ``x_t + dt * xs``.
+
 ``check_inputs=True`` (RainIce, AdjustRainIce and IceAdjust) first counts the NaN and
 infinite input values on the device (``ice3_count_nonfinite``) and raises ValueError
 naming the fields that hold any; the reference has no such check, so it is off by
@@ -26,26 +41,28 @@ import ctypes
 import torch
 
 from . import _lib
-from ._fields import as_columns, from_columns, ptr, stream_ptr
+from ._fields import as_columns, check_layout, from_columns, ptr, resolve_layout, stream_ptr
 from .ice_adjust import check_configuration
 
 _PREC = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}
 
 
 class _Base:
-    def __init__(self, constants=None, layout: str = "columns", device=None):
+    def __init__(self, constants=None, layout: str = "auto", device=None):
         if not isinstance(constants, dict):  # a constants dict is accepted and not read
             check_configuration(constants, "constants")
-        if layout not in ("columns", "jax"):
-            raise ValueError("layout must be 'columns' or 'jax'")
-        self.layout = layout
+        self.layout = check_layout(layout)
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         self._lib = _lib.load()
         self._ctx = _lib.context(self.device.index)
         self.last_counters = None
 
-    def _col(self, x, name):
-        return as_columns(x, self.layout, device=self.device, name=name)
+    def _col(self, x, name, layout):
+        return as_columns(x, layout, device=self.device, name=name)
+
+    @staticmethod
+    def _shape(x):
+        return tuple(x.shape) if hasattr(x, "shape") else tuple(torch.as_tensor(x).shape)
 
     def _stats(self, n, with_calls):
         counters = torch.zeros(3, dtype=torch.int64, device=self.device)
@@ -67,12 +84,13 @@ class RainIce(_Base):
                  precision: str = "fp64", return_calls: bool = False, check_inputs: bool = False):
         if lsedim_after or ldeposc or ldsoft or not lfeedbackt:
             raise ValueError("only LSEDIM_AFTER=F, LDEPOSC=F, LFEEDBACKT=T are implemented")
-        shape = tuple(torch.as_tensor(state["th_t"]).shape)
-        g = lambda k: self._col(state[k], k)  # noqa: E731
+        shape = self._shape(state["th_t"])
+        layout = resolve_layout(self.layout, state["th_t"])
+        g = lambda k: self._col(state[k], k, layout)  # noqa: E731
         th = g("th_t")
         nlev, ncol = th.shape
         z = torch.zeros((nlev, ncol), dtype=torch.float64, device=self.device)
-        opt = lambda k, d: self._col(state[k], k) if k in state else d  # noqa: E731
+        opt = lambda k, d: self._col(state[k], k, layout) if k in state else d  # noqa: E731
         exn = g("exn")
         rhodref = g("rhodref")
         inp = dict(exn=exn, dzz=g("dzz"), rhodj=opt("rhodj", rhodref), rhodref=rhodref,
@@ -103,13 +121,13 @@ class RainIce(_Base):
         self._finish_counters(counters)
         new = dict(state)
         for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs"):
-            new[k] = from_columns(io[k], self.layout, shape)
-        new["ci_t"] = from_columns(io["ci"], self.layout, shape)
-        new["evap3d"] = from_columns(out["evap3d"], self.layout, shape)
-        new["rainfr"] = from_columns(out["rainfr"], self.layout, shape)
-        diags = {k: out[k] for k in ("inprc", "inprr", "inpri", "inprs", "inprg")}
+            new[k] = from_columns(io[k], layout, shape)
+        new["ci_t"] = from_columns(io["ci"], layout, shape)
+        new["evap3d"] = from_columns(out["evap3d"], layout, shape)
+        new["rainfr"] = from_columns(out["rainfr"], layout, shape)
+        diags = {k: from_columns(out[k], layout, shape) for k in ("inprc", "inprr", "inpri", "inprs", "inprg")}
         if return_calls:
-            diags["calls"] = from_columns(calls.view(nlev, ncol), self.layout, shape)
+            diags["calls"] = from_columns(calls.view(nlev, ncol), layout, shape)
         return new, diags
 
 
@@ -123,12 +141,13 @@ class AdjustRainIce(_Base):
 
     def __call__(self, f: dict, dt: float, precision: str = "fp64", return_calls: bool = False,
                  check_inputs: bool = False):
-        shape = tuple(torch.as_tensor(f["th"]).shape)
-        g = {k: self._col(f[k], k) for k in _lib.PIPE_IN_FIELDS if k != "sigqsat"}
+        shape = self._shape(f["th"])
+        layout = resolve_layout(self.layout, f["th"])
+        g = {k: self._col(f[k], k, layout) for k in _lib.PIPE_IN_FIELDS if k != "sigqsat"}
         nlev, ncol = g["th"].shape
         sq = torch.as_tensor(f["sigqsat"], dtype=torch.float64, device=self.device)
-        if sq.dim() > 1:
-            sq = self._col(sq, "sigqsat")[0]
+        if sq.dim() > 1 and sq.numel() != ncol:  # a full field: per column, the lowest level
+            sq = self._col(sq, "sigqsat", layout)[0]
         sq = sq.reshape(-1).contiguous()
         if sq.numel() != ncol:
             raise ValueError(f"sigqsat: {sq.numel()} values for {ncol} columns")
@@ -150,7 +169,7 @@ class AdjustRainIce(_Base):
                                                 stream_ptr(self.device))
         _lib.check(rc, "ice3_adjust_rain_ice")
         self._finish_counters(counters)
-        res = {k: (from_columns(v, self.layout, shape) if v.dim() == 2 else v) for k, v in out.items()}
+        res = {k: from_columns(v, layout, shape) for k, v in out.items()}
         if return_calls:
-            res["calls"] = from_columns(calls.view(nlev, ncol), self.layout, shape)
+            res["calls"] = from_columns(calls.view(nlev, ncol), layout, shape)
         return res

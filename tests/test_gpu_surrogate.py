@@ -1,5 +1,6 @@
 """The surrogate's matrix-core kernels against the CPU restatement (oracle/mlp_oracle.py),
 its training on FP64 pairs, and the fused step with the surrogate and its fallback."""
+import os
 import numpy as np
 import pytest
 
@@ -160,3 +161,97 @@ def test_surrogate_step_fallback_and_budget(dev, small_surrogate):
     ratios = [e_t[k] / e_u[k] for k in ("rvs", "rcs", "rrs", "ris", "rss", "rgs")]
     assert max(ratios) < 1.0, ratios
     assert float(np.exp(np.mean(np.log(ratios)))) < 0.5, ratios
+
+
+# Error of the committed surrogate against the FP64 step at 1024x1024x90 on fresh generator
+# columns: RMS error over all points relative to the field's RMS, per output.  The network
+# answers at least SG_MIN_NETWORK of the points.
+SG_RMS_REL = {"ths": 2e-3, "rvs": 5e-2, "rcs": 5e-2, "rrs": 0.15, "ris": 8e-2, "rss": 0.15, "rgs": 0.10,
+              "ci": 0.15}
+SG_MIN_NETWORK = 0.90
+
+
+def _fresh_columns(dev, ncol, seeds):
+    """(nlev, ncol) device fields of the pipeline generator, one 512x512 slab per seed (new
+    columns in every slab: no slab is a copy of another)."""
+    from ice3_amd import synthetic
+    per = ncol // len(seeds)
+    g = None
+    for s, seed in enumerate(seeds):
+        f = synthetic.generate(per, 90, seed=seed, variant="pipeline")
+        f["rhodref"] = f["rho_dry_ref"]
+        if g is None:
+            g = {k: torch.empty((v.shape[0], ncol) if v.ndim == 2 else (ncol,), dtype=torch.float64, device=dev)
+                 for k, v in f.items()}
+        for k, v in f.items():
+            g[k][..., s * per:(s + 1) * per].copy_(torch.from_numpy(v))
+        del f
+    return g
+
+
+@pytest.mark.timeout(600)
+def test_committed_surrogate_full_size(dev):
+    """The committed network (ice3_amd/data/surrogate.npz, trained on 50M FP64 pairs) in the
+    fused step at its stated configuration, 1024x1024 columns x 90 levels of fresh generator
+    columns (seeds the training never drew): at least SG_MIN_NETWORK of the points answered
+    by the network; per-field RMS error against the FP64 step within SG_RMS_REL, and well
+    below the error of leaving the microphysics out; every fallback point equal to the FP64
+    step; sources of rv..rg non-negative; every column's water changed only by its surface
+    precipitation."""
+    import ctypes
+    from ice3_amd import AdjustRainIce, _lib
+    from ice3_amd import surrogate as S
+    from ice3_amd._fields import ptr
+    if not os.path.exists(S.DEFAULT_WEIGHTS):
+        pytest.skip("no committed weights")
+    ncol = 1024 * 1024
+    g = _fresh_columns(dev, ncol, (20260501, 20260502, 20260503, 20260504))
+    ref = AdjustRainIce()(g, 50.0, return_calls=True)
+    sg = S.TrainedSurrogate.load(S.DEFAULT_WEIGHTS, device=dev.index)
+    run = S.AdjustRainIceSurrogate(sg)
+    out = run(g, 50.0, return_calls=True)
+    c = run.last_counters
+    npts = ncol * 90
+    frac = c["network"] / npts
+    print("committed surrogate, 1024x1024x90 fresh columns:", c, f"network fraction {frac:.4f}")
+    assert c["network"] + c["out_of_range"] + c["negative"] + c["water_residual"] + c["no_hydrometeor"] == npts
+    assert frac >= SG_MIN_NETWORK
+    net = (ref["calls"] != 0) & (out["calls"] == 0)
+    assert int(net.sum()) == c["network"]
+    fall = ~net
+    for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "ci", "evap3d"):
+        assert torch.equal(out[k][fall], ref[k][fall]), k
+    for k in ("rvs", "rcs", "rrs", "ris", "rss", "rgs"):
+        assert bool((out[k] >= 0).all()), k
+    rho, dz = g["rhodref"], g["dzz"]
+    w = ("rvs", "rcs", "rrs", "ris", "rss", "rgs")
+    col0 = (rho * dz * sum(g[k] for k in w)).sum(0) * 50.0
+    col1 = (rho * dz * sum(out[k] for k in w)).sum(0) * 50.0
+    pr = -1000.0 * (out["inprr"] + out["inprs"] + out["inprg"]) * 50.0
+    berr = ((col1 - col0 - pr).abs() / col0).max().item()
+    print("surrogate column water budget max rel err", berr)
+    assert berr < 1e-12
+
+    # the method without the microphysics: ICE_ADJUST and the sedimentation only
+    lib, ctx = _lib.load(), _lib.context(dev.index)
+    none = {k: torch.empty((90, ncol), dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[:15]}
+    none.update({k: torch.empty(ncol, dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[15:]})
+    cin = _lib.PipeIn(*[ptr(g[k]) for k in _lib.PIPE_IN_FIELDS])
+    cout = _lib.PipeOut(*[ptr(none[k]) for k in _lib.PIPE_OUT_FIELDS])
+    rc = lib.ice3_adjust_rain_ice_stages(ctx, 0, ncol, 90, ncol, 50.0, ctypes.byref(cin), ctypes.byref(cout),
+                                         None, _lib.STAGE_ADJUST | _lib.STAGE_SEDIMENTATION,
+                                         ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+    assert rc == 0
+    none["ci"] = g["ci"]
+    e_net, e_none = {}, {}
+    for k in SG_RMS_REL:
+        rms = ref[k].pow(2).mean().sqrt()
+        e_net[k] = ((out[k] - ref[k]).pow(2).mean().sqrt() / rms).item()
+        e_none[k] = ((none[k] - ref[k]).pow(2).mean().sqrt() / rms).item()
+    print("relative RMS error vs FP64: network", {k: f"{v:.2e}" for k, v in e_net.items()},
+          "without microphysics", {k: f"{v:.2e}" for k, v in e_none.items()})
+    bad = {k: v for k, v in e_net.items() if not v <= SG_RMS_REL[k]}
+    assert not bad, bad
+    gain = [e_net[k] / e_none[k] for k in ("rvs", "rcs", "rrs", "ris", "rss", "rgs") if e_none[k] > 0]
+    print("network error / error without microphysics:", [f"{x:.3f}" for x in gain])
+    assert max(gain) < 0.5, gain

@@ -86,3 +86,72 @@ def test_surrogate_structs_match_header():
     body = re.search(r"typedef struct \{([^}]*)\} ice3_surrogate;", hdr).group(1)
     names = re.findall(r"(\w+);", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
     assert names == [f for f, _ in _lib.Surrogate._fields_]
+
+
+def _torch_net(p):
+    """The surrogate network in torch float64 from the flat parameter vector: layer l maps
+    x -> x W_l + b_l with W_l (K x N) row-major, SiLU after the four hidden layers."""
+    import torch
+    import torch.nn.functional as F
+    pt = torch.as_tensor(p, dtype=torch.float64)
+    ws, bs, o = [], [], 0
+    for k, n in mo.DIMS:
+        ws.append(pt[o:o + k * n].reshape(k, n))
+        o += k * n
+    ob = mo.NW
+    for k, n in mo.DIMS:
+        bs.append(pt[ob:ob + n])
+        ob += n
+
+    def run(x):
+        h = torch.as_tensor(x, dtype=torch.float64)
+        for l in range(mo.LAYERS):
+            z = F.linear(h, ws[l].t(), bs[l])
+            h = F.silu(z) if l < mo.LAYERS - 1 else z
+        return h
+    return run
+
+
+def test_forward_matches_torch_float64():
+    """The restatement's float64 forward pass (no bf16 rounding) against torch's own
+    F.linear and F.silu: a wrong activation, a transposed weight or a missing bias would
+    show here, independently of the finite-difference check of the backward pass."""
+    p, x, _ = _net(3)
+    y, zs, hs = mo.forward(p, x, emulate=False)
+    yt = _torch_net(p)(x).numpy()
+    err = np.abs(y - yt).max() / np.abs(yt).max()
+    print("forward vs torch float64: max rel err", err)
+    assert err < 1e-13
+
+
+def test_gradient_matches_torch_autograd():
+    """The restatement's float64 loss and gradient (no bf16 rounding) against torch autograd
+    on the same network and mean-squared-error loss (sum over the 8 outputs, scaled by
+    1/(B*8) in the gradient as the kernels scale dY)."""
+    import torch
+    p, x, t = _net(4)
+    loss, g = mo.loss_and_grad(p, x, t, emulate=False)
+    pt = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    net = _torch_net(pt)
+    y = net(x)[:, :mo.NTARG]
+    d = y - torch.as_tensor(t)
+    lt = (d ** 2).sum()
+    (lt / (x.shape[0] * mo.NTARG)).backward()
+    assert abs(loss - lt.item()) <= 1e-12 * lt.item()
+    gt = pt.grad.numpy()
+    err = np.abs(g - gt).max() / np.abs(gt).max()
+    print("gradient vs torch autograd: max rel err", err)
+    assert err < 1e-11
+
+
+def test_bf16_rounding_matches_torch():
+    """The restatement's bf16 rounding (round to nearest even) against torch's conversion,
+    on random values, exact ties and values near the float32 range limits."""
+    import torch
+    rng = np.random.Generator(np.random.Philox(9))
+    v = np.concatenate([rng.normal(0, 1, 4096), rng.normal(0, 1e-30, 256), rng.normal(0, 1e30, 256)]).astype(np.float32)
+    u = v.view(np.uint32)
+    ties = ((u & 0xFFFF0000) | 0x8000).view(np.float32)  # exactly half-way between two bf16
+    allv = np.concatenate([v, ties])
+    want = torch.from_numpy(allv).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(mo.bf16(allv), want)
