@@ -7,8 +7,9 @@ sedimentation -> ICE4 microphysics -> total tendencies -> negativity correction 
 all columns: one ice3_adjust_rain_ice call, i.e. the kernels k_adjust_sedim (ICE_ADJUST
 and the sedimentation in one column walk) and k_micro on one stream.  Inputs are
 resident in HBM before the timed region;
-the K steps are bracketed by synchronisations and timed with HIP events on the
-launch stream.
+the K steps are bracketed by synchronisations, each step is timed by its own pair of
+HIP events on the launch stream, and the reported time is the median step (the mean
+is reported beside it); every call's return code is checked after the timed loop.
 
 Also reported: the duration of each stage (HIP events recorded by the library
 around each kernel, in a second pass of K steps), the roofline of the dominant
@@ -79,6 +80,13 @@ def cpu_baseline(budget_s: float = 20.0, workers: int | None = None) -> dict:
     from oracle import ref_lib
 
     kind = "reference" if ref_lib.available() else "port"
+    build = None
+    if kind == "reference":
+        if ref_lib.available(fast=True):
+            ref_lib.use_fast_build()  # inherited by the forked workers
+            build = "flang -O3 -march=x86-64-v3 -fdefault-real-8 (FMA contraction)"
+        else:
+            build = "flang -O2 -fdefault-real-8 (x86-64 baseline, no FMA)"
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     workers = workers or max(1, min(16, ncpu))
     per_col = _cpu_slab((kind, 256, SEED))[1] / 256          # calibration on one core
@@ -89,7 +97,7 @@ def cpu_baseline(budget_s: float = 20.0, workers: int | None = None) -> dict:
             40 * budget_s + 300)
     wall = max(t for _, t in res)
     ncol = ncol_w * workers
-    return {"value": ncol / wall, "unit": "columns/s", "cores": workers, "kind": kind,
+    return {"value": ncol / wall, "unit": "columns/s", "cores": workers, "kind": kind, "build": build,
             "sample": f"{workers} processes x {ncol_w} columns x {NLEV} levels of the same generator "
                       f"(seeds {SEED + 1}..{SEED + workers}), fused ice_adjust+rain_ice FP64, "
                       f"slowest worker {wall:.1f} s",
@@ -139,8 +147,8 @@ def _cpu_slab(arg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ncol", type=int, default=512 * 512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -196,17 +204,10 @@ def main():
                                         ctypes.byref(cout), None, sp)
 
     def run(prec, out, steps, warmup):
+        """(median, mean) ms per step over `steps` steps, each bracketed by its own HIP events
+        on the launch stream; every call's return code is checked after the timed loop."""
         cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
-        for _ in range(warmup):
-            _lib.check(call(prec, cout), "warmup")
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(steps):
-            call(prec, cout)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / steps  # ms per step
+        return timed_steps(lambda: call(prec, cout), stream, steps, warmup)
 
     def stage_times(prec, out, steps):
         """Average duration of each stage's kernel, from the library's HIP events."""
@@ -224,7 +225,7 @@ def main():
         return dict(zip(("ice_adjust", "sedimentation", "microphysics"), [x / steps for x in acc]))
 
     out64 = outputs()
-    ms = run(_lib.PREC_FP64, out64, args.steps, args.warmup)
+    ms, ms_mean = run(_lib.PREC_FP64, out64, args.steps, args.warmup)
     stages = stage_times(_lib.PREC_FP64, out64, args.steps)
     npts = ncol * NLEV
     alg_bytes = BYTES_PER_POINT * npts + BYTES_PER_COLUMN * ncol
@@ -239,6 +240,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
+        "ms_per_step_mean": ms_mean,
+        "timing": f"median of {args.steps} steps, each between its own HIP events on the launch stream",
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -289,7 +292,7 @@ def main():
     res["ice4_substeps"] = substeps(_lib.PREC_FP64, out64)
     if not args.no_mixed:
         out32 = outputs()
-        ms32 = run(_lib.PREC_MIXED, out32, args.steps, args.warmup)
+        ms32, _ = run(_lib.PREC_MIXED, out32, args.steps, args.warmup)
         stages32 = stage_times(_lib.PREC_MIXED, out32, args.steps)
         err = {}
         for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs"):
@@ -314,6 +317,30 @@ def main():
     print(json.dumps(res))
 
 
+def timed_steps(fn, stream, steps, warmup):
+    """Run fn (one step, returning a C return code) warmup times, synchronise, then `steps`
+    times with a HIP event pair around each call on `stream`; returns (median, mean) ms.
+    Return codes are collected and checked after the loop so the checks do not perturb it."""
+    import statistics
+
+    import torch
+    from ice3_amd import _lib
+    for _ in range(warmup):
+        _lib.check(fn(), "warmup")
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    rcs = []
+    for e0, e1 in ev:
+        e0.record(stream)
+        rcs.append(fn())
+        e1.record(stream)
+    torch.cuda.synchronize()
+    for rc in rcs:
+        _lib.check(rc, "timed step")
+    ms = [e0.elapsed_time(e1) for e0, e1 in ev]
+    return statistics.median(ms), sum(ms) / len(ms)
+
+
 MLP_FLOP_PER_POINT = 2.0 * (14 * 128 + 3 * 128 * 128 + 128 * 8)   # 103,936
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
@@ -330,17 +357,8 @@ def input_check(lib, g, ncol, stream, sp, steps):
     cp = ctypes.c_void_p(counts.data_ptr())
 
     def call():
-        _lib.check(lib.ice3_count_nonfinite(len(names), fields, rows, ncol, ncol, cp, sp),
-                   "ice3_count_nonfinite")
-    call()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(steps):
-        call()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
+        return lib.ice3_count_nonfinite(len(names), fields, rows, ncol, ncol, cp, sp)
+    ms, _ = timed_steps(call, stream, steps, 1)
     nbytes = sum(g[k].numel() for k in names) * 8
     return {"kernel": "k_count_nonfinite", "fields": len(names), "ms": ms,
             "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
@@ -348,19 +366,30 @@ def input_check(lib, g, ncol, stream, sp, steps):
 
 
 def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
-    """The learned surrogate at 1024x1024 columns x 90 levels (the 512x512 slab tiled 4x along
-    the columns): FP64 step vs surrogate step (network + FP64 fallback), per-field error of the
-    surrogate against FP64, fallback statistics, water budget, matrix-core roofline of the
-    network kernel (103,936 FLOP per point)."""
+    """The learned surrogate at 1024x1024 columns x 90 levels: the 512x512 slab of the
+    headline workload followed by three more slabs of fresh generator columns (seeds
+    SEED + 1001..1003).  FP64 step vs surrogate step (network + FP64 fallback), per-field
+    error of the surrogate against FP64, fallback statistics, water budget, matrix-core
+    roofline of the network kernel (103,936 FLOP per point)."""
     import torch
 
-    from ice3_amd import _lib
+    from ice3_amd import _lib, synthetic
     from ice3_amd import surrogate as S
     from ice3_amd._fields import ptr
     if not os.path.exists(S.DEFAULT_WEIGHTS):
         return {"skipped": "no trained weights (" + os.path.relpath(S.DEFAULT_WEIGHTS, ROOT) + ")"}
     sg = S.TrainedSurrogate.load(S.DEFAULT_WEIGHTS, device=dev.index)
-    G = {k: (v.repeat(1, reps) if v.dim() == 2 else v.repeat(reps)).contiguous() for k, v in g.items()}
+    per = g["th"].shape[1]
+    G = {k: torch.empty((v.shape[0], per * reps) if v.dim() == 2 else (per * reps,), dtype=v.dtype, device=dev)
+         for k, v in g.items()}
+    for k, v in g.items():
+        G[k][..., :per].copy_(v)
+    for r in range(1, reps):
+        f = synthetic.generate(per, NLEV, seed=SEED + 1000 + r, variant="pipeline", dt=DT)
+        f["rhodref"] = f.pop("rho_dry_ref")
+        for k, v in f.items():
+            G[k][..., r * per:(r + 1) * per].copy_(torch.from_numpy(v))
+        del f
     nlev, ncol = G["th"].shape
     npts = nlev * ncol
 
@@ -386,20 +415,8 @@ def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
         return lib.ice3_adjust_rain_ice_surrogate(ctx, ncol, nlev, ncol, DT, ctypes.byref(cin), ctypes.byref(cout),
                                                   ctypes.byref(h), ptr(work), ptr(sgc), None, sp)
 
-    def timed(fn):
-        for _ in range(args.warmup):
-            _lib.check(fn(), "warmup")
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(args.steps):
-            fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / args.steps
-
-    ms_plain = timed(plain)
-    ms_sg = timed(surrogate)
+    ms_plain, _ = timed_steps(plain, stream, args.steps, args.warmup)
+    ms_sg, _ = timed_steps(surrogate, stream, args.steps, args.warmup)
     _lib.check(lib.ice3_stage_timing(ctx, 1), "ice3_stage_timing")
     acc = [0.0] * 6
     ms4 = (ctypes.c_float * 6)()
@@ -478,16 +495,7 @@ def ice_adjust_alone(lib, ctx, dev, args, ncol: int = 256 * 256, ocnd2: bool = F
         def call():
             return lib.ice3_ice_adjust(ctx, _lib.PREC_FP64, ncol, NLEV, ncol, DT, ctypes.byref(cin),
                                        ctypes.byref(cout), sp)
-    for _ in range(args.warmup):
-        _lib.check(call(), "ice_adjust warmup")
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        call()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
+    ms, _ = timed_steps(call, stream, args.steps, args.warmup)
     gbs = (bpp * ncol * NLEV + bpc * ncol) / (ms * 1e-3) / 1e9
     return {"columns_per_s": ncol / (ms * 1e-3), "ms_per_step": ms, "columns": ncol,
             "kernel": kernel, "bytes_per_point": bpp, "achieved_GBs": gbs,

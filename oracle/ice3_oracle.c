@@ -1260,7 +1260,7 @@ static void sedimentation_column(int ncol, int nlev, int i, double dt, const dou
   }
 }
 
-/* RAIN_ICE (micro/rain_ice.F90:7-803) for KRR=6 with the commented-out
+/* RAIN_ICE (micro/rain_ice.F90:7-830) for KRR=6 with the commented-out
  * sedimentation (:336-348), total tendencies (:649-684) and negativity
  * correction (:712-714) reinstated. */
 int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
@@ -1281,7 +1281,7 @@ int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE
       double exn = in[RI_EXN][j], rhodref = in[RI_RHODREF][j], pabs = in[RI_PABS][j];
       double tht = in[RI_THT][j], rvt = in[RI_RVT][j], rct = in[RI_RCT][j], rrt = in[RI_RRT][j];
       double rit = in[RI_RIT][j], rst = in[RI_RST][j], rgt = in[RI_RGT][j];
-      /* 1. factors and mask (rain_ice.F90:225-273) */
+      /* 1. factors and mask (rain_ice.F90:334-384) */
       double zrice = rit + rst + rgt;
       double zdevide = P.XCPD + P.XCPV * rvt + P.XCL * (rct + rrt) + P.XCI * zrice;
       double zt = tht * exn;

@@ -7,18 +7,30 @@
 # bridge (bridge/phyex_bridge.F90 does not compile in FP64) and the shallow
 # convection scheme, which is not on the ice_adjust/rain_ice path.
 #
-# No -march=native: the x86-64 baseline has no FMA, so flang cannot contract
-# a*b+c and the reference arithmetic stays the plain IEEE sequence.
+# Two variants from the same sources:
+# * VARIANT=parity (default, oracle/_ref/): -O2 for the x86-64 baseline, which has no
+#   FMA, so flang cannot contract a*b+c and the reference arithmetic stays the plain IEEE
+#   sequence.  It writes the golden vectors and pins the C restatement.
+# * VARIANT=fast (oracle/_ref/fast/): -O3 for x86-64-v3 (AVX2 + FMA), timing only: the
+#   optimised CPU baseline.  The library is built here and run on the GPU box's host, so
+#   it targets an instruction set both hosts have rather than -march=native of this one.
 #
 # Usage (this container only; the GPU box receives the built library):
-#   make -f oracle/ref_build.mk -j8
+#   make -f oracle/ref_build.mk [VARIANT=fast]
 
 REF      ?= /root/reference/PHYEX-IAL_CY50T1
+VARIANT  ?= parity
+ifeq ($(VARIANT),fast)
+OUT      := oracle/_ref/fast
+OPT      := -O3 -march=x86-64-v3
+else
 OUT      := oracle/_ref
+OPT      := -O2
+endif
 OBJ      := $(OUT)/obj
 MOD      := $(OUT)/mod
 FC       := flang
-FFLAGS   := -O2 -fPIC -cpp -fdefault-real-8 -fdefault-double-8 -module-dir $(MOD) -I$(MOD) -I$(REF)/stubs/include -I$(REF)/micro
+FFLAGS   := $(OPT) -fPIC -cpp -fdefault-real-8 -fdefault-double-8 -module-dir $(MOD) -I$(MOD) -I$(REF)/stubs/include -I$(REF)/micro
 
 STUB := stubs/parkind1.F90 stubs/ec_parkind.F90 stubs/yomhook.F90 stubs/yomlun.F90 stubs/abor1.F90 \
         stubs/fmlook_ll.F90 stubs/posnam.F90

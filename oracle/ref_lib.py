@@ -15,12 +15,24 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_ref", "libphyex_ref.so")
+# The optimised build of the same sources (-O3, x86-64-v3 with FMA contraction): timing only,
+# never a checker (its arithmetic is not the plain IEEE sequence of the parity build).
+FAST_LIB_PATH = os.path.join(_HERE, "_ref", "fast", "libphyex_ref.so")
 _lib = None
 _init_state = None
 
 
-def available() -> bool:
-    return os.path.exists(LIB_PATH)
+def available(fast: bool = False) -> bool:
+    return os.path.exists(FAST_LIB_PATH if fast else LIB_PATH)
+
+
+def use_fast_build():
+    """Load the optimised timing build instead of the parity build (before the first call;
+    for the CPU baseline of bench.py)."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the reference library is already loaded")
+    LIB_PATH = FAST_LIB_PATH
 
 
 def _load():
