@@ -503,6 +503,13 @@ __global__ void __launch_bounds__(256) k_features(MicroArgs m, unsigned int npts
 
 }  // namespace ice3
 
+// Scratch of the allocation-free surrogate step (ice3_adjust_rain_ice_surrogate_noalloc):
+// the same layout as the caller's work area of ice3_adjust_rain_ice_surrogate.  One copy per
+// GPU, held until the process ends.  Host code never uses the name: a kernel argument made
+// from it would be a host address; ice3_ctx_create takes the device address with
+// hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[ICE3_SURROGATE_SCRATCH_BYTES];
+
 // ------------------------------------------------------------------ C ABI
 
 struct ice3_ctx {
@@ -513,6 +520,7 @@ struct ice3_ctx {
   double h_tab[ICE3_T_TOTAL];
   double *d_tiwmx;    // TIWMX tables of the OCND2 branch: ESTABW then ESTABI
   unsigned long long *d_capped;  // points that hit the sub-step cap since the last status query
+  void *d_scratch;    // device address of this device's copy of scratch
   int max_calls;      // sub-step cap per point
   int timing;         // record ev[] around the stages of each call
   hipEvent_t ev[7];   // before adjust, after adjust, after sedimentation, then after the
@@ -553,7 +561,9 @@ extern "C" int ice3_ctx_create(ice3_ctx **out) {
       hipMemcpy(c->d_tiwmx, tiwmx, sizeof(double) * 2 * ICE3_TIWMX_N, hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMalloc((void **)&c->d_capped, sizeof(unsigned long long)) != hipSuccess ||
-      hipMemset(c->d_capped, 0, sizeof(unsigned long long)) != hipSuccess) {
+      hipMemset(c->d_capped, 0, sizeof(unsigned long long)) != hipSuccess ||
+      hipGetSymbolAddress(&c->d_scratch, HIP_SYMBOL(scratch)) != hipSuccess ||
+      c->d_scratch == nullptr) {
     if (c->d_tab) (void)hipFree(c->d_tab);
     if (c->d_tiwmx) (void)hipFree(c->d_tiwmx);
     if (c->d_capped) (void)hipFree(c->d_capped);
@@ -635,6 +645,9 @@ extern "C" const char *ice3_strerror(int code) {
     case ICE3_EDEVICE: return "HIP runtime error";
     case ICE3_ESUBSTEP: return "a point reached the sub-step cap of the ICE4 time splitting";
     case ICE3_ECONFIG: return "constants compiled into the kernels differ from the host builder (rebuild)";
+    case ICE3_ETOOLARGE:
+      return "the problem needs more scratch than the library's static area "
+             "(ICE3_SURROGATE_SCRATCH_BYTES); use ice3_adjust_rain_ice_surrogate with a work area";
     default: return "unknown error";
   }
 }
@@ -925,22 +938,34 @@ extern "C" size_t ice3_surrogate_work_bytes(int ncol, int nlev) {
   return sg_work_layout((size_t)ncol * nlev).total;
 }
 
-extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev, int ld,
-                                              double dt, const ice3_pipeline_in *in,
-                                              const ice3_pipeline_out *out,
-                                              const ice3_surrogate *sg, void *work,
-                                              unsigned long long *sg_counters,
-                                              const ice3_stats *stats, void *stream) {
+// Argument checks of the surrogate step (both entry points); ICE3_OK with *empty set when
+// there is nothing to do.
+static int surrogate_check(const ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                           const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                           const ice3_surrogate *sg, const unsigned long long *sg_counters,
+                           bool *empty) {
+  *empty = false;
   if (!ctx || !in || !out || !sg || !sg_counters) return ICE3_EINVAL;
-  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;  // the work area is then 0 bytes
-  if (!work || bad_shape(ncol, nlev, ld, dt)) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) {  // the work area is then 0 bytes
+    *empty = true;
+    return ICE3_OK;
+  }
+  if (bad_shape(ncol, nlev, ld, dt)) return ICE3_EINVAL;
   if (nlev > 65535 || (double)nlev * ncol >= 4.0e9 || (double)nlev * ld >= 4.0e9)
     return ICE3_EINVAL;
-  if (((uintptr_t)work & 255) != 0) return ICE3_EINVAL;
   if (!sg->weights_packed || !sg->params || !sg->norm || !(sg->budget_tol >= 0.f) ||
       !(sg->neg_tol >= 0.f))
     return ICE3_EINVAL;
   if (bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
+  return ICE3_OK;
+}
+
+// The surrogate step on a checked problem, with the scratch at work (sg_work_layout).
+static int surrogate_step(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                          const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                          const ice3_surrogate *sg, void *work,
+                          unsigned long long *sg_counters, const ice3_stats *stats,
+                          void *stream) {
   AdjustArgs aa;
   SedimArgs sa;
   MicroArgs m;
@@ -988,6 +1013,33 @@ extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev,
   HIPCHK(hipGetLastError());
   MARK(6);
   return ICE3_OK;
+}
+
+extern "C" int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev, int ld,
+                                              double dt, const ice3_pipeline_in *in,
+                                              const ice3_pipeline_out *out,
+                                              const ice3_surrogate *sg, void *work,
+                                              unsigned long long *sg_counters,
+                                              const ice3_stats *stats, void *stream) {
+  bool empty;
+  const int rc = surrogate_check(ctx, ncol, nlev, ld, dt, in, out, sg, sg_counters, &empty);
+  if (rc != ICE3_OK || empty) return rc;
+  if (!work || ((uintptr_t)work & 255) != 0) return ICE3_EINVAL;
+  return surrogate_step(ctx, ncol, nlev, ld, dt, in, out, sg, work, sg_counters, stats, stream);
+}
+
+extern "C" int ice3_adjust_rain_ice_surrogate_noalloc(ice3_ctx *ctx, int ncol, int nlev, int ld,
+                                                      double dt, const ice3_pipeline_in *in,
+                                                      const ice3_pipeline_out *out,
+                                                      const ice3_surrogate *sg,
+                                                      unsigned long long *sg_counters,
+                                                      const ice3_stats *stats, void *stream) {
+  bool empty;
+  const int rc = surrogate_check(ctx, ncol, nlev, ld, dt, in, out, sg, sg_counters, &empty);
+  if (rc != ICE3_OK || empty) return rc;
+  if (sg_work_layout((size_t)ncol * nlev).total > ICE3_SURROGATE_SCRATCH_BYTES) return ICE3_ETOOLARGE;
+  return surrogate_step(ctx, ncol, nlev, ld, dt, in, out, sg, ctx->d_scratch, sg_counters, stats,
+                        stream);
 }
 
 extern "C" int ice3_surrogate_features(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,

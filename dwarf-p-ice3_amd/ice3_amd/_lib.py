@@ -12,7 +12,8 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libice3_hip.so")
 
-ICE3_OK, ICE3_EINVAL, ICE3_EDEVICE, ICE3_ESUBSTEP = 0, -1, -2, -3
+ICE3_OK, ICE3_EINVAL, ICE3_EDEVICE, ICE3_ESUBSTEP, ICE3_ECONFIG, ICE3_ETOOLARGE = 0, -1, -2, -3, -4, -5
+SURROGATE_SCRATCH_BYTES = 64 << 20
 ICE3_MAX_CHECK_FIELDS = 32
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -75,6 +76,7 @@ EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_co
            "ice3_ice_adjust", "ice3_ice_adjust_ocnd2", "ice3_rain_ice", "ice3_adjust_rain_ice",
            "ice3_stage_timing",
            "ice3_stage_times", "ice3_adjust_rain_ice_stages", "ice3_adjust_rain_ice_surrogate",
+           "ice3_adjust_rain_ice_surrogate_noalloc",
            "ice3_surrogate_features", "ice3_surrogate_work_bytes", "ice3_mlp_sizes", "ice3_mlp_forward", "ice3_mlp_backward",
            "ice3_mlp_adam", "ice3_count_nonfinite")
 
@@ -115,6 +117,9 @@ def load():
         lib.ice3_adjust_rain_ice_surrogate.argtypes = [
             _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(PipeIn),
             ctypes.POINTER(PipeOut), ctypes.POINTER(Surrogate), _P, _P, ctypes.POINTER(Stats), _P]
+        lib.ice3_adjust_rain_ice_surrogate_noalloc.argtypes = [
+            _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(PipeIn),
+            ctypes.POINTER(PipeOut), ctypes.POINTER(Surrogate), _P, ctypes.POINTER(Stats), _P]
         lib.ice3_surrogate_work_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
         lib.ice3_surrogate_work_bytes.restype = ctypes.c_size_t
         lib.ice3_surrogate_features.argtypes = [

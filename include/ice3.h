@@ -37,6 +37,7 @@ extern "C" {
 #define ICE3_EDEVICE (-2)    /* HIP runtime error (launch, allocation, copy) */
 #define ICE3_ESUBSTEP (-3)   /* a point hit the sub-step cap (ice3_ctx_status, ice3_stats) */
 #define ICE3_ECONFIG (-4)    /* kernels built with constants other than the host builder's */
+#define ICE3_ETOOLARGE (-5)  /* problem larger than the static scratch of a _noalloc entry point */
 
 /* arithmetic of the microphysics kernels */
 #define ICE3_PREC_FP64 0     /* FP64 everywhere: parity with the FP64 reference */
@@ -225,6 +226,22 @@ int ice3_adjust_rain_ice_surrogate(ice3_ctx *ctx, int ncol, int nlev, int ld, do
                                    const ice3_surrogate *sg, void *work,
                                    unsigned long long *sg_counters, const ice3_stats *stats,
                                    void *stream);
+
+/* The same step with no work argument and no allocation (no counterpart in the reference:
+ * for callers that cannot allocate device memory, such as code under HIP graph capture or
+ * serving loops that forbid it).  Its scratch is a static device array of
+ * ICE3_SURROGATE_SCRATCH_BYTES per GPU (64 MiB), in place from the first ice3_ctx_create
+ * on that GPU until the process ends; a problem whose ice3_surrogate_work_bytes(ncol, nlev)
+ * exceeds it (about 650,000 points: 7,000 columns of 90 levels) returns ICE3_ETOOLARGE.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other contexts, other threads) race and must be ordered by the caller.  Nothing in the
+ * scratch needs to start at zero, and no call leaves state there that a later call reads. */
+#define ICE3_SURROGATE_SCRATCH_BYTES ((size_t)64 << 20)
+int ice3_adjust_rain_ice_surrogate_noalloc(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                           const ice3_pipeline_in *in,
+                                           const ice3_pipeline_out *out, const ice3_surrogate *sg,
+                                           unsigned long long *sg_counters,
+                                           const ice3_stats *stats, void *stream);
 
 /* Surrogate features of the state between the sedimentation and microphysics stages
  * (in/out as after ice3_adjust_rain_ice_stages(ADJUST | SEDIMENTATION)).  raw: [npts][18]
