@@ -164,7 +164,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    t_start = time.time()
+
+    def progress(what):
+        print(f"bench: {what} ({time.time() - t_start:.0f} s)", file=sys.stderr, flush=True)
+
     cpu = None if args.no_cpu else cpu_baseline()
+    progress("CPU baseline done" if cpu else "CPU baseline skipped")
 
     import numpy as np
     import torch
@@ -225,8 +231,10 @@ def main():
         return dict(zip(("ice_adjust", "sedimentation", "microphysics"), [x / steps for x in acc]))
 
     out64 = outputs()
+    progress("inputs resident on the GPU")
     ms, ms_mean = run(_lib.PREC_FP64, out64, args.steps, args.warmup)
     stages = stage_times(_lib.PREC_FP64, out64, args.steps)
+    progress(f"FP64 step timed: {ms:.3f} ms median")
     npts = ncol * NLEV
     alg_bytes = BYTES_PER_POINT * npts + BYTES_PER_COLUMN * ncol
     dom = max(stages, key=stages.get)
@@ -300,6 +308,7 @@ def main():
             scale = out64[k].abs().max().item()
             err[k] = {"max_abs": d.max().item(), "rms_abs": d.pow(2).mean().sqrt().item(),
                       "max_rel_to_field_max": d.max().item() / scale if scale > 0 else 0.0}
+        progress(f"mixed step timed: {ms32:.3f} ms median")
         res["variants"] = {"mixed": {"columns_per_s": ncol / (ms32 * 1e-3), "ms_per_step": ms32,
                                      "speedup_vs_fp64": ms / ms32, "stages_ms": stages32,
                                      "water_budget_max_rel_err": budget_err(out32),
@@ -309,6 +318,7 @@ def main():
         if not args.no_mixed:
             del out32
         res.setdefault("variants", {})["surrogate_1024x1024"] = surrogate_variant(lib, ctx, dev, g, args)
+        progress("surrogate line done")
     del g
     if not args.no_adjust:
         res.setdefault("variants", {})["ice_adjust_256x256"] = ice_adjust_alone(lib, ctx, dev, args)
