@@ -63,11 +63,11 @@ __global__ void __launch_bounds__(64) k_adjust_sedim(AdjustArgs aa, SedimArgs a)
   adjust_sedim_column<double>(aa, a, i);
 }
 
-template <class R>
+template <class R, bool ZZZ = false>
 __global__ void __launch_bounds__(256) k_sedim(SedimArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.ncol) return;
-  sedim_column<R>(a, i);
+  sedim_column<R, ZZZ>(a, i);
 }
 
 // Sum of v over the 64 lanes of the wavefront (every lane gets the total).
@@ -184,7 +184,7 @@ __device__ __forceinline__ unsigned int park_load(const MicroPark<R> &pk, int e,
   return pk.q[e];
 }
 
-template <class R>
+template <class R, bool OCND2 = false>
 __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
   extern __shared__ __align__(16) unsigned char micro_lds[];
   R *tab = reinterpret_cast<R *>(micro_lds);
@@ -219,17 +219,20 @@ __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
         double rvheni = 0., evap3d = 0.;
         int st = 0;
         calls = 0;
-        if (micro_mask(t)) {
+        if (micro_mask<OCND2>(t, a, j)) {
           ++nmicro;
           step_point(dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j], s, t, p);
 #pragma unroll
           for (int f = 0; f < 7; ++f) vart[f] = t[f];
           time = 0.;
           const PointConst<R> pc = point_const<R>(p.rhodref);
-          st = step_call<R, false>(dt, p, pc, vart, cit, time, S, calls, DEFAULT_MAX_CALLS, tab);
+          MicroOcnd2 xo;
+          if constexpr (OCND2) xo = ocnd2_point(a, j);
+          st = step_call<R, false, OCND2>(dt, p, pc, vart, cit, time, S, calls, DEFAULT_MAX_CALLS,
+                                          tab, &xo);
           if (st == 0) evap3d = (double)S.rrevav;
         } else {
-          micro_nucleation<R>(a, j, t, cit, rvheni);
+          micro_nucleation<R, OCND2>(a, j, t, cit, rvheni);
 #pragma unroll
           for (int f = 0; f < 7; ++f) vart[f] = t[f];
         }
@@ -262,7 +265,8 @@ __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
       }
       parked -= need < parked ? need : parked;
       if (active) {
-        const int st = step_call<R, true>(dt, p, pc_soft, vart, cit, time, S, calls, DEFAULT_MAX_CALLS, tab);
+        const int st = step_call<R, true, OCND2>(dt, p, pc_soft, vart, cit, time, S, calls,
+                                                 DEFAULT_MAX_CALLS, tab);
         if (st <= 0) {
           ncalls += calls;
           ncap += st < 0;
@@ -300,13 +304,13 @@ __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
 // run time (k_micro has the default cap compiled in: the register allocator fails on
 // k_micro<double> with a run-time bound).  Launched when a context sets another cap; the
 // per-point arithmetic is k_micro's, so the results are the same for the same cap.
-template <class R>
+template <class R, bool OCND2 = false>
 __global__ void __launch_bounds__(256) k_micro_points(MicroArgs a) {
   const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long ncalls = 0, ncap = 0, nmicro = 0;
   if (q < (unsigned int)a.nlev * (unsigned int)a.ncol) {
     const unsigned int k = q / (unsigned int)a.ncol;
-    const int c = micro_at<R>(a, (size_t)k * a.ld + (q - k * (unsigned int)a.ncol), a.tab);
+    const int c = micro_at<R, OCND2>(a, (size_t)k * a.ld + (q - k * (unsigned int)a.ncol), a.tab);
     ncalls = c < 0 ? -c : c;
     ncap = c < 0;
     nmicro = c != 0;
@@ -518,7 +522,7 @@ struct ice3_ctx {
   ice3_params params;
   double *d_tab;
   double h_tab[ICE3_T_TOTAL];
-  double *d_tiwmx;    // TIWMX tables of the OCND2 branch: ESTABW then ESTABI
+  double *d_tiwmx;    // TIWMX tables of the OCND2 branch (ice3_tiwmx_tables order)
   unsigned long long *d_capped;  // points that hit the sub-step cap since the last status query
   void *d_scratch;    // device address of this device's copy of scratch
   int max_calls;      // sub-step cap per point
@@ -545,20 +549,21 @@ extern "C" int ice3_ctx_create(ice3_ctx **out) {
     free(c);
     return ICE3_ECONFIG;
   }
-  double *tiwmx = (double *)malloc(sizeof(double) * 2 * ICE3_TIWMX_N);
+  constexpr size_t TIWMX_BYTES = sizeof(double) * ICE3_TW_NTAB * ICE3_TIWMX_N;
+  double *tiwmx = (double *)malloc(TIWMX_BYTES);
   if (!tiwmx) {
     free(c);
     return ICE3_EDEVICE;
   }
-  ice3_build_tiwmx(&c->params, tiwmx, tiwmx + ICE3_TIWMX_N);
+  ice3_build_tiwmx(&c->params, tiwmx);
   if (hipGetDevice(&c->device) != hipSuccess ||
       hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
           hipSuccess || c->ncu <= 0 ||
       hipMalloc((void **)&c->d_tab, sizeof(double) * ICE3_T_TOTAL) != hipSuccess ||
       hipMemcpy(c->d_tab, c->h_tab, sizeof(double) * ICE3_T_TOTAL, hipMemcpyHostToDevice) !=
           hipSuccess ||
-      hipMalloc((void **)&c->d_tiwmx, sizeof(double) * 2 * ICE3_TIWMX_N) != hipSuccess ||
-      hipMemcpy(c->d_tiwmx, tiwmx, sizeof(double) * 2 * ICE3_TIWMX_N, hipMemcpyHostToDevice) !=
+      hipMalloc((void **)&c->d_tiwmx, TIWMX_BYTES) != hipSuccess ||
+      hipMemcpy(c->d_tiwmx, tiwmx, TIWMX_BYTES, hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMalloc((void **)&c->d_capped, sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(c->d_capped, 0, sizeof(unsigned long long)) != hipSuccess ||
@@ -678,6 +683,7 @@ extern "C" long ice3_dump_constants(const ice3_ctx *ctx, char *buf, long buflen)
   D(XEPSILO); D(XALPW); D(XBETAW); D(XGAMW); D(XALPI); D(XBETAI); D(XGAMI); D(XESTT); D(XP00);
   D(XRHOLW); D(XPI); D(XG); D(XMNH_TINY);
   put(s, "XRTMIN", P.XRTMIN + 1, 6);
+  put(s, "XFRMIN", P.XFRMIN + 1, 40);
   D(XCEXVT); D(XLBR); D(XLBEXR); D(XLBS); D(XLBEXS); D(XLBG); D(XLBEXG); D(XLBI); D(XLBEXI);
   D(XLBDAS_MAX); D(XDI); D(XBS); D(XCXS); D(XCXG); D(XDG); D(XCONC_LAND);
   D(XCRIAUTC); D(XCRIAUTI); D(XACRIAUTI); D(XBCRIAUTI); D(XT0CRIAUTI); D(XNU10); D(XALPHA1);
@@ -821,6 +827,19 @@ static hipError_t launch_micro(const ice3_ctx *ctx, int prec, const MicroArgs &m
   return hipGetLastError();
 }
 
+// The OCND2 microphysics (FP64): one workgroup per compute unit, or the point kernel when the
+// context sets another sub-step cap.
+static hipError_t launch_micro_ocnd2(const ice3_ctx *ctx, const MicroArgs &m, hipStream_t s) {
+  if (m.max_calls != DEFAULT_MAX_CALLS) {
+    const unsigned int npts = (unsigned int)m.nlev * (unsigned int)m.ncol;
+    hipLaunchKernelGGL((k_micro_points<double, true>), dim3((npts + 255) / 256), dim3(256), 0, s, m);
+  } else {
+    hipLaunchKernelGGL((k_micro<double, true>), dim3(ctx->ncu), dim3(64 * micro_waves<double>()),
+                       micro_lds_bytes<double>(), s, m);
+  }
+  return hipGetLastError();
+}
+
 // Launch the sedimentation then the microphysics stage on one stream.
 static int launch_rain_ice(ice3_ctx *ctx, int prec, const SedimArgs &sa, const MicroArgs &ma,
                            void *stream) {
@@ -856,6 +875,37 @@ extern "C" int ice3_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld
   hipStream_t s = (hipStream_t)stream;
   MARK(0); MARK(1);
   return launch_rain_ice(ctx, prec, sa, m, stream);
+}
+
+extern "C" int ice3_rain_ice_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                   const ice3_rain_ice_in *in, const ice3_rain_ocnd2_in *in2,
+                                   const ice3_rain_ice_inout *io, const ice3_rain_ice_out *out,
+                                   const ice3_stats *stats, void *stream) {
+  if (!ctx || !in || !in2 || !io || !out) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev > 65535) return ICE3_EINVAL;
+  NN(in->exn); NN(in->dzz); NN(in->rhodref); NN(in->exn_ref); NN(in->pabs); NN(in->cldfr);
+  NN(in->sigs); NN(in->tht); NN(in->rvt); NN(in->rct); NN(in->rrt); NN(in->rit); NN(in->rst);
+  NN(in->rgt); NN(in2->icldfr); NN(in2->ssio); NN(in2->ssiu); NN(in2->ifr);
+  NN(io->ths); NN(io->rvs); NN(io->rcs); NN(io->rrs); NN(io->ris); NN(io->rss); NN(io->rgs);
+  NN(io->ci);
+  NN(out->evap3d); NN(out->rainfr); NN(out->inprc); NN(out->inprr); NN(out->inpri);
+  NN(out->inprs); NN(out->inprg);
+  SedimArgs sa;
+  MicroArgs m;
+  rain_ice_args(ncol, nlev, ld, dt, ctx->d_tab, in, io, out, stats, sa, m);
+  m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
+  m.icldfr = in2->icldfr; m.ssio = in2->ssio; m.ssiu = in2->ssiu; m.ifr = in2->ifr;
+  m.tiwmx = ctx->d_tiwmx;
+  sa.zzz = out->rainfr;  // the model level height, read by the microphysics, then overwritten
+  hipStream_t s = (hipStream_t)stream;
+  MARK(0); MARK(1);
+  hipLaunchKernelGGL((k_sedim<double, true>), dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
+  HIPCHK(hipGetLastError());
+  MARK(2);
+  HIPCHK(launch_micro_ocnd2(ctx, m, s));
+  MARK(3); MARK(4); MARK(5); MARK(6);
+  return ICE3_OK;
 }
 
 static bool bad_pipeline_ptrs(const ice3_pipeline_in *in, const ice3_pipeline_out *out) {
@@ -923,6 +973,39 @@ extern "C" int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev,
                                      ICE3_STAGE_ADJUST | ICE3_STAGE_SEDIMENTATION |
                                          ICE3_STAGE_MICROPHYSICS,
                                      stream);
+}
+
+extern "C" int ice3_adjust_rain_ice_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                          const ice3_pipeline_in *in, const ice3_ocnd2_in *in2,
+                                          const ice3_pipeline_out *out,
+                                          const ice3_ocnd2_out *out2, const ice3_stats *stats,
+                                          void *stream) {
+  if (!ctx || !in || !in2 || !out || !out2) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev < 2 || nlev > 65535) return ICE3_EINVAL;
+  if (bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
+  NN(in2->zz); NN(in2->ice_cld_wgt);
+  NN(out2->icldfr); NN(out2->wcldfr); NN(out2->ssio); NN(out2->ssiu); NN(out2->ifr);
+  AdjustArgs aa;
+  SedimArgs sa;
+  MicroArgs m;
+  pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, stats, aa, sa, m);
+  m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
+  m.icldfr = out2->icldfr; m.ssio = out2->ssio; m.ssiu = out2->ssiu; m.ifr = out2->ifr;
+  m.tiwmx = ctx->d_tiwmx;
+  sa.zzz = out->rainfr;  // the model level height, read by the microphysics, then overwritten
+  const Ocnd2Args x{*in2, *out2, ctx->d_tiwmx};
+  hipStream_t s = (hipStream_t)stream;
+  MARK(0);
+  hipLaunchKernelGGL(k_ice_adjust_ocnd2, dim3((ncol + 255) / 256, nlev), dim3(256), 0, s, aa, x);
+  HIPCHK(hipGetLastError());
+  MARK(1);
+  hipLaunchKernelGGL((k_sedim<double, true>), dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
+  HIPCHK(hipGetLastError());
+  MARK(2);
+  HIPCHK(launch_micro_ocnd2(ctx, m, s));
+  MARK(3); MARK(4); MARK(5); MARK(6);
+  return ICE3_OK;
 }
 
 static int n_cus(int device) {

@@ -147,6 +147,13 @@ void ice3_build_params(ice3_params *P, double *tables) {
   P->XLBDAS_MAX = 100000.0;
   if (XCCS > 0. && XCXS > 0.) P->XLBDAS_MAX = std::pow(ZCONC_MAX / XCCS, 1. / XCXS);
   P->XCONC_LAND = 3E8;
+  // PARAM_ICE XFRMIN_NAM of the pinned AROME configuration, copied by INI_RAIN_ICE
+  // (micro/mode_ini_rain_ice.F90:203): used by the OCND2 branch of RAIN_ICE
+  for (int j = 0; j <= 40; ++j) P->XFRMIN[j] = 0.;
+  P->XFRMIN[7] = 1.; P->XFRMIN[8] = 1.; P->XFRMIN[9] = 1.; P->XFRMIN[10] = 10.; P->XFRMIN[11] = 1.;
+  P->XFRMIN[13] = 1.0E-15; P->XFRMIN[14] = 120.; P->XFRMIN[15] = 1.0E-4; P->XFRMIN[21] = 1.;
+  P->XFRMIN[22] = 1.; P->XFRMIN[23] = 0.5; P->XFRMIN[24] = 1.5; P->XFRMIN[25] = 30.;
+  P->XFRMIN[30] = 1.; P->XFRMIN[39] = 0.25; P->XFRMIN[40] = 0.15;
   P->XCEXVT = 0.4;
   const double XCEXVT = P->XCEXVT;
   double ZE = (50. / 100.) * std::exp(P->XALPW - P->XBETAW / 293.15 - P->XGAMW * std::log(293.15));
@@ -300,16 +307,29 @@ void ice3_build_params(ice3_params *P, double *tables) {
   P->LN_10 = std::log(10.);
 }
 
-// INI_TIWMX (micro/mode_ini_tiwmx.F90:14-40) for the two tables the OCND2 branch reads, with
-// ESATW / ESATI of micro/mode_tiwmx_fun.F90:8-14 at T = jk / XNDEGR; above 273.15 K the ice
-// table holds the liquid-water pressure.
-void ice3_build_tiwmx(const ice3_params *P, double *esw, double *esi) {
+// INI_TIWMX (micro/mode_ini_tiwmx.F90:14-49) for the tables the OCND2 branch reads, with the
+// functions of micro/mode_tiwmx_fun.F90:36-106 at T = jk / XNDEGR.  Above 273.15 K the ice
+// table holds the liquid-water pressure and AM3 is 0.  ESATI (the function) is the pressure
+// over ice below XTT and over water from XTT up; x**2 and x**3 are products, as the
+// reference's integer powers.
+void ice3_build_tiwmx(const ice3_params *P, double *tab) {
+  double *esw = tab + ICE3_TW_ESW * ICE3_TIWMX_N, *esi = tab + ICE3_TW_ESI * ICE3_TIWMX_N;
+  double *a2 = tab + ICE3_TW_A2 * ICE3_TIWMX_N, *bb3 = tab + ICE3_TW_BB3 * ICE3_TIWMX_N;
+  double *am3 = tab + ICE3_TW_AM3 * ICE3_TIWMX_N, *redin = tab + ICE3_TW_REDIN * ICE3_TIWMX_N;
   for (int jk = ICE3_TIWMX_START; jk <= ICE3_TIWMX_STOP; ++jk) {
+    const int e = jk - ICE3_TIWMX_START;
     const double t = (double)jk / 100.;
     const double w = std::exp(P->XALPW - P->XBETAW / t - P->XGAMW * std::log(t));
     const double i = std::exp(P->XALPI - P->XBETAI / t - P->XGAMI * std::log(t));
     const double sgn = t - P->XTT >= 0. ? 0.5 : -0.5;  // SIGN(0.5, TT - XTT)
-    esw[jk - ICE3_TIWMX_START] = w;
-    esi[jk - ICE3_TIWMX_START] = t > 273.15 ? w : (0.5 + sgn) * w - (sgn - 0.5) * i;
+    const double esati = (0.5 + sgn) * w - (sgn - 0.5) * i;
+    esw[e] = w;
+    esi[e] = t > 273.15 ? w : esati;
+    const double l = P->XLSTT + (P->XCPV - P->XCI) * (t - P->XTT);
+    a2[e] = l * l / (2.38E-2 + 0.0071E-2 * (t - P->XTT)) / (t * t * P->XRV);
+    bb3[e] = P->XRV / (0.211E-4 * std::pow(t / P->XTT, 1.94) * P->XP00) * t / esati;
+    am3[e] = t > 273.15 ? 0. : 1000. * std::exp(12.96 * (w / esati - 1.) - 0.639);
+    const double zzt = std::fmax(0., std::fmin(1., (P->XTT - t) / 20.));
+    redin[e] = 1. - (1. - zzt) / std::pow(zzt * zzt * zzt + (1. - zzt) * (1. - zzt) * (1. - zzt), .333);
   }
 }

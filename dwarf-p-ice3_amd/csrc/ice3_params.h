@@ -26,6 +26,7 @@ struct ice3_params {
       XLBSDRYG1, XLBSDRYG2, XLBSDRYG3, XFRDRYG, XLBRDRYG1, XLBRDRYG2, XLBRDRYG3, XDRYINTP1R,
       XDRYINTP2R, XDRYINTP1S, XDRYINTP2S, XDRYINTP1G, XDRYINTP2G, XFSEDR, XEXSEDR, XFSEDS,
       XEXSEDS, XFSEDG, XEXSEDG, XFSEDI, XEXCSEDI;
+  double XFRMIN[41];  // 1-based, [1..40]: the PARAM_ICE XFRMIN_NAM tuning values (OCND2 branch)
   int NGAMINC, NACCLBDAS, NACCLBDAR, NDRYLBDAR, NDRYLBDAS, NDRYLBDAG;
   // natural logarithms of constants, for powers evaluated as exp(c*ln x) on the device
   double LN_XLBR, LN_XLBS, LN_XLBG, LN_XLBI, LN_XLBDAS_MAX, LN_XTT, LN_1E8, LN_10;
@@ -47,10 +48,20 @@ enum ice3_table_offsets {
 // Builds every constant and the table block (tables must hold ICE3_T_TOTAL doubles).
 void ice3_build_params(ice3_params *p, double *tables);
 
-// TIWMX saturation-pressure tables of the OCND2 branch (micro/modd_tiwmx.F90: XNDEGR = 100,
-// NSTART = 13200, NSTOP = 37316): entry jk - ICE3_TIWMX_START of ESTABW / ESTABI holds the
-// pressure at jk / 100 K, computed as INI_TIWMX does (micro/mode_ini_tiwmx.F90:14-40).
+// TIWMX tables of the OCND2 branch (micro/modd_tiwmx.F90: XNDEGR = 100, NSTART = 13200,
+// NSTOP = 37316): entry jk - ICE3_TIWMX_START of each table holds its function at jk / 100 K,
+// computed as INI_TIWMX does (micro/mode_ini_tiwmx.F90:14-49).  The tables follow one another
+// in one block, ICE3_TIWMX_N doubles each, in the order of ice3_tiwmx_tables.
 enum { ICE3_TIWMX_START = 13200, ICE3_TIWMX_STOP = 37316,
        ICE3_TIWMX_N = ICE3_TIWMX_STOP - ICE3_TIWMX_START + 1 };
-// esw and esi must hold ICE3_TIWMX_N doubles each; p are the constants ice3_build_params made.
-void ice3_build_tiwmx(const ice3_params *p, double *esw, double *esi);
+enum ice3_tiwmx_tables {
+  ICE3_TW_ESW = 0,  // ESTABW: saturation pressure over water
+  ICE3_TW_ESI,      // ESTABI: over ice below 273.15 K, over water above
+  ICE3_TW_A2,       // A2TAB: AA2, the heat-conduction term of the ice growth function
+  ICE3_TW_BB3,      // BB3TAB: BB3, the vapour-diffusion term (times pressure)
+  ICE3_TW_AM3,      // AM3TAB: Meyers ice-nucleus concentration, 0 above 273.15 K
+  ICE3_TW_REDIN,    // REDINTAB: reduction of ice nuclei between 0 and -20 C
+  ICE3_TW_NTAB
+};
+// tab must hold ICE3_TW_NTAB * ICE3_TIWMX_N doubles; p are the constants ice3_build_params made.
+void ice3_build_tiwmx(const ice3_params *p, double *tab);

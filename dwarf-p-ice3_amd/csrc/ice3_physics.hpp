@@ -232,8 +232,8 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   double pv, piv, frac, esw_t = 0.;
   if constexpr (OCND2) {
     const int it = tiwmx_index(zt);
-    esw_t = x->tiwmx[it];
-    const double esi_t = x->tiwmx[ICE3_TIWMX_N + it];
+    esw_t = x->tiwmx[ICE3_TW_ESW * ICE3_TIWMX_N + it];
+    const double esi_t = x->tiwmx[ICE3_TW_ESI * ICE3_TIWMX_N + it];
     icecloud(pabs, x->z, x->dz, zt, rv, esw_t, esi_t, x->icldfr, x->ssio, x->ssiu);
     pv = dmin(esw_t, .99 * pabs);
     piv = dmin(esi_t, .99 * pabs);
@@ -367,14 +367,38 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
 
 // ---------------------------------------------------------------- rain_ice
 
-// ICE4_NUCLEATION (micro/ice4_nucleation.func.h:68-143), .NOT.LOCND2, LFEEDBACKT
-template <class R>
+// Per-point inputs of RAIN_ICE's OCND2 branch: the TIWMX tables, ICE_ADJUST's ice cloud
+// fraction and super-/sub-saturation over ice (PICLDFR, PSSIO, PSSIU, PIFR) and RAIN_ICE's
+// model level height ZZZZ (micro/rain_ice.F90:387-395).
+struct MicroOcnd2 {
+  const double *tiwmx;
+  double icldfr, ssio, ssiu, ifr, zzz;
+};
+
+// Entry of a TIWMX table at temperature t (the reference's NINT(XNDEGR * T) lookup).
+__device__ __forceinline__ double tiwmx_at(const double *tiwmx, int table, double t) {
+  return tiwmx[table * ICE3_TIWMX_N + tiwmx_index(t)];
+}
+
+// ICE4_NUCLEATION (micro/ice4_nucleation.func.h:68-143), LFEEDBACKT.  With OCND2 (FP64 only)
+// the nucleated concentration is the Meyers function of the tables, reduced between 0 and
+// -20 C and with height, times the ice cloud fraction, and no vapour is taken (:127-139).
+template <class R, bool OCND2 = false>
 __device__ __forceinline__ void nucleation(bool ldcompute, double tht, double pabst, double rhodref,
                                   double exn, double lsfact, double t, double rvt, double &cit,
-                                  double &rvheni_mr) {
+                                  double &rvheni_mr, const MicroOcnd2 *x = nullptr) {
   const bool gnegt = ldcompute && t < P.XTT && rvt > P.XRTMIN[1];
   rvheni_mr = 0.;
   if (!gnegt) return;
+  if constexpr (OCND2) {
+    const double zredin = tiwmx_at(x->tiwmx, ICE3_TW_REDIN, t);
+    const double zam3 = tiwmx_at(x->tiwmx, ICE3_TW_AM3, dmax(P.XFRMIN[27], t));
+    const double q = (20000. - dmin(20000., x->zzz)) / 20000.;
+    double zzw = zredin * dmax(0.1, sq(q) * sq(q)) * zam3 * (0.0001 + 0.9999 * x->icldfr);
+    zzw = dmin(zzw, 50.E3);
+    cit = dmax(zzw + cit, cit);
+    return;
+  }
   const R rt = (R)t;
   const R lt = rlog<R>(rt);
   R usw = rexp<R>((R)P.XALPW - (R)P.XBETAW / rt - (R)P.XGAMW * lt);
@@ -482,20 +506,30 @@ template <class R> __device__ __forceinline__ R powl_(R lnx, R c) { return rexp<
 // Outputs: pa (rates) and pb (instantaneous changes), both double.
 // Soft calls (reuse of the hard call's rates) skip everything that only feeds
 // rate evaluations guarded by .NOT.LDSOFT, such as the slope parameters.
-template <class R, class TB>
+//
+// OCND2 (PARAM_ICE LOCND2 = .TRUE., FP64 only; x holds the point's OCND2 inputs, read by
+// hard calls): the nucleation of the tables; ZESI and ZAI from the TIWMX tables (:304-310);
+// the collision factors ZCOLF, ZACRF (:321-345) in the riming and the rain accretion of cloud
+// water; the deposition tuning XFRMIN(40)/XFRMIN(39) (micro/mode_ice4_slow.F90:92-100); no
+// snow-graupel collection (micro/mode_ice4_fast_rg.F90:185-188,207-209); ICE4_FAST_RI_RS
+// (micro/mode_ice4_fast_ri_rs.F90) in place of ICE4_FAST_RI; their PA terms (:625-691).
+template <class R, bool OCND2 = false, class TB>
 __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref, double lvfact,
                                   double lsfact, double pres, double cf, double sigma_rc,
                                   double &cit, double pt, const double *pvart,
                                   StepState<R> &S, const PointConst<R> &pc,
-                                  const TB *__restrict__ tab, double *pa, double *pb) {
+                                  const TB *__restrict__ tab, double *pa, double *pb,
+                                  const MicroOcnd2 *x = nullptr) {
+  static_assert(!OCND2 || sizeof(R) == sizeof(double), "the OCND2 branch is FP64");
   double zt = pt;
   double zv[7];
 #pragma unroll
   for (int jv = 0; jv < 7; ++jv) { zv[jv] = pvart[jv]; pa[jv] = 0.; pb[jv] = 0.; }
   R ka = 0, dv = 0, ai = 0, cj = 0, lnt = 0, esi = 0, esw = 0;
+  double esi_tab = 0., colf = 1., acrf = 1.;  // OCND2: tabulated ZESI, ZCOLF, ZACRF
   if (!soft) {
     double rvheni, rrhong, rimltc;
-    nucleation<R>(true, zv[ITH], pres, rhodref, exn, lsfact, zt, zv[IRV], cit, rvheni);
+    nucleation<R, OCND2>(true, zv[ITH], pres, rhodref, exn, lsfact, zt, zv[IRV], cit, rvheni, x);
     zv[ITH] = zv[ITH] + rvheni * lsfact;
     zt = zv[ITH] * exn;
     zv[IRV] = zv[IRV] - rvheni;
@@ -532,13 +566,37 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     lnt = rlog<R>(t);
     esi = rexp<R>((R)P.XALPI - (R)P.XBETAI / t - (R)P.XGAMI * lnt);
     esw = rexp<R>((R)P.XALPW - (R)P.XBETAW / t - (R)P.XGAMW * lnt);
-    const R zzw = esi;
-    S.ssi = (R)zv[IRV] * ((R)pres - zzw) / ((R)P.XEPSILO * zzw) - (R)1.0;
-    ka = (R)2.38E-2 + (R)0.0071E-2 * (t - (R)P.XTT);
-    dv = (R)0.211E-4 * powl_<R>(lnt - (R)P.LN_XTT, (R)1.94) * ((R)P.XP00 / (R)pres);
-    ai = sq((R)P.XLSTT + (R)(P.XCPV - P.XCI) * (t - (R)P.XTT)) / (ka * (R)P.XRV * sq(t)) +
-         ((R)P.XRV * t) / (dv * zzw);
+    if constexpr (OCND2) {
+      esi_tab = tiwmx_at(x->tiwmx, ICE3_TW_ESI, zt);
+      S.ssi = (R)(zv[IRV] * (pres - esi_tab) / (P.XEPSILO * esi_tab) - 1.0);
+      ka = (R)2.38E-2 + (R)0.0071E-2 * (t - (R)P.XTT);
+      dv = (R)0.211E-4 * powl_<R>(lnt - (R)P.LN_XTT, (R)1.94) * ((R)P.XP00 / (R)pres);
+      ai = (R)(tiwmx_at(x->tiwmx, ICE3_TW_A2, zt) + tiwmx_at(x->tiwmx, ICE3_TW_BB3, zt) * pres);
+    } else {
+      const R zzw = esi;
+      S.ssi = (R)zv[IRV] * ((R)pres - zzw) / ((R)P.XEPSILO * zzw) - (R)1.0;
+      ka = (R)2.38E-2 + (R)0.0071E-2 * (t - (R)P.XTT);
+      dv = (R)0.211E-4 * powl_<R>(lnt - (R)P.LN_XTT, (R)1.94) * ((R)P.XP00 / (R)pres);
+      ai = sq((R)P.XLSTT + (R)(P.XCPV - P.XCI) * (t - (R)P.XTT)) / (ka * (R)P.XRV * sq(t)) +
+           ((R)P.XRV * t) / (dv * zzw);
+    }
     cj = (R)P.XSCFAC * pc.rho03 / sqrt((R)1.718E-5 + (R)0.0049E-5 * (t - (R)P.XTT));
+    if constexpr (OCND2) {  // collision factors, PCONC = XCONC_LAND (:321-345)
+      colf = 0.00001;
+      acrf = 0.00001;
+      if (zv[IRC] > 1.0E-10) {
+        const double conc = P.XCONC_LAND * 0.000001;
+        const double rcw = 0.1 * rpow<double>(0.75 * zv[IRC] * rhodref / (P.XPI * conc), 0.333);
+        double vt = rcw < 0.0065 ? 1.19E6 * sq(rcw) : 8000. * rcw;
+        vt = dmin(10., vt);
+        const double st = dmax(0.01, 2. * (100. - vt) * vt / (P.XG * 10.));
+        if (st > 0.1) colf = dmax(0.01, dmin(1., 0.939 * rpow<double>(st, 2.657)));
+        if (zv[IRR] > 1.0E-10 && rcw > 1.0E-5) {
+          const double r20 = exp_f64(rcw * 2000.);
+          acrf = (r20 - 1.) / (r20 + 1.);
+        }
+      }
+    }
   }
   const R t = (R)zt, rho = (R)rhodref;
   const R rv = (R)zv[IRV], rc = (R)zv[IRC], rr = (R)zv[IRR], ri = (R)zv[IRI], rs = (R)zv[IRS],
@@ -611,7 +669,10 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     if (!soft) S.rchoni = dmin((R)1000., (R)P.XHON * rho * rc * rexp<R>((R)P.XALPHA3 * (t - (R)P.XTT) - (R)P.XBETA3));
   } else S.rchoni = 0;
   if (rv_ok && rs_ok) {
-    if (!soft) S.rvdeps = (ssi / (rho * ai)) * vent_s;
+    if (!soft) {
+      S.rvdeps = (ssi / (rho * ai)) * vent_s;
+      if constexpr (OCND2) S.rvdeps = S.rvdeps * (R)P.XFRMIN[40];
+    }
   } else S.rvdeps = 0;
   if (ri_ok && rs_ok) {
     if (!soft) S.riaggs = (R)P.XFIAGGS * rexp<R>((R)P.XCOLEXIS * (t - (R)P.XTT)) * ri * powl_<R>(lls, (R)P.XEXIAGGS) * rhomcexvt;
@@ -622,15 +683,27 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     }
   } else S.riauts = 0;
   if (rv_ok && rg_ok) {
-    if (!soft) S.rvdepg = (ssi / (rho * ai)) * vent_g;
+    if (!soft) {
+      S.rvdepg = (ssi / (rho * ai)) * vent_g;
+      if constexpr (OCND2) S.rvdepg = S.rvdepg * (R)P.XFRMIN[39];
+    }
   } else S.rvdepg = 0;
+  // graupel deposition partly to snow with OCND2 (micro/mode_ice4_tendencies.F90:445-457)
+  double zdepg_s = 0., znodepg_s = 1.;
+  if constexpr (OCND2) {
+    if (P.XFRMIN[5] > 1.0E-12 && P.XFRMIN[6] > 0.01) {
+      zdepg_s = dmax(0., dmin(1., (P.XFRMIN[5] - zv[IRG] / P.XFRMIN[5]))) *
+                dmax(0., dmin(1., ((double)ssi / P.XFRMIN[6])));
+      znodepg_s = 1. - zdepg_s;
+    }
+  }
 
   // ICE4_WARM (micro/mode_ice4_warm.F90:100-197)
   if (hlc_hrc > (R)P.XRTMIN[2] && hlc_hcf > (R)0.) {
     if (!soft) S.rcautr = (R)P.XTIMAUTC * dmax(hlc_hrc - hlc_hcf * (R)P.XCRIAUTC / rho, (R)0.0);
   } else S.rcautr = 0;
   if (rc_ok && rr_ok) {
-    if (!soft) S.rcaccr = (R)P.XFCACCR * rc * (R)1.0 * powl_<R>(llr, (R)P.XEXCACCR) * rhomcexvt;
+    if (!soft) S.rcaccr = (R)P.XFCACCR * rc * (R)acrf * powl_<R>(llr, (R)P.XEXCACCR) * rhomcexvt;
   } else S.rcaccr = 0;
   if (rr_ok && !rc_ok) {
     if (!soft) {
@@ -664,8 +737,8 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     const R zzw2 = interp1d<R>(lls, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM2);
     const R zzw3 = interp1d<R>(lls, P.NGAMINC, (R)P.XRIMINTP1, (R)P.XRIMINTP2, tab + ICE3_T_RIM4);
     const R lexc = powl_<R>(lls, (R)P.XEXCRIMSS);
-    S.rs_rcrimss = (R)P.XCRIMSS * zzw1 * rc * (R)1.0 * lexc * rhomcexvt;
-    S.rs_rcrims = (R)P.XCRIMSG * rc * (R)1.0 * lexc * rhomcexvt;
+    S.rs_rcrimss = (R)P.XCRIMSS * zzw1 * rc * (R)colf * lexc * rhomcexvt;
+    S.rs_rcrims = (R)P.XCRIMSG * rc * (R)colf * lexc * rhomcexvt;
     const R zzw = S.rs_rcrims - S.rs_rcrimss;
     const R x = (R)P.XSRIMCG * powl_<R>(lls, (R)P.XEXSRIMCG) * ((R)1.0 - zzw2);
     S.rs_rsrimcg = zzw * x / dmax((R)1.E-20, (R)P.XSRIMCG3 * (R)P.XSRIMCG2 * powl_<R>(lls, (R)P.XEXSRIMCG2) * ((R)1. - zzw3) - (R)P.XSRIMCG3 * x);
@@ -696,9 +769,11 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       const R zzw = (R)P.XFRACCSS * (powl_<R>(lls, (R)P.XCXS)) * rhom *
                     ((R)P.XLBRACCS1 / (sq(lbdas)) + (R)P.XLBRACCS2 / (lbdas * lbdar) + (R)P.XLBRACCS3 / (lr2)) / (lr2 * lr2);
       S.rs_rraccss = zzw1 * zzw;
+      if constexpr (OCND2) S.rs_rraccss = S.rs_rraccss * (R)P.XFRMIN[7];
       S.rs_rraccs = zzw2 * zzw;
-      S.rs_rsaccrg = (R)P.XFSACCRG * zzw3 * lbs_cxs_bs * rhom *
-                     ((R)P.XLBSACCR1 / (lr2) + (R)P.XLBSACCR2 / (lbdar * lbdas) + (R)P.XLBSACCR3 / (sq(lbdas))) / lbdar;
+      if (!OCND2 || zv[IRS] > P.XFRMIN[1])
+        S.rs_rsaccrg = (R)P.XFSACCRG * zzw3 * lbs_cxs_bs * rhom *
+                       ((R)P.XLBSACCR1 / (lr2) + (R)P.XLBSACCR2 / (lbdar * lbdas) + (R)P.XLBSACCR3 / (sq(lbdas))) / lbdar;
     }
   }
   R rraccss = 0, rraccsg = 0, rsaccrg = 0;
@@ -753,8 +828,8 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   }
   const R rhom1 = (!soft && rg_ok) ? pc.rhom : (R)0;
   const bool gdrys = rs_ok && rg_ok;
-  if (!gdrys) { S.rg_rsdryg = 0; S.rg_rswetg = 0; }
-  if (!soft && gdrys) {
+  if (!gdrys || OCND2) { S.rg_rsdryg = 0; S.rg_rswetg = 0; }
+  if (!OCND2 && !soft && gdrys) {
     const Idx2<R> ix = interp2d_idx<R>(llg, lls, P.NDRYLBDAG, P.NDRYLBDAS, (R)P.XDRYINTP1G,
                                        (R)P.XDRYINTP2G, (R)P.XDRYINTP1S, (R)P.XDRYINTP2S);
     const R zzw = interp2d<R>(ix, P.NDRYLBDAG, tab + ICE3_T_SDRYG);
@@ -811,8 +886,10 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     }
   } else S.rgmltr = 0;
 
-  // ICE4_FAST_RI (micro/mode_ice4_fast_ri.F90:70-81)
-  if (ssi > (R)0. && rc_ok && ri_ok && cit > 1.E-20) {
+  // ICE4_FAST_RI (micro/mode_ice4_fast_ri.F90:70-81); not called with OCND2 (tendencies :568-578)
+  if (OCND2) {
+    S.rcberi = 0;
+  } else if (ssi > (R)0. && rc_ok && ri_ok && cit > 1.E-20) {
     if (!soft) {
       R lli = (R)P.LN_XLBI + (R)P.XLBEXI * rlog<R>(rho * ri / (R)cit);
       R l;
@@ -825,6 +902,36 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       S.rcberi = (ssi / (rho * ai)) * (R)cit * ((R)P.X0DEPI / l + (R)P.X2DEPI * cj * cj / powl_<R>(lli, (R)(P.XDI + 2.0)));
     }
   } else S.rcberi = 0;
+
+  // ICE4_FAST_RI_RS (micro/mode_ice4_fast_ri_rs.F90:95-250) with OCND2, LMODICEDEP=F: large
+  // ice crystals to snow (RILARS) and deposition on non-spherical crystals in and out of the
+  // ice cloud (RVDEPI); both are 0 in soft calls (:137,:168).
+  double rilars = 0., rvdepi = 0.;
+  if constexpr (OCND2) {
+    if (!soft) {
+      const double ztimesc = P.XFRMIN[14], zdicrit = P.XFRMIN[15], zwcitred = 0.1;
+      const double ri_d = zv[IRI], ssi_d = (double)ssi, ai_d = (double)ai;
+      if (ri_d > 0.0 && ssi_d > 0.001) rilars = dmin(ri_d, ri_d / (87.5 * sq(zdicrit) * ai_d / ssi_d));
+      if ((pres * 0.5 - esi_tab) > 0. && cit > 0.) {
+        const double zw2d = 1. / (x->ifr * x->icldfr + 1. - x->icldfr);
+        const double ztc = dmax(-18., dmin(-1., zt - P.XTT));
+        const double zhu = dmin(0.15, dmax(0., ssi_d));
+        const double zcrysha = 1.1 + 3. * zhu * (1. + sin(0.64 * ztc - 1.3));
+        const double zqimax = 366.5 * (zdicrit * zdicrit * zdicrit) * cit * zwcitred / rhodref;
+        double zci2s = 0.;
+        if (ri_d > 1.0e-12 && ztimesc > 0.)
+          zci2s = ri_d * (1. - dmin(1., 0.5 * zqimax / ri_d)) * (1. - x->icldfr) * zw2d / (ztimesc * 0.5);
+        const double zzwc = zcrysha * 0.878 / ai_d * rpow<double>(cit / rhodref, 0.667) *
+                            rpow<double>(dmax(P.XRTMIN[4], ri_d) * zw2d, 0.333);
+        if (x->ssio > 0. && x->icldfr > 0.02)
+          rvdepi = rvdepi + zzwc * rpow<double>(x->ifr, 0.333) * x->ssio * x->icldfr;
+        if (x->ssiu < 0. && x->icldfr < 0.98) {
+          rilars = rilars + zci2s;
+          rvdepi = rvdepi + zzwc * rpow<double>(zwcitred, 0.667) * x->ssiu * (1. - x->icldfr);
+        }
+      }
+    }
+  }
 
   // aggregation into PA / PB (micro/mode_ice4_tendencies.F90:606-717), KRR=6
   const double lsl = lsfact - lvfact;
@@ -843,11 +950,13 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   a = a + ((double)rcdryg + (double)rrdryg) * lsl;
   a = a - (double)S.rgmltr * lsl;
   a = a + (double)S.rcberi * lsl;
+  if constexpr (OCND2) a = a + rvdepi * lsfact;
   pa[ITH] = a;
   a = 0.;
   a = a - (double)S.rvdepg;
   a = a - (double)S.rvdeps;
   a = a + (double)S.rrevav;
+  if constexpr (OCND2) a = a - rvdepi;
   pa[IRV] = a;
   a = 0.;
   a = a - (double)S.rchoni;
@@ -880,9 +989,14 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   a = a - (double)riwetg;
   a = a - (double)ridryg;
   a = a + (double)S.rcberi;
+  if constexpr (OCND2) {
+    a = a + rvdepi;
+    a = a - rilars;
+  }
   pa[IRI] = a;
   a = 0.;
   a = a + (double)S.rvdeps;
+  if constexpr (OCND2) a = a + (double)S.rvdepg * zdepg_s;
   a = a + (double)S.riaggs;
   a = a + (double)S.riauts;
   a = a + (double)rcrimss;
@@ -892,9 +1006,10 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   a = a - (double)S.rsmltg;
   a = a - (double)rswetg;
   a = a - (double)rsdryg;
+  if constexpr (OCND2) a = a + rilars;
   pa[IRS] = a;
   a = 0.;
-  a = a + (double)S.rvdepg * 1.0;
+  a = a + (double)S.rvdepg * znodepg_s;
   a = a + (double)rcrimsg + (double)rsrimcg;
   a = a + (double)rraccsg + (double)rsaccrg;
   a = a + (double)S.rsmltg;
@@ -935,11 +1050,12 @@ __device__ __forceinline__ void step_point(double dt, double exn, double rhodref
 // tendencies integrated so far taken out again (as the complete path does at :520-537 for
 // the whole step), so that the total tendencies count them once, through the sources.
 // pc is read by hard calls only.
-template <class R, bool SOFT, class TB>
+template <class R, bool SOFT, bool OCND2 = false, class TB>
 __device__ __forceinline__ int step_call(double dt, const StepPoint &p, const PointConst<R> &pc,
                                          double *vart, double &cit, double &time_,
                                          StepState<R> &S, int &calls, int max_calls,
-                                         const TB *__restrict__ tab) {
+                                         const TB *__restrict__ tab,
+                                         const MicroOcnd2 *x = nullptr) {
   if (++calls > max_calls) {
 #pragma unroll
     for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] - p.extpk[jv] * time_;
@@ -953,8 +1069,8 @@ __device__ __forceinline__ int step_call(double dt, const StepPoint &p, const Po
   const double zzt = vart[ITH] * exn;
   const double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
   const double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
-  tendencies<R>(SOFT, exn, p.rhodref, lvfact, lsfact, p.pres, p.cf, p.sigma_rc, cit, zzt, vart,
-                S, pc, tab, za, zb);
+  tendencies<R, OCND2>(SOFT, exn, p.rhodref, lvfact, lsfact, p.pres, p.cf, p.sigma_rc, cit, zzt,
+                       vart, S, pc, tab, za, zb, x);
 #pragma unroll
   for (int jv = 0; jv < 7; ++jv) za[jv] = za[jv] + p.extpk[jv];
   const double time = time_;
@@ -987,19 +1103,20 @@ __device__ __forceinline__ int step_call(double dt, const StepPoint &p, const Po
 // the sources.  Each point's time step is one outer iteration: a hard call, then soft
 // calls until the step is complete.  Returns the number of tendency calls, or -calls if
 // the cap was hit.
-template <class R, class TB>
+template <class R, bool OCND2 = false, class TB>
 __device__ __forceinline__ int stepping_point(double dt, double exn, double rhodref, double pres,
                                               double cf, double sigs, double &cit, double *vart,
                                               const double *s, double &rrevav, int max_calls,
-                                              const TB *__restrict__ tab) {
+                                              const TB *__restrict__ tab,
+                                              const MicroOcnd2 *x = nullptr) {
   StepPoint p;
   step_point(dt, exn, rhodref, pres, cf, sigs, s, vart, p);
   StepState<R> S;
   double time = 0.;
   int calls = 0;
   const PointConst<R> pc = point_const<R>(rhodref);
-  int st = step_call<R, false>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab);
-  while (st > 0) st = step_call<R, true>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab);
+  int st = step_call<R, false, OCND2>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x);
+  while (st > 0) st = step_call<R, true, OCND2>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x);
   if (st == 0) rrevav = (double)S.rrevav;
   return st < 0 ? -calls : calls;
 }

@@ -93,18 +93,25 @@ struct SedimArgs {
   const double *src_rr, *src_ri, *src_rs, *src_rg;
   double *dst_rr, *dst_ri, *dst_rs, *dst_rg;
   double *inprc, *inprr, *inpri, *inprs, *inprg;
+  double *zzz;  // OCND2 branch: receives the model level height ZZZZ (micro/rain_ice.F90:387-395)
 };
 
 // ICE4_SEDIMENTATION_STAT (micro/mode_ice4_sedimentation_stat.F90:143-229) for column i,
 // followed by the surface precipitation of RAIN_ICE (PINPRS += PINPRI, PINPRC = 0 with
-// LSEDIC=F).
-template <class R>
+// LSEDIC=F).  With ZZZ the walk also writes RAIN_ICE's model level height ZZZZ into a.zzz:
+// PDZZ accumulated from the top level down, minus half the level's own (rain_ice.F90:387-395,
+// JK = IKTE..IKTB with level 1 the lowest).
+template <class R, bool ZZZ = false>
 __device__ __forceinline__ void sedim_column(const SedimArgs &a, int i) {
   const double dt = a.dt;
-  double fr = 0., fi = 0., fs = 0., fg = 0.;
+  double fr = 0., fi = 0., fs = 0., fg = 0., zzsum = 0.;
   for (int k = a.nlev - 1; k >= 0; --k) {
     const size_t j = (size_t)k * a.ld + i;
     const double rho = a.rhodref[j], dz = a.dzz[j];
+    if constexpr (ZZZ) {
+      zzsum = zzsum + dz;
+      a.zzz[j] = zzsum - dz * 0.5;
+    }
     const R lnrho = rlog<R>((R)rho);
     double rrs = a.src_rr[j], ris = a.src_ri[j], rss = a.src_rs[j], rgs = a.src_rg[j];
     sediment_species<R>(IRR, dt, rho, lnrho, dz, rrs, fr);
@@ -166,6 +173,10 @@ struct MicroArgs {
   unsigned long long *counters;  // [0] tendency calls, [1] capped points, [2] micro points
   int max_calls;                 // sub-step cap per point
   unsigned long long *capped;    // context counter of capped points (always counted)
+  // OCND2 branch only: ICE_ADJUST's OCND2 outputs, the TIWMX tables, and the model level
+  // height ZZZZ, which the sedimentation's column walk writes into rainfr before the
+  // microphysics reads it and writes the rain fraction over it
+  const double *icldfr, *ssio, *ssiu, *ifr, *tiwmx;
 };
 
 // ICE4_CORRECT_NEGATIVITIES (micro/mode_ice4_correct_negativities.F90:51-171) on one
@@ -212,23 +223,36 @@ __device__ __forceinline__ void rain_ice_factors(const double *t, double exn, do
   lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide;
 }
 
-// RAIN_ICE's microphysics mask (micro/rain_ice.F90:365-381): a hydrometeor above XRTMIN.
-__device__ __forceinline__ bool micro_mask(const double *t) {
+// RAIN_ICE's microphysics mask (micro/rain_ice.F90:365-381): a hydrometeor above XRTMIN;
+// with OCND2 (:348-364), ice supersaturation in the ice cloud above XFRMIN(12) or a
+// hydrometeor above XFRMIN(13).
+template <bool OCND2 = false>
+__device__ __forceinline__ bool micro_mask(const double *t, const MicroArgs &a, size_t j) {
+  if constexpr (OCND2)
+    return a.ssio[j] > P.XFRMIN[12] || t[IRC] > P.XFRMIN[13] || t[IRR] > P.XFRMIN[13] ||
+           t[IRI] > P.XFRMIN[13] || t[IRS] > P.XFRMIN[13] || t[IRG] > P.XFRMIN[13];
   return t[IRC] > P.XRTMIN[2] || t[IRR] > P.XRTMIN[3] || t[IRI] > P.XRTMIN[4] ||
          t[IRS] > P.XRTMIN[5] || t[IRG] > P.XRTMIN[6];
 }
 
+// The OCND2 inputs of point j (the model level height is staged in rainfr).
+__device__ __forceinline__ MicroOcnd2 ocnd2_point(const MicroArgs &a, size_t j) {
+  return MicroOcnd2{a.tiwmx, a.icldfr[j], a.ssio[j], a.ssiu[j], a.ifr[j], a.rainfr[j]};
+}
+
 // Nucleation outside the microphysics mask (micro/rain_ice.F90:505-540) at point j with
 // t-state t: cit is reset and updated, rvheni receives the nucleated vapour.
-template <class R>
+template <class R, bool OCND2 = false>
 __device__ __forceinline__ void micro_nucleation(const MicroArgs &a, size_t j, const double *t,
                                                  double &cit, double &rvheni) {
   const double exn = a.exn[j];
   double lsfact, lvfact;
   rain_ice_factors(t, exn, lsfact, lvfact);
   cit = 0.;
-  nucleation<R>(true, t[ITH], a.pabs[j], a.rhodref[j], exn, lsfact / exn, t[ITH] * exn, t[IRV],
-                cit, rvheni);
+  MicroOcnd2 x;
+  if constexpr (OCND2) x = ocnd2_point(a, j);
+  nucleation<R, OCND2>(true, t[ITH], a.pabs[j], a.rhodref[j], exn, lsfact / exn, t[ITH] * exn,
+                       t[IRV], cit, rvheni, &x);
 }
 
 // RAIN_ICE section 8 at point j (micro/rain_ice.F90:612-652 and the reinstated 754-823):
@@ -271,17 +295,20 @@ __device__ __forceinline__ void micro_finish(const MicroArgs &a, size_t j, const
 // RAIN_ICE sections 1, 4, 5, 8 (micro/rain_ice.F90:334-384, 505-540, 612-652 and the
 // reinstated 754-823) for point j, all in one thread.  Returns the number of tendency
 // calls, negative if the sub-step cap was hit.
-template <class R, class TB>
+template <class R, bool OCND2 = false, class TB>
 __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j, const TB *__restrict__ tab) {
   double wr[7], s[7];
   micro_load(a, j, wr, s);
   double cit = a.src_ci[j], rvheni = 0., evap3d = 0.;
   int calls = 0;
-  if (micro_mask(wr))
-    calls = stepping_point<R>(a.dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j], cit,
-                              wr, s, evap3d, a.max_calls, tab);
-  else
-    micro_nucleation<R>(a, j, wr, cit, rvheni);
+  if (micro_mask<OCND2>(wr, a, j)) {
+    MicroOcnd2 x;
+    if constexpr (OCND2) x = ocnd2_point(a, j);
+    calls = stepping_point<R, OCND2>(a.dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j],
+                                     cit, wr, s, evap3d, a.max_calls, tab, &x);
+  } else {
+    micro_nucleation<R, OCND2>(a, j, wr, cit, rvheni);
+  }
   micro_finish(a, j, wr, cit, evap3d, rvheni, calls);
   return calls;
 }

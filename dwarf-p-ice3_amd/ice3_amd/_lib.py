@@ -35,6 +35,7 @@ RAIN_IN_FIELDS = ("exn", "dzz", "rhodj", "rhodref", "exn_ref", "pabs", "cldfr", 
                   "rst", "rgt")
 RAIN_IO_FIELDS = ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "ci")
 RAIN_OUT_FIELDS = ("evap3d", "rainfr", "inprc", "inprr", "inpri", "inprs", "inprg")
+RAIN_OCND2_IN_FIELDS = ("icldfr", "ssio", "ssiu", "ifr")
 PIPE_IN_FIELDS = ("pabs", "sigs", "th", "exn", "exn_ref", "rhodref", "rhodj", "dzz", "rv", "rc",
                   "ri", "rr", "rs", "rg", "ci", "cf_mf", "rc_mf", "ri_mf", "ths", "rvs", "rcs",
                   "rrs", "ris", "rss", "rgs", "sigqsat")
@@ -49,6 +50,7 @@ Ocnd2Out = _struct("ice3_ocnd2_out", OCND2_OUT_FIELDS)
 RainIn = _struct("ice3_rain_ice_in", RAIN_IN_FIELDS)
 RainIO = _struct("ice3_rain_ice_inout", RAIN_IO_FIELDS)
 RainOut = _struct("ice3_rain_ice_out", RAIN_OUT_FIELDS)
+RainOcnd2In = _struct("ice3_rain_ocnd2_in", RAIN_OCND2_IN_FIELDS)
 PipeIn = _struct("ice3_pipeline_in", PIPE_IN_FIELDS)
 PipeOut = _struct("ice3_pipeline_out", PIPE_OUT_FIELDS)
 Stats = _struct("ice3_stats", ("calls", "counters"))
@@ -74,6 +76,7 @@ class Surrogate(ctypes.Structure):
 EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_constants",
            "ice3_ctx_set_substep_cap", "ice3_ctx_status",
            "ice3_ice_adjust", "ice3_ice_adjust_ocnd2", "ice3_rain_ice", "ice3_adjust_rain_ice",
+           "ice3_rain_ice_ocnd2", "ice3_adjust_rain_ice_ocnd2",
            "ice3_stage_timing",
            "ice3_stage_times", "ice3_adjust_rain_ice_stages", "ice3_adjust_rain_ice_surrogate",
            "ice3_adjust_rain_ice_surrogate_noalloc",
@@ -112,6 +115,13 @@ def load():
                                                ctypes.POINTER(RainOut), ctypes.POINTER(Stats), _P]
         lib.ice3_adjust_rain_ice.argtypes = common + [ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut),
                                                       ctypes.POINTER(Stats), _P]
+        c5 = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        lib.ice3_rain_ice_ocnd2.argtypes = c5 + [ctypes.POINTER(RainIn), ctypes.POINTER(RainOcnd2In),
+                                                 ctypes.POINTER(RainIO), ctypes.POINTER(RainOut),
+                                                 ctypes.POINTER(Stats), _P]
+        lib.ice3_adjust_rain_ice_ocnd2.argtypes = c5 + [ctypes.POINTER(PipeIn), ctypes.POINTER(Ocnd2In),
+                                                        ctypes.POINTER(PipeOut), ctypes.POINTER(Ocnd2Out),
+                                                        ctypes.POINTER(Stats), _P]
         lib.ice3_adjust_rain_ice_stages.argtypes = common + [
             ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut), ctypes.POINTER(Stats), ctypes.c_int, _P]
         lib.ice3_adjust_rain_ice_surrogate.argtypes = [

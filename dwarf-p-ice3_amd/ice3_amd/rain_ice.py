@@ -79,6 +79,16 @@ class _Base:
 class RainIce(_Base):
     """RAIN_ICE on MI355X (see module docstring)."""
 
+    _ENTRY = "ice3_rain_ice"
+
+    def _extra_inputs(self, state, layout, nlev, ncol):
+        return {}
+
+    def _launch(self, precision, ncol, nlev, dt, cin, extra, cio, cout, st):
+        return self._lib.ice3_rain_ice(self._ctx, _PREC[precision], ncol, nlev, ncol, float(dt),
+                                       ctypes.byref(cin), ctypes.byref(cio), ctypes.byref(cout),
+                                       ctypes.byref(st), stream_ptr(self.device))
+
     def __call__(self, state: dict, dt: float, lsedim_after: bool = False, ldeposc: bool = False,
                  ldsoft: bool = False, lfeedbackt: bool = True, max_iterations: int | None = None,
                  precision: str = "fp64", return_calls: bool = False, check_inputs: bool = False):
@@ -109,15 +119,16 @@ class RainIce(_Base):
                 raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
         if check_inputs:
             _lib.require_finite({**inp, **io}, self.device)
+        extra = self._extra_inputs(state, layout, nlev, ncol)
+        if check_inputs:
+            _lib.require_finite(extra, self.device)
         counters, calls, st = self._stats(nlev * ncol, return_calls)
         cin = _lib.RainIn(*[ptr(inp[k]) for k in _lib.RAIN_IN_FIELDS])
         cio = _lib.RainIO(*[ptr(io[k]) for k in _lib.RAIN_IO_FIELDS])
         cout = _lib.RainOut(*[ptr(out[k]) for k in _lib.RAIN_OUT_FIELDS])
         with torch.cuda.device(self.device):
-            rc = self._lib.ice3_rain_ice(self._ctx, _PREC[precision], ncol, nlev, ncol, float(dt),
-                                         ctypes.byref(cin), ctypes.byref(cio), ctypes.byref(cout),
-                                         ctypes.byref(st), stream_ptr(self.device))
-        _lib.check(rc, "ice3_rain_ice")
+            rc = self._launch(precision, ncol, nlev, dt, cin, extra, cio, cout, st)
+        _lib.check(rc, self._ENTRY)
         self._finish_counters(counters)
         new = dict(state)
         for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs"):
@@ -141,21 +152,26 @@ class AdjustRainIce(_Base):
 
     def __call__(self, f: dict, dt: float, precision: str = "fp64", return_calls: bool = False,
                  check_inputs: bool = False):
+        return self._fused(f, dt, precision, return_calls, check_inputs)
+
+    def _fused(self, f, dt, precision, return_calls, check_inputs, ocnd2=False):
         shape = self._shape(f["th"])
         layout = resolve_layout(self.layout, f["th"])
         g = {k: self._col(f[k], k, layout) for k in _lib.PIPE_IN_FIELDS if k != "sigqsat"}
         nlev, ncol = g["th"].shape
-        sq = torch.as_tensor(f["sigqsat"], dtype=torch.float64, device=self.device)
-        if sq.dim() > 1 and sq.numel() != ncol:  # a full field: per column, the lowest level
-            sq = self._col(sq, "sigqsat", layout)[0]
-        sq = sq.reshape(-1).contiguous()
-        if sq.numel() != ncol:
-            raise ValueError(f"sigqsat: {sq.numel()} values for {ncol} columns")
-        for k, v in g.items():
+        sq = self._per_column(f["sigqsat"], "sigqsat", ncol, layout)
+        extra = {}
+        if ocnd2:
+            extra["zz"] = self._col(f["zz"], "zz", layout)
+            wgt = self._per_column(f["ice_cld_wgt"], "ice_cld_wgt", ncol, layout)
+            if nlev < 2:
+                raise ValueError("the OCND2 branch needs at least two levels")
+        for k, v in list(g.items()) + list(extra.items()):
             if tuple(v.shape) != (nlev, ncol):
                 raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
         if check_inputs:
-            _lib.require_finite({**g, "sigqsat": sq}, self.device)
+            _lib.require_finite({**g, **extra, "sigqsat": sq, **({"ice_cld_wgt": wgt} if ocnd2 else {})},
+                                self.device)
         out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device)
                for k in _lib.PIPE_OUT_FIELDS[:15]}
         out.update({k: torch.empty(ncol, dtype=torch.float64, device=self.device)
@@ -164,12 +180,79 @@ class AdjustRainIce(_Base):
         cin = _lib.PipeIn(*[ptr(g[k]) for k in _lib.PIPE_IN_FIELDS[:-1]], ptr(sq))
         cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
         with torch.cuda.device(self.device):
-            rc = self._lib.ice3_adjust_rain_ice(self._ctx, _PREC[precision], ncol, nlev, ncol, float(dt),
-                                                ctypes.byref(cin), ctypes.byref(cout), ctypes.byref(st),
-                                                stream_ptr(self.device))
-        _lib.check(rc, "ice3_adjust_rain_ice")
+            if ocnd2:
+                x = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device)
+                     for k in _lib.OCND2_OUT_FIELDS}
+                cin2 = _lib.Ocnd2In(ptr(extra["zz"]), ptr(wgt))
+                cout2 = _lib.Ocnd2Out(*[ptr(x[k]) for k in _lib.OCND2_OUT_FIELDS])
+                rc = self._lib.ice3_adjust_rain_ice_ocnd2(self._ctx, ncol, nlev, ncol, float(dt),
+                                                          ctypes.byref(cin), ctypes.byref(cin2),
+                                                          ctypes.byref(cout), ctypes.byref(cout2),
+                                                          ctypes.byref(st), stream_ptr(self.device))
+                out.update(x)
+            else:
+                rc = self._lib.ice3_adjust_rain_ice(self._ctx, _PREC[precision], ncol, nlev, ncol, float(dt),
+                                                    ctypes.byref(cin), ctypes.byref(cout), ctypes.byref(st),
+                                                    stream_ptr(self.device))
+        _lib.check(rc, "ice3_adjust_rain_ice_ocnd2" if ocnd2 else "ice3_adjust_rain_ice")
         self._finish_counters(counters)
         res = {k: from_columns(v, layout, shape) for k, v in out.items()}
         if return_calls:
             res["calls"] = from_columns(calls.view(nlev, ncol), layout, shape)
         return res
+
+    def _per_column(self, x, name, ncol, layout):
+        t = torch.as_tensor(x, dtype=torch.float64, device=self.device)
+        if t.dim() > 1 and t.numel() != ncol:  # a full field: per column, the lowest level
+            t = self._col(t, name, layout)[0]
+        t = t.reshape(-1).contiguous()
+        if t.numel() != ncol:
+            raise ValueError(f"{name}: {t.numel()} values for {ncol} columns")
+        return t
+
+
+class RainIceOCND2(RainIce):
+    """RAIN_ICE with PARAM_ICE LOCND2 = .TRUE. on MI355X (``ice3_rain_ice_ocnd2``, FP64).
+
+    The reference reaches this branch through the Fortran RAIN_ICE only (its JAX component
+    has no OCND2).  The state takes RainIce's keys plus ICE_ADJUST's OCND2 outputs
+    ``icldfr, ssio, ssiu, ifr`` (PICLDFR, PSSIO, PSSIU, PIFR: ``IceAdjustOCND2`` returns them);
+    results as RainIce.  A ``constants`` object must have ``param_icen.LCOND2 = 1``.
+    """
+
+    _ENTRY = "ice3_rain_ice_ocnd2"
+
+    def __init__(self, constants=None, layout: str = "auto", device=None):
+        if not isinstance(constants, dict):
+            check_configuration(constants, "constants", ocnd2=True)
+        super().__init__({}, layout, device)
+
+    def _extra_inputs(self, state, layout, nlev, ncol):
+        x = {k: self._col(state[k], k, layout) for k in _lib.RAIN_OCND2_IN_FIELDS}
+        for k, v in x.items():
+            if tuple(v.shape) != (nlev, ncol):
+                raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
+        return x
+
+    def _launch(self, precision, ncol, nlev, dt, cin, extra, cio, cout, st):
+        if precision != "fp64":
+            raise ValueError("the OCND2 branch is implemented in FP64 only")
+        cin2 = _lib.RainOcnd2In(*[ptr(extra[k]) for k in _lib.RAIN_OCND2_IN_FIELDS])
+        return self._lib.ice3_rain_ice_ocnd2(self._ctx, ncol, nlev, ncol, float(dt), ctypes.byref(cin),
+                                             ctypes.byref(cin2), ctypes.byref(cio), ctypes.byref(cout),
+                                             ctypes.byref(st), stream_ptr(self.device))
+
+
+class AdjustRainIceOCND2(AdjustRainIce):
+    """The fused step with OCND2 in both halves (``ice3_adjust_rain_ice_ocnd2``, FP64):
+    IceAdjustOCND2's ICE_ADJUST, then RainIceOCND2's sedimentation and microphysics on the
+    adjusted state and ice cloud.  Inputs are AdjustRainIce's plus ``zz`` (level heights, m)
+    and ``ice_cld_wgt`` (per column); the result adds ``icldfr, wcldfr, ssio, ssiu, ifr``."""
+
+    def __init__(self, constants=None, layout: str = "auto", device=None):
+        if not isinstance(constants, dict):
+            check_configuration(constants, "constants", ocnd2=True)
+        super().__init__({}, layout, device)
+
+    def __call__(self, f: dict, dt: float, return_calls: bool = False, check_inputs: bool = False):
+        return self._fused(f, dt, "fp64", return_calls, check_inputs, ocnd2=True)

@@ -154,6 +154,22 @@ int ice3_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double dt
                   const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
                   const ice3_rain_ice_out *out, const ice3_stats *stats, void *stream);
 
+/* ---- rain_ice with OCND2: RAIN_ICE called with PARAM_ICE LOCND2 = .TRUE. (FP64).  The
+ * microphysics mask on PSSIO and XFRMIN(12:13) (micro/rain_ice.F90:348-364), nucleation from
+ * the tabulated Meyers and reduction functions (micro/ice4_nucleation.func.h:127-139), and in
+ * ICE4_TENDENCIES the tabulated ZESI and ZAI, the OCND2 collision factors, ICE4_FAST_RI_RS in
+ * place of ICE4_FAST_RI and the OCND2 terms (micro/mode_ice4_tendencies.F90:304-346,
+ * 445-457,568-599,625-691).  The reference reaches it through the Fortran RAIN_ICE only.
+ * Its extra inputs are ICE_ADJUST's OCND2 outputs (ice3_ice_adjust_ocnd2). */
+typedef struct {   /* PICLDFR, PSSIO, PSSIU, PIFR, (nlev, ld) */
+  const double *icldfr, *ssio, *ssiu, *ifr;
+} ice3_rain_ocnd2_in;
+
+int ice3_rain_ice_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                        const ice3_rain_ice_in *in, const ice3_rain_ocnd2_in *in2,
+                        const ice3_rain_ice_inout *io, const ice3_rain_ice_out *out,
+                        const ice3_stats *stats, void *stream);
+
 /* ---- fused ice_adjust + rain_ice (the "adjust then microphysics" time step) ----
  * ice_adjust runs on the state at t; its updated sources, times dt, are the
  * t-state of rain_ice (the synthetic and AROME-like inputs carry the state at t
@@ -175,6 +191,15 @@ typedef struct {
 int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double dt,
                          const ice3_pipeline_in *in, const ice3_pipeline_out *out,
                          const ice3_stats *stats, void *stream);
+
+/* The fused step with OCND2 in both halves: ice3_ice_adjust_ocnd2's ICE_ADJUST (in2: level
+ * heights and PICE_CLD_WGT; out2: its ice cloud outputs), then the sedimentation and
+ * ice3_rain_ice_ocnd2's microphysics on the adjusted state, with out2 as its OCND2 inputs.
+ * FP64; nlev >= 2. */
+int ice3_adjust_rain_ice_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                               const ice3_pipeline_in *in, const ice3_ocnd2_in *in2,
+                               const ice3_pipeline_out *out, const ice3_ocnd2_out *out2,
+                               const ice3_stats *stats, void *stream);
 
 /* Stages of the fused step, for ice3_adjust_rain_ice_stages: run any subset, in this order,
  * on the same in/out buffers (the outputs of one call are the state the next call reads). */

@@ -41,6 +41,7 @@ typedef struct {
   double XALPW, XBETAW, XGAMW, XALPI, XBETAI, XGAMI, XESTT, XP00, XRHOLW, XPI, XG, XMNH_TINY;
   /* RAIN_ICE_DESCR_t */
   double XRTMIN[7]; /* 1-based */
+  double XFRMIN[41]; /* 1-based, RAIN_ICE_PARAM_t XFRMIN (micro/modd_rain_ice_paramn.F90:189) */
   double XCEXVT, XLBR, XLBEXR, XLBS, XLBEXS, XLBG, XLBEXG, XLBI, XLBEXI, XLBDAS_MAX, XDI, XBS,
       XCXS, XCXG, XDG, XCONC_LAND;
   /* RAIN_ICE_PARAM_t */
@@ -152,6 +153,7 @@ static int read_constants(const char *path) {
     for (int j = 0; s[j].name; ++j)
       if (!strcmp(s[j].name, name)) { *s[j].dst = buf[0]; used = 1; nfound++; }
     if (!strcmp(name, "XRTMIN")) { for (int j = 0; j < 6; ++j) P.XRTMIN[j + 1] = buf[j]; used = 1; }
+    else if (!strcmp(name, "XFRMIN") && n == 40) { for (int j = 0; j < 40; ++j) P.XFRMIN[j + 1] = buf[j]; }
     else if (!strcmp(name, "NGAMINC")) { P.NGAMINC = (int)buf[0]; }
     else if (!strcmp(name, "NACCLBDAS")) { P.NACCLBDAS = (int)buf[0]; }
     else if (!strcmp(name, "NACCLBDAR")) { P.NACCLBDAR = (int)buf[0]; }
@@ -409,6 +411,32 @@ static double esati_tab(double t) {
   if (tj > 273.15) return esat_w(tj);               /* mode_ini_tiwmx.F90:29-31 */
   return esat_i(tj);                                /* mode_tiwmx_fun.F90:12-13, TT < XTT */
 }
+/* ESATI of micro/mode_tiwmx_fun.F90:44-48 (the function, not the table): ice below XTT */
+static double esati_fn(double t) {
+  double sgn = t - P.XTT >= 0. ? 0.5 : -0.5;        /* SIGN(0.5, TT - XTT) */
+  return (0.5 + sgn) * esat_w(t) - (sgn - 0.5) * esat_i(t);
+}
+/* The ice-crystal tables RAIN_ICE reads with OCND2 (micro/mode_ini_tiwmx.F90:26-49 over
+ * micro/mode_tiwmx_fun.F90:63-106), each evaluated at its tabulated temperature. */
+static double aa2_tab(double t) {                   /* A2TAB: AA2, :63-67 */
+  double tj = tiwmx_temp(t);
+  return sq(P.XLSTT + (P.XCPV - P.XCI) * (tj - P.XTT)) / (2.38E-2 + 0.0071E-2 * (tj - P.XTT)) /
+         (sq(tj) * P.XRV);
+}
+static double bb3_tab(double t) {                   /* BB3TAB: BB3, :77-80 */
+  double tj = tiwmx_temp(t);
+  return P.XRV / (0.211E-4 * pow(tj / P.XTT, 1.94) * P.XP00) * tj / esati_fn(tj);
+}
+static double am3_tab(double t) {                   /* AM3TAB: AM3, :89-92; 0 above 273.15 K */
+  double tj = tiwmx_temp(t);
+  if (tj > 273.15) return 0.;
+  return 1000. * exp(12.96 * (esat_w(tj) / esati_fn(tj) - 1.) - 0.639);
+}
+static double redin_tab(double t) {                 /* REDINTAB: REDIN, :101-106 */
+  double tj = tiwmx_temp(t);
+  double zzt = dmax(0., dmin(1., (P.XTT - tj) / 20.));
+  return 1. - (1. - zzt) / pow(zzt * zzt * zzt + (1. - zzt) * (1. - zzt) * (1. - zzt), .333);
+}
 
 /* ICECLOUD (micro/mode_icecloud.F90:52-205) for one point as CONDENSATION calls it
  * (condensation.F90:379-382): PR = rv, PTSTEP = 1, PPBLH = -1, PWCLD = -1, XW2D = 10.
@@ -662,15 +690,34 @@ int oracle_ice_adjust_ocnd2(int ncol, int nlev, double dt, const double *sigqsat
 
 /* ============================================================== RAIN_ICE */
 
-/* ICE4_NUCLEATION (micro/ice4_nucleation.func.h:6-144), .NOT.LOCND2, LFEEDBACKT */
+/* Per-point inputs of RAIN_ICE's OCND2 branch: ICE_ADJUST's ice cloud fraction and
+ * super-/sub-saturation over ice (PICLDFR, PSSIO, PSSIU, PIFR) and RAIN_ICE's model level
+ * height ZZZZ (micro/rain_ice.F90:387-395).  A NULL pointer selects the plain branch. */
+typedef struct { double icldfr, ssio, ssiu, ifr, zzz; } ocnd2_pt_t;
+
+/* ICE4_NUCLEATION (micro/ice4_nucleation.func.h:6-144), LFEEDBACKT; with OCND2 (x != NULL)
+ * the Meyers concentration of the tables, reduced between 0 and -20 C and with height, and
+ * no vapour taken (:127-139). */
 static void ice4_nucleation(int ldcompute, double tht, double pabst, double rhodref, double exn,
-                            double lsfact, double t, double rvt, double *cit, double *rvheni_mr) {
+                            double lsfact, double t, double rvt, double *cit, double *rvheni_mr,
+                            const ocnd2_pt_t *x) {
   int gnegt = ldcompute ? (t < P.XTT && rvt > P.XRTMIN[1]) : 0;
   double usw = 0., zzw = 0., ssi = 0., zw;
   if (gnegt) {
     zzw = log(t);
     usw = exp(P.XALPW - P.XBETAW / t - P.XGAMW * zzw);
     zzw = exp(P.XALPI - P.XBETAI / t - P.XGAMI * zzw);
+  }
+  if (x) {
+    if (gnegt) {
+      double zredin = redin_tab(t), zam3 = am3_tab(dmax(P.XFRMIN[27], t));
+      double q = (20000. - dmin(20000., x->zzz)) / 20000.;
+      zzw = zredin * dmax(0.1, pow4(q)) * zam3 * (0.0001 + 0.9999 * x->icldfr);
+      zzw = dmin(zzw, 50.E3);
+    }
+    *rvheni_mr = 0.;
+    if (gnegt) *cit = dmax(zzw + *cit, *cit);
+    return;
   }
   if (gnegt) {
     zzw = dmin(pabst / 2., zzw);
@@ -718,17 +765,18 @@ typedef struct {
 static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, double lvfact,
                             double lsfact, double pres, double cf, double sigma_rc, double *cit,
                             double pt, const double *pvart, step_state_t *S, double *pa,
-                            double *pb, double *rainfr) {
+                            double *pb, double *rainfr, const ocnd2_pt_t *x) {
   double *bu = S->bu, *rs_tend = S->rs_tend, *rg_tend = S->rg_tend;
   double zt = pt, zvart[7];
-  double ka = 0., dv = 0., ai = 0., cj = 0.;
+  double ka = 0., dv = 0., ai = 0., cj = 0., esi_tab = 0.;
+  double colf = 1.0, acrf = 1.0; /* ZCOLF, ZACRF (:323-324) */
   for (int jv = 0; jv < 7; ++jv) { zvart[jv] = pvart[jv]; pa[jv] = 0.; pb[jv] = 0.; }
   if (soft) {
     bu[IRVHENI_MR] = 0.; bu[IRRHONG_MR] = 0.; bu[IRIMLTC_MR] = 0.; bu[IRSRIMCG_MR] = 0.;
   } else {
     /* 2. nucleation (:180-193) */
     ice4_nucleation(ldc, zvart[ITH], pres, rhodref, exn, lsfact, zt, zvart[IRV], cit,
-                    &bu[IRVHENI_MR]);
+                    &bu[IRVHENI_MR], x);
     zvart[ITH] = zvart[ITH] + bu[IRVHENI_MR] * lsfact;
     zt = zvart[ITH] * exn;
     zvart[IRV] = zvart[IRV] - bu[IRVHENI_MR];
@@ -770,16 +818,41 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     pb[IRS] = pb[IRS] - bu[IRSRIMCG_MR];
     pb[IRG] = pb[IRG] + bu[IRRHONG_MR];
     pb[IRG] = pb[IRG] + bu[IRSRIMCG_MR];
-    /* derived fields (:303-346), not LOCND2 */
-    double zzw = exp(P.XALPI - P.XBETAI / zt - P.XGAMI * log(zt));
-    S->ssi = zvart[IRV] * (pres - zzw) / (P.XEPSILO * zzw) - 1.0;
-    ka = 2.38E-2 + 0.0071E-2 * (zt - P.XTT);
-    dv = 0.211E-4 * pow(zt / P.XTT, 1.94) * (P.XP00 / pres);
-    ai = sq(P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / (ka * P.XRV * sq(zt)) +
-         (P.XRV * zt) / (dv * zzw);
+    /* derived fields (:303-346): with OCND2 (LLTAB) es_i and AI from the TIWMX tables */
+    if (x) {
+      esi_tab = esati_tab(zt);
+      S->ssi = zvart[IRV] * (pres - esi_tab) / (P.XEPSILO * esi_tab) - 1.0;
+      ka = 2.38E-2 + 0.0071E-2 * (zt - P.XTT);
+      dv = 0.211E-4 * pow(zt / P.XTT, 1.94) * (P.XP00 / pres);
+      ai = aa2_tab(zt) + bb3_tab(zt) * pres;
+    } else {
+      double zzw = exp(P.XALPI - P.XBETAI / zt - P.XGAMI * log(zt));
+      S->ssi = zvart[IRV] * (pres - zzw) / (P.XEPSILO * zzw) - 1.0;
+      ka = 2.38E-2 + 0.0071E-2 * (zt - P.XTT);
+      dv = 0.211E-4 * pow(zt / P.XTT, 1.94) * (P.XP00 / pres);
+      ai = sq(P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / (ka * P.XRV * sq(zt)) +
+           (P.XRV * zt) / (dv * zzw);
+    }
     cj = P.XSCFAC * pow(rhodref, 0.3) / sqrt(1.718E-5 + 0.0049E-5 * (zt - P.XTT));
+    /* collision factors cloud water - rain (ZACRF) and - snow/graupel (ZCOLF), LLCOL with
+     * OCND2 (:321-345); PCONC = XCONC_LAND */
+    if (x) {
+      colf = 0.00001;
+      acrf = 0.00001;
+      double conc = P.XCONC_LAND * 0.000001;
+      if (zvart[IRC] > 1.0E-10) {
+        double rcw = 0.1 * pow(0.75 * zvart[IRC] * rhodref / (P.XPI * conc), 0.333);
+        double vt = rcw < 0.0065 ? 1.19E6 * sq(rcw) : 8000. * rcw;
+        vt = dmin(10., vt);
+        double st = dmax(0.01, 2. * (100. - vt) * vt / (P.XG * 10.));
+        if (st > 0.1) colf = dmax(0.01, dmin(1., 0.939 * pow(st, 2.657)));
+        if (zvart[IRR] > 1.0E-10 && rcw > 1.0E-5) {
+          double r20 = exp(rcw * 2000.);
+          acrf = (r20 - 1.) / (r20 + 1.);
+        }
+      }
+    }
   }
-  const double colf = 1.0, acrf = 1.0; /* ZCOLF, ZACRF without LOCND2 (:323-324) */
 
   /* ICE4_COMPUTE_PDF (mode_ice4_compute_pdf.F90), rc: 'PDF'+'SIGM', ri: 'NONE' */
   {
@@ -818,7 +891,9 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
                rs = zvart[IRS], rg = zvart[IRG];
   const double ssi = S->ssi;
 
-  /* ICE4_SLOW (mode_ice4_slow.F90:106-217), LNEWCOEFF=F, ZREDGR=ZREDSN=1 */
+  /* ICE4_SLOW (mode_ice4_slow.F90:106-217), LNEWCOEFF=F; deposition tuning ZREDSN/ZREDGR
+   * = XFRMIN(40)/XFRMIN(39) with OCND2 and not LMODICEDEP (:92-100), else 1 */
+  const double zredsn = x ? P.XFRMIN[40] : 1., zredgr = x ? P.XFRMIN[39] : 1.;
   if (zt < P.XTT - 35.0 && rc > P.XRTMIN[2] && ldc) {
     if (!soft)
       bu[IRCHONI] = dmin(1000., P.XHON * rhodref * rc * exp(P.XALPHA3 * (zt - P.XTT) - P.XBETA3));
@@ -827,7 +902,7 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     if (!soft) {
       bu[IRVDEPS] = (ssi / (rhodref * ai)) *
                     (P.X0DEPS * pow(lbdas, P.XEX0DEPS) + P.X1DEPS * cj * pow(lbdas, P.XEX1DEPS));
-      bu[IRVDEPS] = bu[IRVDEPS] * 1.;
+      bu[IRVDEPS] = bu[IRVDEPS] * zredsn;
     }
   } else bu[IRVDEPS] = 0.;
   if (ri > P.XRTMIN[4] && rs > P.XRTMIN[5] && ldc) {
@@ -846,9 +921,16 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     if (!soft) {
       bu[IRVDEPG] = (ssi / (rhodref * ai)) *
                     (P.X0DEPG * pow(lbdag, P.XEX0DEPG) + P.X1DEPG * cj * pow(lbdag, P.XEX1DEPG));
-      bu[IRVDEPG] = bu[IRVDEPG] * 1.;
+      bu[IRVDEPG] = bu[IRVDEPG] * zredgr;
     }
   } else bu[IRVDEPG] = 0.;
+
+  /* graupel deposition partly to snow with OCND2 (mode_ice4_tendencies.F90:445-457) */
+  double zdepg_s = 0.0, znodepg_s = 1.0;
+  if (x && P.XFRMIN[5] > 1.0E-12 && P.XFRMIN[6] > 0.01) {
+    zdepg_s = dmax(0., dmin(1., (P.XFRMIN[5] - rg / P.XFRMIN[5]))) * dmax(0., dmin(1., (ssi / P.XFRMIN[6])));
+    znodepg_s = 1. - zdepg_s;
+  }
 
   /* ICE4_WARM (mode_ice4_warm.F90:100-197), HSUBG_RC_RR_ACCR='NONE', HSUBG_RR_EVAP='NONE' */
   if (S->hlc_hrc > P.XRTMIN[2] && S->hlc_hcf > 0. && ldc) {
@@ -926,11 +1008,13 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
                    (P.XLBRACCS1 / (sq(lbdas)) + P.XLBRACCS2 / (lbdas * lbdar) +
                     P.XLBRACCS3 / (sq(lbdar))) / pow4(lbdar);
       rs_tend[RS_RRACCSS] = zzw1 * zzw;
+      if (x) rs_tend[RS_RRACCSS] = rs_tend[RS_RRACCSS] * P.XFRMIN[7]; /* :327 */
       rs_tend[RS_RRACCS] = zzw2 * zzw;
-      rs_tend[RS_RSACCRG] = P.XFSACCRG * zzw3 * (pow(lbdas, P.XCXS - P.XBS)) *
-                            (pow(rhodref, -P.XCEXVT - 1.)) *
-                            (P.XLBSACCR1 / (sq(lbdar)) + P.XLBSACCR2 / (lbdar * lbdas) +
-                             P.XLBSACCR3 / (sq(lbdas))) / lbdar;
+      if (!x || rs > P.XFRMIN[1]) /* :347 */
+        rs_tend[RS_RSACCRG] = P.XFSACCRG * zzw3 * (pow(lbdas, P.XCXS - P.XBS)) *
+                              (pow(rhodref, -P.XCEXVT - 1.)) *
+                              (P.XLBSACCR1 / (sq(lbdar)) + P.XLBSACCR2 / (lbdar * lbdas) +
+                               P.XLBSACCR3 / (sq(lbdas))) / lbdar;
     }
   }
   if (gacc && zt < P.XTT) {
@@ -993,15 +1077,16 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     rg_tend[RG_RIDRYG] = 0.; rg_tend[RG_RIWETG] = 0.;
   }
   int gdry = rs > P.XRTMIN[5] && rg > P.XRTMIN[6] && ldc;
-  if (!gdry) { rg_tend[RG_RSDRYG] = 0.; rg_tend[RG_RSWETG] = 0.; }
+  if (!gdry || x) { rg_tend[RG_RSDRYG] = 0.; rg_tend[RG_RSWETG] = 0.; } /* OCND2: :185-188 */
   if (!soft && gdry) {
     double zzw;
     interp2d(lbdag, lbdas, P.NDRYLBDAG, P.NDRYLBDAS, P.XDRYINTP1G, P.XDRYINTP2G, P.XDRYINTP1S,
              P.XDRYINTP2S, P.KER_SDRYG, &zzw, NULL, NULL, NULL, NULL);
-    rg_tend[RG_RSWETG] = P.XFSDRYG * zzw / P.XCOLSG * (pow(lbdas, P.XCXS - P.XBS)) *
-                         (pow(lbdag, P.XCXG)) * (pow(rhodref, -P.XCEXVT - 1.)) *
-                         (P.XLBSDRYG1 / (sq(lbdag)) + P.XLBSDRYG2 / (lbdag * lbdas) +
-                          P.XLBSDRYG3 / (sq(lbdas)));
+    if (!x) /* OCND2 disregards the snow-graupel collision (:207-209) */
+      rg_tend[RG_RSWETG] = P.XFSDRYG * zzw / P.XCOLSG * (pow(lbdas, P.XCXS - P.XBS)) *
+                           (pow(lbdag, P.XCXG)) * (pow(rhodref, -P.XCEXVT - 1.)) *
+                           (P.XLBSDRYG1 / (sq(lbdag)) + P.XLBSDRYG2 / (lbdag * lbdas) +
+                            P.XLBSDRYG3 / (sq(lbdas)));
     rg_tend[RG_RSDRYG] = rg_tend[RG_RSWETG] * P.XCOLSG * exp(P.XCOLEXSG * (zt - P.XTT));
   }
   gdry = rr > P.XRTMIN[3] && rg > P.XRTMIN[6] && ldc;
@@ -1074,8 +1159,10 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     }
   } else bu[IRGMLTR] = 0.;
 
-  /* ICE4_FAST_RI (mode_ice4_fast_ri.F90:70-81) */
-  if (ssi > 0. && rc > P.XRTMIN[2] && ri > P.XRTMIN[4] && *cit > 1.E-20 && ldc) {
+  /* ICE4_FAST_RI (mode_ice4_fast_ri.F90:70-81); not called with OCND2 (tendencies :568-578) */
+  if (x) {
+    bu[IRCBERI] = 0.0;
+  } else if (ssi > 0. && rc > P.XRTMIN[2] && ri > P.XRTMIN[4] && *cit > 1.E-20 && ldc) {
     if (!soft) {
       double l = dmin(1.E8, P.XLBI * pow(rhodref * ri / *cit, P.XLBEXI));
       bu[IRCBERI] = (ssi / (rhodref * ai)) * *cit *
@@ -1083,7 +1170,38 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     }
   } else bu[IRCBERI] = 0.;
 
-  /* aggregation into rates PA and instantaneous changes PB (:606-717), KRR=6, no LOCND2 */
+  /* ICE4_FAST_RI_RS (mode_ice4_fast_ri_rs.F90:4-263) with OCND2, LMODICEDEP=F: conversion of
+   * large ice crystals to snow and deposition on non-spherical crystals, in and out of the
+   * ice cloud (tendencies :585-596); both are 0 in soft calls (:137,:168). */
+  bu[IRILARS] = 0.;
+  bu[IRVDEPI] = 0.;
+  if (x) {
+    const double ztimesc = P.XFRMIN[14], zdicrit = P.XFRMIN[15], zwcitred = 0.1;
+    if (!soft && ri > 0.0 && ssi > 0.001)
+      bu[IRILARS] = dmin(ri, ri / (87.5 * sq(zdicrit) * ai / ssi));
+    const double zw2d = 1. / (x->ifr * x->icldfr + 1. - x->icldfr);
+    if (!soft && (pres * 0.5 - esi_tab) > 0. && *cit > 0. && ldc) {
+      double xw2d13 = pow(x->ifr, 0.333);
+      double zwcitred23 = pow(zwcitred, 0.667);
+      double ztc = dmax(-18., dmin(-1., zt - P.XTT));
+      double zhu = dmin(0.15, dmax(0., ssi));
+      double zcrysha = 1.1 + 3. * zhu * (1. + sin(0.64 * ztc - 1.3));
+      double zqimax = 366.5 * (zdicrit * zdicrit * zdicrit) * *cit * zwcitred / rhodref;
+      double zci2s = 0.;
+      if (ri > 1.0e-12 && ztimesc > 0.)
+        zci2s = ri * (1. - dmin(1., 0.5 * zqimax / ri)) * (1. - x->icldfr) * zw2d / (ztimesc * 0.5);
+      double zzwc = zcrysha * 0.878 / ai * pow(*cit / rhodref, 0.667) *
+                    pow(dmax(P.XRTMIN[4], ri) * zw2d, 0.333);
+      if (x->ssio > 0. && x->icldfr > 0.02)
+        bu[IRVDEPI] = bu[IRVDEPI] + zzwc * xw2d13 * x->ssio * x->icldfr;
+      if (x->ssiu < 0. && x->icldfr < 0.98) {
+        bu[IRILARS] = bu[IRILARS] + zci2s;
+        bu[IRVDEPI] = bu[IRVDEPI] + zzwc * zwcitred23 * x->ssiu * (1. - x->icldfr);
+      }
+    }
+  }
+
+  /* aggregation into rates PA and instantaneous changes PB (:606-717), KRR=6 */
   double ls_lv = lsfact - lvfact;
   pa[ITH] = pa[ITH] + bu[IRVDEPG] * lsfact;
   pa[ITH] = pa[ITH] + bu[IRCHONI] * ls_lv;
@@ -1098,10 +1216,12 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   pa[ITH] = pa[ITH] + (bu[IRCDRYG] + bu[IRRDRYG]) * ls_lv;
   pa[ITH] = pa[ITH] - bu[IRGMLTR] * ls_lv;
   pa[ITH] = pa[ITH] + bu[IRCBERI] * ls_lv;
+  if (x) pa[ITH] = pa[ITH] + bu[IRVDEPI] * lsfact;
 
   pa[IRV] = pa[IRV] - bu[IRVDEPG];
   pa[IRV] = pa[IRV] - bu[IRVDEPS];
   pa[IRV] = pa[IRV] + bu[IRREVAV];
+  if (x) pa[IRV] = pa[IRV] - bu[IRVDEPI];
 
   pa[IRC] = pa[IRC] - bu[IRCHONI];
   pa[IRC] = pa[IRC] - bu[IRCAUTR];
@@ -1131,8 +1251,11 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   pa[IRI] = pa[IRI] - bu[IRIWETG];
   pa[IRI] = pa[IRI] - bu[IRIDRYG];
   pa[IRI] = pa[IRI] + bu[IRCBERI];
+  if (x) pa[IRI] = pa[IRI] + bu[IRVDEPI];
+  if (x) pa[IRI] = pa[IRI] - bu[IRILARS];
 
   pa[IRS] = pa[IRS] + bu[IRVDEPS];
+  if (x) pa[IRS] = pa[IRS] + bu[IRVDEPG] * zdepg_s;
   pa[IRS] = pa[IRS] + bu[IRIAGGS];
   pa[IRS] = pa[IRS] + bu[IRIAUTS];
   pa[IRS] = pa[IRS] + bu[IRCRIMSS];
@@ -1142,8 +1265,9 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   pa[IRS] = pa[IRS] - bu[IRSMLTG];
   pa[IRS] = pa[IRS] - bu[IRSWETG];
   pa[IRS] = pa[IRS] - bu[IRSDRYG];
+  if (x) pa[IRS] = pa[IRS] + bu[IRILARS];
 
-  pa[IRG] = pa[IRG] + bu[IRVDEPG] * 1.0;
+  pa[IRG] = pa[IRG] + bu[IRVDEPG] * znodepg_s;
   pa[IRG] = pa[IRG] + bu[IRCRIMSG] + bu[IRSRIMCG];
   pa[IRG] = pa[IRG] + bu[IRRACCSG] + bu[IRSACCRG];
   pa[IRG] = pa[IRG] + bu[IRSMLTG];
@@ -1161,7 +1285,7 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
  * XTSTEP_TS=0, XMRSTEP=0, LFEEDBACKT=T, LDEXT_TEND=T, LDSIGMA_RC=T. */
 static int ice4_stepping_point(double dt, double exn, double rhodref, double pres, double cf,
                                double sigs, double *cit, double *vart, double *extpk,
-                               double *rainfr, double *rrevav, int *ncalls) {
+                               double *rainfr, double *rrevav, int *ncalls, const ocnd2_pt_t *x) {
   step_state_t S;
   memset(&S, 0, sizeof(S));
   double zinv = 1. / dt;
@@ -1181,7 +1305,7 @@ static int ice4_stepping_point(double dt, double exn, double rhodref, double pre
       double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
       double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
       ice4_tendencies(soft, llcompute, exn, rhodref, lvfact, lsfact, pres, cf, sigma_rc, cit, zzt,
-                      vart, &S, za, zb, rainfr);
+                      vart, &S, za, zb, rainfr, x);
       for (int jv = 0; jv <= 6; ++jv) za[jv] = za[jv] + extpk[jv];          /* :320-324 */
       double maxtime = llcompute ? (dt - time) : 0.;                        /* :335-341 */
       double zx = P.XTT / exn;                                              /* :344-360 */
@@ -1261,14 +1385,30 @@ static void sedimentation_column(int ncol, int nlev, int i, double dt, const dou
 }
 
 /* RAIN_ICE (micro/rain_ice.F90:7-830) for KRR=6 with the commented-out
- * sedimentation (:336-348), total tendencies (:649-684) and negativity
- * correction (:712-714) reinstated. */
-int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
-                    double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
-                    int *iters) {
+ * sedimentation (:442-457), total tendencies (:754-793) and negativity
+ * correction (:821-823) reinstated.  x4 (NULL for the plain branch) holds the OCND2
+ * inputs PICLDFR, PSSIO, PSSIU, PIFR. */
+static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
+                         const double *const *x4, double *const io[ORACLE_RAIN_NIO],
+                         double *const col[ORACLE_RAIN_NCOL], int *iters) {
   if (!P_loaded) return -1;
   int status = 0;
   double zinv = 1. / dt;
+  /* 1.1 model level height ZZZZ (:387-395): from the top level down, as the reference
+   * accumulates PDZZ over JK = IKTE..IKTB */
+  double *zzzz = NULL;
+  if (x4) {
+    zzzz = (double *)malloc(sizeof(double) * (size_t)ncol * nlev);
+    if (!zzzz) return -3;
+    for (int i = 0; i < ncol; ++i) {
+      double zzsum = 0.;
+      for (int k = nlev - 1; k >= 0; --k) {
+        size_t j = (size_t)k * ncol + i;
+        zzsum = zzsum + in[RI_DZZ][j];
+        zzzz[j] = zzsum - in[RI_DZZ][j] * 0.5;
+      }
+    }
+  }
   /* 2. sedimentation before microphysics, per column */
   for (int i = 0; i < ncol; ++i) {
     col[RC_INPRC][i] = 0.;
@@ -1287,19 +1427,29 @@ int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE
       double zt = tht * exn;
       double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide;
       double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide;
-      int llmicro = rct > P.XRTMIN[2] || rrt > P.XRTMIN[3] || rit > P.XRTMIN[4] ||
-                    rst > P.XRTMIN[5] || rgt > P.XRTMIN[6];
-      /* 3. saved state (:357-381) */
+      ocnd2_pt_t xp, *x = NULL;
+      int llmicro;
+      if (x4) {
+        xp.icldfr = x4[0][j]; xp.ssio = x4[1][j]; xp.ssiu = x4[2][j]; xp.ifr = x4[3][j];
+        xp.zzz = zzzz[j];
+        x = &xp;
+        llmicro = xp.ssio > P.XFRMIN[12] || rct > P.XFRMIN[13] || rrt > P.XFRMIN[13] ||
+                  rit > P.XFRMIN[13] || rst > P.XFRMIN[13] || rgt > P.XFRMIN[13];
+      } else {
+        llmicro = rct > P.XRTMIN[2] || rrt > P.XRTMIN[3] || rit > P.XRTMIN[4] ||
+                  rst > P.XRTMIN[5] || rgt > P.XRTMIN[6];
+      }
+      /* 3. saved state (:466-490) */
       double wr[7] = {tht, rvt, rct, rrt, rit, rst, rgt};
       double evap3d = 0., rainfr;
       double cit = io[RO_CI][j];
-      /* 4.1 nucleation outside the mask (:395-431) */
+      /* 4.1 nucleation outside the mask (:505-540) */
       double rvheni = 0.;
       if (!llmicro) cit = 0.;
       ice4_nucleation(!llmicro, tht, pabs, rhodref, exn, llmicro ? 0. : lsfact / exn, zt, rvt,
-                      &cit, &rvheni);
+                      &cit, &rvheni, x);
       rvheni = dmin(io[RO_RVS][j], rvheni / dt);
-      rainfr = 1.; /* 4.2 (:495) */
+      rainfr = 1.; /* 4.2 (:555-608) */
       /* 5. tendencies (ICE4_PACK -> ICE4_STEPPING), external tendencies = sources */
       int ncalls = 0;
       if (llmicro) {
@@ -1307,12 +1457,12 @@ int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE
                            io[RO_RIS][j], io[RO_RSS][j], io[RO_RGS][j]};
         double rrevav = 0.;
         if (ice4_stepping_point(dt, exn, rhodref, pabs, in[RI_CLDFR][j], in[RI_SIGS][j], &cit, wr,
-                                extpk, &rainfr, &rrevav, &ncalls) != 0)
+                                extpk, &rainfr, &rrevav, &ncalls, x) != 0)
           status = -2;
         evap3d = rrevav;
       }
       if (iters) iters[j] = ncalls;
-      /* 8.1 total tendencies (:649-684) */
+      /* 8.1 total tendencies (:754-793) */
       double exnref = in[RI_EXNREF][j];
       lsfact = lsfact / exnref;
       lvfact = lvfact / exnref;
@@ -1327,7 +1477,7 @@ int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE
       double ris = io[RO_RIS][j] + wri + rvheni;
       double rss = io[RO_RSS][j] + wrs;
       double rgs = io[RO_RGS][j] + wrg;
-      /* 8.2 ICE4_CORRECT_NEGATIVITIES (mode_ice4_correct_negativities.F90:51-171) */
+      /* 8.2 ICE4_CORRECT_NEGATIVITIES (:821-823; mode_ice4_correct_negativities.F90:51-171) */
       double zw;
       zw = rcs - dmax(rcs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rcs = rcs - zw;
       zw = rrs - dmax(rrs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rrs = rrs - zw;
@@ -1349,5 +1499,19 @@ int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE
       io[RO_RIS][j] = ris; io[RO_RSS][j] = rss; io[RO_RGS][j] = rgs; io[RO_CI][j] = cit;
       io[RO_EVAP3D][j] = evap3d; io[RO_RAINFR][j] = rainfr;
     }
+  free(zzzz);
   return status;
+}
+
+int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
+                    double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
+                    int *iters) {
+  return rain_ice_core(ncol, nlev, dt, in, NULL, io, col, iters);
+}
+
+int oracle_rain_ice_ocnd2(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
+                          const double *const x4[ORACLE_RAIN_NOCND2],
+                          double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
+                          int *iters) {
+  return rain_ice_core(ncol, nlev, dt, in, x4, io, col, iters);
 }

@@ -62,6 +62,15 @@ int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE
                     double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
                     int *iters);
 
+/* RAIN_ICE with PARAMI%LOCND2 = .TRUE.: as oracle_rain_ice, plus ICE_ADJUST's OCND2 outputs
+ * x4 = {PICLDFR, PSSIO, PSSIU, PIFR}, each (nlev, ncol).  Returns as oracle_rain_ice, or -3
+ * when a work array cannot be allocated. */
+enum { RX_ICLDFR, RX_SSIO, RX_SSIU, RX_IFR, ORACLE_RAIN_NOCND2 };
+int oracle_rain_ice_ocnd2(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
+                          const double *const x4[ORACLE_RAIN_NOCND2],
+                          double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
+                          int *iters);
+
 #ifdef __cplusplus
 }
 #endif

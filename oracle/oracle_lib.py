@@ -112,3 +112,30 @@ def rain_ice(s: dict, dt: float = 50.0, with_iters: bool = False) -> dict:
     if with_iters:
         out["iters"] = iters
     return out
+
+
+RAIN_OCND2 = ("icldfr", "ssio", "ssiu", "ifr")
+
+
+def rain_ice_ocnd2(s: dict, dt: float = 50.0, with_iters: bool = False) -> dict:
+    """RAIN_ICE with LOCND2; ``s`` as for rain_ice plus ICE_ADJUST's OCND2 outputs
+    ``icldfr, ssio, ssiu, ifr``."""
+    lib = _load()
+    nlev, ncol = s["tht"].shape
+    ins = [_chk(s[k]) for k in RAIN_IN]
+    x4 = [_chk(s[k]) for k in RAIN_OCND2]
+    io = {k: (np.array(s[k], dtype=np.float64, order="C") if k in s else np.zeros((nlev, ncol)))
+          for k in RAIN_IO}
+    col = {k: np.zeros(ncol) for k in RAIN_COL}
+    iters = np.zeros((nlev, ncol), dtype=np.int32)
+    rc = lib.oracle_rain_ice_ocnd2(ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt), _arr(ins),
+                                   _arr(x4), _arr([io[k] for k in RAIN_IO]), _arr([col[k] for k in RAIN_COL]),
+                                   ctypes.c_void_p(iters.ctypes.data))
+    if rc in (-1, -3):
+        raise RuntimeError(f"oracle_rain_ice_ocnd2 failed: {rc}")
+    out = dict(io)
+    out.update(col)
+    out["status"] = rc
+    if with_iters:
+        out["iters"] = iters
+    return out

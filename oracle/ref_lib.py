@@ -165,6 +165,42 @@ def _rain_ice(s: dict, dt: float, nproma: int) -> dict:
     return io
 
 
+def rain_ice_ocnd2(s: dict, dt: float = 50.0, nproma: int = 1) -> dict:
+    """RAIN_ICE with LOCND2 = .TRUE.; ``s`` holds rain_ice's fields plus ICE_ADJUST's OCND2
+    outputs ``icldfr, ssio, ssiu, ifr``."""
+    return _chunked(_rain_ice_ocnd2, s, dt, nproma)
+
+
+def _rain_ice_ocnd2(s: dict, dt: float, nproma: int) -> dict:
+    lib = init(dt, nproma)
+    nlev, ncol = s["tht"].shape
+    io = {k: s[k].copy() for k in ("ci", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
+                                   "ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs")}
+    col = {k: np.zeros(ncol) for k in ("inprc", "inprr", "inpri", "inprs", "inprg")}
+    o2 = {k: np.zeros((nlev, ncol)) for k in ("evap3d", "rainfr")}
+    lib.ref_rain_ice_ocnd2(
+        ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt),
+        *[_p(s[k]) for k in ("exn", "dzz", "rhodj", "rhodref", "exn_ref", "pabs")],
+        _p(io["ci"]), _p(s["cldfr"]),
+        *[_p(io[k]) for k in ("hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf")],
+        *[_p(np.ascontiguousarray(s[k], dtype=np.float64)) for k in ("icldfr", "ssio", "ssiu", "ifr")],
+        *[_p(s[k]) for k in ("tht", "rvt", "rct", "rrt", "rit", "rst", "rgt", "sigs")],
+        *[_p(io[k]) for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs")],
+        *[_p(col[k]) for k in ("inprc", "inprr", "inpri", "inprs", "inprg")],
+        _p(o2["evap3d"]), _p(o2["rainfr"]))
+    io.update(col)
+    io.update(o2)
+    return io
+
+
+def pipeline_state_ocnd2(f: dict, adj: dict, dt: float = 50.0) -> dict:
+    """rain_ice inputs of the fused step with OCND2: pipeline_state plus ICE_ADJUST's OCND2
+    outputs (adj from ice_adjust_ocnd2)."""
+    s = pipeline_state(f, adj, dt)
+    s.update({k: adj[k] for k in ("icldfr", "ssio", "ssiu", "ifr")})
+    return s
+
+
 def pipeline_state(f: dict, adj: dict, dt: float = 50.0) -> dict:
     """rain_ice inputs for the fused ice_adjust + rain_ice workload.
 

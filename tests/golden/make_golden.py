@@ -17,8 +17,12 @@ Runs in the build container only (it needs oracle/_ref/libphyex_ref.so, built by
 * ``ice_adjust_ocnd2_fp64.npz``  (``python3 tests/golden/make_golden.py ocnd2``) ICE_ADJUST
   with PARAMI%LOCND2 = .TRUE. on the "adjust" generator plus level heights and a per-column
   ice-cloud weight, and a small case with mass-flux inputs; keys ``<tag>_in_*``/``<tag>_out_*``.
+* ``rain_ice_ocnd2_fp64.npz``  (``python3 tests/golden/make_golden.py rain_ocnd2``) the fused
+  step with LOCND2: ICE_ADJUST with OCND2, then RAIN_ICE with OCND2 on its adjusted sources
+  and its PICLDFR, PSSIO, PSSIU, PIFR; the "pipeline" generator (``base``, 32 columns) and the
+  precipitation-band coverage case (``cov``, 16 columns); keys ``<tag>_in_*``/``<tag>_out_*``.
 
-Usage: python3 tests/golden/make_golden.py [ocnd2]
+Usage: python3 tests/golden/make_golden.py [ocnd2 | rain_ocnd2]
 """
 from __future__ import annotations
 
@@ -90,6 +94,48 @@ def ocnd2():
     np.savez_compressed(os.path.join(HERE, "ice_adjust_ocnd2_fp64.npz"), dt=DT, **out)
 
 
+def band_case(ncol, seed):
+    """The coverage case's columns: the "adjust" generator with one 20-level precipitating
+    band per column, band bases spread from the ground to level 70."""
+    f = synthetic.generate(ncol, 90, seed=seed, variant="adjust", dt=DT)
+    rng = np.random.Generator(np.random.Philox(seed + 1))
+    lev = np.arange(90)[:, None]
+    start = np.linspace(0, 70, ncol).astype(int)[None, :]
+    band = (lev >= start) & (lev < start + 20)
+    for k in ("rr", "rs", "rg"):
+        f[k] = np.ascontiguousarray(np.where(band, rng.uniform(1.0e-4, 5.0e-3, band.shape), f[k]))
+        f[k + "s"] = f[k] / DT
+    return f
+
+
+def with_heights(f, seed):
+    """Level heights zz (the generator's hydrostatic heights) and PICE_CLD_WGT U(0.5, 1.5)."""
+    ncol = f["th"].shape[1]
+    f["zz"] = np.ascontiguousarray(np.broadcast_to(synthetic.level_profile(90)[2][:, None], f["th"].shape))
+    f["ice_cld_wgt"] = np.random.Generator(np.random.Philox(seed)).uniform(0.5, 1.5, ncol)
+    return f
+
+
+RAIN_OCND2_IN = RAIN_IN + ("icldfr", "ssio", "ssiu", "ifr")
+
+
+def rain_ocnd2():
+    ref_lib.init(DT, 1)
+    out = {}
+    cases = (("base", with_heights(synthetic.generate(32, 90, seed=20260041, variant="pipeline", dt=DT),
+                                   20260042)),
+             ("cov", with_heights(band_case(16, 20260043), 20260045)))
+    for tag, f in cases:
+        a = ref_lib.ice_adjust_ocnd2(f, DT)
+        s = ref_lib.pipeline_state_ocnd2(f, a, DT)
+        r = ref_lib.rain_ice_ocnd2(s, DT, nproma=1)
+        for k in RAIN_OCND2_IN:
+            out[f"{tag}_in_{k}"] = s[k]
+        for k in RAIN_OUT:
+            out[f"{tag}_out_{k}"] = r[k]
+    np.savez_compressed(os.path.join(HERE, "rain_ice_ocnd2_fp64.npz"), dt=DT, **out)
+
+
 def main():
     ref_lib.init(DT, 1)
     ref_lib.dump_constants(os.path.join(HERE, "constants.txt"), DT)
@@ -136,4 +182,4 @@ def main():
 
 
 if __name__ == "__main__":
-    ocnd2() if sys.argv[1:] == ["ocnd2"] else main()
+    {"ocnd2": ocnd2, "rain_ocnd2": rain_ocnd2}.get(sys.argv[1] if sys.argv[1:] else "", main)()

@@ -66,3 +66,12 @@ def check_with_flips(out, ref, names, flips, tol, max_bad=2e-3, max_flips=1e-2):
     over = {k: v for k, v in worst.items() if not v <= tol}
     assert not over, over
     return worst, int(flips.sum()), int(bad.sum())
+
+
+def flips_of(out, ref, calls="calls", iters="iters"):
+    """Branch flips of the ICE4 time splitting between two runs of the same points: a
+    different number of tendency calls, or the pristine-ice concentration reset to 0 on one
+    side only (RAIN_ICE sets cit = 0 when ri ends a sub-step at or below 0, and where a
+    process such as OCND2's RILARS empties the ice, the sign of ri's rounding residual
+    decides it; micro/mode_ice4_stepping.F90:471-474)."""
+    return (out[calls] != ref[iters]) | ((out["ci"] == 0.0) != (ref["ci"] == 0.0))

@@ -64,6 +64,13 @@ def load():
         lib.hst_adjust_rain_ice.argtypes = [ctypes.c_int] + common + [
             ctypes.POINTER(L.PipeIn), ctypes.POINTER(L.PipeOut), ctypes.POINTER(L.Stats)]
         lib.hst_set_substep_cap.argtypes = [ctypes.c_int]
+        lib.hst_tiwmx.argtypes = [dp]
+        lib.hst_rain_ice_ocnd2.argtypes = common + [
+            ctypes.POINTER(L.RainIn), ctypes.POINTER(L.RainOcnd2In), ctypes.POINTER(L.RainIO),
+            ctypes.POINTER(L.RainOut), ctypes.POINTER(L.Stats)]
+        lib.hst_adjust_rain_ice_ocnd2.argtypes = common + [
+            ctypes.POINTER(L.PipeIn), ctypes.POINTER(L.Ocnd2In), ctypes.POINTER(L.PipeOut),
+            ctypes.POINTER(L.Ocnd2Out), ctypes.POINTER(L.Stats)]
         _lib = lib
     return _lib
 
@@ -154,3 +161,56 @@ def elementwise(fn: str, x):
     dp = ctypes.POINTER(ctypes.c_double)
     getattr(lib, f"hst_{fn}_f64")(x.ctypes.data_as(dp), y.ctypes.data_as(dp), x.size)
     return y
+
+
+def adjust_rain_ice_ocnd2(f: dict, dt: float = 50.0):
+    """The fused step with OCND2 on host arrays: the ice3_pipeline_in fields plus zz and
+    ice_cld_wgt.  Returns the pipeline outputs, the OCND2 outputs, calls and counters."""
+    import numpy as np
+    from ice3_amd import _lib as L
+    lib = load()
+    nlev, ncol = f["th"].shape
+    inp = {k: np.ascontiguousarray(f[k], dtype=np.float64) for k in L.PIPE_IN_FIELDS}
+    zz, wgt = np.ascontiguousarray(f["zz"], dtype=np.float64), np.ascontiguousarray(f["ice_cld_wgt"], dtype=np.float64)
+    out = {k: np.empty((nlev, ncol)) for k in L.PIPE_OUT_FIELDS[:15]}
+    out.update({k: np.empty(ncol) for k in L.PIPE_OUT_FIELDS[15:]})
+    x = {k: np.empty((nlev, ncol)) for k in L.OCND2_OUT_FIELDS}
+    calls = np.zeros((nlev, ncol), dtype=np.int32)
+    counters = np.zeros(3, dtype=np.uint64)
+    cin = L.PipeIn(*[_p(inp[k]) for k in L.PIPE_IN_FIELDS])
+    cout = L.PipeOut(*[_p(out[k]) for k in L.PIPE_OUT_FIELDS])
+    st = L.Stats(_p(calls), _p(counters))
+    rc = lib.hst_adjust_rain_ice_ocnd2(ncol, nlev, ncol, dt, ctypes.byref(cin), ctypes.byref(L.Ocnd2In(_p(zz), _p(wgt))),
+                                       ctypes.byref(cout), ctypes.byref(L.Ocnd2Out(*[_p(x[k]) for k in L.OCND2_OUT_FIELDS])),
+                                       ctypes.byref(st))
+    assert rc == 0
+    out.update(x)
+    out["calls"] = calls
+    out["counters"] = counters
+    return out
+
+
+def rain_ice_ocnd2(s: dict, dt: float = 50.0):
+    """RAIN_ICE with OCND2 on host arrays (rain_ice's fields plus icldfr, ssio, ssiu, ifr)."""
+    import numpy as np
+    from ice3_amd import _lib as L
+    lib = load()
+    nlev, ncol = s["tht"].shape
+    names = {"exn_ref": "exn_ref"}
+    inp = {k: np.ascontiguousarray(s[names.get(k, k)], dtype=np.float64) for k in L.RAIN_IN_FIELDS}
+    x4 = [np.ascontiguousarray(s[k], dtype=np.float64) for k in L.RAIN_OCND2_IN_FIELDS]
+    io = {k: np.array(s[k if k != "ci" else "ci"], dtype=np.float64, order="C") for k in L.RAIN_IO_FIELDS}
+    out = {k: np.empty((nlev, ncol)) for k in ("evap3d", "rainfr")}
+    out.update({k: np.empty(ncol) for k in ("inprc", "inprr", "inpri", "inprs", "inprg")})
+    calls = np.zeros((nlev, ncol), dtype=np.int32)
+    counters = np.zeros(3, dtype=np.uint64)
+    rc = lib.hst_rain_ice_ocnd2(ncol, nlev, ncol, dt, ctypes.byref(L.RainIn(*[_p(inp[k]) for k in L.RAIN_IN_FIELDS])),
+                                ctypes.byref(L.RainOcnd2In(*[_p(a) for a in x4])),
+                                ctypes.byref(L.RainIO(*[_p(io[k]) for k in L.RAIN_IO_FIELDS])),
+                                ctypes.byref(L.RainOut(*[_p(out[k]) for k in L.RAIN_OUT_FIELDS])),
+                                ctypes.byref(L.Stats(_p(calls), _p(counters))))
+    assert rc == 0
+    io.update(out)
+    io["calls"] = calls
+    io["counters"] = counters
+    return io

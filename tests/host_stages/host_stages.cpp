@@ -25,11 +25,11 @@ extern "C" int hst_set_substep_cap(int max_calls) {
 extern "C" int hst_init(void) {
   if (!g_tab || !g_tiwmx) {
     double *tab = (double *)malloc(sizeof(double) * ICE3_T_TOTAL);
-    double *tiwmx = (double *)malloc(sizeof(double) * 2 * ICE3_TIWMX_N);
+    double *tiwmx = (double *)malloc(sizeof(double) * ICE3_TW_NTAB * ICE3_TIWMX_N);
     ice3_params p;
     if (tab && tiwmx) {
       ice3_build_params(&p, tab);
-      ice3_build_tiwmx(&p, tiwmx, tiwmx + ICE3_TIWMX_N);
+      ice3_build_tiwmx(&p, tiwmx);
     }
     if (!tab || !tiwmx || memcmp(&p, &P, sizeof p) != 0) {  // no memory, or stale constants
       free(tab);
@@ -42,13 +42,21 @@ extern "C" int hst_init(void) {
   return 0;
 }
 
-template <class R>
+// The TIWMX tables the kernels read (ice3_tiwmx_tables order), copied into out
+// (ICE3_TW_NTAB * ICE3_TIWMX_N doubles).
+extern "C" int hst_tiwmx(double *out) {
+  if (hst_init()) return -1;
+  memcpy(out, g_tiwmx, sizeof(double) * ICE3_TW_NTAB * ICE3_TIWMX_N);
+  return ICE3_TW_NTAB;
+}
+
+template <class R, bool OCND2 = false>
 static void run_rain_ice(const SedimArgs &sa, const MicroArgs &m) {
-  for (int i = 0; i < sa.ncol; ++i) sedim_column<R>(sa, i);
+  for (int i = 0; i < sa.ncol; ++i) sedim_column<R, OCND2>(sa, i);
   unsigned long long n = 0, c = 0, p = 0;
   for (int k = 0; k < m.nlev; ++k)
     for (int i = 0; i < m.ncol; ++i) {
-      const int calls = micro_at<R>(m, (size_t)k * m.ld + i, m.tab);
+      const int calls = micro_at<R, OCND2>(m, (size_t)k * m.ld + i, m.tab);
       n += calls < 0 ? -calls : calls;
       c += calls < 0;
       p += calls != 0;
@@ -104,6 +112,41 @@ extern "C" int hst_adjust_rain_ice(int prec, int ncol, int nlev, int ld, double 
     for (int i = 0; i < ncol; ++i) adjust_at(aa, i, (size_t)k * ld + i);
   if (prec == ICE3_PREC_FP64) run_rain_ice<double>(sa, m);
   else run_rain_ice<float>(sa, m);
+  return 0;
+}
+
+extern "C" int hst_rain_ice_ocnd2(int ncol, int nlev, int ld, double dt, const ice3_rain_ice_in *in,
+                                  const ice3_rain_ocnd2_in *in2, const ice3_rain_ice_inout *io,
+                                  const ice3_rain_ice_out *out, const ice3_stats *stats) {
+  if (hst_init()) return -1;
+  SedimArgs sa;
+  MicroArgs m;
+  rain_ice_args(ncol, nlev, ld, dt, g_tab, in, io, out, stats, sa, m);
+  m.max_calls = g_max_calls;
+  m.icldfr = in2->icldfr; m.ssio = in2->ssio; m.ssiu = in2->ssiu; m.ifr = in2->ifr;
+  m.tiwmx = g_tiwmx;
+  sa.zzz = out->rainfr;
+  run_rain_ice<double, true>(sa, m);
+  return 0;
+}
+
+extern "C" int hst_adjust_rain_ice_ocnd2(int ncol, int nlev, int ld, double dt,
+                                         const ice3_pipeline_in *in, const ice3_ocnd2_in *in2,
+                                         const ice3_pipeline_out *out, const ice3_ocnd2_out *out2,
+                                         const ice3_stats *stats) {
+  if (hst_init()) return -1;
+  AdjustArgs aa;
+  SedimArgs sa;
+  MicroArgs m;
+  pipeline_args(ncol, nlev, ld, dt, g_tab, in, out, stats, aa, sa, m);
+  m.max_calls = g_max_calls;
+  m.icldfr = out2->icldfr; m.ssio = out2->ssio; m.ssiu = out2->ssiu; m.ifr = out2->ifr;
+  m.tiwmx = g_tiwmx;
+  sa.zzz = out->rainfr;
+  const Ocnd2Args x{*in2, *out2, g_tiwmx};
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) adjust_at<true>(aa, i, (size_t)k * ld + i, k, &x);
+  run_rain_ice<double, true>(sa, m);
   return 0;
 }
 

@@ -551,3 +551,119 @@ def test_fused_step_jax_layout_bitwise(dev):
             assert a.shape == (16, 16), k
             a = a.reshape(-1)
         assert np.array_equal(a, _np(v), equal_nan=True), k
+
+
+# ------------------------------------------------------------- RAIN_ICE with OCND2
+
+def _with_heights(f, seed):
+    from ice3_amd import synthetic
+    ncol = f["th"].shape[1]
+    f["zz"] = np.ascontiguousarray(np.broadcast_to(synthetic.level_profile(90)[2][:, None], f["th"].shape))
+    f["ice_cld_wgt"] = np.random.Generator(np.random.Philox(seed)).uniform(0.5, 1.5, ncol)
+    return f
+
+
+@pytest.mark.parametrize("tag", ["base", "cov"])
+def test_rain_ice_ocnd2_golden(dev, golden_dir, tag):
+    """RAIN_ICE with LOCND2 through RainIceOCND2 against the compiled reference's vectors,
+    flips counted (call counts and the ri <= 0 ice-concentration reset)."""
+    from helpers import flips_of
+    from ice3_amd import RainIceOCND2
+    from oracle import oracle_lib
+    g = np.load(os.path.join(golden_dir, "rain_ice_ocnd2_fp64.npz"))
+    s = {k[len(tag) + 4:]: g[k] for k in g.files if k.startswith(tag + "_in_")}
+    ref = {k: g[f"{tag}_out_{k}"] for k in RAIN_OUT}
+    state = _rain_state(s)
+    state.update({k: s[k] for k in ("icldfr", "ssio", "ssiu", "ifr")})
+    new, diags = RainIceOCND2()(state, float(g["dt"]), return_calls=True)
+    out = {k: _np(new[k]) for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "evap3d", "rainfr")}
+    out["ci"] = _np(new["ci_t"])
+    out.update({k: _np(v) for k, v in diags.items()})
+    ref["iters"] = oracle_lib.rain_ice_ocnd2(s, float(g["dt"]), with_iters=True)["iters"]
+    w, nf, nb = check_with_flips(out, ref, RAIN_OUT, flips_of(out, ref), TOL)
+    print(f"rain_ice OCND2 golden {tag}: {nf} flips, {nb} divergent of {out['calls'].size}, worst {max(w.values()):.2e}")
+
+
+def test_pipeline_ocnd2_vs_oracle(dev):
+    """The fused step with OCND2 in both halves on the benchmark generator (2,048 columns)
+    against the oracle, flips counted; the column water budget holds; the two OCND2 entry
+    points (fused, and ice_adjust then rain_ice) agree bit for bit."""
+    from helpers import flips_of
+    from ice3_amd import AdjustRainIceOCND2, IceAdjustOCND2, RainIceOCND2, synthetic
+    from oracle import oracle_lib, ref_lib
+    f = _with_heights(synthetic.generate(2048, 90, seed=20260048, variant="pipeline"), 20260049)
+    f["rhodref"] = f["rho_dry_ref"]
+    res = AdjustRainIceOCND2()(f, 50.0, return_calls=True)
+    out = {k: _np(v) for k, v in res.items()}
+    a = oracle_lib.ice_adjust_ocnd2(f)
+    ref = oracle_lib.rain_ice_ocnd2(ref_lib.pipeline_state_ocnd2(f, a), with_iters=True)
+    w, nf, nb = check_with_flips(out, ref, RAIN_OUT, flips_of(out, ref), TOL)
+    _check(out, a, ("cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf", "icldfr", "wcldfr", "ssio", "ssiu", "ifr"))
+    print(f"fused OCND2 step, 2048 columns: {nf} flips, {nb} divergent of {out['calls'].size}, "
+          f"worst {max(w.values()):.2e}")
+    assert _column_budget(dev, f, res) < 1e-12
+    # the same step through the two separate OCND2 entry points
+    names = ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc", "ri", "rr", "rs", "rg",
+             "cf_mf", "rc_mf", "ri_mf", "rvs", "rcs", "ris", "ths")
+    adj = IceAdjustOCND2()(f["sigqsat"], *[f[k] for k in names], timestep=50.0, zz=f["zz"],
+                            ice_cld_wgt=f["ice_cld_wgt"])
+    ad = dict(zip(ADJ_OUT + OCND2_X, adj))
+    t = lambda k: torch.as_tensor(f[k], device=dev)  # noqa: E731
+    state = dict(exn=t("exn"), dzz=t("dzz"), rhodj=t("rhodj"), rhodref=t("rho_dry_ref"), exnref=t("exn_ref"),
+                 pres=t("pabs"), cf=ad["cldfr"], hlc_hrc=ad["hlc_hrc"], hlc_hcf=ad["hlc_hcf"],
+                 hli_hri=ad["hli_hri"], hli_hcf=ad["hli_hcf"], sigs=t("sigs"),
+                 th_t=ad["ths"] * 50.0, rv_t=ad["rvs"] * 50.0, rc_t=ad["rcs"] * 50.0, rr_t=t("rrs") * 50.0,
+                 ri_t=ad["ris"] * 50.0, rs_t=t("rss") * 50.0, rg_t=t("rgs") * 50.0, ci_t=t("ci"),
+                 ths=ad["ths"], rvs=ad["rvs"], rcs=ad["rcs"], rrs=t("rrs"), ris=ad["ris"], rss=t("rss"),
+                 rgs=t("rgs"), icldfr=ad["icldfr"], ssio=ad["ssio"], ssiu=ad["ssiu"], ifr=ad["ifr"])
+    new, diags = RainIceOCND2()(state, 50.0)
+    for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "evap3d", "rainfr"):
+        assert torch.equal(new[k], res[k]), k
+    assert torch.equal(new["ci_t"], res["ci"])
+    for k in ("inprr", "inpri", "inprs", "inprg"):
+        assert torch.equal(diags[k], res[k]), k
+
+
+def test_pipeline_ocnd2_full_size(dev, full512):
+    """The fused OCND2 step at 512x512x90: column water budget, non-negative condensed
+    sources, and every 127th column (all 64 lanes) against the oracle, flips counted."""
+    from helpers import flips_of
+    from ice3_amd import AdjustRainIceOCND2
+    from oracle import oracle_lib, ref_lib
+    f0, cols, _ = full512
+    f = _with_heights(dict(f0), 20260050)
+    res = AdjustRainIceOCND2()({**f, "rhodref": f["rho_dry_ref"]}, 50.0, return_calls=True)
+    err = _column_budget(dev, f, res)
+    assert err < 1e-12, err
+    for k in ("rcs", "rrs", "ris", "rss", "rgs"):
+        assert bool((res[k] >= 0).all()), k
+    got = _sampled(res, cols, dev)
+    del res
+    g = {k: np.ascontiguousarray(v[:, cols] if v.ndim == 2 else v[cols]) for k, v in f.items()}
+    ref = oracle_lib.rain_ice_ocnd2(ref_lib.pipeline_state_ocnd2(g, oracle_lib.ice_adjust_ocnd2(g)),
+                                    with_iters=True)
+    w, nf, nb = check_with_flips(got, ref, RAIN_OUT, flips_of(got, ref), TOL)
+    print(f"fused OCND2 step 512x512x90, sampled columns: budget {err:.1e}, {nf} flips, {nb} divergent "
+          f"of {got['calls'].size}, worst {max(w.values()):.2e}")
+
+
+@pytest.mark.parametrize("ocnd2", [False, True])
+def test_persistent_and_point_kernels_bitwise(dev, ocnd2):
+    """The persistent microphysics kernel (64-point chunks, LDS parking) and the one-thread-
+    per-point kernel (taken when a context sets a sub-step cap other than the default; 4095
+    is never reached here) give the same results bit for bit, plain and OCND2."""
+    import ctypes
+    from ice3_amd import AdjustRainIce, AdjustRainIceOCND2, _lib, synthetic
+    lib, ctx = _lib.load(), _lib.context(0)
+    f = _with_heights(synthetic.generate(1500, 90, seed=20260051, variant="pipeline"), 20260052)
+    f["rhodref"] = f["rho_dry_ref"]
+    step = AdjustRainIceOCND2() if ocnd2 else AdjustRainIce()
+    a = step(f, 50.0, return_calls=True)
+    assert lib.ice3_ctx_set_substep_cap(ctx, 4095) == _lib.ICE3_OK
+    try:
+        b = step(f, 50.0, return_calls=True)
+    finally:
+        assert lib.ice3_ctx_set_substep_cap(ctx, 4096) == _lib.ICE3_OK
+    assert int((a["calls"] > 1).sum()) > 1000
+    for k in a:
+        assert torch.equal(torch.nan_to_num(a[k], nan=1.0), torch.nan_to_num(b[k], nan=1.0)), k

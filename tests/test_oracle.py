@@ -222,3 +222,78 @@ def test_ocnd2_oracle_matches_compiled_reference_fresh_seed():
     o = oracle_lib.ice_adjust_ocnd2(f)
     for k in ("rvs", "rcs", "ris", "ths", "cldfr", "srcs", "icldfr", "wcldfr", "ssio", "ssiu", "ifr"):
         assert rel_err(o[k], r[k], floor_for(k, 1e-14)).max() <= 1e-14, k
+
+
+# ------------------------------------------------------------- RAIN_ICE with OCND2
+
+def _rain_ocnd2_case(g, tag):
+    s = {k[len(tag) + 4:]: g[k] for k in g.files if k.startswith(tag + "_in_")}
+    return s, {k: g[f"{tag}_out_{k}"] for k in RAIN_OUT}
+
+
+@pytest.mark.parametrize("tag", ["base", "cov"])
+def test_rain_ice_ocnd2_matches_reference_golden(golden_dir, tag):
+    """RAIN_ICE with LOCND2 (mask on PSSIO, tabulated nucleation, ZESI/ZAI from the TIWMX
+    tables, OCND2 collision factors, ICE4_FAST_RI_RS) against the compiled reference."""
+    g = np.load(os.path.join(golden_dir, "rain_ice_ocnd2_fp64.npz"))
+    s, ref = _rain_ocnd2_case(g, tag)
+    o = oracle_lib.rain_ice_ocnd2(s, float(g["dt"]))
+    assert o["status"] == 0
+    for k in RAIN_OUT:
+        r = rel_err(o[k], ref[k], floor_for(k, 1e-12))
+        assert r.max() <= 1e-12, (k, r.max())
+
+
+def test_rain_ice_ocnd2_fixtures_exercise_the_branch(golden_dir):
+    """The OCND2 fixtures reach what the branch adds: ice-supersaturated points inside the ice
+    cloud and subsaturated ones outside it with pristine ice (ICE4_FAST_RI_RS), nucleation
+    below 0 C outside the microphysics mask, mask points kept only by PSSIO, multi-call time
+    splitting; and the branch changes the results against the plain RAIN_ICE."""
+    g = np.load(os.path.join(golden_dir, "rain_ice_ocnd2_fp64.npz"))
+    for tag in ("base", "cov"):
+        s, ref = _rain_ocnd2_case(g, tag)
+        o = oracle_lib.rain_ice_ocnd2(s, float(g["dt"]), with_iters=True)
+        t = s["tht"] * s["exn"]
+        ri = s["rit"] > 1e-10
+        assert ((s["ssio"] > 0) & (s["icldfr"] > 0.02) & ri & (s["ci"] > 0)).sum() > 50, tag
+        assert ((s["ssiu"] < 0) & (s["icldfr"] < 0.98) & ri & (s["ci"] > 0)).sum() > 20, tag
+        hyd = np.stack([s[k] for k in ("rct", "rrt", "rit", "rst", "rgt")])
+        only_ssio = (s["ssio"] > 0) & ~(hyd > 1e-15).any(0)
+        assert only_ssio.sum() > 0 or tag == "cov", tag
+        assert ((o["iters"] == 0) & (t < 273.16)).sum() > 0 or tag == "cov", tag
+        assert (o["iters"] >= 2).sum() > 100, tag
+        plain = oracle_lib.rain_ice(s, float(g["dt"]))
+        assert np.abs(plain["ris"] - ref["ris"]).max() > 1e-3 * np.abs(ref["ris"]).max(), tag
+
+
+def test_rain_ice_ocnd2_column_water_budget():
+    """The OCND2 processes move water between species (vapour -> ice in RVDEPI, ice -> snow in
+    RILARS, nucleation takes no vapour): each column's total water still changes only by its
+    surface precipitation, and every condensed source ends non-negative."""
+    f = synthetic.generate(64, 90, seed=20260046, variant="pipeline")
+    f["zz"] = np.ascontiguousarray(np.broadcast_to(synthetic.level_profile(90)[2][:, None], f["th"].shape))
+    f["ice_cld_wgt"] = np.full(64, 1.0)
+    a = oracle_lib.ice_adjust_ocnd2(f)
+    s = ref_lib.pipeline_state_ocnd2(f, a)
+    o = oracle_lib.rain_ice_ocnd2(s)
+    dt = 50.0
+    mass = f["rho_dry_ref"] * f["dzz"]
+    w = ("rvs", "rcs", "rrs", "ris", "rss", "rgs")
+    col0 = (mass * sum(s[k] for k in w)).sum(0) * dt
+    col1 = (mass * sum(o[k] for k in w)).sum(0) * dt
+    pr = -1000.0 * (o["inprr"] + o["inprs"] + o["inprg"]) * dt
+    assert np.max(np.abs(col1 - col0 - pr) / col0) < 1e-13
+    for k in ("rcs", "rrs", "ris", "rss", "rgs"):
+        assert np.all(o[k] >= 0.0), k
+
+
+@pytest.mark.skipif(not ref_lib.available(), reason="compiled reference not built here")
+def test_rain_ice_ocnd2_oracle_matches_compiled_reference_fresh_seed():
+    f = synthetic.generate(40, 90, seed=404, variant="pipeline")
+    f["zz"] = np.ascontiguousarray(np.broadcast_to(synthetic.level_profile(90)[2][:, None], f["th"].shape))
+    f["ice_cld_wgt"] = np.linspace(0.2, 2.0, 40)
+    s = ref_lib.pipeline_state_ocnd2(f, ref_lib.ice_adjust_ocnd2(f))
+    r = ref_lib.rain_ice_ocnd2(s, nproma=1)
+    o = oracle_lib.rain_ice_ocnd2(s)
+    for k in RAIN_OUT:
+        assert rel_err(o[k], r[k], floor_for(k, 1e-12)).max() <= 1e-12, k
