@@ -1,6 +1,7 @@
 /*
  * ice3_oracle.c -- CPU restatement (plain C, FP64, scalar) of the PHYEX CY50T1
- * ice_adjust and rain_ice path, pinned AROME ICE3 configuration (KRR=6).
+ * ice_adjust and rain_ice path, pinned AROME ICE3 configuration (KRR=6), and the same
+ * configuration with hail (KRR=7, HCLOUD='ICE4').
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke entry
  * point and the cpu_baseline leg of bench.py may load it; the product (the HIP
@@ -40,10 +41,10 @@ typedef struct {
   double XTT, XLVTT, XLSTT, XLMTT, XCPD, XCPV, XCL, XCI, XRD, XRV, XEPSILO;
   double XALPW, XBETAW, XGAMW, XALPI, XBETAI, XGAMI, XESTT, XP00, XRHOLW, XPI, XG, XMNH_TINY;
   /* RAIN_ICE_DESCR_t */
-  double XRTMIN[7]; /* 1-based */
+  double XRTMIN[8]; /* 1-based; [7] (hail) from the HCLOUD='ICE4' initialisation */
   double XFRMIN[41]; /* 1-based, RAIN_ICE_PARAM_t XFRMIN (micro/modd_rain_ice_paramn.F90:189) */
   double XCEXVT, XLBR, XLBEXR, XLBS, XLBEXS, XLBG, XLBEXG, XLBI, XLBEXI, XLBDAS_MAX, XDI, XBS,
-      XCXS, XCXG, XDG, XCONC_LAND;
+      XCXS, XCXG, XDG, XBG, XCONC_LAND;
   /* RAIN_ICE_PARAM_t */
   double XCRIAUTC, XCRIAUTI, XACRIAUTI, XBCRIAUTI, XNU10, XALPHA1, XBETA1, XNU20, XALPHA2,
       XBETA2, XMNU0, XALPHA3, XBETA3, XHON, XSCFAC, X0DEPI, X2DEPI, X0DEPS, X1DEPS, XEX0DEPS,
@@ -56,12 +57,22 @@ typedef struct {
       XLBSDRYG1, XLBSDRYG2, XLBSDRYG3, XFRDRYG, XLBRDRYG1, XLBRDRYG2, XLBRDRYG3, XDRYINTP1R,
       XDRYINTP2R, XDRYINTP1S, XDRYINTP2S, XDRYINTP1G, XDRYINTP2G, XFSEDR, XEXSEDR, XFSEDS,
       XEXSEDS, XFSEDG, XEXSEDG, XFSEDI, XEXCSEDI;
+  /* hail, KRR=7 (RAIN_ICE_DESCR_t, RAIN_ICE_PARAM_t) */
+  double XLBH, XLBEXH, XCXH, XDH, XFSEDH, XEXSEDH, X0DEPH, X1DEPH, XEX0DEPH, XEX1DEPH, XCOLIH,
+      XCOLEXIH, XCOLSH, XCOLEXSH, XCOLGH, XCOLEXGH, XFWETH, XFSWETH, XLBSWETH1, XLBSWETH2,
+      XLBSWETH3, XFGWETH, XLBGWETH1, XLBGWETH2, XLBGWETH3, XFRWETH, XLBRWETH1, XLBRWETH2,
+      XLBRWETH3, XWETINTP1S, XWETINTP2S, XWETINTP1G, XWETINTP2G, XWETINTP1R, XWETINTP2R,
+      XWETINTP1H, XWETINTP2H;
   int NGAMINC, NACCLBDAS, NACCLBDAR, NDRYLBDAR, NDRYLBDAS, NDRYLBDAG;
+  int NWETLBDAS, NWETLBDAG, NWETLBDAR, NWETLBDAH;
   double *GAMINC_RIM1, *GAMINC_RIM2, *GAMINC_RIM4;     /* NGAMINC */
   double *KER_RACCSS, *KER_RACCS;                       /* (NACCLBDAS, NACCLBDAR) */
   double *KER_SACCRG;                                   /* (NACCLBDAR, NACCLBDAS) */
   double *KER_SDRYG;                                    /* (NDRYLBDAG, NDRYLBDAS) */
   double *KER_RDRYG;                                    /* (NDRYLBDAG, NDRYLBDAR) */
+  double *KER_SWETH;                                    /* (NWETLBDAH, NWETLBDAS) */
+  double *KER_GWETH;                                    /* (NWETLBDAH, NWETLBDAG) */
+  double *KER_RWETH;                                    /* (NWETLBDAH, NWETLBDAR) */
 } params_t;
 
 static params_t P;
@@ -72,7 +83,7 @@ static int P_loaded = 0;
 #define XTMAXMIX 273.16
 
 /* MODD_FIELDS_ADDRESS (micro/modd_fields_address.F90) */
-enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6 };
+enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6, IRH = 7 };
 enum {
   IRCHONI = 1, IRVDEPS, IRIAGGS, IRIAUTS, IRVDEPG, IRCAUTR, IRCACCR, IRREVAV, IRCBERI, IRHMLTR,
   IRSMLTG, IRCMLTSR, IRRACCSS, IRRACCSG, IRSACCRG, IRCRIMSS, IRCRIMSG, IRSRIMCG, IRICFRRG,
@@ -84,6 +95,9 @@ enum {
 /* local PRS_TEND / PRG_TEND slots (mode_ice4_fast_rs.F90:87, mode_ice4_fast_rg.F90:98) */
 enum { RS_RCRIMS = 1, RS_RCRIMSS, RS_RSRIMCG, RS_RRACCS, RS_RRACCSS, RS_RSACCRG, RS_FREEZ1, RS_FREEZ2 };
 enum { RG_RCDRYG = 1, RG_RIDRYG, RG_RIWETG, RG_RSDRYG, RG_RSWETG, RG_RRDRYG, RG_FREEZ1, RG_FREEZ2 };
+/* local PRH_TEND slots (mode_ice4_fast_rh.F90:94-95) */
+enum { RH_RCWETH = 1, RH_RRWETH, RH_RIDRYH, RH_RIWETH, RH_RSDRYH, RH_RSWETH, RH_RGDRYH, RH_RGWETH,
+       RH_FREEZ1, RH_FREEZ2 };
 
 /* ZSRC_1D(-22:11), micro/condensation.F90:208-216 */
 static const double ZSRC_1D[34] = {
@@ -111,7 +125,7 @@ static int read_constants(const char *path) {
       {"XLBEXR", &P.XLBEXR}, {"XLBS", &P.XLBS}, {"XLBEXS", &P.XLBEXS}, {"XLBG", &P.XLBG},
       {"XLBEXG", &P.XLBEXG}, {"XLBI", &P.XLBI}, {"XLBEXI", &P.XLBEXI},
       {"XLBDAS_MAX", &P.XLBDAS_MAX}, {"XDI", &P.XDI}, {"XBS", &P.XBS}, {"XCXS", &P.XCXS},
-      {"XCXG", &P.XCXG}, {"XDG", &P.XDG}, {"XCONC_LAND", &P.XCONC_LAND},
+      {"XCXG", &P.XCXG}, {"XDG", &P.XDG}, {"XBG", &P.XBG}, {"XCONC_LAND", &P.XCONC_LAND},
       {"XCRIAUTC", &P.XCRIAUTC}, {"XCRIAUTI", &P.XCRIAUTI}, {"XACRIAUTI", &P.XACRIAUTI},
       {"XBCRIAUTI", &P.XBCRIAUTI}, {"XNU10", &P.XNU10}, {"XALPHA1", &P.XALPHA1},
       {"XBETA1", &P.XBETA1}, {"XNU20", &P.XNU20}, {"XALPHA2", &P.XALPHA2}, {"XBETA2", &P.XBETA2},
@@ -142,7 +156,21 @@ static int read_constants(const char *path) {
       {"XDRYINTP2S", &P.XDRYINTP2S}, {"XDRYINTP1G", &P.XDRYINTP1G},
       {"XDRYINTP2G", &P.XDRYINTP2G}, {"XFSEDR", &P.XFSEDR}, {"XEXSEDR", &P.XEXSEDR},
       {"XFSEDS", &P.XFSEDS}, {"XEXSEDS", &P.XEXSEDS}, {"XFSEDG", &P.XFSEDG},
-      {"XEXSEDG", &P.XEXSEDG}, {"XFSEDI", &P.XFSEDI}, {"XEXCSEDI", &P.XEXCSEDI}, {NULL, NULL}};
+      {"XEXSEDG", &P.XEXSEDG}, {"XFSEDI", &P.XFSEDI}, {"XEXCSEDI", &P.XEXCSEDI},
+      {"XLBH", &P.XLBH}, {"XLBEXH", &P.XLBEXH}, {"XCXH", &P.XCXH}, {"XDH", &P.XDH},
+      {"XFSEDH", &P.XFSEDH}, {"XEXSEDH", &P.XEXSEDH}, {"X0DEPH", &P.X0DEPH},
+      {"X1DEPH", &P.X1DEPH}, {"XEX0DEPH", &P.XEX0DEPH}, {"XEX1DEPH", &P.XEX1DEPH},
+      {"XCOLIH", &P.XCOLIH}, {"XCOLEXIH", &P.XCOLEXIH}, {"XCOLSH", &P.XCOLSH},
+      {"XCOLEXSH", &P.XCOLEXSH}, {"XCOLGH", &P.XCOLGH}, {"XCOLEXGH", &P.XCOLEXGH},
+      {"XFWETH", &P.XFWETH}, {"XFSWETH", &P.XFSWETH}, {"XLBSWETH1", &P.XLBSWETH1},
+      {"XLBSWETH2", &P.XLBSWETH2}, {"XLBSWETH3", &P.XLBSWETH3}, {"XFGWETH", &P.XFGWETH},
+      {"XLBGWETH1", &P.XLBGWETH1}, {"XLBGWETH2", &P.XLBGWETH2}, {"XLBGWETH3", &P.XLBGWETH3},
+      {"XFRWETH", &P.XFRWETH}, {"XLBRWETH1", &P.XLBRWETH1}, {"XLBRWETH2", &P.XLBRWETH2},
+      {"XLBRWETH3", &P.XLBRWETH3}, {"XWETINTP1S", &P.XWETINTP1S},
+      {"XWETINTP2S", &P.XWETINTP2S}, {"XWETINTP1G", &P.XWETINTP1G},
+      {"XWETINTP2G", &P.XWETINTP2G}, {"XWETINTP1R", &P.XWETINTP1R},
+      {"XWETINTP2R", &P.XWETINTP2R}, {"XWETINTP1H", &P.XWETINTP1H},
+      {"XWETINTP2H", &P.XWETINTP2H}, {NULL, NULL}};
   char name[64];
   int n, nfound = 0;
   while (fscanf(fp, "%63s %d", name, &n) == 2) {
@@ -152,7 +180,7 @@ static int read_constants(const char *path) {
     int used = 0;
     for (int j = 0; s[j].name; ++j)
       if (!strcmp(s[j].name, name)) { *s[j].dst = buf[0]; used = 1; nfound++; }
-    if (!strcmp(name, "XRTMIN")) { for (int j = 0; j < 6; ++j) P.XRTMIN[j + 1] = buf[j]; used = 1; }
+    if (!strcmp(name, "XRTMIN")) { for (int j = 0; j < n && j < 7; ++j) P.XRTMIN[j + 1] = buf[j]; used = 1; }
     else if (!strcmp(name, "XFRMIN") && n == 40) { for (int j = 0; j < 40; ++j) P.XFRMIN[j + 1] = buf[j]; }
     else if (!strcmp(name, "NGAMINC")) { P.NGAMINC = (int)buf[0]; }
     else if (!strcmp(name, "NACCLBDAS")) { P.NACCLBDAS = (int)buf[0]; }
@@ -160,6 +188,10 @@ static int read_constants(const char *path) {
     else if (!strcmp(name, "NDRYLBDAR")) { P.NDRYLBDAR = (int)buf[0]; }
     else if (!strcmp(name, "NDRYLBDAS")) { P.NDRYLBDAS = (int)buf[0]; }
     else if (!strcmp(name, "NDRYLBDAG")) { P.NDRYLBDAG = (int)buf[0]; }
+    else if (!strcmp(name, "NWETLBDAS")) { P.NWETLBDAS = (int)buf[0]; }
+    else if (!strcmp(name, "NWETLBDAG")) { P.NWETLBDAG = (int)buf[0]; }
+    else if (!strcmp(name, "NWETLBDAR")) { P.NWETLBDAR = (int)buf[0]; }
+    else if (!strcmp(name, "NWETLBDAH")) { P.NWETLBDAH = (int)buf[0]; }
     if (!strcmp(name, "XGAMINC_RIM1")) { P.GAMINC_RIM1 = buf; used = 2; }
     else if (!strcmp(name, "XGAMINC_RIM2")) { P.GAMINC_RIM2 = buf; used = 2; }
     else if (!strcmp(name, "XGAMINC_RIM4")) { P.GAMINC_RIM4 = buf; used = 2; }
@@ -168,6 +200,9 @@ static int read_constants(const char *path) {
     else if (!strcmp(name, "XKER_SACCRG")) { P.KER_SACCRG = buf; used = 2; }
     else if (!strcmp(name, "XKER_SDRYG")) { P.KER_SDRYG = buf; used = 2; }
     else if (!strcmp(name, "XKER_RDRYG")) { P.KER_RDRYG = buf; used = 2; }
+    else if (!strcmp(name, "XKER_SWETH")) { P.KER_SWETH = buf; used = 2; }
+    else if (!strcmp(name, "XKER_GWETH")) { P.KER_GWETH = buf; used = 2; }
+    else if (!strcmp(name, "XKER_RWETH")) { P.KER_RWETH = buf; used = 2; }
     if (used != 2) free(buf);
   }
   fclose(fp);
@@ -229,18 +264,20 @@ static inline double esat_i(double t) { return exp(P.XALPI - P.XBETAI / t - P.XG
 /* ICE_ADJUST (micro/ice_adjust.F90:7-485) + ITERATION (:488-565) + CONDENSATION
  * (micro/condensation.F90:7-658) for one point; branch: LSUBG_COND=T, OSIGMAS=T,
  * LSTATNW=F, LCONDBORN=F, OCND2=F, HCONDENS='CB02', HLAMBDA3='CB', HFRAC_ICE='T',
- * CSUBG_MF_PDF='TRIANGLE', ITERMAX=1, PWEIGHT_MF_CLOUD=0, KRR=6. */
+ * CSUBG_MF_PDF='TRIANGLE', ITERMAX=1, PWEIGHT_MF_CLOUD=0, KRR=6, or KRR=7 (hail) when rh is
+ * not NULL. */
 static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs, double th,
                              double exn, double exnref, double rhodref, double rv, double rc,
-                             double ri, double rr, double rs, double rg, double cf_mf,
-                             double rc_mf, double ri_mf, double *rvs, double *rcs, double *ris,
-                             double *ths, double *o) {
+                             double ri, double rr, double rs, double rg, const double *rh,
+                             double cf_mf, double rc_mf, double ri_mf, double *rvs, double *rcs,
+                             double *ris, double *ths, double *o) {
   /* 2.3 (ice_adjust.F90:264-270) */
   double zt = th * exn;
   double lv = P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT);
   double ls = P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT);
-  /* ITERATION, KRR=6 (:507-510) */
-  double cph = P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg);
+  /* ITERATION, KRR=6 (:507-510) or 7 (:503-506) */
+  double cph = rh ? P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg + *rh)
+                  : P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg);
   /* CONDENSATION */
   double rt = rv + rc + ri * 1.0;                                   /* :275 */
   double pv = dmin(esat_w(zt), .99 * pabs);                         /* :407 */
@@ -372,8 +409,9 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
   o[OA_TH] = zt / exn;
 }
 
-int oracle_ice_adjust(int ncol, int nlev, double dt, const double *sigqsat,
-                      const double *const in[ORACLE_ADJ_NIN], double *const out[ORACLE_ADJ_NOUT]) {
+static int ice_adjust_krr(int ncol, int nlev, double dt, const double *sigqsat,
+                          const double *const in[ORACLE_ADJ_NIN], const double *rh,
+                          double *const out[ORACLE_ADJ_NOUT]) {
   if (!P_loaded) return -1;
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) {
@@ -382,8 +420,9 @@ int oracle_ice_adjust(int ncol, int nlev, double dt, const double *sigqsat,
       double o[ORACLE_ADJ_NOUT_POINT];
       ice_adjust_point(dt, sigqsat[i], in[IA_PABS][j], in[IA_SIGS][j], in[IA_TH][j], in[IA_EXN][j],
                        in[IA_EXNREF][j], in[IA_RHODREF][j], in[IA_RV][j], in[IA_RC][j],
-                       in[IA_RI][j], in[IA_RR][j], in[IA_RS][j], in[IA_RG][j], in[IA_CF_MF][j],
-                       in[IA_RC_MF][j], in[IA_RI_MF][j], &rvs, &rcs, &ris, &ths, o);
+                       in[IA_RI][j], in[IA_RR][j], in[IA_RS][j], in[IA_RG][j],
+                       rh ? rh + j : NULL, in[IA_CF_MF][j], in[IA_RC_MF][j], in[IA_RI_MF][j],
+                       &rvs, &rcs, &ris, &ths, o);
       for (int m = 0; m < ORACLE_ADJ_NOUT_POINT; ++m) out[m][j] = o[m];
       out[OA_RVS][j] = rvs;
       out[OA_RCS][j] = rcs;
@@ -391,6 +430,17 @@ int oracle_ice_adjust(int ncol, int nlev, double dt, const double *sigqsat,
       out[OA_THS][j] = ths;
     }
   return 0;
+}
+
+int oracle_ice_adjust(int ncol, int nlev, double dt, const double *sigqsat,
+                      const double *const in[ORACLE_ADJ_NIN], double *const out[ORACLE_ADJ_NOUT]) {
+  return ice_adjust_krr(ncol, nlev, dt, sigqsat, in, NULL, out);
+}
+
+int oracle_ice_adjust_hail(int ncol, int nlev, double dt, const double *sigqsat,
+                           const double *const in[ORACLE_ADJ_NIN], const double *rh,
+                           double *const out[ORACLE_ADJ_NOUT]) {
+  return ice_adjust_krr(ncol, nlev, dt, sigqsat, in, rh, out);
 }
 
 /* ------------------------------------------------ ICE_ADJUST, OCND2 branch */
@@ -751,26 +801,36 @@ static void ice4_nucleation(int ldcompute, double tht, double pabst, double rhod
   if (gnegt) *cit = dmax(zzw + *cit, *cit);
 }
 
+/* Branch record of the tendency calls (test coverage only): bits of ORACLE_BR_* ORed over
+ * the calls of one point; see oracle_record_branches. */
+static unsigned g_branches;
+static unsigned *g_branch_out;
+#define BR(bit, cond) do { if (cond) g_branches |= (1u << (bit)); } while (0)
+
+void oracle_record_branches(unsigned *flags) { g_branch_out = flags; }
+
 /* Per-point persistent state of ICE4_STEPPING (micro/mode_ice4_stepping.F90:128-158) */
 typedef struct {
   double bu[IBUNUM + 1];   /* ZBU_INST */
   double rs_tend[9];       /* ZRS_TEND (1-based) */
   double rg_tend[9];       /* ZRG_TEND (1-based) */
+  double rh_tend[11];      /* ZRH_TEND (1-based), KRR=7 */
   double ssi;              /* ZSSI */
   double hlc_hcf, hlc_lcf, hlc_hrc, hlc_lrc, hli_hcf, hli_lcf, hli_hri, hli_lri;
 } step_state_t;
 
 /* ICE4_TENDENCIES (micro/mode_ice4_tendencies.F90:9-726) for one point, with the
- * process routines it calls inlined in call order. */
-static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, double lvfact,
+ * process routines it calls inlined in call order.  krr = 6 or 7 (hail: pvart, pa and pb
+ * then have 8 entries, the last one hail). */
+static void ice4_tendencies(int krr, int soft, int ldc, double exn, double rhodref, double lvfact,
                             double lsfact, double pres, double cf, double sigma_rc, double *cit,
                             double pt, const double *pvart, step_state_t *S, double *pa,
                             double *pb, double *rainfr, const ocnd2_pt_t *x) {
-  double *bu = S->bu, *rs_tend = S->rs_tend, *rg_tend = S->rg_tend;
-  double zt = pt, zvart[7];
+  double *bu = S->bu, *rs_tend = S->rs_tend, *rg_tend = S->rg_tend, *rh_tend = S->rh_tend;
+  double zt = pt, zvart[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
   double ka = 0., dv = 0., ai = 0., cj = 0., esi_tab = 0.;
   double colf = 1.0, acrf = 1.0; /* ZCOLF, ZACRF (:323-324) */
-  for (int jv = 0; jv < 7; ++jv) { zvart[jv] = pvart[jv]; pa[jv] = 0.; pb[jv] = 0.; }
+  for (int jv = 0; jv <= krr; ++jv) { zvart[jv] = pvart[jv]; pa[jv] = 0.; pb[jv] = 0.; }
   if (soft) {
     bu[IRVHENI_MR] = 0.; bu[IRRHONG_MR] = 0.; bu[IRIMLTC_MR] = 0.; bu[IRSRIMCG_MR] = 0.;
   } else {
@@ -789,6 +849,7 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     } else {
       bu[IRRHONG_MR] = 0.;
     }
+    BR(ORACLE_BR_RRHONG, bu[IRRHONG_MR] > 0.);
     zvart[ITH] = zvart[ITH] + bu[IRRHONG_MR] * (lsfact - lvfact);
     zt = zvart[ITH] * exn;
     zvart[IRR] = zvart[IRR] - bu[IRRHONG_MR];
@@ -801,6 +862,7 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     } else {
       bu[IRIMLTC_MR] = 0.;
     }
+    BR(ORACLE_BR_RIMLTC, bu[IRIMLTC_MR] > 0.);
     zvart[ITH] = zvart[ITH] - bu[IRIMLTC_MR] * (lsfact - lvfact);
     zt = zvart[ITH] * exn;
     zvart[IRC] = zvart[IRC] + bu[IRIMLTC_MR];
@@ -881,14 +943,16 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   }
   *rainfr = 1.; /* no PRFR option (:379-383) */
 
-  /* slope parameters (:389-433), LSNOW_T=F, KRR=6 */
+  /* slope parameters (:389-433), LSNOW_T=F */
   double lbdar = zvart[IRR] > 0. ? P.XLBR * pow(rhodref * dmax(zvart[IRR], P.XRTMIN[3]), P.XLBEXR) : 0.;
   double lbdas = zvart[IRS] > 0.
                      ? dmin(P.XLBDAS_MAX, P.XLBS * pow(rhodref * dmax(zvart[IRS], P.XRTMIN[5]), P.XLBEXS))
                      : 0.;
   double lbdag = zvart[IRG] > 0. ? P.XLBG * pow(rhodref * dmax(zvart[IRG], P.XRTMIN[6]), P.XLBEXG) : 0.;
+  double lbdah = 0.;
+  if (krr == 7 && zvart[IRH] > 0.) lbdah = P.XLBH * pow(rhodref * dmax(zvart[IRH], P.XRTMIN[7]), P.XLBEXH);
   const double rv = zvart[IRV], rc = zvart[IRC], rr = zvart[IRR], ri = zvart[IRI],
-               rs = zvart[IRS], rg = zvart[IRG];
+               rs = zvart[IRS], rg = zvart[IRG], rh = zvart[IRH];
   const double ssi = S->ssi;
 
   /* ICE4_SLOW (mode_ice4_slow.F90:106-217), LNEWCOEFF=F; deposition tuning ZREDSN/ZREDGR
@@ -1047,7 +1111,11 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     bu[IRSMLTG] = 0.; bu[IRCMLTSR] = 0.;
   }
 
-  /* ICE4_FAST_RG (mode_ice4_fast_rg.F90:121-410), KRR=6, LCRFLIMIT=T, LNULLWETG=T, LWETGPOST=T */
+  /* graupel produced by the other processes (mode_ice4_tendencies.F90:509-514) */
+  const double rgsi = bu[IRVDEPG] + bu[IRSMLTG] + bu[IRRACCSG] + bu[IRSACCRG] + bu[IRCRIMSG] +
+                      bu[IRSRIMCG];
+  const double rgsi_mr = bu[IRRHONG_MR] + bu[IRSRIMCG_MR];
+  /* ICE4_FAST_RG (mode_ice4_fast_rg.F90:121-410), LCRFLIMIT=T, LNULLWETG=T, LWETGPOST=T */
   if (ri > P.XRTMIN[4] && rr > P.XRTMIN[3] && ldc) {
     if (!soft) {
       bu[IRICFRRG] = P.XICFRR * ri * pow(lbdar, P.XEXICFRR) * pow(rhodref, -P.XCEXVT);
@@ -1128,7 +1196,16 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     rg_tend[RG_FREEZ1] = 0.; rg_tend[RG_FREEZ2] = 0.;
     rwetg_init = 0.; ldwetg = 0; lldryg = 0;
   }
+  /* part of the wet growth converted into hail, KRR=7 (:330-351) */
+  BR(ORACLE_BR_WETG, ldwetg);
+  BR(ORACLE_BR_DRYG, lldryg);
   bu[IRWETGH] = 0.; bu[IRWETGH_MR] = 0.;
+  if (krr == 7 && ldwetg) {
+    bu[IRWETGH] = (dmax(0., rgsi + bu[IRICFRRG] + bu[IRRCFRIG]) + rwetg_init) * rdryg_init /
+                  (rwetg_init + rdryg_init);
+    bu[IRWETGH_MR] = dmax(0., rgsi_mr) * rdryg_init / (rwetg_init + rdryg_init);
+    BR(ORACLE_BR_WETGH, bu[IRWETGH] > 0.);
+  }
   if (ldwetg) {
     bu[IRRWETG] = -(rg_tend[RG_RIWETG] + rg_tend[RG_RSWETG] + rg_tend[RG_RCDRYG] - rwetg_init);
     bu[IRCWETG] = rg_tend[RG_RCDRYG];
@@ -1158,6 +1235,130 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
                                  (rhodref * P.XLMTT));
     }
   } else bu[IRGMLTR] = 0.;
+
+  /* ICE4_FAST_RH (mode_ice4_fast_rh.F90:109-405), KRR=7: LNULLWETH=T, LWETHPOST=T,
+   * LCONVHG=T, LEVLIMIT=T, LNEWCOEFF=F; ldwetg is ICE4_FAST_RG's wet-growth mask */
+  if (krr == 7) {
+    const int rh_ok = rh > P.XRTMIN[7] && ldc;
+    if (rh_ok && rc > P.XRTMIN[2]) {
+      if (!soft) {
+        rh_tend[RH_RCWETH] = pow(lbdah, P.XCXH - P.XDH - 2.0) * pow(rhodref, -P.XCEXVT);
+        rh_tend[RH_RCWETH] = P.XFWETH * rc * rh_tend[RH_RCWETH];
+      }
+    } else rh_tend[RH_RCWETH] = 0.;
+    if (rh_ok && ri > P.XRTMIN[4]) {
+      if (!soft) {
+        rh_tend[RH_RIWETH] = pow(lbdah, P.XCXH - P.XDH - 2.0) * pow(rhodref, -P.XCEXVT);
+        rh_tend[RH_RIWETH] = P.XFWETH * ri * rh_tend[RH_RIWETH];
+        rh_tend[RH_RIDRYH] = rh_tend[RH_RIWETH] * (P.XCOLIH * exp(P.XCOLEXIH * (zt - P.XTT)));
+      }
+    } else {
+      rh_tend[RH_RIWETH] = 0.; rh_tend[RH_RIDRYH] = 0.;
+    }
+    double zzw;
+    int gwet = rh_ok && rs > P.XRTMIN[5]; /* 7.2.1 snow on hail (:140-182) */
+    if (!gwet) { rh_tend[RH_RSWETH] = 0.; rh_tend[RH_RSDRYH] = 0.; }
+    if (!soft && gwet) {
+      interp2d(lbdah, lbdas, P.NWETLBDAH, P.NWETLBDAS, P.XWETINTP1H, P.XWETINTP2H, P.XWETINTP1S,
+               P.XWETINTP2S, P.KER_SWETH, &zzw, NULL, NULL, NULL, NULL);
+      rh_tend[RH_RSWETH] = P.XFSWETH * zzw * (pow(lbdas, P.XCXS - P.XBS)) * (pow(lbdah, P.XCXH)) *
+                           (pow(rhodref, -P.XCEXVT - 1.)) *
+                           (P.XLBSWETH1 / (sq(lbdah)) + P.XLBSWETH2 / (lbdah * lbdas) +
+                            P.XLBSWETH3 / (sq(lbdas)));
+      rh_tend[RH_RSDRYH] = rh_tend[RH_RSWETH] * (P.XCOLSH * exp(P.XCOLEXSH * (zt - P.XTT)));
+    }
+    gwet = rh_ok && rg > P.XRTMIN[6]; /* 7.2.6 graupel on hail (:187-227) */
+    if (!gwet) { rh_tend[RH_RGWETH] = 0.; rh_tend[RH_RGDRYH] = 0.; }
+    if (!soft && gwet) {
+      interp2d(lbdah, lbdag, P.NWETLBDAH, P.NWETLBDAG, P.XWETINTP1H, P.XWETINTP2H, P.XWETINTP1G,
+               P.XWETINTP2G, P.KER_GWETH, &zzw, NULL, NULL, NULL, NULL);
+      rh_tend[RH_RGWETH] = P.XFGWETH * zzw * (pow(lbdag, P.XCXG - P.XBG)) * (pow(lbdah, P.XCXH)) *
+                           (pow(rhodref, -P.XCEXVT - 1.)) *
+                           (P.XLBGWETH1 / (sq(lbdah)) + P.XLBGWETH2 / (lbdah * lbdag) +
+                            P.XLBGWETH3 / (sq(lbdag)));
+      rh_tend[RH_RGDRYH] = rh_tend[RH_RGWETH];
+      /* wet graupel keeps its collection efficiency (:221-223) */
+      if (!ldwetg)
+        rh_tend[RH_RGDRYH] = rh_tend[RH_RGDRYH] * (P.XCOLGH * exp(P.XCOLEXGH * (zt - P.XTT)));
+    }
+    gwet = rh_ok && rr > P.XRTMIN[3]; /* 7.2.11 rain on hail (:232-262) */
+    if (!gwet) rh_tend[RH_RRWETH] = 0.;
+    if (!soft && gwet) {
+      interp2d(lbdah, lbdar, P.NWETLBDAH, P.NWETLBDAR, P.XWETINTP1H, P.XWETINTP2H, P.XWETINTP1R,
+               P.XWETINTP2R, P.KER_RWETH, &zzw, NULL, NULL, NULL, NULL);
+      rh_tend[RH_RRWETH] = P.XFRWETH * zzw * (1.0 / pow4(lbdar)) * (pow(lbdah, P.XCXH)) *
+                           (pow(rhodref, -P.XCEXVT - 1.)) *
+                           (P.XLBRWETH1 / (sq(lbdah)) + P.XLBRWETH2 / (lbdah * lbdar) +
+                            P.XLBRWETH3 / (sq(lbdar)));
+    }
+    double rdryh_init = rh_tend[RH_RCWETH] + rh_tend[RH_RIDRYH] + rh_tend[RH_RSDRYH] +
+                        rh_tend[RH_RRWETH] + rh_tend[RH_RGDRYH];
+    double rweth_init = 0.;
+    int llweth = 0, lldryh = 0;
+    if (rh_ok) { /* 7.3-7.4 wet growth and the wet/dry choice (:274-318) */
+      if (!soft) {
+        double f1 = rv * pres / (P.XEPSILO + rv);
+        f1 = dmin(f1, exp(P.XALPI - P.XBETAI / zt - P.XGAMI * log(zt)));
+        f1 = ka * (P.XTT - zt) +
+             (dv * (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) * (P.XESTT - f1) / (P.XRV * zt));
+        f1 = f1 * (P.X0DEPH * pow(lbdah, P.XEX0DEPH) + P.X1DEPH * cj * pow(lbdah, P.XEX1DEPH)) /
+             (rhodref * (P.XLMTT - P.XCL * (P.XTT - zt)));
+        rh_tend[RH_FREEZ1] = f1;
+        rh_tend[RH_FREEZ2] = (rhodref * (P.XLMTT + (P.XCI - P.XCL) * (P.XTT - zt))) /
+                             (rhodref * (P.XLMTT - P.XCL * (P.XTT - zt)));
+      }
+      double isg = rh_tend[RH_RIWETH] + rh_tend[RH_RSWETH] + rh_tend[RH_RGWETH];
+      rweth_init = dmax(isg, dmax(0., rh_tend[RH_FREEZ1] + rh_tend[RH_FREEZ2] * isg));
+      llweth = dmax(0., rweth_init - rh_tend[RH_RIWETH] - rh_tend[RH_RSWETH] - rh_tend[RH_RGWETH]) <=
+               dmax(0., rdryh_init - rh_tend[RH_RIDRYH] - rh_tend[RH_RSDRYH] - rh_tend[RH_RGDRYH]);
+      llweth = llweth && rdryh_init > 0.;
+      lldryh = zt < P.XTT && rdryh_init > 1.E-20 &&
+               dmax(0., rweth_init - rh_tend[RH_RIWETH] - rh_tend[RH_RSWETH]) >
+                   dmax(0., rdryh_init - rh_tend[RH_RIDRYH] - rh_tend[RH_RSDRYH]);
+    } else {
+      rh_tend[RH_FREEZ1] = 0.; rh_tend[RH_FREEZ2] = 0.;
+    }
+    /* conversion of hail into graupel, LCONVHG (:322-336) */
+    double rdryhg = lldryh ? rdryh_init * rweth_init / (rdryh_init + rweth_init) : 0.;
+    BR(ORACLE_BR_WETH, llweth);
+    BR(ORACLE_BR_DRYH, lldryh);
+    BR(ORACLE_BR_DRYHG, rdryhg > 0.);
+    if (llweth) {
+      bu[IRCWETH] = rh_tend[RH_RCWETH];
+      bu[IRIWETH] = rh_tend[RH_RIWETH];
+      bu[IRSWETH] = rh_tend[RH_RSWETH];
+      bu[IRGWETH] = rh_tend[RH_RGWETH];
+      bu[IRRWETH] = (rweth_init - rh_tend[RH_RIWETH] - rh_tend[RH_RSWETH] - rh_tend[RH_RGWETH] -
+                     rh_tend[RH_RCWETH]);
+    } else {
+      bu[IRCWETH] = 0.; bu[IRIWETH] = 0.; bu[IRSWETH] = 0.; bu[IRGWETH] = 0.; bu[IRRWETH] = 0.;
+    }
+    if (lldryh) {
+      bu[IRCDRYH] = rh_tend[RH_RCWETH];
+      bu[IRIDRYH] = rh_tend[RH_RIDRYH];
+      bu[IRSDRYH] = rh_tend[RH_RSDRYH];
+      bu[IRRDRYH] = rh_tend[RH_RRWETH];
+      bu[IRGDRYH] = rh_tend[RH_RGDRYH];
+      bu[IRDRYHG] = rdryhg;
+    } else {
+      bu[IRCDRYH] = 0.; bu[IRIDRYH] = 0.; bu[IRSDRYH] = 0.; bu[IRRDRYH] = 0.; bu[IRGDRYH] = 0.;
+      bu[IRDRYHG] = 0.;
+    }
+    if (rh_ok && zt > P.XTT) { /* 7.5 melting (:382-403) */
+      if (!soft) {
+        double m = rv * pres / (P.XEPSILO + rv);
+        m = dmin(m, exp(P.XALPW - P.XBETAW / zt - P.XGAMW * log(zt)));
+        m = ka * (P.XTT - zt) +
+            dv * (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) * (P.XESTT - m) / (P.XRV * zt);
+        bu[IRHMLTR] = dmax(0., (-m * (P.X0DEPH * pow(lbdah, P.XEX0DEPH) +
+                                       P.X1DEPH * cj * pow(lbdah, P.XEX1DEPH)) -
+                                (rh_tend[RH_RCWETH] + rh_tend[RH_RRWETH]) *
+                                    (rhodref * P.XCL * (P.XTT - zt))) /
+                                   (rhodref * P.XLMTT));
+      }
+    } else bu[IRHMLTR] = 0.;
+    BR(ORACLE_BR_HMLTR, bu[IRHMLTR] > 0.);
+  }
 
   /* ICE4_FAST_RI (mode_ice4_fast_ri.F90:70-81); not called with OCND2 (tendencies :568-578) */
   if (x) {
@@ -1201,7 +1402,11 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
     }
   }
 
-  /* aggregation into rates PA and instantaneous changes PB (:606-717), KRR=6 */
+  BR(ORACLE_BR_RIMING, bu[IRCRIMSG] > 0.);
+  BR(ORACLE_BR_RACCSG, bu[IRRACCSG] > 0.);
+  BR(ORACLE_BR_SMLTG, bu[IRSMLTG] > 0.);
+  BR(ORACLE_BR_GMLTR, bu[IRGMLTR] > 0.);
+  /* aggregation into rates PA and instantaneous changes PB (:606-717) */
   double ls_lv = lsfact - lvfact;
   pa[ITH] = pa[ITH] + bu[IRVDEPG] * lsfact;
   pa[ITH] = pa[ITH] + bu[IRCHONI] * ls_lv;
@@ -1215,6 +1420,11 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   pa[ITH] = pa[ITH] + (bu[IRCWETG] + bu[IRRWETG]) * ls_lv;
   pa[ITH] = pa[ITH] + (bu[IRCDRYG] + bu[IRRDRYG]) * ls_lv;
   pa[ITH] = pa[ITH] - bu[IRGMLTR] * ls_lv;
+  if (krr == 7) {
+    pa[ITH] = pa[ITH] + (bu[IRRWETH] + bu[IRCWETH]) * ls_lv;
+    pa[ITH] = pa[ITH] + (bu[IRCDRYH] + bu[IRRDRYH]) * ls_lv;
+    pa[ITH] = pa[ITH] - bu[IRHMLTR] * ls_lv;
+  }
   pa[ITH] = pa[ITH] + bu[IRCBERI] * ls_lv;
   if (x) pa[ITH] = pa[ITH] + bu[IRVDEPI] * lsfact;
 
@@ -1231,6 +1441,10 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   pa[IRC] = pa[IRC] - bu[IRCMLTSR];
   pa[IRC] = pa[IRC] - bu[IRCWETG];
   pa[IRC] = pa[IRC] - bu[IRCDRYG];
+  if (krr == 7) {
+    pa[IRC] = pa[IRC] - bu[IRCWETH];
+    pa[IRC] = pa[IRC] - bu[IRCDRYH];
+  }
   pa[IRC] = pa[IRC] - bu[IRCBERI];
 
   pa[IRR] = pa[IRR] + bu[IRCAUTR];
@@ -1243,6 +1457,11 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   pa[IRR] = pa[IRR] - bu[IRRWETG];
   pa[IRR] = pa[IRR] - bu[IRRDRYG];
   pa[IRR] = pa[IRR] + bu[IRGMLTR];
+  if (krr == 7) {
+    pa[IRR] = pa[IRR] - bu[IRRWETH];
+    pa[IRR] = pa[IRR] - bu[IRRDRYH];
+    pa[IRR] = pa[IRR] + bu[IRHMLTR];
+  }
 
   pa[IRI] = pa[IRI] + bu[IRCHONI];
   pa[IRI] = pa[IRI] - bu[IRIAGGS];
@@ -1250,6 +1469,10 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   pa[IRI] = pa[IRI] - bu[IRICFRRG] - bu[IRICFRR];
   pa[IRI] = pa[IRI] - bu[IRIWETG];
   pa[IRI] = pa[IRI] - bu[IRIDRYG];
+  if (krr == 7) {
+    pa[IRI] = pa[IRI] - bu[IRIWETH];
+    pa[IRI] = pa[IRI] - bu[IRIDRYH];
+  }
   pa[IRI] = pa[IRI] + bu[IRCBERI];
   if (x) pa[IRI] = pa[IRI] + bu[IRVDEPI];
   if (x) pa[IRI] = pa[IRI] - bu[IRILARS];
@@ -1265,6 +1488,10 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   pa[IRS] = pa[IRS] - bu[IRSMLTG];
   pa[IRS] = pa[IRS] - bu[IRSWETG];
   pa[IRS] = pa[IRS] - bu[IRSDRYG];
+  if (krr == 7) {
+    pa[IRS] = pa[IRS] - bu[IRSWETH];
+    pa[IRS] = pa[IRS] - bu[IRSDRYH];
+  }
   if (x) pa[IRS] = pa[IRS] + bu[IRILARS];
 
   pa[IRG] = pa[IRG] + bu[IRVDEPG] * znodepg_s;
@@ -1277,36 +1504,47 @@ static void ice4_tendencies(int soft, int ldc, double exn, double rhodref, doubl
   pb[IRG] = pb[IRG] - bu[IRWETGH_MR];
   pa[IRG] = pa[IRG] + bu[IRCDRYG] + bu[IRIDRYG] + bu[IRSDRYG] + bu[IRRDRYG];
   pa[IRG] = pa[IRG] - bu[IRGMLTR];
+  if (krr == 7) {
+    pa[IRG] = pa[IRG] - bu[IRGWETH];
+    pa[IRG] = pa[IRG] - bu[IRGDRYH] + bu[IRDRYHG];
+
+    pa[IRH] = pa[IRH] + bu[IRWETGH];
+    pb[IRH] = pb[IRH] + bu[IRWETGH_MR];
+    pa[IRH] = pa[IRH] + bu[IRCWETH] + bu[IRIWETH] + bu[IRSWETH] + bu[IRGWETH] + bu[IRRWETH];
+    pa[IRH] = pa[IRH] + bu[IRCDRYH] + bu[IRIDRYH] + bu[IRSDRYH] + bu[IRRDRYH] + bu[IRGDRYH] -
+              bu[IRDRYHG];
+    pa[IRH] = pa[IRH] - bu[IRHMLTR];
+  }
 }
 
 #define ORACLE_MAX_CALLS 100000
 
 /* ICE4_STEPPING (micro/mode_ice4_stepping.F90:180-537) for one microphysics point,
- * XTSTEP_TS=0, XMRSTEP=0, LFEEDBACKT=T, LDEXT_TEND=T, LDSIGMA_RC=T. */
-static int ice4_stepping_point(double dt, double exn, double rhodref, double pres, double cf,
-                               double sigs, double *cit, double *vart, double *extpk,
+ * XTSTEP_TS=0, XMRSTEP=0, LFEEDBACKT=T, LDEXT_TEND=T, LDSIGMA_RC=T; vart and extpk have
+ * krr + 1 entries. */
+static int ice4_stepping_point(int krr, double dt, double exn, double rhodref, double pres,
+                               double cf, double sigs, double *cit, double *vart, double *extpk,
                                double *rainfr, double *rrevav, int *ncalls, const ocnd2_pt_t *x) {
   step_state_t S;
   memset(&S, 0, sizeof(S));
   double zinv = 1. / dt;
-  for (int jv = 0; jv <= 6; ++jv) extpk[jv] = extpk[jv] - vart[jv] * zinv; /* :201-205 */
+  for (int jv = 0; jv <= krr; ++jv) extpk[jv] = extpk[jv] - vart[jv] * zinv; /* :201-205 */
   double sigma_rc = sigs * 2.;                                              /* :211-213 */
-  double time = 0., za[7], zb[7];
+  double time = 0., za[8], zb[8];
   int calls = 0;
   while (time < dt) {                                                      /* :253 */
     int llcompute = 1, soft = 0;
     while (llcompute) {                                                     /* :275 */
       if (++calls > ORACLE_MAX_CALLS) { *ncalls = calls; return -2; }
-      double sum2 = vart[IRI];
-      sum2 = sum2 + vart[IRS];
-      sum2 = sum2 + vart[IRG];
+      double sum2 = vart[IRI];                                              /* :277-283 */
+      for (int jv = IRI + 1; jv <= krr; ++jv) sum2 = sum2 + vart[jv];
       double zdevide = (P.XCPD + P.XCPV * vart[IRV] + P.XCL * (vart[IRC] + vart[IRR]) + P.XCI * sum2) * exn;
       double zzt = vart[ITH] * exn;
       double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
       double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
-      ice4_tendencies(soft, llcompute, exn, rhodref, lvfact, lsfact, pres, cf, sigma_rc, cit, zzt,
-                      vart, &S, za, zb, rainfr, x);
-      for (int jv = 0; jv <= 6; ++jv) za[jv] = za[jv] + extpk[jv];          /* :320-324 */
+      ice4_tendencies(krr, soft, llcompute, exn, rhodref, lvfact, lsfact, pres, cf, sigma_rc, cit,
+                      zzt, vart, &S, za, zb, rainfr, x);
+      for (int jv = 0; jv <= krr; ++jv) za[jv] = za[jv] + extpk[jv];        /* :320-324 */
       double maxtime = llcompute ? (dt - time) : 0.;                        /* :335-341 */
       double zx = P.XTT / exn;                                              /* :344-360 */
       if ((vart[ITH] - zx) * (vart[ITH] + zb[ITH] - zx) < 0.) maxtime = 0.;
@@ -1314,40 +1552,46 @@ static int ice4_stepping_point(double dt, double exn, double rhodref, double pre
         double tth = (zx - zb[ITH] - vart[ITH]) / za[ITH];
         if (tth > 0.) maxtime = dmin(maxtime, tth);
       }
-      for (int jv = 1; jv <= 6; ++jv)                                       /* :367-375 */
+      for (int jv = 1; jv <= krr; ++jv)                                     /* :367-375 */
         if (za[jv] < -1.E-20 && vart[jv] > P.XRTMIN[jv]) {
           maxtime = dmin(maxtime, -(zb[jv] + vart[jv]) / za[jv]);
           maxtime = dmax(maxtime, P.XMNH_TINY);
         }
       if (time + maxtime >= dt) llcompute = 0;                              /* :380-384 */
-      for (int jv = 0; jv <= 6; ++jv) vart[jv] = vart[jv] + za[jv] * maxtime + zb[jv]; /* :461-467 */
+      for (int jv = 0; jv <= krr; ++jv) vart[jv] = vart[jv] + za[jv] * maxtime + zb[jv]; /* :461-467 */
       if (vart[IRI] <= 0.) *cit = 0.;                                       /* :471-474 */
       time = time + maxtime;
       soft = 1;                                                             /* :516 */
     }
   }
-  for (int jv = 0; jv <= 6; ++jv) vart[jv] = vart[jv] - extpk[jv] * dt;   /* :524-530 */
+  for (int jv = 0; jv <= krr; ++jv) vart[jv] = vart[jv] - extpk[jv] * dt; /* :524-530 */
   *rrevav = S.bu[IRREVAV];                                                  /* :535-537 */
   *ncalls = calls;
   return 0;
 }
 
+/* Hail arrays of RAIN_ICE with KRR=7: the state PRHT, the source PRHS (updated) and the
+ * surface rate PINPRH (per column). */
+typedef struct { const double *rht; double *rhs; double *inprh; } hail_t;
+
 /* ICE4_SEDIMENTATION_STAT (micro/mode_ice4_sedimentation_stat.F90:9-455) for one
  * column, LSEDIC=F, LNEWCOEFF=F; r?s are the source arrays (updated), r?t the
- * mixing ratios PR?S*PTSTEP (mode_ice4_sedimentation.F90:115-124). */
+ * mixing ratios PR?S*PTSTEP (mode_ice4_sedimentation.F90:115-124); hail (species 7) when
+ * h is not NULL. */
 static void sedimentation_column(int ncol, int nlev, int i, double dt, const double *dzz,
                                  const double *rhodref, double *const io[ORACLE_RAIN_NIO],
-                                 double *const col[ORACLE_RAIN_NCOL]) {
+                                 double *const col[ORACLE_RAIN_NCOL], const hail_t *h) {
   double zinv = 1. / dt;
-  double above[7] = {0, 0, 0, 0, 0, 0, 0}, cur[7];
+  double above[8] = {0, 0, 0, 0, 0, 0, 0, 0}, cur[8];
   const int slot[7] = {-1, -1, RO_RCS, RO_RRS, RO_RIS, RO_RSS, RO_RGS};
+  const int nsp = h ? 7 : 6;
   for (int k = nlev - 1; k >= 0; --k) {
     size_t j = (size_t)k * ncol + i;
     double rho = rhodref[j], dz = dzz[j];
     double tsorhodz = dt / (rho * dz);
     cur[2] = 0.; /* cloud: LSEDIC = F */
-    for (int jrr = 3; jrr <= 6; ++jrr) {
-      double rx = io[slot[jrr]][j] * dt; /* ZR?T = PR?S*PTSTEP */
+    for (int jrr = 3; jrr <= nsp; ++jrr) {
+      double rx = (jrr == 7 ? h->rhs[j] : io[slot[jrr]][j]) * dt; /* ZR?T = PR?S*PTSTEP */
       double qp = above[jrr] * tsorhodz, w1 = 0., w2 = 0.;
       if (jrr == 4) { /* PRISTINE_ICE (:303-333) */
         if ((rx > P.XRTMIN[4]) || (qp > P.XRTMIN[4])) {
@@ -1359,8 +1603,8 @@ static void sedimentation_column(int ncol, int nlev, int i, double dt, const dou
                  pow(dmax(0.05E6, -0.15319E6 - 0.021454E6 * log(rho * qp)), P.XEXCSEDI);
         }
       } else { /* OTHER_SPECIES (:403-427) */
-        double fsed = jrr == 3 ? P.XFSEDR : (jrr == 5 ? P.XFSEDS : P.XFSEDG);
-        double exsed = jrr == 3 ? P.XEXSEDR : (jrr == 5 ? P.XEXSEDS : P.XEXSEDG);
+        double fsed = jrr == 3 ? P.XFSEDR : (jrr == 5 ? P.XFSEDS : (jrr == 6 ? P.XFSEDG : P.XFSEDH));
+        double exsed = jrr == 3 ? P.XEXSEDR : (jrr == 5 ? P.XEXSEDS : (jrr == 6 ? P.XEXSEDG : P.XEXSEDH));
         if ((rx > P.XRTMIN[jrr]) || (qp > P.XRTMIN[jrr])) {
           if (rx > P.XRTMIN[jrr]) w1 = fsed * pow(rx, exsed - 1) * pow(rho, exsed - P.XCEXVT - 1);
           if (qp > P.XRTMIN[jrr]) w2 = fsed * pow(qp, exsed - 1) * pow(rho, exsed - P.XCEXVT - 1);
@@ -1374,24 +1618,29 @@ static void sedimentation_column(int ncol, int nlev, int i, double dt, const dou
     }
     for (int jrr = 2; jrr <= 6; ++jrr) /* :202-211 */
       io[slot[jrr]][j] = io[slot[jrr]][j] + tsorhodz * (above[jrr] - cur[jrr]) * zinv;
+    if (h) h->rhs[j] = h->rhs[j] + tsorhodz * (above[7] - cur[7]) * zinv;
     if (k == 0) { /* :213-224 */
       col[RC_INPRR][i] = cur[3] / P.XRHOLW;
       col[RC_INPRI][i] = cur[4] / P.XRHOLW;
       col[RC_INPRS][i] = cur[5] / P.XRHOLW;
       col[RC_INPRG][i] = cur[6] / P.XRHOLW;
+      if (h) h->inprh[i] = cur[7] / P.XRHOLW;
     }
-    for (int jrr = 2; jrr <= 6; ++jrr) above[jrr] = cur[jrr];
+    for (int jrr = 2; jrr <= nsp; ++jrr) above[jrr] = cur[jrr];
   }
 }
 
-/* RAIN_ICE (micro/rain_ice.F90:7-830) for KRR=6 with the commented-out
- * sedimentation (:442-457), total tendencies (:754-793) and negativity
- * correction (:821-823) reinstated.  x4 (NULL for the plain branch) holds the OCND2
- * inputs PICLDFR, PSSIO, PSSIU, PIFR. */
+/* RAIN_ICE (micro/rain_ice.F90:7-830) with the commented-out sedimentation (:442-457),
+ * total tendencies (:754-793) and negativity correction (:821-823) reinstated.  x4 (NULL
+ * for the plain branch) holds the OCND2 inputs PICLDFR, PSSIO, PSSIU, PIFR; h (NULL for
+ * KRR=6) the hail arrays of KRR=7. */
 static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
-                         const double *const *x4, double *const io[ORACLE_RAIN_NIO],
-                         double *const col[ORACLE_RAIN_NCOL], int *iters) {
+                         const double *const *x4, const hail_t *h,
+                         double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
+                         int *iters) {
   if (!P_loaded) return -1;
+  if (h && (!P.KER_SWETH || !P.KER_GWETH || !P.KER_RWETH || !(P.XRTMIN[7] > 0.))) return -1;
+  const int krr = h ? 7 : 6;
   int status = 0;
   double zinv = 1. / dt;
   /* 1.1 model level height ZZZZ (:387-395): from the top level down, as the reference
@@ -1412,7 +1661,7 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
   /* 2. sedimentation before microphysics, per column */
   for (int i = 0; i < ncol; ++i) {
     col[RC_INPRC][i] = 0.;
-    sedimentation_column(ncol, nlev, i, dt, in[RI_DZZ], in[RI_RHODREF], io, col);
+    sedimentation_column(ncol, nlev, i, dt, in[RI_DZZ], in[RI_RHODREF], io, col, h);
     col[RC_INPRS][i] = col[RC_INPRS][i] + col[RC_INPRI][i]; /* mode_ice4_sedimentation.F90:134 */
   }
   for (int k = 0; k < nlev; ++k)
@@ -1421,8 +1670,10 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
       double exn = in[RI_EXN][j], rhodref = in[RI_RHODREF][j], pabs = in[RI_PABS][j];
       double tht = in[RI_THT][j], rvt = in[RI_RVT][j], rct = in[RI_RCT][j], rrt = in[RI_RRT][j];
       double rit = in[RI_RIT][j], rst = in[RI_RST][j], rgt = in[RI_RGT][j];
+      double rht = h ? h->rht[j] : 0.;
       /* 1. factors and mask (rain_ice.F90:334-384) */
       double zrice = rit + rst + rgt;
+      if (h) zrice = zrice + rht;
       double zdevide = P.XCPD + P.XCPV * rvt + P.XCL * (rct + rrt) + P.XCI * zrice;
       double zt = tht * exn;
       double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide;
@@ -1435,12 +1686,14 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
         x = &xp;
         llmicro = xp.ssio > P.XFRMIN[12] || rct > P.XFRMIN[13] || rrt > P.XFRMIN[13] ||
                   rit > P.XFRMIN[13] || rst > P.XFRMIN[13] || rgt > P.XFRMIN[13];
+        if (h) llmicro = llmicro || rht > P.XFRMIN[13];
       } else {
         llmicro = rct > P.XRTMIN[2] || rrt > P.XRTMIN[3] || rit > P.XRTMIN[4] ||
                   rst > P.XRTMIN[5] || rgt > P.XRTMIN[6];
+        if (h) llmicro = llmicro || rht > P.XRTMIN[7];
       }
       /* 3. saved state (:466-490) */
-      double wr[7] = {tht, rvt, rct, rrt, rit, rst, rgt};
+      double wr[8] = {tht, rvt, rct, rrt, rit, rst, rgt, rht};
       double evap3d = 0., rainfr;
       double cit = io[RO_CI][j];
       /* 4.1 nucleation outside the mask (:505-540) */
@@ -1452,16 +1705,18 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
       rainfr = 1.; /* 4.2 (:555-608) */
       /* 5. tendencies (ICE4_PACK -> ICE4_STEPPING), external tendencies = sources */
       int ncalls = 0;
+      g_branches = 0;
       if (llmicro) {
-        double extpk[7] = {io[RO_THS][j], io[RO_RVS][j], io[RO_RCS][j], io[RO_RRS][j],
-                           io[RO_RIS][j], io[RO_RSS][j], io[RO_RGS][j]};
+        double extpk[8] = {io[RO_THS][j], io[RO_RVS][j], io[RO_RCS][j], io[RO_RRS][j],
+                           io[RO_RIS][j], io[RO_RSS][j], io[RO_RGS][j], h ? h->rhs[j] : 0.};
         double rrevav = 0.;
-        if (ice4_stepping_point(dt, exn, rhodref, pabs, in[RI_CLDFR][j], in[RI_SIGS][j], &cit, wr,
-                                extpk, &rainfr, &rrevav, &ncalls, x) != 0)
+        if (ice4_stepping_point(krr, dt, exn, rhodref, pabs, in[RI_CLDFR][j], in[RI_SIGS][j], &cit,
+                                wr, extpk, &rainfr, &rrevav, &ncalls, x) != 0)
           status = -2;
         evap3d = rrevav;
       }
       if (iters) iters[j] = ncalls;
+      if (g_branch_out) g_branch_out[j] = g_branches;
       /* 8.1 total tendencies (:754-793) */
       double exnref = in[RI_EXNREF][j];
       lsfact = lsfact / exnref;
@@ -1469,7 +1724,8 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
       double wrv = (wr[IRV] - rvt) * zinv, wrc = (wr[IRC] - rct) * zinv;
       double wrr = (wr[IRR] - rrt) * zinv, wri = (wr[IRI] - rit) * zinv;
       double wrs = (wr[IRS] - rst) * zinv, wrg = (wr[IRG] - rgt) * zinv;
-      double wth = (wrc + wrr) * lvfact + (wri + wrs + wrg + 0.) * lsfact;
+      double wrh = h ? (wr[IRH] - rht) * zinv : 0.;
+      double wth = (wrc + wrr) * lvfact + (wri + wrs + wrg + wrh) * lsfact;
       double ths = io[RO_THS][j] + wth + rvheni * lsfact;
       double rvs = io[RO_RVS][j] + wrv - rvheni;
       double rcs = io[RO_RCS][j] + wrc;
@@ -1477,6 +1733,7 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
       double ris = io[RO_RIS][j] + wri + rvheni;
       double rss = io[RO_RSS][j] + wrs;
       double rgs = io[RO_RGS][j] + wrg;
+      double rhs = h ? h->rhs[j] + wrh : 0.;
       /* 8.2 ICE4_CORRECT_NEGATIVITIES (:821-823; mode_ice4_correct_negativities.F90:51-171) */
       double zw;
       zw = rcs - dmax(rcs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rcs = rcs - zw;
@@ -1484,6 +1741,7 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
       zw = ris - dmax(ris, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; ris = ris - zw;
       zw = rss - dmax(rss, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rss = rss - zw;
       zw = rgs - dmax(rgs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rgs = rgs - zw;
+      if (h) { zw = rhs - dmax(rhs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rhs = rhs - zw; }
       zw = dmin(1., dmax(P.XRTMIN[1] - rvs, 0.) / dmax(rcs + ris, 1.E-20));
       ths = ths - zw * (rcs * lvfact + ris * lsfact);
       rvs = rvs + zw * (rcs + ris);
@@ -1495,6 +1753,11 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
       rvs = rvs + zw; rss = rss - zw; ths = ths - zw * lsfact;
       zw = dmin(dmax(rgs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
       rvs = rvs + zw; rgs = rgs - zw; ths = ths - zw * lsfact;
+      if (h) {
+        zw = dmin(dmax(rhs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+        rvs = rvs + zw; rhs = rhs - zw; ths = ths - zw * lsfact;
+        h->rhs[j] = rhs;
+      }
       io[RO_THS][j] = ths; io[RO_RVS][j] = rvs; io[RO_RCS][j] = rcs; io[RO_RRS][j] = rrs;
       io[RO_RIS][j] = ris; io[RO_RSS][j] = rss; io[RO_RGS][j] = rgs; io[RO_CI][j] = cit;
       io[RO_EVAP3D][j] = evap3d; io[RO_RAINFR][j] = rainfr;
@@ -1506,12 +1769,20 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
 int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
                     double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
                     int *iters) {
-  return rain_ice_core(ncol, nlev, dt, in, NULL, io, col, iters);
+  return rain_ice_core(ncol, nlev, dt, in, NULL, NULL, io, col, iters);
+}
+
+int oracle_rain_ice_hail(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
+                         const double *rht, double *rhs, double *inprh,
+                         double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
+                         int *iters) {
+  hail_t h = {rht, rhs, inprh};
+  return rain_ice_core(ncol, nlev, dt, in, NULL, &h, io, col, iters);
 }
 
 int oracle_rain_ice_ocnd2(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
                           const double *const x4[ORACLE_RAIN_NOCND2],
                           double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
                           int *iters) {
-  return rain_ice_core(ncol, nlev, dt, in, x4, io, col, iters);
+  return rain_ice_core(ncol, nlev, dt, in, x4, NULL, io, col, iters);
 }

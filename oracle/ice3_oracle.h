@@ -62,6 +62,29 @@ int oracle_rain_ice(int ncol, int nlev, double dt, const double *const in[ORACLE
                     double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
                     int *iters);
 
+/* ICE_ADJUST with KRR = 7: the hail mixing ratio rh (nlev, ncol) enters the heat capacity. */
+int oracle_ice_adjust_hail(int ncol, int nlev, double dt, const double *sigqsat,
+                           const double *const in[ORACLE_ADJ_NIN], const double *rh,
+                           double *const out[ORACLE_ADJ_NOUT]);
+
+/* RAIN_ICE with KRR = 7 (hail): as oracle_rain_ice, plus the hail state rht, its source rhs
+ * (updated in place) and the per-column surface hail rate inprh.  Returns as
+ * oracle_rain_ice; -1 also when the constants hold no hail tables. */
+int oracle_rain_ice_hail(int ncol, int nlev, double dt, const double *const in[ORACLE_RAIN_NIN],
+                         const double *rht, double *rhs, double *inprh,
+                         double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
+                         int *iters);
+
+/* Branch record for coverage tests: while flags (ncol*nlev unsigned) is set, every rain_ice
+ * call writes there, per point, the ORACLE_BR_* bits of the branches its tendency calls
+ * took (ORed over the calls); NULL stops the recording. */
+enum {
+  ORACLE_BR_RRHONG, ORACLE_BR_RIMLTC, ORACLE_BR_RIMING, ORACLE_BR_RACCSG, ORACLE_BR_SMLTG,
+  ORACLE_BR_WETG, ORACLE_BR_DRYG, ORACLE_BR_GMLTR, ORACLE_BR_WETGH, ORACLE_BR_WETH,
+  ORACLE_BR_DRYH, ORACLE_BR_DRYHG, ORACLE_BR_HMLTR
+};
+void oracle_record_branches(unsigned *flags);
+
 /* RAIN_ICE with PARAMI%LOCND2 = .TRUE.: as oracle_rain_ice, plus ICE_ADJUST's OCND2 outputs
  * x4 = {PICLDFR, PSSIO, PSSIU, PIFR}, each (nlev, ncol).  Returns as oracle_rain_ice, or -3
  * when a work array cannot be allocated. */

@@ -139,3 +139,68 @@ def rain_ice_ocnd2(s: dict, dt: float = 50.0, with_iters: bool = False) -> dict:
     if with_iters:
         out["iters"] = iters
     return out
+
+
+def ice_adjust_hail(f: dict, dt: float = 50.0) -> dict:
+    """ICE_ADJUST with KRR = 7; ``f`` adds the hail mixing ratio ``rh``."""
+    lib = _load()
+    nlev, ncol = f["th"].shape
+    ins = [_chk(f[k]) for k in ADJ_IN]
+    outs = {k: np.empty((nlev, ncol)) for k in ADJ_OUT}
+    sq, rh = _chk(f["sigqsat"]), _chk(f["rh"])
+    rc = lib.oracle_ice_adjust_hail(ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt),
+                                    ctypes.c_void_p(sq.ctypes.data), _arr(ins),
+                                    ctypes.c_void_p(rh.ctypes.data), _arr([outs[k] for k in ADJ_OUT]))
+    if rc != 0:
+        raise RuntimeError(f"oracle_ice_adjust_hail failed: {rc}")
+    return outs
+
+
+def rain_ice_hail(s: dict, dt: float = 50.0, with_iters: bool = False) -> dict:
+    """RAIN_ICE with KRR = 7; ``s`` as for rain_ice plus the hail state ``rht`` and source
+    ``rhs``.  Returns rain_ice's outputs plus the updated ``rhs`` and ``inprh``."""
+    lib = _load()
+    nlev, ncol = s["tht"].shape
+    ins = [_chk(s[k]) for k in RAIN_IN]
+    io = {k: (np.array(s[k], dtype=np.float64, order="C") if k in s else np.zeros((nlev, ncol)))
+          for k in RAIN_IO}
+    col = {k: np.zeros(ncol) for k in RAIN_COL}
+    rht = _chk(s["rht"])
+    rhs = np.array(s["rhs"], dtype=np.float64, order="C")
+    inprh = np.zeros(ncol)
+    iters = np.zeros((nlev, ncol), dtype=np.int32)
+    rc = lib.oracle_rain_ice_hail(ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt), _arr(ins),
+                                  ctypes.c_void_p(rht.ctypes.data), ctypes.c_void_p(rhs.ctypes.data),
+                                  ctypes.c_void_p(inprh.ctypes.data), _arr([io[k] for k in RAIN_IO]),
+                                  _arr([col[k] for k in RAIN_COL]), ctypes.c_void_p(iters.ctypes.data))
+    if rc == -1:
+        raise RuntimeError("oracle_rain_ice_hail: constants (with the hail tables) not loaded")
+    out = dict(io)
+    out.update(col)
+    out.update(rhs=rhs, inprh=inprh, status=rc)
+    if with_iters:
+        out["iters"] = iters
+    return out
+
+
+BRANCHES = ("rrhong", "rimltc", "riming", "raccsg", "smltg", "wetg", "dryg", "gmltr", "wetgh",
+            "weth", "dryh", "dryhg", "hmltr")
+
+
+def record_branches(fn, *a, **kw):
+    """Runs one rain_ice call of this module (fn) with the branch record on; returns its
+    outputs and, per point, the mask of the ORACLE_BR_* branches taken (BRANCHES order)."""
+    lib = _load()
+    s = a[0]
+    flags = np.zeros(s["tht"].shape, dtype=np.uint32)
+    lib.oracle_record_branches(ctypes.c_void_p(flags.ctypes.data))
+    try:
+        out = fn(*a, **kw)
+    finally:
+        lib.oracle_record_branches(ctypes.c_void_p(None))
+    return out, flags
+
+
+def branch_counts(flags) -> dict:
+    """Number of points that took each branch."""
+    return {b: int(((flags >> i) & 1).sum()) for i, b in enumerate(BRANCHES)}

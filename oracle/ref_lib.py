@@ -41,22 +41,23 @@ def _load():
         if not available():
             raise FileNotFoundError(f"{LIB_PATH} not built (make -f oracle/ref_build.mk)")
         _lib = ctypes.CDLL(LIB_PATH)
-        _lib.ref_init.argtypes = [ctypes.c_double, ctypes.c_int]
+        _lib.ref_init.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int]
         _lib.ref_set_nproma.argtypes = [ctypes.c_int]
         _lib.ref_dump_constants.argtypes = [ctypes.c_char_p, ctypes.c_int]
     return _lib
 
 
-def init(dt: float = 50.0, nproma: int = 1):
-    """INI_CST + INI_RAIN_ICE for the pinned AROME ICE3 configuration."""
+def init(dt: float = 50.0, nproma: int = 1, krr: int = 6):
+    """INI_CST + INI_RAIN_ICE for the pinned AROME configuration: HCLOUD='ICE3' (krr=6) or
+    'ICE4' (krr=7, hail)."""
     global _init_state
     lib = _load()
-    if _init_state is None or _init_state[0] != dt:
-        lib.ref_init(dt, nproma)
-        _init_state = (dt, nproma)
+    if _init_state is None or _init_state[0] != dt or _init_state[2] != krr:
+        lib.ref_init(dt, nproma, krr)
+        _init_state = (dt, nproma, krr)
     elif _init_state[1] != nproma:
         lib.ref_set_nproma(nproma)
-        _init_state = (dt, nproma)
+        _init_state = (dt, nproma, krr)
     return lib
 
 
@@ -66,7 +67,9 @@ def _p(a):
 
 
 def dump_constants(path: str, dt: float = 50.0):
-    lib = init(dt)
+    """Constants and tables after the HCLOUD='ICE4' initialisation: the 'ICE3' one gives the
+    same values with XRTMIN(1:6) only."""
+    lib = init(dt, krr=7)
     b = path.encode()
     lib.ref_dump_constants(b, len(b))
 
@@ -103,6 +106,30 @@ def _ice_adjust(f: dict, dt: float) -> dict:
         ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt), _p(f["sigqsat"]),
         *[_p(f[k]) for k in ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc", "ri",
                              "rr", "rs", "rg", "cf_mf", "rc_mf", "ri_mf")],
+        _p(rvs), _p(rcs), _p(ris), _p(ths),
+        *[_p(out[k]) for k in names])
+    out["t"] = out["th_out"] * f["exn"]
+    out.update(rvs=rvs, rcs=rcs, ris=ris, ths=ths)
+    return out
+
+
+def ice_adjust_hail(f: dict, dt: float = 50.0) -> dict:
+    """ICE_ADJUST with KRR = 7: ``f`` adds the hail mixing ratio ``rh`` to the ice_adjust
+    fields (it enters the heat capacity only)."""
+    return _chunked(_ice_adjust_hail, f, dt)
+
+
+def _ice_adjust_hail(f: dict, dt: float) -> dict:
+    lib = init(dt, krr=7)
+    nlev, ncol = f["th"].shape
+    rvs, rcs, ris, ths = (f[k].copy() for k in ("rvs", "rcs", "ris", "ths"))
+    names = ("cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf", "rv_out", "rc_out", "ri_out",
+             "th_out", "srcs", "cph", "lv", "ls")
+    out = {k: np.zeros((nlev, ncol)) for k in names}
+    lib.ref_ice_adjust_hail(
+        ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt), _p(f["sigqsat"]),
+        *[_p(f[k]) for k in ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc", "ri",
+                             "rr", "rs", "rg", "rh", "cf_mf", "rc_mf", "ri_mf")],
         _p(rvs), _p(rcs), _p(ris), _p(ths),
         *[_p(out[k]) for k in names])
     out["t"] = out["th_out"] * f["exn"]
@@ -191,6 +218,41 @@ def _rain_ice_ocnd2(s: dict, dt: float, nproma: int) -> dict:
     io.update(col)
     io.update(o2)
     return io
+
+
+def rain_ice_hail(s: dict, dt: float = 50.0, nproma: int = 1) -> dict:
+    """RAIN_ICE with KRR = 7; ``s`` holds rain_ice's fields plus the hail state ``rht`` and
+    source ``rhs``.  Returns rain_ice's outputs plus the updated ``rhs`` and ``inprh``."""
+    return _chunked(_rain_ice_hail, s, dt, nproma)
+
+
+def _rain_ice_hail(s: dict, dt: float, nproma: int) -> dict:
+    lib = init(dt, nproma, krr=7)
+    nlev, ncol = s["tht"].shape
+    io = {k: s[k].copy() for k in ("ci", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
+                                   "ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "rhs")}
+    col = {k: np.zeros(ncol) for k in ("inprc", "inprr", "inpri", "inprs", "inprg", "inprh")}
+    o2 = {k: np.zeros((nlev, ncol)) for k in ("evap3d", "rainfr")}
+    lib.ref_rain_ice_hail(
+        ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt),
+        *[_p(s[k]) for k in ("exn", "dzz", "rhodj", "rhodref", "exn_ref", "pabs")],
+        _p(io["ci"]), _p(s["cldfr"]),
+        *[_p(io[k]) for k in ("hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf")],
+        *[_p(s[k]) for k in ("tht", "rvt", "rct", "rrt", "rit", "rst", "rgt", "rht", "sigs")],
+        *[_p(io[k]) for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "rhs")],
+        *[_p(col[k]) for k in ("inprc", "inprr", "inpri", "inprs", "inprg", "inprh")],
+        _p(o2["evap3d"]), _p(o2["rainfr"]))
+    io.update(col)
+    io.update(o2)
+    return io
+
+
+def pipeline_state_hail(f: dict, adj: dict, dt: float = 50.0) -> dict:
+    """rain_ice inputs of the fused step with hail: pipeline_state plus the hail state and
+    source (rht = rhs * dt, as for the other species)."""
+    s = pipeline_state(f, adj, dt)
+    s.update(rht=f["rhs"] * dt, rhs=f["rhs"])
+    return s
 
 
 def pipeline_state_ocnd2(f: dict, adj: dict, dt: float = 50.0) -> dict:

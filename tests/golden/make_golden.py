@@ -22,7 +22,14 @@ Runs in the build container only (it needs oracle/_ref/libphyex_ref.so, built by
   and its PICLDFR, PSSIO, PSSIU, PIFR; the "pipeline" generator (``base``, 32 columns) and the
   precipitation-band coverage case (``cov``, 16 columns); keys ``<tag>_in_*``/``<tag>_out_*``.
 
-Usage: python3 tests/golden/make_golden.py [ocnd2 | rain_ocnd2]
+* ``rain_ice_hail_fp64.npz``  (``python3 tests/golden/make_golden.py rain_hail``) the fused
+  step with hail (KRR=7, the reference initialised for HCLOUD='ICE4'): ICE_ADJUST with the
+  hail mixing ratio, then RAIN_ICE with hail on its adjusted sources; the "pipeline" generator
+  with hail where it has graupel (``base``, 32 columns) and the precipitation-band case with a
+  hail band (``cov``, 16 columns); keys ``<tag>_adj_in_*``/``<tag>_adj_out_*`` (ICE_ADJUST)
+  and ``<tag>_in_*``/``<tag>_out_*`` (RAIN_ICE).
+
+Usage: python3 tests/golden/make_golden.py [ocnd2 | rain_ocnd2 | rain_hail]
 """
 from __future__ import annotations
 
@@ -136,6 +143,49 @@ def rain_ocnd2():
     np.savez_compressed(os.path.join(HERE, "rain_ice_ocnd2_fp64.npz"), dt=DT, **out)
 
 
+RAIN_HAIL_IN = RAIN_IN + ("rht", "rhs")
+RAIN_HAIL_OUT = RAIN_OUT + ("rhs", "inprh")
+
+
+def hail_pipeline_case(ncol, seed):
+    """The "pipeline" generator with a hail mixing ratio where it holds graupel: rh = rg times
+    a factor U(0.2, 2), and rhs = rh / dt as for the other species."""
+    f = synthetic.generate(ncol, 90, seed=seed, variant="pipeline", dt=DT)
+    rng = np.random.Generator(np.random.Philox(seed + 7))
+    f["rh"] = np.ascontiguousarray(f["rg"] * rng.uniform(0.2, 2.0, f["rg"].shape))
+    f["rhs"] = f["rh"] / DT
+    return f
+
+
+def hail_band_case(ncol, seed):
+    """band_case with a hail band in the same levels: rh U(1e-4, 5e-3) inside, 0 outside."""
+    f = band_case(ncol, seed)
+    rng = np.random.Generator(np.random.Philox(seed + 7))
+    lev = np.arange(90)[:, None]
+    start = np.linspace(0, 70, ncol).astype(int)[None, :]
+    band = (lev >= start) & (lev < start + 20)
+    f["rh"] = np.ascontiguousarray(np.where(band, rng.uniform(1.0e-4, 5.0e-3, band.shape), 0.0))
+    f["rhs"] = f["rh"] / DT
+    return f
+
+
+def rain_hail():
+    out = {}
+    for tag, f in (("base", hail_pipeline_case(32, 20260071)), ("cov", hail_band_case(16, 20260073))):
+        a = ref_lib.ice_adjust_hail(f, DT)
+        for k in ADJ_IN + ("sigqsat", "rh"):
+            out[f"{tag}_adj_in_{k}"] = f[k]
+        for k in ADJ_OUT:
+            out[f"{tag}_adj_out_{k}"] = a[k]
+        s = ref_lib.pipeline_state_hail(f, a, DT)
+        r = ref_lib.rain_ice_hail(s, DT, nproma=1)
+        for k in RAIN_HAIL_IN:
+            out[f"{tag}_in_{k}"] = s[k]
+        for k in RAIN_HAIL_OUT:
+            out[f"{tag}_out_{k}"] = r[k]
+    np.savez_compressed(os.path.join(HERE, "rain_ice_hail_fp64.npz"), dt=DT, **out)
+
+
 def main():
     ref_lib.init(DT, 1)
     ref_lib.dump_constants(os.path.join(HERE, "constants.txt"), DT)
@@ -182,4 +232,5 @@ def main():
 
 
 if __name__ == "__main__":
-    {"ocnd2": ocnd2, "rain_ocnd2": rain_ocnd2}.get(sys.argv[1] if sys.argv[1:] else "", main)()
+    {"ocnd2": ocnd2, "rain_ocnd2": rain_ocnd2, "rain_hail": rain_hail}.get(
+        sys.argv[1] if sys.argv[1:] else "", main)()

@@ -17,9 +17,9 @@ def rel_err(a, b, floor=0.0):
 
 
 MIXING_RATIOS = ("rv_out", "rc_out", "ri_out", "hlc_hrc", "hli_hri", "rvt", "rct", "rrt", "rit",
-                 "rst", "rgt")
-SOURCES = ("rvs", "rcs", "ris", "rrs", "rss", "rgs", "evap3d")
-PRECIP = ("inprc", "inprr", "inpri", "inprs", "inprg")
+                 "rst", "rgt", "rht")
+SOURCES = ("rvs", "rcs", "ris", "rrs", "rss", "rgs", "rhs", "evap3d")
+PRECIP = ("inprc", "inprr", "inpri", "inprs", "inprg", "inprh")
 
 
 def floor_for(name, tol, dt=50.0):

@@ -297,3 +297,115 @@ def test_rain_ice_ocnd2_oracle_matches_compiled_reference_fresh_seed():
     o = oracle_lib.rain_ice_ocnd2(s)
     for k in RAIN_OUT:
         assert rel_err(o[k], r[k], floor_for(k, 1e-12)).max() <= 1e-12, k
+
+
+# ------------------------------------------------------------- hail (KRR=7)
+
+RAIN_HAIL_OUT = RAIN_OUT + ("rhs", "inprh")
+
+
+def _hail_case(g, tag):
+    s = {k[len(tag) + 4:]: g[k] for k in g.files if k.startswith(tag + "_in_")}
+    return s, {k: g[f"{tag}_out_{k}"] for k in RAIN_HAIL_OUT}
+
+
+@pytest.mark.parametrize("tag", ["base", "cov"])
+def test_ice_adjust_hail_matches_reference_golden(golden_dir, tag):
+    """ICE_ADJUST with KRR=7: the hail mixing ratio in the heat capacity."""
+    g = np.load(os.path.join(golden_dir, "rain_ice_hail_fp64.npz"))
+    p = f"{tag}_adj_in_"
+    f = {k[len(p):]: g[k] for k in g.files if k.startswith(p)}
+    o = oracle_lib.ice_adjust_hail(f, float(g["dt"]))
+    for k in ADJ_OUT:
+        ref = g[f"{tag}_adj_out_{k}"]
+        assert np.all(np.isnan(o[k]) == np.isnan(ref)), k
+        assert rel_err(o[k], ref, floor_for(k, 1e-14)).max() <= 1e-14, k
+
+
+@pytest.mark.parametrize("tag", ["base", "cov"])
+def test_rain_ice_hail_matches_reference_golden(golden_dir, tag):
+    """RAIN_ICE with KRR=7 (ICE4_FAST_RH, the SWETH/GWETH/RWETH kernels, graupel-to-hail
+    conversion, hail sedimentation and negativity correction) against the compiled reference."""
+    g = np.load(os.path.join(golden_dir, "rain_ice_hail_fp64.npz"))
+    s, ref = _hail_case(g, tag)
+    o = oracle_lib.rain_ice_hail(s, float(g["dt"]))
+    assert o["status"] == 0
+    for k in RAIN_HAIL_OUT:
+        r = rel_err(o[k], ref[k], floor_for(k, 1e-12))
+        assert r.max() <= 1e-12, (k, r.max())
+
+
+def test_rain_ice_hail_fixtures_exercise_the_branch(golden_dir):
+    """The hail fixtures take every branch of ICE4_FAST_RH (wet growth, dry growth, conversion
+    of hail to graupel, melting) and the graupel-to-hail conversion of ICE4_FAST_RG, and hail
+    changes the results against KRR=6."""
+    g = np.load(os.path.join(golden_dir, "rain_ice_hail_fp64.npz"))
+    total, change = {}, 0.0
+    for tag in ("base", "cov"):
+        s, ref = _hail_case(g, tag)
+        o, fl = oracle_lib.record_branches(oracle_lib.rain_ice_hail, s, float(g["dt"]))
+        for b, n in oracle_lib.branch_counts(fl).items():
+            total[b] = total.get(b, 0) + n
+        plain = oracle_lib.rain_ice(s, float(g["dt"]))
+        change = max(change, np.abs(plain["rgs"] - ref["rgs"]).max() / np.abs(ref["rgs"]).max())
+    assert change > 1e-2, change
+    for b in ("weth", "dryh", "dryhg", "hmltr", "wetgh", "wetg", "dryg"):
+        assert total[b] >= 5, (b, total)
+
+
+def _hail_pipeline(ncol, seed, hail=True, graupel=True):
+    f = synthetic.generate(ncol, 90, seed=seed, variant="pipeline")
+    rng = np.random.Generator(np.random.Philox(seed + 7))
+    if not graupel:
+        f["rg"] = np.zeros_like(f["rg"])
+        f["rgs"] = np.zeros_like(f["rgs"])
+    f["rh"] = np.ascontiguousarray(f["rg"] * rng.uniform(0.2, 2.0, f["rg"].shape)) if hail else np.zeros_like(f["rg"])
+    f["rhs"] = f["rh"] / 50.0
+    return f
+
+
+def test_rain_ice_hail_column_water_budget():
+    """With hail, each column's total water (vapour, the five condensed species and hail)
+    changes only by its surface precipitation, hail included, and every condensed source ends
+    non-negative."""
+    f = _hail_pipeline(64, 20260075)
+    a = oracle_lib.ice_adjust_hail(f)
+    s = ref_lib.pipeline_state_hail(f, a)
+    o = oracle_lib.rain_ice_hail(s)
+    dt = 50.0
+    mass = f["rho_dry_ref"] * f["dzz"]
+    w = ("rvs", "rcs", "rrs", "ris", "rss", "rgs", "rhs")
+    col0 = (mass * sum(s[k] for k in w)).sum(0) * dt
+    col1 = (mass * sum(o[k] for k in w)).sum(0) * dt
+    pr = -1000.0 * (o["inprr"] + o["inprs"] + o["inprg"] + o["inprh"]) * dt
+    assert np.max(np.abs(col1 - col0 - pr) / col0) < 1e-13
+    assert (o["inprh"] > 0).sum() > 10
+    for k in ("rcs", "rrs", "ris", "rss", "rgs", "rhs"):
+        assert np.all(o[k] >= 0.0), k
+
+
+def test_rain_ice_hail_without_hail_or_graupel_is_krr6():
+    """Without hail and graupel nothing feeds the hail processes (no graupel wet growth to
+    convert, no hail to grow, melt or sediment): KRR=7 gives KRR=6's results bit for bit and
+    leaves the hail source and surface rate at 0."""
+    f = _hail_pipeline(48, 20260077, hail=False, graupel=False)
+    a6 = oracle_lib.ice_adjust(f)
+    a7 = oracle_lib.ice_adjust_hail(f)
+    for k in ADJ_OUT:
+        assert np.array_equal(a6[k], a7[k], equal_nan=True), k
+    s = ref_lib.pipeline_state_hail(f, a7)
+    o6 = oracle_lib.rain_ice(s)
+    o7 = oracle_lib.rain_ice_hail(s)
+    for k in RAIN_OUT:
+        assert np.array_equal(o6[k], o7[k]), k
+    assert np.all(o7["rhs"] == 0.0) and np.all(o7["inprh"] == 0.0)
+
+
+@pytest.mark.skipif(not ref_lib.available(), reason="compiled reference not built here")
+def test_rain_ice_hail_oracle_matches_compiled_reference_fresh_seed():
+    f = _hail_pipeline(40, 505)
+    s = ref_lib.pipeline_state_hail(f, ref_lib.ice_adjust_hail(f))
+    r = ref_lib.rain_ice_hail(s, nproma=1)
+    o = oracle_lib.rain_ice_hail(s)
+    for k in RAIN_HAIL_OUT:
+        assert rel_err(o[k], r[k], floor_for(k, 1e-12)).max() <= 1e-12, k
