@@ -70,6 +70,27 @@ __global__ void __launch_bounds__(256) k_sedim(SedimArgs a) {
   sedim_column<R, ZZZ>(a, i);
 }
 
+// ---- hail (KRR=7), FP64: ICE_ADJUST with the hail heat capacity (one thread per point),
+// the sedimentation with hail (one thread per column), the two in one column walk for the
+// fused step, and the microphysics one thread per point (k_micro_points<double, false, true>).
+__global__ void __launch_bounds__(256) k_ice_adjust_hail(AdjustArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncol) return;
+  adjust_at<false, true>(a, i, (size_t)blockIdx.y * a.ld + i);
+}
+
+__global__ void __launch_bounds__(256) k_sedim_hail(SedimArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncol) return;
+  sedim_column<double, false, true>(a, i);
+}
+
+__global__ void __launch_bounds__(64) k_adjust_sedim_hail(AdjustArgs aa, SedimArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncol) return;
+  adjust_sedim_column<double, true>(aa, a, i);
+}
+
 // Sum of v over the 64 lanes of the wavefront (every lane gets the total).
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll
@@ -125,7 +146,7 @@ __global__ void __launch_bounds__(256) k_count_nonfinite(CountArgs a) {
 // slower with three (168 VGPRs, spills, 49-slot parks: +22 %) or four (+120 %) per SIMD.
 template <class R> constexpr int micro_waves() { return 8; }
 template <class R> constexpr size_t micro_tab_bytes() {
-  return ((size_t)ICE3_T_TOTAL * sizeof(R) + 15) / 16 * 16;
+  return ((size_t)ICE3_T_ICE3 * sizeof(R) + 15) / 16 * 16;
 }
 // Park slots per wavefront: what the LDS holds beside the tables, at most 63 (31 in FP64,
 // 63 in the mixed path).
@@ -188,7 +209,7 @@ template <class R, bool OCND2 = false>
 __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
   extern __shared__ __align__(16) unsigned char micro_lds[];
   R *tab = reinterpret_cast<R *>(micro_lds);
-  for (int e = threadIdx.x; e < ICE3_T_TOTAL; e += blockDim.x) tab[e] = (R)a.tab[e];
+  for (int e = threadIdx.x; e < ICE3_T_ICE3; e += blockDim.x) tab[e] = (R)a.tab[e];
   __syncthreads();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   MicroPark<R> &pk = reinterpret_cast<MicroPark<R> *>(micro_lds + micro_tab_bytes<R>())[wid];
@@ -303,14 +324,15 @@ __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
 // The microphysics one thread per point, with the context's sub-step cap a.max_calls read at
 // run time (k_micro has the default cap compiled in: the register allocator fails on
 // k_micro<double> with a run-time bound).  Launched when a context sets another cap; the
-// per-point arithmetic is k_micro's, so the results are the same for the same cap.
-template <class R, bool OCND2 = false>
+// per-point arithmetic is k_micro's, so the results are the same for the same cap.  With HAIL
+// (KRR=7) it is the microphysics of the hail path, with the wet-growth tables of hail.
+template <class R, bool OCND2 = false, bool HAIL = false>
 __global__ void __launch_bounds__(256) k_micro_points(MicroArgs a) {
   const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long ncalls = 0, ncap = 0, nmicro = 0;
   if (q < (unsigned int)a.nlev * (unsigned int)a.ncol) {
     const unsigned int k = q / (unsigned int)a.ncol;
-    const int c = micro_at<R, OCND2>(a, (size_t)k * a.ld + (q - k * (unsigned int)a.ncol), a.tab);
+    const int c = micro_at<R, OCND2, HAIL>(a, (size_t)k * a.ld + (q - k * (unsigned int)a.ncol), a.tab);
     ncalls = c < 0 ? -c : c;
     ncap = c < 0;
     nmicro = c != 0;
@@ -682,7 +704,7 @@ extern "C" long ice3_dump_constants(const ice3_ctx *ctx, char *buf, long buflen)
   D(XTT); D(XLVTT); D(XLSTT); D(XLMTT); D(XCPD); D(XCPV); D(XCL); D(XCI); D(XRD); D(XRV);
   D(XEPSILO); D(XALPW); D(XBETAW); D(XGAMW); D(XALPI); D(XBETAI); D(XGAMI); D(XESTT); D(XP00);
   D(XRHOLW); D(XPI); D(XG); D(XMNH_TINY);
-  put(s, "XRTMIN", P.XRTMIN + 1, 6);
+  put(s, "XRTMIN", P.XRTMIN + 1, 7);
   put(s, "XFRMIN", P.XFRMIN + 1, 40);
   D(XCEXVT); D(XLBR); D(XLBEXR); D(XLBS); D(XLBEXS); D(XLBG); D(XLBEXG); D(XLBI); D(XLBEXI);
   D(XLBDAS_MAX); D(XDI); D(XBS); D(XCXS); D(XCXG); D(XDG); D(XCONC_LAND);
@@ -700,6 +722,12 @@ extern "C" long ice3_dump_constants(const ice3_ctx *ctx, char *buf, long buflen)
   D(XLBSDRYG3); D(XFRDRYG); D(XLBRDRYG1); D(XLBRDRYG2); D(XLBRDRYG3); D(XDRYINTP1R);
   D(XDRYINTP2R); D(XDRYINTP1S); D(XDRYINTP2S); D(XDRYINTP1G); D(XDRYINTP2G); D(XFSEDR);
   D(XEXSEDR); D(XFSEDS); D(XEXSEDS); D(XFSEDG); D(XEXSEDG); D(XFSEDI); D(XEXCSEDI);
+  D(XLBH); D(XLBEXH); D(XCXH); D(XDH); D(XBG); D(XFSEDH); D(XEXSEDH); D(X0DEPH); D(X1DEPH);
+  D(XEX0DEPH); D(XEX1DEPH); D(XCOLIH); D(XCOLEXIH); D(XCOLSH); D(XCOLEXSH); D(XCOLGH);
+  D(XCOLEXGH); D(XFWETH); D(XFSWETH); D(XLBSWETH1); D(XLBSWETH2); D(XLBSWETH3); D(XFGWETH);
+  D(XLBGWETH1); D(XLBGWETH2); D(XLBGWETH3); D(XFRWETH); D(XLBRWETH1); D(XLBRWETH2);
+  D(XLBRWETH3); D(XWETINTP1S); D(XWETINTP2S); D(XWETINTP1G); D(XWETINTP2G); D(XWETINTP1R);
+  D(XWETINTP2R); D(XWETINTP1H); D(XWETINTP2H);
 #undef D
   put(s, "XGAMINC_RIM1", c->h_tab + ICE3_T_RIM1, 80);
   put(s, "XGAMINC_RIM2", c->h_tab + ICE3_T_RIM2, 80);
@@ -709,6 +737,9 @@ extern "C" long ice3_dump_constants(const ice3_ctx *ctx, char *buf, long buflen)
   put(s, "XKER_SACCRG", c->h_tab + ICE3_T_SACCRG, 1600);
   put(s, "XKER_SDRYG", c->h_tab + ICE3_T_SDRYG, 3200);
   put(s, "XKER_RDRYG", c->h_tab + ICE3_T_RDRYG, 1600);
+  put(s, "XKER_SWETH", c->h_tab + ICE3_T_SWETH, 3200);
+  put(s, "XKER_GWETH", c->h_tab + ICE3_T_GWETH, 1600);
+  put(s, "XKER_RWETH", c->h_tab + ICE3_T_RWETH, 1600);
   long n = (long)s.size();
   if (buf && buflen > 0) {
     long m = n < buflen - 1 ? n : buflen - 1;
@@ -1004,6 +1035,98 @@ extern "C" int ice3_adjust_rain_ice_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int
   HIPCHK(hipGetLastError());
   MARK(2);
   HIPCHK(launch_micro_ocnd2(ctx, m, s));
+  MARK(3); MARK(4); MARK(5); MARK(6);
+  return ICE3_OK;
+}
+
+// ---- hail (KRR=7) entry points, FP64
+
+extern "C" int ice3_ice_adjust_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                    const ice3_ice_adjust_in *in, const double *rh,
+                                    const ice3_ice_adjust_out *out, void *stream) {
+  if (!ctx || !in || !out) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev > 65535) return ICE3_EINVAL;
+  NN(in->pabs); NN(in->sigs); NN(in->th); NN(in->exn); NN(in->exn_ref); NN(in->rho_dry_ref);
+  NN(in->rv); NN(in->rc); NN(in->ri); NN(in->rr); NN(in->rs); NN(in->rg); NN(in->cf_mf);
+  NN(in->rc_mf); NN(in->ri_mf); NN(in->rvs); NN(in->rcs); NN(in->ris); NN(in->ths);
+  NN(in->sigqsat); NN(rh);
+  NN(out->t); NN(out->rv_out); NN(out->rc_out); NN(out->ri_out); NN(out->cldfr);
+  NN(out->hlc_hrc); NN(out->hlc_hcf); NN(out->hli_hri); NN(out->hli_hcf); NN(out->cph);
+  NN(out->lv); NN(out->ls); NN(out->rvs); NN(out->rcs); NN(out->ris); NN(out->ths);
+  AdjustArgs a = AdjustArgs{};
+  a.ncol = ncol; a.nlev = nlev; a.ld = ld; a.dt = dt; a.in = *in; a.out = *out; a.rh = rh;
+  hipStream_t s = (hipStream_t)stream;
+  MARK(0);
+  hipLaunchKernelGGL(k_ice_adjust_hail, dim3((ncol + 255) / 256, nlev), dim3(256), 0, s, a);
+  HIPCHK(hipGetLastError());
+  MARK(1); MARK(2); MARK(3); MARK(4); MARK(5); MARK(6);
+  return ICE3_OK;
+}
+
+// The hail microphysics: one thread per point (the sub-step cap is read at run time).
+static hipError_t launch_micro_hail(const MicroArgs &m, hipStream_t s) {
+  const unsigned int npts = (unsigned int)m.nlev * (unsigned int)m.ncol;
+  hipLaunchKernelGGL((k_micro_points<double, false, true>), dim3((npts + 255) / 256), dim3(256), 0,
+                     s, m);
+  return hipGetLastError();
+}
+
+extern "C" int ice3_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                  const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
+                                  const ice3_rain_ice_out *out, const ice3_hail *hail,
+                                  const ice3_stats *stats, void *stream) {
+  if (!ctx || !in || !io || !out || !hail) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev > 65535) return ICE3_EINVAL;
+  NN(in->exn); NN(in->dzz); NN(in->rhodref); NN(in->exn_ref); NN(in->pabs); NN(in->cldfr);
+  NN(in->sigs); NN(in->tht); NN(in->rvt); NN(in->rct); NN(in->rrt); NN(in->rit); NN(in->rst);
+  NN(in->rgt);
+  NN(io->ths); NN(io->rvs); NN(io->rcs); NN(io->rrs); NN(io->ris); NN(io->rss); NN(io->rgs);
+  NN(io->ci);
+  NN(out->evap3d); NN(out->rainfr); NN(out->inprc); NN(out->inprr); NN(out->inpri);
+  NN(out->inprs); NN(out->inprg);
+  NN(hail->rht); NN(hail->rhs); NN(hail->inprh);
+  SedimArgs sa;
+  MicroArgs m;
+  rain_ice_args(ncol, nlev, ld, dt, ctx->d_tab, in, io, out, stats, sa, m);
+  m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
+  sa.src_rh = hail->rhs; sa.dst_rh = hail->rhs; sa.inprh = hail->inprh;
+  m.t[IRH] = hail->rht; m.src[IRH] = hail->rhs; m.dst[IRH] = hail->rhs;
+  hipStream_t s = (hipStream_t)stream;
+  MARK(0); MARK(1);
+  hipLaunchKernelGGL(k_sedim_hail, dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
+  HIPCHK(hipGetLastError());
+  MARK(2);
+  HIPCHK(launch_micro_hail(m, s));
+  MARK(3); MARK(4); MARK(5); MARK(6);
+  return ICE3_OK;
+}
+
+extern "C" int ice3_adjust_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                         const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                                         const ice3_pipeline_hail *hail, const ice3_stats *stats,
+                                         void *stream) {
+  if (!ctx || !in || !out || !hail) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev > 65535) return ICE3_EINVAL;
+  if (bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
+  NN(hail->rh); NN(hail->rhs); NN(hail->rhs_out); NN(hail->inprh);
+  if ((const double *)hail->rhs_out == hail->rhs) return ICE3_EINVAL;
+  AdjustArgs aa;
+  SedimArgs sa;
+  MicroArgs m;
+  pipeline_args(ncol, nlev, ld, dt, ctx->d_tab, in, out, stats, aa, sa, m);
+  m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
+  aa.rh = hail->rh;
+  sa.src_rh = hail->rhs; sa.dst_rh = hail->rhs_out; sa.inprh = hail->inprh;
+  m.t[IRH] = hail->rhs; m.src[IRH] = hail->rhs_out; m.dst[IRH] = hail->rhs_out;
+  hipStream_t s = (hipStream_t)stream;
+  MARK(0);
+  hipLaunchKernelGGL(k_adjust_sedim_hail, dim3((ncol + 63) / 64), dim3(64), 0, s, aa, sa);
+  HIPCHK(hipGetLastError());
+  MARK(1); MARK(2);
+  HIPCHK(launch_micro_hail(m, s));
   MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;
 }

@@ -2,7 +2,8 @@
 //
 // Restates INI_CST (aux/mode_ini_cst.F90:77-193) and the parts of INI_RAIN_ICE
 // (micro/mode_ini_rain_ice.F90) that the ice_adjust / rain_ice path reads, for
-// the pinned AROME ICE3 configuration: HPROGRAM='AROME', HCLOUD='ICE3' (KRR=6),
+// the pinned AROME ICE3 configuration: HPROGRAM='AROME', HCLOUD='ICE3' (KRR=6) and
+// HCLOUD='ICE4' (KRR=7: XRTMIN(7) and the hail constants),
 // LRED=F, LSNOW_T=F (so LNEWCOEFF=F), CPRISTINE_ICE='PLAT', LCRIAUTI=F,
 // XFRACM90=0.1, XFRMIN(16)=0.  The special functions are restated from the same
 // published methods the reference uses: GAMMA (aux/gamma.F90, 6-term Lanczos of
@@ -124,6 +125,7 @@ void ice3_build_params(ice3_params *P, double *tables) {
   const double XPI = P->XPI;
   P->XRTMIN[1] = 1.0E-20; P->XRTMIN[2] = 1.0E-20; P->XRTMIN[3] = 1.0E-20;
   P->XRTMIN[4] = 1.0E-20; P->XRTMIN[5] = 1.0E-15; P->XRTMIN[6] = 1.0E-15;
+  P->XRTMIN[7] = 1.0E-15;  // HCLOUD='ICE4' (:258)
   const double XAR = (XPI / 6.0) * P->XRHOLW, XBR = 3.0, XCR = 842., XDR = 0.8, XCCR = 8.E6;
   const double ZXR = -1., XF0R = 1.00, XF1R = 0.26, XC1R = 1. / 2.;
   const double XAI = 0.82, XBI = 2.5, XC_I = 800., XDI = 1.0, XC1I = 1. / XPI, XF0I = 1.00,
@@ -132,8 +134,10 @@ void ice3_build_params(ice3_params *P, double *tables) {
                XF1S = 0.28, XC1S = 1. / XPI;
   const double XAG = 19.6, XBG = 2.8, XCG = 124., XDG = 0.66, XCCG = 5.E5, XCXG = -0.5,
                XF0G = 0.86, XF1G = 0.28, XC1G = 1. / 2.;
+  const double XAH = 470., XBH = 3.0, XCH = 207., XDH = 0.64, XCCH = 4.E4, XCXH = -1.0,
+               XF0H = 0.86, XF1H = 0.28, XC1H = 1. / 2.;  // hail (:375-392)
   const double XALPHAR = 1.0, XNUR = 1.0, XALPHAI = 3.0, XNUI = 3.0, XALPHAS = 1.0, XNUS = 1.0,
-               XALPHAG = 1.0, XNUG = 1.0;
+               XALPHAG = 1.0, XNUG = 1.0, XALPHAH = 1.0, XNUH = 8.0;
   P->XDI = XDI; P->XBS = XBS; P->XCXS = XCXS; P->XCXG = XCXG; P->XDG = XDG;
   P->XLBEXR = 1.0 / (-1.0 - XBR);
   P->XLBR = std::pow(XAR * XCCR * momg(XALPHAR, XNUR, XBR), -P->XLBEXR);
@@ -143,6 +147,9 @@ void ice3_build_params(ice3_params *P, double *tables) {
   P->XLBS = std::pow(XAS * XCCS * momg(XALPHAS, XNUS, XBS), -P->XLBEXS);
   P->XLBEXG = 1.0 / (XCXG - XBG);
   P->XLBG = std::pow(XAG * XCCG * momg(XALPHAG, XNUG, XBG), -P->XLBEXG);
+  P->XLBEXH = 1.0 / (XCXH - XBH);
+  P->XLBH = std::pow(XAH * XCCH * momg(XALPHAH, XNUH, XBH), -P->XLBEXH);
+  P->XCXH = XCXH; P->XDH = XDH; P->XBG = XBG;
   const double ZCONC_MAX = 1.E6;
   P->XLBDAS_MAX = 100000.0;
   if (XCCS > 0. && XCXS > 0.) P->XLBDAS_MAX = std::pow(ZCONC_MAX / XCCS, 1. / XCXS);
@@ -175,6 +182,9 @@ void ice3_build_params(ice3_params *P, double *tables) {
   P->XEXSEDG = (XBG + XDG - XCXG) / (XBG - XCXG);
   P->XFSEDG = XCG * XAG * XCCG * momg(XALPHAG, XNUG, XBG + XDG) *
               std::pow(XAG * XCCG * momg(XALPHAG, XNUG, XBG), -P->XEXSEDG) * std::pow(ZRHO00, XCEXVT);
+  P->XEXSEDH = (XBH + XDH - XCXH) / (XBH - XCXH);
+  P->XFSEDH = XCH * XAH * XCCH * momg(XALPHAH, XNUH, XBH + XDH) *
+              std::pow(XAH * XCCH * momg(XALPHAH, XNUH, XBH), -P->XEXSEDH) * std::pow(ZRHO00, XCEXVT);
   // nucleation (:555-591)
   P->XNU10 = 50.; P->XALPHA1 = 4.5; P->XBETA1 = 0.6;
   P->XNU20 = 1000.; P->XALPHA2 = 12.96; P->XBETA2 = 0.639;
@@ -193,6 +203,10 @@ void ice3_build_params(ice3_params *P, double *tables) {
   P->X1DEPG = (4.0 * XPI) * XCCG * XC1G * XF1G * std::sqrt(XCG) * momg(XALPHAG, XNUG, 0.5 * XDG + 1.5);
   P->XEX0DEPG = XCXG - 1.0;
   P->XEX1DEPG = XCXG - 0.5 * (XDG + 3.0);
+  P->X0DEPH = (4.0 * XPI) * XCCH * XC1H * XF0H * momg(XALPHAH, XNUH, 1.);
+  P->X1DEPH = (4.0 * XPI) * XCCH * XC1H * XF1H * std::sqrt(XCH) * momg(XALPHAH, XNUH, 0.5 * XDH + 1.5);
+  P->XEX0DEPH = XCXH - 1.0;
+  P->XEX1DEPH = XCXH - 0.5 * (XDH + 3.0);
   // aggregation / autoconversion (:648-717)
   P->XTIMAUTI = 1.E-3;
   P->XTEXAUTI = 0.015;
@@ -290,12 +304,51 @@ void ice3_build_params(ice3_params *P, double *tables) {
   zrate = std::log(1.0E7 / 1.0E3) / (double)(P->NDRYLBDAG - 1);
   P->XDRYINTP1G = 1.0 / zrate;
   P->XDRYINTP2G = 1.0 - std::log(1.0E3) / zrate;
+  // hail wet and dry growth (:1225-1296)
+  P->XCOLIH = 0.01;
+  P->XCOLEXIH = 0.1;
+  P->XFWETH = (XPI / 4.0) * XCCH * XCH * std::pow(ZRHO00, XCEXVT) * momg(XALPHAH, XNUH, XDH + 2.0);
+  P->XCOLSH = 0.01;
+  P->XCOLEXSH = 0.1;
+  P->XFSWETH = (XPI / 4.0) * XCCH * XCCS * XAS * std::pow(ZRHO00_SNOW, XCEXVT);
+  P->XLBSWETH1 = momg(XALPHAH, XNUH, 2.) * momg(XALPHAS, XNUS, XBS);
+  P->XLBSWETH2 = 2. * momg(XALPHAH, XNUH, 1.) * momg(XALPHAS, XNUS, XBS + 1.);
+  P->XLBSWETH3 = momg(XALPHAS, XNUS, XBS + 2.);
+  P->XCOLGH = 0.01;
+  P->XCOLEXGH = 0.1;
+  P->XFGWETH = (XPI / 4.0) * XCCH * XCCG * XAG * std::pow(ZRHO00, XCEXVT);
+  P->XLBGWETH1 = momg(XALPHAH, XNUH, 2.) * momg(XALPHAG, XNUG, XBG);
+  P->XLBGWETH2 = 2. * momg(XALPHAH, XNUH, 1.) * momg(XALPHAG, XNUG, XBG + 1.);
+  P->XLBGWETH3 = momg(XALPHAG, XNUG, XBG + 2.);
+  P->XFRWETH = (XPI / 4.0) * XCCH * XCCR * XAR * std::pow(ZRHO00, XCEXVT);
+  P->XLBRWETH1 = momg(XALPHAH, XNUH, 2.) * momg(XALPHAR, XNUR, XBR);
+  P->XLBRWETH2 = 2. * momg(XALPHAH, XNUH, 1.) * momg(XALPHAR, XNUR, XBR + 1.);
+  P->XLBRWETH3 = momg(XALPHAR, XNUR, XBR + 2.);
+  P->NWETLBDAS = 80;
+  zrate = std::log(2.5E9 / 2.5E1) / (double)(P->NWETLBDAS - 1);
+  P->XWETINTP1S = 1.0 / zrate;
+  P->XWETINTP2S = 1.0 - std::log(2.5E1) / zrate;
+  P->NWETLBDAG = 40;
+  zrate = std::log(1.0E7 / 1.0E3) / (double)(P->NWETLBDAG - 1);
+  P->XWETINTP1G = 1.0 / zrate;
+  P->XWETINTP2G = 1.0 - std::log(1.0E3) / zrate;
+  P->NWETLBDAR = 40;
+  zrate = std::log(1.0E7 / 1.0E3) / (double)(P->NWETLBDAR - 1);
+  P->XWETINTP1R = 1.0 / zrate;
+  P->XWETINTP2R = 1.0 - std::log(1.0E3) / zrate;
+  P->NWETLBDAH = 40;
+  zrate = std::log(1.0E7 / 1.0E3) / (double)(P->NWETLBDAH - 1);
+  P->XWETINTP1H = 1.0 / zrate;
+  P->XWETINTP2H = 1.0 - std::log(1.0E3) / zrate;
   // tabulated collection kernels
   std::memcpy(tables + ICE3_T_RACCSS, XKER_RACCSS_DATA, sizeof(XKER_RACCSS_DATA));
   std::memcpy(tables + ICE3_T_RACCS, XKER_RACCS_DATA, sizeof(XKER_RACCS_DATA));
   std::memcpy(tables + ICE3_T_SACCRG, XKER_SACCRG_DATA, sizeof(XKER_SACCRG_DATA));
   std::memcpy(tables + ICE3_T_SDRYG, XKER_SDRYG_DATA, sizeof(XKER_SDRYG_DATA));
   std::memcpy(tables + ICE3_T_RDRYG, XKER_RDRYG_DATA, sizeof(XKER_RDRYG_DATA));
+  std::memcpy(tables + ICE3_T_SWETH, XKER_SWETH_DATA, sizeof(XKER_SWETH_DATA));
+  std::memcpy(tables + ICE3_T_GWETH, XKER_GWETH_DATA, sizeof(XKER_GWETH_DATA));
+  std::memcpy(tables + ICE3_T_RWETH, XKER_RWETH_DATA, sizeof(XKER_RWETH_DATA));
   // logarithms used by the device code's power evaluations
   P->LN_XLBR = std::log(P->XLBR);
   P->LN_XLBS = std::log(P->XLBS);
@@ -305,6 +358,7 @@ void ice3_build_params(ice3_params *P, double *tables) {
   P->LN_XTT = std::log(P->XTT);
   P->LN_1E8 = std::log(1.E8);
   P->LN_10 = std::log(10.);
+  P->LN_XLBH = std::log(P->XLBH);
 }
 
 // INI_TIWMX (micro/mode_ini_tiwmx.F90:14-49) for the tables the OCND2 branch reads, with the

@@ -1,7 +1,8 @@
 // ice3_params.h -- constants and lookup tables of the AROME ICE3 configuration
-// (KRR=6), as INI_CST (aux/mode_ini_cst.F90:77-193) and INI_RAIN_ICE
-// (micro/mode_ini_rain_ice.F90:10-1571) define them.  Plain data shared by the
-// host builder (ice3_params.cpp) and the device kernels.
+// (KRR=6) and of its hail variant (KRR=7, HCLOUD='ICE4'), as INI_CST
+// (aux/mode_ini_cst.F90:77-193) and INI_RAIN_ICE (micro/mode_ini_rain_ice.F90:10-1571)
+// define them.  Plain data shared by the host builder (ice3_params.cpp) and the device
+// kernels.
 #pragma once
 
 struct ice3_params {
@@ -11,7 +12,7 @@ struct ice3_params {
   // NEB_t (pinned: 'T' ice fraction between these temperatures)
   double XTMINMIX, XTMAXMIX;
   // RAIN_ICE_DESCR_t
-  double XRTMIN[7];  // 1-based, [1..6]
+  double XRTMIN[8];  // 1-based, [1..7]; [7] (hail) is read by the KRR=7 kernels only
   double XCEXVT, XLBR, XLBEXR, XLBS, XLBEXS, XLBG, XLBEXG, XLBI, XLBEXI, XLBDAS_MAX, XDI, XBS,
       XCXS, XCXG, XDG, XCONC_LAND;
   // RAIN_ICE_PARAM_t
@@ -26,10 +27,18 @@ struct ice3_params {
       XLBSDRYG1, XLBSDRYG2, XLBSDRYG3, XFRDRYG, XLBRDRYG1, XLBRDRYG2, XLBRDRYG3, XDRYINTP1R,
       XDRYINTP2R, XDRYINTP1S, XDRYINTP2S, XDRYINTP1G, XDRYINTP2G, XFSEDR, XEXSEDR, XFSEDS,
       XEXSEDS, XFSEDG, XEXSEDG, XFSEDI, XEXCSEDI;
+  // hail, KRR=7 (RAIN_ICE_DESCR_t, RAIN_ICE_PARAM_t; micro/mode_ini_rain_ice.F90:375-392,
+  // 436-437,462-463,547-549,633-636,1225-1296)
+  double XLBH, XLBEXH, XCXH, XDH, XBG, XFSEDH, XEXSEDH, X0DEPH, X1DEPH, XEX0DEPH, XEX1DEPH,
+      XCOLIH, XCOLEXIH, XCOLSH, XCOLEXSH, XCOLGH, XCOLEXGH, XFWETH, XFSWETH, XLBSWETH1,
+      XLBSWETH2, XLBSWETH3, XFGWETH, XLBGWETH1, XLBGWETH2, XLBGWETH3, XFRWETH, XLBRWETH1,
+      XLBRWETH2, XLBRWETH3, XWETINTP1S, XWETINTP2S, XWETINTP1G, XWETINTP2G, XWETINTP1R,
+      XWETINTP2R, XWETINTP1H, XWETINTP2H;
   double XFRMIN[41];  // 1-based, [1..40]: the PARAM_ICE XFRMIN_NAM tuning values (OCND2 branch)
   int NGAMINC, NACCLBDAS, NACCLBDAR, NDRYLBDAR, NDRYLBDAS, NDRYLBDAG;
+  int NWETLBDAS, NWETLBDAG, NWETLBDAR, NWETLBDAH;
   // natural logarithms of constants, for powers evaluated as exp(c*ln x) on the device
-  double LN_XLBR, LN_XLBS, LN_XLBG, LN_XLBI, LN_XLBDAS_MAX, LN_XTT, LN_1E8, LN_10;
+  double LN_XLBR, LN_XLBS, LN_XLBG, LN_XLBI, LN_XLBDAS_MAX, LN_XTT, LN_1E8, LN_10, LN_XLBH;
 };
 
 // Table block: one contiguous float64 array, offsets in doubles.
@@ -42,7 +51,11 @@ enum ice3_table_offsets {
   ICE3_T_SACCRG = ICE3_T_RACCS + 1600,  // (40,40) lbda_r x lbda_s
   ICE3_T_SDRYG = ICE3_T_SACCRG + 1600,  // (40,80) lbda_g x lbda_s
   ICE3_T_RDRYG = ICE3_T_SDRYG + 3200,   // (40,40) lbda_g x lbda_r
-  ICE3_T_TOTAL = ICE3_T_RDRYG + 1600    // 9840 doubles
+  ICE3_T_SWETH = ICE3_T_RDRYG + 1600,   // (40,80) lbda_h x lbda_s, hail wet growth
+  ICE3_T_GWETH = ICE3_T_SWETH + 3200,   // (40,40) lbda_h x lbda_g
+  ICE3_T_RWETH = ICE3_T_GWETH + 1600,   // (40,40) lbda_h x lbda_r
+  ICE3_T_TOTAL = ICE3_T_RWETH + 1600,   // 16240 doubles
+  ICE3_T_ICE3 = ICE3_T_SWETH            // 9840: the part the KRR=6 kernels read (staged in LDS)
 };
 
 // Builds every constant and the table block (tables must hold ICE3_T_TOTAL doubles).

@@ -24,7 +24,7 @@ namespace ice3 {
 // (ice3_params_gen.inc, generated from ice3_build_params): every P.X below is a
 // literal in the generated code.
 
-enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6 };
+enum { ITH = 0, IRV = 1, IRC = 2, IRR = 3, IRI = 4, IRS = 5, IRG = 6, IRH = 7 };
 
 // ---------------------------------------------------------------- FP64 exp and log
 // The physics evaluates about ten FP64 logarithms and as many exponentials per point and
@@ -215,18 +215,20 @@ __device__ __forceinline__ void icecloud(double p, double z, double dz, double t
 // the branch of PARAM_ICE LOCND2 = .TRUE. (x then carries its inputs and outputs):
 // ZPRIFACT = 0 (condensation.F90:265), ICECLOUD and the tabulated saturation pressures
 // (:373-394), no ice fraction (:412), and the liquid condensate with its noise check and the
-// separate ice cloud (:588-637).
-template <bool OCND2 = false>
+// separate ice cloud (:588-637).  HAIL (KRR=7): the hail mixing ratio rh enters the heat
+// capacity (micro/ice_adjust.F90:503-507).
+template <bool OCND2 = false, bool HAIL = false>
 __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs,
                                         double th, double exn, double exnref, double rhodref,
                                         double rv, double rc, double ri, double rr, double rs,
                                         double rg, double cf_mf, double rc_mf, double ri_mf,
                                         double &rvs, double &rcs, double &ris, double &ths,
-                                        AdjustOut &o, Ocnd2Point *x = nullptr) {
+                                        AdjustOut &o, Ocnd2Point *x = nullptr, double rh = 0.) {
   double zt = th * exn;
   const double lv = P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT);
   const double ls = P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT);
-  const double cph = P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg);
+  const double cph = HAIL ? P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg + rh)
+                          : P.XCPD + P.XCPV * rv + P.XCL * (rc + rr) + P.XCI * (ri + rs + rg);
   const double prifact = OCND2 ? 0. : 1.0;
   const double rt = rv + rc + ri * prifact;
   double pv, piv, frac, esw_t = 0.;
@@ -436,6 +438,15 @@ template <class R> struct StepState {
   R ssi;
 };
 
+// With hail (KRR=7) also ZRH_TEND (micro/mode_ice4_stepping.F90:156) and the hail melting
+// rate of ZBU_INST.
+template <class R> struct StepStateHail : StepState<R> {
+  R rh_rcweth, rh_rrweth, rh_ridryh, rh_riweth, rh_rsdryh, rh_rsweth, rh_rgdryh, rh_rgweth,
+      rh_freez1, rh_freez2, rhmltr;
+};
+template <class R, bool HAIL> struct StepStateFor { using type = StepState<R>; };
+template <class R> struct StepStateFor<R, true> { using type = StepStateHail<R>; };
+
 // INTERP_MICRO_1D (micro/interp_micro.func.h:34-88) for one point
 // (the argument is ln x: the callers keep the logarithm of each slope parameter)
 template <class R, class TB>
@@ -513,18 +524,26 @@ template <class R> __device__ __forceinline__ R powl_(R lnx, R c) { return rexp<
 // water; the deposition tuning XFRMIN(40)/XFRMIN(39) (micro/mode_ice4_slow.F90:92-100); no
 // snow-graupel collection (micro/mode_ice4_fast_rg.F90:185-188,207-209); ICE4_FAST_RI_RS
 // (micro/mode_ice4_fast_ri_rs.F90) in place of ICE4_FAST_RI; their PA terms (:625-691).
-template <class R, bool OCND2 = false, class TB>
+//
+// HAIL (KRR=7, HCLOUD='ICE4'; S is then a StepStateHail): pvart, pa and pb have an eighth
+// entry, hail; the hail slope parameter (:426-432), the conversion of wet-growing graupel
+// into hail in ICE4_FAST_RG (micro/mode_ice4_fast_rg.F90:330-351), ICE4_FAST_RH
+// (micro/mode_ice4_fast_rh.F90) and their PA / PB terms (:619-716).
+template <class R, bool OCND2 = false, bool HAIL = false, class TB, class SS>
 __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref, double lvfact,
                                   double lsfact, double pres, double cf, double sigma_rc,
                                   double &cit, double pt, const double *pvart,
-                                  StepState<R> &S, const PointConst<R> &pc,
+                                  SS &S, const PointConst<R> &pc,
                                   const TB *__restrict__ tab, double *pa, double *pb,
                                   const MicroOcnd2 *x = nullptr) {
   static_assert(!OCND2 || sizeof(R) == sizeof(double), "the OCND2 branch is FP64");
+  static_assert(!(OCND2 && HAIL), "OCND2 with hail is not built");
+  constexpr int NV = HAIL ? 8 : 7;
   double zt = pt;
-  double zv[7];
+  double zv[NV];
 #pragma unroll
-  for (int jv = 0; jv < 7; ++jv) { zv[jv] = pvart[jv]; pa[jv] = 0.; pb[jv] = 0.; }
+  for (int jv = 0; jv < NV; ++jv) { zv[jv] = pvart[jv]; pa[jv] = 0.; pb[jv] = 0.; }
+  double rrhong_mr = 0.;  // RRHONG of the hard call (graupel produced at once)
   R ka = 0, dv = 0, ai = 0, cj = 0, lnt = 0, esi = 0, esw = 0;
   double esi_tab = 0., colf = 1., acrf = 1.;  // OCND2: tabulated ZESI, ZCOLF, ZACRF
   if (!soft) {
@@ -543,6 +562,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     zt = zv[ITH] * exn;
     zv[IRR] = zv[IRR] - rrhong;
     zv[IRG] = zv[IRG] + rrhong;
+    rrhong_mr = rrhong;
     // RIMLTC (micro/mode_ice4_rimltc.F90:64-74)
     if (zv[IRI] > 0. && zt > P.XTT)
       rimltc = dmin(zv[IRI], dmax(0., (zv[ITH] - P.XTT / exn) / (lsfact - lvfact)));
@@ -645,6 +665,13 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       lbdag = rexp<R>(llg);
     }
   }
+  R lbdah = 0, llh = 0, vent_h = 0;
+  if constexpr (HAIL) {
+    if (!soft && zv[IRH] > 0.) {
+      llh = (R)P.LN_XLBH + (R)P.XLBEXH * rlog<R>(rho * dmax((R)zv[IRH], (R)P.XRTMIN[7]));
+      lbdah = rexp<R>(llh);
+    }
+  }
   const R rhomcexvt = soft ? (R)0 : pc.rhomcexvt;
   // powers of the snow and graupel slopes used by several processes of the hard call
   // (vapour deposition, freezing budget, melting; accretion and dry growth)
@@ -658,6 +685,9 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       vent_g = (R)P.X0DEPG * powl_<R>(llg, (R)P.XEX0DEPG) + (R)P.X1DEPG * cj * powl_<R>(llg, (R)P.XEX1DEPG);
       lbg_cxg = powl_<R>(llg, (R)P.XCXG);
     }
+    if constexpr (HAIL)
+      if (zv[IRH] > 0.)
+        vent_h = (R)P.X0DEPH * powl_<R>(llh, (R)P.XEX0DEPH) + (R)P.X1DEPH * cj * powl_<R>(llh, (R)P.XEX1DEPH);
   }
   const R ssi = S.ssi;
   const bool rv_ok = zv[IRV] > P.XRTMIN[1], rc_ok = zv[IRC] > P.XRTMIN[2],
@@ -868,6 +898,18 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   } else {
     S.rg_freez1 = 0; S.rg_freez2 = 0;
   }
+  // part of the wet growth converted into hail (micro/mode_ice4_fast_rg.F90:330-351), from
+  // the graupel the other processes produce (micro/mode_ice4_tendencies.F90:509-514)
+  R rwetgh = 0, rwetgh_mr = 0;
+  if constexpr (HAIL) {
+    if (ldwetg) {
+      const R rgsi = S.rvdepg + S.rsmltg + rraccsg + rsaccrg + rcrimsg + rsrimcg;
+      const R rgsi_mr = (R)rrhong_mr;
+      rwetgh = (dmax((R)0., rgsi + S.ricfrrg + S.rrcfrig) + rwetg_init) * rdryg_init /
+               (rwetg_init + rdryg_init);
+      rwetgh_mr = dmax((R)0., rgsi_mr) * rdryg_init / (rwetg_init + rdryg_init);
+    }
+  }
   R rrwetg = 0, rcwetg = 0, riwetg = 0, rswetg = 0, rcdryg = 0, rrdryg = 0, ridryg = 0, rsdryg = 0;
   if (ldwetg) {
     rrwetg = -(S.rg_riwetg + S.rg_rswetg + S.rg_rcdryg - rwetg_init);
@@ -885,6 +927,98 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
                               (S.rg_rcdryg + S.rg_rrdryg) * (rho * (R)P.XCL * ((R)P.XTT - t))) / (rho * (R)P.XLMTT));
     }
   } else S.rgmltr = 0;
+
+  // ICE4_FAST_RH (micro/mode_ice4_fast_rh.F90:109-405), KRR=7: LNULLWETH=T, LWETHPOST=T,
+  // LCONVHG=T, LEVLIMIT=T, LNEWCOEFF=F.  Collection of c, i, s, g, r by hail, the wet/dry
+  // growth choice, hail to graupel in dry growth, melting.
+  R rcweth = 0, riweth = 0, rsweth = 0, rgweth = 0, rrweth = 0;
+  R rcdryh = 0, ridryh = 0, rsdryh = 0, rrdryh = 0, rgdryh = 0, rdryhg = 0;
+  if constexpr (HAIL) {
+    const bool rh_ok = zv[IRH] > P.XRTMIN[7];
+    const R lhx = (!soft && rh_ok) ? powl_<R>(llh, (R)(P.XCXH - P.XDH - 2.0)) * rhomcexvt : (R)0;
+    const R lbh_cxh = (!soft && rh_ok) ? powl_<R>(llh, (R)P.XCXH) : (R)0;
+    if (rh_ok && rc_ok) {
+      if (!soft) S.rh_rcweth = (R)P.XFWETH * rc * lhx;
+    } else S.rh_rcweth = 0;
+    if (rh_ok && ri_ok) {
+      if (!soft) {
+        S.rh_riweth = (R)P.XFWETH * ri * lhx;
+        S.rh_ridryh = S.rh_riweth * ((R)P.XCOLIH * rexp<R>((R)P.XCOLEXIH * (t - (R)P.XTT)));
+      }
+    } else {
+      S.rh_riweth = 0; S.rh_ridryh = 0;
+    }
+    if (!(rh_ok && rs_ok)) { S.rh_rsweth = 0; S.rh_rsdryh = 0; }
+    if (!soft && rh_ok && rs_ok) {
+      const Idx2<R> ix = interp2d_idx<R>(llh, lls, P.NWETLBDAH, P.NWETLBDAS, (R)P.XWETINTP1H,
+                                         (R)P.XWETINTP2H, (R)P.XWETINTP1S, (R)P.XWETINTP2S);
+      const R zzw = interp2d<R>(ix, P.NWETLBDAH, tab + ICE3_T_SWETH);
+      S.rh_rsweth = (R)P.XFSWETH * zzw * lbs_cxs_bs * lbh_cxh * pc.rhom *
+                    ((R)P.XLBSWETH1 / (sq(lbdah)) + (R)P.XLBSWETH2 / (lbdah * lbdas) + (R)P.XLBSWETH3 / (sq(lbdas)));
+      S.rh_rsdryh = S.rh_rsweth * ((R)P.XCOLSH * rexp<R>((R)P.XCOLEXSH * (t - (R)P.XTT)));
+    }
+    if (!(rh_ok && rg_ok)) { S.rh_rgweth = 0; S.rh_rgdryh = 0; }
+    if (!soft && rh_ok && rg_ok) {
+      const Idx2<R> ix = interp2d_idx<R>(llh, llg, P.NWETLBDAH, P.NWETLBDAG, (R)P.XWETINTP1H,
+                                         (R)P.XWETINTP2H, (R)P.XWETINTP1G, (R)P.XWETINTP2G);
+      const R zzw = interp2d<R>(ix, P.NWETLBDAH, tab + ICE3_T_GWETH);
+      S.rh_rgweth = (R)P.XFGWETH * zzw * powl_<R>(llg, (R)(P.XCXG - P.XBG)) * lbh_cxh * pc.rhom *
+                    ((R)P.XLBGWETH1 / (sq(lbdah)) + (R)P.XLBGWETH2 / (lbdah * lbdag) + (R)P.XLBGWETH3 / (sq(lbdag)));
+      S.rh_rgdryh = S.rh_rgweth;
+      // wet graupel keeps its collection efficiency (:221-223)
+      if (!ldwetg) S.rh_rgdryh = S.rh_rgdryh * ((R)P.XCOLGH * rexp<R>((R)P.XCOLEXGH * (t - (R)P.XTT)));
+    }
+    if (!(rh_ok && rr_ok)) S.rh_rrweth = 0;
+    if (!soft && rh_ok && rr_ok) {
+      const Idx2<R> ix = interp2d_idx<R>(llh, llr, P.NWETLBDAH, P.NWETLBDAR, (R)P.XWETINTP1H,
+                                         (R)P.XWETINTP2H, (R)P.XWETINTP1R, (R)P.XWETINTP2R);
+      const R zzw = interp2d<R>(ix, P.NWETLBDAH, tab + ICE3_T_RWETH);
+      const R lr2 = lbdar * lbdar;
+      S.rh_rrweth = (R)P.XFRWETH * zzw * ((R)1.0 / (lr2 * lr2)) * lbh_cxh * pc.rhom *
+                    ((R)P.XLBRWETH1 / (sq(lbdah)) + (R)P.XLBRWETH2 / (lbdah * lbdar) + (R)P.XLBRWETH3 / (lr2));
+    }
+    const R rdryh_init = S.rh_rcweth + S.rh_ridryh + S.rh_rsdryh + S.rh_rrweth + S.rh_rgdryh;
+    R rweth_init = 0;
+    bool llweth = false, lldryh = false;
+    if (rh_ok) {
+      if (!soft) {
+        R f1 = rv * (R)pres / ((R)P.XEPSILO + rv);
+        f1 = dmin(f1, esi);
+        f1 = ka * ((R)P.XTT - t) + (dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - f1) / ((R)P.XRV * t));
+        f1 = f1 * vent_h / (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
+        S.rh_freez1 = f1;
+        S.rh_freez2 = (rho * ((R)P.XLMTT + (R)(P.XCI - P.XCL) * ((R)P.XTT - t))) / (rho * ((R)P.XLMTT - (R)P.XCL * ((R)P.XTT - t)));
+      }
+      const R isg = S.rh_riweth + S.rh_rsweth + S.rh_rgweth;
+      rweth_init = dmax(isg, dmax((R)0., S.rh_freez1 + S.rh_freez2 * isg));
+      llweth = dmax((R)0., rweth_init - S.rh_riweth - S.rh_rsweth - S.rh_rgweth) <=
+                   dmax((R)0., rdryh_init - S.rh_ridryh - S.rh_rsdryh - S.rh_rgdryh) &&
+               rdryh_init > (R)0.;
+      lldryh = cold && rdryh_init > (R)1.E-20 &&
+               dmax((R)0., rweth_init - S.rh_riweth - S.rh_rsweth) >
+                   dmax((R)0., rdryh_init - S.rh_ridryh - S.rh_rsdryh);
+    } else {
+      S.rh_freez1 = 0; S.rh_freez2 = 0;
+    }
+    if (llweth) {
+      rcweth = S.rh_rcweth; riweth = S.rh_riweth; rsweth = S.rh_rsweth; rgweth = S.rh_rgweth;
+      rrweth = (rweth_init - S.rh_riweth - S.rh_rsweth - S.rh_rgweth - S.rh_rcweth);
+    }
+    if (lldryh) {
+      rcdryh = S.rh_rcweth; ridryh = S.rh_ridryh; rsdryh = S.rh_rsdryh; rrdryh = S.rh_rrweth;
+      rgdryh = S.rh_rgdryh;
+      rdryhg = rdryh_init * rweth_init / (rdryh_init + rweth_init);
+    }
+    if (rh_ok && warm) {
+      if (!soft) {
+        R m = rv * (R)pres / ((R)P.XEPSILO + rv);
+        m = dmin(m, esw);
+        m = ka * ((R)P.XTT - t) + dv * ((R)P.XLVTT + (R)(P.XCPV - P.XCL) * (t - (R)P.XTT)) * ((R)P.XESTT - m) / ((R)P.XRV * t);
+        S.rhmltr = dmax((R)0., (-m * vent_h - (S.rh_rcweth + S.rh_rrweth) * (rho * (R)P.XCL * ((R)P.XTT - t))) /
+                                   (rho * (R)P.XLMTT));
+      }
+    } else S.rhmltr = 0;
+  }
 
   // ICE4_FAST_RI (micro/mode_ice4_fast_ri.F90:70-81); not called with OCND2 (tendencies :568-578)
   if (OCND2) {
@@ -933,7 +1067,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     }
   }
 
-  // aggregation into PA / PB (micro/mode_ice4_tendencies.F90:606-717), KRR=6
+  // aggregation into PA / PB (micro/mode_ice4_tendencies.F90:606-717)
   const double lsl = lsfact - lvfact;
   double a;
   a = 0.;
@@ -949,6 +1083,11 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   a = a + ((double)rcwetg + (double)rrwetg) * lsl;
   a = a + ((double)rcdryg + (double)rrdryg) * lsl;
   a = a - (double)S.rgmltr * lsl;
+  if constexpr (HAIL) {
+    a = a + ((double)rrweth + (double)rcweth) * lsl;
+    a = a + ((double)rcdryh + (double)rrdryh) * lsl;
+    a = a - (double)S.rhmltr * lsl;
+  }
   a = a + (double)S.rcberi * lsl;
   if constexpr (OCND2) a = a + rvdepi * lsfact;
   pa[ITH] = a;
@@ -967,6 +1106,10 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   a = a - (double)S.rcmltsr;
   a = a - (double)rcwetg;
   a = a - (double)rcdryg;
+  if constexpr (HAIL) {
+    a = a - (double)rcweth;
+    a = a - (double)rcdryh;
+  }
   a = a - (double)S.rcberi;
   pa[IRC] = a;
   a = 0.;
@@ -980,6 +1123,11 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   a = a - (double)rrwetg;
   a = a - (double)rrdryg;
   a = a + (double)S.rgmltr;
+  if constexpr (HAIL) {
+    a = a - (double)rrweth;
+    a = a - (double)rrdryh;
+    a = a + (double)S.rhmltr;
+  }
   pa[IRR] = a;
   a = 0.;
   a = a + (double)S.rchoni;
@@ -988,6 +1136,10 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   a = a - (double)S.ricfrrg - (double)S.ricfrr;
   a = a - (double)riwetg;
   a = a - (double)ridryg;
+  if constexpr (HAIL) {
+    a = a - (double)riweth;
+    a = a - (double)ridryh;
+  }
   a = a + (double)S.rcberi;
   if constexpr (OCND2) {
     a = a + rvdepi;
@@ -1006,6 +1158,10 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   a = a - (double)S.rsmltg;
   a = a - (double)rswetg;
   a = a - (double)rsdryg;
+  if constexpr (HAIL) {
+    a = a - (double)rsweth;
+    a = a - (double)rsdryh;
+  }
   if constexpr (OCND2) a = a + rilars;
   pa[IRS] = a;
   a = 0.;
@@ -1015,29 +1171,49 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   a = a + (double)S.rsmltg;
   a = a + (double)S.ricfrrg + (double)S.rrcfrig;
   a = a + (double)rcwetg + (double)riwetg + (double)rswetg + (double)rrwetg;
+  if constexpr (HAIL) {
+    a = a - (double)rwetgh;
+    pb[IRG] = pb[IRG] - (double)rwetgh_mr;
+  }
   a = a + (double)rcdryg + (double)ridryg + (double)rsdryg + (double)rrdryg;
   a = a - (double)S.rgmltr;
+  if constexpr (HAIL) {
+    a = a - (double)rgweth;
+    a = a - (double)rgdryh + (double)rdryhg;
+  }
   pa[IRG] = a;
+  if constexpr (HAIL) {
+    a = 0.;
+    a = a + (double)rwetgh;
+    pb[IRH] = pb[IRH] + (double)rwetgh_mr;
+    a = a + (double)rcweth + (double)riweth + (double)rsweth + (double)rgweth + (double)rrweth;
+    a = a + (double)rcdryh + (double)ridryh + (double)rsdryh + (double)rrdryh + (double)rgdryh -
+        (double)rdryhg;
+    a = a - (double)S.rhmltr;
+    pa[IRH] = a;
+  }
 }
 
 // Default sub-step cap per point (the reference's loop has none); a context can set another.
 constexpr int DEFAULT_MAX_CALLS = 4096;
 
-// Fixed inputs of ICE4_STEPPING for one point.
-struct StepPoint {
+// Fixed inputs of ICE4_STEPPING for one point; NV = KRR + 1 variables (7, or 8 with hail).
+template <int NV> struct StepPointN {
   double exn, rhodref, pres, cf, sigma_rc;
-  double extpk[7];  // external tendencies: sources minus t-state / dt
+  double extpk[NV];  // external tendencies: sources minus t-state / dt
 };
+using StepPoint = StepPointN<7>;
 
 // Set-up of ICE4_STEPPING (micro/mode_ice4_stepping.F90:199-240) for one point: the
 // external tendencies from the sources s and the t-state vart.
+template <int NV>
 __device__ __forceinline__ void step_point(double dt, double exn, double rhodref, double pres,
                                            double cf, double sigs, const double *s,
-                                           const double *vart, StepPoint &p) {
+                                           const double *vart, StepPointN<NV> &p) {
   const double zinv = 1. / dt;
   p.exn = exn; p.rhodref = rhodref; p.pres = pres; p.cf = cf; p.sigma_rc = sigs * 2.;
 #pragma unroll
-  for (int jv = 0; jv < 7; ++jv) p.extpk[jv] = s[jv] - vart[jv] * zinv;
+  for (int jv = 0; jv < NV; ++jv) p.extpk[jv] = s[jv] - vart[jv] * zinv;
 }
 
 // One pass of the inner loop of ICE4_STEPPING (:256-516): an ICE4_TENDENCIES call (hard
@@ -1049,30 +1225,32 @@ __device__ __forceinline__ void step_point(double dt, double exn, double rhodref
 // calls: vart then holds the state after the last completed call with the external
 // tendencies integrated so far taken out again (as the complete path does at :520-537 for
 // the whole step), so that the total tendencies count them once, through the sources.
-// pc is read by hard calls only.
-template <class R, bool SOFT, bool OCND2 = false, class TB>
-__device__ __forceinline__ int step_call(double dt, const StepPoint &p, const PointConst<R> &pc,
+// pc is read by hard calls only.  With HAIL, vart, p and S carry hail (8 variables).
+template <class R, bool SOFT, bool OCND2 = false, bool HAIL = false, class TB, class SS, int NV>
+__device__ __forceinline__ int step_call(double dt, const StepPointN<NV> &p, const PointConst<R> &pc,
                                          double *vart, double &cit, double &time_,
-                                         StepState<R> &S, int &calls, int max_calls,
+                                         SS &S, int &calls, int max_calls,
                                          const TB *__restrict__ tab,
                                          const MicroOcnd2 *x = nullptr) {
+  static_assert(NV == (HAIL ? 8 : 7), "variables of the time splitting");
   if (++calls > max_calls) {
 #pragma unroll
-    for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] - p.extpk[jv] * time_;
+    for (int jv = 0; jv < NV; ++jv) vart[jv] = vart[jv] - p.extpk[jv] * time_;
     return -1;
   }
-  double za[7], zb[7];
+  double za[NV], zb[NV];
   const double exn = p.exn;
-  const double sum2 = vart[IRI] + vart[IRS] + vart[IRG];
+  double sum2 = vart[IRI] + vart[IRS] + vart[IRG];
+  if constexpr (HAIL) sum2 = sum2 + vart[IRH];
   const double zdevide =
       (P.XCPD + P.XCPV * vart[IRV] + P.XCL * (vart[IRC] + vart[IRR]) + P.XCI * sum2) * exn;
   const double zzt = vart[ITH] * exn;
   const double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
   const double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
-  tendencies<R, OCND2>(SOFT, exn, p.rhodref, lvfact, lsfact, p.pres, p.cf, p.sigma_rc, cit, zzt,
-                       vart, S, pc, tab, za, zb, x);
+  tendencies<R, OCND2, HAIL>(SOFT, exn, p.rhodref, lvfact, lsfact, p.pres, p.cf, p.sigma_rc, cit,
+                             zzt, vart, S, pc, tab, za, zb, x);
 #pragma unroll
-  for (int jv = 0; jv < 7; ++jv) za[jv] = za[jv] + p.extpk[jv];
+  for (int jv = 0; jv < NV; ++jv) za[jv] = za[jv] + p.extpk[jv];
   const double time = time_;
   double maxtime = dt - time;
   const double zx = P.XTT / exn;
@@ -1082,19 +1260,19 @@ __device__ __forceinline__ int step_call(double dt, const StepPoint &p, const Po
     if (tth > 0.) maxtime = dmin(maxtime, tth);
   }
 #pragma unroll
-  for (int jv = 1; jv < 7; ++jv)
+  for (int jv = 1; jv < NV; ++jv)
     if (za[jv] < -1.E-20 && vart[jv] > P.XRTMIN[jv]) {
       maxtime = dmin(maxtime, -(zb[jv] + vart[jv]) / za[jv]);
       maxtime = dmax(maxtime, P.XMNH_TINY);
     }
   const bool done = time + maxtime >= dt;
 #pragma unroll
-  for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] + za[jv] * maxtime + zb[jv];
+  for (int jv = 0; jv < NV; ++jv) vart[jv] = vart[jv] + za[jv] * maxtime + zb[jv];
   if (vart[IRI] <= 0.) cit = 0.;
   time_ = time + maxtime;
   if (!done) return 1;
 #pragma unroll
-  for (int jv = 0; jv < 7; ++jv) vart[jv] = vart[jv] - p.extpk[jv] * dt;
+  for (int jv = 0; jv < NV; ++jv) vart[jv] = vart[jv] - p.extpk[jv] * dt;
   return 0;
 }
 
@@ -1102,28 +1280,29 @@ __device__ __forceinline__ int step_call(double dt, const StepPoint &p, const Po
 // calls in a row: vart (t-state in, end state out), cit and rrevav are updated, s holds
 // the sources.  Each point's time step is one outer iteration: a hard call, then soft
 // calls until the step is complete.  Returns the number of tendency calls, or -calls if
-// the cap was hit.
-template <class R, bool OCND2 = false, class TB>
+// the cap was hit.  With HAIL, vart and s have 8 entries (hail last).
+template <class R, bool OCND2 = false, bool HAIL = false, class TB>
 __device__ __forceinline__ int stepping_point(double dt, double exn, double rhodref, double pres,
                                               double cf, double sigs, double &cit, double *vart,
                                               const double *s, double &rrevav, int max_calls,
                                               const TB *__restrict__ tab,
                                               const MicroOcnd2 *x = nullptr) {
-  StepPoint p;
+  StepPointN<HAIL ? 8 : 7> p;
   step_point(dt, exn, rhodref, pres, cf, sigs, s, vart, p);
-  StepState<R> S;
+  typename StepStateFor<R, HAIL>::type S;
   double time = 0.;
   int calls = 0;
   const PointConst<R> pc = point_const<R>(rhodref);
-  int st = step_call<R, false, OCND2>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x);
-  while (st > 0) st = step_call<R, true, OCND2>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x);
+  int st = step_call<R, false, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x);
+  while (st > 0)
+    st = step_call<R, true, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x);
   if (st == 0) rrevav = (double)S.rrevav;
   return st < 0 ? -calls : calls;
 }
 
 // Fall speed of one species at one level, OTHER_SPECIES / PRISTINE_ICE
-// (micro/mode_ice4_sedimentation_stat.F90:295-431); jrr = 3 (r), 4 (i), 5 (s), 6 (g).
-// lnrho = ln(rhodref) of the level; the powers are evaluated as exp(c*ln x).
+// (micro/mode_ice4_sedimentation_stat.F90:295-431); jrr = 3 (r), 4 (i), 5 (s), 6 (g),
+// 7 (h, :184-188).  lnrho = ln(rhodref) of the level; the powers are evaluated as exp(c*ln x).
 template <class R>
 __device__ __forceinline__ R fall_speed(int jrr, double rx, R lnrho) {
   if (jrr == 4) {
@@ -1134,15 +1313,15 @@ __device__ __forceinline__ R fall_speed(int jrr, double rx, R lnrho) {
                    rlog<R>(dmax((R)0.05E6, (R)-0.15319E6 - (R)0.021454E6 * (lnrho + lnrx))));
   }
   if (!(rx > P.XRTMIN[jrr])) return (R)0;
-  const double fsed = jrr == 3 ? P.XFSEDR : (jrr == 5 ? P.XFSEDS : P.XFSEDG);
-  const double exsed = jrr == 3 ? P.XEXSEDR : (jrr == 5 ? P.XEXSEDS : P.XEXSEDG);
+  const double fsed = jrr == 3 ? P.XFSEDR : (jrr == 5 ? P.XFSEDS : (jrr == 6 ? P.XFSEDG : P.XFSEDH));
+  const double exsed = jrr == 3 ? P.XEXSEDR : (jrr == 5 ? P.XEXSEDS : (jrr == 6 ? P.XEXSEDG : P.XEXSEDH));
   return (R)fsed * rexp<R>((R)(exsed - 1) * rlog<R>((R)rx) + (R)(exsed - P.XCEXVT - 1) * lnrho);
 }
 
 // One species at one level of ICE4_SEDIMENTATION_STAT (:143-229) for one column:
 // flux holds the flux leaving the level above on entry and this level's on exit
 // (double); rs is the species' source, updated by the flux divergence.
-// jrr = 3 (r), 4 (i), 5 (s), 6 (g); the species are independent, so the
+// jrr = 3 (r), 4 (i), 5 (s), 6 (g), 7 (h); the species are independent, so the
 // reference's species-outer loop order and this level-outer order agree.
 template <class R>
 __device__ __forceinline__ void sediment_species(int jrr, double dt, double rho, R lnrho, double dz,

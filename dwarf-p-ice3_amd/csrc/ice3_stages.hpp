@@ -23,6 +23,7 @@ struct AdjustArgs {
   ice3_ice_adjust_in in;
   ice3_ice_adjust_out out;  // t, rv_out, rc_out, ri_out, cph, lv, ls, srcs may be NULL
   double *ris_copy;         // optional second copy of the adjusted ris
+  const double *rh;         // hail mixing ratio at t (KRR=7 only)
 };
 
 // The OCND2 branch's extra inputs and outputs (include/ice3.h: ice3_ocnd2_in/_out).
@@ -35,7 +36,8 @@ struct Ocnd2Args {
 // ICE_ADJUST at point j (level k) of column i (micro/ice_adjust.F90:257-472); returns the
 // adjusted ris.  With OCND2 the branch's extra fields come from x: the layer thickness is
 // ZDZ(k) = PZZ(kp) - PZZ(kp-1), kp = min(k+1, nlev-1) (condensation.F90:366,376-378).
-template <bool OCND2 = false>
+// HAIL (KRR=7): a.rh enters the heat capacity.
+template <bool OCND2 = false, bool HAIL = false>
 __device__ __forceinline__ double adjust_at(const AdjustArgs &a, int i, size_t j, int k = 0,
                                             const Ocnd2Args *x = nullptr) {
   const ice3_ice_adjust_in &in = a.in;
@@ -51,10 +53,10 @@ __device__ __forceinline__ double adjust_at(const AdjustArgs &a, int i, size_t j
     xp.cld_wgt = x->in.ice_cld_wgt[i];
     xp.bottom = k == 0;
   }
-  ice_adjust_point<OCND2>(a.dt, in.sigqsat[i], in.pabs[j], in.sigs[j], in.th[j], in.exn[j],
-                          in.exn_ref[j], in.rho_dry_ref[j], in.rv[j], in.rc[j], in.ri[j], in.rr[j],
-                          in.rs[j], in.rg[j], in.cf_mf[j], in.rc_mf[j], in.ri_mf[j], rvs, rcs, ris,
-                          ths, o, &xp);
+  ice_adjust_point<OCND2, HAIL>(a.dt, in.sigqsat[i], in.pabs[j], in.sigs[j], in.th[j], in.exn[j],
+                                in.exn_ref[j], in.rho_dry_ref[j], in.rv[j], in.rc[j], in.ri[j],
+                                in.rr[j], in.rs[j], in.rg[j], in.cf_mf[j], in.rc_mf[j], in.ri_mf[j],
+                                rvs, rcs, ris, ths, o, &xp, HAIL ? a.rh[j] : 0.);
   if constexpr (OCND2) {
     x->out.icldfr[j] = xp.icldfr;
     x->out.wcldfr[j] = xp.wcldfr;
@@ -94,17 +96,20 @@ struct SedimArgs {
   double *dst_rr, *dst_ri, *dst_rs, *dst_rg;
   double *inprc, *inprr, *inpri, *inprs, *inprg;
   double *zzz;  // OCND2 branch: receives the model level height ZZZZ (micro/rain_ice.F90:387-395)
+  const double *src_rh;  // hail (KRR=7): source in, source out, surface rate
+  double *dst_rh, *inprh;
 };
 
 // ICE4_SEDIMENTATION_STAT (micro/mode_ice4_sedimentation_stat.F90:143-229) for column i,
 // followed by the surface precipitation of RAIN_ICE (PINPRS += PINPRI, PINPRC = 0 with
 // LSEDIC=F).  With ZZZ the walk also writes RAIN_ICE's model level height ZZZZ into a.zzz:
 // PDZZ accumulated from the top level down, minus half the level's own (rain_ice.F90:387-395,
-// JK = IKTE..IKTB with level 1 the lowest).
-template <class R, bool ZZZ = false>
+// JK = IKTE..IKTB with level 1 the lowest).  HAIL: hail sediments too (species 7,
+// micro/mode_ice4_sedimentation_stat.F90:184-188,208-210,220-222).
+template <class R, bool ZZZ = false, bool HAIL = false>
 __device__ __forceinline__ void sedim_column(const SedimArgs &a, int i) {
   const double dt = a.dt;
-  double fr = 0., fi = 0., fs = 0., fg = 0., zzsum = 0.;
+  double fr = 0., fi = 0., fs = 0., fg = 0., fh = 0., zzsum = 0.;
   for (int k = a.nlev - 1; k >= 0; --k) {
     const size_t j = (size_t)k * a.ld + i;
     const double rho = a.rhodref[j], dz = a.dzz[j];
@@ -119,26 +124,32 @@ __device__ __forceinline__ void sedim_column(const SedimArgs &a, int i) {
     sediment_species<R>(IRS, dt, rho, lnrho, dz, rss, fs);
     sediment_species<R>(IRG, dt, rho, lnrho, dz, rgs, fg);
     a.dst_rr[j] = rrs; a.dst_ri[j] = ris; a.dst_rs[j] = rss; a.dst_rg[j] = rgs;
+    if constexpr (HAIL) {
+      double rhs = a.src_rh[j];
+      sediment_species<R>(IRH, dt, rho, lnrho, dz, rhs, fh);
+      a.dst_rh[j] = rhs;
+    }
   }
   a.inprc[i] = 0.;
   a.inprr[i] = fr / P.XRHOLW;
   a.inpri[i] = fi / P.XRHOLW;
   a.inprs[i] = fs / P.XRHOLW + fi / P.XRHOLW;
   a.inprg[i] = fg / P.XRHOLW;
+  if constexpr (HAIL) a.inprh[i] = fh / P.XRHOLW;
 }
 
 // ICE_ADJUST and the sedimentation of column i in one top-down walk (the FP64 fused step):
 // at each level the point is adjusted, then its precipitating species sediment, ri from the
 // adjusted source held in a register.  The sedimentation of a level needs only that level
 // and the flux from above, so the results are those of adjust_at over all points followed
-// by sedim_column.
-template <class R>
+// by sedim_column.  HAIL: ICE_ADJUST with the hail heat capacity, and hail sediments.
+template <class R, bool HAIL = false>
 __device__ __forceinline__ void adjust_sedim_column(const AdjustArgs &aa, const SedimArgs &a, int i) {
   const double dt = a.dt;
-  double fr = 0., fi = 0., fs = 0., fg = 0.;
+  double fr = 0., fi = 0., fs = 0., fg = 0., fh = 0.;
   for (int k = a.nlev - 1; k >= 0; --k) {
     const size_t j = (size_t)k * a.ld + i;
-    double ris = adjust_at(aa, i, j);
+    double ris = adjust_at<false, HAIL>(aa, i, j);
     const double rho = a.rhodref[j], dz = a.dzz[j];
     const R lnrho = rlog<R>((R)rho);
     double rrs = a.src_rr[j], rss = a.src_rs[j], rgs = a.src_rg[j];
@@ -147,27 +158,34 @@ __device__ __forceinline__ void adjust_sedim_column(const AdjustArgs &aa, const 
     sediment_species<R>(IRS, dt, rho, lnrho, dz, rss, fs);
     sediment_species<R>(IRG, dt, rho, lnrho, dz, rgs, fg);
     a.dst_rr[j] = rrs; a.dst_ri[j] = ris; a.dst_rs[j] = rss; a.dst_rg[j] = rgs;
+    if constexpr (HAIL) {
+      double rhs = a.src_rh[j];
+      sediment_species<R>(IRH, dt, rho, lnrho, dz, rhs, fh);
+      a.dst_rh[j] = rhs;
+    }
   }
   a.inprc[i] = 0.;
   a.inprr[i] = fr / P.XRHOLW;
   a.inpri[i] = fi / P.XRHOLW;
   a.inprs[i] = fs / P.XRHOLW + fi / P.XRHOLW;
   a.inprg[i] = fg / P.XRHOLW;
+  if constexpr (HAIL) a.inprh[i] = fh / P.XRHOLW;
 }
 
 // Microphysics stage arguments.  The t-state of species v is t[v][j] * tscale
 // (tscale = 1 when the caller hands the t-state over, dt when it is the adjusted
 // sources); src[v] are the sources after sedimentation, dst[v] the results (may
-// alias src).  Species order: th, rv, rc, rr, ri, rs, rg.
+// alias src).  Species order: th, rv, rc, rr, ri, rs, rg, and hail (entry 7, read by the
+// KRR=7 kernels only).
 struct MicroArgs {
   int ncol, nlev, ld;
   double dt, tscale;
   const double *tab;
   const double *exn, *rhodref, *exn_ref, *pabs, *cldfr, *sigs;
-  const double *t[7];
-  const double *src[7];
+  const double *t[8];
+  const double *src[8];
   const double *src_ci;
-  double *dst[7];
+  double *dst[8];
   double *ci, *evap3d, *rainfr;
   int *calls;
   unsigned long long *counters;  // [0] tendency calls, [1] capped points, [2] micro points
@@ -181,16 +199,22 @@ struct MicroArgs {
 
 // ICE4_CORRECT_NEGATIVITIES (micro/mode_ice4_correct_negativities.F90:51-171) on one
 // point's sources: negative condensate goes back to vapour with its latent heat, then
-// vapour below XRTMIN(1) is restored from cloud + ice, rain, snow and graupel.
+// vapour below XRTMIN(1) is restored from cloud + ice, rain, snow and graupel; with HAIL,
+// hail takes part in both steps after graupel (:115-120,165-170).
+template <bool HAIL = false>
 __device__ __forceinline__ void correct_negativities(double &ths, double &rvs, double &rcs,
                                                      double &rrs, double &ris, double &rss,
-                                                     double &rgs, double lvfact, double lsfact) {
+                                                     double &rgs, double lvfact, double lsfact,
+                                                     double *rhs = nullptr) {
   double zw;
   zw = rcs - dmax(rcs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rcs = rcs - zw;
   zw = rrs - dmax(rrs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rrs = rrs - zw;
   zw = ris - dmax(ris, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; ris = ris - zw;
   zw = rss - dmax(rss, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rss = rss - zw;
   zw = rgs - dmax(rgs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rgs = rgs - zw;
+  if constexpr (HAIL) {
+    zw = *rhs - dmax(*rhs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; *rhs = *rhs - zw;
+  }
   zw = dmin(1., dmax(P.XRTMIN[1] - rvs, 0.) / dmax(rcs + ris, 1.E-20));
   ths = ths - zw * (rcs * lvfact + ris * lsfact);
   rvs = rvs + zw * (rcs + ris);
@@ -202,21 +226,29 @@ __device__ __forceinline__ void correct_negativities(double &ths, double &rvs, d
   rvs = rvs + zw; rss = rss - zw; ths = ths - zw * lsfact;
   zw = dmin(dmax(rgs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
   rvs = rvs + zw; rgs = rgs - zw; ths = ths - zw * lsfact;
+  if constexpr (HAIL) {
+    zw = dmin(dmax(*rhs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
+    rvs = rvs + zw; *rhs = *rhs - zw; ths = ths - zw * lsfact;
+  }
 }
 
-// t-state (times tscale) and sources of the seven variables at point j.
+// t-state (times tscale) and sources of the first NV variables at point j (7, or 8 with hail).
+template <int NV = 7>
 __device__ __forceinline__ void micro_load(const MicroArgs &a, size_t j, double *t, double *s) {
   const double ts = a.tscale;
 #pragma unroll
-  for (int v = 0; v < 7; ++v) t[v] = a.t[v][j] * ts;
+  for (int v = 0; v < NV; ++v) t[v] = a.t[v][j] * ts;
 #pragma unroll
-  for (int v = 0; v < 7; ++v) s[v] = a.src[v][j];
+  for (int v = 0; v < NV; ++v) s[v] = a.src[v][j];
 }
 
-// RAIN_ICE's latent heat factors L / cph (micro/rain_ice.F90:334-347) from the t-state t.
+// RAIN_ICE's latent heat factors L / cph (micro/rain_ice.F90:334-347) from the t-state t;
+// with HAIL the ice mixing ratio includes hail (:337-338).
+template <bool HAIL = false>
 __device__ __forceinline__ void rain_ice_factors(const double *t, double exn, double &lsfact,
                                                  double &lvfact) {
-  const double zrice = t[IRI] + t[IRS] + t[IRG];
+  double zrice = t[IRI] + t[IRS] + t[IRG];
+  if constexpr (HAIL) zrice = zrice + t[IRH];
   const double zdevide = P.XCPD + P.XCPV * t[IRV] + P.XCL * (t[IRC] + t[IRR]) + P.XCI * zrice;
   const double zt = t[ITH] * exn;
   lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide;
@@ -225,12 +257,15 @@ __device__ __forceinline__ void rain_ice_factors(const double *t, double exn, do
 
 // RAIN_ICE's microphysics mask (micro/rain_ice.F90:365-381): a hydrometeor above XRTMIN;
 // with OCND2 (:348-364), ice supersaturation in the ice cloud above XFRMIN(12) or a
-// hydrometeor above XFRMIN(13).
-template <bool OCND2 = false>
+// hydrometeor above XFRMIN(13).  HAIL: hail above XRTMIN(7) too (:366-372).
+template <bool OCND2 = false, bool HAIL = false>
 __device__ __forceinline__ bool micro_mask(const double *t, const MicroArgs &a, size_t j) {
   if constexpr (OCND2)
     return a.ssio[j] > P.XFRMIN[12] || t[IRC] > P.XFRMIN[13] || t[IRR] > P.XFRMIN[13] ||
            t[IRI] > P.XFRMIN[13] || t[IRS] > P.XFRMIN[13] || t[IRG] > P.XFRMIN[13];
+  if constexpr (HAIL)
+    return t[IRC] > P.XRTMIN[2] || t[IRR] > P.XRTMIN[3] || t[IRI] > P.XRTMIN[4] ||
+           t[IRS] > P.XRTMIN[5] || t[IRG] > P.XRTMIN[6] || t[IRH] > P.XRTMIN[7];
   return t[IRC] > P.XRTMIN[2] || t[IRR] > P.XRTMIN[3] || t[IRI] > P.XRTMIN[4] ||
          t[IRS] > P.XRTMIN[5] || t[IRG] > P.XRTMIN[6];
 }
@@ -242,12 +277,12 @@ __device__ __forceinline__ MicroOcnd2 ocnd2_point(const MicroArgs &a, size_t j) 
 
 // Nucleation outside the microphysics mask (micro/rain_ice.F90:505-540) at point j with
 // t-state t: cit is reset and updated, rvheni receives the nucleated vapour.
-template <class R, bool OCND2 = false>
+template <class R, bool OCND2 = false, bool HAIL = false>
 __device__ __forceinline__ void micro_nucleation(const MicroArgs &a, size_t j, const double *t,
                                                  double &cit, double &rvheni) {
   const double exn = a.exn[j];
   double lsfact, lvfact;
-  rain_ice_factors(t, exn, lsfact, lvfact);
+  rain_ice_factors<HAIL>(t, exn, lsfact, lvfact);
   cit = 0.;
   MicroOcnd2 x;
   if constexpr (OCND2) x = ocnd2_point(a, j);
@@ -258,14 +293,17 @@ __device__ __forceinline__ void micro_nucleation(const MicroArgs &a, size_t j, c
 // RAIN_ICE section 8 at point j (micro/rain_ice.F90:612-652 and the reinstated 754-823):
 // total tendencies from the end state wr of the time splitting, ICE4_CORRECT_NEGATIVITIES,
 // and the stores.  The t-state and sources are read again from memory (they are not
-// written before this point's stores).  rvheni: vapour nucleated outside the mask.
+// written before this point's stores).  rvheni: vapour nucleated outside the mask.  HAIL:
+// wr has hail last, and the hail source takes its tendency and negativity correction.
+template <bool HAIL = false>
 __device__ __forceinline__ void micro_finish(const MicroArgs &a, size_t j, const double *wr,
                                              double cit, double evap3d, double rvheni, int calls) {
+  constexpr int NV = HAIL ? 8 : 7;
   const double dt = a.dt, zinv = 1. / dt;
-  double t[7], s[7];
-  micro_load(a, j, t, s);
+  double t[NV], s[NV];
+  micro_load<NV>(a, j, t, s);
   double lsfact, lvfact;
-  rain_ice_factors(t, a.exn[j], lsfact, lvfact);
+  rain_ice_factors<HAIL>(t, a.exn[j], lsfact, lvfact);
   rvheni = dmin(s[IRV], rvheni / dt);
   // 8.1 total tendencies
   const double exnref = a.exn_ref[j];
@@ -274,7 +312,8 @@ __device__ __forceinline__ void micro_finish(const MicroArgs &a, size_t j, const
   const double wrv = (wr[IRV] - t[IRV]) * zinv, wrc = (wr[IRC] - t[IRC]) * zinv;
   const double wrr = (wr[IRR] - t[IRR]) * zinv, wri = (wr[IRI] - t[IRI]) * zinv;
   const double wrs = (wr[IRS] - t[IRS]) * zinv, wrg = (wr[IRG] - t[IRG]) * zinv;
-  const double wth = (wrc + wrr) * lvfact + (wri + wrs + wrg + 0.) * lsfact;
+  const double wrh = HAIL ? (wr[NV - 1] - t[NV - 1]) * zinv : 0.;
+  const double wth = (wrc + wrr) * lvfact + (wri + wrs + wrg + wrh) * lsfact;
   double ths = s[ITH] + wth + rvheni * lsfact;
   double rvs = s[IRV] + wrv - rvheni;
   double rcs = s[IRC] + wrc;
@@ -282,10 +321,12 @@ __device__ __forceinline__ void micro_finish(const MicroArgs &a, size_t j, const
   double ris = s[IRI] + wri + rvheni;
   double rss = s[IRS] + wrs;
   double rgs = s[IRG] + wrg;
+  double rhs = HAIL ? s[NV - 1] + wrh : 0.;
   // 8.2 ICE4_CORRECT_NEGATIVITIES
-  correct_negativities(ths, rvs, rcs, rrs, ris, rss, rgs, lvfact, lsfact);
+  correct_negativities<HAIL>(ths, rvs, rcs, rrs, ris, rss, rgs, lvfact, lsfact, &rhs);
   a.dst[ITH][j] = ths; a.dst[IRV][j] = rvs; a.dst[IRC][j] = rcs; a.dst[IRR][j] = rrs;
   a.dst[IRI][j] = ris; a.dst[IRS][j] = rss; a.dst[IRG][j] = rgs;
+  if constexpr (HAIL) a.dst[NV - 1][j] = rhs;
   a.ci[j] = cit;
   a.evap3d[j] = evap3d;
   a.rainfr[j] = 1.;
@@ -294,22 +335,23 @@ __device__ __forceinline__ void micro_finish(const MicroArgs &a, size_t j, const
 
 // RAIN_ICE sections 1, 4, 5, 8 (micro/rain_ice.F90:334-384, 505-540, 612-652 and the
 // reinstated 754-823) for point j, all in one thread.  Returns the number of tendency
-// calls, negative if the sub-step cap was hit.
-template <class R, bool OCND2 = false, class TB>
+// calls, negative if the sub-step cap was hit.  HAIL: KRR=7, hail the eighth variable.
+template <class R, bool OCND2 = false, bool HAIL = false, class TB>
 __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j, const TB *__restrict__ tab) {
-  double wr[7], s[7];
-  micro_load(a, j, wr, s);
+  constexpr int NV = HAIL ? 8 : 7;
+  double wr[NV], s[NV];
+  micro_load<NV>(a, j, wr, s);
   double cit = a.src_ci[j], rvheni = 0., evap3d = 0.;
   int calls = 0;
-  if (micro_mask<OCND2>(wr, a, j)) {
+  if (micro_mask<OCND2, HAIL>(wr, a, j)) {
     MicroOcnd2 x;
     if constexpr (OCND2) x = ocnd2_point(a, j);
-    calls = stepping_point<R, OCND2>(a.dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j],
-                                     cit, wr, s, evap3d, a.max_calls, tab, &x);
+    calls = stepping_point<R, OCND2, HAIL>(a.dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j],
+                                           a.sigs[j], cit, wr, s, evap3d, a.max_calls, tab, &x);
   } else {
-    micro_nucleation<R, OCND2>(a, j, wr, cit, rvheni);
+    micro_nucleation<R, OCND2, HAIL>(a, j, wr, cit, rvheni);
   }
-  micro_finish(a, j, wr, cit, evap3d, rvheni, calls);
+  micro_finish<HAIL>(a, j, wr, cit, evap3d, rvheni, calls);
   return calls;
 }
 

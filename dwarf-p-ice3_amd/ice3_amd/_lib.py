@@ -51,9 +51,13 @@ RainIn = _struct("ice3_rain_ice_in", RAIN_IN_FIELDS)
 RainIO = _struct("ice3_rain_ice_inout", RAIN_IO_FIELDS)
 RainOut = _struct("ice3_rain_ice_out", RAIN_OUT_FIELDS)
 RainOcnd2In = _struct("ice3_rain_ocnd2_in", RAIN_OCND2_IN_FIELDS)
+HAIL_FIELDS = ("rht", "rhs", "inprh")
+PIPE_HAIL_FIELDS = ("rh", "rhs", "rhs_out", "inprh")
 PipeIn = _struct("ice3_pipeline_in", PIPE_IN_FIELDS)
 PipeOut = _struct("ice3_pipeline_out", PIPE_OUT_FIELDS)
 Stats = _struct("ice3_stats", ("calls", "counters"))
+Hail = _struct("ice3_hail", HAIL_FIELDS)
+PipeHail = _struct("ice3_pipeline_hail", PIPE_HAIL_FIELDS)
 
 STAGE_ADJUST, STAGE_SEDIMENTATION, STAGE_MICROPHYSICS = 1, 2, 4
 NFEAT, NTARG = 18, 8
@@ -77,6 +81,7 @@ EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_co
            "ice3_ctx_set_substep_cap", "ice3_ctx_status",
            "ice3_ice_adjust", "ice3_ice_adjust_ocnd2", "ice3_rain_ice", "ice3_adjust_rain_ice",
            "ice3_rain_ice_ocnd2", "ice3_adjust_rain_ice_ocnd2",
+           "ice3_ice_adjust_hail", "ice3_rain_ice_hail", "ice3_adjust_rain_ice_hail",
            "ice3_stage_timing",
            "ice3_stage_times", "ice3_adjust_rain_ice_stages", "ice3_adjust_rain_ice_surrogate",
            "ice3_adjust_rain_ice_surrogate_noalloc",
@@ -122,6 +127,12 @@ def load():
         lib.ice3_adjust_rain_ice_ocnd2.argtypes = c5 + [ctypes.POINTER(PipeIn), ctypes.POINTER(Ocnd2In),
                                                         ctypes.POINTER(PipeOut), ctypes.POINTER(Ocnd2Out),
                                                         ctypes.POINTER(Stats), _P]
+        lib.ice3_ice_adjust_hail.argtypes = c5 + [ctypes.POINTER(AdjustIn), _P, ctypes.POINTER(AdjustOut), _P]
+        lib.ice3_rain_ice_hail.argtypes = c5 + [ctypes.POINTER(RainIn), ctypes.POINTER(RainIO),
+                                                ctypes.POINTER(RainOut), ctypes.POINTER(Hail),
+                                                ctypes.POINTER(Stats), _P]
+        lib.ice3_adjust_rain_ice_hail.argtypes = c5 + [ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut),
+                                                       ctypes.POINTER(PipeHail), ctypes.POINTER(Stats), _P]
         lib.ice3_adjust_rain_ice_stages.argtypes = common + [
             ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut), ctypes.POINTER(Stats), ctypes.c_int, _P]
         lib.ice3_adjust_rain_ice_surrogate.argtypes = [

@@ -188,3 +188,38 @@ class IceAdjustOCND2(IceAdjust):
         if srcs is not None:
             srcs = from_columns(srcs, layout, shape)
         return (*res, srcs) if return_srcs else tuple(res)
+
+
+class IceAdjustHail(IceAdjust):
+    """ICE_ADJUST with KRR = 7 (hail, HCLOUD='ICE4') on MI355X (``ice3_ice_adjust_hail``, FP64):
+    the hail mixing ratio enters the heat capacity (micro/ice_adjust.F90:503-507).  The
+    reference reaches it through the Fortran ICE_ADJUST with KRR=7 and PRH.  The call takes
+    IceAdjust's arguments plus ``rh`` (hail mixing ratio at t) and returns IceAdjust's 16-tuple.
+    """
+
+    def __call__(self, sigqsat, pabs, sigs, th, exn, exn_ref, rho_dry_ref, rv, rc, ri, rr, rs, rg,
+                 cf_mf, rc_mf, ri_mf, rvs, rcs, ris, ths, timestep: float, rh=None,
+                 return_srcs: bool = False, check_inputs: bool = False):
+        if rh is None:
+            raise ValueError("KRR=7 needs the hail mixing ratio rh")
+        args = dict(pabs=pabs, sigs=sigs, th=th, exn=exn, exn_ref=exn_ref, rho_dry_ref=rho_dry_ref,
+                    rv=rv, rc=rc, ri=ri, rr=rr, rs=rs, rg=rg, cf_mf=cf_mf, rc_mf=rc_mf, ri_mf=ri_mf,
+                    rvs=rvs, rcs=rcs, ris=ris, ths=ths, rh=rh)
+        shape = tuple(th.shape) if hasattr(th, "shape") else tuple(torch.as_tensor(th).shape)
+        layout = resolve_layout(self.layout, th)
+        f, sq, nlev, ncol = self._inputs(sigqsat, args, layout)
+        if check_inputs:
+            _lib.require_finite({**f, "sigqsat": sq}, self.device)
+        out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) for k in _OUT_ORDER}
+        srcs = torch.empty((nlev, ncol), dtype=torch.float64, device=self.device) if return_srcs else None
+        cin = _lib.AdjustIn(*[ptr(f[k]) for k in _lib.ADJ_IN_FIELDS[:-1]], ptr(sq))
+        cout = _lib.AdjustOut(*[ptr(out[k]) for k in _lib.ADJ_OUT_FIELDS[:-1]], ptr(srcs))
+        with torch.cuda.device(self.device):
+            rc_ = self._lib.ice3_ice_adjust_hail(self._ctx, ncol, nlev, ncol, float(timestep),
+                                                 ctypes.byref(cin), ptr(f["rh"]), ctypes.byref(cout),
+                                                 stream_ptr(self.device))
+        _lib.check(rc_, "ice3_ice_adjust_hail")
+        res = [from_columns(out[k], layout, shape) for k in _OUT_ORDER]
+        if srcs is not None:
+            srcs = from_columns(srcs, layout, shape)
+        return (*res, srcs) if return_srcs else tuple(res)

@@ -84,6 +84,9 @@ class RainIce(_Base):
     def _extra_inputs(self, state, layout, nlev, ncol):
         return {}
 
+    def _extra_results(self, extra, new, diags, layout, shape):
+        pass
+
     def _launch(self, precision, ncol, nlev, dt, cin, extra, cio, cout, st):
         return self._lib.ice3_rain_ice(self._ctx, _PREC[precision], ncol, nlev, ncol, float(dt),
                                        ctypes.byref(cin), ctypes.byref(cio), ctypes.byref(cout),
@@ -137,6 +140,7 @@ class RainIce(_Base):
         new["evap3d"] = from_columns(out["evap3d"], layout, shape)
         new["rainfr"] = from_columns(out["rainfr"], layout, shape)
         diags = {k: from_columns(out[k], layout, shape) for k in ("inprc", "inprr", "inpri", "inprs", "inprg")}
+        self._extra_results(extra, new, diags, layout, shape)
         if return_calls:
             diags["calls"] = from_columns(calls.view(nlev, ncol), layout, shape)
         return new, diags
@@ -154,7 +158,7 @@ class AdjustRainIce(_Base):
                  check_inputs: bool = False):
         return self._fused(f, dt, precision, return_calls, check_inputs)
 
-    def _fused(self, f, dt, precision, return_calls, check_inputs, ocnd2=False):
+    def _fused(self, f, dt, precision, return_calls, check_inputs, ocnd2=False, hail=False):
         shape = self._shape(f["th"])
         layout = resolve_layout(self.layout, f["th"])
         g = {k: self._col(f[k], k, layout) for k in _lib.PIPE_IN_FIELDS if k != "sigqsat"}
@@ -166,6 +170,9 @@ class AdjustRainIce(_Base):
             wgt = self._per_column(f["ice_cld_wgt"], "ice_cld_wgt", ncol, layout)
             if nlev < 2:
                 raise ValueError("the OCND2 branch needs at least two levels")
+        if hail:
+            extra["rh"] = self._col(f["rh"], "rh", layout)
+            extra["rhs"] = self._col(f["rhs"], "rhs", layout)
         for k, v in list(g.items()) + list(extra.items()):
             if tuple(v.shape) != (nlev, ncol):
                 raise ValueError(f"{k}: shape {tuple(v.shape)} != {(nlev, ncol)}")
@@ -190,11 +197,19 @@ class AdjustRainIce(_Base):
                                                           ctypes.byref(cout), ctypes.byref(cout2),
                                                           ctypes.byref(st), stream_ptr(self.device))
                 out.update(x)
+            elif hail:
+                out["rhs"] = torch.empty((nlev, ncol), dtype=torch.float64, device=self.device)
+                out["inprh"] = torch.empty(ncol, dtype=torch.float64, device=self.device)
+                h = _lib.PipeHail(ptr(extra["rh"]), ptr(extra["rhs"]), ptr(out["rhs"]), ptr(out["inprh"]))
+                rc = self._lib.ice3_adjust_rain_ice_hail(self._ctx, ncol, nlev, ncol, float(dt),
+                                                         ctypes.byref(cin), ctypes.byref(cout), ctypes.byref(h),
+                                                         ctypes.byref(st), stream_ptr(self.device))
             else:
                 rc = self._lib.ice3_adjust_rain_ice(self._ctx, _PREC[precision], ncol, nlev, ncol, float(dt),
                                                     ctypes.byref(cin), ctypes.byref(cout), ctypes.byref(st),
                                                     stream_ptr(self.device))
-        _lib.check(rc, "ice3_adjust_rain_ice_ocnd2" if ocnd2 else "ice3_adjust_rain_ice")
+        _lib.check(rc, "ice3_adjust_rain_ice_ocnd2" if ocnd2 else
+                   ("ice3_adjust_rain_ice_hail" if hail else "ice3_adjust_rain_ice"))
         self._finish_counters(counters)
         res = {k: from_columns(v, layout, shape) for k, v in out.items()}
         if return_calls:
@@ -256,3 +271,47 @@ class AdjustRainIceOCND2(AdjustRainIce):
 
     def __call__(self, f: dict, dt: float, return_calls: bool = False, check_inputs: bool = False):
         return self._fused(f, dt, "fp64", return_calls, check_inputs, ocnd2=True)
+
+
+class RainIceHail(RainIce):
+    """RAIN_ICE with KRR = 7 (hail, HCLOUD='ICE4') on MI355X (``ice3_rain_ice_hail``, FP64).
+
+    The reference reaches hail through the Fortran RAIN_ICE with KRR=7 and PRHT / PRHS /
+    PINPRH (micro/rain_ice.F90:18,247-249); its JAX component has no hail.  The state takes
+    RainIce's keys plus ``rh_t`` (hail mixing ratio at t) and ``rhs`` (its source); the
+    returned state has the updated ``rhs`` and the diagnostics add ``inprh`` (surface hail).
+    """
+
+    _ENTRY = "ice3_rain_ice_hail"
+
+    def _extra_inputs(self, state, layout, nlev, ncol):
+        x = {"rht": self._col(state["rh_t"], "rh_t", layout),
+             "rhs": self._col(state["rhs"], "rhs", layout).clone()}
+        for k in ("rht", "rhs"):
+            if tuple(x[k].shape) != (nlev, ncol):
+                raise ValueError(f"{k}: shape {tuple(x[k].shape)} != {(nlev, ncol)}")
+        return x
+
+    def _launch(self, precision, ncol, nlev, dt, cin, extra, cio, cout, st):
+        if precision != "fp64":
+            raise ValueError("the hail path is implemented in FP64 only")
+        extra["inprh"] = torch.empty(ncol, dtype=torch.float64, device=self.device)
+        h = _lib.Hail(ptr(extra["rht"]), ptr(extra["rhs"]), ptr(extra["inprh"]))
+        return self._lib.ice3_rain_ice_hail(self._ctx, ncol, nlev, ncol, float(dt), ctypes.byref(cin),
+                                            ctypes.byref(cio), ctypes.byref(cout), ctypes.byref(h),
+                                            ctypes.byref(st), stream_ptr(self.device))
+
+    def _extra_results(self, extra, new, diags, layout, shape):
+        new["rhs"] = from_columns(extra["rhs"], layout, shape)
+        diags["inprh"] = from_columns(extra["inprh"], layout, shape)
+
+
+class AdjustRainIceHail(AdjustRainIce):
+    """The fused step with hail (``ice3_adjust_rain_ice_hail``, FP64): IceAdjustHail's
+    ICE_ADJUST, then RainIceHail's sedimentation and microphysics on the adjusted state.  Inputs
+    are AdjustRainIce's plus ``rh`` (hail mixing ratio at t) and ``rhs`` (hail source; times dt,
+    the hail t-state of the microphysics, as for the other species); the result adds ``rhs``
+    (final hail source) and ``inprh``."""
+
+    def __call__(self, f: dict, dt: float, return_calls: bool = False, check_inputs: bool = False):
+        return self._fused(f, dt, "fp64", return_calls, check_inputs, hail=True)

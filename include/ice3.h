@@ -19,6 +19,8 @@
  *   ice3_rain_ice        c_rain_ice     (bridge/phyex_bridge.F90:242-254) and
  *                        RainIceJAX.__call__ (src/ice3/jax/rain_ice.py:181-420)
  *   ice3_adjust_rain_ice the two calls in sequence, fused on the device
+ *   ..._ocnd2, ..._hail  the same with PARAM_ICE LOCND2 = .TRUE., or with hail (KRR=7):
+ *                        the Fortran ICE_ADJUST / RAIN_ICE with those options
  *   ice3_ctx_create      c_ini_rain_ice (bridge/phyex_bridge.F90:215-239): INI_CST +
  *                        INI_RAIN_ICE constants and lookup tables, uploaded once
  */
@@ -170,6 +172,35 @@ int ice3_rain_ice_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
                         const ice3_rain_ice_inout *io, const ice3_rain_ice_out *out,
                         const ice3_stats *stats, void *stream);
 
+/* ---- hail: the configuration with HCLOUD='ICE4' (KRR=7), FP64.  The reference reaches it
+ * through the Fortran ICE_ADJUST and RAIN_ICE with KRR=7 and their optional hail arguments
+ * (micro/ice_adjust.F90:18,193; micro/rain_ice.F90:18,247-249): the hail mixing ratio in the
+ * heat capacities and the microphysics mask (micro/ice_adjust.F90:503-507,
+ * micro/rain_ice.F90:337-372), hail sedimentation (micro/mode_ice4_sedimentation_stat.F90:
+ * 184-188,208-222), ICE4_FAST_RH with the SWETH/GWETH/RWETH wet-growth kernels
+ * (micro/mode_ice4_fast_rh.F90, micro/mode_read_xker_{sweth,gweth,rweth}.F90), the conversion
+ * of wet-growing graupel into hail (micro/mode_ice4_fast_rg.F90:330-351), the hail terms of
+ * the tendencies (micro/mode_ice4_tendencies.F90:619-716), of the total tendencies
+ * (micro/rain_ice.F90:771-791) and of the negativity correction
+ * (micro/mode_ice4_correct_negativities.F90:115-120,165-170).  The bridge's c_ice_adjust and
+ * c_rain_ice (bridge/phyex_bridge.F90:27-34,242-254) pass no hail array. */
+typedef struct {
+  const double *rht;    /* hail mixing ratio at t, (nlev, ld) */
+  double *rhs;          /* hail source, read and updated in place */
+  double *inprh;        /* per column: surface hail precipitation rate (m/s) */
+} ice3_hail;
+
+/* ICE_ADJUST with KRR=7: rh (hail mixing ratio, (nlev, ld)) enters the heat capacity. */
+int ice3_ice_adjust_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                         const ice3_ice_adjust_in *in, const double *rh,
+                         const ice3_ice_adjust_out *out, void *stream);
+
+/* RAIN_ICE with KRR=7: as ice3_rain_ice, plus the hail arrays. */
+int ice3_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                       const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
+                       const ice3_rain_ice_out *out, const ice3_hail *hail,
+                       const ice3_stats *stats, void *stream);
+
 /* ---- fused ice_adjust + rain_ice (the "adjust then microphysics" time step) ----
  * ice_adjust runs on the state at t; its updated sources, times dt, are the
  * t-state of rain_ice (the synthetic and AROME-like inputs carry the state at t
@@ -191,6 +222,21 @@ typedef struct {
 int ice3_adjust_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld, double dt,
                          const ice3_pipeline_in *in, const ice3_pipeline_out *out,
                          const ice3_stats *stats, void *stream);
+
+/* The fused step with hail (KRR=7): rh is the hail mixing ratio at t (ICE_ADJUST's heat
+ * capacity), rhs the hail source at entry (times dt, the hail t-state of rain_ice, as for the
+ * other species), rhs_out the final hail source (must not alias rhs) and inprh the surface hail
+ * rate.  FP64. */
+typedef struct {
+  const double *rh, *rhs;
+  double *rhs_out;
+  double *inprh;        /* per column */
+} ice3_pipeline_hail;
+
+int ice3_adjust_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                              const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                              const ice3_pipeline_hail *hail, const ice3_stats *stats,
+                              void *stream);
 
 /* The fused step with OCND2 in both halves: ice3_ice_adjust_ocnd2's ICE_ADJUST (in2: level
  * heights and PICE_CLD_WGT; out2: its ice cloud outputs), then the sedimentation and

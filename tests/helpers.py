@@ -75,3 +75,85 @@ def flips_of(out, ref, calls="calls", iters="iters"):
     process such as OCND2's RILARS empties the ice, the sign of ri's rounding residual
     decides it; micro/mode_ice4_stepping.F90:471-474)."""
     return (out[calls] != ref[iters]) | ((out["ci"] == 0.0) != (ref["ci"] == 0.0))
+
+
+def ill_conditioned(run, s, names, tol, fields, delta=4.5e-16, draws=2, seed=20260090):
+    """Points whose oracle outputs are not determined to tol by FP64 inputs: run(s) against
+    run(s') where every field of `fields` is multiplied elementwise by 1 +- delta (a random
+    sign per value; delta is two units in the last place), for `draws` sign patterns.  A
+    point where any output of `names` moves by more than max(tol*|ref|, atol) sits on a
+    discontinuity of the reference's algorithm (a threshold or a time-step limiter that a
+    last-bit change decides the other way, without changing the number of tendency calls),
+    so two correct FP64 evaluations may disagree there: it is counted as a branch flip."""
+    base = run(s)
+    rng = np.random.Generator(np.random.Philox(seed))
+    mask = None
+    for _ in range(draws):
+        sp = dict(s)
+        for k in fields:
+            v = np.asarray(s[k], dtype=np.float64)
+            sp[k] = v * (1.0 + delta * rng.choice([-1.0, 1.0], size=v.shape))
+        o = run(sp)
+        for k in names:
+            r = rel_err(o[k], base[k], floor_for(k, tol))
+            if r.ndim != 2:
+                continue
+            m = ~(r <= tol)
+            mask = m if mask is None else (mask | m)
+    return mask
+
+
+def reproduced_by_oracle(out, run, s, names, tol, fields, points, scales=(1e-15, 1e-14, 1e-13)):
+    """Of the given (level, column) points, those where the oracle, run on the point's column
+    with one input value of the point changed by a relative 1e-15..1e-13 (one field at a
+    time, both signs), gives out's values at the point within tol for every output of
+    `names`.  Such a point sits on a discontinuity of the reference's algorithm (a species
+    emptied to exactly 0 or to a rounding residue, a threshold met to the last bit) and out
+    took the branch that a neighbouring input takes: a branch flip, shown by the oracle
+    itself.  A wrong result is not reproduced this way."""
+    found = np.zeros(len(points), dtype=bool)
+    for n, (k, i) in enumerate(points):
+        col = {f: (np.ascontiguousarray(v[:, i:i + 1]) if getattr(v, "ndim", 0) == 2 else
+                   (np.ascontiguousarray(v[i:i + 1]) if getattr(v, "ndim", 0) == 1 else v))
+               for f, v in s.items()}
+        for f in fields:
+            for e in scales:
+                for sg in (1.0, -1.0):
+                    sp = dict(col)
+                    v = col[f].copy()
+                    v[k, 0] = v[k, 0] * (1.0 + sg * e)
+                    sp[f] = v
+                    o = run(sp)
+                    ok = True
+                    for q in names:
+                        if np.ndim(o[q]) != 2:
+                            continue
+                        r = rel_err(o[q][k, 0], out[q][k, i], floor_for(q, tol))
+                        if not r <= tol:
+                            ok = False
+                            break
+                    if ok:
+                        found[n] = True
+                        break
+                if found[n]:
+                    break
+            if found[n]:
+                break
+    return found
+
+
+def explained_flips(out, ref, run, s, names, tol, fields, calls="calls", iters="iters"):
+    """The branch flips of out against the oracle's ref: flips_of, then ill_conditioned, then
+    the remaining divergent points that reproduced_by_oracle shows on a discontinuity."""
+    flips = flips_of(out, ref, calls, iters) | ill_conditioned(run, s, names, tol, fields)
+    bad = np.zeros(flips.shape, dtype=bool)
+    for k in names:
+        r = rel_err(out[k], ref[k], floor_for(k, tol))
+        if r.shape == flips.shape:
+            bad |= ~(r <= tol)
+    pts = [tuple(p) for p in np.argwhere(bad & ~flips)]
+    if pts and len(pts) <= 64:
+        rep = reproduced_by_oracle(out, run, s, names, tol, fields, pts)
+        for p, r in zip(pts, rep):
+            flips[p] = flips[p] or r
+    return flips

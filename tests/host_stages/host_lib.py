@@ -64,6 +64,14 @@ def load():
         lib.hst_adjust_rain_ice.argtypes = [ctypes.c_int] + common + [
             ctypes.POINTER(L.PipeIn), ctypes.POINTER(L.PipeOut), ctypes.POINTER(L.Stats)]
         lib.hst_set_substep_cap.argtypes = [ctypes.c_int]
+        lib.hst_ice_adjust_hail.argtypes = common + [ctypes.POINTER(L.AdjustIn), ctypes.c_void_p,
+                                                     ctypes.POINTER(L.AdjustOut)]
+        lib.hst_rain_ice_hail.argtypes = common + [
+            ctypes.POINTER(L.RainIn), ctypes.POINTER(L.RainIO), ctypes.POINTER(L.RainOut),
+            ctypes.POINTER(L.Hail), ctypes.POINTER(L.Stats)]
+        lib.hst_adjust_rain_ice_hail.argtypes = common + [
+            ctypes.POINTER(L.PipeIn), ctypes.POINTER(L.PipeOut), ctypes.POINTER(L.PipeHail),
+            ctypes.POINTER(L.Stats)]
         lib.hst_tiwmx.argtypes = [dp]
         lib.hst_rain_ice_ocnd2.argtypes = common + [
             ctypes.POINTER(L.RainIn), ctypes.POINTER(L.RainOcnd2In), ctypes.POINTER(L.RainIO),
@@ -214,3 +222,66 @@ def rain_ice_ocnd2(s: dict, dt: float = 50.0):
     io["calls"] = calls
     io["counters"] = counters
     return io
+
+
+def ice_adjust_hail(f: dict, dt: float = 50.0):
+    """ICE_ADJUST with KRR=7 on host arrays: the ice_adjust fields plus the hail mixing ratio rh."""
+    import numpy as np
+    from ice3_amd import _lib as L
+    lib = load()
+    nlev, ncol = f["th"].shape
+    inp = {k: np.ascontiguousarray(f[k], dtype=np.float64) for k in L.ADJ_IN_FIELDS}
+    rh = np.ascontiguousarray(f["rh"], dtype=np.float64)
+    out = {k: np.empty((nlev, ncol)) for k in L.ADJ_OUT_FIELDS}
+    rc = lib.hst_ice_adjust_hail(ncol, nlev, ncol, dt, ctypes.byref(L.AdjustIn(*[_p(inp[k]) for k in L.ADJ_IN_FIELDS])),
+                                 _p(rh), ctypes.byref(L.AdjustOut(*[_p(out[k]) for k in L.ADJ_OUT_FIELDS])))
+    assert rc == 0
+    return out
+
+
+def rain_ice_hail(s: dict, dt: float = 50.0):
+    """RAIN_ICE with KRR=7 on host arrays (rain_ice's fields plus rht and rhs)."""
+    import numpy as np
+    from ice3_amd import _lib as L
+    lib = load()
+    nlev, ncol = s["tht"].shape
+    inp = {k: np.ascontiguousarray(s[k], dtype=np.float64) for k in L.RAIN_IN_FIELDS}
+    io = {k: np.array(s[k], dtype=np.float64, order="C") for k in L.RAIN_IO_FIELDS}
+    out = {k: np.empty((nlev, ncol)) for k in ("evap3d", "rainfr")}
+    out.update({k: np.empty(ncol) for k in ("inprc", "inprr", "inpri", "inprs", "inprg", "inprh")})
+    rht = np.ascontiguousarray(s["rht"], dtype=np.float64)
+    rhs = np.array(s["rhs"], dtype=np.float64, order="C")
+    calls = np.zeros((nlev, ncol), dtype=np.int32)
+    counters = np.zeros(3, dtype=np.uint64)
+    rc = lib.hst_rain_ice_hail(ncol, nlev, ncol, dt, ctypes.byref(L.RainIn(*[_p(inp[k]) for k in L.RAIN_IN_FIELDS])),
+                               ctypes.byref(L.RainIO(*[_p(io[k]) for k in L.RAIN_IO_FIELDS])),
+                               ctypes.byref(L.RainOut(*[_p(out[k]) for k in L.RAIN_OUT_FIELDS])),
+                               ctypes.byref(L.Hail(_p(rht), _p(rhs), _p(out["inprh"]))),
+                               ctypes.byref(L.Stats(_p(calls), _p(counters))))
+    assert rc == 0
+    io.update(out)
+    io.update(rhs=rhs, calls=calls, counters=counters)
+    return io
+
+
+def adjust_rain_ice_hail(f: dict, dt: float = 50.0):
+    """The fused step with hail on host arrays: the ice3_pipeline_in fields plus rh and rhs."""
+    import numpy as np
+    from ice3_amd import _lib as L
+    lib = load()
+    nlev, ncol = f["th"].shape
+    inp = {k: np.ascontiguousarray(f[k], dtype=np.float64) for k in L.PIPE_IN_FIELDS}
+    out = {k: np.empty((nlev, ncol)) for k in L.PIPE_OUT_FIELDS[:15]}
+    out.update({k: np.empty(ncol) for k in L.PIPE_OUT_FIELDS[15:]})
+    rh, rhs = (np.ascontiguousarray(f[k], dtype=np.float64) for k in ("rh", "rhs"))
+    rhs_out, inprh = np.empty((nlev, ncol)), np.empty(ncol)
+    calls = np.zeros((nlev, ncol), dtype=np.int32)
+    counters = np.zeros(3, dtype=np.uint64)
+    rc = lib.hst_adjust_rain_ice_hail(ncol, nlev, ncol, dt,
+                                      ctypes.byref(L.PipeIn(*[_p(inp[k]) for k in L.PIPE_IN_FIELDS])),
+                                      ctypes.byref(L.PipeOut(*[_p(out[k]) for k in L.PIPE_OUT_FIELDS])),
+                                      ctypes.byref(L.PipeHail(_p(rh), _p(rhs), _p(rhs_out), _p(inprh))),
+                                      ctypes.byref(L.Stats(_p(calls), _p(counters))))
+    assert rc == 0
+    out.update(rhs=rhs_out, inprh=inprh, calls=calls, counters=counters)
+    return out

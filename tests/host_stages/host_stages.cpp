@@ -150,6 +150,71 @@ extern "C" int hst_adjust_rain_ice_ocnd2(int ncol, int nlev, int ld, double dt,
   return 0;
 }
 
+// ---- hail (KRR=7): the kernels' order (all columns' sedimentation, then every point's
+// microphysics), with the hail stage functions.
+static void run_rain_ice_hail(const SedimArgs &sa, const MicroArgs &m) {
+  for (int i = 0; i < sa.ncol; ++i) sedim_column<double, false, true>(sa, i);
+  unsigned long long n = 0, c = 0, p = 0;
+  for (int k = 0; k < m.nlev; ++k)
+    for (int i = 0; i < m.ncol; ++i) {
+      const int calls = micro_at<double, false, true>(m, (size_t)k * m.ld + i, m.tab);
+      n += calls < 0 ? -calls : calls;
+      c += calls < 0;
+      p += calls != 0;
+    }
+  if (m.counters) { m.counters[0] += n; m.counters[1] += c; m.counters[2] += p; }
+}
+
+extern "C" int hst_ice_adjust_hail(int ncol, int nlev, int ld, double dt,
+                                   const ice3_ice_adjust_in *in, const double *rh,
+                                   const ice3_ice_adjust_out *out) {
+  if (hst_init()) return -1;
+  AdjustArgs a = AdjustArgs{};
+  a.ncol = ncol; a.nlev = nlev; a.ld = ld; a.dt = dt; a.in = *in; a.out = *out; a.rh = rh;
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) adjust_at<false, true>(a, i, (size_t)k * ld + i);
+  return 0;
+}
+
+extern "C" int hst_rain_ice_hail(int ncol, int nlev, int ld, double dt, const ice3_rain_ice_in *in,
+                                 const ice3_rain_ice_inout *io, const ice3_rain_ice_out *out,
+                                 const ice3_hail *hail, const ice3_stats *stats) {
+  if (hst_init()) return -1;
+  SedimArgs sa;
+  MicroArgs m;
+  rain_ice_args(ncol, nlev, ld, dt, g_tab, in, io, out, stats, sa, m);
+  m.max_calls = g_max_calls;
+  sa.src_rh = hail->rhs; sa.dst_rh = hail->rhs; sa.inprh = hail->inprh;
+  m.t[IRH] = hail->rht; m.src[IRH] = hail->rhs; m.dst[IRH] = hail->rhs;
+  run_rain_ice_hail(sa, m);
+  return 0;
+}
+
+extern "C" int hst_adjust_rain_ice_hail(int ncol, int nlev, int ld, double dt,
+                                        const ice3_pipeline_in *in, const ice3_pipeline_out *out,
+                                        const ice3_pipeline_hail *hail, const ice3_stats *stats) {
+  if (hst_init()) return -1;
+  AdjustArgs aa;
+  SedimArgs sa;
+  MicroArgs m;
+  pipeline_args(ncol, nlev, ld, dt, g_tab, in, out, stats, aa, sa, m);
+  m.max_calls = g_max_calls;
+  aa.rh = hail->rh;
+  sa.src_rh = hail->rhs; sa.dst_rh = hail->rhs_out; sa.inprh = hail->inprh;
+  m.t[IRH] = hail->rhs; m.src[IRH] = hail->rhs_out; m.dst[IRH] = hail->rhs_out;
+  for (int i = 0; i < ncol; ++i) adjust_sedim_column<double, true>(aa, sa, i);
+  for (int k = 0; k < m.nlev; ++k)
+    for (int i = 0; i < m.ncol; ++i) {
+      const int calls = micro_at<double, false, true>(m, (size_t)k * m.ld + i, m.tab);
+      if (stats && stats->counters) {
+        stats->counters[0] += calls < 0 ? -calls : calls;
+        stats->counters[1] += calls < 0;
+        stats->counters[2] += calls != 0;
+      }
+    }
+  return 0;
+}
+
 #ifdef HST_MAIN
 // Sanitizer driver: host_stages_san <dir> <ncol> <nlev> <dt> <prec> reads
 // <dir>/<field>.bin (float64, (nlev, ncol), sigqsat (ncol)) for every

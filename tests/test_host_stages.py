@@ -279,3 +279,60 @@ def test_host_rain_ice_ocnd2_golden(golden_dir, tag):
     ref["iters"] = oracle_lib.rain_ice_ocnd2(s, float(g["dt"]), with_iters=True)["iters"]
     w, nf, nb = check_with_flips(out, ref, RAIN_OUT, flips_of(out, ref), TOL)
     print(f"host stages rain_ice OCND2 golden {tag}: flips {nf}, divergent {nb}, worst {max(w.values()):.2e}")
+
+
+# ------------------------------------------------------------- hail (KRR=7)
+
+HAIL_OUT = RAIN_OUT + ("rhs", "inprh")
+HAIL_STATE = ("tht", "rvt", "rct", "rrt", "rit", "rst", "rgt", "rht", "ths", "rvs", "rcs", "rrs",
+              "ris", "rss", "rgs", "rhs")
+
+
+def _hail_flips(out, s, run):
+    """Branch flips of a hail case against the oracle (helpers.explained_flips)."""
+    from helpers import explained_flips
+    ref = run(s, with_iters=True)
+    return explained_flips(out, ref, run, s, HAIL_OUT, TOL, HAIL_STATE), ref
+
+
+@pytest.mark.parametrize("tag", ["base", "cov"])
+def test_host_rain_ice_hail_golden(golden_dir, tag):
+    """The kernels' hail stage source (KRR=7) against the compiled reference's hail golden
+    vectors at the FP64 bar, flips counted."""
+    from oracle import oracle_lib
+    g = np.load(os.path.join(golden_dir, "rain_ice_hail_fp64.npz"))
+    s = {k[len(tag) + 4:]: g[k] for k in g.files if k.startswith(tag + "_in_")}
+    ref = {k: g[f"{tag}_out_{k}"] for k in HAIL_OUT}
+    out = host_lib.rain_ice_hail(s, float(g["dt"]))
+    flips, _ = _hail_flips(out, s, lambda x, **kw: oracle_lib.rain_ice_hail(x, float(g["dt"]), **kw))
+    w, nf, nb = check_with_flips(out, ref, HAIL_OUT, flips, TOL)
+    print(f"host stages rain_ice hail golden {tag}: flips {nf}, divergent {nb}, worst {max(w.values()):.2e}")
+    p = f"{tag}_adj_in_"
+    a = host_lib.ice_adjust_hail({k[len(p):]: g[k] for k in g.files if k.startswith(p)}, float(g["dt"]))
+    aref = {k: g[f"{tag}_adj_out_{k}"] for k in ADJ_OUT}
+    wa = _worst(a, aref, ADJ_OUT)
+    assert max(wa.values()) <= TOL, wa
+
+
+def test_host_pipeline_hail_matches_oracle():
+    """The fused step with hail (ICE_ADJUST with the hail heat capacity, hail sedimentation in
+    the same column walk, the KRR=7 microphysics) against the oracle, flips counted; column
+    water (hail and surface hail included) is conserved."""
+    from oracle import oracle_lib, ref_lib
+    f = _pipeline_input(96, 20260079)
+    rng = np.random.Generator(np.random.Philox(20260080))
+    f["rh"] = np.ascontiguousarray(f["rg"] * rng.uniform(0.2, 2.0, f["rg"].shape))
+    f["rhs"] = f["rh"] / 50.0
+    out = host_lib.adjust_rain_ice_hail(f, 50.0)
+    a = oracle_lib.ice_adjust_hail(f)
+    s = ref_lib.pipeline_state_hail(f, a)
+    flips, _ = _hail_flips(out, s, lambda x, **kw: oracle_lib.rain_ice_hail(x, **kw))
+    ref = oracle_lib.rain_ice_hail(s)
+    w, nf, nb = check_with_flips(out, ref, HAIL_OUT, flips, TOL)
+    print(f"host stages vs oracle, fused hail step: flips {nf}, divergent {nb}, worst {max(w.values()):.2e}")
+    rho, dz = f["rhodref"], f["dzz"]
+    ws = ("rvs", "rcs", "rrs", "ris", "rss", "rgs", "rhs")
+    col0 = (rho * dz * sum(f[k] for k in ws)).sum(0) * 50.0
+    col1 = (rho * dz * sum(out[k] for k in ws)).sum(0) * 50.0
+    pr = -1000.0 * (out["inprr"] + out["inprs"] + out["inprg"] + out["inprh"]) * 50.0
+    assert np.max(np.abs(col1 - col0 - pr) / col0) < 1e-12

@@ -32,7 +32,9 @@ def test_struct_layouts_match_header():
                       ("ice3_rain_ice_in", _lib.RainIn), ("ice3_rain_ice_inout", _lib.RainIO),
                       ("ice3_rain_ice_out", _lib.RainOut), ("ice3_pipeline_in", _lib.PipeIn),
                       ("ice3_pipeline_out", _lib.PipeOut), ("ice3_stats", _lib.Stats),
-                      ("ice3_ocnd2_in", _lib.Ocnd2In), ("ice3_ocnd2_out", _lib.Ocnd2Out)):
+                      ("ice3_ocnd2_in", _lib.Ocnd2In), ("ice3_ocnd2_out", _lib.Ocnd2Out),
+                      ("ice3_rain_ocnd2_in", _lib.RainOcnd2In), ("ice3_hail", _lib.Hail),
+                      ("ice3_pipeline_hail", _lib.PipeHail)):
         body = re.search(r"typedef struct \{([^}]*)\} %s;" % cname, hdr).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         names = re.findall(r"\*\s*(\w+)", body)
@@ -45,6 +47,9 @@ def test_bad_arguments_are_rejected_without_a_device(lib):
     assert lib.ice3_rain_ice(None, 0, 4, 90, 4, 50.0, None, None, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_adjust_rain_ice(None, 0, 4, 90, 4, 50.0, None, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_ice_adjust_ocnd2(None, 4, 90, 4, 50.0, None, None, None, None, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_ice_adjust_hail(None, 4, 90, 4, 50.0, None, None, None, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_rain_ice_hail(None, 4, 90, 4, 50.0, None, None, None, None, None, None) == _lib.ICE3_EINVAL
+    assert lib.ice3_adjust_rain_ice_hail(None, 4, 90, 4, 50.0, None, None, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_ctx_create(None) == _lib.ICE3_EINVAL
     # input validation: argument checks come before any launch
     fp = (p * 2)(None, None)
@@ -97,7 +102,8 @@ def test_constants_match_reference_ini_rain_ice(golden_dir):
 def test_collection_kernel_tables_are_bit_exact(golden_dir):
     ref = _golden_constants(golden_dir)
     mine = _lib.host_constants()
-    for k in ("XKER_RACCSS", "XKER_RACCS", "XKER_SACCRG", "XKER_SDRYG", "XKER_RDRYG"):
+    for k in ("XKER_RACCSS", "XKER_RACCS", "XKER_SACCRG", "XKER_SDRYG", "XKER_RDRYG", "XKER_SWETH",
+              "XKER_GWETH", "XKER_RWETH"):
         assert np.array_equal(np.array(mine[k]), ref[k]), k
 
 
