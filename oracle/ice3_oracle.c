@@ -807,6 +807,16 @@ static unsigned g_branches;
 static unsigned *g_branch_out;
 #define BR(bit, cond) do { if (cond) g_branches |= (1u << (bit)); } while (0)
 
+/* A wet/dry growth choice compares max(0, w) with max(0, d), where w and d are differences
+ * of sums of the same rates (ICE4_FAST_RG :303-323, ICE4_FAST_RH :291-311).  When the
+ * collected species are all solid, both are 0 in exact arithmetic and each evaluates to
+ * the rounding residual of its sum: the choice then rests on the signs of those residuals,
+ * which any last-bit difference of a rate changes.  Recorded as ORACLE_BR_TIE. */
+static int rounding_tie(double w, double wscale, double d, double dscale) {
+  const double eps = 2.220446049250313e-16;
+  return fabs(w) <= 8. * eps * fabs(wscale) && fabs(d) <= 8. * eps * fabs(dscale);
+}
+
 void oracle_record_branches(unsigned *flags) { g_branch_out = flags; }
 
 /* Per-point persistent state of ICE4_STEPPING (micro/mode_ice4_stepping.F90:128-158) */
@@ -1192,6 +1202,9 @@ static void ice4_tendencies(int krr, int soft, int ldc, double exn, double rhodr
     lldryg = zt < P.XTT && rdryg_init > 1.E-20 &&
              dmax(0., rwetg_init - rg_tend[RG_RIWETG] - rg_tend[RG_RSWETG]) >
                  dmax(0., rdryg_init - rg_tend[RG_RIDRYG] - rg_tend[RG_RSDRYG]);
+    BR(ORACLE_BR_TIE, rdryg_init > 0. &&
+                          rounding_tie(rwetg_init - rg_tend[RG_RIWETG] - rg_tend[RG_RSWETG], rwetg_init,
+                                       rdryg_init - rg_tend[RG_RIDRYG] - rg_tend[RG_RSDRYG], rdryg_init));
   } else {
     rg_tend[RG_FREEZ1] = 0.; rg_tend[RG_FREEZ2] = 0.;
     rwetg_init = 0.; ldwetg = 0; lldryg = 0;
@@ -1315,6 +1328,14 @@ static void ice4_tendencies(int krr, int soft, int ldc, double exn, double rhodr
       lldryh = zt < P.XTT && rdryh_init > 1.E-20 &&
                dmax(0., rweth_init - rh_tend[RH_RIWETH] - rh_tend[RH_RSWETH]) >
                    dmax(0., rdryh_init - rh_tend[RH_RIDRYH] - rh_tend[RH_RSDRYH]);
+      BR(ORACLE_BR_TIE,
+         rdryh_init > 0. &&
+             (rounding_tie(rweth_init - rh_tend[RH_RIWETH] - rh_tend[RH_RSWETH] - rh_tend[RH_RGWETH],
+                           rweth_init,
+                           rdryh_init - rh_tend[RH_RIDRYH] - rh_tend[RH_RSDRYH] - rh_tend[RH_RGDRYH],
+                           rdryh_init) ||
+              rounding_tie(rweth_init - rh_tend[RH_RIWETH] - rh_tend[RH_RSWETH], rweth_init,
+                           rdryh_init - rh_tend[RH_RIDRYH] - rh_tend[RH_RSDRYH], rdryh_init)));
     } else {
       rh_tend[RH_FREEZ1] = 0.; rh_tend[RH_FREEZ2] = 0.;
     }

@@ -81,7 +81,8 @@ int oracle_rain_ice_hail(int ncol, int nlev, double dt, const double *const in[O
 enum {
   ORACLE_BR_RRHONG, ORACLE_BR_RIMLTC, ORACLE_BR_RIMING, ORACLE_BR_RACCSG, ORACLE_BR_SMLTG,
   ORACLE_BR_WETG, ORACLE_BR_DRYG, ORACLE_BR_GMLTR, ORACLE_BR_WETGH, ORACLE_BR_WETH,
-  ORACLE_BR_DRYH, ORACLE_BR_DRYHG, ORACLE_BR_HMLTR
+  ORACLE_BR_DRYH, ORACLE_BR_DRYHG, ORACLE_BR_HMLTR,
+  ORACLE_BR_TIE  /* a graupel or hail wet/dry choice decided by rounding residuals */
 };
 void oracle_record_branches(unsigned *flags);
 

@@ -184,7 +184,7 @@ def rain_ice_hail(s: dict, dt: float = 50.0, with_iters: bool = False) -> dict:
 
 
 BRANCHES = ("rrhong", "rimltc", "riming", "raccsg", "smltg", "wetg", "dryg", "gmltr", "wetgh",
-            "weth", "dryh", "dryhg", "hmltr")
+            "weth", "dryh", "dryhg", "hmltr", "tie")
 
 
 def record_branches(fn, *a, **kw):
@@ -204,3 +204,11 @@ def record_branches(fn, *a, **kw):
 def branch_counts(flags) -> dict:
     """Number of points that took each branch."""
     return {b: int(((flags >> i) & 1).sum()) for i, b in enumerate(BRANCHES)}
+
+
+def rounding_ties(fn, *a, **kw):
+    """Points where a graupel or hail wet/dry growth choice of the oracle's run rests on the
+    signs of rounding residuals (the "tie" branch record): their branch is decided by the
+    last bits of the rates."""
+    _, flags = record_branches(fn, *a, **kw)
+    return ((flags >> BRANCHES.index("tie")) & 1).astype(bool)

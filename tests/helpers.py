@@ -142,15 +142,19 @@ def reproduced_by_oracle(out, run, s, names, tol, fields, points, scales=(1e-15,
     return found
 
 
-def explained_flips(out, ref, run, s, names, tol, fields, calls="calls", iters="iters"):
+def explained_flips(out, ref, run, s, names, tol, fields, calls="calls", iters="iters", ties=None):
     """The branch flips of out against the oracle's ref: flips_of, then ill_conditioned, then
-    the remaining divergent points that reproduced_by_oracle shows on a discontinuity."""
+    the remaining divergent points where the oracle's wet/dry growth choice rests on rounding
+    residuals (ties: the oracle's "tie" branch record, oracle_lib.rounding_ties), then those
+    that reproduced_by_oracle shows on a discontinuity."""
     flips = flips_of(out, ref, calls, iters) | ill_conditioned(run, s, names, tol, fields)
     bad = np.zeros(flips.shape, dtype=bool)
     for k in names:
         r = rel_err(out[k], ref[k], floor_for(k, tol))
         if r.shape == flips.shape:
             bad |= ~(r <= tol)
+    if ties is not None:
+        flips |= bad & ties
     pts = [tuple(p) for p in np.argwhere(bad & ~flips)]
     if pts and len(pts) <= 64:
         rep = reproduced_by_oracle(out, run, s, names, tol, fields, pts)

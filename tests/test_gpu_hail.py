@@ -6,9 +6,10 @@ hail and surface hail, and the reduction to KRR=6 without hail and graupel.
 
 FP64 bar as in test_gpu_parity: relative error <= 1e-11 with absolute floors; branch flips
 are the points whose number of tendency calls or ice-concentration reset differs from the
-oracle's, the points where the oracle is not determined to the bar by FP64 inputs, and the
-points where the oracle reproduces the GPU's value from an input changed in its last digits
-(helpers.explained_flips)."""
+oracle's, the points where the oracle is not determined to the bar by FP64 inputs, the
+points where the oracle's graupel or hail wet/dry growth choice rests on rounding residuals,
+and the points where the oracle reproduces the GPU's value from an input changed in its last
+digits (helpers.explained_flips)."""
 import os
 
 import numpy as np
@@ -46,6 +47,11 @@ def _np(t):
 def _oracle_run(dt=DT):
     from oracle import oracle_lib
     return lambda x, **kw: oracle_lib.rain_ice_hail(x, dt, **kw)
+
+
+def _ties(s, dt=DT):
+    from oracle import oracle_lib
+    return oracle_lib.rounding_ties(oracle_lib.rain_ice_hail, s, dt)
 
 
 def _hail_state(s):
@@ -119,7 +125,7 @@ def test_rain_ice_hail_golden(dev, golden_dir, tag):
     out.update({k: _np(v) for k, v in diags.items()})
     run = _oracle_run(dt)
     o = run(s, with_iters=True)
-    flips = explained_flips(out, o, run, s, HAIL_OUT, TOL, HAIL_STATE)
+    flips = explained_flips(out, o, run, s, HAIL_OUT, TOL, HAIL_STATE, ties=_ties(s, dt))
     w, nf, nb = check_with_flips(out, ref, HAIL_OUT, flips, TOL)
     print(f"rain_ice hail golden {tag}: {nf} flips, {nb} divergent of {flips.size}, worst {max(w.values()):.2e}")
 
@@ -137,7 +143,7 @@ def test_pipeline_hail_vs_oracle(dev):
     s = ref_lib.pipeline_state_hail(f, a)
     run = _oracle_run()
     ref = run(s, with_iters=True)
-    flips = explained_flips(out, ref, run, s, HAIL_OUT, TOL, HAIL_STATE)
+    flips = explained_flips(out, ref, run, s, HAIL_OUT, TOL, HAIL_STATE, ties=_ties(s))
     w, nf, nb = check_with_flips(out, ref, HAIL_OUT, flips, TOL)
     for k in ("cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf"):
         assert rel_err(out[k], a[k], floor_for(k, TOL)).max() <= TOL, k
@@ -190,7 +196,7 @@ def test_pipeline_hail_full_size(dev):
     s = ref_lib.pipeline_state_hail(g, oracle_lib.ice_adjust_hail(g))
     run = _oracle_run()
     ref = run(s, with_iters=True)
-    flips = explained_flips(got, ref, run, s, HAIL_OUT, TOL, HAIL_STATE)
+    flips = explained_flips(got, ref, run, s, HAIL_OUT, TOL, HAIL_STATE, ties=_ties(s))
     w, nf, nb = check_with_flips(got, ref, HAIL_OUT, flips, TOL)
     print(f"fused hail step 512x512x90, sampled columns: budget {err:.1e}, {nf} flips, {nb} divergent "
           f"of {flips.size}, worst {max(w.values()):.2e}")
