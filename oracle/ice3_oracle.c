@@ -1651,6 +1651,171 @@ static void sedimentation_column(int ncol, int nlev, int i, double dt, const dou
   }
 }
 
+/* ---------------------------------------------------------------- split sedimentation */
+
+/* Sedimentation scheme of the rain_ice calls (PARAM_ICE CSEDIM and XSPLIT_MAXCFL,
+ * micro/modd_param_icen.F90:83,347): ORACLE_SEDIM_STAT (the pinned AROME default) or
+ * ORACLE_SEDIM_SPLI with the CFL bound of its sub-steps. */
+static int g_sedim = ORACLE_SEDIM_STAT;
+static double g_maxcfl = 0.8;
+
+int oracle_set_sedimentation(int scheme, double maxcfl) {
+  if ((scheme != ORACLE_SEDIM_STAT && scheme != ORACLE_SEDIM_SPLI) || !(maxcfl > 0.))
+    return -1;
+  g_sedim = scheme;
+  g_maxcfl = maxcfl;
+  return 0;
+}
+
+/* Mass flux ZWSED of species jrr (3 rain, 4 pristine ice, 5 snow, 6 graupel, 7 hail) at a
+ * level with mixing ratio r above the species' threshold, LNEWCOEFF=F
+ * (micro/mode_ice4_sedimentation_split.F90: pristine ice :574-578; snow :630-633 and the
+ * other species :747-749, one power law). */
+static double spli_flux(int jrr, double r, double rho) {
+  if (jrr == 4)
+    return P.XFSEDI * r * pow(rho, 1. - P.XCEXVT) *
+           pow(dmax(0.05E6, -0.15319E6 - 0.021454E6 * log(rho * r)), P.XEXCSEDI);
+  double fsed = jrr == 3 ? P.XFSEDR : (jrr == 5 ? P.XFSEDS : (jrr == 6 ? P.XFSEDG : P.XFSEDH));
+  double exsed = jrr == 3 ? P.XEXSEDR : (jrr == 5 ? P.XEXSEDS : (jrr == 6 ? P.XEXSEDG : P.XEXSEDH));
+  return fsed * pow(r, exsed) * pow(rho, exsed - P.XCEXVT);
+}
+
+/* INTERNAL_SEDIM_SPLI (micro/mode_ice4_sedimentation_split.F90:359-849) for column i and
+ * species jrr, LSEDIC=F, OELEC=F, one column at a time (the reference loops over the block
+ * until no column has time left; a column whose time is spent changes no more, :488,838-846).
+ * r: the column's working mixing ratio ZR?T (nlev, updated), prs: the external tendency
+ * PPRXS (nlev), oorhodz: POORHODZ (nlev), wsed: work (nlev + 1), s: the source array
+ * (nlev, ncol), updated.  Returns the surface rate PINPRX; *nsub receives the number of
+ * sub-steps. */
+static double spli_species(int ncol, int nlev, int i, double dt, int jrr, double maxcfl,
+                           const double *dzz, const double *rhodref, double *r, const double *prs,
+                           const double *oorhodz, double *wsed, double *s, int *nsub) {
+  double zinv = 1. / dt, remaint = dt, inpr = 0.;
+  double thr = jrr == 4 ? dmax(P.XRTMIN[4], 1.0E-7) : P.XRTMIN[jrr]; /* :574, :630, :747 */
+  int n = 0;
+  wsed[nlev] = 0.; /* ZWSED(:, NKT+1): nothing enters through the top */
+  while (remaint > 0.) { /* :488, :838-846 */
+    for (int k = 0; k < nlev; ++k) {
+      size_t j = (size_t)k * ncol + i;
+      wsed[k] = r[k] > thr ? spli_flux(jrr, r[k], rhodref[j]) : 0.;
+    }
+    double step = remaint; /* ZMAX_TSTEP1D (:783-810) */
+    for (int k = 0; k < nlev; ++k) {
+      size_t j = (size_t)k * ncol + i;
+      if (r[k] > P.XRTMIN[jrr] && wsed[k] > 1.E-20)
+        step = dmin(step, maxcfl * rhodref[j] * r[k] * dzz[j] / wsed[k]);
+    }
+    remaint = remaint - step; /* :814-815 */
+    inpr = inpr + wsed[0] / P.XRHOLW * (step * zinv);
+    for (int k = 0; k < nlev; ++k) { /* :820-824, IKL = 1 */
+      size_t j = (size_t)k * ncol + i;
+      double mrchange = step * oorhodz[k] * (wsed[k + 1] - wsed[k]);
+      r[k] = r[k] + mrchange + prs[k] * step;
+      s[j] = s[j] + mrchange * zinv;
+    }
+    ++n;
+  }
+  if (nsub) *nsub = n;
+  return inpr;
+}
+
+/* ICE4_SEDIMENTATION_SPLIT (micro/mode_ice4_sedimentation_split.F90:9-352) for every
+ * column: rt[q] the mixing ratios at t and rs[q] the sources (updated) of rain, pristine
+ * ice, snow, graupel and, with krr = 7, hail (q = 0..4); inpr[q] their surface rates.
+ * nsub (optional, 5 x ncol) receives the sub-step counts. */
+int oracle_sedim_split(int ncol, int nlev, double dt, int krr, double maxcfl, const double *dzz,
+                       const double *rhodref, const double *const rt[5], double *const rs[5],
+                       double *const inpr[5], int *nsub) {
+  if (!P_loaded) return -1;
+  if ((krr != 6 && krr != 7) || !(maxcfl > 0.)) return -2;
+  double *work = (double *)malloc(sizeof(double) * (4 * (size_t)nlev + 1));
+  if (!work) return -3;
+  double *r = work, *prs = work + nlev, *oorhodz = work + 2 * nlev, *wsed = work + 3 * nlev;
+  double zinv = 1. / dt;
+  for (int i = 0; i < ncol; ++i) {
+    for (int k = 0; k < nlev; ++k) { /* ZW (:263) */
+      size_t j = (size_t)k * ncol + i;
+      oorhodz[k] = 1. / (rhodref[j] * dzz[j]);
+    }
+    for (int q = 0; q < (krr == 7 ? 5 : 4); ++q) {
+      for (int k = 0; k < nlev; ++k) { /* external tendencies and working copies (:241-262) */
+        size_t j = (size_t)k * ncol + i;
+        prs[k] = rs[q][j] - rt[q][j] * zinv;
+        r[k] = rt[q][j];
+      }
+      int n = 0;
+      inpr[q][i] = spli_species(ncol, nlev, i, dt, q + 3, maxcfl, dzz, rhodref, r, prs, oorhodz,
+                                wsed, rs[q], &n);
+      if (nsub) nsub[(size_t)q * ncol + i] = n;
+    }
+  }
+  free(work);
+  return 0;
+}
+
+/* ICE4_CORRECT_NEGATIVITIES (micro/mode_ice4_correct_negativities.F90:51-171) on one
+ * point's sources with the latent heat factors lvfact, lsfact; rhs (hail) when not NULL. */
+static void correct_negativities(double *ths, double *rvs, double *rcs, double *rrs, double *ris,
+                                 double *rss, double *rgs, double *rhs, double lvfact,
+                                 double lsfact) {
+  double zw;
+  zw = *rcs - dmax(*rcs, 0.); *rvs = *rvs + zw; *ths = *ths - zw * lvfact; *rcs = *rcs - zw;
+  zw = *rrs - dmax(*rrs, 0.); *rvs = *rvs + zw; *ths = *ths - zw * lvfact; *rrs = *rrs - zw;
+  zw = *ris - dmax(*ris, 0.); *rvs = *rvs + zw; *ths = *ths - zw * lsfact; *ris = *ris - zw;
+  zw = *rss - dmax(*rss, 0.); *rvs = *rvs + zw; *ths = *ths - zw * lsfact; *rss = *rss - zw;
+  zw = *rgs - dmax(*rgs, 0.); *rvs = *rvs + zw; *ths = *ths - zw * lsfact; *rgs = *rgs - zw;
+  if (rhs) { zw = *rhs - dmax(*rhs, 0.); *rvs = *rvs + zw; *ths = *ths - zw * lsfact; *rhs = *rhs - zw; }
+  zw = dmin(1., dmax(P.XRTMIN[1] - *rvs, 0.) / dmax(*rcs + *ris, 1.E-20));
+  *ths = *ths - zw * (*rcs * lvfact + *ris * lsfact);
+  *rvs = *rvs + zw * (*rcs + *ris);
+  *rcs = (1. - zw) * *rcs;
+  *ris = (1. - zw) * *ris;
+  zw = dmin(dmax(*rrs, 0.), dmax(P.XRTMIN[1] - *rvs, 0.));
+  *rvs = *rvs + zw; *rrs = *rrs - zw; *ths = *ths - zw * lvfact;
+  zw = dmin(dmax(*rss, 0.), dmax(P.XRTMIN[1] - *rvs, 0.));
+  *rvs = *rvs + zw; *rss = *rss - zw; *ths = *ths - zw * lsfact;
+  zw = dmin(dmax(*rgs, 0.), dmax(P.XRTMIN[1] - *rvs, 0.));
+  *rvs = *rvs + zw; *rgs = *rgs - zw; *ths = *ths - zw * lsfact;
+  if (rhs) {
+    zw = dmin(dmax(*rhs, 0.), dmax(P.XRTMIN[1] - *rvs, 0.));
+    *rvs = *rvs + zw; *rhs = *rhs - zw; *ths = *ths - zw * lsfact;
+  }
+}
+
+/* ICE4_SEDIMENTATION with CSEDIM='SPLI' (micro/mode_ice4_sedimentation.F90:196-219), LSEDIC=F:
+ * ICE4_SEDIMENTATION_SPLIT on the state at t, PINPRS += PINPRI, then ICE4_CORRECT_NEGATIVITIES
+ * of every source with RAIN_ICE's latent heat factors ZZ_LVFACT, ZZ_LSFACT of the state at t
+ * (rain_ice.F90:335-345, passed at :448).  t = {th, rv, rc, rr, ri, rs, rg, rh} at t and
+ * src the sources of the same variables (updated); rh entries are used with krr = 7 only.
+ * inpr (ncol each) = {inprc, inprr, inpri, inprs, inprg, inprh}. */
+int oracle_sedimentation_spli(int ncol, int nlev, double dt, int krr, double maxcfl,
+                              const double *dzz, const double *rhodref, const double *exn,
+                              const double *const t[8], double *const src[8],
+                              double *const inpr[6], int *nsub) {
+  const double *rt[5] = {t[3], t[4], t[5], t[6], krr == 7 ? t[7] : NULL};
+  double *rs[5] = {src[3], src[4], src[5], src[6], krr == 7 ? src[7] : NULL};
+  double *pr[5] = {inpr[1], inpr[2], inpr[3], inpr[4], krr == 7 ? inpr[5] : NULL};
+  int rc = oracle_sedim_split(ncol, nlev, dt, krr, maxcfl, dzz, rhodref, rt, rs, pr, nsub);
+  if (rc != 0) return rc;
+  for (int i = 0; i < ncol; ++i) {
+    inpr[0][i] = 0.;
+    inpr[3][i] = inpr[3][i] + inpr[2][i];
+  }
+  for (int k = 0; k < nlev; ++k)
+    for (int i = 0; i < ncol; ++i) {
+      size_t j = (size_t)k * ncol + i;
+      double zrice = t[4][j] + t[5][j] + t[6][j];
+      if (krr == 7) zrice = zrice + t[7][j];
+      double zdevide = P.XCPD + P.XCPV * t[1][j] + P.XCL * (t[2][j] + t[3][j]) + P.XCI * zrice;
+      double zt = t[0][j] * exn[j];
+      double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT)) / zdevide;
+      double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT)) / zdevide;
+      correct_negativities(&src[0][j], &src[1][j], &src[2][j], &src[3][j], &src[4][j],
+                           &src[5][j], &src[6][j], krr == 7 ? &src[7][j] : NULL, lvfact, lsfact);
+    }
+  return 0;
+}
+
 /* RAIN_ICE (micro/rain_ice.F90:7-830) with the commented-out sedimentation (:442-457),
  * total tendencies (:754-793) and negativity correction (:821-823) reinstated.  x4 (NULL
  * for the plain branch) holds the OCND2 inputs PICLDFR, PSSIO, PSSIU, PIFR; h (NULL for
@@ -1679,11 +1844,27 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
       }
     }
   }
-  /* 2. sedimentation before microphysics, per column */
-  for (int i = 0; i < ncol; ++i) {
-    col[RC_INPRC][i] = 0.;
-    sedimentation_column(ncol, nlev, i, dt, in[RI_DZZ], in[RI_RHODREF], io, col, h);
-    col[RC_INPRS][i] = col[RC_INPRS][i] + col[RC_INPRI][i]; /* mode_ice4_sedimentation.F90:134 */
+  /* 2. sedimentation before microphysics */
+  if (g_sedim == ORACLE_SEDIM_STAT) {
+    for (int i = 0; i < ncol; ++i) {
+      col[RC_INPRC][i] = 0.;
+      sedimentation_column(ncol, nlev, i, dt, in[RI_DZZ], in[RI_RHODREF], io, col, h);
+      col[RC_INPRS][i] = col[RC_INPRS][i] + col[RC_INPRI][i]; /* mode_ice4_sedimentation.F90:134 */
+    }
+  } else {
+    /* CSEDIM='SPLI' */
+    const double *t[8] = {in[RI_THT], in[RI_RVT], in[RI_RCT], in[RI_RRT], in[RI_RIT], in[RI_RST],
+                          in[RI_RGT], h ? h->rht : NULL};
+    double *src[8] = {io[RO_THS], io[RO_RVS], io[RO_RCS], io[RO_RRS], io[RO_RIS], io[RO_RSS],
+                      io[RO_RGS], h ? h->rhs : NULL};
+    double *inpr[6] = {col[RC_INPRC], col[RC_INPRR], col[RC_INPRI], col[RC_INPRS],
+                       col[RC_INPRG], h ? h->inprh : NULL};
+    int rc = oracle_sedimentation_spli(ncol, nlev, dt, krr, g_maxcfl, in[RI_DZZ], in[RI_RHODREF],
+                                       in[RI_EXN], t, src, inpr, NULL);
+    if (rc != 0) {
+      free(zzzz);
+      return rc == -3 ? -3 : -1;
+    }
   }
   for (int k = 0; k < nlev; ++k)
     for (int i = 0; i < ncol; ++i) {
@@ -1755,30 +1936,10 @@ static int rain_ice_core(int ncol, int nlev, double dt, const double *const in[O
       double rss = io[RO_RSS][j] + wrs;
       double rgs = io[RO_RGS][j] + wrg;
       double rhs = h ? h->rhs[j] + wrh : 0.;
-      /* 8.2 ICE4_CORRECT_NEGATIVITIES (:821-823; mode_ice4_correct_negativities.F90:51-171) */
-      double zw;
-      zw = rcs - dmax(rcs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rcs = rcs - zw;
-      zw = rrs - dmax(rrs, 0.); rvs = rvs + zw; ths = ths - zw * lvfact; rrs = rrs - zw;
-      zw = ris - dmax(ris, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; ris = ris - zw;
-      zw = rss - dmax(rss, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rss = rss - zw;
-      zw = rgs - dmax(rgs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rgs = rgs - zw;
-      if (h) { zw = rhs - dmax(rhs, 0.); rvs = rvs + zw; ths = ths - zw * lsfact; rhs = rhs - zw; }
-      zw = dmin(1., dmax(P.XRTMIN[1] - rvs, 0.) / dmax(rcs + ris, 1.E-20));
-      ths = ths - zw * (rcs * lvfact + ris * lsfact);
-      rvs = rvs + zw * (rcs + ris);
-      rcs = (1. - zw) * rcs;
-      ris = (1. - zw) * ris;
-      zw = dmin(dmax(rrs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
-      rvs = rvs + zw; rrs = rrs - zw; ths = ths - zw * lvfact;
-      zw = dmin(dmax(rss, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
-      rvs = rvs + zw; rss = rss - zw; ths = ths - zw * lsfact;
-      zw = dmin(dmax(rgs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
-      rvs = rvs + zw; rgs = rgs - zw; ths = ths - zw * lsfact;
-      if (h) {
-        zw = dmin(dmax(rhs, 0.), dmax(P.XRTMIN[1] - rvs, 0.));
-        rvs = rvs + zw; rhs = rhs - zw; ths = ths - zw * lsfact;
-        h->rhs[j] = rhs;
-      }
+      /* 8.2 ICE4_CORRECT_NEGATIVITIES (:821-823) */
+      correct_negativities(&ths, &rvs, &rcs, &rrs, &ris, &rss, &rgs, h ? &rhs : NULL, lvfact,
+                           lsfact);
+      if (h) h->rhs[j] = rhs;
       io[RO_THS][j] = ths; io[RO_RVS][j] = rvs; io[RO_RCS][j] = rcs; io[RO_RRS][j] = rrs;
       io[RO_RIS][j] = ris; io[RO_RSS][j] = rss; io[RO_RGS][j] = rgs; io[RO_CI][j] = cit;
       io[RO_EVAP3D][j] = evap3d; io[RO_RAINFR][j] = rainfr;

@@ -95,6 +95,32 @@ int oracle_rain_ice_ocnd2(int ncol, int nlev, double dt, const double *const in[
                           double *const io[ORACLE_RAIN_NIO], double *const col[ORACLE_RAIN_NCOL],
                           int *iters);
 
+/* Sedimentation scheme of every rain_ice call that follows (PARAM_ICE CSEDIM and
+ * XSPLIT_MAXCFL): ORACLE_SEDIM_STAT, the default, or ORACLE_SEDIM_SPLI with the CFL bound
+ * maxcfl (> 0) of its sub-steps.  Returns 0, or -1 for an unknown scheme or maxcfl <= 0. */
+enum { ORACLE_SEDIM_STAT, ORACLE_SEDIM_SPLI };
+int oracle_set_sedimentation(int scheme, double maxcfl);
+
+/* ICE4_SEDIMENTATION_SPLIT alone: rt[q] the mixing ratios at t and rs[q] the sources
+ * (updated in place) of rain, pristine ice, snow, graupel and, with krr = 7, hail
+ * (q = 0..4; rt[4], rs[4], inpr[4] unused with krr = 6), inpr[q] (ncol each) their surface
+ * rates; nsub (optional, 5 x ncol ints) the number of sub-steps of each species and column.
+ * Returns 0, -1 without constants, -2 for a bad krr or maxcfl, -3 when out of memory. */
+int oracle_sedim_split(int ncol, int nlev, double dt, int krr, double maxcfl, const double *dzz,
+                       const double *rhodref, const double *const rt[5], double *const rs[5],
+                       double *const inpr[5], int *nsub);
+
+/* ICE4_SEDIMENTATION with CSEDIM='SPLI' as RAIN_ICE calls it before the microphysics: the
+ * split sedimentation on the state at t, the ice rate added to the snow rate, then
+ * ICE4_CORRECT_NEGATIVITIES of every source.  t = {th, rv, rc, rr, ri, rs, rg, rh} at t,
+ * src their sources (updated in place), exn the Exner function; the rh entries are read
+ * with krr = 7 only.  inpr = {inprc, inprr, inpri, inprs, inprg, inprh} (ncol each; inprh
+ * with krr = 7).  nsub as for oracle_sedim_split.  Returns as oracle_sedim_split. */
+int oracle_sedimentation_spli(int ncol, int nlev, double dt, int krr, double maxcfl,
+                              const double *dzz, const double *rhodref, const double *exn,
+                              const double *const t[8], double *const src[8],
+                              double *const inpr[6], int *nsub);
+
 #ifdef __cplusplus
 }
 #endif

@@ -183,6 +183,81 @@ def rain_ice_hail(s: dict, dt: float = 50.0, with_iters: bool = False) -> dict:
     return out
 
 
+SEDIM = {"STAT": 0, "SPLI": 1}
+
+
+def set_sedimentation(scheme: str = "STAT", maxcfl: float = 0.8):
+    """Sedimentation scheme of the rain_ice calls that follow (PARAM_ICE CSEDIM, and
+    XSPLIT_MAXCFL for "SPLI"; the AROME default is "STAT")."""
+    lib = _load()
+    rc = lib.oracle_set_sedimentation(ctypes.c_int(SEDIM[scheme]), ctypes.c_double(maxcfl))
+    if rc != 0:
+        raise ValueError(f"oracle_set_sedimentation({scheme}, {maxcfl}) failed: {rc}")
+
+
+SPLIT_SPECIES = ("r", "i", "s", "g", "h")
+
+
+def sedim_split(f: dict, dt: float = 50.0, krr: int = 6, maxcfl: float = 0.8) -> dict:
+    """ICE4_SEDIMENTATION_SPLIT alone.  ``f`` holds ``dzz``, ``rhodref`` and, for each species
+    x of rain, pristine ice, snow, graupel (and hail with krr = 7), the mixing ratio at t
+    ``rxt`` and the source ``rxs`` (nlev, ncol).  Returns the updated sources ``rxs``, the
+    surface rates ``inprx`` (ncol) and the sub-step counts ``nsub`` (5, ncol)."""
+    lib = _load()
+    nlev, ncol = f["dzz"].shape
+    sp = SPLIT_SPECIES[:5 if krr == 7 else 4]
+    dummy = np.zeros((nlev, ncol))
+    rt = [_chk(f[f"r{x}t"]) if x in sp else dummy for x in SPLIT_SPECIES]
+    rs = [np.array(f[f"r{x}s"], dtype=np.float64, order="C") if x in sp else dummy.copy()
+          for x in SPLIT_SPECIES]
+    inpr = [np.zeros(ncol) for _ in SPLIT_SPECIES]
+    nsub = np.zeros((5, ncol), dtype=np.int32)
+    dzz, rho = _chk(f["dzz"]), _chk(f["rhodref"])
+    rc = lib.oracle_sedim_split(ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt),
+                                ctypes.c_int(krr), ctypes.c_double(maxcfl),
+                                ctypes.c_void_p(dzz.ctypes.data), ctypes.c_void_p(rho.ctypes.data),
+                                _arr(rt), _arr(rs), _arr(inpr), ctypes.c_void_p(nsub.ctypes.data))
+    if rc != 0:
+        raise RuntimeError(f"oracle_sedim_split failed: {rc}")
+    out = {f"r{x}s": rs[q] for q, x in enumerate(SPLIT_SPECIES) if x in sp}
+    out.update({f"inpr{x}": inpr[q] for q, x in enumerate(SPLIT_SPECIES) if x in sp})
+    out["nsub"] = nsub
+    return out
+
+
+SPLI_T = ("tht", "rvt", "rct", "rrt", "rit", "rst", "rgt", "rht")
+SPLI_S = ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "rhs")
+SPLI_PR = ("inprc", "inprr", "inpri", "inprs", "inprg", "inprh")
+
+
+def sedimentation_spli(s: dict, dt: float = 50.0, krr: int = 6, maxcfl: float = 0.8) -> dict:
+    """ICE4_SEDIMENTATION with CSEDIM='SPLI' as RAIN_ICE runs it before the microphysics: the
+    split sedimentation, then the negativity correction of every source.  ``s`` holds
+    ``dzz``, ``rhodref``, ``exn``, the state at t ``tht .. rgt`` (``rht`` with krr = 7) and
+    the sources ``ths .. rgs`` (``rhs``).  Returns the updated sources, the surface rates
+    ``inprc .. inprg`` (``inprh``) and the sub-step counts ``nsub`` (5, ncol)."""
+    lib = _load()
+    nlev, ncol = s["dzz"].shape
+    n = 8 if krr == 7 else 7
+    dummy = np.zeros((nlev, ncol))
+    t = [_chk(s[k]) for k in SPLI_T[:n]] + [dummy] * (8 - n)
+    src = [np.array(s[k], dtype=np.float64, order="C") for k in SPLI_S[:n]] + [dummy.copy()] * (8 - n)
+    pr = [np.zeros(ncol) for _ in SPLI_PR]
+    nsub = np.zeros((5, ncol), dtype=np.int32)
+    dzz, rho, exn = _chk(s["dzz"]), _chk(s["rhodref"]), _chk(s["exn"])
+    rc = lib.oracle_sedimentation_spli(ctypes.c_int(ncol), ctypes.c_int(nlev), ctypes.c_double(dt),
+                                       ctypes.c_int(krr), ctypes.c_double(maxcfl),
+                                       ctypes.c_void_p(dzz.ctypes.data), ctypes.c_void_p(rho.ctypes.data),
+                                       ctypes.c_void_p(exn.ctypes.data), _arr(t), _arr(src), _arr(pr),
+                                       ctypes.c_void_p(nsub.ctypes.data))
+    if rc != 0:
+        raise RuntimeError(f"oracle_sedimentation_spli failed: {rc}")
+    out = {k: src[q] for q, k in enumerate(SPLI_S[:n])}
+    out.update({k: pr[q] for q, k in enumerate(SPLI_PR[:5 if krr == 6 else 6])})
+    out["nsub"] = nsub
+    return out
+
+
 BRANCHES = ("rrhong", "rimltc", "riming", "raccsg", "smltg", "wetg", "dryg", "gmltr", "wetgh",
             "weth", "dryh", "dryhg", "hmltr", "tie")
 
