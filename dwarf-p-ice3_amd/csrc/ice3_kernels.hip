@@ -126,6 +126,191 @@ __global__ void __launch_bounds__(256) k_count_nonfinite(CountArgs a) {
   if ((threadIdx.x & 63) == 0) atomicAdd(&a.counts[q], n);
 }
 
+// ---- k_sedim_split: ICE4_SEDIMENTATION with CSEDIM='SPLI' (micro/mode_ice4_sedimentation.F90:
+// 196-219, micro/mode_ice4_sedimentation_split.F90), FP64.
+//
+// Each species of a column is integrated in sub-steps of one common length, the largest that
+// keeps every level's CFL number under XSPLIT_MAXCFL, until the time step is spent.  One
+// wavefront holds one column in registers, level k in lane k % 64, slot k / 64 (L slots):
+// a sub-step's fluxes are computed by all lanes at once, its length is a minimum over the
+// wavefront, and the flux entering level k from above comes from the next lane.  A workgroup
+// of split_cols wavefronts takes as many adjacent columns, so that the inputs are read in rows
+// of split_cols doubles, through LDS, and handed to the column's wavefront: 16 (a 128-byte row,
+// four wavefronts per SIMD, 128 registers each) while a wavefront's state fits 128 registers,
+// else 8 (256 registers).
+template <int L, bool HAIL> constexpr int split_cols() { return L == 1 || (L == 2 && !HAIL) ? 16 : 8; }
+constexpr int SPLIT_MAX_SUBSTEPS = 1 << 16;  // per species and column; the reference has no
+                                             // bound (its loop ends when the time is spent)
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = dmin(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// Stage field f (times scale) of the workgroup's columns through LDS; slot q of this lane
+// receives level lane + 64 q of the wavefront's column (0 outside the domain).  Every thread
+// of the workgroup calls it.
+template <int L, int COLS>
+__device__ __forceinline__ void split_stage(double (*tile)[COLS + 1], const SplitArgs &a,
+                                            const double *f, double scale, int c0, double (&v)[L]) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < L * 64 * COLS; e += 64 * COLS) {
+    const int k = e / COLS, c = e - k * COLS, col = c0 + c;
+    tile[k][c] = (k < a.nlev && col < a.ncol) ? f[(size_t)k * a.ld + col] * scale : 0.;
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < L; ++q) v[q] = tile[lane + 64 * q][w];
+}
+
+// Mass flux ZWSED of species JRR at mixing ratio r above its threshold; rf is the density
+// factor rho^(1-XCEXVT) (pristine ice) or rho^(XEXSED-XCEXVT) (the power-law species), fixed
+// over the sub-steps (micro/mode_ice4_sedimentation_split.F90:574-578,630-633,747-749).
+template <int JRR>
+__device__ __forceinline__ double split_flux(double r, double rho, double rf) {
+  if constexpr (JRR == 4) {
+    return P.XFSEDI * r * rf *
+           rpow<double>(dmax(0.05E6, -0.15319E6 - 0.021454E6 * rlog<double>(rho * r)), P.XEXCSEDI);
+  } else {
+    constexpr double fsed = JRR == 3 ? P.XFSEDR : (JRR == 5 ? P.XFSEDS : (JRR == 6 ? P.XFSEDG : P.XFSEDH));
+    constexpr double exsed = JRR == 3 ? P.XEXSEDR : (JRR == 5 ? P.XEXSEDS : (JRR == 6 ? P.XEXSEDG : P.XEXSEDH));
+    return fsed * rpow<double>(r, exsed) * rf;
+  }
+}
+
+// INTERNAL_SEDIM_SPLI (micro/mode_ice4_sedimentation_split.F90:359-849) of species JRR for the
+// wavefront's column: r (the state at t on entry, the working state after), s (the source,
+// updated); returns the surface rate PINPRX.  Sets *capped when the sub-step cap is reached.
+template <int L, int JRR>
+__device__ __forceinline__ double split_species(const SplitArgs &a, int lane, const double (&rho)[L],
+                                                const double (&dz)[L], const double (&oorhodz)[L],
+                                                double (&r)[L], double (&s)[L], bool *capped) {
+  constexpr double thr = JRR == 4 ? (P.XRTMIN[4] > 1.0E-7 ? P.XRTMIN[4] : 1.0E-7) : P.XRTMIN[JRR];
+  const double dt = a.dt, zinv = 1. / dt;
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < L; ++q) any = any || (lane + 64 * q < a.nlev && r[q] > thr);
+  // nothing above the fall threshold: one sub-step of dt changes no source (:786-826)
+  if (!__any(any)) return 0.;
+  double prs[L], rf[L];
+#pragma unroll
+  for (int q = 0; q < L; ++q) {
+    prs[q] = s[q] - r[q] * zinv;  // external tendency (:241-253)
+    rf[q] = lane + 64 * q < a.nlev
+                ? rpow<double>(rho[q], JRR == 4 ? 1. - P.XCEXVT
+                                                : (JRR == 3 ? P.XEXSEDR : JRR == 5 ? P.XEXSEDS
+                                                   : JRR == 6 ? P.XEXSEDG : P.XEXSEDH) - P.XCEXVT)
+                : 0.;
+  }
+  double remaint = dt, inpr = 0.;
+  for (int n = 0; remaint > 0.; ++n) {  // wave-uniform: remaint is the column's
+    if (n == SPLIT_MAX_SUBSTEPS) {
+      *capped = true;
+      break;
+    }
+    double w[L], m = INFINITY;
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+      const bool in = lane + 64 * q < a.nlev;
+      w[q] = in && r[q] > thr ? split_flux<JRR>(r[q], rho[q], rf[q]) : 0.;
+      if (in && r[q] > P.XRTMIN[JRR] && w[q] > 1.E-20)
+        m = dmin(m, a.maxcfl * rho[q] * r[q] * dz[q] / w[q]);  // :795-808
+    }
+    const double step = dmin(remaint, wave_min(m));
+    remaint = remaint - step;  // :814-815
+    inpr = inpr + __shfl(w[0], 0, 64) / P.XRHOLW * (step * zinv);
+#pragma unroll
+    for (int q = 0; q < L; ++q) {  // :820-824; level k + 1 is the next lane (or slot)
+      const double next = q + 1 < L ? __shfl(w[q + 1 < L ? q + 1 : q], 0, 64) : 0.;
+      double up = __shfl_down(w[q], 1, 64);
+      if (lane == 63) up = next;
+      const double mrchange = step * oorhodz[q] * (up - w[q]);
+      r[q] = r[q] + mrchange + prs[q] * step;
+      s[q] = s[q] + mrchange * zinv;
+    }
+  }
+  return inpr;
+}
+
+template <int L, bool HAIL, bool ZZZ>
+__global__ void __launch_bounds__((64 * split_cols<L, HAIL>())) k_sedim_split(SplitArgs a) {
+  constexpr int COLS = split_cols<L, HAIL>();
+  __shared__ double tile[L * 64][COLS + 1];
+  constexpr int NV = HAIL ? 8 : 7;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c0 = blockIdx.x * COLS, i = c0 + w;
+  // RAIN_ICE's L/cph factors of the state at t (rain_ice.F90:335-345)
+  double lv[L], ls[L], r[NV - 3][L];
+  {
+    double t[NV][L], exn[L];
+    split_stage<L, COLS>(tile, a, a.exn, 1., c0, exn);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) split_stage<L, COLS>(tile, a, a.t[v], a.tscale, c0, t[v]);
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+      double tq[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) tq[v] = t[v][q];
+      rain_ice_factors<HAIL>(tq, exn[q], ls[q], lv[q]);
+    }
+#pragma unroll
+    for (int v = 3; v < NV; ++v)
+#pragma unroll
+      for (int q = 0; q < L; ++q) r[v - 3][q] = t[v][q];
+  }
+  double rho[L], dz[L], oorhodz[L], src[NV][L];
+  split_stage<L, COLS>(tile, a, a.rhodref, 1., c0, rho);
+  split_stage<L, COLS>(tile, a, a.dzz, 1., c0, dz);
+  if constexpr (ZZZ) {  // RAIN_ICE's model level height (rain_ice.F90:387-395), in its order
+    if (lane == 0 && i < a.ncol) {
+      double zzsum = 0.;
+      for (int k = a.nlev - 1; k >= 0; --k) {
+        zzsum = zzsum + tile[k][w];
+        a.zzz[(size_t)k * a.ld + i] = zzsum - tile[k][w] * 0.5;
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) split_stage<L, COLS>(tile, a, a.src[v], 1., c0, src[v]);
+#pragma unroll
+  for (int q = 0; q < L; ++q)
+    oorhodz[q] = lane + 64 * q < a.nlev ? 1. / (rho[q] * dz[q]) : 0.;  // ZW (:263)
+  if (i >= a.ncol) return;  // after the last barrier
+  bool capped = false;
+  const double pr = split_species<L, 3>(a, lane, rho, dz, oorhodz, r[0], src[IRR], &capped);
+  const double pi = split_species<L, 4>(a, lane, rho, dz, oorhodz, r[1], src[IRI], &capped);
+  const double ps = split_species<L, 5>(a, lane, rho, dz, oorhodz, r[2], src[IRS], &capped);
+  const double pg = split_species<L, 6>(a, lane, rho, dz, oorhodz, r[3], src[IRG], &capped);
+  double ph = 0.;
+  if constexpr (HAIL) ph = split_species<L, 7>(a, lane, rho, dz, oorhodz, r[4], src[IRH], &capped);
+  if (lane == 0) {
+    a.inprc[i] = 0.;
+    a.inprr[i] = pr;
+    a.inpri[i] = pi;
+    a.inprs[i] = ps + pi;  // mode_ice4_sedimentation.F90:212
+    a.inprg[i] = pg;
+    if constexpr (HAIL) a.inprh[i] = ph;
+    if (capped && a.capped) atomicAdd(a.capped, 1ull);
+  }
+#pragma unroll
+  for (int q = 0; q < L; ++q) {  // ICE4_CORRECT_NEGATIVITIES (:216-218), every point
+    const int k = lane + 64 * q;
+    if (k >= a.nlev) continue;
+    double x[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) x[v] = src[v][q];
+    double rh = HAIL ? x[NV - 1] : 0.;
+    correct_negativities<HAIL>(x[ITH], x[IRV], x[IRC], x[IRR], x[IRI], x[IRS], x[IRG], lv[q],
+                               ls[q], &rh);
+    if constexpr (HAIL) x[NV - 1] = rh;
+    const size_t j = (size_t)k * a.ld + i;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a.dst[v][j] = x[v];
+  }
+}
+
 // ---- k_micro: the ICE4 microphysics, persistent, with the time splitting compacted
 // across each wavefront.
 //
@@ -548,6 +733,10 @@ struct ice3_ctx {
   unsigned long long *d_capped;  // points that hit the sub-step cap since the last status query
   void *d_scratch;    // device address of this device's copy of scratch
   int max_calls;      // sub-step cap per point
+  int sedim;          // ICE3_SEDIM_STAT or ICE3_SEDIM_SPLI
+  double maxcfl;      // XSPLIT_MAXCFL of the split sedimentation
+  void *d_work;       // work area of the fused step with the split sedimentation
+  size_t work_bytes;
   int timing;         // record ev[] around the stages of each call
   hipEvent_t ev[7];   // before adjust, after adjust, after sedimentation, then after the
                       // microphysics (plain step: events 3..6 together) or, for the surrogate
@@ -600,6 +789,8 @@ extern "C" int ice3_ctx_create(ice3_ctx **out) {
   }
   free(tiwmx);
   c->max_calls = DEFAULT_MAX_CALLS;
+  c->sedim = ICE3_SEDIM_STAT;
+  c->maxcfl = 0.8;  // PARAM_ICE XSPLIT_MAXCFL default (micro/modd_param_icen.F90:347)
   *out = c;
   return ICE3_OK;
 }
@@ -609,6 +800,7 @@ extern "C" void ice3_ctx_destroy(ice3_ctx *c) {
   if (c->d_tab) (void)hipFree(c->d_tab);
   if (c->d_tiwmx) (void)hipFree(c->d_tiwmx);
   if (c->d_capped) (void)hipFree(c->d_capped);
+  if (c->d_work) (void)hipFree(c->d_work);
   if (c->timing)
     for (int q = 0; q < 7; ++q) (void)hipEventDestroy(c->ev[q]);
   free(c);
@@ -636,6 +828,15 @@ extern "C" int ice3_stage_times(ice3_ctx *c, float ms[6]) {
 extern "C" int ice3_ctx_set_substep_cap(ice3_ctx *c, int max_calls) {
   if (!c || max_calls < 1) return ICE3_EINVAL;
   c->max_calls = max_calls;
+  return ICE3_OK;
+}
+
+extern "C" int ice3_ctx_set_sedimentation(ice3_ctx *c, int scheme, double max_cfl) {
+  if (!c || (scheme != ICE3_SEDIM_STAT && scheme != ICE3_SEDIM_SPLI) || !(max_cfl > 0.) ||
+      !(max_cfl < INFINITY))
+    return ICE3_EINVAL;
+  c->sedim = scheme;
+  c->maxcfl = max_cfl;
   return ICE3_OK;
 }
 
@@ -871,12 +1072,73 @@ static hipError_t launch_micro_ocnd2(const ice3_ctx *ctx, const MicroArgs &m, hi
   return hipGetLastError();
 }
 
+// Levels a wavefront of k_sedim_split holds (4 slots per lane).
+constexpr int SPLIT_MAX_LEVELS = 4 * 64;
+
+// Whether the context's sedimentation scheme can run nlev levels.
+static bool sedim_fits(const ice3_ctx *ctx, int nlev) {
+  return ctx->sedim != ICE3_SEDIM_SPLI || nlev <= SPLIT_MAX_LEVELS;
+}
+
+template <int L, bool HAIL, bool ZZZ>
+static void launch_split_k(const SplitArgs &a, hipStream_t s) {
+  constexpr int C = split_cols<L, HAIL>();
+  hipLaunchKernelGGL((k_sedim_split<L, HAIL, ZZZ>), dim3((a.ncol + C - 1) / C), dim3(64 * C), 0,
+                     s, a);
+}
+
+template <bool HAIL, bool ZZZ>
+static void launch_split_l(const SplitArgs &a, hipStream_t s) {
+  switch ((a.nlev + 63) / 64) {
+    case 1: launch_split_k<1, HAIL, ZZZ>(a, s); break;
+    case 2: launch_split_k<2, HAIL, ZZZ>(a, s); break;
+    case 3: launch_split_k<3, HAIL, ZZZ>(a, s); break;
+    default: launch_split_k<4, HAIL, ZZZ>(a, s); break;
+  }
+}
+
+// The split sedimentation (CSEDIM='SPLI') of the stages described by sa (sedimentation) and m
+// (the state at t and the th, rv, rc sources); sa.zzz selects the OCND2 model level height,
+// sa.src_rh hail.  nlev <= SPLIT_MAX_LEVELS (checked by the entry points).
+static hipError_t launch_split(const ice3_ctx *ctx, const SedimArgs &sa, const MicroArgs &m,
+                               hipStream_t s) {
+  const SplitArgs a = split_args(sa, m, ctx->maxcfl, ctx->d_capped);
+  if (sa.src_rh) launch_split_l<true, false>(a, s);
+  else if (sa.zzz) launch_split_l<false, true>(a, s);
+  else launch_split_l<false, false>(a, s);
+  return hipGetLastError();
+}
+
+// The fused step with the split sedimentation: its negativity correction updates the th, rv
+// and rc sources, whose values before it, times dt, are the microphysics' state at t.  They
+// are copied to the context's work area (3 fields, grown on demand; growing it waits for the
+// device), and m reads its state at t there.
+static int keep_tstate(ice3_ctx *ctx, int ncol, int nlev, int ld, MicroArgs &m, hipStream_t s) {
+  const size_t n = (size_t)nlev * ld, used = (size_t)(nlev - 1) * ld + ncol;
+  const size_t bytes = 3 * n * sizeof(double);
+  if (bytes > ctx->work_bytes) {
+    HIPCHK(hipDeviceSynchronize());
+    if (ctx->d_work) HIPCHK(hipFree(ctx->d_work));
+    ctx->d_work = nullptr;
+    ctx->work_bytes = 0;
+    HIPCHK(hipMalloc(&ctx->d_work, bytes));
+    ctx->work_bytes = bytes;
+  }
+  double *w = (double *)ctx->d_work;
+  for (int q = 0; q < 3; ++q) {
+    HIPCHK(hipMemcpyAsync(w + q * n, m.src[q], used * sizeof(double), hipMemcpyDeviceToDevice, s));
+    m.t[q] = w + q * n;
+  }
+  return ICE3_OK;
+}
+
 // Launch the sedimentation then the microphysics stage on one stream.
 static int launch_rain_ice(ice3_ctx *ctx, int prec, const SedimArgs &sa, const MicroArgs &ma,
                            void *stream) {
   hipStream_t s = (hipStream_t)stream;
   const dim3 gs((sa.ncol + 255) / 256);
-  if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
+  if (ctx->sedim == ICE3_SEDIM_SPLI) HIPCHK(launch_split(ctx, sa, ma, s));
+  else if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
   else hipLaunchKernelGGL(k_sedim<float>, gs, dim3(256), 0, s, sa);
   HIPCHK(hipGetLastError());
   MARK(2);
@@ -899,6 +1161,7 @@ extern "C" int ice3_rain_ice(ice3_ctx *ctx, int prec, int ncol, int nlev, int ld
   NN(io->ci);
   NN(out->evap3d); NN(out->rainfr); NN(out->inprc); NN(out->inprr); NN(out->inpri);
   NN(out->inprs); NN(out->inprg);
+  if (!sedim_fits(ctx, nlev)) return ICE3_EINVAL;
   SedimArgs sa;
   MicroArgs m;
   rain_ice_args(ncol, nlev, ld, dt, ctx->d_tab, in, io, out, stats, sa, m);
@@ -929,9 +1192,13 @@ extern "C" int ice3_rain_ice_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int ld, do
   m.icldfr = in2->icldfr; m.ssio = in2->ssio; m.ssiu = in2->ssiu; m.ifr = in2->ifr;
   m.tiwmx = ctx->d_tiwmx;
   sa.zzz = out->rainfr;  // the model level height, read by the microphysics, then overwritten
+  if (!sedim_fits(ctx, nlev)) return ICE3_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   MARK(0); MARK(1);
-  hipLaunchKernelGGL((k_sedim<double, true>), dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
+  if (ctx->sedim == ICE3_SEDIM_SPLI)
+    HIPCHK(launch_split(ctx, sa, m, s));
+  else
+    hipLaunchKernelGGL((k_sedim<double, true>), dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
   HIPCHK(hipGetLastError());
   MARK(2);
   HIPCHK(launch_micro_ocnd2(ctx, m, s));
@@ -963,6 +1230,11 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
   if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
   if (bad_shape(ncol, nlev, ld, dt) || nlev > 65535) return ICE3_EINVAL;
   if (bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
+  if ((stages & ICE3_STAGE_SEDIMENTATION) && !sedim_fits(ctx, nlev)) return ICE3_EINVAL;
+  // with SPLI the microphysics' state at t is kept by the sedimentation stage of its call
+  if (ctx->sedim == ICE3_SEDIM_SPLI && (stages & ICE3_STAGE_MICROPHYSICS) &&
+      !(stages & ICE3_STAGE_SEDIMENTATION))
+    return ICE3_EINVAL;
   AdjustArgs aa;
   SedimArgs sa;
   MicroArgs m;
@@ -970,8 +1242,9 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
   m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
   hipStream_t s = (hipStream_t)stream;
   const dim3 gs((ncol + 255) / 256), gp((ncol + 255) / 256, nlev);
+  const bool split = ctx->sedim == ICE3_SEDIM_SPLI;
   MARK(0);
-  if (prec == ICE3_PREC_FP64 && (stages & ICE3_STAGE_ADJUST) &&
+  if (prec == ICE3_PREC_FP64 && !split && (stages & ICE3_STAGE_ADJUST) &&
       (stages & ICE3_STAGE_SEDIMENTATION)) {
     // one column walk for both stages (in the mixed path the float sedimentation is cheap
     // and the two kernels are faster apart)
@@ -984,7 +1257,12 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
   }
   MARK(1);
   if (stages & ICE3_STAGE_SEDIMENTATION) {
-    if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
+    if (split && (stages & ICE3_STAGE_MICROPHYSICS)) {
+      const int rc = keep_tstate(ctx, ncol, nlev, ld, m, s);
+      if (rc != ICE3_OK) return rc;
+    }
+    if (split) HIPCHK(launch_split(ctx, sa, m, s));
+    else if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
     else hipLaunchKernelGGL(k_sedim<float>, gs, dim3(256), 0, s, sa);
     HIPCHK(hipGetLastError());
   }
@@ -1025,13 +1303,20 @@ extern "C" int ice3_adjust_rain_ice_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int
   m.icldfr = out2->icldfr; m.ssio = out2->ssio; m.ssiu = out2->ssiu; m.ifr = out2->ifr;
   m.tiwmx = ctx->d_tiwmx;
   sa.zzz = out->rainfr;  // the model level height, read by the microphysics, then overwritten
+  if (!sedim_fits(ctx, nlev)) return ICE3_EINVAL;
   const Ocnd2Args x{*in2, *out2, ctx->d_tiwmx};
   hipStream_t s = (hipStream_t)stream;
   MARK(0);
   hipLaunchKernelGGL(k_ice_adjust_ocnd2, dim3((ncol + 255) / 256, nlev), dim3(256), 0, s, aa, x);
   HIPCHK(hipGetLastError());
   MARK(1);
-  hipLaunchKernelGGL((k_sedim<double, true>), dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
+  if (ctx->sedim == ICE3_SEDIM_SPLI) {
+    const int rc = keep_tstate(ctx, ncol, nlev, ld, m, s);
+    if (rc != ICE3_OK) return rc;
+    HIPCHK(launch_split(ctx, sa, m, s));
+  } else {
+    hipLaunchKernelGGL((k_sedim<double, true>), dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
+  }
   HIPCHK(hipGetLastError());
   MARK(2);
   HIPCHK(launch_micro_ocnd2(ctx, m, s));
@@ -1093,9 +1378,13 @@ extern "C" int ice3_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, dou
   m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
   sa.src_rh = hail->rhs; sa.dst_rh = hail->rhs; sa.inprh = hail->inprh;
   m.t[IRH] = hail->rht; m.src[IRH] = hail->rhs; m.dst[IRH] = hail->rhs;
+  if (!sedim_fits(ctx, nlev)) return ICE3_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   MARK(0); MARK(1);
-  hipLaunchKernelGGL(k_sedim_hail, dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
+  if (ctx->sedim == ICE3_SEDIM_SPLI)
+    HIPCHK(launch_split(ctx, sa, m, s));
+  else
+    hipLaunchKernelGGL(k_sedim_hail, dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
   HIPCHK(hipGetLastError());
   MARK(2);
   HIPCHK(launch_micro_hail(m, s));
@@ -1121,11 +1410,22 @@ extern "C" int ice3_adjust_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int 
   aa.rh = hail->rh;
   sa.src_rh = hail->rhs; sa.dst_rh = hail->rhs_out; sa.inprh = hail->inprh;
   m.t[IRH] = hail->rhs; m.src[IRH] = hail->rhs_out; m.dst[IRH] = hail->rhs_out;
+  if (!sedim_fits(ctx, nlev)) return ICE3_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   MARK(0);
-  hipLaunchKernelGGL(k_adjust_sedim_hail, dim3((ncol + 63) / 64), dim3(64), 0, s, aa, sa);
-  HIPCHK(hipGetLastError());
-  MARK(1); MARK(2);
+  if (ctx->sedim == ICE3_SEDIM_SPLI) {
+    hipLaunchKernelGGL(k_ice_adjust_hail, dim3((ncol + 255) / 256, nlev), dim3(256), 0, s, aa);
+    HIPCHK(hipGetLastError());
+    MARK(1);
+    const int rc = keep_tstate(ctx, ncol, nlev, ld, m, s);
+    if (rc != ICE3_OK) return rc;
+    HIPCHK(launch_split(ctx, sa, m, s));
+  } else {
+    hipLaunchKernelGGL(k_adjust_sedim_hail, dim3((ncol + 63) / 64), dim3(64), 0, s, aa, sa);
+    HIPCHK(hipGetLastError());
+    MARK(1);
+  }
+  MARK(2);
   HIPCHK(launch_micro_hail(m, s));
   MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;
@@ -1162,6 +1462,7 @@ static int surrogate_check(const ice3_ctx *ctx, int ncol, int nlev, int ld, doub
   if (!sg->weights_packed || !sg->params || !sg->norm || !(sg->budget_tol >= 0.f) ||
       !(sg->neg_tol >= 0.f))
     return ICE3_EINVAL;
+  if (ctx->sedim != ICE3_SEDIM_STAT) return ICE3_EINVAL;  // trained on the statistical scheme
   if (bad_pipeline_ptrs(in, out)) return ICE3_EINVAL;
   return ICE3_OK;
 }

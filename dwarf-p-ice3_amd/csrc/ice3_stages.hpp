@@ -100,6 +100,21 @@ struct SedimArgs {
   double *dst_rh, *inprh;
 };
 
+// ICE4_SEDIMENTATION with CSEDIM='SPLI' (micro/mode_ice4_sedimentation.F90:196-219): the
+// split sedimentation of rain, pristine ice, snow, graupel (and hail) on the state at t, then
+// ICE4_CORRECT_NEGATIVITIES of every source.  FP64 whatever the precision of the call.
+struct SplitArgs {
+  int ncol, nlev, ld;
+  double dt, tscale, maxcfl;
+  const double *rhodref, *dzz, *exn;
+  const double *t[8];    // state at t (times tscale): th, rv, rc, rr, ri, rs, rg, rh
+  const double *src[8];  // sources read
+  double *dst[8];        // sources written (may alias src)
+  double *inprc, *inprr, *inpri, *inprs, *inprg, *inprh;
+  double *zzz;           // OCND2 branch: receives the model level height ZZZZ
+  unsigned long long *capped;  // columns whose species hit the sub-step cap
+};
+
 // ICE4_SEDIMENTATION_STAT (micro/mode_ice4_sedimentation_stat.F90:143-229) for column i,
 // followed by the surface precipitation of RAIN_ICE (PINPRS += PINPRI, PINPRC = 0 with
 // LSEDIC=F).  With ZZZ the walk also writes RAIN_ICE's model level height ZZZZ into a.zzz:
@@ -400,6 +415,27 @@ __device__ __forceinline__ void latent_factors(const MicroArgs &a, size_t j, dou
 
 // rain_ice: sedimentation updates the precipitating sources in place, then the
 // microphysics reads the handed-over t-state and updates all sources in place.
+// The split sedimentation's arguments for the stages whose sedimentation is described by sa
+// and whose microphysics (the state at t, th/rv/rc sources) by m.
+inline SplitArgs split_args(const SedimArgs &sa, const MicroArgs &m, double maxcfl,
+                            unsigned long long *capped) {
+  SplitArgs a = SplitArgs{};
+  a.ncol = sa.ncol; a.nlev = sa.nlev; a.ld = sa.ld; a.dt = sa.dt; a.tscale = m.tscale;
+  a.maxcfl = maxcfl;
+  a.rhodref = sa.rhodref; a.dzz = sa.dzz; a.exn = m.exn;
+  for (int q = 0; q < 8; ++q) a.t[q] = m.t[q];
+  for (int q = 0; q < 3; ++q) { a.src[q] = m.src[q]; a.dst[q] = m.dst[q]; }
+  a.src[3] = sa.src_rr; a.src[4] = sa.src_ri; a.src[5] = sa.src_rs; a.src[6] = sa.src_rg;
+  a.src[7] = sa.src_rh;
+  a.dst[3] = sa.dst_rr; a.dst[4] = sa.dst_ri; a.dst[5] = sa.dst_rs; a.dst[6] = sa.dst_rg;
+  a.dst[7] = sa.dst_rh;
+  a.inprc = sa.inprc; a.inprr = sa.inprr; a.inpri = sa.inpri; a.inprs = sa.inprs;
+  a.inprg = sa.inprg; a.inprh = sa.inprh;
+  a.zzz = sa.zzz;
+  a.capped = capped;
+  return a;
+}
+
 inline void rain_ice_args(int ncol, int nlev, int ld, double dt, const double *tab,
                           const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
                           const ice3_rain_ice_out *out, const ice3_stats *stats,

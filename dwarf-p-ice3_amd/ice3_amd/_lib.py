@@ -60,6 +60,7 @@ Hail = _struct("ice3_hail", HAIL_FIELDS)
 PipeHail = _struct("ice3_pipeline_hail", PIPE_HAIL_FIELDS)
 
 STAGE_ADJUST, STAGE_SEDIMENTATION, STAGE_MICROPHYSICS = 1, 2, 4
+SEDIM = {"STAT": 0, "SPLI": 1}   # ICE3_SEDIM_STAT, ICE3_SEDIM_SPLI
 NFEAT, NTARG = 18, 8
 
 
@@ -78,7 +79,7 @@ class Surrogate(ctypes.Structure):
 
 
 EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_constants",
-           "ice3_ctx_set_substep_cap", "ice3_ctx_status",
+           "ice3_ctx_set_substep_cap", "ice3_ctx_status", "ice3_ctx_set_sedimentation",
            "ice3_ice_adjust", "ice3_ice_adjust_ocnd2", "ice3_rain_ice", "ice3_adjust_rain_ice",
            "ice3_rain_ice_ocnd2", "ice3_adjust_rain_ice_ocnd2",
            "ice3_ice_adjust_hail", "ice3_rain_ice_hail", "ice3_adjust_rain_ice_hail",
@@ -109,6 +110,7 @@ def load():
         lib.ice3_strerror.restype = ctypes.c_char_p
         lib.ice3_ctx_set_substep_cap.argtypes = [_P, ctypes.c_int]
         lib.ice3_ctx_status.argtypes = [_P, _P, ctypes.POINTER(ctypes.c_ulonglong)]
+        lib.ice3_ctx_set_sedimentation.argtypes = [_P, ctypes.c_int, ctypes.c_double]
         lib.ice3_dump_constants.argtypes = [_P, ctypes.c_char_p, ctypes.c_long]
         lib.ice3_dump_constants.restype = ctypes.c_long
         common = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]

@@ -36,12 +36,24 @@ _OUT_ORDER = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", 
 _PINNED = {
     "nebn": {"LSUBG_COND": 1, "LSIGMAS": 1, "LSTATNW": 0, "LHGT_QS": 0, "CONDENS": 0,
              "LAMBDA3": 0, "FRAC_ICE_ADJUST": 0},
-    "param_icen": {"LCOND2": 0, "SUBG_MF_PDF": 1, "SEDIM": 1, "LRED": 0, "MRSTEP": 0.0,
+    "param_icen": {"LCOND2": 0, "SUBG_MF_PDF": 1, "LRED": 0, "MRSTEP": 0.0,
                    "TSTEP_TS": 0.0, "LWARM": 1, "LFEEDBACKT": 1, "LEVLIMIT": 1, "LNULLWETG": 1,
                    "LWETGPOST": 1, "LCRFLIMIT": 1, "LSNOW_T": 0, "LSEDIM_AFTER": 0, "LDEPOSC": 0,
                    "SNOW_RIMING": 0, "SUBG_RC_RR_ACCR": 0, "SUBG_RR_EVAP": 0, "SUBG_PR_PDF": 0,
                    "SUBG_AUCV_RC": 1, "SUBG_AUCV_RI": 0, "PRISTINE_ICE": "PLAT"},
 }
+
+
+def sedimentation_of(cfg) -> tuple[str, float]:
+    """The sedimentation scheme a configuration object selects: its ``param_icen.SEDIM``
+    (the reference's Sedim enum, src/ice3/phyex_common/ice_parameters.py:62-64: 0 = SPLI,
+    1 = STAT) and ``param_icen.SPLIT_MAXCFL``; ("STAT", 0.8) without them (AROME)."""
+    p = getattr(cfg, "param_icen", None)
+    sedim = getattr(p, "SEDIM", 1)
+    sedim = getattr(sedim, "value", sedim)
+    if sedim not in (0, 1):
+        raise ValueError(f"param_icen.SEDIM = {sedim!r}: 0 (SPLI) or 1 (STAT)")
+    return ("SPLI" if sedim == 0 else "STAT"), float(getattr(p, "SPLIT_MAXCFL", 0.8))
 
 
 def check_configuration(cfg, what: str = "phyex", ocnd2: bool = False):
@@ -54,6 +66,7 @@ def check_configuration(cfg, what: str = "phyex", ocnd2: bool = False):
     prog = getattr(cfg, "program", cfg)
     if not (prog is None or (isinstance(prog, str) and prog.upper() == "AROME")):
         raise ValueError(f"{what}: only the pinned AROME ICE3 configuration is implemented, got {prog!r}")
+    sedimentation_of(cfg)  # SEDIM: either scheme (rain_ice reads it)
     for group, fields in _PINNED.items():
         obj = getattr(cfg, group, None)
         if obj is None:

@@ -36,26 +36,51 @@ default.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 
 import torch
 
 from . import _lib
 from ._fields import as_columns, check_layout, from_columns, ptr, resolve_layout, stream_ptr
-from .ice_adjust import check_configuration
+from .ice_adjust import check_configuration, sedimentation_of
 
 _PREC = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}
 
 
 class _Base:
-    def __init__(self, constants=None, layout: str = "auto", device=None):
+    """sedimentation: "STAT" (ICE4_SEDIMENTATION_STAT, the AROME configuration) or "SPLI"
+    (ICE4_SEDIMENTATION_SPLIT with CFL bound split_max_cfl, then the negativity correction);
+    by default what the configuration object selects (param_icen.SEDIM and SPLIT_MAXCFL),
+    else "STAT"."""
+
+    def __init__(self, constants=None, layout: str = "auto", device=None,
+                 sedimentation: str | None = None, split_max_cfl: float | None = None):
+        sedim, cfl = "STAT", 0.8
         if not isinstance(constants, dict):  # a constants dict is accepted and not read
             check_configuration(constants, "constants")
+            sedim, cfl = sedimentation_of(constants)
+        self.sedimentation = (sedimentation or sedim).upper()
+        self.split_max_cfl = float(cfl if split_max_cfl is None else split_max_cfl)
+        if self.sedimentation not in _lib.SEDIM or not self.split_max_cfl > 0.0:
+            raise ValueError(f"sedimentation {sedimentation!r} / split_max_cfl {split_max_cfl!r}: "
+                             "\"STAT\" or \"SPLI\" with a CFL bound > 0")
         self.layout = check_layout(layout)
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         self._lib = _lib.load()
         self._ctx = _lib.context(self.device.index)
         self.last_counters = None
+
+    @contextlib.contextmanager
+    def _scheme(self):
+        """The context's sedimentation scheme set to this component's for its launches (the
+        context is shared by the components of a device), then back to STAT."""
+        _lib.check(self._lib.ice3_ctx_set_sedimentation(self._ctx, _lib.SEDIM[self.sedimentation],
+                                                        self.split_max_cfl), "ice3_ctx_set_sedimentation")
+        try:
+            yield
+        finally:
+            self._lib.ice3_ctx_set_sedimentation(self._ctx, _lib.SEDIM["STAT"], 0.8)
 
     def _col(self, x, name, layout):
         return as_columns(x, layout, device=self.device, name=name)
@@ -129,7 +154,7 @@ class RainIce(_Base):
         cin = _lib.RainIn(*[ptr(inp[k]) for k in _lib.RAIN_IN_FIELDS])
         cio = _lib.RainIO(*[ptr(io[k]) for k in _lib.RAIN_IO_FIELDS])
         cout = _lib.RainOut(*[ptr(out[k]) for k in _lib.RAIN_OUT_FIELDS])
-        with torch.cuda.device(self.device):
+        with torch.cuda.device(self.device), self._scheme():
             rc = self._launch(precision, ncol, nlev, dt, cin, extra, cio, cout, st)
         _lib.check(rc, self._ENTRY)
         self._finish_counters(counters)
@@ -186,7 +211,7 @@ class AdjustRainIce(_Base):
         counters, calls, st = self._stats(nlev * ncol, return_calls)
         cin = _lib.PipeIn(*[ptr(g[k]) for k in _lib.PIPE_IN_FIELDS[:-1]], ptr(sq))
         cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
-        with torch.cuda.device(self.device):
+        with torch.cuda.device(self.device), self._scheme():
             if ocnd2:
                 x = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=self.device)
                      for k in _lib.OCND2_OUT_FIELDS}
@@ -237,10 +262,14 @@ class RainIceOCND2(RainIce):
 
     _ENTRY = "ice3_rain_ice_ocnd2"
 
-    def __init__(self, constants=None, layout: str = "auto", device=None):
+    def __init__(self, constants=None, layout: str = "auto", device=None,
+                 sedimentation: str | None = None, split_max_cfl: float | None = None):
+        sedim, cfl = None, None
         if not isinstance(constants, dict):
             check_configuration(constants, "constants", ocnd2=True)
-        super().__init__({}, layout, device)
+            sedim, cfl = sedimentation_of(constants)
+        super().__init__({}, layout, device, sedimentation or sedim,
+                         cfl if split_max_cfl is None else split_max_cfl)
 
     def _extra_inputs(self, state, layout, nlev, ncol):
         x = {k: self._col(state[k], k, layout) for k in _lib.RAIN_OCND2_IN_FIELDS}
@@ -264,10 +293,14 @@ class AdjustRainIceOCND2(AdjustRainIce):
     adjusted state and ice cloud.  Inputs are AdjustRainIce's plus ``zz`` (level heights, m)
     and ``ice_cld_wgt`` (per column); the result adds ``icldfr, wcldfr, ssio, ssiu, ifr``."""
 
-    def __init__(self, constants=None, layout: str = "auto", device=None):
+    def __init__(self, constants=None, layout: str = "auto", device=None,
+                 sedimentation: str | None = None, split_max_cfl: float | None = None):
+        sedim, cfl = None, None
         if not isinstance(constants, dict):
             check_configuration(constants, "constants", ocnd2=True)
-        super().__init__({}, layout, device)
+            sedim, cfl = sedimentation_of(constants)
+        super().__init__({}, layout, device, sedimentation or sedim,
+                         cfl if split_max_cfl is None else split_max_cfl)
 
     def __call__(self, f: dict, dt: float, return_calls: bool = False, check_inputs: bool = False):
         return self._fused(f, dt, "fp64", return_calls, check_inputs, ocnd2=True)

@@ -69,6 +69,28 @@ const char *ice3_strerror(int code);
 int ice3_ctx_set_substep_cap(ice3_ctx *ctx, int max_calls);
 int ice3_ctx_status(ice3_ctx *ctx, void *stream, unsigned long long *capped);
 
+/* ---- sedimentation scheme: PARAM_ICE CSEDIM and XSPLIT_MAXCFL (micro/modd_param_icen.F90:83,
+ * 347; the dispatch of micro/mode_ice4_sedimentation.F90:167-223) ----
+ * ICE3_SEDIM_STAT (the default, the AROME configuration): ICE4_SEDIMENTATION_STAT.
+ * ICE3_SEDIM_SPLI: ICE4_SEDIMENTATION_SPLIT (micro/mode_ice4_sedimentation_split.F90) on the
+ * state at t, in sub-steps whose CFL number stays under max_cfl (> 0; the reference's
+ * default is 0.8), followed by ICE4_CORRECT_NEGATIVITIES of every source.  The reference's
+ * Python package has no split scheme, and its Fortran file for it is outside its build.
+ * Applies to the rain_ice calls made after it on ctx (ice3_rain_ice*, ice3_adjust_rain_ice*,
+ * ICE3_STAGE_SEDIMENTATION).  With SPLI: nlev <= 256 (else ICE3_EINVAL); the sedimentation
+ * is FP64 in both precisions; the surrogate entry points return ICE3_EINVAL (the network was
+ * trained with the statistical scheme); a species of a column that would take more than 65536
+ * sub-steps stops there and the column is counted by ice3_ctx_status.  The fused entry points
+ * keep the microphysics' state at t of th, rv and rc (the adjusted sources, which the
+ * sedimentation's negativity correction updates) in a work area of the context, 24 bytes per
+ * point of nlev x ld, allocated on first use and grown when a larger call needs it (growing
+ * waits for the device); fused calls with SPLI on one context must not overlap, and
+ * ice3_adjust_rain_ice_stages returns ICE3_EINVAL for a MICROPHYSICS stage without the
+ * SEDIMENTATION stage in the same call. */
+#define ICE3_SEDIM_STAT 0
+#define ICE3_SEDIM_SPLI 1
+int ice3_ctx_set_sedimentation(ice3_ctx *ctx, int scheme, double max_cfl);
+
 /* Host copy of the derived constants, written as text lines "NAME n v1 .. vn"
  * (the format of the reference constants dump) into buf; returns the number
  * of bytes needed (excluding the terminating NUL).  ctx may be NULL: the

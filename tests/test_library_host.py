@@ -51,6 +51,7 @@ def test_bad_arguments_are_rejected_without_a_device(lib):
     assert lib.ice3_rain_ice_hail(None, 4, 90, 4, 50.0, None, None, None, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_adjust_rain_ice_hail(None, 4, 90, 4, 50.0, None, None, None, None, None) == _lib.ICE3_EINVAL
     assert lib.ice3_ctx_create(None) == _lib.ICE3_EINVAL
+    assert lib.ice3_ctx_set_sedimentation(None, 1, 0.8) == _lib.ICE3_EINVAL
     # input validation: argument checks come before any launch
     fp = (p * 2)(None, None)
     rows = (ctypes.c_int * 2)(90, 1)
@@ -147,8 +148,15 @@ def test_configuration_objects():
         return c
 
     check_configuration(arome())
+    # SEDIM selects the sedimentation scheme of rain_ice: 1 (STAT, AROME) or 0 (SPLI)
+    from ice3_amd.ice_adjust import sedimentation_of
+    assert sedimentation_of(arome()) == ("STAT", 0.8)
+    c = arome()
+    c.param_icen.SEDIM, c.param_icen.SPLIT_MAXCFL = 0, 0.5
+    check_configuration(c)
+    assert sedimentation_of(c) == ("SPLI", 0.5)
     for group, name, value in (("nebn", "LSUBG_COND", 0), ("nebn", "FRAC_ICE_ADJUST", 3),
-                               ("param_icen", "SEDIM", 0), ("param_icen", "LCOND2", 1),
+                               ("param_icen", "SEDIM", 2), ("param_icen", "LCOND2", 1),
                                ("param_icen", "PRISTINE_ICE", "COLU")):
         c = arome()
         setattr(getattr(c, group), name, value)
