@@ -50,12 +50,13 @@ STAGE_BYTES = {"ice_adjust": (232.0, 8.0),      # 19 reads + 10 writes; sigqsat
 STAGE_KERNELS = {"ice_adjust": "k_ice_adjust", "sedimentation": "k_sedim",
                  "ice_adjust+sedimentation": "k_adjust_sedim", "microphysics": "k_micro"}
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector
 # HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this workload
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def measured_traffic(kernel: str, ncol: int):
-    """Profiled HBM bytes per dispatch of `kernel` for the headline workload, else None."""
+def _profiled(table: str, kernel: str, ncol: int):
+    """Entry `kernel` of `table` in traffic.json (profiled on the headline workload), else None."""
     try:
         with open(TRAFFIC_FILE) as fh:
             t = json.load(fh)
@@ -63,7 +64,18 @@ def measured_traffic(kernel: str, ncol: int):
         return None
     if t.get("workload", {}).get("columns") != ncol or t["workload"].get("levels") != NLEV:
         return None
-    return t.get("bytes_per_dispatch", {}).get(kernel)
+    return t.get(table, {}).get(kernel)
+
+
+def measured_traffic(kernel: str, ncol: int):
+    """Profiled HBM bytes per dispatch of `kernel` for the headline workload, else None."""
+    return _profiled("bytes_per_dispatch", kernel, ncol)
+
+
+def measured_fp64_flops(kernel: str, ncol: int):
+    """FP64 VALU FLOPs per dispatch of `kernel` for the headline workload (rocprofv3
+    SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes), else None."""
+    return _profiled("fp64_flops_per_dispatch", kernel, ncol)
 
 
 def cpu_baseline(budget_s: float = 20.0, workers: int | None = None) -> dict:
@@ -154,6 +166,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjust", action="store_true", help="skip the ice_adjust-alone line")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate line")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-sedimentation line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device twice (a plain copy of known size, "
                          "to scale HBM byte counters read by a profiler)")
@@ -215,8 +228,10 @@ def main():
         cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
         return timed_steps(lambda: call(prec, cout), stream, steps, warmup)
 
-    def stage_times(prec, out, steps):
-        """Average duration of each stage's kernel, from the library's HIP events."""
+    def stage_times(prec, out, steps, merged=None):
+        """Average duration of each stage's kernel, from the library's HIP events (merged:
+        ICE_ADJUST and the sedimentation reported together, as one kernel runs both in the
+        FP64 statistical step)."""
         cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
         _lib.check(lib.ice3_stage_timing(ctx, 1), "ice3_stage_timing")
         acc = [0.0, 0.0, 0.0]
@@ -226,7 +241,7 @@ def main():
             _lib.check(lib.ice3_stage_times(ctx, ms3), "ice3_stage_times")
             acc = [x + y for x, y in zip(acc, ms3)]
         _lib.check(lib.ice3_stage_timing(ctx, 0), "ice3_stage_timing")
-        if prec == _lib.PREC_FP64:  # one kernel for ICE_ADJUST and the sedimentation
+        if merged if merged is not None else prec == _lib.PREC_FP64:
             return {"ice_adjust+sedimentation": (acc[0] + acc[1]) / steps, "microphysics": acc[2] / steps}
         return dict(zip(("ice_adjust", "sedimentation", "microphysics"), [x / steps for x in acc]))
 
@@ -262,6 +277,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom_achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(STAGE_KERNELS[dom], ncol),
                      "bytes_per_point": bp, "bytes_per_column": bc, "ms_per_launch": stages[dom]},
+        "valu_roofline": valu_roofline(STAGE_KERNELS[dom], stages[dom], ncol),
         "stages_ms": stages,
         "step_algorithmic_GBs": alg_bytes / (ms * 1e-3) / 1e9,
         "cpu_baseline": cpu,
@@ -298,6 +314,22 @@ def main():
 
     res["water_budget_max_rel_err"] = budget_err(out64)
     res["ice4_substeps"] = substeps(_lib.PREC_FP64, out64)
+    if not args.no_split:
+        # the same step with the split sedimentation (PARAM_ICE CSEDIM='SPLI', XSPLIT_MAXCFL 0.8)
+        outs = outputs()
+        _lib.check(lib.ice3_ctx_set_sedimentation(ctx, _lib.SEDIM["SPLI"], 0.8), "SPLI")
+        try:
+            ms_s, _ = run(_lib.PREC_FP64, outs, args.steps, args.warmup)
+            st_s = stage_times(_lib.PREC_FP64, outs, args.steps, merged=False)
+        finally:
+            _lib.check(lib.ice3_ctx_set_sedimentation(ctx, _lib.SEDIM["STAT"], 0.8), "STAT")
+        res.setdefault("variants", {})["split_sedimentation"] = {
+            "columns_per_s": ncol / (ms_s * 1e-3), "ms_per_step": ms_s, "speedup_vs_stat": ms / ms_s,
+            "stages_ms": st_s, "water_budget_max_rel_err": budget_err(outs),
+            "note": "fused step with CSEDIM='SPLI': k_ice_adjust, the state at t of th/rv/rc kept, "
+                    "k_sedim_split (FP64), k_micro"}
+        progress(f"SPLI step timed: {ms_s:.3f} ms median")
+        del outs
     if not args.no_mixed:
         out32 = outputs()
         ms32, _ = run(_lib.PREC_MIXED, out32, args.steps, args.warmup)
@@ -327,6 +359,17 @@ def main():
     print(json.dumps(res))
 
 
+def valu_roofline(kernel: str, ms: float, ncol: int) -> dict:
+    """FP64 VALU roofline of `kernel`: its profiled FP64 FLOPs per dispatch over the
+    event-timed duration `ms` of this run, against the FP64 vector peak."""
+    flops = measured_fp64_flops(kernel, ncol)
+    tf = flops / (ms * 1e-3) / 1e12 if flops else None
+    return {"bound": "valu_fp64", "kernel": kernel, "achieved": tf, "peak": FP64_VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": tf / FP64_VALU_PEAK_TFLOPS if tf else None,
+            "flops_per_launch": flops, "ms_per_launch": ms,
+            "flops_source": "rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 on this workload (profiles/traffic.json)"}
+
+
 def timed_steps(fn, stream, steps, warmup):
     """Run fn (one step, returning a C return code) warmup times, synchronise, then `steps`
     times with a HIP event pair around each call on `stream`; returns (median, mean) ms.
@@ -353,6 +396,16 @@ def timed_steps(fn, stream, steps, warmup):
 
 MLP_FLOP_PER_POINT = 2.0 * (14 * 128 + 3 * 128 * 128 + 128 * 8)   # 103,936
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+# Floor of the network stage.  The matrix cores run the padded widths (32 inputs, 32 outputs):
+MLP_FLOP_EXECUTED = 2.0 * (32 * 128 + 3 * 128 * 128 + 128 * 32)   # 114,688 per point
+MLP_ACTIVATIONS = 4 * 128                                         # SiLU evaluations per point
+# Full-rate FP32 VALU lane operations per second: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz.
+VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
+# VALU issue slots per activation in k_mlp_infer's epilogue as compiled: v_exp_f32 and
+# v_rcp_f32 (quarter rate, 4 slots each), 2 v_add_f32, 2 v_mul_f32, a bf16 convert and an
+# accumulator read; with a single quarter-rate operation and the bias add, multiply and
+# convert, 7.
+EPILOGUE_SLOTS_AS_BUILT, EPILOGUE_SLOTS_MIN = 14, 7
 
 
 def input_check(lib, g, ncol, stream, sp, steps):
@@ -456,6 +509,20 @@ def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
     pr = -1000.0 * (out["inprr"] + out["inprs"] + out["inprg"]) * DT
     budget = ((col1 - col0 - pr).abs() / col0).max().item()
     tflops = MLP_FLOP_PER_POINT * npts / (stages["network"] * 1e-3) / 1e12
+    # what the network stage cannot go below on this hardware, and the step built on it
+    mfma_ms = MLP_FLOP_EXECUTED * npts / (MFMA_BF16_PEAK_TFLOPS * 1e12) * 1e3
+    acts = MLP_ACTIVATIONS * npts
+    ep_ms, ep_min_ms = (acts * n / VALU_LANE_OPS * 1e3 for n in (EPILOGUE_SLOTS_AS_BUILT, EPILOGUE_SLOTS_MIN))
+    floor = {"network_mfma_ms": mfma_ms, "network_epilogue_valu_ms_as_built": ep_ms,
+             "network_epilogue_valu_ms_one_transcendental": ep_min_ms,
+             "network_measured_ms": stages["network"],
+             "surrogate_step_floor_ms": stages["ice_adjust+sedimentation"] + max(mfma_ms, ep_min_ms)
+             + stages["fallback"],
+             "fp64_step_ms": ms_plain,
+             "basis": "matrix-core time at the dense bf16 peak; SiLU epilogue at the full-rate FP32 VALU "
+                      "issue rate (quarter-rate exp/rcp = 4 slots); step floor = the column walk and the "
+                      "FP64 fallback as measured, plus the network at its floor, with features and decode "
+                      "fused away"}
     return {"columns": ncol, "levels": nlev, "weights": os.path.relpath(S.DEFAULT_WEIGHTS, ROOT),
             "train_pairs": int(sg.meta.get("pairs", 0)), "budget_tol": sg.budget_tol, "neg_tol": sg.neg_tol,
             "fp64_ms_per_step": ms_plain, "fp64_columns_per_s": ncol / (ms_plain * 1e-3),
@@ -464,6 +531,7 @@ def surrogate_variant(lib, ctx, dev, g, args, reps: int = 4):
             "points": npts, "network_points": c[0], "fallback": {"out_of_range": c[1], "negative_source": c[2],
                                                                  "water_residual": c[3], "no_hydrometeor": c[4]},
             "network_fraction": c[0] / npts, "water_budget_max_rel_err": budget, "error_vs_fp64": err,
+            "floor": floor,
             "roofline": {"bound": "mfma", "kernel": "k_mlp_infer", "achieved": tflops,
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
                          "flop_per_point": MLP_FLOP_PER_POINT, "ms_per_launch": stages["network"]}}
