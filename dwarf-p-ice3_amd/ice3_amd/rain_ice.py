@@ -4,10 +4,11 @@
 state dict's sources ``ths, rvs, rcs, rrs, ris, rss, rgs`` and ``ci_t`` are replaced by
 their updated values, ``evap3d``/``rainfr`` are added, and the diagnostics hold the
 surface precipitation ``inprc, inprr, inpri, inprs, inprg``.  Arithmetic is PHYEX
-RAIN_ICE (micro/rain_ice.F90) with statistical sedimentation before the
-microphysics, the per-point ICE4 time splitting, the total tendencies and the
-negativity correction, evaluated by the HIP kernels ``k_sedim`` (one column per thread)
-and ``k_micro`` (the persistent per-point microphysics).
+RAIN_ICE (micro/rain_ice.F90) with the sedimentation before the microphysics, the
+per-point ICE4 time splitting, the total tendencies and the negativity correction,
+evaluated by the HIP kernels ``k_sedim`` (the statistical sedimentation, one column per
+thread) or ``k_sedim_split`` (``sedimentation="SPLI"``: the split scheme, one column per
+wavefront) and ``k_micro`` (the persistent per-point microphysics).
 
 State keys read (RainIceJAX names): exn, rhodref, pres, dzz, th_t, rv_t, rc_t, rr_t, ri_t,
 rs_t, rg_t, ci_t, ths, rvs, rcs, rrs, ris, rss, rgs; optional: exnref (default exn),
