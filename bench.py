@@ -341,10 +341,10 @@ def main():
             err[k] = {"max_abs": d.max().item(), "rms_abs": d.pow(2).mean().sqrt().item(),
                       "max_rel_to_field_max": d.max().item() / scale if scale > 0 else 0.0}
         progress(f"mixed step timed: {ms32:.3f} ms median")
-        res["variants"] = {"mixed": {"columns_per_s": ncol / (ms32 * 1e-3), "ms_per_step": ms32,
+        res.setdefault("variants", {})["mixed"] = {"columns_per_s": ncol / (ms32 * 1e-3), "ms_per_step": ms32,
                                      "speedup_vs_fp64": ms / ms32, "stages_ms": stages32,
                                      "water_budget_max_rel_err": budget_err(out32),
-                                     "error_vs_fp64": err}}
+                                     "error_vs_fp64": err}
     if not args.no_surrogate:
         del out64
         if not args.no_mixed:

@@ -134,11 +134,15 @@ __global__ void __launch_bounds__(256) k_count_nonfinite(CountArgs a) {
 // wavefront holds one column in registers, level k in lane k % 64, slot k / 64 (L slots):
 // a sub-step's fluxes are computed by all lanes at once, its length is a minimum over the
 // wavefront, and the flux entering level k from above comes from the next lane.  A workgroup
-// of split_cols wavefronts takes as many adjacent columns, so that the inputs are read in rows
-// of split_cols doubles, through LDS, and handed to the column's wavefront: 16 (a 128-byte row,
-// four wavefronts per SIMD, 128 registers each) while a wavefront's state fits 128 registers,
-// else 8 (256 registers).
-template <int L, bool HAIL> constexpr int split_cols() { return L == 1 || (L == 2 && !HAIL) ? 16 : 8; }
+// of SPLIT_COLS wavefronts takes as many adjacent columns: their inputs are read in rows of
+// SPLIT_COLS doubles, a batch of up to SPLIT_BATCH fields at a time (every load of a batch
+// issued before the first wait), and handed through LDS to the columns' wavefronts; the
+// outputs go back the same way.  The two workgroups that take the halves of a 16-column row
+// run on the same XCD, so that the row's cache lines are fetched once.  While a wavefront's
+// state fits 128 registers (one slot, or two without hail) two workgroups share a compute
+// unit, one's loads overlapping the other's sub-steps.
+constexpr int SPLIT_COLS = 8, SPLIT_BATCH = 8;
+template <int L, bool HAIL> constexpr int split_wpe() { return L == 1 || (L == 2 && !HAIL) ? 4 : 2; }
 constexpr int SPLIT_MAX_SUBSTEPS = 1 << 16;  // per species and column; the reference has no
                                              // bound (its loop ends when the time is spent)
 
@@ -148,21 +152,50 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-// Stage field f (times scale) of the workgroup's columns through LDS; slot q of this lane
-// receives level lane + 64 q of the wavefront's column (0 outside the domain).  Every thread
-// of the workgroup calls it.
-template <int L, int COLS>
-__device__ __forceinline__ void split_stage(double (*tile)[COLS + 1], const SplitArgs &a,
-                                            const double *f, double scale, int c0, double (&v)[L]) {
-  __syncthreads();
-  for (int e = threadIdx.x; e < L * 64 * COLS; e += 64 * COLS) {
-    const int k = e / COLS, c = e - k * COLS, col = c0 + c;
-    tile[k][c] = (k < a.nlev && col < a.ncol) ? f[(size_t)k * a.ld + col] * scale : 0.;
+// Column group of block b: blocks b and b + 8 of a full group of 16 blocks, which the
+// dispatcher sends to the same XCD, take the two halves of a 16-column row; the blocks of the
+// last, partial group keep their order.
+__device__ __forceinline__ int split_group(int b, int nblk) {
+  if (b >= (nblk & ~15)) return b;
+  return (b & ~15) + ((b & 7) << 1) + ((b >> 3) & 1);
+}
+
+// Stage NF fields (field f times scale[f]) of the workgroup's SPLIT_COLS columns from c0 on
+// through LDS: v[f][q] of this lane receives level lane + 64 q of its wavefront's column (0
+// outside the domain).  Every thread of the workgroup calls it.
+template <int L, int NF>
+__device__ __forceinline__ void split_load(double (*tile)[L * 64][SPLIT_COLS + 1],
+                                           const SplitArgs &a, const double *const (&f)[NF],
+                                           const double (&scale)[NF], int c0, double (&v)[NF][L]) {
+  // every load of the batch is issued unconditionally (outside the domain, element 0 is read
+  // and dropped after the barrier), so that none waits for another
+  size_t idx[L];
+  bool in[L];
+#pragma unroll
+  for (int p = 0; p < L; ++p) {
+    const int e = threadIdx.x + p * 64 * SPLIT_COLS, k = e / SPLIT_COLS, col = c0 + e - k * SPLIT_COLS;
+    in[p] = k < a.nlev && col < a.ncol;
+    idx[p] = in[p] ? (size_t)k * a.ld + col : 0;
   }
+  double x[NF][L];
+#pragma unroll
+  for (int g = 0; g < NF; ++g)
+#pragma unroll
+    for (int p = 0; p < L; ++p) x[g][p] = f[g][idx[p]];
+  __syncthreads();  // the previous batch has been read from the tile
+#pragma unroll
+  for (int g = 0; g < NF; ++g)
+#pragma unroll
+    for (int p = 0; p < L; ++p) {
+      const int e = threadIdx.x + p * 64 * SPLIT_COLS, k = e / SPLIT_COLS;
+      tile[g][k][e - k * SPLIT_COLS] = in[p] ? x[g][p] * scale[g] : 0.;
+    }
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-  for (int q = 0; q < L; ++q) v[q] = tile[lane + 64 * q][w];
+  for (int g = 0; g < NF; ++g)
+#pragma unroll
+    for (int q = 0; q < L; ++q) v[g][q] = tile[g][lane + 64 * q][w];
 }
 
 // Mass flux ZWSED of species JRR at mixing ratio r above its threshold; rf is the density
@@ -235,80 +268,122 @@ __device__ __forceinline__ double split_species(const SplitArgs &a, int lane, co
 }
 
 template <int L, bool HAIL, bool ZZZ>
-__global__ void __launch_bounds__((64 * split_cols<L, HAIL>())) k_sedim_split(SplitArgs a) {
-  constexpr int COLS = split_cols<L, HAIL>();
-  __shared__ double tile[L * 64][COLS + 1];
+__global__ void __launch_bounds__(64 * SPLIT_COLS)
+    __attribute__((amdgpu_waves_per_eu(split_wpe<L, HAIL>()))) k_sedim_split(SplitArgs a) {
+  __shared__ double tile[SPLIT_BATCH][L * 64][SPLIT_COLS + 1];
   constexpr int NV = HAIL ? 8 : 7;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int c0 = blockIdx.x * COLS, i = c0 + w;
-  // RAIN_ICE's L/cph factors of the state at t (rain_ice.F90:335-345)
-  double lv[L], ls[L], r[NV - 3][L];
+  const int c0 = split_group(blockIdx.x, gridDim.x) * SPLIT_COLS, i = c0 + w;
+  const double ts = a.tscale;
+  double lv[L], ls[L], r[NV - 3][L], rho[L], dz[L], src[NV][L];
+  // batch 1: exn and the state at t of th .. rg; batch 2: hail's state at t, rho, dz and
+  // sources; batch 3: the last sources.  RAIN_ICE's L/cph factors of the state at t
+  // (rain_ice.F90:335-345) as soon as that state is in.
+  double b1[8][L];
   {
-    double t[NV][L], exn[L];
-    split_stage<L, COLS>(tile, a, a.exn, 1., c0, exn);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) split_stage<L, COLS>(tile, a, a.t[v], a.tscale, c0, t[v]);
+    const double *const f1[8] = {a.exn, a.t[0], a.t[1], a.t[2], a.t[3], a.t[4], a.t[5], a.t[6]};
+    const double s1[8] = {1., ts, ts, ts, ts, ts, ts, ts};
+    split_load<L, 8>(tile, a, f1, s1, c0, b1);
+  }
+  auto factors = [&](const double (&rh_t)[L]) {
 #pragma unroll
     for (int q = 0; q < L; ++q) {
-      double tq[NV];
+      double tq[8];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) tq[v] = t[v][q];
-      rain_ice_factors<HAIL>(tq, exn[q], ls[q], lv[q]);
+      for (int v = 0; v < 7; ++v) tq[v] = b1[v + 1][q];
+      tq[7] = rh_t[q];
+      rain_ice_factors<HAIL>(tq, b1[0][q], ls[q], lv[q]);
+#pragma unroll
+      for (int v = 3; v < 7; ++v) r[v - 3][q] = b1[v + 1][q];
+      if constexpr (HAIL) r[4][q] = rh_t[q];
     }
-#pragma unroll
-    for (int v = 3; v < NV; ++v)
-#pragma unroll
-      for (int q = 0; q < L; ++q) r[v - 3][q] = t[v][q];
+  };
+  if constexpr (!HAIL) {
+    const double zero[L] = {};
+    factors(zero);
   }
-  double rho[L], dz[L], oorhodz[L], src[NV][L];
-  split_stage<L, COLS>(tile, a, a.rhodref, 1., c0, rho);
-  split_stage<L, COLS>(tile, a, a.dzz, 1., c0, dz);
-  if constexpr (ZZZ) {  // RAIN_ICE's model level height (rain_ice.F90:387-395), in its order
-    if (lane == 0 && i < a.ncol) {
-      double zzsum = 0.;
-      for (int k = a.nlev - 1; k >= 0; --k) {
-        zzsum = zzsum + tile[k][w];
-        a.zzz[(size_t)k * a.ld + i] = zzsum - tile[k][w] * 0.5;
+  {
+    constexpr int N2 = HAIL ? 8 : 7;
+    double b2[N2][L];
+    const double *const f2[8] = {a.rhodref, a.dzz, a.src[0], a.src[1], a.src[2], a.src[3],
+                                 a.src[4], a.t[7]};
+    const double s2[8] = {1., 1., 1., 1., 1., 1., 1., ts};
+    split_load<L, N2>(tile, a, reinterpret_cast<const double *const (&)[N2]>(f2),
+                      reinterpret_cast<const double (&)[N2]>(s2), c0, b2);
+    if constexpr (ZZZ) {  // RAIN_ICE's model level height (rain_ice.F90:387-395), in its order
+      if (lane == 0 && i < a.ncol) {
+        double zzsum = 0.;
+        for (int k = a.nlev - 1; k >= 0; --k) {
+          zzsum = zzsum + tile[1][k][w];
+          a.zzz[(size_t)k * a.ld + i] = zzsum - tile[1][k][w] * 0.5;
+        }
       }
     }
-  }
+    if constexpr (HAIL) factors(b2[7]);
 #pragma unroll
-  for (int v = 0; v < NV; ++v) split_stage<L, COLS>(tile, a, a.src[v], 1., c0, src[v]);
+    for (int q = 0; q < L; ++q) {
+      rho[q] = b2[0][q];
+      dz[q] = b2[1][q];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) src[v][q] = b2[v + 2][q];
+    }
+  }
+  {
+    constexpr int N3 = HAIL ? 3 : 2;
+    double b3[N3][L];
+    const double *const f3[3] = {a.src[5], a.src[6], a.src[7]};
+    const double s3[3] = {1., 1., 1.};
+    split_load<L, N3>(tile, a, reinterpret_cast<const double *const (&)[N3]>(f3),
+                      reinterpret_cast<const double (&)[N3]>(s3), c0, b3);
+#pragma unroll
+    for (int v = 0; v < N3; ++v)
+#pragma unroll
+      for (int q = 0; q < L; ++q) src[v + 5][q] = b3[v][q];
+  }
+  double oorhodz[L];
 #pragma unroll
   for (int q = 0; q < L; ++q)
     oorhodz[q] = lane + 64 * q < a.nlev ? 1. / (rho[q] * dz[q]) : 0.;  // ZW (:263)
-  if (i >= a.ncol) return;  // after the last barrier
-  bool capped = false;
-  const double pr = split_species<L, 3>(a, lane, rho, dz, oorhodz, r[0], src[IRR], &capped);
-  const double pi = split_species<L, 4>(a, lane, rho, dz, oorhodz, r[1], src[IRI], &capped);
-  const double ps = split_species<L, 5>(a, lane, rho, dz, oorhodz, r[2], src[IRS], &capped);
-  const double pg = split_species<L, 6>(a, lane, rho, dz, oorhodz, r[3], src[IRG], &capped);
-  double ph = 0.;
-  if constexpr (HAIL) ph = split_species<L, 7>(a, lane, rho, dz, oorhodz, r[4], src[IRH], &capped);
-  if (lane == 0) {
-    a.inprc[i] = 0.;
-    a.inprr[i] = pr;
-    a.inpri[i] = pi;
-    a.inprs[i] = ps + pi;  // mode_ice4_sedimentation.F90:212
-    a.inprg[i] = pg;
-    if constexpr (HAIL) a.inprh[i] = ph;
-    if (capped && a.capped) atomicAdd(a.capped, 1ull);
+  if (i < a.ncol) {
+    bool capped = false;
+    const double pr = split_species<L, 3>(a, lane, rho, dz, oorhodz, r[0], src[IRR], &capped);
+    const double pi = split_species<L, 4>(a, lane, rho, dz, oorhodz, r[1], src[IRI], &capped);
+    const double ps = split_species<L, 5>(a, lane, rho, dz, oorhodz, r[2], src[IRS], &capped);
+    const double pg = split_species<L, 6>(a, lane, rho, dz, oorhodz, r[3], src[IRG], &capped);
+    double ph = 0.;
+    if constexpr (HAIL) ph = split_species<L, 7>(a, lane, rho, dz, oorhodz, r[4], src[IRH], &capped);
+    if (lane == 0) {
+      a.inprc[i] = 0.;
+      a.inprr[i] = pr;
+      a.inpri[i] = pi;
+      a.inprs[i] = ps + pi;  // mode_ice4_sedimentation.F90:212
+      a.inprg[i] = pg;
+      if constexpr (HAIL) a.inprh[i] = ph;
+      if (capped && a.capped) atomicAdd(a.capped, 1ull);
+    }
+#pragma unroll
+    for (int q = 0; q < L; ++q) {  // ICE4_CORRECT_NEGATIVITIES (:216-218), every point
+      double rh = HAIL ? src[NV - 1][q] : 0.;
+      correct_negativities<HAIL>(src[ITH][q], src[IRV][q], src[IRC][q], src[IRR][q], src[IRI][q],
+                                 src[IRS][q], src[IRG][q], lv[q], ls[q], &rh);
+      if constexpr (HAIL) src[NV - 1][q] = rh;
+    }
   }
+  // the sources back through LDS, in rows of SPLIT_COLS columns
+  __syncthreads();  // every wavefront has read the last batch
 #pragma unroll
-  for (int q = 0; q < L; ++q) {  // ICE4_CORRECT_NEGATIVITIES (:216-218), every point
-    const int k = lane + 64 * q;
-    if (k >= a.nlev) continue;
-    double x[NV];
+  for (int v = 0; v < NV; ++v)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) x[v] = src[v][q];
-    double rh = HAIL ? x[NV - 1] : 0.;
-    correct_negativities<HAIL>(x[ITH], x[IRV], x[IRC], x[IRR], x[IRI], x[IRS], x[IRG], lv[q],
-                               ls[q], &rh);
-    if constexpr (HAIL) x[NV - 1] = rh;
-    const size_t j = (size_t)k * a.ld + i;
+    for (int q = 0; q < L; ++q) tile[v][lane + 64 * q][w] = src[v][q];
+  __syncthreads();
 #pragma unroll
-    for (int v = 0; v < NV; ++v) a.dst[v][j] = x[v];
-  }
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int p = 0; p < L; ++p) {
+      const int e = threadIdx.x + p * 64 * SPLIT_COLS, k = e / SPLIT_COLS,
+                c = e - k * SPLIT_COLS, col = c0 + c;
+      if (k < a.nlev && col < a.ncol) a.dst[v][(size_t)k * a.ld + col] = tile[v][k][c];
+    }
 }
 
 // ---- k_micro: the ICE4 microphysics, persistent, with the time splitting compacted
@@ -1082,9 +1157,8 @@ static bool sedim_fits(const ice3_ctx *ctx, int nlev) {
 
 template <int L, bool HAIL, bool ZZZ>
 static void launch_split_k(const SplitArgs &a, hipStream_t s) {
-  constexpr int C = split_cols<L, HAIL>();
-  hipLaunchKernelGGL((k_sedim_split<L, HAIL, ZZZ>), dim3((a.ncol + C - 1) / C), dim3(64 * C), 0,
-                     s, a);
+  hipLaunchKernelGGL((k_sedim_split<L, HAIL, ZZZ>), dim3((a.ncol + SPLIT_COLS - 1) / SPLIT_COLS),
+                     dim3(64 * SPLIT_COLS), 0, s, a);
 }
 
 template <bool HAIL, bool ZZZ>

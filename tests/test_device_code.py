@@ -1,12 +1,12 @@
 """Properties of the gfx950 device code itself, read from the assembly hipcc emits for each HIP
 source with the library's own build flags (CPU only: hipcc cross-compiles).
 
-Every kernel keeps its private data in its own registers or scratch frame: no device-side
-calls (every helper is inlined), no flat load or store (private memory is reached only by
-scratch instructions at fixed frame offsets, global memory only by global instructions), and
-no pointer formed from the private aperture base.  A flat access is the one kind of memory
-instruction that the hardware translates through the private and LDS apertures, so code
-without any cannot raise an aperture violation."""
+Every kernel keeps its private data in its own registers or a small scratch frame of register
+spills: no device-side calls (every helper is inlined), no flat load or store (private memory
+is reached only by scratch instructions at fixed frame offsets, global memory only by global
+instructions), and no pointer formed from the private aperture base.  A flat access is the one
+kind of memory instruction that the hardware translates through the private and LDS
+apertures, so code without any cannot raise an aperture violation."""
 import os
 import re
 import shutil
@@ -55,5 +55,6 @@ def test_no_calls_no_flat_access(src, tmp_path):
     for name, desc in ks.items():
         assert ".amdhsa_uses_dynamic_stack 0" in desc, name
         frame = int(re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", desc).group(1))
-        # register spills only (the largest, k_micro's, is 24 bytes per lane)
-        assert frame <= 64, (name, frame)
+        # register spills only: 24 bytes per lane in k_micro, at most 68 in the split
+        # sedimentation's instances for 150 to 256 levels with hail
+        assert frame <= 128, (name, frame)
