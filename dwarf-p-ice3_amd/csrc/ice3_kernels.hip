@@ -146,10 +146,51 @@ template <int L, bool HAIL> constexpr int split_wpe() { return L == 1 || (L == 2
 constexpr int SPLIT_MAX_SUBSTEPS = 1 << 16;  // per species and column; the reference has no
                                              // bound (its loop ends when the time is spent)
 
+// Lane moves of an FP64 value inside the VALU, as two dword moves: a DPP move (lanes whose
+// source lane is outside the wavefront keep old) and the gfx950 permlane swaps, which give
+// a the lower and b the upper 16-lane rows of each 32-lane half (16) or the lower and upper
+// halves (32) of the wavefront.  Every lane of the wavefront must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v, double old) {
+  const int lo = __builtin_amdgcn_update_dpp((int)__double2loint(old), (int)__double2loint(v), CTRL,
+                                             0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)__double2hiint(old), (int)__double2hiint(v), CTRL,
+                                             0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+constexpr int DPP_QUAD_1032 = 0xB1, DPP_QUAD_2301 = 0x4E, DPP_ROW_MIRROR = 0x140,
+              DPP_ROW_HALF_MIRROR = 0x141, DPP_WAVE_SHL1 = 0x130;
+__device__ __forceinline__ void swap16_f64(double v, double &a, double &b) {
+  const auto l = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+  a = __hiloint2double(h[0], l[0]);
+  b = __hiloint2double(h[1], l[1]);
+}
+__device__ __forceinline__ void swap32_f64(double v, double &a, double &b) {
+  const auto l = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+  a = __hiloint2double(h[0], l[0]);
+  b = __hiloint2double(h[1], l[1]);
+}
+
+// Minimum over the wavefront, in every lane: within quads, half-rows and rows by DPP, then
+// across rows and halves by the permlane swaps (min is exact, so the order is immaterial).
 __device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = dmin(v, __shfl_xor(v, m, 64));
-  return v;
+  v = dmin(v, dpp_f64<DPP_QUAD_1032>(v, v));
+  v = dmin(v, dpp_f64<DPP_QUAD_2301>(v, v));
+  v = dmin(v, dpp_f64<DPP_ROW_HALF_MIRROR>(v, v));
+  v = dmin(v, dpp_f64<DPP_ROW_MIRROR>(v, v));
+  double a, b;
+  swap16_f64(v, a, b);
+  v = dmin(a, b);
+  swap32_f64(v, a, b);
+  return dmin(a, b);
+}
+
+// Lane 0's value, in a scalar register.
+__device__ __forceinline__ double lane0_f64(double v) {
+  return __hiloint2double(__builtin_amdgcn_readlane((int)__double2hiint(v), 0),
+                          __builtin_amdgcn_readlane((int)__double2loint(v), 0));
 }
 
 // Column group of block b: blocks b and b + 8 of a full group of 16 blocks, which the
@@ -253,12 +294,11 @@ __device__ __forceinline__ double split_species(const SplitArgs &a, int lane, co
     }
     const double step = dmin(remaint, wave_min(m));
     remaint = remaint - step;  // :814-815
-    inpr = inpr + __shfl(w[0], 0, 64) / P.XRHOLW * (step * zinv);
+    inpr = inpr + lane0_f64(w[0]) / P.XRHOLW * (step * zinv);
 #pragma unroll
     for (int q = 0; q < L; ++q) {  // :820-824; level k + 1 is the next lane (or slot)
-      const double next = q + 1 < L ? __shfl(w[q + 1 < L ? q + 1 : q], 0, 64) : 0.;
-      double up = __shfl_down(w[q], 1, 64);
-      if (lane == 63) up = next;
+      const double next = q + 1 < L ? lane0_f64(w[q + 1 < L ? q + 1 : q]) : 0.;
+      const double up = dpp_f64<DPP_WAVE_SHL1>(w[q], next);
       const double mrchange = step * oorhodz[q] * (up - w[q]);
       r[q] = r[q] + mrchange + prs[q] * step;
       s[q] = s[q] + mrchange * zinv;
