@@ -167,6 +167,7 @@ def main():
     ap.add_argument("--no-adjust", action="store_true", help="skip the ice_adjust-alone line")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate line")
     ap.add_argument("--no-split", action="store_true", help="skip the split-sedimentation line")
+    ap.add_argument("--no-configs", action="store_true", help="skip the OCND2 and hail lines")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device twice (a plain copy of known size, "
                          "to scale HBM byte counters read by a profiler)")
@@ -228,22 +229,28 @@ def main():
         cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
         return timed_steps(lambda: call(prec, cout), stream, steps, warmup)
 
-    def stage_times(prec, out, steps, merged=None):
-        """Average duration of each stage's kernel, from the library's HIP events (merged:
-        ICE_ADJUST and the sedimentation reported together, as one kernel runs both in the
-        FP64 statistical step)."""
-        cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
+    def stage_times_of(fn, steps, merged):
+        """Average duration of each stage's kernel over `steps` calls of fn, from the library's
+        HIP events (merged: ICE_ADJUST and the sedimentation reported together, where one
+        kernel runs both)."""
         _lib.check(lib.ice3_stage_timing(ctx, 1), "ice3_stage_timing")
         acc = [0.0, 0.0, 0.0]
         ms3 = (ctypes.c_float * 6)()
         for _ in range(steps):
-            _lib.check(call(prec, cout), "timed step")
+            _lib.check(fn(), "timed step")
             _lib.check(lib.ice3_stage_times(ctx, ms3), "ice3_stage_times")
             acc = [x + y for x, y in zip(acc, ms3)]
         _lib.check(lib.ice3_stage_timing(ctx, 0), "ice3_stage_timing")
-        if merged if merged is not None else prec == _lib.PREC_FP64:
+        if merged:
             return {"ice_adjust+sedimentation": (acc[0] + acc[1]) / steps, "microphysics": acc[2] / steps}
         return dict(zip(("ice_adjust", "sedimentation", "microphysics"), [x / steps for x in acc]))
+
+    def stage_times(prec, out, steps, merged=None):
+        """Stage times of the fused step in precision prec (merged by default for the FP64
+        statistical step, whose column walk runs ICE_ADJUST and the sedimentation)."""
+        cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
+        return stage_times_of(lambda: call(prec, cout), steps,
+                              merged if merged is not None else prec == _lib.PREC_FP64)
 
     out64 = outputs()
     progress("inputs resident on the GPU")
@@ -330,6 +337,15 @@ def main():
                     "k_sedim_split (FP64), k_micro"}
         progress(f"SPLI step timed: {ms_s:.3f} ms median")
         del outs
+    if not args.no_configs:
+        # the other configurations of the same step on the same inputs: OCND2 in ICE_ADJUST
+        # and RAIN_ICE (PARAMI%LOCND2), and KRR=7 hail
+        V = res.setdefault("variants", {})
+        V["ocnd2"] = ocnd2_step(lib, ctx, dev, g, cin, sp, stream, args, outputs, budget_err,
+                                stage_times_of, ms)
+        progress(f"OCND2 step timed: {V['ocnd2']['ms_per_step']:.3f} ms median")
+        V["hail"] = hail_step(lib, ctx, dev, g, cin, sp, stream, args, outputs, stage_times_of, ms)
+        progress(f"hail step timed: {V['hail']['ms_per_step']:.3f} ms median")
     if not args.no_mixed:
         out32 = outputs()
         ms32, _ = run(_lib.PREC_MIXED, out32, args.steps, args.warmup)
@@ -368,6 +384,70 @@ def valu_roofline(kernel: str, ms: float, ncol: int) -> dict:
             "unit": "TFLOP/s", "frac": tf / FP64_VALU_PEAK_TFLOPS if tf else None,
             "flops_per_launch": flops, "ms_per_launch": ms,
             "flops_source": "rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 on this workload (profiles/traffic.json)"}
+
+
+def ocnd2_step(lib, ctx, dev, g, cin, sp, stream, args, outputs, budget_err, stage_times_of, ms_stat):
+    """The fused step with OCND2 in both halves (ice3_adjust_rain_ice_ocnd2) on the headline
+    inputs, with the generator's level heights in every column and PICE_CLD_WGT drawn from
+    U(0.5, 1.5)."""
+    import numpy as np
+    import torch
+    from ice3_amd import _lib, synthetic
+    from ice3_amd._fields import ptr
+    ncol = g["th"].shape[1]
+    zz = torch.as_tensor(np.ascontiguousarray(np.broadcast_to(synthetic.level_profile(NLEV)[2][:, None],
+                                                              (NLEV, ncol))), device=dev)
+    wgt = torch.as_tensor(np.random.Generator(np.random.Philox(SEED + 1)).uniform(0.5, 1.5, ncol), device=dev)
+    out = outputs()
+    x = {k: torch.empty((NLEV, ncol), dtype=torch.float64, device=dev) for k in _lib.OCND2_OUT_FIELDS}
+    cin2 = _lib.Ocnd2In(ptr(zz), ptr(wgt))
+    cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
+    cout2 = _lib.Ocnd2Out(*[ptr(x[k]) for k in _lib.OCND2_OUT_FIELDS])
+
+    def fn():
+        return lib.ice3_adjust_rain_ice_ocnd2(ctx, ncol, NLEV, ncol, DT, ctypes.byref(cin), ctypes.byref(cin2),
+                                              ctypes.byref(cout), ctypes.byref(cout2), None, sp)
+    ms, _ = timed_steps(fn, stream, args.steps, args.warmup)
+    return {"columns_per_s": ncol / (ms * 1e-3), "ms_per_step": ms, "speedup_vs_stat_krr6": ms_stat / ms,
+            "stages_ms": stage_times_of(fn, args.steps, False),
+            "water_budget_max_rel_err": budget_err(out),
+            "note": "fused step with PARAMI%LOCND2: k_ice_adjust_ocnd2, k_sedim<double> (which also "
+                    "writes the model level height), k_micro<double, OCND2> (persistent)"}
+
+
+def hail_step(lib, ctx, dev, g, cin, sp, stream, args, outputs, stage_times_of, ms_stat):
+    """The fused step with KRR=7 (ice3_adjust_rain_ice_hail) on the headline inputs with hail
+    added where there is graupel: rh = rg x U(0.2, 2), its source rh / dt."""
+    import numpy as np
+    import torch
+    from ice3_amd import _lib
+    from ice3_amd._fields import ptr
+    ncol = g["th"].shape[1]
+    rng = np.random.Generator(np.random.Philox(SEED + 7))
+    rh = g["rg"] * torch.as_tensor(rng.uniform(0.2, 2.0, (NLEV, ncol)), device=dev)
+    rhs = rh / DT
+    rhs_out = torch.empty_like(rh)
+    inprh = torch.empty(ncol, dtype=torch.float64, device=dev)
+    out = outputs()
+    cout = _lib.PipeOut(*[ptr(out[k]) for k in _lib.PIPE_OUT_FIELDS])
+    hail = _lib.PipeHail(ptr(rh), ptr(rhs), ptr(rhs_out), ptr(inprh))
+
+    def fn():
+        return lib.ice3_adjust_rain_ice_hail(ctx, ncol, NLEV, ncol, DT, ctypes.byref(cin), ctypes.byref(cout),
+                                             ctypes.byref(hail), None, sp)
+    ms, _ = timed_steps(fn, stream, args.steps, args.warmup)
+    stages = stage_times_of(fn, args.steps, True)
+    # column water budget with hail: change of the column's water = surface precipitation
+    w = ("rvs", "rcs", "rrs", "ris", "rss", "rgs")
+    mass = g["rhodref"] * g["dzz"]
+    col0 = (mass * (sum(g[k] for k in w) + rhs)).sum(0) * DT
+    col1 = (mass * (sum(out[k] for k in w) + rhs_out)).sum(0) * DT
+    prec = -1000.0 * (out["inprr"] + out["inprs"] + out["inprg"] + inprh) * DT
+    budget = ((col1 - col0 - prec).abs() / col0).max().item()
+    return {"columns_per_s": ncol / (ms * 1e-3), "ms_per_step": ms, "speedup_vs_stat_krr6": ms_stat / ms,
+            "stages_ms": stages, "water_budget_max_rel_err": budget,
+            "note": "fused step with KRR=7: k_adjust_sedim_hail (column walk), then the hail "
+                    "microphysics k_micro<double, false, HAIL> (persistent, one wavefront per SIMD)"}
 
 
 def timed_steps(fn, stream, steps, warmup):

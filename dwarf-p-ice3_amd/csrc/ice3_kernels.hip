@@ -72,7 +72,8 @@ __global__ void __launch_bounds__(256) k_sedim(SedimArgs a) {
 
 // ---- hail (KRR=7), FP64: ICE_ADJUST with the hail heat capacity (one thread per point),
 // the sedimentation with hail (one thread per column), the two in one column walk for the
-// fused step, and the microphysics one thread per point (k_micro_points<double, false, true>).
+// fused step; the microphysics is k_micro<double, false, true> (k_micro_points<double, false,
+// true> when the context sets another sub-step cap).
 __global__ void __launch_bounds__(256) k_ice_adjust_hail(AdjustArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.ncol) return;
@@ -444,75 +445,99 @@ __global__ void __launch_bounds__(64 * SPLIT_COLS)
 // one waits on.  The workgroup's wavefronts share the lookup tables, staged once into LDS.
 // Wavefronts per workgroup.  The mixed path (183 VGPRs at two wavefronts per SIMD) measured
 // slower with three (168 VGPRs, spills, 49-slot parks: +22 %) or four (+120 %) per SIMD.
-template <class R> constexpr int micro_waves() { return 8; }
-template <class R> constexpr size_t micro_tab_bytes() {
-  return ((size_t)ICE3_T_ICE3 * sizeof(R) + 15) / 16 * 16;
+// The hail state needs about 306 registers: one wavefront per SIMD (at two, 55 registers
+// spill and the kernel is slower than one thread per point), with 63-slot parks.
+template <class R, bool HAIL = false> constexpr int micro_waves() { return HAIL ? 4 : 8; }
+// Lookup tables staged in LDS: the ICE3 block (KRR=6); the hail kernel reads its larger
+// block (the wet-growth kernels of hail) from device memory and gives the LDS to its parks.
+template <class R, bool HAIL = false> constexpr size_t micro_tab_bytes() {
+  return HAIL ? 0 : ((size_t)ICE3_T_ICE3 * sizeof(R) + 15) / 16 * 16;
+}
+// Variables of the time splitting (KRR + 1) and rates kept between tendency calls.
+template <bool HAIL> constexpr int micro_nv() { return HAIL ? 8 : 7; }
+template <class R, bool HAIL> constexpr int micro_ns() {
+  return (int)(sizeof(typename StepStateFor<R, HAIL>::type) / sizeof(R));
 }
 // Park slots per wavefront: what the LDS holds beside the tables, at most 63 (31 in FP64,
-// 63 in the mixed path).
-template <class R> constexpr int micro_park_cap() {
-  const size_t slot = 9 * sizeof(double) + 32 * sizeof(R) + 2 * sizeof(int);
-  const size_t n = (160 * 1024 - micro_tab_bytes<R>()) / micro_waves<R>() / slot;
+// 63 in the mixed path, 47 with hail).
+template <class R, bool HAIL = false> constexpr int micro_park_cap() {
+  const size_t slot = (micro_nv<HAIL>() + 2) * sizeof(double) + micro_ns<R, HAIL>() * sizeof(R) +
+                      2 * sizeof(int);
+  const size_t n = (160 * 1024 - micro_tab_bytes<R, HAIL>()) / micro_waves<R, HAIL>() / slot;
   return n < 63 ? (int)n : 63;
 }
 
 // Points parked by one wavefront: state between tendency calls, one row per field.
 // Capacity: what fits the LDS beside the tables (fewer than 64).
-template <class R> struct MicroPark {
-  static constexpr int CAP = micro_park_cap<R>();
-  double d[9][CAP];        // vart[7], cit, time
-  R s[32][CAP];            // StepState
+template <class R, bool HAIL = false> struct MicroPark {
+  static constexpr int NV = micro_nv<HAIL>(), NS = micro_ns<R, HAIL>();
+  static constexpr int CAP = micro_park_cap<R, HAIL>();
+  double d[NV + 2][CAP];   // vart[NV], cit, time
+  R s[NS][CAP];            // StepState (StepStateHail)
   unsigned int q[CAP];     // point index k * ncol + i
   int calls[CAP];
 };
 static_assert(sizeof(StepState<double>) == 32 * sizeof(double), "StepState layout");
 static_assert(sizeof(StepState<float>) == 32 * sizeof(float), "StepState layout");
+static_assert(sizeof(StepStateHail<double>) == 43 * sizeof(double), "StepStateHail layout");
 
-template <class R> constexpr size_t micro_lds_bytes() {
-  return micro_tab_bytes<R>() + micro_waves<R>() * sizeof(MicroPark<R>);
+template <class R, bool HAIL = false> constexpr size_t micro_lds_bytes() {
+  return micro_tab_bytes<R, HAIL>() + micro_waves<R, HAIL>() * sizeof(MicroPark<R, HAIL>);
 }
-static_assert(MicroPark<double>::CAP == 31 && MicroPark<float>::CAP == 63, "park capacity");
+static_assert(MicroPark<double>::CAP == 31 && MicroPark<float>::CAP == 63 &&
+              MicroPark<double, true>::CAP == 63, "park capacity");
 static_assert(micro_lds_bytes<double>() <= 160 * 1024, "k_micro<double> LDS");
 static_assert(micro_lds_bytes<float>() <= 160 * 1024, "k_micro<float> LDS");
+static_assert(micro_lds_bytes<double, true>() <= 160 * 1024, "k_micro<double, false, true> LDS");
 
-template <class R>
-__device__ __forceinline__ void park_store(MicroPark<R> &pk, int e, unsigned int q,
+template <class R, bool HAIL, class SS>
+__device__ __forceinline__ void park_store(MicroPark<R, HAIL> &pk, int e, unsigned int q,
                                            const double *vart, double cit, double time,
-                                           const StepState<R> &S, int calls) {
+                                           const SS &S, int calls) {
+  constexpr int NV = MicroPark<R, HAIL>::NV, NS = MicroPark<R, HAIL>::NS;
 #pragma unroll
-  for (int f = 0; f < 7; ++f) pk.d[f][e] = vart[f];
-  pk.d[7][e] = cit;
-  pk.d[8][e] = time;
+  for (int f = 0; f < NV; ++f) pk.d[f][e] = vart[f];
+  pk.d[NV][e] = cit;
+  pk.d[NV + 1][e] = time;
   const R *sv = reinterpret_cast<const R *>(&S);
 #pragma unroll
-  for (int f = 0; f < 32; ++f) pk.s[f][e] = sv[f];
+  for (int f = 0; f < NS; ++f) pk.s[f][e] = sv[f];
   pk.q[e] = q;
   pk.calls[e] = calls;
 }
 
-template <class R>
-__device__ __forceinline__ unsigned int park_load(const MicroPark<R> &pk, int e, double *vart,
-                                                  double &cit, double &time, StepState<R> &S,
-                                                  int &calls) {
+template <class R, bool HAIL, class SS>
+__device__ __forceinline__ unsigned int park_load(const MicroPark<R, HAIL> &pk, int e, double *vart,
+                                                  double &cit, double &time, SS &S, int &calls) {
+  constexpr int NV = MicroPark<R, HAIL>::NV, NS = MicroPark<R, HAIL>::NS;
 #pragma unroll
-  for (int f = 0; f < 7; ++f) vart[f] = pk.d[f][e];
-  cit = pk.d[7][e];
-  time = pk.d[8][e];
+  for (int f = 0; f < NV; ++f) vart[f] = pk.d[f][e];
+  cit = pk.d[NV][e];
+  time = pk.d[NV + 1][e];
   R *sv = reinterpret_cast<R *>(&S);
 #pragma unroll
-  for (int f = 0; f < 32; ++f) sv[f] = pk.s[f][e];
+  for (int f = 0; f < NS; ++f) sv[f] = pk.s[f][e];
   calls = pk.calls[e];
   return pk.q[e];
 }
 
-template <class R, bool OCND2 = false>
-__global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
+template <class R, bool OCND2 = false, bool HAIL = false>
+__global__ void __launch_bounds__((64 * micro_waves<R, HAIL>())) k_micro(MicroArgs a) {
   extern __shared__ __align__(16) unsigned char micro_lds[];
-  R *tab = reinterpret_cast<R *>(micro_lds);
-  for (int e = threadIdx.x; e < ICE3_T_ICE3; e += blockDim.x) tab[e] = (R)a.tab[e];
+  constexpr int NV = micro_nv<HAIL>();
+  R *lds_tab = reinterpret_cast<R *>(micro_lds);
+  if constexpr (!HAIL) {
+    for (int e = threadIdx.x; e < ICE3_T_ICE3; e += blockDim.x) lds_tab[e] = (R)a.tab[e];
+  }
   __syncthreads();
+  // KRR=6: the tables in LDS; hail: in device memory (a.tab, the hail block included)
+  const auto *tab = [&] {
+    if constexpr (HAIL) return a.tab;
+    else return lds_tab;
+  }();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  MicroPark<R> &pk = reinterpret_cast<MicroPark<R> *>(micro_lds + micro_tab_bytes<R>())[wid];
+  MicroPark<R, HAIL> &pk =
+      reinterpret_cast<MicroPark<R, HAIL> *>(micro_lds + micro_tab_bytes<R, HAIL>())[wid];
   const unsigned long long below = (1ull << lane) - 1ull;  // lanes under this one
   const int cpl = (a.ncol + 63) >> 6;                       // chunks per level
   const int nchunk = cpl * a.nlev;
@@ -522,47 +547,47 @@ __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
   bool active = false;            // this lane holds a point whose time step goes on
   unsigned int q = 0;             // that point: index k * ncol + i, offset j
   size_t j = 0;
-  StepPoint p;
-  double vart[7], cit = 0., time = 0.;
-  StepState<R> S;
+  StepPointN<NV> p;
+  double vart[NV], cit = 0., time = 0.;
+  typename StepStateFor<R, HAIL>::type S;
   int calls = 0;
   unsigned long long ncalls = 0, ncap = 0, nmicro = 0;
-  for (int c = blockIdx.x * micro_waves<R>() + wid;; c += gridDim.x * micro_waves<R>()) {
+  for (int c = blockIdx.x * micro_waves<R, HAIL>() + wid;; c += gridDim.x * micro_waves<R, HAIL>()) {
     const bool more = c < nchunk;  // wave-uniform
     if (more) {
       const int k = c / cpl, i = (c - k * cpl) * 64 + lane;
       if (i < a.ncol) {
         q = (unsigned int)k * (unsigned int)a.ncol + (unsigned int)i;
         j = (size_t)k * a.ld + i;
-        double t[7], s[7];
-        micro_load(a, j, t, s);
+        double t[NV], s[NV];
+        micro_load<NV>(a, j, t, s);
         cit = a.src_ci[j];
         double rvheni = 0., evap3d = 0.;
         int st = 0;
         calls = 0;
-        if (micro_mask<OCND2>(t, a, j)) {
+        if (micro_mask<OCND2, HAIL>(t, a, j)) {
           ++nmicro;
           step_point(dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j], s, t, p);
 #pragma unroll
-          for (int f = 0; f < 7; ++f) vart[f] = t[f];
+          for (int f = 0; f < NV; ++f) vart[f] = t[f];
           time = 0.;
           const PointConst<R> pc = point_const<R>(p.rhodref);
           MicroOcnd2 xo;
           if constexpr (OCND2) xo = ocnd2_point(a, j);
-          st = step_call<R, false, OCND2>(dt, p, pc, vart, cit, time, S, calls, DEFAULT_MAX_CALLS,
-                                          tab, &xo);
+          st = step_call<R, false, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls,
+                                                DEFAULT_MAX_CALLS, tab, &xo);
           if (st == 0) evap3d = (double)S.rrevav;
         } else {
-          micro_nucleation<R, OCND2>(a, j, t, cit, rvheni);
+          micro_nucleation<R, OCND2, HAIL>(a, j, t, cit, rvheni);
 #pragma unroll
-          for (int f = 0; f < 7; ++f) vart[f] = t[f];
+          for (int f = 0; f < NV; ++f) vart[f] = t[f];
         }
         if (st > 0) {
           active = true;
         } else {
           ncalls += calls;
           ncap += st < 0;
-          micro_finish(a, j, vart, cit, evap3d, rvheni, st < 0 ? -calls : calls);
+          micro_finish<HAIL>(a, j, vart, cit, evap3d, rvheni, st < 0 ? -calls : calls);
         }
       }
     }
@@ -571,28 +596,28 @@ __global__ void __launch_bounds__(64 * micro_waves<R>()) k_micro(MicroArgs a) {
     for (;;) {
       const unsigned long long am = __ballot(active);
       const int nact = __popcll(am), need = 64 - nact;
-      if (more ? parked < need && nact + parked <= MicroPark<R>::CAP : nact + parked == 0) break;
+      if (more ? parked < need && nact + parked <= MicroPark<R, HAIL>::CAP : nact + parked == 0) break;
       if (!active) {
         const int r = __popcll(~am & below);  // rank among the idle lanes
         if (r < parked) {
           q = park_load(pk, parked - 1 - r, vart, cit, time, S, calls);
           const unsigned int k = q / (unsigned int)a.ncol;
           j = (size_t)k * a.ld + (q - k * (unsigned int)a.ncol);
-          double t[7], s[7];
-          micro_load(a, j, t, s);
+          double t[NV], s[NV];
+          micro_load<NV>(a, j, t, s);
           step_point(dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j], a.sigs[j], s, t, p);
           active = true;
         }
       }
       parked -= need < parked ? need : parked;
       if (active) {
-        const int st = step_call<R, true, OCND2>(dt, p, pc_soft, vart, cit, time, S, calls,
-                                                 DEFAULT_MAX_CALLS, tab);
+        const int st = step_call<R, true, OCND2, HAIL>(dt, p, pc_soft, vart, cit, time, S, calls,
+                                                       DEFAULT_MAX_CALLS, tab);
         if (st <= 0) {
           ncalls += calls;
           ncap += st < 0;
-          micro_finish(a, j, vart, cit, st == 0 ? (double)S.rrevav : 0., 0.,
-                       st < 0 ? -calls : calls);
+          micro_finish<HAIL>(a, j, vart, cit, st == 0 ? (double)S.rrevav : 0., 0.,
+                             st < 0 ? -calls : calls);
           active = false;
         }
       }
@@ -1463,11 +1488,17 @@ extern "C" int ice3_ice_adjust_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, d
   return ICE3_OK;
 }
 
-// The hail microphysics: one thread per point (the sub-step cap is read at run time).
-static hipError_t launch_micro_hail(const MicroArgs &m, hipStream_t s) {
-  const unsigned int npts = (unsigned int)m.nlev * (unsigned int)m.ncol;
-  hipLaunchKernelGGL((k_micro_points<double, false, true>), dim3((npts + 255) / 256), dim3(256), 0,
-                     s, m);
+// The hail microphysics (FP64): one workgroup per compute unit, or the point kernel when the
+// context sets another sub-step cap (it reads the cap at run time).
+static hipError_t launch_micro_hail(const ice3_ctx *ctx, const MicroArgs &m, hipStream_t s) {
+  if (m.max_calls != DEFAULT_MAX_CALLS) {
+    const unsigned int npts = (unsigned int)m.nlev * (unsigned int)m.ncol;
+    hipLaunchKernelGGL((k_micro_points<double, false, true>), dim3((npts + 255) / 256), dim3(256),
+                       0, s, m);
+  } else {
+    hipLaunchKernelGGL((k_micro<double, false, true>), dim3(ctx->ncu),
+                       dim3(64 * (micro_waves<double, true>())), (micro_lds_bytes<double, true>()), s, m);
+  }
   return hipGetLastError();
 }
 
@@ -1501,7 +1532,7 @@ extern "C" int ice3_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, dou
     hipLaunchKernelGGL(k_sedim_hail, dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
   HIPCHK(hipGetLastError());
   MARK(2);
-  HIPCHK(launch_micro_hail(m, s));
+  HIPCHK(launch_micro_hail(ctx, m, s));
   MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;
 }
@@ -1540,7 +1571,7 @@ extern "C" int ice3_adjust_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int 
     MARK(1);
   }
   MARK(2);
-  HIPCHK(launch_micro_hail(m, s));
+  HIPCHK(launch_micro_hail(ctx, m, s));
   MARK(3); MARK(4); MARK(5); MARK(6);
   return ICE3_OK;
 }
