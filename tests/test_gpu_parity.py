@@ -647,17 +647,21 @@ def test_pipeline_ocnd2_full_size(dev, full512):
           f"of {got['calls'].size}, worst {max(w.values()):.2e}")
 
 
-@pytest.mark.parametrize("ocnd2", [False, True])
-def test_persistent_and_point_kernels_bitwise(dev, ocnd2):
+@pytest.mark.parametrize("kind", ["plain", "ocnd2", "hail"])
+def test_persistent_and_point_kernels_bitwise(dev, kind):
     """The persistent microphysics kernel (64-point chunks, LDS parking) and the one-thread-
     per-point kernel (taken when a context sets a sub-step cap other than the default; 4095
-    is never reached here) give the same results bit for bit, plain and OCND2."""
-    import ctypes
-    from ice3_amd import AdjustRainIce, AdjustRainIceOCND2, _lib, synthetic
+    is never reached here) give the same results bit for bit, plain, OCND2 and with hail
+    (KRR=7: rh = rg U(0.2, 2) and its source rh / dt)."""
+    from ice3_amd import AdjustRainIce, AdjustRainIceHail, AdjustRainIceOCND2, _lib, synthetic
     lib, ctx = _lib.load(), _lib.context(0)
     f = _with_heights(synthetic.generate(1500, 90, seed=20260051, variant="pipeline"), 20260052)
     f["rhodref"] = f["rho_dry_ref"]
-    step = AdjustRainIceOCND2() if ocnd2 else AdjustRainIce()
+    if kind == "hail":
+        rng = np.random.Generator(np.random.Philox(20260053))
+        f["rh"] = np.ascontiguousarray(f["rg"] * rng.uniform(0.2, 2.0, f["rg"].shape))
+        f["rhs"] = f["rh"] / 50.0
+    step = {"plain": AdjustRainIce, "ocnd2": AdjustRainIceOCND2, "hail": AdjustRainIceHail}[kind]()
     a = step(f, 50.0, return_calls=True)
     assert lib.ice3_ctx_set_substep_cap(ctx, 4095) == _lib.ICE3_OK
     try:
