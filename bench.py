@@ -333,8 +333,8 @@ def main():
         res.setdefault("variants", {})["split_sedimentation"] = {
             "columns_per_s": ncol / (ms_s * 1e-3), "ms_per_step": ms_s, "speedup_vs_stat": ms / ms_s,
             "stages_ms": st_s, "water_budget_max_rel_err": budget_err(outs),
-            "note": "fused step with CSEDIM='SPLI': k_ice_adjust, the state at t of th/rv/rc kept, "
-                    "k_sedim_split (FP64), k_micro"}
+            "note": "fused step with CSEDIM='SPLI': k_ice_adjust, k_sedim_split (FP64; it also keeps "
+                    "the state at t of th/rv/rc for the microphysics), k_micro"}
         progress(f"SPLI step timed: {ms_s:.3f} ms median")
         del outs
     if not args.no_configs:

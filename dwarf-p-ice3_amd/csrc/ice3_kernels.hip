@@ -204,8 +204,9 @@ __device__ __forceinline__ int split_group(int b, int nblk) {
 
 // Stage NF fields (field f times scale[f]) of the workgroup's SPLIT_COLS columns from c0 on
 // through LDS: v[f][q] of this lane receives level lane + 64 q of its wavefront's column (0
-// outside the domain).  Every thread of the workgroup calls it.
-template <int L, int NF>
+// outside the domain).  Every thread of the workgroup calls it.  With KEEP, fields 1 to 3 are
+// also stored as read into a.keep[0..2] (when set), in the same rows.
+template <int L, int NF, bool KEEP = false>
 __device__ __forceinline__ void split_load(double (*tile)[L * 64][SPLIT_COLS + 1],
                                            const SplitArgs &a, const double *const (&f)[NF],
                                            const double (&scale)[NF], int c0, double (&v)[NF][L]) {
@@ -232,6 +233,15 @@ __device__ __forceinline__ void split_load(double (*tile)[L * 64][SPLIT_COLS + 1
       const int e = threadIdx.x + p * 64 * SPLIT_COLS, k = e / SPLIT_COLS;
       tile[g][k][e - k * SPLIT_COLS] = in[p] ? x[g][p] * scale[g] : 0.;
     }
+  if constexpr (KEEP) {
+    if (a.keep[0]) {  // uniform
+#pragma unroll
+      for (int g = 0; g < 3; ++g)
+#pragma unroll
+        for (int p = 0; p < L; ++p)
+          if (in[p]) a.keep[g][idx[p]] = x[g + 1][p];
+    }
+  }
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
@@ -324,7 +334,7 @@ __global__ void __launch_bounds__(64 * SPLIT_COLS)
   {
     const double *const f1[8] = {a.exn, a.t[0], a.t[1], a.t[2], a.t[3], a.t[4], a.t[5], a.t[6]};
     const double s1[8] = {1., ts, ts, ts, ts, ts, ts, ts};
-    split_load<L, 8>(tile, a, f1, s1, c0, b1);
+    split_load<L, 8, true>(tile, a, f1, s1, c0, b1);
   }
   auto factors = [&](const double (&rh_t)[L]) {
 #pragma unroll
@@ -1240,8 +1250,10 @@ static void launch_split_l(const SplitArgs &a, hipStream_t s) {
 // (the state at t and the th, rv, rc sources); sa.zzz selects the OCND2 model level height,
 // sa.src_rh hail.  nlev <= SPLIT_MAX_LEVELS (checked by the entry points).
 static hipError_t launch_split(const ice3_ctx *ctx, const SedimArgs &sa, const MicroArgs &m,
-                               hipStream_t s) {
-  const SplitArgs a = split_args(sa, m, ctx->maxcfl, ctx->d_capped);
+                               hipStream_t s, double *const *keep = nullptr) {
+  SplitArgs a = split_args(sa, m, ctx->maxcfl, ctx->d_capped);
+  if (keep)
+    for (int q = 0; q < 3; ++q) a.keep[q] = keep[q];
   if (sa.src_rh) launch_split_l<true, false>(a, s);
   else if (sa.zzz) launch_split_l<false, true>(a, s);
   else launch_split_l<false, false>(a, s);
@@ -1249,11 +1261,12 @@ static hipError_t launch_split(const ice3_ctx *ctx, const SedimArgs &sa, const M
 }
 
 // The fused step with the split sedimentation: its negativity correction updates the th, rv
-// and rc sources, whose values before it, times dt, are the microphysics' state at t.  They
-// are copied to the context's work area (3 fields, grown on demand; growing it waits for the
-// device), and m reads its state at t there.
-static int keep_tstate(ice3_ctx *ctx, int ncol, int nlev, int ld, MicroArgs &m, hipStream_t s) {
-  const size_t n = (size_t)nlev * ld, used = (size_t)(nlev - 1) * ld + ncol;
+// and rc sources, whose values before it, times dt, are the microphysics' state at t.  The
+// split kernel stores them as it reads them into the context's work area (3 fields, grown on
+// demand; growing it waits for the device), and m then reads its state at t there.
+static int launch_split_keep(ice3_ctx *ctx, int nlev, int ld, const SedimArgs &sa, MicroArgs &m,
+                             hipStream_t s) {
+  const size_t n = (size_t)nlev * ld;
   const size_t bytes = 3 * n * sizeof(double);
   if (bytes > ctx->work_bytes) {
     HIPCHK(hipDeviceSynchronize());
@@ -1264,10 +1277,9 @@ static int keep_tstate(ice3_ctx *ctx, int ncol, int nlev, int ld, MicroArgs &m, 
     ctx->work_bytes = bytes;
   }
   double *w = (double *)ctx->d_work;
-  for (int q = 0; q < 3; ++q) {
-    HIPCHK(hipMemcpyAsync(w + q * n, m.src[q], used * sizeof(double), hipMemcpyDeviceToDevice, s));
-    m.t[q] = w + q * n;
-  }
+  double *const keep[3] = {w, w + n, w + 2 * n};
+  HIPCHK(launch_split(ctx, sa, m, s, keep));
+  for (int q = 0; q < 3; ++q) m.t[q] = keep[q];
   return ICE3_OK;
 }
 
@@ -1397,11 +1409,11 @@ extern "C" int ice3_adjust_rain_ice_stages(ice3_ctx *ctx, int prec, int ncol, in
   MARK(1);
   if (stages & ICE3_STAGE_SEDIMENTATION) {
     if (split && (stages & ICE3_STAGE_MICROPHYSICS)) {
-      const int rc = keep_tstate(ctx, ncol, nlev, ld, m, s);
+      const int rc = launch_split_keep(ctx, nlev, ld, sa, m, s);
       if (rc != ICE3_OK) return rc;
-    }
-    if (split) HIPCHK(launch_split(ctx, sa, m, s));
-    else if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
+    } else if (split) {
+      HIPCHK(launch_split(ctx, sa, m, s));
+    } else if (prec == ICE3_PREC_FP64) hipLaunchKernelGGL(k_sedim<double>, gs, dim3(256), 0, s, sa);
     else hipLaunchKernelGGL(k_sedim<float>, gs, dim3(256), 0, s, sa);
     HIPCHK(hipGetLastError());
   }
@@ -1450,9 +1462,8 @@ extern "C" int ice3_adjust_rain_ice_ocnd2(ice3_ctx *ctx, int ncol, int nlev, int
   HIPCHK(hipGetLastError());
   MARK(1);
   if (ctx->sedim == ICE3_SEDIM_SPLI) {
-    const int rc = keep_tstate(ctx, ncol, nlev, ld, m, s);
+    const int rc = launch_split_keep(ctx, nlev, ld, sa, m, s);
     if (rc != ICE3_OK) return rc;
-    HIPCHK(launch_split(ctx, sa, m, s));
   } else {
     hipLaunchKernelGGL((k_sedim<double, true>), dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
   }
@@ -1562,9 +1573,8 @@ extern "C" int ice3_adjust_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int 
     hipLaunchKernelGGL(k_ice_adjust_hail, dim3((ncol + 255) / 256, nlev), dim3(256), 0, s, aa);
     HIPCHK(hipGetLastError());
     MARK(1);
-    const int rc = keep_tstate(ctx, ncol, nlev, ld, m, s);
+    const int rc = launch_split_keep(ctx, nlev, ld, sa, m, s);
     if (rc != ICE3_OK) return rc;
-    HIPCHK(launch_split(ctx, sa, m, s));
   } else {
     hipLaunchKernelGGL(k_adjust_sedim_hail, dim3((ncol + 63) / 64), dim3(64), 0, s, aa, sa);
     HIPCHK(hipGetLastError());

@@ -113,6 +113,7 @@ struct SplitArgs {
   double *inprc, *inprr, *inpri, *inprs, *inprg, *inprh;
   double *zzz;           // OCND2 branch: receives the model level height ZZZZ
   unsigned long long *capped;  // columns whose species hit the sub-step cap
+  double *keep[3];       // optional: receive t[0..2] as read (th, rv, rc at t, unscaled)
 };
 
 // ICE4_SEDIMENTATION_STAT (micro/mode_ice4_sedimentation_stat.F90:143-229) for column i,
