@@ -456,12 +456,12 @@ __global__ void __launch_bounds__(64 * SPLIT_COLS)
 // Wavefronts per workgroup.  The mixed path (183 VGPRs at two wavefronts per SIMD) measured
 // slower with three (168 VGPRs, spills, 49-slot parks: +22 %) or four (+120 %) per SIMD.
 // The hail state needs about 306 registers: one wavefront per SIMD (at two, 55 registers
-// spill and the kernel is slower than one thread per point), with 63-slot parks.
+// spill and the kernel is slower than one thread per point), with 49-slot parks.
 template <class R, bool HAIL = false> constexpr int micro_waves() { return HAIL ? 4 : 8; }
-// Lookup tables staged in LDS: the ICE3 block (KRR=6); the hail kernel reads its larger
-// block (the wet-growth kernels of hail) from device memory and gives the LDS to its parks.
+// Lookup tables staged in LDS: the ICE3 block; the hail kernel reads the wet-growth kernels
+// of hail, which follow it, from device memory.
 template <class R, bool HAIL = false> constexpr size_t micro_tab_bytes() {
-  return HAIL ? 0 : ((size_t)ICE3_T_ICE3 * sizeof(R) + 15) / 16 * 16;
+  return ((size_t)ICE3_T_ICE3 * sizeof(R) + 15) / 16 * 16;
 }
 // Variables of the time splitting (KRR + 1) and rates kept between tendency calls.
 template <bool HAIL> constexpr int micro_nv() { return HAIL ? 8 : 7; }
@@ -469,7 +469,7 @@ template <class R, bool HAIL> constexpr int micro_ns() {
   return (int)(sizeof(typename StepStateFor<R, HAIL>::type) / sizeof(R));
 }
 // Park slots per wavefront: what the LDS holds beside the tables, at most 63 (31 in FP64,
-// 63 in the mixed path, 47 with hail).
+// 63 in the mixed path, 49 with hail).
 template <class R, bool HAIL = false> constexpr int micro_park_cap() {
   const size_t slot = (micro_nv<HAIL>() + 2) * sizeof(double) + micro_ns<R, HAIL>() * sizeof(R) +
                       2 * sizeof(int);
@@ -495,7 +495,7 @@ template <class R, bool HAIL = false> constexpr size_t micro_lds_bytes() {
   return micro_tab_bytes<R, HAIL>() + micro_waves<R, HAIL>() * sizeof(MicroPark<R, HAIL>);
 }
 static_assert(MicroPark<double>::CAP == 31 && MicroPark<float>::CAP == 63 &&
-              MicroPark<double, true>::CAP == 63, "park capacity");
+              MicroPark<double, true>::CAP == 49, "park capacity");
 static_assert(micro_lds_bytes<double>() <= 160 * 1024, "k_micro<double> LDS");
 static_assert(micro_lds_bytes<float>() <= 160 * 1024, "k_micro<float> LDS");
 static_assert(micro_lds_bytes<double, true>() <= 160 * 1024, "k_micro<double, false, true> LDS");
@@ -535,16 +535,11 @@ template <class R, bool OCND2 = false, bool HAIL = false>
 __global__ void __launch_bounds__((64 * micro_waves<R, HAIL>())) k_micro(MicroArgs a) {
   extern __shared__ __align__(16) unsigned char micro_lds[];
   constexpr int NV = micro_nv<HAIL>();
-  R *lds_tab = reinterpret_cast<R *>(micro_lds);
-  if constexpr (!HAIL) {
-    for (int e = threadIdx.x; e < ICE3_T_ICE3; e += blockDim.x) lds_tab[e] = (R)a.tab[e];
-  }
+  R *tab = reinterpret_cast<R *>(micro_lds);
+  for (int e = threadIdx.x; e < ICE3_T_ICE3; e += blockDim.x) tab[e] = (R)a.tab[e];
   __syncthreads();
-  // KRR=6: the tables in LDS; hail: in device memory (a.tab, the hail block included)
-  const auto *tab = [&] {
-    if constexpr (HAIL) return a.tab;
-    else return lds_tab;
-  }();
+  // hail: the wet-growth kernels of hail from device memory
+  const double *htab = HAIL ? a.tab + ICE3_T_ICE3 : nullptr;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   MicroPark<R, HAIL> &pk =
       reinterpret_cast<MicroPark<R, HAIL> *>(micro_lds + micro_tab_bytes<R, HAIL>())[wid];
@@ -585,7 +580,7 @@ __global__ void __launch_bounds__((64 * micro_waves<R, HAIL>())) k_micro(MicroAr
           MicroOcnd2 xo;
           if constexpr (OCND2) xo = ocnd2_point(a, j);
           st = step_call<R, false, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls,
-                                                DEFAULT_MAX_CALLS, tab, &xo);
+                                                DEFAULT_MAX_CALLS, tab, &xo, htab);
           if (st == 0) evap3d = (double)S.rrevav;
         } else {
           micro_nucleation<R, OCND2, HAIL>(a, j, t, cit, rvheni);
@@ -622,7 +617,7 @@ __global__ void __launch_bounds__((64 * micro_waves<R, HAIL>())) k_micro(MicroAr
       parked -= need < parked ? need : parked;
       if (active) {
         const int st = step_call<R, true, OCND2, HAIL>(dt, p, pc_soft, vart, cit, time, S, calls,
-                                                       DEFAULT_MAX_CALLS, tab);
+                                                       DEFAULT_MAX_CALLS, tab, nullptr, htab);
         if (st <= 0) {
           ncalls += calls;
           ncap += st < 0;

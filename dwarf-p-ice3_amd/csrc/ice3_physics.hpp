@@ -528,14 +528,16 @@ template <class R> __device__ __forceinline__ R powl_(R lnx, R c) { return rexp<
 // HAIL (KRR=7, HCLOUD='ICE4'; S is then a StepStateHail): pvart, pa and pb have an eighth
 // entry, hail; the hail slope parameter (:426-432), the conversion of wet-growing graupel
 // into hail in ICE4_FAST_RG (micro/mode_ice4_fast_rg.F90:330-351), ICE4_FAST_RH
-// (micro/mode_ice4_fast_rh.F90) and their PA / PB terms (:619-716).
+// (micro/mode_ice4_fast_rh.F90) and their PA / PB terms (:619-716); htab then holds the
+// wet-growth kernels of hail (the block that follows the ICE3 tables, ICE3_T_ICE3 on).
 template <class R, bool OCND2 = false, bool HAIL = false, class TB, class SS>
 __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref, double lvfact,
                                   double lsfact, double pres, double cf, double sigma_rc,
                                   double &cit, double pt, const double *pvart,
                                   SS &S, const PointConst<R> &pc,
                                   const TB *__restrict__ tab, double *pa, double *pb,
-                                  const MicroOcnd2 *x = nullptr) {
+                                  const MicroOcnd2 *x = nullptr,
+                                  const double *__restrict__ htab = nullptr) {
   static_assert(!OCND2 || sizeof(R) == sizeof(double), "the OCND2 branch is FP64");
   static_assert(!(OCND2 && HAIL), "OCND2 with hail is not built");
   constexpr int NV = HAIL ? 8 : 7;
@@ -952,7 +954,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     if (!soft && rh_ok && rs_ok) {
       const Idx2<R> ix = interp2d_idx<R>(llh, lls, P.NWETLBDAH, P.NWETLBDAS, (R)P.XWETINTP1H,
                                          (R)P.XWETINTP2H, (R)P.XWETINTP1S, (R)P.XWETINTP2S);
-      const R zzw = interp2d<R>(ix, P.NWETLBDAH, tab + ICE3_T_SWETH);
+      const R zzw = interp2d<R>(ix, P.NWETLBDAH, htab + (ICE3_T_SWETH - ICE3_T_ICE3));
       S.rh_rsweth = (R)P.XFSWETH * zzw * lbs_cxs_bs * lbh_cxh * pc.rhom *
                     ((R)P.XLBSWETH1 / (sq(lbdah)) + (R)P.XLBSWETH2 / (lbdah * lbdas) + (R)P.XLBSWETH3 / (sq(lbdas)));
       S.rh_rsdryh = S.rh_rsweth * ((R)P.XCOLSH * rexp<R>((R)P.XCOLEXSH * (t - (R)P.XTT)));
@@ -961,7 +963,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     if (!soft && rh_ok && rg_ok) {
       const Idx2<R> ix = interp2d_idx<R>(llh, llg, P.NWETLBDAH, P.NWETLBDAG, (R)P.XWETINTP1H,
                                          (R)P.XWETINTP2H, (R)P.XWETINTP1G, (R)P.XWETINTP2G);
-      const R zzw = interp2d<R>(ix, P.NWETLBDAH, tab + ICE3_T_GWETH);
+      const R zzw = interp2d<R>(ix, P.NWETLBDAH, htab + (ICE3_T_GWETH - ICE3_T_ICE3));
       S.rh_rgweth = (R)P.XFGWETH * zzw * powl_<R>(llg, (R)(P.XCXG - P.XBG)) * lbh_cxh * pc.rhom *
                     ((R)P.XLBGWETH1 / (sq(lbdah)) + (R)P.XLBGWETH2 / (lbdah * lbdag) + (R)P.XLBGWETH3 / (sq(lbdag)));
       S.rh_rgdryh = S.rh_rgweth;
@@ -972,7 +974,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
     if (!soft && rh_ok && rr_ok) {
       const Idx2<R> ix = interp2d_idx<R>(llh, llr, P.NWETLBDAH, P.NWETLBDAR, (R)P.XWETINTP1H,
                                          (R)P.XWETINTP2H, (R)P.XWETINTP1R, (R)P.XWETINTP2R);
-      const R zzw = interp2d<R>(ix, P.NWETLBDAH, tab + ICE3_T_RWETH);
+      const R zzw = interp2d<R>(ix, P.NWETLBDAH, htab + (ICE3_T_RWETH - ICE3_T_ICE3));
       const R lr2 = lbdar * lbdar;
       S.rh_rrweth = (R)P.XFRWETH * zzw * ((R)1.0 / (lr2 * lr2)) * lbh_cxh * pc.rhom *
                     ((R)P.XLBRWETH1 / (sq(lbdah)) + (R)P.XLBRWETH2 / (lbdah * lbdar) + (R)P.XLBRWETH3 / (lr2));
@@ -1231,7 +1233,8 @@ __device__ __forceinline__ int step_call(double dt, const StepPointN<NV> &p, con
                                          double *vart, double &cit, double &time_,
                                          SS &S, int &calls, int max_calls,
                                          const TB *__restrict__ tab,
-                                         const MicroOcnd2 *x = nullptr) {
+                                         const MicroOcnd2 *x = nullptr,
+                                         const double *__restrict__ htab = nullptr) {
   static_assert(NV == (HAIL ? 8 : 7), "variables of the time splitting");
   if (++calls > max_calls) {
 #pragma unroll
@@ -1248,7 +1251,7 @@ __device__ __forceinline__ int step_call(double dt, const StepPointN<NV> &p, con
   const double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
   const double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
   tendencies<R, OCND2, HAIL>(SOFT, exn, p.rhodref, lvfact, lsfact, p.pres, p.cf, p.sigma_rc, cit,
-                             zzt, vart, S, pc, tab, za, zb, x);
+                             zzt, vart, S, pc, tab, za, zb, x, htab);
 #pragma unroll
   for (int jv = 0; jv < NV; ++jv) za[jv] = za[jv] + p.extpk[jv];
   const double time = time_;
@@ -1286,16 +1289,19 @@ __device__ __forceinline__ int stepping_point(double dt, double exn, double rhod
                                               double cf, double sigs, double &cit, double *vart,
                                               const double *s, double &rrevav, int max_calls,
                                               const TB *__restrict__ tab,
-                                              const MicroOcnd2 *x = nullptr) {
+                                              const MicroOcnd2 *x = nullptr,
+                                              const double *__restrict__ htab = nullptr) {
   StepPointN<HAIL ? 8 : 7> p;
   step_point(dt, exn, rhodref, pres, cf, sigs, s, vart, p);
   typename StepStateFor<R, HAIL>::type S;
   double time = 0.;
   int calls = 0;
   const PointConst<R> pc = point_const<R>(rhodref);
-  int st = step_call<R, false, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x);
+  int st = step_call<R, false, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x,
+                                            htab);
   while (st > 0)
-    st = step_call<R, true, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x);
+    st = step_call<R, true, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x,
+                                         htab);
   if (st == 0) rrevav = (double)S.rrevav;
   return st < 0 ? -calls : calls;
 }

@@ -363,7 +363,8 @@ __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j, const TB *
     MicroOcnd2 x;
     if constexpr (OCND2) x = ocnd2_point(a, j);
     calls = stepping_point<R, OCND2, HAIL>(a.dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j],
-                                           a.sigs[j], cit, wr, s, evap3d, a.max_calls, tab, &x);
+                                           a.sigs[j], cit, wr, s, evap3d, a.max_calls, tab, &x,
+                                           HAIL ? a.tab + ICE3_T_ICE3 : nullptr);
   } else {
     micro_nucleation<R, OCND2, HAIL>(a, j, wr, cit, rvheni);
   }
