@@ -55,6 +55,6 @@ def test_no_calls_no_flat_access(src, tmp_path):
     for name, desc in ks.items():
         assert ".amdhsa_uses_dynamic_stack 0" in desc, name
         frame = int(re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", desc).group(1))
-        # register spills only: 24 bytes per lane in k_micro, at most 68 in the split
-        # sedimentation's instances for 150 to 256 levels with hail
+        # register spills only: 24 bytes per lane in k_micro, at most 12 in the split
+        # sedimentation's instances (the one for 193 to 256 levels with hail)
         assert frame <= 128, (name, frame)
