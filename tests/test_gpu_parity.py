@@ -324,6 +324,67 @@ def test_fused_layout_and_stages_bitwise(dev):
             assert np.array_equal(np.nan_to_num(other[k], nan=1.0), np.nan_to_num(dense[k], nan=1.0)), (ld, stages, k)
 
 
+def _fused_hail_raw(lib, ctx, dev, f, ncol, nlev, ld):
+    """The fused KRR=7 step (ice3_adjust_rain_ice_hail) through the C ABI on fields padded to a
+    row stride ld >= ncol."""
+    import ctypes
+    from ice3_amd import _lib
+    from ice3_amd._fields import ptr
+
+    def padded(v, fill):
+        v = torch.as_tensor(v, dtype=torch.float64, device=dev)
+        if v.dim() == 2:
+            pad = torch.full((nlev, ld), fill, dtype=torch.float64, device=dev)
+            pad[:, :ncol] = v
+            v = pad
+        return v.contiguous()
+    g = {k: padded(f[k], -1.0) for k in _lib.PIPE_IN_FIELDS}
+    rh, rhs = padded(f["rh"], -1.0), padded(f["rhs"], -1.0)
+    o = {k: torch.full((nlev, ld), -7.0, dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[:15]}
+    o.update({k: torch.full((ncol,), -7.0, dtype=torch.float64, device=dev) for k in _lib.PIPE_OUT_FIELDS[15:]})
+    o["rhs"] = torch.full((nlev, ld), -7.0, dtype=torch.float64, device=dev)
+    o["inprh"] = torch.full((ncol,), -7.0, dtype=torch.float64, device=dev)
+    c = torch.zeros(nlev * ld, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(3, dtype=torch.int64, device=dev)
+    st = _lib.Stats(ptr(c), ptr(cnt))
+    cin = _lib.PipeIn(*[ptr(g[k]) for k in _lib.PIPE_IN_FIELDS])
+    cout = _lib.PipeOut(*[ptr(o[k]) for k in _lib.PIPE_OUT_FIELDS])
+    hail = _lib.PipeHail(ptr(rh), ptr(rhs), ptr(o["rhs"]), ptr(o["inprh"]))
+    sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    rc = lib.ice3_adjust_rain_ice_hail(ctx, ncol, nlev, ld, 50.0, ctypes.byref(cin), ctypes.byref(cout),
+                                       ctypes.byref(hail), ctypes.byref(st), sp)
+    assert rc == 0, rc
+    torch.cuda.synchronize()
+    res = {k: (v[:, :ncol] if v.dim() == 2 else v).cpu().numpy() for k, v in o.items()}
+    res["calls"] = c.view(nlev, ld)[:, :ncol].cpu().numpy()
+    res["counters"] = cnt.cpu().numpy()
+    return res
+
+
+@pytest.mark.parametrize("ncol,level", [(1000, None), (333, 1)])
+def test_fused_hail_layout_bitwise(dev, ncol, level):
+    """The persistent hail microphysics (64-column chunks, parks) with a row stride ld > ncol,
+    a partial last chunk of each level and a single level (one level of the 12-level
+    generator) gives the dense call's results bit for bit (KRR=7: rh = rg U(0.2, 2), its
+    source rh / dt)."""
+    from ice3_amd import _lib, synthetic
+    lib, ctx = _lib.load(), _lib.context(0)
+    f = synthetic.generate(ncol, 12, seed=20260061, variant="pipeline")
+    if level is not None:
+        f = {k: np.ascontiguousarray(v[level:level + 1]) if v.ndim == 2 else v for k, v in f.items()}
+    nlev = f["th"].shape[0]
+    f["rhodref"] = f["rho_dry_ref"]
+    rng = np.random.Generator(np.random.Philox(20260062))
+    f["rh"] = np.ascontiguousarray(f["rg"] * rng.uniform(0.2, 2.0, f["rg"].shape))
+    f["rhs"] = f["rh"] / 50.0
+    dense = _fused_hail_raw(lib, ctx, dev, f, ncol, nlev, ncol)
+    assert dense["counters"][1] == 0 and (dense["calls"] > 1).any()
+    for ld in (ncol + 3, ncol + 64):
+        other = _fused_hail_raw(lib, ctx, dev, f, ncol, nlev, ld)
+        for k in dense:
+            assert np.array_equal(np.nan_to_num(other[k], nan=1.0), np.nan_to_num(dense[k], nan=1.0)), (ld, k)
+
+
 def test_nonfinite_inputs(dev):
     """ice3_count_nonfinite counts NaN and infinities per field (and nothing in the row
     padding beyond ncol); the drop-ins' check_inputs=True names the fields that hold any
