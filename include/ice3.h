@@ -60,7 +60,10 @@ const char *ice3_strerror(int code);
  * outputs are then the processes integrated up to the time reached, with the external
  * tendencies counted once (through the sources).  Every such point is counted in a
  * context-owned device counter, whatever the stats argument of the call.
- * ice3_ctx_set_substep_cap sets max_calls (>= 1) for the calls made after it.
+ * ice3_ctx_set_substep_cap sets max_calls (>= 1) for the calls made after it.  The
+ * microphysics kernels that compact the time splitting across a wavefront have the default
+ * compiled in; under another cap the microphysics runs one thread per point, with the same
+ * results, more slowly.
  * ice3_ctx_status waits for the work queued on stream, returns in *capped (may be NULL)
  * the number of capped points of all calls on ctx since the previous query, resets the
  * counter, and returns ICE3_ESUBSTEP if that number is not 0, else ICE3_OK.  Calls on
