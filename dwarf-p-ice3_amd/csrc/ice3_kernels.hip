@@ -679,6 +679,19 @@ __global__ void __launch_bounds__(256) k_micro_points(MicroArgs a) {
   }
 }
 
+// The microphysics one thread per point with the branch record of each point (the test
+// hook of ice3_rain_ice_branches; the counters of ice3_stats are not kept).
+template <bool HAIL>
+__global__ void __launch_bounds__(256) k_micro_branches(MicroArgs a) {
+  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (unsigned int)a.nlev * (unsigned int)a.ncol) return;
+  const unsigned int k = q / (unsigned int)a.ncol;
+  const size_t j = (size_t)k * a.ld + (q - k * (unsigned int)a.ncol);
+  unsigned br = 0;
+  micro_at<double, false, HAIL>(a, j, a.tab, &br);
+  a.branches[j] = br;
+}
+
 struct SurrogateArgs {
   MicroArgs m;
   unsigned int npts;                 // nlev * ncol; point q = k * ncol + i
@@ -1506,6 +1519,48 @@ static hipError_t launch_micro_hail(const ice3_ctx *ctx, const MicroArgs &m, hip
                        dim3(64 * (micro_waves<double, true>())), (micro_lds_bytes<double, true>()), s, m);
   }
   return hipGetLastError();
+}
+
+extern "C" int ice3_rain_ice_branches(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                      const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
+                                      const ice3_rain_ice_out *out, const ice3_hail *hail,
+                                      unsigned *branches, void *stream) {
+  if (!ctx || !in || !io || !out || !branches) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev > 65535) return ICE3_EINVAL;
+  NN(in->exn); NN(in->dzz); NN(in->rhodref); NN(in->exn_ref); NN(in->pabs); NN(in->cldfr);
+  NN(in->sigs); NN(in->tht); NN(in->rvt); NN(in->rct); NN(in->rrt); NN(in->rit); NN(in->rst);
+  NN(in->rgt);
+  NN(io->ths); NN(io->rvs); NN(io->rcs); NN(io->rrs); NN(io->ris); NN(io->rss); NN(io->rgs);
+  NN(io->ci);
+  NN(out->evap3d); NN(out->rainfr); NN(out->inprc); NN(out->inprr); NN(out->inpri);
+  NN(out->inprs); NN(out->inprg);
+  if (hail) { NN(hail->rht); NN(hail->rhs); NN(hail->inprh); }
+  if (!sedim_fits(ctx, nlev)) return ICE3_EINVAL;
+  SedimArgs sa;
+  MicroArgs m;
+  rain_ice_args(ncol, nlev, ld, dt, ctx->d_tab, in, io, out, nullptr, sa, m);
+  m.max_calls = ctx->max_calls; m.capped = ctx->d_capped;
+  m.branches = branches;
+  if (hail) {
+    sa.src_rh = hail->rhs; sa.dst_rh = hail->rhs; sa.inprh = hail->inprh;
+    m.t[IRH] = hail->rht; m.src[IRH] = hail->rhs; m.dst[IRH] = hail->rhs;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (ctx->sedim == ICE3_SEDIM_SPLI)
+    HIPCHK(launch_split(ctx, sa, m, s));
+  else if (hail)
+    hipLaunchKernelGGL(k_sedim_hail, dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
+  else
+    hipLaunchKernelGGL(k_sedim<double>, dim3((ncol + 255) / 256), dim3(256), 0, s, sa);
+  HIPCHK(hipGetLastError());
+  const unsigned int npts = (unsigned int)nlev * (unsigned int)ncol;
+  if (hail)
+    hipLaunchKernelGGL(k_micro_branches<true>, dim3((npts + 255) / 256), dim3(256), 0, s, m);
+  else
+    hipLaunchKernelGGL(k_micro_branches<false>, dim3((npts + 255) / 256), dim3(256), 0, s, m);
+  HIPCHK(hipGetLastError());
+  return ICE3_OK;
 }
 
 extern "C" int ice3_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,

@@ -537,7 +537,8 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
                                   SS &S, const PointConst<R> &pc,
                                   const TB *__restrict__ tab, double *pa, double *pb,
                                   const MicroOcnd2 *x = nullptr,
-                                  const double *__restrict__ htab = nullptr) {
+                                  const double *__restrict__ htab = nullptr,
+                                  unsigned *br = nullptr) {
   static_assert(!OCND2 || sizeof(R) == sizeof(double), "the OCND2 branch is FP64");
   static_assert(!(OCND2 && HAIL), "OCND2 with hail is not built");
   constexpr int NV = HAIL ? 8 : 7;
@@ -560,6 +561,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       rrhong = dmin(zv[IRR], dmax(0., ((P.XTT - 35.) / exn - zv[ITH]) / (lsfact - lvfact)));
     else
       rrhong = 0.;
+    if (br && rrhong > 0.) *br |= 1u << ICE3_BR_RRHONG;
     zv[ITH] = zv[ITH] + rrhong * (lsfact - lvfact);
     zt = zv[ITH] * exn;
     zv[IRR] = zv[IRR] - rrhong;
@@ -570,6 +572,7 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       rimltc = dmin(zv[IRI], dmax(0., (zv[ITH] - P.XTT / exn) / (lsfact - lvfact)));
     else
       rimltc = 0.;
+    if (br && rimltc > 0.) *br |= 1u << ICE3_BR_RIMLTC;
     zv[ITH] = zv[ITH] - rimltc * (lsfact - lvfact);
     zt = zv[ITH] * exn;
     zv[IRC] = zv[IRC] + rimltc;
@@ -1011,6 +1014,9 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
       rgdryh = S.rh_rgdryh;
       rdryhg = rdryh_init * rweth_init / (rdryh_init + rweth_init);
     }
+    if (br)
+      *br |= (llweth ? 1u << ICE3_BR_WETH : 0u) | (lldryh ? 1u << ICE3_BR_DRYH : 0u) |
+             (rdryhg > (R)0 ? 1u << ICE3_BR_DRYHG : 0u);
     if (rh_ok && warm) {
       if (!soft) {
         R m = rv * (R)pres / ((R)P.XEPSILO + rv);
@@ -1072,6 +1078,20 @@ __device__ __forceinline__ void tendencies(bool soft, double exn, double rhodref
   // aggregation into PA / PB (micro/mode_ice4_tendencies.F90:606-717)
   const double lsl = lsfact - lvfact;
   double a;
+  if (br) {  // the process branches this call took (a test hook; compiled out with br null)
+    unsigned b = 0;
+    if (rcrimsg > (R)0) b |= 1u << ICE3_BR_RIMING;
+    if (rraccsg > (R)0) b |= 1u << ICE3_BR_RACCSG;
+    if (S.rsmltg > (R)0) b |= 1u << ICE3_BR_SMLTG;
+    if (ldwetg) b |= 1u << ICE3_BR_WETG;
+    if (lldryg) b |= 1u << ICE3_BR_DRYG;
+    if (S.rgmltr > (R)0) b |= 1u << ICE3_BR_GMLTR;
+    if constexpr (HAIL) {
+      if (rwetgh > (R)0) b |= 1u << ICE3_BR_WETGH;
+      if (S.rhmltr > (R)0) b |= 1u << ICE3_BR_HMLTR;
+    }
+    *br |= b;
+  }
   a = 0.;
   a = a + (double)S.rvdepg * lsfact;
   a = a + (double)S.rchoni * lsl;
@@ -1234,7 +1254,8 @@ __device__ __forceinline__ int step_call(double dt, const StepPointN<NV> &p, con
                                          SS &S, int &calls, int max_calls,
                                          const TB *__restrict__ tab,
                                          const MicroOcnd2 *x = nullptr,
-                                         const double *__restrict__ htab = nullptr) {
+                                         const double *__restrict__ htab = nullptr,
+                                         unsigned *br = nullptr) {
   static_assert(NV == (HAIL ? 8 : 7), "variables of the time splitting");
   if (++calls > max_calls) {
 #pragma unroll
@@ -1251,7 +1272,7 @@ __device__ __forceinline__ int step_call(double dt, const StepPointN<NV> &p, con
   const double lsfact = (P.XLSTT + (P.XCPV - P.XCI) * (zzt - P.XTT)) / zdevide;
   const double lvfact = (P.XLVTT + (P.XCPV - P.XCL) * (zzt - P.XTT)) / zdevide;
   tendencies<R, OCND2, HAIL>(SOFT, exn, p.rhodref, lvfact, lsfact, p.pres, p.cf, p.sigma_rc, cit,
-                             zzt, vart, S, pc, tab, za, zb, x, htab);
+                             zzt, vart, S, pc, tab, za, zb, x, htab, br);
 #pragma unroll
   for (int jv = 0; jv < NV; ++jv) za[jv] = za[jv] + p.extpk[jv];
   const double time = time_;
@@ -1290,7 +1311,8 @@ __device__ __forceinline__ int stepping_point(double dt, double exn, double rhod
                                               const double *s, double &rrevav, int max_calls,
                                               const TB *__restrict__ tab,
                                               const MicroOcnd2 *x = nullptr,
-                                              const double *__restrict__ htab = nullptr) {
+                                              const double *__restrict__ htab = nullptr,
+                                              unsigned *br = nullptr) {
   StepPointN<HAIL ? 8 : 7> p;
   step_point(dt, exn, rhodref, pres, cf, sigs, s, vart, p);
   typename StepStateFor<R, HAIL>::type S;
@@ -1298,10 +1320,10 @@ __device__ __forceinline__ int stepping_point(double dt, double exn, double rhod
   int calls = 0;
   const PointConst<R> pc = point_const<R>(rhodref);
   int st = step_call<R, false, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x,
-                                            htab);
+                                            htab, br);
   while (st > 0)
     st = step_call<R, true, OCND2, HAIL>(dt, p, pc, vart, cit, time, S, calls, max_calls, tab, x,
-                                         htab);
+                                         htab, br);
   if (st == 0) rrevav = (double)S.rrevav;
   return st < 0 ? -calls : calls;
 }

@@ -211,6 +211,7 @@ struct MicroArgs {
   // height ZZZZ, which the sedimentation's column walk writes into rainfr before the
   // microphysics reads it and writes the rain fraction over it
   const double *icldfr, *ssio, *ssiu, *ifr, *tiwmx;
+  unsigned *branches;  // test hook only (ice3_rain_ice_branches): per point, ICE3_BR_* bits
 };
 
 // ICE4_CORRECT_NEGATIVITIES (micro/mode_ice4_correct_negativities.F90:51-171) on one
@@ -353,7 +354,8 @@ __device__ __forceinline__ void micro_finish(const MicroArgs &a, size_t j, const
 // reinstated 754-823) for point j, all in one thread.  Returns the number of tendency
 // calls, negative if the sub-step cap was hit.  HAIL: KRR=7, hail the eighth variable.
 template <class R, bool OCND2 = false, bool HAIL = false, class TB>
-__device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j, const TB *__restrict__ tab) {
+__device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j, const TB *__restrict__ tab,
+                                        unsigned *br = nullptr) {
   constexpr int NV = HAIL ? 8 : 7;
   double wr[NV], s[NV];
   micro_load<NV>(a, j, wr, s);
@@ -364,7 +366,7 @@ __device__ __forceinline__ int micro_at(const MicroArgs &a, size_t j, const TB *
     if constexpr (OCND2) x = ocnd2_point(a, j);
     calls = stepping_point<R, OCND2, HAIL>(a.dt, a.exn[j], a.rhodref[j], a.pabs[j], a.cldfr[j],
                                            a.sigs[j], cit, wr, s, evap3d, a.max_calls, tab, &x,
-                                           HAIL ? a.tab + ICE3_T_ICE3 : nullptr);
+                                           HAIL ? a.tab + ICE3_T_ICE3 : nullptr, br);
   } else {
     micro_nucleation<R, OCND2, HAIL>(a, j, wr, cit, rvheni);
   }

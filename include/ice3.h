@@ -226,6 +226,27 @@ int ice3_rain_ice_hail(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
                        const ice3_rain_ice_out *out, const ice3_hail *hail,
                        const ice3_stats *stats, void *stream);
 
+/* ---- branch record of the microphysics (no counterpart in the reference: a test hook for
+ *      the per-point branch accounting of the parity tests) ----
+ * As ice3_rain_ice in FP64 (ice3_rain_ice_hail when hail is not NULL), with the microphysics
+ * run one thread per point; branches ((nlev, ld) unsigned) receives for each point the
+ * ICE3_BR_* bits of the process branches its tendency calls took, ORed over the calls (0
+ * outside the microphysics mask): RRHONG and RIMLTC (hard calls); snow riming into graupel,
+ * rain accreted by snow into graupel, snow melting into graupel (ICE4_FAST_RS); graupel wet
+ * growth, dry growth and melting (ICE4_FAST_RG); with hail, wet-growing graupel converted into
+ * hail, hail wet and dry growth, hail converted into graupel and hail melting (ICE4_FAST_RH).
+ * A bit is set when the process's rate is positive, or for the wet/dry choices when the
+ * choice is made. */
+enum {
+  ICE3_BR_RRHONG, ICE3_BR_RIMLTC, ICE3_BR_RIMING, ICE3_BR_RACCSG, ICE3_BR_SMLTG, ICE3_BR_WETG,
+  ICE3_BR_DRYG, ICE3_BR_GMLTR, ICE3_BR_WETGH, ICE3_BR_WETH, ICE3_BR_DRYH, ICE3_BR_DRYHG,
+  ICE3_BR_HMLTR
+};
+int ice3_rain_ice_branches(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                           const ice3_rain_ice_in *in, const ice3_rain_ice_inout *io,
+                           const ice3_rain_ice_out *out, const ice3_hail *hail,
+                           unsigned *branches, void *stream);
+
 /* ---- fused ice_adjust + rain_ice (the "adjust then microphysics" time step) ----
  * ice_adjust runs on the state at t; its updated sources, times dt, are the
  * t-state of rain_ice (the synthetic and AROME-like inputs carry the state at t

@@ -817,6 +817,16 @@ static int rounding_tie(double w, double wscale, double d, double dscale) {
   return fabs(w) <= 8. * eps * fabs(wscale) && fabs(d) <= 8. * eps * fabs(dscale);
 }
 
+/* a - b is a rounding residual of a (a > 0): its sign is set by the last bits.  For the
+ * graupel shares of riming and rain accretion (mode_ice4_fast_rs.F90:257-274): the snow share
+ * ss of the freezing budget freez when they agree to rounding, or the whole rate r and the snow
+ * share taken from it (for small snow the GAMINC factor of the snow share is 1 to rounding).
+ * Recorded as ORACLE_BR_SHARE_TIE. */
+static int residual(double a, double b) {
+  const double eps = 2.220446049250313e-16;
+  return a > 0. && fabs(a - b) <= 8. * eps * a;
+}
+
 void oracle_record_branches(unsigned *flags) { g_branch_out = flags; }
 
 /* Per-point persistent state of ICE4_STEPPING (micro/mode_ice4_stepping.F90:128-158) */
@@ -1059,6 +1069,8 @@ static void ice4_tendencies(int krr, int soft, int ldc, double exn, double rhodr
                                            P.XSRIMCG3 * x);
   }
   if (grim && zt < P.XTT) {
+    BR(ORACLE_BR_SHARE_TIE, residual(freez, rs_tend[RS_RCRIMSS]) ||
+                                residual(rs_tend[RS_RCRIMS], dmin(freez, rs_tend[RS_RCRIMSS])));
     bu[IRCRIMSS] = dmin(freez, rs_tend[RS_RCRIMSS]);
     freez = dmax(0., freez - bu[IRCRIMSS]);
     double zzw0d = dmin(1., freez / dmax(1.E-20, rs_tend[RS_RCRIMS] - bu[IRCRIMSS]));
@@ -1092,6 +1104,8 @@ static void ice4_tendencies(int krr, int soft, int ldc, double exn, double rhodr
     }
   }
   if (gacc && zt < P.XTT) {
+    BR(ORACLE_BR_SHARE_TIE, residual(freez, rs_tend[RS_RRACCSS]) ||
+                                residual(rs_tend[RS_RRACCS], dmin(freez, rs_tend[RS_RRACCSS])));
     bu[IRRACCSS] = dmin(freez, rs_tend[RS_RRACCSS]);
     freez = dmax(0., freez - bu[IRRACCSS]);
     double zzw = dmin(1., freez / dmax(1.E-20, rs_tend[RS_RRACCS] - bu[IRRACCSS]));

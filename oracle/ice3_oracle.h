@@ -82,7 +82,8 @@ enum {
   ORACLE_BR_RRHONG, ORACLE_BR_RIMLTC, ORACLE_BR_RIMING, ORACLE_BR_RACCSG, ORACLE_BR_SMLTG,
   ORACLE_BR_WETG, ORACLE_BR_DRYG, ORACLE_BR_GMLTR, ORACLE_BR_WETGH, ORACLE_BR_WETH,
   ORACLE_BR_DRYH, ORACLE_BR_DRYHG, ORACLE_BR_HMLTR,
-  ORACLE_BR_TIE  /* a graupel or hail wet/dry choice decided by rounding residuals */
+  ORACLE_BR_TIE,  /* a graupel or hail wet/dry choice decided by rounding residuals */
+  ORACLE_BR_SHARE_TIE  /* a graupel share of riming or rain accretion that is a rounding residual */
 };
 void oracle_record_branches(unsigned *flags);
 

@@ -259,7 +259,7 @@ def sedimentation_spli(s: dict, dt: float = 50.0, krr: int = 6, maxcfl: float = 
 
 
 BRANCHES = ("rrhong", "rimltc", "riming", "raccsg", "smltg", "wetg", "dryg", "gmltr", "wetgh",
-            "weth", "dryh", "dryhg", "hmltr", "tie")
+            "weth", "dryh", "dryhg", "hmltr", "tie", "share_tie")
 
 
 def record_branches(fn, *a, **kw):
