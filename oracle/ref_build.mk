@@ -12,10 +12,12 @@
 #   FMA, so flang cannot contract a*b+c and the reference arithmetic stays the plain IEEE
 #   sequence.  It writes the golden vectors and pins the C restatement.
 # * VARIANT=fast (oracle/_ref/fast/): -O3 for x86-64-v3 (AVX2 + FMA), timing only: the
-#   optimised CPU baseline.  The library is built here and run on the GPU box's host, so
-#   it targets an instruction set both hosts have rather than -march=native of this one.
+#   optimised CPU baseline.  The library is built on one machine and timed on the host of
+#   the machine with the GPU, so it targets an instruction set both hosts have rather than
+#   -march=native of the build machine.
 #
-# Usage (this container only; the GPU box receives the built library):
+# Usage (where /root/reference holds the sources; the timing machine receives the built
+# library, not the sources):
 #   make -f oracle/ref_build.mk [VARIANT=fast]
 
 REF      ?= /root/reference/PHYEX-IAL_CY50T1

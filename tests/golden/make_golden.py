@@ -67,7 +67,7 @@ def _adjust_case(f, tag, out):
     return a
 
 
-# cph, lv and ls do not depend on OCND2 (the driver restates them for the plain call only)
+# cph, lv and ls do not depend on OCND2 (oracle/ref_driver.F90 returns them for the plain call only)
 OCND2_OUT = ("t", "rv_out", "rc_out", "ri_out", "cldfr", "hlc_hrc", "hlc_hcf", "hli_hri", "hli_hcf",
              "rvs", "rcs", "ris", "ths", "srcs", "th_out", "icldfr", "wcldfr", "ssio", "ssiu", "ifr")
 
