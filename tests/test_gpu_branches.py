@@ -139,3 +139,18 @@ def test_branch_record_pipeline(dev):
     taken = _compare(dev, s, DT, False, "pipeline 2048 columns")
     for b in ("riming", "raccsg", "wetg", "dryg"):
         assert taken.get(b, (0, 0))[1] > 0, (b, taken)
+
+
+def test_branch_record_hail_pipeline(dev):
+    """KRR=7 on 2,048 columns of the benchmark generator with hail where there is graupel
+    (rh = rg U(0.2, 2), its source rh / dt; the oracle's ICE_ADJUST with hail first)."""
+    from ice3_amd import synthetic
+    from oracle import oracle_lib, ref_lib
+    f = synthetic.generate(2048, 90, seed=20260072, variant="pipeline")
+    rng = np.random.Generator(np.random.Philox(20260073))
+    f["rh"] = np.ascontiguousarray(f["rg"] * rng.uniform(0.2, 2.0, f["rg"].shape))
+    f["rhs"] = f["rh"] / DT
+    s = ref_lib.pipeline_state_hail(f, oracle_lib.ice_adjust_hail(f))
+    taken = _compare(dev, s, DT, True, "hail pipeline 2048 columns")
+    for b in ("wetgh", "weth", "dryh", "hmltr"):
+        assert taken.get(b, (0, 0))[1] > 0, (b, taken)
