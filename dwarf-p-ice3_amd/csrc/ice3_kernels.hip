@@ -679,6 +679,18 @@ __global__ void __launch_bounds__(256) k_micro_points(MicroArgs a) {
   }
 }
 
+// ICE_ADJUST one thread per point with the record of the branches each point took (the test
+// hook of ice3_ice_adjust_branches).
+template <bool HAIL>
+__global__ void __launch_bounds__(256) k_ice_adjust_branches(AdjustArgs a, unsigned *branches) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncol) return;
+  const size_t j = (size_t)blockIdx.y * a.ld + i;
+  unsigned br = 0;
+  adjust_at<false, HAIL>(a, i, j, 0, nullptr, &br);
+  branches[j] = br;
+}
+
 // The microphysics one thread per point with the branch record of each point (the test
 // hook of ice3_rain_ice_branches; the counters of ice3_stats are not kept).
 template <bool HAIL>
@@ -1519,6 +1531,32 @@ static hipError_t launch_micro_hail(const ice3_ctx *ctx, const MicroArgs &m, hip
                        dim3(64 * (micro_waves<double, true>())), (micro_lds_bytes<double, true>()), s, m);
   }
   return hipGetLastError();
+}
+
+extern "C" int ice3_ice_adjust_branches(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                                        const ice3_ice_adjust_in *in, const double *rh,
+                                        const ice3_ice_adjust_out *out, unsigned *branches,
+                                        void *stream) {
+  if (!ctx || !in || !out || !branches) return ICE3_EINVAL;
+  if (empty_domain(ncol, nlev, ld, dt)) return ICE3_OK;
+  if (bad_shape(ncol, nlev, ld, dt) || nlev > 65535) return ICE3_EINVAL;
+  NN(in->pabs); NN(in->sigs); NN(in->th); NN(in->exn); NN(in->exn_ref); NN(in->rho_dry_ref);
+  NN(in->rv); NN(in->rc); NN(in->ri); NN(in->rr); NN(in->rs); NN(in->rg); NN(in->cf_mf);
+  NN(in->rc_mf); NN(in->ri_mf); NN(in->rvs); NN(in->rcs); NN(in->ris); NN(in->ths);
+  NN(in->sigqsat);
+  NN(out->t); NN(out->rv_out); NN(out->rc_out); NN(out->ri_out); NN(out->cldfr);
+  NN(out->hlc_hrc); NN(out->hlc_hcf); NN(out->hli_hri); NN(out->hli_hcf); NN(out->cph);
+  NN(out->lv); NN(out->ls); NN(out->rvs); NN(out->rcs); NN(out->ris); NN(out->ths);
+  AdjustArgs a = AdjustArgs{};
+  a.ncol = ncol; a.nlev = nlev; a.ld = ld; a.dt = dt; a.in = *in; a.out = *out; a.rh = rh;
+  const dim3 grid((ncol + 255) / 256, nlev);
+  hipStream_t s = (hipStream_t)stream;
+  if (rh)
+    hipLaunchKernelGGL(k_ice_adjust_branches<true>, grid, dim3(256), 0, s, a, branches);
+  else
+    hipLaunchKernelGGL(k_ice_adjust_branches<false>, grid, dim3(256), 0, s, a, branches);
+  HIPCHK(hipGetLastError());
+  return ICE3_OK;
 }
 
 extern "C" int ice3_rain_ice_branches(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,

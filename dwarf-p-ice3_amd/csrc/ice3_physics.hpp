@@ -223,7 +223,9 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
                                         double rv, double rc, double ri, double rr, double rs,
                                         double rg, double cf_mf, double rc_mf, double ri_mf,
                                         double &rvs, double &rcs, double &ris, double &ths,
-                                        AdjustOut &o, Ocnd2Point *x = nullptr, double rh = 0.) {
+                                        AdjustOut &o, Ocnd2Point *x = nullptr, double rh = 0.,
+                                        unsigned *br = nullptr) {
+  // br (a test hook; compiled out when null): the ICE3_ABR_* record of the branches taken
   double zt = th * exn;
   const double lv = P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT);
   const double ls = P.XLSTT + (P.XCPV - P.XCI) * (zt - P.XTT);
@@ -270,6 +272,10 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   inq1 = inq1 < -22 ? -22 : (inq1 > 10 ? 10 : inq1);
   const double inc = 2. * q1 - (double)inq1;
   double sigrc = dmin(1., (1. - inc) * ZSRC_1D[inq1 + 22] + inc * ZSRC_1D[inq1 + 23]);
+  if (br)
+    *br = (q1 > 0. && q1 <= 2 ? 1u : q1 > 2. ? 2u : 0u) << ICE3_ABR_Q1 |
+          (cldfr == 0. ? 1u : 0u) << ICE3_ABR_NOCLOUD |
+          (unsigned)(inq1 + 22) << ICE3_ABR_INQ1;
   double rc_out = (1. - frac) * cond;
   double ri_out = frac * cond;
   if constexpr (OCND2) {
@@ -296,6 +302,7 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   // 5.0-5.1 sources
   double w1 = (rc_out - rc) / dt;
   double w2 = (ri_out - ri) / dt;
+  if (br) *br |= (w1 < 0.0 ? 1u << ICE3_ABR_ZW1NEG : 0u) | (w2 < 0.0 ? 1u << ICE3_ABR_ZW2NEG : 0u);
   w1 = w1 < 0.0 ? dmax(w1, -rcs) : dmin(w1, rvs);
   rvs = rvs - w1;
   rcs = rcs + w1;
@@ -308,6 +315,7 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   w1 = rc_mf / dt;
   w2 = ri_mf / dt;
   if (w1 + w2 > rvs) {
+    if (br) *br |= 1u << ICE3_ABR_MFLIMIT;
     w1 = w1 * rvs / (w1 + w2);
     w2 = rvs - w1;
   }
@@ -320,6 +328,7 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   {
     const double criaut = P.XCRIAUTC / rhodref, x = w1 * dt, c = criaut * cf_mf;
     double hcf, hr;
+    if (br) *br |= (x > cf_mf * criaut ? 0u : 2. * w1 * dt <= cf_mf * criaut ? 1u : 2u) << ICE3_ABR_PDFC;
     if (x > cf_mf * criaut) {
       hcf = 1. - .5 * sq(criaut * cf_mf / dmax(1.E-20, x));
       hr = x - (c * c * c) / (3 * sq(dmax(1.E-20, x)));
@@ -333,11 +342,14 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
     hlc_hcf = dmin(1., hlc_hcf + hcf * cf_mf);
     hlc_hrc = hlc_hrc + hr;
   }
-  // the 10**(aT+b) threshold only matters with mass-flux ice or cloud fraction
+  // the 10**(aT+b) threshold only matters with mass-flux ice or cloud fraction (without
+  // either the ice pdf takes its no-cloud case, 2 w2 dt = 0 <= 0)
+  if (br && !(w2 != 0. || cf_mf != 0.)) *br |= 1u << ICE3_ABR_PDFI;
   if (w2 != 0. || cf_mf != 0.) {
     const double criaut = dmin(P.XCRIAUTI, exp_f64(P.LN_10 * (P.XACRIAUTI * (zt - P.XTT) + P.XBCRIAUTI)));
     const double x = w2 * dt, c = criaut * cf_mf;
     double hcf, hr;
+    if (br) *br |= (x > cf_mf * criaut ? 0u : 2. * w2 * dt <= cf_mf * criaut ? 1u : 2u) << ICE3_ABR_PDFI;
     if (x > cf_mf * criaut) {
       hcf = 1. - .5 * sq(criaut * cf_mf / x);
       hr = x - (c * c * c) / (3 * sq(x));
@@ -353,6 +365,7 @@ __device__ __forceinline__ void ice_adjust_point(double dt, double sigqsat, doub
   }
   {  // adjusted state with the mass-flux condensate (POUT_*, :449-466)
     double v1 = rc_mf, v2 = ri_mf;
+    if (br && v1 + v2 > rv_out) *br |= 1u << ICE3_ABR_MFOUT;
     if (v1 + v2 > rv_out) {
       v1 = v1 * rv_out / (v1 + v2);
       v2 = rv_out - v1;

@@ -39,7 +39,7 @@ struct Ocnd2Args {
 // HAIL (KRR=7): a.rh enters the heat capacity.
 template <bool OCND2 = false, bool HAIL = false>
 __device__ __forceinline__ double adjust_at(const AdjustArgs &a, int i, size_t j, int k = 0,
-                                            const Ocnd2Args *x = nullptr) {
+                                            const Ocnd2Args *x = nullptr, unsigned *br = nullptr) {
   const ice3_ice_adjust_in &in = a.in;
   double rvs = in.rvs[j], rcs = in.rcs[j], ris = in.ris[j], ths = in.ths[j];
   AdjustOut o;
@@ -56,7 +56,7 @@ __device__ __forceinline__ double adjust_at(const AdjustArgs &a, int i, size_t j
   ice_adjust_point<OCND2, HAIL>(a.dt, in.sigqsat[i], in.pabs[j], in.sigs[j], in.th[j], in.exn[j],
                                 in.exn_ref[j], in.rho_dry_ref[j], in.rv[j], in.rc[j], in.ri[j],
                                 in.rr[j], in.rs[j], in.rg[j], in.cf_mf[j], in.rc_mf[j], in.ri_mf[j],
-                                rvs, rcs, ris, ths, o, &xp, HAIL ? a.rh[j] : 0.);
+                                rvs, rcs, ris, ths, o, &xp, HAIL ? a.rh[j] : 0., br);
   if constexpr (OCND2) {
     x->out.icldfr[j] = xp.icldfr;
     x->out.wcldfr[j] = xp.wcldfr;

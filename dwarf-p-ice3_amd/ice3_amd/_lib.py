@@ -83,7 +83,7 @@ EXPORTS = ("ice3_ctx_create", "ice3_ctx_destroy", "ice3_strerror", "ice3_dump_co
            "ice3_ice_adjust", "ice3_ice_adjust_ocnd2", "ice3_rain_ice", "ice3_adjust_rain_ice",
            "ice3_rain_ice_ocnd2", "ice3_adjust_rain_ice_ocnd2",
            "ice3_ice_adjust_hail", "ice3_rain_ice_hail", "ice3_adjust_rain_ice_hail",
-           "ice3_rain_ice_branches",
+           "ice3_rain_ice_branches", "ice3_ice_adjust_branches",
            "ice3_stage_timing",
            "ice3_stage_times", "ice3_adjust_rain_ice_stages", "ice3_adjust_rain_ice_surrogate",
            "ice3_adjust_rain_ice_surrogate_noalloc",
@@ -137,6 +137,8 @@ def load():
         lib.ice3_rain_ice_branches.argtypes = c5 + [ctypes.POINTER(RainIn), ctypes.POINTER(RainIO),
                                                     ctypes.POINTER(RainOut), ctypes.POINTER(Hail),
                                                     _P, _P]
+        lib.ice3_ice_adjust_branches.argtypes = c5 + [ctypes.POINTER(AdjustIn), _P,
+                                                      ctypes.POINTER(AdjustOut), _P, _P]
         lib.ice3_adjust_rain_ice_hail.argtypes = c5 + [ctypes.POINTER(PipeIn), ctypes.POINTER(PipeOut),
                                                        ctypes.POINTER(PipeHail), ctypes.POINTER(Stats), _P]
         lib.ice3_adjust_rain_ice_stages.argtypes = common + [

@@ -247,6 +247,23 @@ int ice3_rain_ice_branches(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
                            const ice3_rain_ice_out *out, const ice3_hail *hail,
                            unsigned *branches, void *stream);
 
+/* The same for ICE_ADJUST (FP64; with the hail heat capacity when rh is not NULL, KRR=6 when
+ * it is): per point the Q1 interval of the CB02 condensate (2 bits at ICE3_ABR_Q1: 0 for
+ * Q1 <= 0, 1 for 0 < Q1 <= 2, 2 for Q1 > 2), no cloud (the condensate below 1e-12 or a zero
+ * cloud fraction before the mass-flux part), ZW1 < 0 and ZW2 < 0, the mass-flux limits on the
+ * sources and on the adjusted state, the case of each TRIANGLE pdf (droplets at
+ * ICE3_ABR_PDFC, ice at ICE3_ABR_PDFI, 2 bits each: 0 above the autoconversion threshold, 1 at
+ * or below half of it, 2 between) and INQ1 + 22 (6 bits at ICE3_ABR_INQ1).  It also writes
+ * out as ice3_ice_adjust (ice3_ice_adjust_hail) does, one thread per point. */
+enum {
+  ICE3_ABR_Q1 = 0, ICE3_ABR_NOCLOUD = 2, ICE3_ABR_ZW1NEG = 3, ICE3_ABR_ZW2NEG = 4,
+  ICE3_ABR_MFLIMIT = 5, ICE3_ABR_MFOUT = 6, ICE3_ABR_PDFC = 7, ICE3_ABR_PDFI = 9,
+  ICE3_ABR_INQ1 = 11
+};
+int ice3_ice_adjust_branches(ice3_ctx *ctx, int ncol, int nlev, int ld, double dt,
+                             const ice3_ice_adjust_in *in, const double *rh,
+                             const ice3_ice_adjust_out *out, unsigned *branches, void *stream);
+
 /* ---- fused ice_adjust + rain_ice (the "adjust then microphysics" time step) ----
  * ice_adjust runs on the state at t; its updated sources, times dt, are the
  * t-state of rain_ice (the synthetic and AROME-like inputs carry the state at t

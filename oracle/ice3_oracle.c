@@ -270,7 +270,8 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
                              double exn, double exnref, double rhodref, double rv, double rc,
                              double ri, double rr, double rs, double rg, const double *rh,
                              double cf_mf, double rc_mf, double ri_mf, double *rvs, double *rcs,
-                             double *ris, double *ths, double *o) {
+                             double *ris, double *ths, double *o, unsigned *br) {
+  /* br (NULL: not recorded): the ORACLE_ABR_* record of the branches this point takes */
   /* 2.3 (ice_adjust.F90:264-270) */
   double zt = th * exn;
   double lv = P.XLVTT + (P.XCPV - P.XCL) * (zt - P.XTT);
@@ -318,6 +319,12 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
   if (inq1 > 10) inq1 = 10;
   double inc = 2. * q1 - (double)inq1;
   double sigrc = dmin(1., (1. - inc) * ZSRC_1D[inq1 + 22] + inc * ZSRC_1D[inq1 + 23]);
+  unsigned b = (unsigned)(inq1 + 22) << ORACLE_ABR_INQ1;
+  if (q1 > 0. && q1 <= 2)
+    b |= 1u << ORACLE_ABR_Q1;
+  else if (q1 > 2.)
+    b |= 2u << ORACLE_ABR_Q1;
+  if (cldfr == 0.) b |= 1u << ORACLE_ABR_NOCLOUD;
   double hlc_hcf = 0., hlc_hrc = 0., hli_hcf = 0., hli_hri = 0.;
   double rc_out = (1. - frac) * cond;                              /* :582 */
   double ri_out = frac * cond;                                     /* :583 */
@@ -328,6 +335,8 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
   /* 5.0-5.1 sources (ice_adjust.F90:313-345) */
   double w1 = (rc_out - rc) / dt;
   double w2 = (ri_out - ri) / dt;
+  if (w1 < 0.0) b |= 1u << ORACLE_ABR_ZW1NEG;
+  if (w2 < 0.0) b |= 1u << ORACLE_ABR_ZW2NEG;
   if (w1 < 0.0) w1 = dmax(w1, -*rcs); else w1 = dmin(w1, *rvs);
   *rvs = *rvs - w1;
   *rcs = *rcs + w1;
@@ -340,6 +349,7 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
   w1 = rc_mf / dt;
   w2 = ri_mf / dt;
   if (w1 + w2 > *rvs) {
+    b |= 1u << ORACLE_ABR_MFLIMIT;
     w1 = w1 * *rvs / (w1 + w2);
     w2 = *rvs - w1;
   }
@@ -355,9 +365,11 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
       hcf = 1. - .5 * sq(criaut * cf_mf / dmax(1.E-20, x));
       hr = x - (c * c * c) / (3 * sq(dmax(1.E-20, x)));
     } else if (2. * w1 * dt <= cf_mf * criaut) {
+      b |= 1u << ORACLE_ABR_PDFC;
       hcf = 0.;
       hr = 0.;
     } else {
+      b |= 2u << ORACLE_ABR_PDFC;
       hcf = sq(2. * w1 * dt - criaut * cf_mf) / (2. * sq(dmax(1.E-20, x)));
       hr = (4. * (x * x * x) - 3. * w1 * dt * sq(c) + (c * c * c)) / (3 * sq(dmax(1.E-20, x)));
     }
@@ -372,9 +384,11 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
       hcf = 1. - .5 * sq(criaut * cf_mf / x);
       hr = x - (c * c * c) / (3 * sq(x));
     } else if (2. * w2 * dt <= cf_mf * criaut) {
+      b |= 1u << ORACLE_ABR_PDFI;
       hcf = 0.;
       hr = 0.;
     } else {
+      b |= 2u << ORACLE_ABR_PDFI;
       hcf = sq(2. * w2 * dt - criaut * cf_mf) / (2. * sq(x));
       hr = (4. * (x * x * x) - 3. * w2 * dt * sq(c) + (c * c * c)) / (3 * sq(x));
     }
@@ -385,6 +399,7 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
   { /* adjusted state with the mass-flux condensate (:449-466) */
     double v1 = rc_mf, v2 = ri_mf;
     if (v1 + v2 > rv_out) {
+      b |= 1u << ORACLE_ABR_MFOUT;
       v1 = v1 * rv_out / (v1 + v2);
       v2 = rv_out - v1;
     }
@@ -407,7 +422,15 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
   o[OA_LS] = ls;
   o[OA_SRCS] = sigrc;
   o[OA_TH] = zt / exn;
+  if (br) *br = b;
 }
+
+/* Destination of the ICE_ADJUST branch record (one mask per point), NULL when not recording;
+ * see oracle_record_adjust_branches. */
+static unsigned *g_adj_branch_out;
+
+void oracle_record_adjust_branches(unsigned *flags) { g_adj_branch_out = flags; }
+
 
 static int ice_adjust_krr(int ncol, int nlev, double dt, const double *sigqsat,
                           const double *const in[ORACLE_ADJ_NIN], const double *rh,
@@ -422,7 +445,7 @@ static int ice_adjust_krr(int ncol, int nlev, double dt, const double *sigqsat,
                        in[IA_EXNREF][j], in[IA_RHODREF][j], in[IA_RV][j], in[IA_RC][j],
                        in[IA_RI][j], in[IA_RR][j], in[IA_RS][j], in[IA_RG][j],
                        rh ? rh + j : NULL, in[IA_CF_MF][j], in[IA_RC_MF][j], in[IA_RI_MF][j],
-                       &rvs, &rcs, &ris, &ths, o);
+                       &rvs, &rcs, &ris, &ths, o, g_adj_branch_out ? g_adj_branch_out + j : NULL);
       for (int m = 0; m < ORACLE_ADJ_NOUT_POINT; ++m) out[m][j] = o[m];
       out[OA_RVS][j] = rvs;
       out[OA_RCS][j] = rcs;

@@ -87,6 +87,20 @@ enum {
 };
 void oracle_record_branches(unsigned *flags);
 
+/* ICE_ADJUST's branch record: while flags is not NULL, oracle_ice_adjust and
+ * oracle_ice_adjust_hail write per point (nlev, ncol) the mask of the branches it took: the Q1
+ * interval of the CB02 condensate (2 bits at ORACLE_ABR_Q1: 0 for Q1 <= 0, 1 for 0 < Q1 <= 2,
+ * 2 for Q1 > 2), no cloud (the condensate below 1e-12 or a zero cloud fraction), ZW1 < 0, ZW2 < 0, the mass-flux limit
+ * on the sources (ice_adjust.F90:377-381) and on the adjusted state (:452-456), the case of
+ * each TRIANGLE pdf (2 bits each, droplets :393-406, ice :420-433: 0 above the
+ * autoconversion threshold, 1 at or below half of it, 2 between) and INQ1 + 22 (6 bits). */
+enum {
+  ORACLE_ABR_Q1 = 0, ORACLE_ABR_NOCLOUD = 2, ORACLE_ABR_ZW1NEG = 3, ORACLE_ABR_ZW2NEG = 4,
+  ORACLE_ABR_MFLIMIT = 5, ORACLE_ABR_MFOUT = 6, ORACLE_ABR_PDFC = 7, ORACLE_ABR_PDFI = 9,
+  ORACLE_ABR_INQ1 = 11
+};
+void oracle_record_adjust_branches(unsigned *flags);
+
 /* RAIN_ICE with PARAMI%LOCND2 = .TRUE.: as oracle_rain_ice, plus ICE_ADJUST's OCND2 outputs
  * x4 = {PICLDFR, PSSIO, PSSIU, PIFR}, each (nlev, ncol).  Returns as oracle_rain_ice, or -3
  * when a work array cannot be allocated. */

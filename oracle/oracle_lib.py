@@ -276,6 +276,30 @@ def record_branches(fn, *a, **kw):
     return out, flags
 
 
+# ICE_ADJUST's branch record (oracle_record_adjust_branches): (name, first bit, width).
+ADJ_BRANCHES = (("q1", 0, 2), ("nocloud", 2, 1), ("zw1neg", 3, 1), ("zw2neg", 4, 1),
+                ("mflimit", 5, 1), ("mfout", 6, 1), ("pdfc", 7, 2), ("pdfi", 9, 2), ("inq1", 11, 6))
+
+
+def record_adjust_branches(fn, f, *a, **kw):
+    """Runs one ice_adjust call of this module (fn: ice_adjust or ice_adjust_hail on the
+    fields f) with ICE_ADJUST's branch record on; returns its outputs and the per-point mask
+    (ADJ_BRANCHES layout)."""
+    lib = _load()
+    flags = np.zeros(np.shape(f["th"]), dtype=np.uint32)
+    lib.oracle_record_adjust_branches(ctypes.c_void_p(flags.ctypes.data))
+    try:
+        out = fn(f, *a, **kw)
+    finally:
+        lib.oracle_record_adjust_branches(ctypes.c_void_p(None))
+    return out, flags
+
+
+def adjust_branch_fields(flags) -> dict:
+    """The ADJ_BRANCHES fields of an ICE_ADJUST branch record."""
+    return {n: (flags >> b) & ((1 << w) - 1) for n, b, w in ADJ_BRANCHES}
+
+
 def branch_counts(flags) -> dict:
     """Number of points that took each branch."""
     return {b: int(((flags >> i) & 1).sum()) for i, b in enumerate(BRANCHES)}

@@ -33,7 +33,7 @@ def _stale(target):
 
 def _compile(args, target):
     os.makedirs(BUILD, exist_ok=True)
-    tmp = target + ".tmp"
+    tmp = f"{target}.{os.getpid()}.tmp"  # parallel test workers may build at once
     cmd = [CXX, "-std=c++17", "-ffp-contract=off", *INCLUDES, *args, *SOURCES, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

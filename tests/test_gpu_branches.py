@@ -9,7 +9,11 @@ rests on rounding residuals: a graupel or hail wet/dry choice decided by the sig
 residuals (its "tie" bit), or a graupel share of the snow freezing budget (riming, rain
 accretion) that is a rounding residual (its "share_tie" bit).  Such a branch is decided by the
 last bits of the rates.  The bits are those of include/ice3.h's ICE3_BR_* and
-oracle/ice3_oracle.h's ORACLE_BR_* (the same order)."""
+oracle/ice3_oracle.h's ORACLE_BR_* (the same order).
+
+ICE_ADJUST's record (ice3_ice_adjust_branches against oracle_record_adjust_branches: the Q1
+interval, no cloud, the ZW1/ZW2 signs, the mass-flux limiters, the TRIANGLE pdf cases, INQ1;
+ICE3_ABR_* / ORACLE_ABR_*) must be the oracle's at every point."""
 import os
 
 import numpy as np
@@ -154,3 +158,79 @@ def test_branch_record_hail_pipeline(dev):
     taken = _compare(dev, s, DT, True, "hail pipeline 2048 columns")
     for b in ("wetgh", "weth", "dryh", "hmltr"):
         assert taken.get(b, (0, 0))[1] > 0, (b, taken)
+
+
+def _gpu_adjust_branches(dev, f, dt, hail):
+    """ice3_ice_adjust_branches on the oracle-format ICE_ADJUST fields f."""
+    import ctypes
+    from ice3_amd import _lib
+    from ice3_amd._fields import ptr
+    lib, ctx = _lib.load(), _lib.context(0)
+    nlev, ncol = f["th"].shape
+
+    def t(v):
+        return torch.as_tensor(np.ascontiguousarray(v), dtype=torch.float64, device=dev).clone()
+    g = {k: t(f[k]) for k in _lib.ADJ_IN_FIELDS}
+    out = {k: torch.empty((nlev, ncol), dtype=torch.float64, device=dev) for k in _lib.ADJ_OUT_FIELDS}
+    rh = t(f["rh"]) if hail else None
+    br = torch.full((nlev, ncol), -1, dtype=torch.int32, device=dev)
+    ain = _lib.AdjustIn(*[ptr(g[k]) for k in _lib.ADJ_IN_FIELDS])
+    aout = _lib.AdjustOut(*[ptr(out[k]) for k in _lib.ADJ_OUT_FIELDS])
+    sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    rc = lib.ice3_ice_adjust_branches(ctx, ncol, nlev, ncol, dt, ctypes.byref(ain),
+                                      ptr(rh) if hail else None, ctypes.byref(aout), ptr(br), sp)
+    assert rc == _lib.ICE3_OK, rc
+    torch.cuda.synchronize()
+    return _np(br).astype(np.uint32), {k: _np(v) for k, v in out.items()}
+
+
+def _compare_adjust(dev, f, dt, hail, label):
+    from oracle import oracle_lib
+    fn = oracle_lib.ice_adjust_hail if hail else oracle_lib.ice_adjust
+    o, ref = oracle_lib.record_adjust_branches(fn, f, dt)
+    got, out = _gpu_adjust_branches(dev, f, dt, hail)
+    # the hook's outputs are those of the production ICE_ADJUST: the oracle's to the parity
+    # tolerance (1e-11 relative, floors of 1e-18 kg/kg on mixing ratios and 1e-18/dt on sources)
+    from helpers import floor_for, rel_err
+    for k in ("t", "rv_out", "rc_out", "ri_out", "cldfr", "rvs", "rcs", "ris", "ths"):
+        assert rel_err(out[k], o[k], floor_for(k, 1e-11, dt)).max() <= 1e-11, k
+    fg, fr = oracle_lib.adjust_branch_fields(got), oracle_lib.adjust_branch_fields(ref)
+    taken = {n: np.unique(fr[n]).tolist() for n in fr if n != "inq1"}
+    differ = {n: int((fg[n] != fr[n]).sum()) for n in fg}
+    print(f"{label}: points with a different record per field {differ} of {got.size}; "
+          f"values taken (oracle): {taken}, INQ1 + 22 in [{fr['inq1'].min()}, {fr['inq1'].max()}]")
+    assert got.shape == ref.shape and bool((got >> 17 == 0).all())
+    assert np.array_equal(got, ref), np.argwhere(got != ref)[:10]
+    return taken
+
+
+@pytest.mark.parametrize("tag", ["base", "mf"])
+def test_adjust_branch_record_golden(dev, golden_dir, tag):
+    """ICE_ADJUST (KRR=6) on the compiled reference's golden inputs: the record is the
+    oracle's at every point."""
+    g = np.load(os.path.join(golden_dir, "ice_adjust_fp64.npz"))
+    f = {k[len(tag) + 4:]: g[k] for k in g.files if k.startswith(tag + "_in_")}
+    taken = _compare_adjust(dev, f, float(g["dt"]), False, f"ice_adjust golden {tag}")
+    assert len(taken["q1"]) >= 2 and len(taken["pdfc"]) >= 1
+
+
+@pytest.mark.parametrize("tag", ["base", "cov"])
+def test_adjust_branch_record_hail_golden(dev, golden_dir, tag):
+    """ICE_ADJUST with hail (KRR=7) on the compiled reference's hail golden inputs."""
+    g = np.load(os.path.join(golden_dir, "rain_ice_hail_fp64.npz"))
+    p = tag + "_adj_in_"
+    f = {k[len(p):]: g[k] for k in g.files if k.startswith(p)}
+    _compare_adjust(dev, f, float(g["dt"]), True, f"ice_adjust hail golden {tag}")
+
+
+@pytest.mark.parametrize("hail", [False, True])
+def test_adjust_branch_record_pipeline(dev, hail):
+    """ICE_ADJUST on 2,048 columns of the benchmark generator (with hail where there is
+    graupel for KRR=7): every branch value is taken and the record is the oracle's."""
+    from ice3_amd import synthetic
+    f = synthetic.generate(2048, 90, seed=20260076, variant="pipeline")
+    if hail:
+        rng = np.random.Generator(np.random.Philox(20260077))
+        f["rh"] = np.ascontiguousarray(f["rg"] * rng.uniform(0.2, 2.0, f["rg"].shape))
+    taken = _compare_adjust(dev, f, DT, hail, f"ice_adjust pipeline 2048 columns{' hail' if hail else ''}")
+    assert taken["q1"] == [0, 1, 2] and taken["nocloud"] == [0, 1], taken

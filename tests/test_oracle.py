@@ -109,6 +109,50 @@ def _pipeline(ncol, seed):
     return f, s
 
 
+def test_ice_adjust_branch_record(golden_dir):
+    """ICE_ADJUST's branch record (oracle_record_adjust_branches) against what the compiled
+    reference's golden outputs show of each branch: no cloud where the cloud fraction is that
+    of the mass flux alone; the adjusted-state limiter where it divided 0/0 (rv_out NaN with
+    no mass-flux condensate); a droplet or ice pdf case 1 exactly where its cloud fraction is
+    zero, case 0 where it is at least half the mass-flux fraction, case 2 below; the sign of
+    ZW1 / ZW2 from the change of rcs / ris without mass-flux condensate; the Q1 interval
+    against INQ1.  Recording leaves the outputs bitwise unchanged."""
+    g = np.load(os.path.join(golden_dir, "ice_adjust_fp64.npz"))
+    seen = {}
+    for tag in ("base", "mf"):
+        f = {k[len(tag) + 4:]: g[k] for k in g.files if k.startswith(tag + "_in_")}
+        ref = {k[len(tag) + 5:]: g[k] for k in g.files if k.startswith(tag + "_out_")}
+        dt = float(g["dt"])
+        o, flags = oracle_lib.record_adjust_branches(oracle_lib.ice_adjust, f, dt)
+        plain = oracle_lib.ice_adjust(f, dt)
+        for k in plain:
+            assert np.array_equal(o[k], plain[k], equal_nan=True), k
+        b = oracle_lib.adjust_branch_fields(flags)
+        for n, v in b.items():
+            seen.setdefault(n, set()).update(np.unique(v).tolist())
+        cf = f["cf_mf"]
+        assert np.array_equal(b["nocloud"] == 1, ref["cldfr"] == np.minimum(1.0, cf))
+        nomf = (f["rc_mf"] == 0) & (f["ri_mf"] == 0)
+        assert np.array_equal((b["mfout"] == 1) & nomf, np.isnan(ref["rv_out"]) & nomf)
+        for pdf, hcf in (("pdfc", ref["hlc_hcf"]), ("pdfi", ref["hli_hcf"])):
+            on = cf > 0
+            assert np.array_equal((b[pdf] == 1) & on, (hcf == 0) & on), pdf
+            assert np.all(hcf[(b[pdf] == 0) & on] >= 0.5 * cf[(b[pdf] == 0) & on] * (1 - 1e-12)), pdf
+            assert np.all(hcf[(b[pdf] == 2) & on] <= 0.5 * cf[(b[pdf] == 2) & on] * (1 + 1e-12)), pdf
+        # without mass-flux condensate the sources change by the limited ZW1 / ZW2 alone
+        dc, di = ref["rcs"] - f["rcs"], ref["ris"] - f["ris"]
+        assert np.all(dc[(b["zw1neg"] == 1) & nomf] <= 0) and np.all(dc[(b["zw1neg"] == 0) & nomf] >= 0)
+        assert np.all(di[(b["zw2neg"] == 1) & nomf] <= 0) and np.all(di[(b["zw2neg"] == 0) & nomf] >= 0)
+        assert np.any(dc[(b["zw1neg"] == 1) & nomf] < 0) and np.any(di[(b["zw2neg"] == 1) & nomf] < 0)
+        # Q1 <= 0: INQ1 <= 0; 0 < Q1 <= 2: 0 <= INQ1 <= 4; Q1 > 2: INQ1 >= 4 (INQ1 + 22 stored)
+        q, i = b["q1"], b["inq1"]
+        assert np.all(i[q == 0] <= 22) and np.all((i[q == 1] >= 22) & (i[q == 1] <= 26))
+        assert np.all(i[q == 2] >= 26) and np.all(i <= 32) and not np.any(q == 3)
+    assert seen["q1"] == {0, 1, 2} and seen["pdfc"] == {0, 1, 2} and seen["pdfi"] == {0, 1, 2}
+    for n in ("nocloud", "zw1neg", "zw2neg", "mflimit", "mfout"):
+        assert seen[n] == {0, 1}, n
+
+
 def test_rain_ice_column_water_budget():
     """Total water of each column changes only by what falls out of its lowest level:
     sum_k rho dz (sum of water sources)(after-before) dt = -rho_lw (inprr+inprs+inprg) dt."""
