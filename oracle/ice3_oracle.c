@@ -426,8 +426,8 @@ static void ice_adjust_point(double dt, double sigqsat, double pabs, double sigs
 }
 
 /* Destination of the ICE_ADJUST branch record (one mask per point), NULL when not recording;
- * see oracle_record_adjust_branches. */
-static unsigned *g_adj_branch_out;
+ * see oracle_record_adjust_branches.  Per thread, as the tendency calls' record. */
+static _Thread_local unsigned *g_adj_branch_out;
 
 void oracle_record_adjust_branches(unsigned *flags) { g_adj_branch_out = flags; }
 
@@ -825,9 +825,10 @@ static void ice4_nucleation(int ldcompute, double tht, double pabst, double rhod
 }
 
 /* Branch record of the tendency calls (test coverage only): bits of ORACLE_BR_* ORed over
- * the calls of one point; see oracle_record_branches. */
-static unsigned g_branches;
-static unsigned *g_branch_out;
+ * the calls of one point; see oracle_record_branches.  Per thread: column slabs may run in
+ * parallel threads, and a record taken in one thread sees only that thread's calls. */
+static _Thread_local unsigned g_branches;
+static _Thread_local unsigned *g_branch_out;
 #define BR(bit, cond) do { if (cond) g_branches |= (1u << (bit)); } while (0)
 
 /* A wet/dry growth choice compares max(0, w) with max(0, d), where w and d are differences

@@ -76,8 +76,9 @@ int oracle_rain_ice_hail(int ncol, int nlev, double dt, const double *const in[O
                          int *iters);
 
 /* Branch record for coverage tests: while flags (ncol*nlev unsigned) is set, every rain_ice
- * call writes there, per point, the ORACLE_BR_* bits of the branches its tendency calls
- * took (ORed over the calls); NULL stops the recording. */
+ * call of the calling thread writes there, per point, the ORACLE_BR_* bits of the branches
+ * its tendency calls took (ORed over the calls); NULL stops the recording.  The setting is
+ * per thread (so is oracle_record_adjust_branches'). */
 enum {
   ORACLE_BR_RRHONG, ORACLE_BR_RIMLTC, ORACLE_BR_RIMING, ORACLE_BR_RACCSG, ORACLE_BR_SMLTG,
   ORACLE_BR_WETG, ORACLE_BR_DRYG, ORACLE_BR_GMLTR, ORACLE_BR_WETGH, ORACLE_BR_WETH,

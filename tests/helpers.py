@@ -142,6 +142,35 @@ def reproduced_by_oracle(out, run, s, names, tol, fields, points, scales=(1e-15,
     return found
 
 
+# The input source and state at t of each output source (rain_ice state names).
+SOURCE_INPUTS = {"ths": ("ths", "tht"), "rvs": ("rvs", "rvt"), "rcs": ("rcs", "rct"),
+                 "rrs": ("rrs", "rrt"), "ris": ("ris", "rit"), "rss": ("rss", "rst"),
+                 "rgs": ("rgs", "rgt"), "rhs": ("rhs", "rht")}
+
+
+def cancellation_residuals(out, ref, s, names, tol, dt, ulps=8):
+    """Points where every output of `names` beyond the bar differs from ref by at most `ulps`
+    units in the last place of the input it was computed from (the input source, or the state
+    at t over dt): an output that cancels its input to a rounding residue (RRHONG freezing all
+    the rain, a species emptied by its sink) is fixed only to that, and where that residue is
+    above the bar's absolute floor two correct FP64 evaluations may disagree there."""
+    eps = np.finfo(np.float64).eps
+    beyond = np.zeros(np.shape(ref[names[0]]), dtype=bool)
+    within = np.ones_like(beyond)
+    for k in names:
+        if np.ndim(ref[k]) != 2:
+            continue
+        b = ~(rel_err(out[k], ref[k], floor_for(k, tol, dt)) <= tol)
+        beyond |= b
+        if k in SOURCE_INPUTS:
+            src, t = SOURCE_INPUTS[k]
+            scale = np.maximum(np.abs(s[src]), np.abs(s[t]) / dt)
+            within &= ~b | (np.abs(out[k] - ref[k]) <= ulps * eps * scale)
+        else:
+            within &= ~b
+    return beyond & within
+
+
 def explained_flips(out, ref, run, s, names, tol, fields, calls="calls", iters="iters", ties=None):
     """The branch flips of out against the oracle's ref: flips_of, then ill_conditioned, then
     the remaining divergent points where the oracle's wet/dry growth choice rests on rounding
