@@ -190,3 +190,65 @@ def explained_flips(out, ref, run, s, names, tol, fields, calls="calls", iters="
         for p, r in zip(pts, rep):
             flips[p] = flips[p] or r
     return flips
+
+
+def every_point(res, f, make_state, run, names, state_fields, tol, dt=50.0, slab=16384, label=""):
+    """Compares a GPU step's results res (device tensors, (nlev, ncol) or (ncol,)) with the
+    oracle at every point.  The oracle runs on slabs of `slab` columns in parallel threads
+    (its C entry points release the interpreter lock; its branch record is per thread):
+    make_state(g) builds the rain_ice state of the fields g of one slab, run(s, **kw) runs the
+    oracle's rain_ice variant on it.  A point beyond the bar must be a flip (flips_of); the
+    divergent points that keep the oracle's call count are checked on their column alone:
+    explained_flips with the ties of the oracle's branch record (wet/dry choices and graupel
+    shares that rest on rounding residuals), and cancellation_residuals.  Returns (worst error
+    per output over the other points, flips, divergent points, columns checked alone)."""
+    import concurrent.futures as cf
+    import os
+
+    from oracle import oracle_lib
+    ncol = f["th"].shape[1]
+    B = oracle_lib.BRANCHES
+
+    def oracle_slab(a):
+        g = {k: np.ascontiguousarray(v[:, a:a + slab] if np.ndim(v) == 2 else v[a:a + slab])
+             for k, v in f.items()}
+        st = make_state(g)
+        return a, st, run(st, with_iters=True)
+
+    def cols(d, c):
+        return {k: (np.ascontiguousarray(v[:, c]) if np.ndim(v) == 2 else
+                    (np.ascontiguousarray(v[c]) if np.ndim(v) == 1 else v)) for k, v in d.items()}
+
+    worst, nf, nb, checked = {}, 0, 0, 0
+    with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+        for a, st, ref in pool.map(oracle_slab, range(0, ncol, slab)):
+            got = {k: v[..., a:a + slab].detach().cpu().numpy() for k, v in res.items()}
+            flips = flips_of(got, ref)
+            bad = np.zeros(flips.shape, dtype=bool)
+            for k in names:
+                r = rel_err(got[k], ref[k], floor_for(k, tol, dt))
+                if r.shape == flips.shape:
+                    bad |= ~(r <= tol)
+            c = np.unique(np.nonzero(bad & ~flips)[1])
+            if c.size:
+                sc, gc, rc = cols(st, c), cols(got, c), cols(ref, c)
+                _, rec = oracle_lib.record_branches(run, sc)
+                ties = (((rec >> B.index("tie")) | (rec >> B.index("share_tie"))) & 1).astype(bool)
+                fc = explained_flips(gc, rc, run, sc, names, tol, state_fields, ties=ties)
+                fc |= cancellation_residuals(gc, rc, sc, names, tol, dt)
+                for k, j in np.argwhere(bad[:, c] & ~flips[:, c] & ~fc)[:20]:
+                    errs = {q: float(rel_err(gc[q][k, j], rc[q][k, j], floor_for(q, tol, dt)))
+                            for q in names if np.ndim(gc[q]) == 2}
+                    print(f"{label} level {k} column {a + c[j]}: calls {gc['calls'][k, j]} / "
+                          f"{rc['iters'][k, j]}, oracle record {[n for i, n in enumerate(B) if rec[k, j] >> i & 1]}, "
+                          f"not explained, beyond the bar "
+                          f"{ {q: e for q, e in errs.items() if not e <= tol} }", flush=True)
+                print(f"{label} columns {a}-{a + slab - 1}: {int((bad & ~flips).sum())} points beyond the "
+                      f"bar with the oracle's call count, in {c.size} columns checked alone; "
+                      f"{int((bad[:, c] & ~flips[:, c] & fc).sum())} explained", flush=True)
+                flips[:, c] |= fc
+                checked += c.size
+            w, n1, n2 = check_with_flips(got, ref, names, flips, tol)
+            worst = {k: max(worst.get(k, 0.0), v) for k, v in w.items()}
+            nf, nb = nf + n1, nb + n2
+    return worst, nf, nb, checked

@@ -173,13 +173,11 @@ def test_pipeline_hail_vs_oracle(dev):
 
 # A stride coprime to 64 over the 512x512 columns: 516 sampled columns, all lanes of a
 # wavefront.
-HAIL_STRIDE = 509
-
-
 def test_pipeline_hail_full_size(dev):
     """The fused hail step at 512x512x90 (the benchmark workload with hail added): column
-    water budget with hail, finite non-negative condensed sources, and every 509th column
-    against the oracle at the FP64 bar, flips counted."""
+    water budget with hail, finite non-negative condensed sources, and every point against
+    the oracle at the FP64 bar, flips counted (helpers.every_point)."""
+    from helpers import every_point
     from ice3_amd import AdjustRainIceHail
     from oracle import oracle_lib, ref_lib
     f = _hail_pipeline(512 * 512, 20260003)
@@ -188,18 +186,11 @@ def test_pipeline_hail_full_size(dev):
     assert err < 1e-12, err
     for k in ("rcs", "rrs", "ris", "rss", "rgs", "rhs"):
         assert bool(torch.isfinite(res[k]).all()) and bool((res[k] >= 0).all()), k
-    cols = np.arange(0, 512 * 512, HAIL_STRIDE)
-    idx = torch.as_tensor(cols, device=dev)
-    got = {k: _np(v.index_select(v.dim() - 1, idx)) for k, v in res.items()}
-    del res
-    g = {k: np.ascontiguousarray(v[:, cols] if v.ndim == 2 else v[cols]) for k, v in f.items()}
-    s = ref_lib.pipeline_state_hail(g, oracle_lib.ice_adjust_hail(g))
-    run = _oracle_run()
-    ref = run(s, with_iters=True)
-    flips = explained_flips(got, ref, run, s, HAIL_OUT, TOL, HAIL_STATE, ties=_ties(s))
-    w, nf, nb = check_with_flips(got, ref, HAIL_OUT, flips, TOL)
-    print(f"fused hail step 512x512x90, sampled columns: budget {err:.1e}, {nf} flips, {nb} divergent "
-          f"of {flips.size}, worst {max(w.values()):.2e}")
+    w, nf, nb, checked = every_point(
+        res, f, lambda g: ref_lib.pipeline_state_hail(g, oracle_lib.ice_adjust_hail(g)),
+        _oracle_run(), HAIL_OUT, HAIL_STATE, TOL, dt=DT, label="fused hail step")
+    print(f"fused hail step 512x512x90, every point: budget {err:.1e}, {nf} flips, {nb} divergent "
+          f"of {90 * 512 * 512}, worst {max(w.values()):.2e}; {checked} columns checked alone")
 
 
 def test_hail_without_hail_or_graupel_is_krr6(dev):

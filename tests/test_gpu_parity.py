@@ -242,66 +242,24 @@ RAIN_STATE = ("tht", "rvt", "rct", "rrt", "rit", "rst", "rgt", "ths", "rvs", "rc
 
 def test_pipeline_full_size_every_point(dev, full512):
     """512x512 columns x 90 levels (the fused benchmark workload), every one of its
-    23,592,960 points against the oracle at the FP64 bar, branch flips counted: the oracle
-    runs on 16 slabs of 16,384 columns in parallel threads (its C entry points release the
-    interpreter lock and keep no shared state), and each slab is compared with the same
-    columns of one GPU step.  A point beyond the bar must be a flip: a different number of
-    tendency calls, or (checked on its column alone, as the 2,048-column tests do) an output
-    that last-bit changes of the inputs move beyond the bar, a wet/dry choice or a graupel
-    share of riming or accretion that the oracle records as resting on rounding residuals, a
-    value the oracle reproduces from an input changed in its last digits, or a source that
-    cancels its input to a rounding residue (within 8 units in the last place of the input
-    source or the state at t over dt, as where RRHONG freezes all the rain)."""
-    import concurrent.futures as cf
-    from helpers import cancellation_residuals, explained_flips, flips_of
+    23,592,960 points against the oracle at the FP64 bar, branch flips counted
+    (helpers.every_point: the oracle on 16 slabs of 16,384 columns in parallel threads).  A
+    point beyond the bar must be a flip: a different number of tendency calls, or (checked on
+    its column alone, as the 2,048-column tests do) an output that last-bit changes of the
+    inputs move beyond the bar, a wet/dry choice or a graupel share of riming or accretion
+    that the oracle records as resting on rounding residuals, a value the oracle reproduces
+    from an input changed in its last digits, or a source that cancels its input to a
+    rounding residue (within 8 units in the last place of the input source or the state at t
+    over dt, as where RRHONG freezes all the rain)."""
+    from helpers import every_point
     from ice3_amd import AdjustRainIce
     from oracle import oracle_lib, ref_lib
     f = full512[0]
-    ncol, slab = 512 * 512, 16384
     res = AdjustRainIce()({**f, "rhodref": f["rho_dry_ref"]}, 50.0, return_calls=True)
-    run = lambda x, **kw: oracle_lib.rain_ice(x, 50.0, **kw)  # noqa: E731
-    B = oracle_lib.BRANCHES
-
-    def oracle_slab(a):
-        g = {k: np.ascontiguousarray(v[:, a:a + slab] if v.ndim == 2 else v[a:a + slab]) for k, v in f.items()}
-        s = ref_lib.pipeline_state(g, oracle_lib.ice_adjust(g))
-        return a, s, run(s, with_iters=True)
-
-    def cols(d, c):
-        return {k: (np.ascontiguousarray(v[:, c]) if np.ndim(v) == 2 else
-                    (np.ascontiguousarray(v[c]) if np.ndim(v) == 1 else v)) for k, v in d.items()}
-
-    worst, nf, nb, checked = {}, 0, 0, 0
-    with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
-        for a, s, ref in pool.map(oracle_slab, range(0, ncol, slab)):
-            got = {k: _np(v[..., a:a + slab]) for k, v in res.items()}
-            flips = flips_of(got, ref)
-            bad = np.zeros(flips.shape, dtype=bool)
-            for k in RAIN_OUT:
-                r = rel_err(got[k], ref[k], floor_for(k, TOL))
-                if r.shape == flips.shape:
-                    bad |= ~(r <= TOL)
-            c = np.unique(np.nonzero(bad & ~flips)[1])
-            if c.size:
-                assert c.size <= 64, c.size
-                sc, gc, rc = cols(s, c), cols(got, c), cols(ref, c)
-                _, rec = oracle_lib.record_branches(oracle_lib.rain_ice, sc, 50.0)
-                ties = ((rec >> B.index("tie")) & 1) | ((rec >> B.index("share_tie")) & 1)
-                fc = explained_flips(gc, rc, run, sc, RAIN_OUT, TOL, RAIN_STATE, ties=ties.astype(bool))
-                fc |= cancellation_residuals(gc, rc, sc, RAIN_OUT, TOL, 50.0)
-                for k, j in np.argwhere(bad[:, c] & ~flips[:, c]):
-                    errs = {q: float(rel_err(gc[q][k, j], rc[q][k, j], floor_for(q, TOL)))
-                            for q in RAIN_OUT if np.ndim(gc[q]) == 2}
-                    print(f"slab {a} level {k} column {a + c[j]}: calls {gc['calls'][k, j]} / "
-                          f"{rc['iters'][k, j]}, oracle record {[n for i, n in enumerate(B) if rec[k, j] >> i & 1]}, "
-                          f"explained {bool(fc[k, j])}, beyond the bar "
-                          f"{ {q: e for q, e in errs.items() if not e <= TOL} }", flush=True)
-                flips[:, c] |= fc
-                checked += c.size
-            w, n1, n2 = check_with_flips(got, ref, RAIN_OUT, flips, TOL)
-            worst = {k: max(worst.get(k, 0.0), v) for k, v in w.items()}
-            nf, nb = nf + n1, nb + n2
-    npts = 90 * ncol
+    worst, nf, nb, checked = every_point(
+        res, f, lambda g: ref_lib.pipeline_state(g, oracle_lib.ice_adjust(g)),
+        lambda x, **kw: oracle_lib.rain_ice(x, 50.0, **kw), RAIN_OUT, RAIN_STATE, TOL, label="fused step")
+    npts = 90 * 512 * 512
     print(f"512x512 step, every point: {nf} branch flips ({nf / npts:.1e}), {nb} divergent points "
           f"(all at flips) of {npts}, worst {max(worst.values()):.2e}; {checked} columns checked "
           f"alone for flips other than a call-count change")
@@ -759,25 +717,22 @@ def test_pipeline_ocnd2_vs_oracle(dev):
 
 def test_pipeline_ocnd2_full_size(dev, full512):
     """The fused OCND2 step at 512x512x90: column water budget, non-negative condensed
-    sources, and every 127th column (all 64 lanes) against the oracle, flips counted."""
-    from helpers import flips_of
+    sources, and every point against the oracle, flips counted (helpers.every_point)."""
+    from helpers import every_point
     from ice3_amd import AdjustRainIceOCND2
     from oracle import oracle_lib, ref_lib
-    f0, cols, _ = full512
-    f = _with_heights(dict(f0), 20260050)
+    f = _with_heights(dict(full512[0]), 20260050)
     res = AdjustRainIceOCND2()({**f, "rhodref": f["rho_dry_ref"]}, 50.0, return_calls=True)
     err = _column_budget(dev, f, res)
     assert err < 1e-12, err
     for k in ("rcs", "rrs", "ris", "rss", "rgs"):
         assert bool((res[k] >= 0).all()), k
-    got = _sampled(res, cols, dev)
-    del res
-    g = {k: np.ascontiguousarray(v[:, cols] if v.ndim == 2 else v[cols]) for k, v in f.items()}
-    ref = oracle_lib.rain_ice_ocnd2(ref_lib.pipeline_state_ocnd2(g, oracle_lib.ice_adjust_ocnd2(g)),
-                                    with_iters=True)
-    w, nf, nb = check_with_flips(got, ref, RAIN_OUT, flips_of(got, ref), TOL)
-    print(f"fused OCND2 step 512x512x90, sampled columns: budget {err:.1e}, {nf} flips, {nb} divergent "
-          f"of {got['calls'].size}, worst {max(w.values()):.2e}")
+    w, nf, nb, checked = every_point(
+        res, f, lambda g: ref_lib.pipeline_state_ocnd2(g, oracle_lib.ice_adjust_ocnd2(g)),
+        lambda x, **kw: oracle_lib.rain_ice_ocnd2(x, 50.0, **kw), RAIN_OUT, RAIN_STATE, TOL,
+        label="fused OCND2 step")
+    print(f"fused OCND2 step 512x512x90, every point: budget {err:.1e}, {nf} flips, {nb} divergent "
+          f"of {90 * 512 * 512}, worst {max(w.values()):.2e}; {checked} columns checked alone")
 
 
 @pytest.mark.parametrize("kind", ["plain", "ocnd2", "hail"])

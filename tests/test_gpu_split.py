@@ -256,13 +256,24 @@ def test_ocnd2_split_vs_oracle(dev):
 
 def test_split_full_size_budget(dev):
     """The fused step with SPLI at 512x512x90: column water budget, finite non-negative
-    condensed sources, and the statistical step is unchanged after it (the drop-in sets the
-    scheme back)."""
+    condensed sources, every point against the oracle with SPLI at the FP64 bar, flips
+    counted (helpers.every_point; the oracle's scheme is set once for all its threads), and
+    the statistical step is unchanged after it (the drop-in sets the scheme back)."""
+    from helpers import every_point
     from ice3_amd import AdjustRainIce, synthetic
+    from oracle import oracle_lib, ref_lib
     f = synthetic.generate(512 * 512, 90, seed=20260003, variant="pipeline")
     f["rhodref"] = f["rho_dry_ref"]
     stat0 = AdjustRainIce()(f, DT)
-    res = AdjustRainIce(sedimentation="SPLI")(f, DT)
+    res = AdjustRainIce(sedimentation="SPLI")(f, DT, return_calls=True)
+    with _oracle_spli():
+        w, nf, nb, checked = every_point(
+            res, f, lambda g: ref_lib.pipeline_state(g, oracle_lib.ice_adjust(g)),
+            lambda x, **kw: oracle_lib.rain_ice(x, DT, **kw), RAIN_OUT, RAIN_STATE, TOL, dt=DT,
+            label="fused SPLI step")
+    print(f"fused SPLI 512x512x90, every point: {nf} flips, {nb} divergent of {90 * 512 * 512}, "
+          f"worst {max(w.values()):.2e}; {checked} columns checked alone")
+    res.pop("calls")
     t = lambda k: torch.as_tensor(f[k], device=dev)  # noqa: E731
     ws = ("rvs", "rcs", "rrs", "ris", "rss", "rgs")
     mass = t("rho_dry_ref") * t("dzz")
