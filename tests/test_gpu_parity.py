@@ -275,31 +275,53 @@ MIXED_RMS_REL = 2e-5
 def test_mixed_precision_full_size_vs_oracle(dev, full512):
     """The mixed path (float process rates, slopes and lookups; double saturation adjustment,
     state update, time splitting and sedimentation fluxes) on the 512x512x90 benchmark
-    workload: on every 127th column, each output within MIXED_MAX_REL of the oracle field's
-    largest magnitude and MIXED_RMS_REL of its RMS; over all columns, the condensed species'
-    sources non-negative and each column's total water changed only by its surface
-    precipitation (the budget is carried in double)."""
+    workload, at every point: each output within MIXED_MAX_REL of the oracle field's largest
+    magnitude and MIXED_RMS_REL of its RMS (the oracle on 16 slabs of 16,384 columns in
+    parallel threads), the points where the ice concentration is reset to 0 on one side only
+    (a branch flip) counted apart and below 1e-4 of the points; the condensed species' sources non-negative and each column's total
+    water changed only by its surface precipitation (the budget is carried in double)."""
+    import concurrent.futures as cf
     from ice3_amd import AdjustRainIce
-    f, cols, ref = full512
+    from oracle import oracle_lib, ref_lib
+    f = full512[0]
     res = AdjustRainIce()({**f, "rhodref": f["rho_dry_ref"]}, 50.0, precision="mixed")
     err = _column_budget(dev, f, res)
     print("mixed, full-size column water budget max rel err", err)
     assert err < 1e-12
     for k in ("rcs", "rrs", "ris", "rss", "rgs"):
         assert bool((res[k] >= 0).all()), k
-    got = _sampled(res, cols, dev)
-    del res
-    stats = {}
-    for k in ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "ci", "inprr", "inprs", "inprg"):
-        d = np.abs(got[k] - ref[k])
-        mx = np.abs(ref[k]).max()
-        rms = np.sqrt(np.mean(ref[k] ** 2))
-        stats[k] = (float(d.max() / mx) if mx > 0 else 0.0,
-                    float(np.sqrt(np.mean(d ** 2)) / rms) if rms > 0 else 0.0)
-    print("mixed vs oracle (512x512x90, every 127th column): max err / field max, RMS err / field RMS",
-          {k: f"{a:.1e}, {b:.1e}" for k, (a, b) in stats.items()})
+    names = ("ths", "rvs", "rcs", "rrs", "ris", "rss", "rgs", "ci", "inprr", "inprs", "inprg")
+    ncol, slab = 512 * 512, 16384
+
+    def oracle_slab(a):
+        g = {k: np.ascontiguousarray(v[:, a:a + slab] if v.ndim == 2 else v[a:a + slab]) for k, v in f.items()}
+        return a, oracle_lib.rain_ice(ref_lib.pipeline_state(g, oracle_lib.ice_adjust(g)))
+
+    # per field: max |err|, max |ref|, sum err^2, sum ref^2, count
+    acc = {k: [0.0, 0.0, 0.0, 0.0, 0] for k in names}
+    resets = 0
+    with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+        for a, ref in pool.map(oracle_slab, range(0, ncol, slab)):
+            for k in names:
+                g = _np(res[k][..., a:a + slab])
+                d = np.abs(g - ref[k])
+                if k == "ci":
+                    # the ice concentration reset to 0 on one side only: a branch decided by
+                    # the sign of ri's rounding residue (micro/mode_ice4_stepping.F90:471-474)
+                    one = (g == 0.0) != (ref[k] == 0.0)
+                    resets += int(one.sum())
+                    d = np.where(one, 0.0, d)
+                m = acc[k]
+                m[0], m[1] = max(m[0], float(d.max())), max(m[1], float(np.abs(ref[k]).max()))
+                m[2], m[3], m[4] = m[2] + float((d ** 2).sum()), m[3] + float((ref[k] ** 2).sum()), m[4] + d.size
+    stats = {k: (m[0] / m[1] if m[1] > 0 else 0.0, float(np.sqrt(m[2] / m[3])) if m[3] > 0 else 0.0)
+             for k, m in acc.items()}
+    print("mixed vs oracle (512x512x90, every point): max err / field max, RMS err / field RMS",
+          {k: f"{a:.1e}, {b:.1e}" for k, (a, b) in stats.items()},
+          f"; ice concentration reset on one side only at {resets} points")
     bad = {k: v for k, v in stats.items() if not (v[0] <= MIXED_MAX_REL and v[1] <= MIXED_RMS_REL)}
     assert not bad, bad
+    assert resets <= 1e-4 * 90 * ncol, resets
 
 
 def _fused_raw(lib, ctx, dev, f, ncol, nlev, ld, stages=(7,), calls=False):
