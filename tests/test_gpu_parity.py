@@ -95,33 +95,49 @@ ADJ_FIELDS = ("pabs", "sigs", "th", "exn", "exn_ref", "rho_dry_ref", "rv", "rc",
               "cf_mf", "rc_mf", "ri_mf", "rvs", "rcs", "ris", "ths")
 
 
-def test_ice_adjust_vs_oracle_config2_full(dev):
-    """ICE_ADJUST alone on the whole 256x256x90 workload it is measured on (the "adjust"
-    generator, seed 20260002, dt 50 s): every one of its 5,898,240 points within the FP64
-    bar of the oracle, except points where the oracle itself is not determined to the bar by
-    FP64 inputs (helpers.ill_conditioned), which must stay below 1e-5 of the points."""
+def _every_adjust_point(out, ref, f, run, names, label):
+    """Every point of an ICE_ADJUST result against the oracle's: within the FP64 bar, except
+    points where the oracle itself is not determined to the bar by FP64 inputs
+    (helpers.ill_conditioned: its outputs move beyond the bar when the inputs change by 2 ulp,
+    as a cloud fraction 0.5 + 0.36 atan(1.55 Q1) that cancels to near 0), which must stay
+    below 1e-5 of the points."""
     from helpers import ill_conditioned
-    from ice3_amd import synthetic
-    from oracle import oracle_lib
-    f = synthetic.generate(256 * 256, 90, seed=20260002, variant="adjust", dt=50.0)
-    out = _adjust_gpu(f)
-    ref = oracle_lib.ice_adjust(f)
-    names = ADJ_OUT + ("srcs",)
     bad = np.zeros(f["th"].shape, dtype=bool)
     for k in names:
         bad |= ~(rel_err(out[k], ref[k], floor_for(k, TOL)) <= TOL)
-    # a point beyond the bar must be one where the oracle's own outputs move beyond it when
-    # its FP64 inputs change by 2 ulp (a cloud fraction 0.5 + 0.36 atan(1.55 Q1) that cancels
-    # to near 0, for example)
-    ill = ill_conditioned(oracle_lib.ice_adjust, f, names, TOL, ADJ_FIELDS) if bad.any() else bad
+    ill = ill_conditioned(run, f, names, TOL, ADJ_FIELDS) if bad.any() else bad
     for k, i in np.argwhere(bad)[:10]:
         print(f"level {k} column {i}: ill-conditioned {bool(ill[k, i])}, "
               f"{ {q: (float(out[q][k, i]), float(ref[q][k, i])) for q in names if not rel_err(out[q][k, i], ref[q][k, i], floor_for(q, TOL)) <= TOL} }")
     assert not (bad & ~ill).any(), np.argwhere(bad & ~ill)[:10]
     assert bad.mean() <= 1e-5, int(bad.sum())
     w = _check(out, ref, names, flips=bad)
-    print(f"ice_adjust 256x256x90, every point: {int(bad.sum())} ill-conditioned points beyond the "
-          f"bar of {bad.size}, worst {max(w.values()):.2e} on the others")
+    print(f"{label}, every point: {int(bad.sum())} ill-conditioned points beyond the bar of "
+          f"{bad.size}, worst {max(w.values()):.2e} on the others")
+
+
+def test_ice_adjust_vs_oracle_config2_full(dev):
+    """ICE_ADJUST alone on the whole 256x256x90 workload it is measured on (the "adjust"
+    generator, seed 20260002, dt 50 s): every one of its 5,898,240 points against the oracle
+    (_every_adjust_point)."""
+    from ice3_amd import synthetic
+    from oracle import oracle_lib
+    f = synthetic.generate(256 * 256, 90, seed=20260002, variant="adjust", dt=50.0)
+    _every_adjust_point(_adjust_gpu(f), oracle_lib.ice_adjust(f), f, oracle_lib.ice_adjust,
+                        ADJ_OUT + ("srcs",), "ice_adjust 256x256x90")
+
+
+def test_ice_adjust_ocnd2_vs_oracle_config2_full(dev):
+    """ICE_ADJUST with OCND2 on the 256x256x90 workload it is measured on (the same inputs,
+    the generator's level heights, ice cloud weight 1): every point against the oracle
+    (_every_adjust_point)."""
+    from ice3_amd import synthetic
+    from oracle import oracle_lib
+    f = synthetic.generate(256 * 256, 90, seed=20260002, variant="adjust", dt=50.0)
+    f["zz"] = np.ascontiguousarray(np.broadcast_to(synthetic.level_profile(90)[2][:, None], f["th"].shape))
+    f["ice_cld_wgt"] = np.ones(256 * 256)
+    _every_adjust_point(_adjust_ocnd2_gpu(f), oracle_lib.ice_adjust_ocnd2(f), f, oracle_lib.ice_adjust_ocnd2,
+                        OCND2_CHECKED, "ice_adjust OCND2 256x256x90")
 
 
 OCND2_X = ("icldfr", "wcldfr", "ssio", "ssiu", "ifr")
